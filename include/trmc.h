@@ -1,0 +1,277 @@
+/*
+ * trmc -- Muskingum-Cunge river routing on AMD Instinct MI355X (gfx950).
+ *
+ * C-ABI of the routing library (noaa-gpu-kernels_amd/libtrmc.so).  Plain
+ * pointers and sizes only; every array is caller-owned host memory unless the
+ * name says _dev.  Every entry point returns an int status (TRMC_OK == 0,
+ * negative on error) and never exits the process; trmc_last_error() gives the
+ * message of the calling thread's last failure.
+ *
+ * Reference interfaces replaced (consigcody94/noaa-gpu-kernels):
+ *   trmc_wavefront_f32in_f64  <- cpu_wavefront_fp64,
+ *                                water-prediction/t-route/wavefront/wavefront_mc.cu:495-507
+ *                                (and the GPU time/level loop of :737-762 with
+ *                                kernel_level_wavefront :342-408)
+ *   trmc_wavefront_f64        <- same call, every array double (FP64 end to end)
+ *   trmc_reach_parallel_f64   <- cpu_mc, water-prediction/t-route/troute_mc_final.cu:167-171
+ *                                and kernel_mc_gpu :142-148
+ *   trmc_mc_solve_f64         <- mc_solve_d, wavefront_mc.cu:215-220 (batched)
+ *   trmc_load_topo / trmc_load_params / trmc_load_forcings
+ *                             <- load_topo wavefront_mc.cu:599-632 and the
+ *                                params/forcings reads :654-680 (format of
+ *                                wavefront/gen_network.py:10-37)
+ *   trmc_compute_stats        <- compute_stats lambda, wavefront_mc.cu:822-858
+ * Entry points with no reference counterpart are marked [new].
+ */
+#ifndef TRMC_H
+#define TRMC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TRMC_OK                 0
+#define TRMC_ERR_ARG           -1  /* bad argument (null pointer, negative size) */
+#define TRMC_ERR_TOPOLOGY      -2  /* level/CSR structure is not a valid DAG schedule */
+#define TRMC_ERR_HIP           -3  /* HIP runtime call failed */
+#define TRMC_ERR_NO_DEVICE     -4  /* no usable gfx950 device */
+#define TRMC_ERR_TIMEOUT       -5  /* persistent-kernel barrier wait exceeded its limit */
+#define TRMC_ERR_NOMEM         -6  /* host or device allocation failed */
+#define TRMC_ERR_IO            -7  /* file missing, short or inconsistent */
+#define TRMC_ERR_STATE         -8  /* call order violated (e.g. run before params) */
+#define TRMC_ERR_NOT_CONVERGED -9  /* converged/learned solve hit its iteration cap */
+
+/* Per-segment channel parameters in the reference's field order:
+ * FP32 as wavefront_mc.cu:41-44, FP64 as troute_mc_final.cu:26-28. */
+typedef struct { float  dx, bw, tw, twcc, n_ch, ncc, cs, s0; } trmc_chparams_f32;
+typedef struct { double dx, bw, tw, twcc, n_ch, ncc, cs, s0; } trmc_chparams_f64;
+
+/* Secant stopping rule.
+ *  REFERENCE: the reference's rule, 1 % relative or 1 cm absolute depth change,
+ *             up to 5 widening attempts (wavefront_mc.cu:242-318).  Bit-for-bit
+ *             drop-in semantics.
+ *  CONVERGED: [new] the same MC residual solved to a tight tolerance from the
+ *             reference's initial bracket (0.67 d, 1.33 d + 0.01).
+ *  LEARNED:   [new] as CONVERGED, but the initial depth comes from the on-device
+ *             MLP (trmc_mlp_*); a segment whose guess does not converge within
+ *             the learned budget restarts from the reference bracket. */
+#define TRMC_MODE_REFERENCE 0
+#define TRMC_MODE_CONVERGED 1
+#define TRMC_MODE_LEARNED   2
+
+/* Schedules of the level-ordered DAG sweep.
+ *  AUTO:    pick from the autotuner's table (or SKEWED).
+ *  SKEWED:  one persistent kernel over super-levels k = level(unit) + t; a
+ *           device-scope barrier between super-levels (L + T - 1 barriers).
+ *  LEVEL:   one launch per (timestep, level), as the reference's
+ *           kernel_level_wavefront loop; the per-step launch sequence is
+ *           captured once in a hipGraph and replayed T times. */
+#define TRMC_SCHED_AUTO   0
+#define TRMC_SCHED_SKEWED 1
+#define TRMC_SCHED_LEVEL  2
+
+typedef struct {
+    int    mode;            /* TRMC_MODE_* (default REFERENCE) */
+    int    schedule;        /* TRMC_SCHED_* (default AUTO) */
+    int    unit_segments;   /* max segments per work unit; 0 = auto (1) */
+    int    block_size;      /* threads per workgroup; 0 = auto (256) */
+    int    n_members;       /* ensemble members sharing params (>= 1) */
+    int    record_history;  /* keep every step's last-segment outflow per reach */
+    double tol;             /* CONVERGED/LEARNED relative depth tolerance (default 1e-13) */
+    int    max_iter;        /* CONVERGED/LEARNED iteration cap (default 60) */
+    int    learned_budget;  /* LEARNED: evaluations before falling back (default 12) */
+    int    grid_blocks;     /* persistent grid size; 0 = fill the device */
+    int    dynamic_queue;   /* SKEWED: 1 = atomic work queue per super-level */
+    int    device;          /* HIP device ordinal */
+} trmc_options;
+
+typedef struct {
+    double kernel_ms;        /* device time of the routing launches (HIP events) */
+    double wall_ms;          /* host wall time of trmc_network_run */
+    long long seg_steps;     /* segments x members x timesteps routed */
+    long long secant_iters;  /* total secant iterations over all solves */
+    long long fallbacks;     /* LEARNED: solves that restarted from the bracket */
+    long long not_converged; /* CONVERGED/LEARNED: solves that hit max_iter */
+    int    launches;         /* kernel launches (or graph replays) issued */
+    int    barriers;         /* device-scope barriers crossed (SKEWED) */
+    int    super_levels;     /* L + T - 1 for SKEWED */
+    int    grid_blocks;      /* persistent grid actually used */
+    int    block_size;
+    double barrier_wait_frac;/* fraction of block-time spent waiting in barriers */
+} trmc_run_stats;
+
+/* ---------------------------------------------------------------- basics */
+const char *trmc_last_error(void);
+int  trmc_version(void);                       /* major*10000 + minor*100 + patch */
+int  trmc_device_count(int *count);            /* gfx950 devices visible */
+void trmc_default_options(trmc_options *opt);
+
+/* ------------------------------------------------------ drop-in routines */
+
+/* cpu_wavefront_fp64 (wavefront_mc.cu:495-507), same argument list: float
+ * params/forcings widened to double, FP64 routing on the GPU, final-step
+ * per-segment qdc and per-reach last-segment qdc as double.  qlat_ts is
+ * [n_timesteps][n_segments].  Uses the REFERENCE stopping rule and the
+ * default schedule. */
+int trmc_wavefront_f32in_f64(const trmc_chparams_f32 *params,
+                             const int *reach_seg_start, const int *reach_seg_len,
+                             const int *reach_up_ptr, const int *reach_up_idx,
+                             const int *level_ptr, const int *level_reach,
+                             int n_reaches, int n_levels, int n_segments,
+                             const float *qlat_ts, const float *qdp0_seg,
+                             const float *dp0_seg, const float *reach_last0,
+                             double *qdc_seg_out_d, double *reach_last_out_d,
+                             int n_timesteps, double dt);
+
+/* Same call with every array double (FP64 end to end).  opt may be NULL. */
+int trmc_wavefront_f64(const trmc_chparams_f64 *params,
+                       const int *reach_seg_start, const int *reach_seg_len,
+                       const int *reach_up_ptr, const int *reach_up_idx,
+                       const int *level_ptr, const int *level_reach,
+                       int n_reaches, int n_levels, int n_segments,
+                       const double *qlat_ts, const double *qdp0_seg,
+                       const double *dp0_seg, const double *reach_last0,
+                       double *qdc_seg_out, double *reach_last_out,
+                       int n_timesteps, double dt, const trmc_options *opt);
+
+/* cpu_mc / kernel_mc_gpu (troute_mc_final.cu:167-171, :142-148): independent
+ * reaches, one step, upstream inflow uq into segment 0 of every reach. */
+int trmc_reach_parallel_f64(const trmc_chparams_f64 *p, const double *ql,
+                            const int *rs, const int *rl,
+                            const double *qu, const double *qd, const double *dp,
+                            double *qo, double *vo, double *dpo,
+                            double uq, double dt, int nr);
+
+/* mc_solve_d (wavefront_mc.cu:215-220) over n independent segment inputs on
+ * the GPU.  in = [n][5] {qup, quc, qdp, ql, depthp}; out = [n][3]
+ * {qdc, depth, velocity}; iters (may be NULL) = [n] secant iterations.
+ * mode selects the stopping rule (REFERENCE or CONVERGED). */
+int trmc_mc_solve_f64(const trmc_chparams_f64 *params, const double *in,
+                      double *out, int *iters, int n, double dt, int mode);
+
+/* ------------------------------------------------- host-only schedule plan */
+/* [new] The schedule the library prepares for a topology, without a device:
+ * reaches cut into units of <= unit_segments segments, ALAP unit levels,
+ * level-sorted device order.  For inspection and tests; trmc_network_create
+ * builds the same plan internally. */
+typedef struct trmc_plan trmc_plan;
+int  trmc_plan_build(const int *reach_seg_start, const int *reach_seg_len,
+                     const int *reach_up_ptr, const int *reach_up_idx,
+                     const int *level_ptr, const int *level_reach,
+                     int n_reaches, int n_levels, int n_segments, int unit_segments,
+                     trmc_plan **out);
+void trmc_plan_destroy(trmc_plan *plan);
+/* info[0..7] = n_units, n_unit_levels, ring depth, super-levels for
+ * n_timesteps, widest super-level in items (x n_members), device segments,
+ * max level span of an edge, upstream links. */
+int  trmc_plan_info(const trmc_plan *plan, int n_timesteps, int n_members,
+                    int64_t *info, int n_info);
+/* Copies plan array `which` into out[n] and returns its length:
+ * 0 level_ptr, 1 unit_level, 2 unit_seg, 3 up_ptr, 4 up_idx, 5 unit_flags,
+ * 6 unit_reach, 7 unit_init, 8 seg_of_dev, 9 reach_last_unit. */
+int  trmc_plan_export(const trmc_plan *plan, int which, int *out, int n);
+
+/* ----------------------------------------------- resident network handle */
+/* [new] Owns device buffers across calls: topology, parameters, forcing and
+ * state stay in HBM; trmc_network_run routes with inputs already resident. */
+typedef struct trmc_network trmc_network;
+
+/* Topology in the reference's CSR/level layout (wavefront_mc.cu:577-632). */
+int trmc_network_create(const int *reach_seg_start, const int *reach_seg_len,
+                        const int *reach_up_ptr, const int *reach_up_idx,
+                        const int *level_ptr, const int *level_reach,
+                        int n_reaches, int n_levels, int n_segments,
+                        const trmc_options *opt, trmc_network **out);
+void trmc_network_destroy(trmc_network *net);
+
+int trmc_network_set_params_f64(trmc_network *net, const trmc_chparams_f64 *params);
+int trmc_network_set_params_f32(trmc_network *net, const trmc_chparams_f32 *params);
+
+/* Initial state per member: qdp0/dp0 are [n_members][n_segments], reach_last0
+ * is [n_members][n_reaches] (NULL -> qdp0 of each reach's last segment,
+ * as wavefront_mc.cu:676-680). */
+int trmc_network_set_state_f64(trmc_network *net, const double *qdp0_seg,
+                               const double *dp0_seg, const double *reach_last0);
+
+/* Lateral inflow [n_members][n_timesteps][n_segments] (double or float). */
+int trmc_network_set_forcing_f64(trmc_network *net, const double *qlat, int n_timesteps);
+int trmc_network_set_forcing_f32(trmc_network *net, const float *qlat, int n_timesteps);
+
+/* Route n_timesteps steps (<= the forcing's) from the current state; the final
+ * state becomes the new current state (calls chain: run(T1) then run(T2) ==
+ * run(T1+T2) bit for bit).  stats may be NULL. */
+int trmc_network_run(trmc_network *net, int n_timesteps, double dt, trmc_run_stats *stats);
+
+/* Restore the state given to the last trmc_network_set_state_f64 (kept in
+ * device memory; device-to-device copies only) and rewind the forcing to its
+ * first step.  Queued on the network's stream. */
+int trmc_network_reset(trmc_network *net);
+
+/* Current state per member in the caller's (reference) segment order.
+ * Any output pointer may be NULL.  qdc/depth/vel are [n_members][n_segments],
+ * reach_last [n_members][n_reaches]. */
+int trmc_network_get_state_f64(const trmc_network *net, double *qdc_seg,
+                               double *depth_seg, double *vel_seg, double *reach_last);
+
+/* With record_history: last-segment outflow of every reach after every step
+ * of the last run, [n_members][n_timesteps][n_reaches]. */
+int trmc_network_get_history_f64(const trmc_network *net, double *reach_hist);
+
+/* Shape facts of the prepared schedule, as trmc_plan_info with the
+ * network's member count. */
+int trmc_network_info(const trmc_network *net, int n_timesteps, int64_t *info, int n_info);
+
+/* ------------------------------------------------- formats and generator */
+/* Binary formats of wavefront/gen_network.py:10-37 (little-endian int32 /
+ * float32).  Arrays are malloc'd by the library; free with trmc_free. */
+typedef struct {
+    int n_reaches, n_segments, n_levels, max_upstream;
+    int *reach_seg_start, *reach_seg_len, *reach_level, *reach_n_up;
+    int *reach_up_ptr, *reach_up_idx, *level_ptr, *level_reach;
+} trmc_topo;
+
+int  trmc_load_topo(const char *path, trmc_topo *out);
+int  trmc_save_topo(const char *path, const trmc_topo *t);
+void trmc_free_topo(trmc_topo *t);
+int  trmc_load_params(const char *path, int n_segments, trmc_chparams_f32 *out);
+int  trmc_save_params(const char *path, int n_segments, const trmc_chparams_f32 *p);
+/* forcings.bin: int32 T; float32 qlat[T*N]; qup0[R]; qdp0[N]; dp0[N].
+ * Pass NULL outputs to query *n_timesteps first. */
+int  trmc_load_forcings(const char *path, int n_reaches, int n_segments, int *n_timesteps,
+                        float *qlat, float *qup0, float *qdp0, float *dp0);
+int  trmc_save_forcings(const char *path, int n_reaches, int n_segments, int n_timesteps,
+                        const float *qlat, const float *qup0, const float *qdp0,
+                        const float *dp0);
+void trmc_free(void *p);
+
+/* [new] Seeded synthetic networks for the benchmark workloads.
+ *  kind 1: binary-confluence tree, reaches of U{1..40} segments.
+ *  kind 2: Shreve-style random tree: uniform random binary topology grown by
+ *          random link attachment (n_reaches/2 headwaters, one single-upstream
+ *          reach when n_reaches is even), reach lengths geometric with mean
+ *          n_segments/n_reaches, clipped to [1, 40], total exactly n_segments.
+ * n_reaches is a target (kind 2) or ignored (kind 1).  Parameters are
+ * float-representable doubles so f32 and f64 entry points see equal values. */
+int trmc_generate_network(int kind, int n_segments, int n_reaches, uint64_t seed,
+                          trmc_topo *out);
+int trmc_generate_params(int n_segments, uint64_t seed, trmc_chparams_f64 *out);
+/* qlat [n_members][n_timesteps][n_segments], lognormal with mean 0.1 m3/s
+ * (sigma_ln 1), member m drawn from its own stream. */
+int trmc_generate_forcing(int n_segments, int n_timesteps, int n_members, uint64_t seed,
+                          double *qlat);
+/* Initial state: qdp0 = q0, dp0 = wide-channel normal depth at q0. */
+int trmc_generate_state(int n_segments, const trmc_chparams_f64 *params, double q0,
+                        double *qdp0, double *dp0);
+
+/* Error statistics of wavefront_mc.cu:822-858: out[0..8] = max_abs, p50, p90,
+ * p99, p99.9, max_rel (relative error over |truth| > 1e-3), fraction < 1 %,
+ * fraction < 10 %, NaN/Inf count. */
+int trmc_compute_stats(const double *pred, const double *truth, int n, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TRMC_H */

@@ -1,0 +1,556 @@
+"""Python host bindings for libtrmc.so -- Muskingum-Cunge routing on MI355X.
+
+This is a thin ctypes layer over the C-ABI declared in include/trmc.h.  Every
+routing call runs the hand-written gfx950 HIP kernels; there is no CPU
+fallback, and a missing or unloadable library raises ``TrmcError``.
+
+Names mirror the reference's host interface (consigcody94/noaa-gpu-kernels,
+water-prediction/t-route/wavefront/wavefront_mc.cu):
+
+=========================  ==================================================
+``wavefront_fp64``         cpu_wavefront_fp64 (wavefront_mc.cu:495-564)
+``wavefront_f64``          same call, FP64 inputs end to end
+``reach_parallel``         cpu_mc / kernel_mc_gpu (troute_mc_final.cu:142-252)
+``mc_solve``               mc_solve_d (wavefront_mc.cu:215-337), batched
+``load_topo``              load_topo (wavefront_mc.cu:599-632)
+``load_params``            params.bin read (wavefront_mc.cu:654-656)
+``load_forcings``          forcings.bin read (wavefront_mc.cu:658-672)
+``compute_stats``          accuracy block (wavefront_mc.cu:822-858)
+``Network``                [new] device-resident handle (trmc_network_*)
+=========================  ==================================================
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtrmc.so")
+
+OK = 0
+ERR_ARG, ERR_TOPOLOGY, ERR_HIP, ERR_NO_DEVICE, ERR_TIMEOUT = -1, -2, -3, -4, -5
+ERR_NOMEM, ERR_IO, ERR_STATE, ERR_NOT_CONVERGED = -6, -7, -8, -9
+MODE_REFERENCE, MODE_CONVERGED, MODE_LEARNED = 0, 1, 2
+SCHED_AUTO, SCHED_SKEWED, SCHED_LEVEL = 0, 1, 2
+
+PARAM_FIELDS = ("dx", "bw", "tw", "twcc", "n_ch", "ncc", "cs", "s0")
+
+
+class TrmcError(RuntimeError):
+    """A C-ABI call returned a negative status (code in ``.code``)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"trmc error {code}: {msg}")
+        self.code = code
+
+
+class ChParamsF32(C.Structure):
+    _fields_ = [(n, C.c_float) for n in PARAM_FIELDS]
+
+
+class ChParamsF64(C.Structure):
+    _fields_ = [(n, C.c_double) for n in PARAM_FIELDS]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int), ("schedule", C.c_int), ("unit_segments", C.c_int),
+        ("block_size", C.c_int), ("n_members", C.c_int), ("record_history", C.c_int),
+        ("tol", C.c_double), ("max_iter", C.c_int), ("learned_budget", C.c_int),
+        ("grid_blocks", C.c_int), ("dynamic_queue", C.c_int), ("device", C.c_int),
+    ]
+
+
+class RunStats(C.Structure):
+    _fields_ = [
+        ("kernel_ms", C.c_double), ("wall_ms", C.c_double), ("seg_steps", C.c_longlong),
+        ("secant_iters", C.c_longlong), ("fallbacks", C.c_longlong),
+        ("not_converged", C.c_longlong), ("launches", C.c_int), ("barriers", C.c_int),
+        ("super_levels", C.c_int), ("grid_blocks", C.c_int), ("block_size", C.c_int),
+        ("barrier_wait_frac", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Topo(C.Structure):
+    _fields_ = [("n_reaches", C.c_int), ("n_segments", C.c_int), ("n_levels", C.c_int),
+                ("max_upstream", C.c_int)] + [
+        (n, C.POINTER(C.c_int)) for n in (
+            "reach_seg_start", "reach_seg_len", "reach_level", "reach_n_up",
+            "reach_up_ptr", "reach_up_idx", "level_ptr", "level_reach")]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libtrmc.so (built by ``make -C noaa-gpu-kernels_amd/csrc``)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TrmcError(ERR_STATE, f"{LIB_PATH} is not built")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _p(t):
+    return C.POINTER(t)
+
+
+def _declare(L: C.CDLL) -> None:
+    I, D, V = C.c_int, C.c_double, C.c_void_p
+    pI, pD, pF = _p(C.c_int), _p(C.c_double), _p(C.c_float)
+    topo = [pI] * 6 + [I, I, I]
+    sig = {
+        "trmc_last_error": (C.c_char_p, []),
+        "trmc_version": (I, []),
+        "trmc_device_count": (I, [pI]),
+        "trmc_default_options": (None, [_p(Options)]),
+        "trmc_wavefront_f32in_f64": (I, [_p(ChParamsF32)] + topo + [pF, pF, pF, pF, pD, pD, I, D]),
+        "trmc_wavefront_f64": (I, [_p(ChParamsF64)] + topo + [pD, pD, pD, pD, pD, pD, I, D, _p(Options)]),
+        "trmc_reach_parallel_f64": (I, [_p(ChParamsF64), pD, pI, pI, pD, pD, pD, pD, pD, pD, D, D, I]),
+        "trmc_mc_solve_f64": (I, [_p(ChParamsF64), pD, pD, pI, I, D, I]),
+        "trmc_plan_build": (I, topo + [I, _p(V)]),
+        "trmc_plan_destroy": (None, [V]),
+        "trmc_plan_info": (I, [V, I, I, _p(C.c_int64), I]),
+        "trmc_plan_export": (I, [V, I, pI, I]),
+        "trmc_network_create": (I, topo + [_p(Options), _p(V)]),
+        "trmc_network_destroy": (None, [V]),
+        "trmc_network_set_params_f64": (I, [V, _p(ChParamsF64)]),
+        "trmc_network_set_params_f32": (I, [V, _p(ChParamsF32)]),
+        "trmc_network_set_state_f64": (I, [V, pD, pD, pD]),
+        "trmc_network_set_forcing_f64": (I, [V, pD, I]),
+        "trmc_network_set_forcing_f32": (I, [V, pF, I]),
+        "trmc_network_run": (I, [V, I, D, _p(RunStats)]),
+        "trmc_network_reset": (I, [V]),
+        "trmc_network_get_state_f64": (I, [V, pD, pD, pD, pD]),
+        "trmc_network_get_history_f64": (I, [V, pD]),
+        "trmc_network_info": (I, [V, I, _p(C.c_int64), I]),
+        "trmc_load_topo": (I, [C.c_char_p, _p(Topo)]),
+        "trmc_save_topo": (I, [C.c_char_p, _p(Topo)]),
+        "trmc_free_topo": (None, [_p(Topo)]),
+        "trmc_load_params": (I, [C.c_char_p, I, _p(ChParamsF32)]),
+        "trmc_save_params": (I, [C.c_char_p, I, _p(ChParamsF32)]),
+        "trmc_load_forcings": (I, [C.c_char_p, I, I, pI, pF, pF, pF, pF]),
+        "trmc_save_forcings": (I, [C.c_char_p, I, I, I, pF, pF, pF, pF]),
+        "trmc_free": (None, [V]),
+        "trmc_generate_network": (I, [I, I, I, C.c_uint64, _p(Topo)]),
+        "trmc_generate_params": (I, [I, C.c_uint64, _p(ChParamsF64)]),
+        "trmc_generate_forcing": (I, [I, I, I, C.c_uint64, pD]),
+        "trmc_generate_state": (I, [I, _p(ChParamsF64), D, pD, pD]),
+        "trmc_compute_stats": (I, [pD, pD, I, pD]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def _check(rc: int) -> int:
+    if rc < 0:
+        msg = lib().trmc_last_error()
+        raise TrmcError(rc, msg.decode() if msg else "")
+    return rc
+
+
+# ---------------------------------------------------------------- arrays
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a: np.ndarray, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def params_f64(p: np.ndarray) -> np.ndarray:
+    """[N, 8] parameter matrix in ChParams field order -> contiguous float64."""
+    p = _f64(p)
+    if p.ndim != 2 or p.shape[1] != 8:
+        raise TrmcError(ERR_ARG, "params must be [n_segments, 8]")
+    return p
+
+
+def params_f32(p: np.ndarray) -> np.ndarray:
+    p = _f32(p)
+    if p.ndim != 2 or p.shape[1] != 8:
+        raise TrmcError(ERR_ARG, "params must be [n_segments, 8]")
+    return p
+
+
+@dataclass
+class Topology:
+    """Reach CSR + level layout of topo.bin (gen_network.py:10-37)."""
+
+    n_reaches: int
+    n_segments: int
+    n_levels: int
+    max_upstream: int
+    reach_seg_start: np.ndarray
+    reach_seg_len: np.ndarray
+    reach_level: np.ndarray
+    reach_n_up: np.ndarray
+    reach_up_ptr: np.ndarray
+    reach_up_idx: np.ndarray
+    level_ptr: np.ndarray
+    level_reach: np.ndarray
+
+    def arrays(self):
+        """The six arrays and three sizes in the reference call order."""
+        a = [_i32(self.reach_seg_start), _i32(self.reach_seg_len), _i32(self.reach_up_ptr),
+             _i32(self.reach_up_idx), _i32(self.level_ptr), _i32(self.level_reach)]
+        return a, (int(self.n_reaches), int(self.n_levels), int(self.n_segments))
+
+    def c_args(self):
+        a, sizes = self.arrays()
+        self._keep = a
+        return [_ptr(x, C.c_int) for x in a] + list(sizes)
+
+    @classmethod
+    def from_c(cls, t: Topo) -> "Topology":
+        R, L = t.n_reaches, t.n_levels
+        nnz = t.reach_up_ptr[R] if R >= 0 else 0
+
+        def take(ptr, n):
+            return np.ctypeslib.as_array(ptr, shape=(n,)).astype(np.int32).copy() if n > 0 else np.zeros(0, np.int32)
+
+        return cls(R, t.n_segments, L, t.max_upstream,
+                   take(t.reach_seg_start, R), take(t.reach_seg_len, R), take(t.reach_level, R),
+                   take(t.reach_n_up, R), take(t.reach_up_ptr, R + 1), take(t.reach_up_idx, nnz),
+                   take(t.level_ptr, L + 1), take(t.level_reach, R))
+
+    def to_c(self) -> Topo:
+        t = Topo()
+        t.n_reaches, t.n_segments, t.n_levels, t.max_upstream = (
+            int(self.n_reaches), int(self.n_segments), int(self.n_levels), int(self.max_upstream))
+        keep = []
+        for name in ("reach_seg_start", "reach_seg_len", "reach_level", "reach_n_up",
+                     "reach_up_ptr", "reach_up_idx", "level_ptr", "level_reach"):
+            a = _i32(getattr(self, name))
+            keep.append(a)
+            setattr(t, name, _ptr(a, C.c_int))
+        t._keep = keep
+        return t
+
+
+# ------------------------------------------------------------ formats
+def load_topo(path: str) -> Topology:
+    t = Topo()
+    _check(lib().trmc_load_topo(path.encode(), C.byref(t)))
+    try:
+        return Topology.from_c(t)
+    finally:
+        lib().trmc_free_topo(C.byref(t))
+
+
+def save_topo(path: str, topo: Topology) -> None:
+    t = topo.to_c()
+    _check(lib().trmc_save_topo(path.encode(), C.byref(t)))
+
+
+def load_params(path: str, n_segments: int) -> np.ndarray:
+    out = np.zeros((n_segments, 8), np.float32)
+    _check(lib().trmc_load_params(path.encode(), n_segments, _ptr(out, ChParamsF32)))
+    return out
+
+
+def save_params(path: str, params: np.ndarray) -> None:
+    p = params_f32(params)
+    _check(lib().trmc_save_params(path.encode(), p.shape[0], _ptr(p, ChParamsF32)))
+
+
+def load_forcings(path: str, n_reaches: int, n_segments: int):
+    """Returns (qlat [T, N], qup0 [R], qdp0 [N], dp0 [N]) as float32."""
+    T = C.c_int()
+    none = C.POINTER(C.c_float)()
+    _check(lib().trmc_load_forcings(path.encode(), n_reaches, n_segments, C.byref(T), none, none, none, none))
+    qlat = np.zeros((T.value, n_segments), np.float32)
+    qup0 = np.zeros(n_reaches, np.float32)
+    qdp0 = np.zeros(n_segments, np.float32)
+    dp0 = np.zeros(n_segments, np.float32)
+    _check(lib().trmc_load_forcings(path.encode(), n_reaches, n_segments, C.byref(T),
+                                    _ptr(qlat, C.c_float), _ptr(qup0, C.c_float),
+                                    _ptr(qdp0, C.c_float), _ptr(dp0, C.c_float)))
+    return qlat, qup0, qdp0, dp0
+
+
+def save_forcings(path: str, qlat, qup0, qdp0, dp0) -> None:
+    qlat = _f32(qlat)
+    T, N = qlat.shape
+    qup0, qdp0, dp0 = _f32(qup0), _f32(qdp0), _f32(dp0)
+    _check(lib().trmc_save_forcings(path.encode(), qup0.shape[0], N, T, _ptr(qlat, C.c_float),
+                                    _ptr(qup0, C.c_float), _ptr(qdp0, C.c_float), _ptr(dp0, C.c_float)))
+
+
+# ---------------------------------------------------------- generator
+def generate_network(kind: int, n_segments: int, n_reaches: int = 0, seed: int = 42) -> Topology:
+    t = Topo()
+    _check(lib().trmc_generate_network(kind, n_segments, n_reaches, seed, C.byref(t)))
+    try:
+        return Topology.from_c(t)
+    finally:
+        lib().trmc_free_topo(C.byref(t))
+
+
+def generate_params(n_segments: int, seed: int = 42) -> np.ndarray:
+    out = np.zeros((n_segments, 8), np.float64)
+    _check(lib().trmc_generate_params(n_segments, seed, _ptr(out, ChParamsF64)))
+    return out
+
+
+def generate_forcing(n_segments: int, n_timesteps: int, n_members: int = 1, seed: int = 42) -> np.ndarray:
+    out = np.zeros((n_members, n_timesteps, n_segments), np.float64)
+    _check(lib().trmc_generate_forcing(n_segments, n_timesteps, n_members, seed, _ptr(out, C.c_double)))
+    return out
+
+
+def generate_state(params: np.ndarray, q0: float = 1.0):
+    p = params_f64(params)
+    n = p.shape[0]
+    qdp0 = np.zeros(n, np.float64)
+    dp0 = np.zeros(n, np.float64)
+    _check(lib().trmc_generate_state(n, _ptr(p, ChParamsF64), q0, _ptr(qdp0, C.c_double), _ptr(dp0, C.c_double)))
+    return qdp0, dp0
+
+
+def reach_last0_from(topo: Topology, qdp0: np.ndarray) -> np.ndarray:
+    """reach_last0[r] = qdp0 of reach r's last segment (wavefront_mc.cu:676-680)."""
+    last = topo.reach_seg_start + topo.reach_seg_len - 1
+    return np.asarray(qdp0)[last].copy()
+
+
+@dataclass
+class Workload:
+    topo: Topology
+    params: np.ndarray      # [N, 8] float64 (float-representable)
+    qlat: np.ndarray        # [M, T, N] float64
+    qdp0: np.ndarray        # [N]
+    dp0: np.ndarray         # [N]
+    reach_last0: np.ndarray  # [R]
+
+
+def make_workload(kind: int, n_segments: int, n_timesteps: int, n_reaches: int = 0,
+                  n_members: int = 1, seed: int = 42) -> Workload:
+    """Seeded synthetic network, parameters, lateral inflow and initial state."""
+    topo = generate_network(kind, n_segments, n_reaches, seed)
+    params = generate_params(topo.n_segments, seed ^ 0x5A5A)
+    qlat = generate_forcing(topo.n_segments, n_timesteps, n_members, seed ^ 0xA1A1)
+    qdp0, dp0 = generate_state(params, 1.0)
+    return Workload(topo, params, qlat, qdp0, dp0, reach_last0_from(topo, qdp0))
+
+
+def compute_stats(pred: np.ndarray, truth: np.ndarray) -> dict:
+    pred, truth = _f64(pred).ravel(), _f64(truth).ravel()
+    out = np.zeros(9, np.float64)
+    _check(lib().trmc_compute_stats(_ptr(pred, C.c_double), _ptr(truth, C.c_double), pred.size,
+                                    _ptr(out, C.c_double)))
+    keys = ("max_abs", "p50", "p90", "p99", "p99_9", "max_rel", "within_1pct", "within_10pct", "nans")
+    return dict(zip(keys, out.tolist()))
+
+
+# ---------------------------------------------------------------- plan
+PLAN_ARRAYS = ("level_ptr", "unit_level", "unit_seg", "up_ptr", "up_idx", "unit_flags",
+               "unit_reach", "unit_init", "seg_of_dev", "reach_last_unit")
+
+
+def plan(topo: Topology, unit_segments: int = 1, n_timesteps: int = 1, n_members: int = 1) -> dict:
+    """Host-only schedule plan (no device needed): info + exported arrays."""
+    h = C.c_void_p()
+    _check(lib().trmc_plan_build(*topo.c_args(), unit_segments, C.byref(h)))
+    try:
+        info = np.zeros(8, np.int64)
+        _check(lib().trmc_plan_info(h, n_timesteps, n_members, _ptr(info, C.c_int64), 8))
+        keys = ("n_units", "n_levels", "ring", "super_levels", "max_items", "n_dev_segments",
+                "max_span", "n_links")
+        out = dict(zip(keys, info.tolist()))
+        cap = int(max(topo.n_segments, topo.n_reaches, out["n_units"], out["n_links"]) + out["n_levels"] + 2)
+        buf = np.zeros(cap, np.int32)
+        for i, name in enumerate(PLAN_ARRAYS):
+            n = _check(lib().trmc_plan_export(h, i, _ptr(buf, C.c_int), cap))
+            out[name] = buf[:n].copy()
+        return out
+    finally:
+        lib().trmc_plan_destroy(h)
+
+
+# -------------------------------------------------------------- routing
+def default_options(**kw) -> Options:
+    o = Options()
+    lib().trmc_default_options(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise TrmcError(ERR_ARG, f"unknown option {k}")
+        setattr(o, k, v)
+    return o
+
+
+def device_count() -> int:
+    n = C.c_int()
+    _check(lib().trmc_device_count(C.byref(n)))
+    return n.value
+
+
+def wavefront_fp64(params32, topo: Topology, qlat_ts, qdp0_seg, dp0_seg, reach_last0,
+                   n_timesteps: int, dt: float = 300.0):
+    """Drop-in for cpu_wavefront_fp64: float32 inputs, FP64 routing on the GPU.
+
+    Returns (qdc_seg_out_d [N], reach_last_out_d [R]) as float64."""
+    p = params_f32(params32)
+    q, qd, dp, rl = _f32(qlat_ts), _f32(qdp0_seg), _f32(dp0_seg), _f32(reach_last0)
+    qout = np.zeros(topo.n_segments, np.float64)
+    rout = np.zeros(topo.n_reaches, np.float64)
+    _check(lib().trmc_wavefront_f32in_f64(
+        _ptr(p, ChParamsF32), *topo.c_args(), _ptr(q, C.c_float), _ptr(qd, C.c_float),
+        _ptr(dp, C.c_float), _ptr(rl, C.c_float), _ptr(qout, C.c_double), _ptr(rout, C.c_double),
+        int(n_timesteps), float(dt)))
+    return qout, rout
+
+
+def wavefront_f64(params, topo: Topology, qlat_ts, qdp0_seg, dp0_seg, reach_last0,
+                  n_timesteps: int, dt: float = 300.0, **options):
+    """FP64-end-to-end variant of the drop-in; options as trmc_options fields."""
+    p = params_f64(params)
+    q, qd, dp, rl = _f64(qlat_ts), _f64(qdp0_seg), _f64(dp0_seg), _f64(reach_last0)
+    qout = np.zeros(topo.n_segments, np.float64)
+    rout = np.zeros(topo.n_reaches, np.float64)
+    o = default_options(**options)
+    _check(lib().trmc_wavefront_f64(
+        _ptr(p, ChParamsF64), *topo.c_args(), _ptr(q, C.c_double), _ptr(qd, C.c_double),
+        _ptr(dp, C.c_double), _ptr(rl, C.c_double), _ptr(qout, C.c_double), _ptr(rout, C.c_double),
+        int(n_timesteps), float(dt), C.byref(o)))
+    return qout, rout
+
+
+def reach_parallel(params, ql, rs, rl, qu, qd, dp, uq: float = 10.0, dt: float = 300.0):
+    """Drop-in for cpu_mc / kernel_mc_gpu: returns (qo, vo, dpo)."""
+    p = params_f64(params)
+    ql, qu, qd, dp = _f64(ql), _f64(qu), _f64(qd), _f64(dp)
+    rs, rl = _i32(rs), _i32(rl)
+    n = p.shape[0]
+    qo, vo, dpo = np.zeros(n), np.zeros(n), np.zeros(n)
+    _check(lib().trmc_reach_parallel_f64(
+        _ptr(p, ChParamsF64), _ptr(ql, C.c_double), _ptr(rs, C.c_int), _ptr(rl, C.c_int),
+        _ptr(qu, C.c_double), _ptr(qd, C.c_double), _ptr(dp, C.c_double), _ptr(qo, C.c_double),
+        _ptr(vo, C.c_double), _ptr(dpo, C.c_double), float(uq), float(dt), len(rs)))
+    return qo, vo, dpo
+
+
+def mc_solve(params, inputs, dt: float = 300.0, mode: int = MODE_REFERENCE):
+    """Batched mc_solve_d on the GPU.  inputs [n, 5] = qup, quc, qdp, ql, depthp.
+
+    Returns (out [n, 3] = qdc, depth, velocity; iters [n])."""
+    p = params_f64(params)
+    x = _f64(inputs).reshape(-1, 5)
+    n = x.shape[0]
+    out = np.zeros((n, 3), np.float64)
+    it = np.zeros(n, np.int32)
+    _check(lib().trmc_mc_solve_f64(_ptr(p, ChParamsF64), _ptr(x, C.c_double), _ptr(out, C.c_double),
+                                   _ptr(it, C.c_int), n, float(dt), int(mode)))
+    return out, it
+
+
+class Network:
+    """Device-resident network (trmc_network_*): topology, parameters, forcing
+    and state stay in HBM across runs."""
+
+    def __init__(self, topo: Topology, **options):
+        self.topo = topo
+        self.options = default_options(**options)
+        self.n_members = int(self.options.n_members)
+        self._h = C.c_void_p()
+        _check(lib().trmc_network_create(*topo.c_args(), C.byref(self.options), C.byref(self._h)))
+        self.last_stats: dict = {}
+
+    def close(self) -> None:
+        if self._h:
+            lib().trmc_network_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params) -> None:
+        p = np.asarray(params)
+        if p.dtype == np.float32:
+            p = params_f32(p)
+            _check(lib().trmc_network_set_params_f32(self._h, _ptr(p, ChParamsF32)))
+        else:
+            p = params_f64(p)
+            _check(lib().trmc_network_set_params_f64(self._h, _ptr(p, ChParamsF64)))
+
+    def set_state(self, qdp0, dp0, reach_last0=None) -> None:
+        M = self.n_members
+        q = _f64(np.broadcast_to(np.asarray(qdp0, np.float64), (M, self.topo.n_segments)))
+        d = _f64(np.broadcast_to(np.asarray(dp0, np.float64), (M, self.topo.n_segments)))
+        rl = None
+        if reach_last0 is not None:
+            rl = _f64(np.broadcast_to(np.asarray(reach_last0, np.float64), (M, self.topo.n_reaches)))
+        _check(lib().trmc_network_set_state_f64(
+            self._h, _ptr(q, C.c_double), _ptr(d, C.c_double),
+            _ptr(rl, C.c_double) if rl is not None else C.POINTER(C.c_double)()))
+
+    def set_forcing(self, qlat) -> None:
+        q = np.asarray(qlat)
+        if q.ndim == 2:
+            q = q[None]
+        if q.shape[0] != self.n_members or q.shape[2] != self.topo.n_segments:
+            raise TrmcError(ERR_ARG, "forcing must be [n_members, T, n_segments]")
+        if q.dtype == np.float32:
+            q = _f32(q)
+            _check(lib().trmc_network_set_forcing_f32(self._h, _ptr(q, C.c_float), q.shape[1]))
+        else:
+            q = _f64(q)
+            _check(lib().trmc_network_set_forcing_f64(self._h, _ptr(q, C.c_double), q.shape[1]))
+
+    def run(self, n_timesteps: int, dt: float = 300.0) -> dict:
+        st = RunStats()
+        _check(lib().trmc_network_run(self._h, int(n_timesteps), float(dt), C.byref(st)))
+        self.last_stats = st.as_dict()
+        return self.last_stats
+
+    def reset(self) -> None:
+        """Back to the initial state and the first forcing step (device copies)."""
+        _check(lib().trmc_network_reset(self._h))
+
+    def state(self):
+        """(qdc, depth, velocity) [M, N] and reach_last [M, R], reference order."""
+        M, N, R = self.n_members, self.topo.n_segments, self.topo.n_reaches
+        q, d, v = np.zeros((M, N)), np.zeros((M, N)), np.zeros((M, N))
+        rl = np.zeros((M, R))
+        _check(lib().trmc_network_get_state_f64(self._h, _ptr(q, C.c_double), _ptr(d, C.c_double),
+                                                _ptr(v, C.c_double), _ptr(rl, C.c_double)))
+        return q, d, v, rl
+
+    def history(self, n_timesteps: int) -> np.ndarray:
+        out = np.zeros((self.n_members, n_timesteps, self.topo.n_reaches))
+        _check(lib().trmc_network_get_history_f64(self._h, _ptr(out, C.c_double)))
+        return out
+
+    def info(self, n_timesteps: int) -> dict:
+        info = np.zeros(8, np.int64)
+        _check(lib().trmc_network_info(self._h, n_timesteps, _ptr(info, C.c_int64), 8))
+        keys = ("n_units", "n_levels", "ring", "super_levels", "max_items", "n_dev_segments",
+                "max_span", "n_links")
+        return dict(zip(keys, info.tolist()))

@@ -1,0 +1,80 @@
+// Device kernel interface shared by the HIP translation unit and the host glue.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace trmc {
+
+// Structure-of-arrays slots of the per-segment constant table (stride n_seg).
+enum SegSlot {
+    SC_DX = 0, SC_BW, SC_TWCC, SC_N, SC_NCC, SC_S0,
+    SC_Z,     // 1/cs (1 when cs == 0)
+    SC_BFD,   // bankfull depth
+    SC_KS,    // sqrt(s0) / n_ch
+    SC_SQ,    // sqrt(1 + z*z)
+    SC_SQS0,  // sqrt(s0)
+    SC_COUNT
+};
+
+
+// Counter slots written by the routing kernels (unsigned long long each).
+enum CounterSlot {
+    CNT_ITERS = 0,      // secant iterations (REFERENCE) / residual evaluations
+    CNT_CAPPED,         // CONVERGED/LEARNED solves that hit the cap
+    CNT_FALLBACK,       // LEARNED solves restarted from the reference bracket
+    CNT_WAIT_TICKS,     // barrier wait, wall-clock ticks summed over blocks
+    CNT_BLOCK_TICKS,    // total wall-clock ticks summed over blocks
+    CNT_COUNT
+};
+
+// status codes the kernels can raise
+enum { KSTATUS_OK = 0, KSTATUS_TIMEOUT = 1 };
+
+struct RouteArgs {
+    int n_units, n_levels, n_steps, n_members, ring, ring_shift, n_reaches;
+    int n_dev_segs;           // stride of the per-segment constant table
+    int t_base;               // forcing row of local step 0
+    int k_begin, k_end;       // super-level range of one persistent launch
+    int groups;               // barrier arrival groups
+    int max_eval;             // CONVERGED/LEARNED evaluation cap
+    int learned_budget;
+    double dt, tol;
+    long long timeout_ticks;  // barrier spin limit, wall-clock ticks
+    const int *level_ptr, *unit_level, *unit_seg, *up_ptr, *up_idx, *unit_flags, *unit_reach;
+    const double *segc;       // [SC_COUNT][n_dev_segs]
+    const double *qlat;       // [forcing rows][n_dev_segs][M]
+    double *q, *d, *v;        // [n_dev_segs][M], updated in place
+    double *ring_buf;         // [ring][n_units][M]
+    double *hist_out;         // [M][n_steps][n_reaches] or null
+    const float *mlp;         // LEARNED weights or null
+    unsigned *bar;            // barrier words (reset before each persistent launch)
+    int *status;
+    unsigned long long *counters;
+    const int *t_dev;         // LEVEL schedule: current local step
+};
+
+// launchers (trmc_kernels.hip); all asynchronous on `stream`
+hipError_t launch_reset(unsigned *bar, int nwords, hipStream_t stream);
+hipError_t launch_route_skewed(const RouteArgs &a, int mode, int grid, int block,
+                               bool cooperative, hipStream_t stream);
+hipError_t launch_route_level(const RouteArgs &a, int mode, int level, long long items, int block,
+                              hipStream_t stream);
+hipError_t launch_step_advance(int *t_dev, hipStream_t stream);
+hipError_t route_skewed_occupancy(int mode, int block, int *blocks_per_cu);
+hipError_t launch_reach_parallel(const double *segc, int n_seg, const double *ql,
+                                 const int *rs, const int *rl, const double *qu,
+                                 const double *qd, const double *dp, double *qo, double *vo,
+                                 double *dpo, double uq, double dt, int nr, hipStream_t stream);
+hipError_t launch_solve_batch(const double *segc, int n, const double *in, double *out,
+                              int *iters, double dt, int mode, double tol, int max_eval,
+                              hipStream_t stream);
+// forcing scatter: dst[(t*nd + ds)*M + m] = src[t*n_src + seg_of_dev[ds]] for one member
+hipError_t launch_scatter_forcing_f64(const double *src, int n_src, int T, const int *seg_of_dev,
+                                      int nd, int M, int m, double *dst, hipStream_t stream);
+hipError_t launch_scatter_forcing_f32(const float *src, int n_src, int T, const int *seg_of_dev,
+                                      int nd, int M, int m, double *dst, hipStream_t stream);
+
+}  // namespace trmc

@@ -1,0 +1,377 @@
+// gfx950 kernels for Muskingum-Cunge routing over a river DAG, FP64.
+//
+// Work item = (unit, member, step).  A unit is a run of consecutive segments
+// of one reach (see trmc_plan.h); its outflow for step t goes to ring slot
+// (t + 1 + ring_shift) mod ring.  Segment state (qdc, depth, velocity) is
+// updated in place: a unit is the only writer of its segments, and it carries
+// the previous-step qdc of segment i-1 in a register for segment i's qup
+// (wavefront_mc.cu:388 reads qdp_prev_seg[idx-1]).
+#include <hip/hip_runtime.h>
+
+#include "trmc_device.h"
+#include "trmc_solver.h"
+
+namespace trmc {
+
+struct Acc {
+    unsigned long long iters, capped, fallback;
+};
+
+template <int MODE>
+__device__ __forceinline__ SolveOut solve_one(const RouteArgs &a, double qup, double quc, double qdp,
+                                              double ql, const Seg &c, double dp)
+{
+    if (MODE == 0) return solve_reference(a.dt, qup, quc, qdp, ql, c, dp);
+    const double d0 = fmax(dp, 0.0);
+    return solve_converged(a.dt, qup, quc, qdp, ql, c, d0 * 0.67, d0 * 1.33 + 0.01, a.tol, a.max_eval);
+}
+
+// One unit, one member, one step.  Upstream reach outflows are summed in the
+// reference's CSR order starting from 0.0 (wavefront_mc.cu:523-529); an
+// internal unit boundary passes its single upstream value directly.
+template <int MODE>
+__device__ __forceinline__ void route_item(const RouteArgs &a, int u, int m, int t, Acc &acc)
+{
+    const int M = a.n_members;
+    const size_t UM = (size_t)a.n_units * (size_t)M;
+    const int D = a.ring;
+    const double *prev = a.ring_buf + (size_t)((t + a.ring_shift) % D) * UM;    // H(t-1)
+    double *cur = a.ring_buf + (size_t)((t + 1 + a.ring_shift) % D) * UM;      // H(t)
+    const int b = a.up_ptr[u], e = a.up_ptr[u + 1];
+    double qup_in, quc_in;
+    if (a.unit_flags[u] & 1) {
+        const size_t v = (size_t)a.up_idx[b] * M + m;
+        qup_in = prev[v];
+        quc_in = cur[v];
+    } else {
+        qup_in = 0.0;
+        quc_in = 0.0;
+        for (int k = b; k < e; ++k) {
+            const size_t v = (size_t)a.up_idx[k] * M + m;
+            qup_in += prev[v];
+            quc_in += cur[v];
+        }
+    }
+    const int s0 = a.unit_seg[u], s1 = a.unit_seg[u + 1];
+    const double *ql_row = a.qlat + (size_t)(a.t_base + t) * (size_t)a.n_dev_segs * M;
+    double last = 0.0, qup_next = 0.0, quc_next = 0.0;
+    for (int s = s0; s < s1; ++s) {
+        const size_t sm = (size_t)s * M + m;
+        const double qup = (s == s0) ? qup_in : qup_next;
+        const double quc = (s == s0) ? quc_in : quc_next;
+        const double qdp = a.q[sm];
+        const double dp = a.d[sm];
+        const double ql = ql_row[sm];
+        const Seg c = load_seg(a.segc, (size_t)a.n_dev_segs, s);
+        const SolveOut o = solve_one<MODE>(a, qup, quc, qdp, ql, c, dp);
+        a.q[sm] = o.q;
+        a.d[sm] = o.depth;
+        a.v[sm] = o.vel;
+        qup_next = qdp;      // previous-step qdc of this segment feeds the next one's qup
+        quc_next = o.q;
+        last = o.q;
+        acc.iters += (unsigned long long)o.iters;
+        acc.capped += (unsigned long long)(o.flags & 1);
+        acc.fallback += (unsigned long long)((o.flags >> 1) & 1);
+    }
+    cur[(size_t)u * M + m] = last;
+    if (a.hist_out) {
+        const int r = a.unit_reach[u];
+        if (r >= 0) a.hist_out[((size_t)m * a.n_steps + t) * (size_t)a.n_reaches + r] = last;
+    }
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long x)
+{
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+__device__ __forceinline__ void flush_acc(const RouteArgs &a, const Acc &acc, long long wait,
+                                          long long span)
+{
+    unsigned long long it = wave_sum(acc.iters), cp = wave_sum(acc.capped), fb = wave_sum(acc.fallback);
+    if ((threadIdx.x & 63) == 0) {
+        if (it) atomicAdd(&a.counters[CNT_ITERS], it);
+        if (cp) atomicAdd(&a.counters[CNT_CAPPED], cp);
+        if (fb) atomicAdd(&a.counters[CNT_FALLBACK], fb);
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&a.counters[CNT_WAIT_TICKS], (unsigned long long)wait);
+        atomicAdd(&a.counters[CNT_BLOCK_TICKS], (unsigned long long)span);
+    }
+}
+
+// Two-level device-scope barrier.  Counters only grow within a launch, so the
+// last arriver of barrier `epoch` in a group sees (epoch+1)*group_size - 1.
+// Word layout (unsigned, 128-byte spaced): [0] release epoch, [32] top
+// counter, [64 + 32 g] group g.  Every wave fences at agent scope before and
+// after, so stores of the super-level are visible across XCDs.
+__device__ __forceinline__ bool grid_sync(const RouteArgs &a, unsigned epoch, long long &wait)
+{
+    __shared__ int s_ok;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const long long t0 = (long long)wall_clock64();
+        const unsigned nb = gridDim.x;
+        const unsigned G = (unsigned)a.groups < nb ? (unsigned)a.groups : nb;
+        const unsigned g = blockIdx.x % G;
+        const unsigned gsize = nb / G + (g < nb % G ? 1u : 0u);
+        unsigned *flag = a.bar;
+        unsigned *top = a.bar + 32;
+        unsigned *grp = a.bar + 64 + 32 * g;
+        const unsigned target = epoch + 1;
+        int ok = 1;
+        unsigned old = __hip_atomic_fetch_add(grp, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == target * gsize - 1) {
+            unsigned old2 = __hip_atomic_fetch_add(top, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (old2 == target * G - 1)
+                __hip_atomic_store(flag, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != KSTATUS_OK) {
+                ok = 0;
+                break;
+            }
+            if ((long long)wall_clock64() - t0 > a.timeout_ticks) {
+                __hip_atomic_store(a.status, (int)KSTATUS_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        wait += (long long)wall_clock64() - t0;
+        s_ok = ok;
+    }
+    __syncthreads();
+    __threadfence();
+    return s_ok != 0;
+}
+
+// Persistent time-skewed sweep: super-level k routes every (unit, member) with
+// level(unit) + t == k, 0 <= t < n_steps.  Units are level-sorted, so the
+// items of k are one contiguous range; item i always goes to thread
+// i mod nthreads, keeping a unit on one CU (and XCD) across super-levels.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_route_skewed(RouteArgs a)
+{
+    const long long t_start = (long long)wall_clock64();
+    const long long nthreads = (long long)gridDim.x * blockDim.x;
+    const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int M = a.n_members, T = a.n_steps, L = a.n_levels;
+    Acc acc = {0, 0, 0};
+    long long wait = 0;
+    unsigned epoch = 0;
+    for (int k = a.k_begin; k < a.k_end; ++k) {
+        const int lo_l = max(0, k - T + 1), hi_l = min(k, L - 1);
+        if (lo_l <= hi_l) {
+            const long long lo = (long long)a.level_ptr[lo_l] * M;
+            const long long hi = (long long)a.level_ptr[hi_l + 1] * M;
+            long long off = gtid - lo % nthreads;
+            if (off < 0) off += nthreads;
+            for (long long i = lo + off; i < hi; i += nthreads) {
+                const int u = (int)(i / M);
+                const int m = (int)(i - (long long)u * M);
+                route_item<MODE>(a, u, m, k - a.unit_level[u], acc);
+            }
+        }
+        if (k + 1 < a.k_end) {
+            if (!grid_sync(a, epoch, wait)) break;
+            ++epoch;
+        }
+    }
+    flush_acc(a, acc, wait, (long long)wall_clock64() - t_start);
+}
+
+// One level of one step (reference-style schedule, graph-captured per step).
+template <int MODE>
+__global__ void __launch_bounds__(256) k_route_level(RouteArgs a, int level)
+{
+    const int M = a.n_members;
+    const long long lo = (long long)a.level_ptr[level] * M;
+    const long long hi = (long long)a.level_ptr[level + 1] * M;
+    const long long i = lo + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    Acc acc = {0, 0, 0};
+    if (i < hi) {
+        const int u = (int)(i / M);
+        const int m = (int)(i - (long long)u * M);
+        route_item<MODE>(a, u, m, *a.t_dev, acc);
+    }
+    unsigned long long it = wave_sum(acc.iters), cp = wave_sum(acc.capped);
+    if ((threadIdx.x & 63) == 0) {
+        if (it) atomicAdd(&a.counters[CNT_ITERS], it);
+        if (cp) atomicAdd(&a.counters[CNT_CAPPED], cp);
+    }
+}
+
+__global__ void k_step_advance(int *t_dev)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *t_dev += 1;
+}
+
+__global__ void k_reset(unsigned *w, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) w[i] = 0u;
+}
+
+// Independent reaches, one step (troute_mc_final.cu:142-164): one thread per
+// reach, sequential segments, quc of segment 0 = uq.
+__global__ void __launch_bounds__(256) k_reach_parallel(const double *__restrict__ segc, int n_seg,
+                                                        const double *__restrict__ ql,
+                                                        const int *__restrict__ rs,
+                                                        const int *__restrict__ rl,
+                                                        const double *__restrict__ qu,
+                                                        const double *__restrict__ qd,
+                                                        const double *__restrict__ dp,
+                                                        double *__restrict__ qo,
+                                                        double *__restrict__ vo,
+                                                        double *__restrict__ dpo, double uq,
+                                                        double dt, int nr)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nr) return;
+    double quc = uq;
+    const int start = rs[r], ns = rl[r];
+    for (int i = 0; i < ns; ++i) {
+        const int idx = start + i;
+        const Seg c = load_seg(segc, (size_t)n_seg, idx);
+        const SolveOut o = solve_reference(dt, qu[idx], quc, qd[idx], ql[idx], c, dp[idx]);
+        qo[idx] = o.q;
+        vo[idx] = o.vel;
+        dpo[idx] = o.depth;
+        quc = o.q;
+    }
+}
+
+// Independent segment solves: in [n][5] {qup, quc, qdp, ql, depthp}.
+__global__ void __launch_bounds__(256) k_solve_batch(const double *__restrict__ segc, int n,
+                                                     const double *__restrict__ in,
+                                                     double *__restrict__ out,
+                                                     int *__restrict__ iters, double dt, int mode,
+                                                     double tol, int max_eval)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *x = in + (size_t)i * 5;
+    const Seg c = load_seg(segc, (size_t)n, i);
+    SolveOut o;
+    if (mode == 0) {
+        o = solve_reference(dt, x[0], x[1], x[2], x[3], c, x[4]);
+    } else {
+        const double d0 = fmax(x[4], 0.0);
+        o = solve_converged(dt, x[0], x[1], x[2], x[3], c, d0 * 0.67, d0 * 1.33 + 0.01, tol, max_eval);
+    }
+    out[(size_t)i * 3 + 0] = o.q;
+    out[(size_t)i * 3 + 1] = o.depth;
+    out[(size_t)i * 3 + 2] = o.vel;
+    if (iters) iters[i] = o.iters;
+}
+
+template <typename TSrc>
+__global__ void k_scatter_forcing(const TSrc *__restrict__ src, int n_src, long long total,
+                                  const int *__restrict__ seg_of_dev, int nd, int M, int m,
+                                  double *__restrict__ dst)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const long long t = i / nd;
+    const int ds = (int)(i - t * nd);
+    dst[(size_t)i * M + m] = (double)src[(size_t)t * n_src + seg_of_dev[ds]];
+}
+
+// ----------------------------------------------------------------- launchers
+
+hipError_t launch_reset(unsigned *bar, int nwords, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, stream, bar, nwords);
+    return hipGetLastError();
+}
+
+template <int MODE>
+static hipError_t launch_skewed_t(const RouteArgs &a, int grid, int block, bool cooperative,
+                                  hipStream_t stream)
+{
+    if (cooperative) {
+        RouteArgs copy = a;
+        void *args[] = {&copy};
+        return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_route_skewed<MODE>),
+                                          dim3(grid), dim3(block), args, 0, stream);
+    }
+    hipLaunchKernelGGL(k_route_skewed<MODE>, dim3(grid), dim3(block), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_route_skewed(const RouteArgs &a, int mode, int grid, int block, bool cooperative,
+                               hipStream_t stream)
+{
+    return mode == 0 ? launch_skewed_t<0>(a, grid, block, cooperative, stream)
+                     : launch_skewed_t<1>(a, grid, block, cooperative, stream);
+}
+
+hipError_t route_skewed_occupancy(int mode, int block, int *blocks_per_cu)
+{
+    return mode == 0
+               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_skewed<0>, block, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_skewed<1>, block, 0);
+}
+
+hipError_t launch_route_level(const RouteArgs &a, int mode, int level, long long items, int block,
+                              hipStream_t stream)
+{
+    if (items <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((items + block - 1) / block));
+    if (mode == 0)
+        hipLaunchKernelGGL(k_route_level<0>, grid, dim3(block), 0, stream, a, level);
+    else
+        hipLaunchKernelGGL(k_route_level<1>, grid, dim3(block), 0, stream, a, level);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_advance(int *t_dev, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, stream, t_dev);
+    return hipGetLastError();
+}
+
+hipError_t launch_reach_parallel(const double *segc, int n_seg, const double *ql, const int *rs,
+                                 const int *rl, const double *qu, const double *qd, const double *dp,
+                                 double *qo, double *vo, double *dpo, double uq, double dt, int nr,
+                                 hipStream_t stream)
+{
+    if (nr <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reach_parallel, dim3((nr + 255) / 256), dim3(256), 0, stream, segc, n_seg,
+                       ql, rs, rl, qu, qd, dp, qo, vo, dpo, uq, dt, nr);
+    return hipGetLastError();
+}
+
+hipError_t launch_solve_batch(const double *segc, int n, const double *in, double *out, int *iters,
+                              double dt, int mode, double tol, int max_eval, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_solve_batch, dim3((n + 255) / 256), dim3(256), 0, stream, segc, n, in, out,
+                       iters, dt, mode, tol, max_eval);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_forcing_f64(const double *src, int n_src, int T, const int *seg_of_dev,
+                                      int nd, int M, int m, double *dst, hipStream_t stream)
+{
+    const long long total = (long long)T * nd;
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_forcing<double>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       stream, src, n_src, total, seg_of_dev, nd, M, m, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_forcing_f32(const float *src, int n_src, int T, const int *seg_of_dev,
+                                      int nd, int M, int m, double *dst, hipStream_t stream)
+{
+    const long long total = (long long)T * nd;
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_forcing<float>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       stream, src, n_src, total, seg_of_dev, nd, M, m, dst);
+    return hipGetLastError();
+}
+
+}  // namespace trmc

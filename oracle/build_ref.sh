@@ -1,0 +1,43 @@
+#!/bin/bash
+# Builds oracle/_ref/libtroute_ref.so from the reference's own FP64 host
+# routines, where they lie under the reference tree.  The .cu files cannot be
+# compiled whole here (no nvcc, no CUDA headers), but the host routines are
+# self-contained C++ using <cmath>/<vector> only.  Their line ranges are cut
+# into a temporary translation unit OUTSIDE this repository (nothing of the
+# reference source is kept), our C driver is appended, and g++ builds a .so
+# into oracle/_ref/ (git-ignored).  Compiled exactly like the oracle:
+# -O2 -ffp-contract=off.
+#
+#   wavefront/wavefront_mc.cu : 20-29 (std includes, skipping cuda_runtime.h at 30)
+#                               41-44 (ChParams, float)
+#                               200-337 (hgeo_inline_d, mc_solve_d)
+#                               495-564 (cpu_wavefront_fp64)
+#   troute_mc_final.cu        : 20-23 (includes, skipping cuda_runtime.h at 24)
+#                               26-28 (ChParams, double)
+#                               167-252 (cpu_mc), 254-274 (gen)
+set -euo pipefail
+REF=${TROUTE_REF_DIR:-/root/reference/water-prediction/t-route}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT="$HERE/_ref"
+if [ ! -f "$REF/wavefront/wavefront_mc.cu" ] || [ ! -f "$REF/troute_mc_final.cu" ]; then
+    echo "build_ref: reference sources not present; oracle/_ref not rebuilt"
+    exit 0
+fi
+TMP=$(mktemp -d /tmp/trmc_ref.XXXXXX)
+trap 'rm -rf "$TMP"' EXIT
+{ sed -n '20,29p;41,44p;200,337p;495,564p' "$REF/wavefront/wavefront_mc.cu"
+  cat "$HERE/ref_driver_wavefront.inc"; } > "$TMP/ref_wavefront.cpp"
+{ sed -n '20,23p;26,28p;167,252p;254,274p' "$REF/troute_mc_final.cu"
+  cat "$HERE/ref_driver_mcfinal.inc"; } > "$TMP/ref_mcfinal.cpp"
+# the cut must land on the routines it names (guards against line drift)
+grep -q "static inline double mc_solve_d(" "$TMP/ref_wavefront.cpp"
+grep -q "static void cpu_wavefront_fp64(" "$TMP/ref_wavefront.cpp"
+grep -q "^void cpu_mc(" "$TMP/ref_mcfinal.cpp"
+grep -q "^void gen(" "$TMP/ref_mcfinal.cpp"
+if grep -q "cuda" "$TMP/ref_wavefront.cpp" "$TMP/ref_mcfinal.cpp"; then
+    echo "build_ref: extracted ranges still reference CUDA" >&2; exit 1
+fi
+mkdir -p "$OUT"
+g++ -O2 -ffp-contract=off -fPIC -shared -std=c++17 -w \
+    -o "$OUT/libtroute_ref.so" "$TMP/ref_wavefront.cpp" "$TMP/ref_mcfinal.cpp"
+echo "build_ref: built $OUT/libtroute_ref.so"

@@ -1,0 +1,533 @@
+/*
+ * CPU oracle for Muskingum-Cunge routing -- TEST INFRASTRUCTURE ONLY.
+ * (Checker for tests/, __graft_entry__.smoke() and bench.py's cpu_baseline.)
+ *
+ * Plain-C restatement of the reference's FP64 host routines.  The arithmetic
+ * keeps the reference's operation order term for term so that, built with
+ * `gcc -O2 -ffp-contract=off`, the results are bit-identical to the reference
+ * routine compiled the same way.  File:line citations are into the reference
+ * tree (water-prediction/t-route/...).
+ */
+#include "mc_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* Per-segment constants derived once, wavefront_mc.cu:221-225. */
+typedef struct {
+    double dx, bw, tw, twcc, n_ch, ncc, s0;
+    double z;    /* 1/cs, or 1 when cs == 0 */
+    double bfd;  /* bankfull depth */
+} chan_t;
+
+/* Hydraulic geometry of the trapezoid + compound channel at depth h,
+ * wavefront_mc.cu:200-213. */
+typedef struct { double twl, R, A, AC, WP, WPC; } geo_t;
+
+/* Muskingum routing coefficients, wavefront_mc.cu:262-266 / :288-292. */
+typedef struct { double c1, c2, c3, c4; } mcoef_t;
+
+static void chan_init(const orc_chparams_f64 *p, chan_t *c)
+{
+    c->dx = p->dx; c->bw = p->bw; c->tw = p->tw; c->twcc = p->twcc;
+    c->n_ch = p->n_ch; c->ncc = p->ncc; c->s0 = p->s0;
+    c->z = (p->cs == 0.0) ? 1.0 : 1.0 / p->cs;
+    if (c->bw > c->tw)       c->bfd = c->bw / 0.00001;
+    else if (c->bw == c->tw) c->bfd = c->bw / (2.0 * c->z);
+    else                     c->bfd = (c->tw - c->bw) / (2.0 * c->z);
+}
+
+/* wavefront_mc.cu:227-229 */
+static int chan_invalid(const chan_t *c)
+{
+    return c->n_ch <= 0.0 || c->s0 <= 0.0 || c->z <= 0.0 || c->bw <= 0.0;
+}
+
+static void geometry(double h, const chan_t *c, geo_t *g)
+{
+    g->twl = c->bw + 2.0 * c->z * h;
+    double over = fmax(h - c->bfd, 0.0);   /* depth above bankfull */
+    double in = fmin(c->bfd, h);           /* depth inside the main channel */
+    if (over > 0.0 && c->twcc <= 0.0) { over = 0.0; in = h; }
+    g->A = (c->bw + in * c->z) * in;
+    g->WP = c->bw + 2.0 * in * sqrt(1.0 + c->z * c->z);
+    g->AC = c->twcc * over;
+    g->WPC = (over > 0.0) ? (c->twcc + 2.0 * over) : 0.0;
+    g->R = (g->WP + g->WPC > 0.0) ? (g->A + g->AC) / (g->WP + g->WPC) : 0.0;
+}
+
+/* Kinematic wave celerity, wavefront_mc.cu:249-254 (h_0) and :276-281 (h). */
+static double celerity(double h, const geo_t *g, const chan_t *c)
+{
+    if (!(h > 0.0)) return 0.0;
+    double r23 = pow(g->R, 2.0 / 3.0);
+    double r53 = pow(g->R, 5.0 / 3.0);
+    return fmax(0.0, (sqrt(c->s0) / c->n_ch) *
+                ((5.0 / 3.0) * r23 - (2.0 / 3.0) * r53 *
+                 (2.0 * sqrt(1.0 + c->z * c->z) / (c->bw + 2.0 * h * c->z))));
+}
+
+/* Storage constant Km, wavefront_mc.cu:255 / :282. */
+static double storage_km(double ck, double dt, const chan_t *c)
+{
+    return (ck > 0.0) ? fmax(dt, c->dx / ck) : dt;
+}
+
+/* Top width used by the weighting factor, wavefront_mc.cu:256 / :283. */
+static double weight_width(double h, const geo_t *g, const chan_t *c)
+{
+    return (h > c->bfd && c->twcc > 0.0) ? c->twcc : g->twl;
+}
+
+/* Muskingum weighting X clamped to [lo, 0.5] from a flow q,
+ * wavefront_mc.cu:257-259 (lo = 0, q = previous Qj_0) and :284-287
+ * (lo = 0.25, q = routed flow with the h_0 coefficients). */
+static double weighting(double ck, double twu, double q, double lo, const chan_t *c)
+{
+    if (ck > 0.0 && twu * c->s0 * ck * c->dx > 0.0)
+        return fmin(0.5, fmax(lo, 0.5 * (1.0 - q / (2.0 * twu * c->s0 * ck * c->dx))));
+    return 0.5;
+}
+
+static void muskingum(double km, double x, double dt, double ql, mcoef_t *m)
+{
+    double D = km * (1.0 - x) + dt / 2.0;
+    if (D == 0.0) D = 1.0;
+    m->c1 = (km * x + dt / 2.0) / D;
+    m->c2 = (dt / 2.0 - km * x) / D;
+    m->c3 = (km * (1.0 - x) - dt / 2.0) / D;
+    m->c4 = (ql * dt) / D;
+}
+
+/* Manning flow of the compound section, wavefront_mc.cu:267-270 / :298-302. */
+static double manning(const geo_t *g, const chan_t *c)
+{
+    if (!(g->WP + g->WPC > 0.0)) return 0.0;
+    return (1.0 / (((g->WP * c->n_ch) + (g->WPC * c->ncc)) / (g->WP + g->WPC)))
+           * (g->A + g->AC) * pow(g->R, 2.0 / 3.0) * sqrt(c->s0);
+}
+
+/* Outflow from the final coefficients, wavefront_mc.cu:321-328. */
+static double outflow(const mcoef_t *m, double qup, double quc, double qdp)
+{
+    double routed = m->c1 * qup + m->c2 * quc + m->c3 * qdp;
+    double qmc = routed + m->c4;
+    if (qmc < 0.0) {
+        if (m->c4 < 0.0 && fabs(m->c4) > routed) return 0.0;
+        return fmax(m->c1 * qup + m->c2 * quc + m->c4, m->c1 * qup + m->c3 * qdp + m->c4);
+    }
+    return qmc;
+}
+
+/* Trapezoid-only velocity at the final depth, wavefront_mc.cu:330-336. */
+static double velocity(double h, const chan_t *c)
+{
+    double twl = c->bw + 2.0 * c->z * h;
+    double Rv = (h * (c->bw + twl) / 2.0) /
+                (c->bw + 2.0 * sqrt(((twl - c->bw) / 2.0) * ((twl - c->bw) / 2.0) + h * h));
+    if (Rv < 0.0) Rv = 0.0;
+    return (1.0 / c->n_ch) * pow(Rv, 2.0 / 3.0) * sqrt(c->s0);
+}
+
+double orc_mc_solve(double dt, double qup, double quc, double qdp, double ql,
+                    const orc_chparams_f64 *p, double depthp,
+                    double *depthc, double *velc, int *iters)
+{
+    chan_t c;
+    chan_init(p, &c);
+    if (iters) *iters = 0;
+    /* guards, wavefront_mc.cu:227-232 */
+    if (chan_invalid(&c) || (ql <= 0.0 && qup <= 0.0 && quc <= 0.0 && qdp <= 0.0)) {
+        *depthc = 0.0; *velc = 0.0;
+        return 0.0;
+    }
+    const double mindepth = 0.01;
+    double d0 = fmax(depthp, 0.0);
+    double h = d0 * 1.33 + mindepth;   /* upper secant point, :235 */
+    double h_0 = d0 * 0.67;            /* lower secant point, :236 */
+    mcoef_t C = {0.0, 0.0, 0.0, 0.0};
+    double X = 0.25, Qj_0 = 0.0, Qj = 0.0;
+    int maxiter = 100, total = 0;
+    (void)X;
+
+    /* attempt loop with bracket widening, :242 and :315-318 */
+    for (int attempt = 0; attempt < 5; ++attempt) {
+        double aerror = 0.01, rerror = 1.0;
+        int iter = 0;
+        while (rerror > 0.01 && aerror >= mindepth && iter <= maxiter) {
+            geo_t g0, g1;
+            mcoef_t m0;
+            geometry(h_0, &c, &g0);
+            double ck0 = celerity(h_0, &g0, &c);
+            double km0 = storage_km(ck0, dt, &c);
+            double x0 = weighting(ck0, weight_width(h_0, &g0, &c), Qj_0, 0.0, &c);
+            muskingum(km0, x0, dt, ql, &m0);
+            double q0 = m0.c1 * qup + m0.c2 * quc + m0.c3 * qdp + m0.c4;
+            Qj_0 = q0 - manning(&g0, &c);
+
+            geometry(h, &c, &g1);
+            double ck1 = celerity(h, &g1, &c);
+            double km1 = storage_km(ck1, dt, &c);
+            X = weighting(ck1, weight_width(h, &g1, &c), q0, 0.25, &c);
+            muskingum(km1, X, dt, ql, &C);
+            double routed = C.c1 * qup + C.c2 * quc + C.c3 * qdp;
+            if (C.c4 < 0.0 && fabs(C.c4) > routed) C.c4 = -routed;
+            Qj = (routed + C.c4) - manning(&g1, &c);
+
+            /* secant update, :306-313 */
+            double h_1 = (Qj_0 - Qj != 0.0) ? h - (Qj * (h_0 - h)) / (Qj_0 - Qj) : h;
+            if (h_1 < 0.0) h_1 = h;
+            if (h > 0.0) { rerror = fabs((h_1 - h) / h); aerror = fabs(h_1 - h); }
+            else { rerror = 0.0; aerror = 0.9; }
+            h_0 = fmax(0.0, h);
+            h = fmax(0.0, h_1);
+            ++iter;
+            if (h < mindepth) break;
+        }
+        total += iter;
+        if (iter < maxiter) break;
+        h *= 1.33;
+        h_0 *= 0.67;
+        maxiter += 25;
+    }
+    if (iters) *iters = total;
+    *velc = velocity(h, &c);
+    *depthc = h;
+    return outflow(&C, qup, quc, qdp);
+}
+
+/* ------------------------------------------------------------------------ */
+/* DAG routing over T steps, wavefront_mc.cu:495-564.                       */
+
+static void route_dag(const orc_chparams_f64 *params,
+                      const int *reach_seg_start, const int *reach_seg_len,
+                      const int *reach_up_ptr, const int *reach_up_idx,
+                      const int *level_ptr, const int *level_reach,
+                      int n_reaches, int n_levels, int n_segments,
+                      const float *qlat_f, const double *qlat_d,
+                      double *q_prev, double *d_prev, double *rl_prev,
+                      double *v_last, double *reach_hist,
+                      int n_timesteps, double dt, int mode, long long *iter_sum)
+{
+    double *q_cur = (double *)calloc((size_t)n_segments + 1, sizeof(double));
+    double *d_cur = (double *)calloc((size_t)n_segments + 1, sizeof(double));
+    double *rl_cur = (double *)calloc((size_t)n_reaches + 1, sizeof(double));
+    long long its = 0;
+
+    for (int t = 0; t < n_timesteps; ++t) {
+        size_t row = (size_t)t * (size_t)n_segments;
+        for (int L = 0; L < n_levels; ++L) {
+            for (int ii = level_ptr[L]; ii < level_ptr[L + 1]; ++ii) {
+                int r = level_reach[ii];
+                /* fixed-order upstream sums in CSR order, :523-529 */
+                double qup_r = 0.0, quc_r = 0.0;
+                for (int k = reach_up_ptr[r]; k < reach_up_ptr[r + 1]; ++k) {
+                    int u = reach_up_idx[k];
+                    qup_r += rl_prev[u];
+                    quc_r += rl_cur[u];
+                }
+                double quc_next = 0.0, last = 0.0;
+                int s0 = reach_seg_start[r];
+                for (int i = 0; i < reach_seg_len[r]; ++i) {
+                    int idx = s0 + i;
+                    double qup = (i == 0) ? qup_r : q_prev[idx - 1];
+                    double quc = (i == 0) ? quc_r : quc_next;
+                    double ql = qlat_f ? (double)qlat_f[row + idx] : qlat_d[row + idx];
+                    double dep, vel, qdc;
+                    int it = 0;
+                    if (mode == ORC_MODE_CONVERGED)
+                        qdc = orc_mc_solve_converged(dt, qup, quc, q_prev[idx], ql,
+                                                     &params[idx], d_prev[idx], &dep, &vel);
+                    else
+                        qdc = orc_mc_solve(dt, qup, quc, q_prev[idx], ql, &params[idx],
+                                           d_prev[idx], &dep, &vel, &it);
+                    its += it;
+                    q_cur[idx] = qdc;
+                    d_cur[idx] = dep;
+                    if (v_last && t == n_timesteps - 1) v_last[idx] = vel;
+                    quc_next = qdc;
+                    last = qdc;
+                }
+                rl_cur[r] = last;
+            }
+        }
+        /* roll state, :554-559 */
+        memcpy(q_prev, q_cur, (size_t)n_segments * sizeof(double));
+        memcpy(d_prev, d_cur, (size_t)n_segments * sizeof(double));
+        memcpy(rl_prev, rl_cur, (size_t)n_reaches * sizeof(double));
+        if (reach_hist) memcpy(reach_hist + (size_t)t * n_reaches, rl_cur, (size_t)n_reaches * sizeof(double));
+        memset(rl_cur, 0, (size_t)n_reaches * sizeof(double));
+        memset(q_cur, 0, (size_t)n_segments * sizeof(double));
+        memset(d_cur, 0, (size_t)n_segments * sizeof(double));
+    }
+    if (iter_sum) *iter_sum += its;
+    free(q_cur); free(d_cur); free(rl_cur);
+}
+
+void orc_wavefront_f32in(const orc_chparams_f32 *params,
+                         const int *reach_seg_start, const int *reach_seg_len,
+                         const int *reach_up_ptr, const int *reach_up_idx,
+                         const int *level_ptr, const int *level_reach,
+                         int n_reaches, int n_levels, int n_segments,
+                         const float *qlat_ts, const float *qdp0_seg,
+                         const float *dp0_seg, const float *reach_last0,
+                         double *qdc_seg_out_d, double *reach_last_out_d,
+                         int n_timesteps, double dt, long long *iter_sum)
+{
+    /* float params are widened field by field, :536-538 */
+    orc_chparams_f64 *pd = (orc_chparams_f64 *)malloc(((size_t)n_segments + 1) * sizeof(*pd));
+    double *q = (double *)malloc(((size_t)n_segments + 1) * sizeof(double));
+    double *d = (double *)malloc(((size_t)n_segments + 1) * sizeof(double));
+    double *rl = (double *)malloc(((size_t)n_reaches + 1) * sizeof(double));
+    for (int i = 0; i < n_segments; ++i) {
+        const orc_chparams_f32 *s = &params[i];
+        orc_chparams_f64 w = {s->dx, s->bw, s->tw, s->twcc, s->n_ch, s->ncc, s->cs, s->s0};
+        pd[i] = w;
+        q[i] = qdp0_seg[i];
+        d[i] = dp0_seg[i];
+    }
+    for (int r = 0; r < n_reaches; ++r) rl[r] = reach_last0[r];
+    route_dag(pd, reach_seg_start, reach_seg_len, reach_up_ptr, reach_up_idx,
+              level_ptr, level_reach, n_reaches, n_levels, n_segments,
+              qlat_ts, NULL, q, d, rl, NULL, NULL, n_timesteps, dt,
+              ORC_MODE_REFERENCE, iter_sum);
+    memcpy(qdc_seg_out_d, q, (size_t)n_segments * sizeof(double));
+    memcpy(reach_last_out_d, rl, (size_t)n_reaches * sizeof(double));
+    free(pd); free(q); free(d); free(rl);
+}
+
+void orc_wavefront_f64(const orc_chparams_f64 *params,
+                       const int *reach_seg_start, const int *reach_seg_len,
+                       const int *reach_up_ptr, const int *reach_up_idx,
+                       const int *level_ptr, const int *level_reach,
+                       int n_reaches, int n_levels, int n_segments,
+                       const double *qlat_ts, const double *qdp0_seg,
+                       const double *dp0_seg, const double *reach_last0,
+                       double *qdc_seg_out, double *reach_last_out,
+                       double *depth_out, double *vel_out, double *reach_hist,
+                       int n_timesteps, double dt, int mode, long long *iter_sum)
+{
+    double *q = (double *)malloc(((size_t)n_segments + 1) * sizeof(double));
+    double *d = (double *)malloc(((size_t)n_segments + 1) * sizeof(double));
+    double *rl = (double *)malloc(((size_t)n_reaches + 1) * sizeof(double));
+    memcpy(q, qdp0_seg, (size_t)n_segments * sizeof(double));
+    memcpy(d, dp0_seg, (size_t)n_segments * sizeof(double));
+    memcpy(rl, reach_last0, (size_t)n_reaches * sizeof(double));
+    if (vel_out) memset(vel_out, 0, (size_t)n_segments * sizeof(double));
+    route_dag(params, reach_seg_start, reach_seg_len, reach_up_ptr, reach_up_idx,
+              level_ptr, level_reach, n_reaches, n_levels, n_segments,
+              NULL, qlat_ts, q, d, rl, vel_out, reach_hist, n_timesteps, dt, mode, iter_sum);
+    memcpy(qdc_seg_out, q, (size_t)n_segments * sizeof(double));
+    memcpy(reach_last_out, rl, (size_t)n_reaches * sizeof(double));
+    if (depth_out) memcpy(depth_out, d, (size_t)n_segments * sizeof(double));
+    free(q); free(d); free(rl);
+}
+
+/* troute_mc_final.cu:167-252.  The inlined solve there is the same
+ * arithmetic as mc_solve_d; the invalid/zero-flow guard zeroes the outputs
+ * and passes quc = 0 to the next segment (:187-190), which is what a zero
+ * return from the solve does as well. */
+void orc_reach_parallel(const orc_chparams_f64 *p, const double *ql,
+                        const int *rs, const int *rl,
+                        const double *qu, const double *qd, const double *dp,
+                        double *qo, double *vo, double *dpo,
+                        double uq, double dt, int nr, long long *iter_sum)
+{
+    long long its = 0;
+    for (int r = 0; r < nr; ++r) {
+        double quc = uq;
+        for (int i = 0; i < rl[r]; ++i) {
+            int idx = rs[r] + i;
+            double dep, vel;
+            int it = 0;
+            double qdc = orc_mc_solve(dt, qu[idx], quc, qd[idx], ql[idx], &p[idx],
+                                      dp[idx], &dep, &vel, &it);
+            its += it;
+            qo[idx] = qdc; vo[idx] = vel; dpo[idx] = dep;
+            quc = qdc;
+        }
+    }
+    if (iter_sum) *iter_sum += its;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Converged mode.  No reference counterpart.
+ *
+ * The reference's secant (mc_solve_d) stops at 1 % / 1 cm, so its answer is
+ * an iterate, not a root.  At a fixed point of that iteration h_0 == h and
+ * Qj_0 == 0, so X0 == 0.5 and the residual reduces to the single-valued
+ *
+ *   F(h) = Qmc(h) - Qmn(h),
+ *   Qmc(h) = C1 qup + C2 quc + C3 qdp + C4 with C(Km(h), X(h)) and the C4
+ *            clamp of wavefront_mc.cu:293-296,
+ *   X(h)   = weighting with lo = 0.25 from the routed flow of the X0 = 0.5
+ *            coefficients at the same h.
+ *
+ * The converged depth h* is the smallest root of F on [mindepth, inf).  The
+ * in-bank and overbank branches of the compound section are each scanned, so
+ * the root below bankfull is found first when one exists; h* = mindepth when
+ * F(mindepth) <= 0.  Outputs then follow wavefront_mc.cu:321-336 at h*.      */
+
+static double residual_at(double h, double dt, double qup, double quc, double qdp,
+                          double ql, const chan_t *c, mcoef_t *Cout)
+{
+    geo_t g;
+    mcoef_t m0, C;
+    geometry(h, c, &g);
+    double ck = celerity(h, &g, c);
+    double km = storage_km(ck, dt, c);
+    double twu = weight_width(h, &g, c);
+    muskingum(km, 0.5, dt, ql, &m0);
+    double q0 = m0.c1 * qup + m0.c2 * quc + m0.c3 * qdp + m0.c4;
+    double X = weighting(ck, twu, q0, 0.25, c);
+    muskingum(km, X, dt, ql, &C);
+    double routed = C.c1 * qup + C.c2 * quc + C.c3 * qdp;
+    if (C.c4 < 0.0 && fabs(C.c4) > routed) C.c4 = -routed;
+    if (Cout) *Cout = C;
+    return (routed + C.c4) - manning(&g, c);
+}
+
+double orc_mc_residual(double dt, double qup, double quc, double qdp, double ql,
+                       const orc_chparams_f64 *p, double h, double *qmc_out)
+{
+    chan_t c;
+    mcoef_t C;
+    chan_init(p, &c);
+    double f = residual_at(h, dt, qup, quc, qdp, ql, &c, &C);
+    if (qmc_out) *qmc_out = C.c1 * qup + C.c2 * quc + C.c3 * qdp + C.c4;
+    return f;
+}
+
+/* Smallest root of f on [lo, inf): geometric scan (x1.25) that also visits
+ * the breakpoint bp (bankfull), then bisection to a 2-ulp bracket.  Returns lo
+ * when f(lo) <= 0. */
+typedef double (*scalar_fn)(double h, const void *ctx);
+
+static double smallest_root(scalar_fn f, const void *ctx, double lo, double bp)
+{
+    double a = lo, fa = f(a, ctx);
+    if (!(fa > 0.0)) return lo;
+    double b = a, fb = fa;
+    int used_bp = !(bp > lo);
+    for (int k = 0; k < 400; ++k) {
+        double nb = b * 1.25;
+        if (!used_bp && nb >= bp) { nb = bp; used_bp = 1; }
+        double fnb = f(nb, ctx);
+        if (!(fnb > 0.0)) { b = nb; fb = fnb; break; }
+        a = nb; fa = fnb; b = nb;
+        if (k == 399) return b;   /* no sign change found: stays at the last probe */
+    }
+    (void)fa; (void)fb;
+    for (int k = 0; k < 200; ++k) {
+        double m = a + (b - a) / 2.0;
+        if (!(m > a && m < b)) break;
+        if (f(m, ctx) > 0.0) a = m; else b = m;
+    }
+    return a + (b - a) / 2.0;
+}
+
+typedef struct { double dt, qup, quc, qdp, ql; const chan_t *c; } route_ctx;
+
+static double route_residual_fn(double h, const void *vctx)
+{
+    const route_ctx *x = (const route_ctx *)vctx;
+    return residual_at(h, x->dt, x->qup, x->quc, x->qdp, x->ql, x->c, NULL);
+}
+
+double orc_mc_solve_converged(double dt, double qup, double quc, double qdp, double ql,
+                              const orc_chparams_f64 *p, double depthp,
+                              double *depthc, double *velc)
+{
+    chan_t c;
+    mcoef_t C;
+    (void)depthp;
+    chan_init(p, &c);
+    if (chan_invalid(&c) || (ql <= 0.0 && qup <= 0.0 && quc <= 0.0 && qdp <= 0.0)) {
+        *depthc = 0.0; *velc = 0.0;
+        return 0.0;
+    }
+    route_ctx x = {dt, qup, quc, qdp, ql, &c};
+    double bp = (c.twcc > 0.0) ? c.bfd : -1.0;
+    double h = smallest_root(route_residual_fn, &x, 0.01, bp);
+    residual_at(h, dt, qup, quc, qdp, ql, &c, &C);
+    *velc = velocity(h, &c);
+    *depthc = h;
+    return outflow(&C, qup, quc, qdp);
+}
+
+double orc_manning_flow(double h, const orc_chparams_f64 *p)
+{
+    chan_t c;
+    geo_t g;
+    chan_init(p, &c);
+    geometry(h, &c, &g);
+    return manning(&g, &c);
+}
+
+typedef struct { double Q; const chan_t *c; } nd_ctx;
+
+static double normal_residual_fn(double h, const void *vctx)
+{
+    const nd_ctx *x = (const nd_ctx *)vctx;
+    geo_t g;
+    geometry(h, x->c, &g);
+    return x->Q - manning(&g, x->c);
+}
+
+double orc_normal_depth(double Q, const orc_chparams_f64 *p)
+{
+    chan_t c;
+    chan_init(p, &c);
+    if (!(Q > 0.0) || chan_invalid(&c)) return 0.0;
+    nd_ctx x = {Q, &c};
+    double bp = (c.twcc > 0.0) ? c.bfd : -1.0;
+    return smallest_root(normal_residual_fn, &x, 1e-6, bp);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Accuracy statistics, wavefront_mc.cu:566-572 (pct) and :822-858.          */
+
+static int cmp_double(const void *a, const void *b)
+{
+    double x = *(const double *)a, y = *(const double *)b;
+    return (x > y) - (x < y);
+}
+
+static double pct_sorted(const double *v, size_t n, double p)
+{
+    if (n == 0) return 0.0;
+    size_t k = (size_t)((p / 100.0) * (double)(n - 1));
+    return v[k];
+}
+
+void orc_compute_stats(const double *pred, const double *truth, int n, double *out)
+{
+    double *rel = (double *)malloc(((size_t)n + 1) * sizeof(double));
+    size_t m = 0, w1 = 0, w10 = 0;
+    int nans = 0;
+    double max_abs = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double g = pred[i], t = truth[i];
+        if (isnan(g) || isinf(g)) { ++nans; continue; }
+        double ae = fabs(g - t);
+        max_abs = fmax(max_abs, ae);
+        if (fabs(t) > 1e-3) {
+            double re = ae / fabs(t);
+            rel[m++] = re;
+            if (re < 0.01) ++w1;
+            if (re < 0.1) ++w10;
+        }
+    }
+    qsort(rel, m, sizeof(double), cmp_double);
+    out[0] = max_abs;
+    out[1] = pct_sorted(rel, m, 50.0);
+    out[2] = pct_sorted(rel, m, 90.0);
+    out[3] = pct_sorted(rel, m, 99.0);
+    out[4] = pct_sorted(rel, m, 99.9);
+    out[5] = m ? rel[m - 1] : 0.0;
+    out[6] = m ? (double)w1 / (double)m : 1.0;
+    out[7] = m ? (double)w10 / (double)m : 1.0;
+    out[8] = (double)nans;
+    free(rel);
+}

@@ -1,0 +1,118 @@
+/*
+ * CPU oracle for the Muskingum-Cunge (MC) routing path -- TEST INFRASTRUCTURE.
+ *
+ * This is the checker, never the product: only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it.  The shipped routing path is the
+ * HIP library under noaa-gpu-kernels_amd/; it has no CPU fallback into this code.
+ *
+ * Every function restates the reference's FP64 host algorithm in plain C with
+ * the same operation order (compile with -O2 -ffp-contract=off so no FMA
+ * contraction changes rounding).  Citations are to the reference tree
+ * (consigcody94/noaa-gpu-kernels):
+ *
+ *   orc_mc_solve            water-prediction/t-route/wavefront/wavefront_mc.cu:215-337 (mc_solve_d)
+ *                           geometry helper :200-213 (hgeo_inline_d)
+ *   orc_wavefront_f32in     wavefront_mc.cu:495-564 (cpu_wavefront_fp64)
+ *   orc_wavefront_f64       same algorithm, every array double (FP64 end to end)
+ *   orc_reach_parallel      water-prediction/t-route/troute_mc_final.cu:167-252 (cpu_mc)
+ *   orc_compute_stats       wavefront_mc.cu:566-572,822-858 (pct / accuracy block)
+ *
+ * Converged mode (no reference counterpart; the reference stops its secant at a
+ * loose 1 % / 1 cm tolerance):
+ *   orc_mc_solve_converged  root of the same MC residual solved to machine
+ *                           precision by a bracketed bisection/secant hybrid
+ *   orc_wavefront_converged DAG + time loop of wavefront_mc.cu:495-564 with the
+ *                           converged solve
+ *   orc_normal_depth        depth h with Manning flow Qmn(h) == Q (training
+ *                           targets for the learned initial guess)
+ *
+ * Parity pinning (how each function is checked, see tests/test_oracle.py):
+ *   - orc_mc_solve, orc_wavefront_f32in, orc_reach_parallel: bit-for-bit against
+ *     the reference's own host routines compiled from the reference sources into
+ *     oracle/_ref (when that build exists) and against committed golden vectors
+ *     written from it (tests/golden/), plus invariants (C1+C2+C3 == 1, steady
+ *     state, guards, hand-traced tiny networks).
+ *   - orc_wavefront_f64: identical to orc_wavefront_f32in on float-representable
+ *     inputs (checked), so it inherits that pin.
+ *   - orc_mc_solve_converged / orc_normal_depth: residual |F(h*)| at the root,
+ *     brute-force dense scan on small cases, the sign change bracketing the root,
+ *     and the steady-state closed form.  No reference counterpart exists, so
+ *     these are pinned by mathematics only.
+ */
+#ifndef MC_ORACLE_H
+#define MC_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Channel parameters of one segment, in the reference's field order
+ * (troute_mc_final.cu:26-28 double, wavefront_mc.cu:41-44 float). */
+typedef struct { double dx, bw, tw, twcc, n_ch, ncc, cs, s0; } orc_chparams_f64;
+typedef struct { float  dx, bw, tw, twcc, n_ch, ncc, cs, s0; } orc_chparams_f32;
+
+/* One segment solve; returns qdc, writes depth and velocity.  *iters (may be
+ * NULL) receives the number of secant iterations performed over all attempts. */
+double orc_mc_solve(double dt, double qup, double quc, double qdp, double ql,
+                    const orc_chparams_f64 *p, double depthp,
+                    double *depthc, double *velc, int *iters);
+
+/* Literal twin of cpu_wavefront_fp64: float params/forcings, double outputs
+ * (final-step qdc per segment and final-step last-segment qdc per reach).
+ * iter_sum (may be NULL) accumulates secant iterations over all solves. */
+void orc_wavefront_f32in(const orc_chparams_f32 *params,
+                         const int *reach_seg_start, const int *reach_seg_len,
+                         const int *reach_up_ptr, const int *reach_up_idx,
+                         const int *level_ptr, const int *level_reach,
+                         int n_reaches, int n_levels, int n_segments,
+                         const float *qlat_ts, const float *qdp0_seg,
+                         const float *dp0_seg, const float *reach_last0,
+                         double *qdc_seg_out_d, double *reach_last_out_d,
+                         int n_timesteps, double dt, long long *iter_sum);
+
+/* Same algorithm, every array double.  depth_out / vel_out (may be NULL) get
+ * the final-step per-segment depth and velocity.  reach_hist (may be NULL)
+ * gets reach_last after every step: reach_hist[t*n_reaches + r]. */
+void orc_wavefront_f64(const orc_chparams_f64 *params,
+                       const int *reach_seg_start, const int *reach_seg_len,
+                       const int *reach_up_ptr, const int *reach_up_idx,
+                       const int *level_ptr, const int *level_reach,
+                       int n_reaches, int n_levels, int n_segments,
+                       const double *qlat_ts, const double *qdp0_seg,
+                       const double *dp0_seg, const double *reach_last0,
+                       double *qdc_seg_out, double *reach_last_out,
+                       double *depth_out, double *vel_out, double *reach_hist,
+                       int n_timesteps, double dt, int mode, long long *iter_sum);
+
+/* cpu_mc: independent reaches, one step, constant upstream inflow uq into
+ * segment 0 of every reach. */
+void orc_reach_parallel(const orc_chparams_f64 *p, const double *ql,
+                        const int *rs, const int *rl,
+                        const double *qu, const double *qd, const double *dp,
+                        double *qo, double *vo, double *dpo,
+                        double uq, double dt, int nr, long long *iter_sum);
+
+/* Converged-mode residual F(h) = Qmc(h) - Qmn(h) (see mc_oracle.c). */
+double orc_mc_residual(double dt, double qup, double quc, double qdp, double ql,
+                       const orc_chparams_f64 *p, double h, double *qmc_out);
+
+double orc_mc_solve_converged(double dt, double qup, double quc, double qdp, double ql,
+                              const orc_chparams_f64 *p, double depthp,
+                              double *depthc, double *velc);
+
+/* Manning normal depth: h >= 0 with Qmn(h) == Q (Q > 0). */
+double orc_normal_depth(double Q, const orc_chparams_f64 *p);
+double orc_manning_flow(double h, const orc_chparams_f64 *p);
+
+/* Error statistics of wavefront_mc.cu:822-858: relative error over entries
+ * with |truth| > 1e-3.  out[0..8] = max_abs, p50, p90, p99, p99.9, max_rel,
+ * frac within 1 %, frac within 10 %, nan count. */
+void orc_compute_stats(const double *pred, const double *truth, int n, double *out);
+
+#define ORC_MODE_REFERENCE 0
+#define ORC_MODE_CONVERGED 1
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,209 @@
+"""ctypes bindings of the CPU oracle (oracle/liborc.so) -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this
+module, and only as the checker or the timed CPU baseline; the routing product
+(noaa-gpu-kernels_amd/libtrmc.so) never calls it.
+
+``load_ref()`` additionally binds oracle/_ref/libtroute_ref.so, the reference's
+own FP64 host routines compiled from the reference sources by
+oracle/build_ref.sh (present only where that build ran).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+D, I, LL = C.c_double, C.c_int, C.c_longlong
+pD, pI, pF = C.POINTER(D), C.POINTER(I), C.POINTER(C.c_float)
+FIELDS = ("dx", "bw", "tw", "twcc", "n_ch", "ncc", "cs", "s0")
+
+
+class P64(C.Structure):
+    _fields_ = [(n, D) for n in FIELDS]
+
+
+class P32(C.Structure):
+    _fields_ = [(n, C.c_float) for n in FIELDS]
+
+
+MODE_REFERENCE, MODE_CONVERGED = 0, 1
+_orc = None
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def lib() -> C.CDLL:
+    global _orc
+    if _orc is None:
+        path = os.path.join(HERE, "liborc.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} not built (make -C oracle)")
+        L = C.CDLL(path)
+        pP64, pP32 = C.POINTER(P64), C.POINTER(P32)
+        topo = [pI] * 6 + [I, I, I]
+        L.orc_mc_solve.restype = D
+        L.orc_mc_solve.argtypes = [D] * 5 + [pP64, D, pD, pD, pI]
+        L.orc_wavefront_f32in.restype = None
+        L.orc_wavefront_f32in.argtypes = [pP32] + topo + [pF] * 4 + [pD, pD, I, D, C.POINTER(LL)]
+        L.orc_wavefront_f64.restype = None
+        L.orc_wavefront_f64.argtypes = [pP64] + topo + [pD] * 4 + [pD] * 5 + [I, D, I, C.POINTER(LL)]
+        L.orc_reach_parallel.restype = None
+        L.orc_reach_parallel.argtypes = [pP64, pD, pI, pI, pD, pD, pD, pD, pD, pD, D, D, I, C.POINTER(LL)]
+        L.orc_mc_residual.restype = D
+        L.orc_mc_residual.argtypes = [D] * 5 + [pP64, D, pD]
+        L.orc_mc_solve_converged.restype = D
+        L.orc_mc_solve_converged.argtypes = [D] * 5 + [pP64, D, pD, pD]
+        L.orc_normal_depth.restype = D
+        L.orc_normal_depth.argtypes = [D, pP64]
+        L.orc_manning_flow.restype = D
+        L.orc_manning_flow.argtypes = [D, pP64]
+        L.orc_compute_stats.restype = None
+        L.orc_compute_stats.argtypes = [pD, pD, I, pD]
+        _orc = L
+    return _orc
+
+
+def _topo_args(topo):
+    keep = [np.ascontiguousarray(topo[k], np.int32) for k in
+            ("reach_seg_start", "reach_seg_len", "reach_up_ptr", "reach_up_idx", "level_ptr", "level_reach")]
+    return keep, [_ptr(a, I) for a in keep] + [int(topo["n_reaches"]), int(topo["n_levels"]),
+                                               int(topo["n_segments"])]
+
+
+def as_topo(t) -> dict:
+    """dict view of a topology (dict, npz-like mapping or object with attributes)."""
+    keys = ("n_reaches", "n_segments", "n_levels", "reach_seg_start", "reach_seg_len",
+            "reach_up_ptr", "reach_up_idx", "level_ptr", "level_reach")
+    if isinstance(t, dict):
+        return {k: t[k] for k in keys}
+    return {k: getattr(t, k) for k in keys}
+
+
+def mc_solve(params8, qup, quc, qdp, ql, depthp, dt=300.0):
+    """One segment: returns (qdc, depth, velocity, iterations)."""
+    p = P64(*map(float, params8))
+    d, v, it = D(), D(), I()
+    q = lib().orc_mc_solve(dt, qup, quc, qdp, ql, C.byref(p), depthp, C.byref(d), C.byref(v), C.byref(it))
+    return q, d.value, v.value, it.value
+
+
+def mc_solve_many(params, inputs, dt=300.0, mode=MODE_REFERENCE):
+    """params [n, 8], inputs [n, 5] (qup, quc, qdp, ql, depthp) -> out [n, 3], iters [n]."""
+    params = np.asarray(params, np.float64)
+    inputs = np.asarray(inputs, np.float64)
+    out = np.zeros((len(params), 3))
+    its = np.zeros(len(params), np.int32)
+    for i in range(len(params)):
+        p = P64(*params[i])
+        d, v, it = D(), D(), I()
+        x = inputs[i]
+        if mode == MODE_REFERENCE:
+            q = lib().orc_mc_solve(dt, x[0], x[1], x[2], x[3], C.byref(p), x[4], C.byref(d), C.byref(v), C.byref(it))
+        else:
+            q = lib().orc_mc_solve_converged(dt, x[0], x[1], x[2], x[3], C.byref(p), x[4], C.byref(d), C.byref(v))
+        out[i] = (q, d.value, v.value)
+        its[i] = it.value
+    return out, its
+
+
+def wavefront_f32in(topo, params32, qlat32, qdp0, dp0, reach_last0, n_timesteps, dt=300.0):
+    """cpu_wavefront_fp64 restated: returns (qdc_out [N], reach_last_out [R], iterations)."""
+    t = as_topo(topo)
+    keep, targs = _topo_args(t)
+    p = np.ascontiguousarray(params32, np.float32)
+    q = np.ascontiguousarray(qlat32, np.float32)
+    q0, d0, r0 = (np.ascontiguousarray(x, np.float32) for x in (qdp0, dp0, reach_last0))
+    qo = np.zeros(int(t["n_segments"]))
+    ro = np.zeros(int(t["n_reaches"]))
+    its = LL(0)
+    lib().orc_wavefront_f32in(_ptr(p, P32), *targs, _ptr(q, C.c_float), _ptr(q0, C.c_float),
+                              _ptr(d0, C.c_float), _ptr(r0, C.c_float), _ptr(qo, D), _ptr(ro, D),
+                              int(n_timesteps), float(dt), C.byref(its))
+    return qo, ro, its.value
+
+
+def wavefront_f64(topo, params, qlat, qdp0, dp0, reach_last0, n_timesteps, dt=300.0,
+                  mode=MODE_REFERENCE, history=False):
+    """All-double DAG routing.  Returns dict with qdc, depth, vel, reach_last,
+    iterations and (history=True) reach_hist [T, R]."""
+    t = as_topo(topo)
+    keep, targs = _topo_args(t)
+    N, R = int(t["n_segments"]), int(t["n_reaches"])
+    p = np.ascontiguousarray(params, np.float64)
+    q = np.ascontiguousarray(qlat, np.float64)
+    q0, d0, r0 = (np.ascontiguousarray(x, np.float64) for x in (qdp0, dp0, reach_last0))
+    qo, do, vo, ro = np.zeros(N), np.zeros(N), np.zeros(N), np.zeros(R)
+    hist = np.zeros((int(n_timesteps), R)) if history else None
+    its = LL(0)
+    lib().orc_wavefront_f64(_ptr(p, P64), *targs, _ptr(q, D), _ptr(q0, D), _ptr(d0, D), _ptr(r0, D),
+                            _ptr(qo, D), _ptr(ro, D), _ptr(do, D), _ptr(vo, D),
+                            _ptr(hist, D) if history else pD(), int(n_timesteps), float(dt), int(mode),
+                            C.byref(its))
+    out = dict(qdc=qo, depth=do, vel=vo, reach_last=ro, iterations=its.value)
+    if history:
+        out["reach_hist"] = hist
+    return out
+
+
+def reach_parallel(params, ql, rs, rl, qu, qd, dp, uq=10.0, dt=300.0):
+    p = np.ascontiguousarray(params, np.float64)
+    ql, qu, qd, dp = (np.ascontiguousarray(x, np.float64) for x in (ql, qu, qd, dp))
+    rs, rl = np.ascontiguousarray(rs, np.int32), np.ascontiguousarray(rl, np.int32)
+    n = len(p)
+    qo, vo, dpo = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().orc_reach_parallel(_ptr(p, P64), _ptr(ql, D), _ptr(rs, I), _ptr(rl, I), _ptr(qu, D), _ptr(qd, D),
+                             _ptr(dp, D), _ptr(qo, D), _ptr(vo, D), _ptr(dpo, D), uq, dt, len(rs), None)
+    return qo, vo, dpo
+
+
+def residual(params8, qup, quc, qdp, ql, h, dt=300.0):
+    p = P64(*map(float, params8))
+    qmc = D()
+    f = lib().orc_mc_residual(dt, qup, quc, qdp, ql, C.byref(p), h, C.byref(qmc))
+    return f, qmc.value
+
+
+def solve_converged(params8, qup, quc, qdp, ql, depthp, dt=300.0):
+    p = P64(*map(float, params8))
+    d, v = D(), D()
+    q = lib().orc_mc_solve_converged(dt, qup, quc, qdp, ql, C.byref(p), depthp, C.byref(d), C.byref(v))
+    return q, d.value, v.value
+
+
+def normal_depth(params8, Q):
+    p = P64(*map(float, params8))
+    return lib().orc_normal_depth(Q, C.byref(p))
+
+
+def manning_flow(params8, h):
+    p = P64(*map(float, params8))
+    return lib().orc_manning_flow(h, C.byref(p))
+
+
+def compute_stats(pred, truth):
+    pred = np.ascontiguousarray(pred, np.float64).ravel()
+    truth = np.ascontiguousarray(truth, np.float64).ravel()
+    out = np.zeros(9)
+    lib().orc_compute_stats(_ptr(pred, D), _ptr(truth, D), pred.size, _ptr(out, D))
+    keys = ("max_abs", "p50", "p90", "p99", "p99_9", "max_rel", "within_1pct", "within_10pct", "nans")
+    return dict(zip(keys, out.tolist()))
+
+
+def load_ref():
+    """The reference's own host routines (oracle/_ref), or None if not built."""
+    path = os.path.join(HERE, "_ref", "libtroute_ref.so")
+    if not os.path.exists(path):
+        return None
+    ref = C.CDLL(path)
+    ref.ref_mc_solve_d.restype = D
+    ref.ref_mc_solve_d.argtypes = [D] * 14 + [pD, pD]
+    ref.ref_cpu_wavefront_fp64.restype = None
+    ref.ref_cpu_wavefront_fp64.argtypes = [C.c_void_p] + [pI] * 6 + [I, I, I] + [pF] * 4 + [pD, pD, I, D]
+    ref.ref_cpu_mc.restype = None
+    ref.ref_cpu_mc.argtypes = [C.c_void_p, pD, pI, pI, pD, pD, pD, pD, pD, pD, D, D, I]
+    return ref

@@ -1,0 +1,259 @@
+"""Pins the CPU oracle (oracle/) before anything is checked against it.
+
+* bit-for-bit against golden vectors written by the reference's own host
+  routines (tests/golden/make_golden.py over oracle/_ref),
+* bit-for-bit against oracle/_ref live where that build exists,
+* against what the mathematics fixes: C1+C2+C3 == 1 (steady state), the DAG
+  fixed point Q_out = Q_in + q_lat, closed-form geometry, guards, hand-traced
+  tiny networks, and the converged-mode root conditions.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden, load_oracle, tiny_cases
+
+orc = load_oracle()
+
+
+def bits(x):
+    return np.asarray(x, np.float64).view(np.uint64)
+
+
+# ----------------------------------------------------------- golden pins
+def test_mc_solve_matches_reference_golden():
+    z = golden("mc_solve_cases.npz")
+    out, _ = orc.mc_solve_many(z["params"], z["inputs"], float(z["dt"]))
+    assert np.array_equal(bits(out), bits(z["expected"])), "oracle mc_solve differs from mc_solve_d"
+
+
+def test_wavefront_c1_matches_reference_golden():
+    z = golden("wavefront_c1.npz")
+    d = {k: z[k] for k in z.files}
+    for T, qk, rk in ((int(d["n_timesteps"]), "qdc_out", "reach_last_out"), (8, "qdc_out_8", "reach_last_out_8")):
+        qo, ro, its = orc.wavefront_f32in(d, d["params"], d["qlat"][:T], d["qdp0"], d["dp0"],
+                                          d["reach_last0"], T, float(d["dt"]))
+        assert np.array_equal(bits(qo), bits(d[qk])), f"qdc differs at T={T}"
+        assert np.array_equal(bits(ro), bits(d[rk])), f"reach_last differs at T={T}"
+        assert its > 0
+
+
+@pytest.mark.parametrize("name", sorted(tiny_cases()))
+def test_wavefront_tiny_matches_reference_golden(name):
+    d = tiny_cases()[name]
+    T = int(d["n_timesteps"])
+    qo, ro, _ = orc.wavefront_f32in(d, d["params"], d["qlat"], d["qdp0"], d["dp0"], d["reach_last0"], T)
+    assert np.array_equal(bits(qo), bits(d["qdc_out"]))
+    assert np.array_equal(bits(ro), bits(d["reach_last_out"]))
+
+
+def test_reach_parallel_matches_reference_golden():
+    z = golden("reach_parallel.npz")
+    qo, vo, dpo = orc.reach_parallel(z["params"], z["ql"], z["rs"], z["rl"], z["qu"], z["qd"], z["dp"],
+                                     float(z["uq"]), float(z["dt"]))
+    assert np.array_equal(bits(qo), bits(z["qo"]))
+    assert np.array_equal(bits(vo), bits(z["vo"]))
+    assert np.array_equal(bits(dpo), bits(z["dpo"]))
+
+
+def test_f64_variant_equals_f32in_on_float_inputs():
+    z = golden("wavefront_c1.npz")
+    d = {k: z[k] for k in z.files}
+    T = 48
+    a, b, _ = orc.wavefront_f32in(d, d["params"], d["qlat"][:T], d["qdp0"], d["dp0"], d["reach_last0"], T)
+    r = orc.wavefront_f64(d, d["params"].astype(np.float64), d["qlat"][:T].astype(np.float64),
+                          d["qdp0"].astype(np.float64), d["dp0"].astype(np.float64),
+                          d["reach_last0"].astype(np.float64), T)
+    assert np.array_equal(bits(a), bits(r["qdc"]))
+    assert np.array_equal(bits(b), bits(r["reach_last"]))
+
+
+def test_oracle_matches_live_reference_build():
+    ref = orc.load_ref()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    rng = np.random.default_rng(99)
+    D = C.c_double
+    for _ in range(3000):
+        p = [rng.uniform(300, 5000), rng.uniform(0.5, 50), 0, 0, rng.uniform(0.02, 0.08), 0,
+             rng.uniform(0.5, 3), 10 ** rng.uniform(-4.5, -2)]
+        p[2] = p[1] + rng.uniform(-2, 20)
+        p[3] = p[2] + rng.uniform(-5, 100)
+        p[5] = p[4] * rng.uniform(1.2, 2)
+        q = 10 ** rng.uniform(-3, 4, 3)
+        ql, dp = rng.uniform(-0.2, 2), rng.uniform(0, 15)
+        a = orc.mc_solve(p, q[0], q[1], q[2], ql, dp)
+        d, v = D(), D()
+        b = ref.ref_mc_solve_d(300.0, q[0], q[1], q[2], ql, *p, dp, C.byref(d), C.byref(v))
+        assert bits([a[0], a[1], a[2]]).tolist() == bits([b, d.value, v.value]).tolist()
+
+
+# ------------------------------------------------------ mathematics pins
+BASE = [2000.0, 10.0, 20.0, 60.0, 0.05, 0.075, 2.0, 1e-3]
+
+
+def test_guards_return_zeros():
+    for p in ([2000, 10, 20, 60, 0.0, 0.075, 2, 1e-3], [2000, 10, 20, 60, 0.05, 0.075, 2, 0.0],
+              [2000, 0.0, 20, 60, 0.05, 0.075, 2, 1e-3], [2000, 10, 20, 60, -0.05, 0.075, 2, 1e-3]):
+        assert orc.mc_solve(p, 5, 5, 5, 0.1, 1.0)[:3] == (0.0, 0.0, 0.0)
+    assert orc.mc_solve(BASE, 0, 0, 0, 0, 1.0)[:3] == (0.0, 0.0, 0.0)
+    assert orc.mc_solve(BASE, -1, 0, -2, -0.5, 1.0)[:3] == (0.0, 0.0, 0.0)
+
+
+def test_steady_state_conserves_flow():
+    """qup = quc = qdp = Q, ql = 0 gives Q (C1 + C2 + C3 = 1) for any iterate."""
+    rng = np.random.default_rng(3)
+    for _ in range(500):
+        p = [rng.uniform(500, 5000), rng.uniform(1, 50), 0, 0, rng.uniform(0.03, 0.08), 0,
+             rng.uniform(1, 3), 10 ** rng.uniform(-4, -2)]
+        p[2] = p[1] + rng.uniform(2, 20)
+        p[3] = p[2] + rng.uniform(5, 100)
+        p[5] = p[4] * rng.uniform(1.2, 2)
+        Q = 10 ** rng.uniform(-2, 4)
+        q, depth, vel, it = orc.mc_solve(p, Q, Q, Q, 0.0, rng.uniform(0.01, 10))
+        assert abs(q - Q) <= 1e-12 * Q
+        assert depth > 0 and vel > 0 and it >= 1
+
+
+def test_lateral_only_is_bounded():
+    """qup = quc = qdp = 0: outflow = C4 = ql dt / D with D >= dt/2, so 0 < q <= 2 ql."""
+    for ql in (1e-3, 0.1, 3.0):
+        q, _, _, _ = orc.mc_solve(BASE, 0, 0, 0, ql, 0.5)
+        assert 0 < q <= 2 * ql * (1 + 1e-15)
+
+
+def test_manning_closed_form_in_bank_and_overbank():
+    n, s0 = 0.05, 1e-3
+    p = [2000, 10.0, 20.0, 60.0, n, 0.075, 0.5, s0]   # z = 2, bankfull (20-10)/(2*2) = 2.5
+    h = 1.0
+    A = (10 + h * 2) * h
+    WP = 10 + 2 * h * math.sqrt(5)
+    want = (1 / n) * A * (A / WP) ** (2 / 3) * math.sqrt(s0)
+    assert abs(orc.manning_flow(p, h) - want) <= 1e-14 * want
+    h = 4.0                                            # 1.5 m above bankfull
+    bf = 2.5
+    A = (10 + bf * 2) * bf
+    WP = 10 + 2 * bf * math.sqrt(5)
+    AC, WPC = 60 * (h - bf), 60 + 2 * (h - bf)
+    R = (A + AC) / (WP + WPC)
+    nw = (WP * n + WPC * 0.075) / (WP + WPC)
+    want = (1 / nw) * (A + AC) * R ** (2 / 3) * math.sqrt(s0)
+    assert abs(orc.manning_flow(p, h) - want) <= 1e-13 * want
+
+
+def test_single_segment_network_is_one_solve():
+    d = tiny_cases()["single"]
+    qo, ro, _ = orc.wavefront_f32in(d, d["params"], d["qlat"][:1], d["qdp0"], d["dp0"], d["reach_last0"], 1)
+    p = d["params"][0].astype(np.float64)
+    q, _, _, _ = orc.mc_solve(p, 0.0, 0.0, float(d["qdp0"][0]), float(d["qlat"][0, 0]), float(d["dp0"][0]))
+    assert qo[0] == q and ro[0] == q
+
+
+def test_y_network_hand_trace():
+    """Two headwater reaches (2 and 3 segments) join reach 2 (4 segments)."""
+    d = tiny_cases()["y"]
+    P = d["params"].astype(np.float64)
+    ql = d["qlat"][0].astype(np.float64)
+    q0 = d["qdp0"].astype(np.float64)
+    d0 = d["dp0"].astype(np.float64)
+    rl0 = d["reach_last0"].astype(np.float64)
+    out = np.zeros(9)
+
+    def reach(r, qup_r, quc_r):
+        s, n = int(d["reach_seg_start"][r]), int(d["reach_seg_len"][r])
+        quc_next = 0.0
+        for i in range(n):
+            idx = s + i
+            qup = qup_r if i == 0 else q0[idx - 1]
+            quc = quc_r if i == 0 else quc_next
+            out[idx] = orc.mc_solve(P[idx], qup, quc, q0[idx], ql[idx], d0[idx])[0]
+            quc_next = out[idx]
+        return out[s + n - 1]
+
+    a = reach(0, 0.0, 0.0)
+    b = reach(1, 0.0, 0.0)
+    c = reach(2, 0.0 + rl0[0] + rl0[1], 0.0 + a + b)
+    qo, ro, _ = orc.wavefront_f32in(d, d["params"], d["qlat"][:1], d["qdp0"], d["dp0"], d["reach_last0"], 1)
+    assert np.array_equal(bits(qo), bits(out))
+    assert np.array_equal(bits(ro), bits([a, b, c]))
+
+
+def test_dag_fixed_point_is_accumulated_lateral_inflow():
+    """Constant forcing: each segment's outflow converges to inflow + q_lat
+    ((1 - C3) = C1 + C2), so the outlet carries the sum of all lateral inflow."""
+    d = tiny_cases()["three_way"]
+    N, T = int(d["n_segments"]), 1500
+    ql = np.full((T, N), 0.0, np.float64)
+    ql[:] = np.linspace(0.05, 0.4, N)
+    r = orc.wavefront_f64(d, d["params"].astype(np.float64), ql, np.ones(N), np.full(N, 0.5),
+                          np.ones(int(d["n_reaches"])), T)
+    outlet = int(d["level_reach"][-1])
+    assert abs(r["reach_last"][outlet] - ql[0].sum()) <= 1e-9 * ql[0].sum()
+
+
+def test_compute_stats_definition():
+    rng = np.random.default_rng(1)
+    truth = rng.lognormal(0, 2, 2000)
+    truth[:50] = 1e-4                        # excluded from relative errors
+    pred = truth * (1 + rng.normal(0, 0.02, truth.size))
+    pred[7] = np.nan
+    s = orc.compute_stats(pred, truth)
+    ok = ~np.isnan(pred)
+    rel = np.abs(pred - truth)[ok & (np.abs(truth) > 1e-3)] / np.abs(truth[ok & (np.abs(truth) > 1e-3)])
+    rs = np.sort(rel)
+    assert s["nans"] == 1
+    assert s["max_rel"] == rs[-1]
+    assert s["p50"] == rs[int(0.5 * (rs.size - 1))]
+    assert s["within_1pct"] == np.mean(rel < 0.01)
+
+
+# ------------------------------------------------------ converged mode pins
+def _rand_params(rng):
+    p = [rng.uniform(500, 5000), rng.uniform(1, 50), 0, 0, rng.uniform(0.03, 0.08), 0,
+         rng.uniform(1, 3), 10 ** rng.uniform(-4, -2)]
+    p[2] = p[1] + rng.uniform(2, 20)
+    p[3] = p[2] + rng.uniform(5, 100)
+    p[5] = p[4] * rng.uniform(1.2, 2)
+    return p
+
+
+def test_converged_root_is_a_sign_change_of_the_residual():
+    rng = np.random.default_rng(11)
+    for _ in range(400):
+        p = _rand_params(rng)
+        Q = 10 ** rng.uniform(-1, 3.5)
+        qup, quc, qdp = Q * rng.uniform(0.8, 1.2, 3)
+        ql = rng.uniform(0, 1)
+        q, h, v = orc.solve_converged(p, qup, quc, qdp, ql, 1.0)
+        if h <= 0.01:
+            assert orc.residual(p, qup, quc, qdp, ql, 0.01)[0] <= 0.0
+            continue
+        lo, hi = h * (1 - 1e-12), h * (1 + 1e-12)
+        assert orc.residual(p, qup, quc, qdp, ql, lo)[0] >= -1e-9 * Q
+        assert orc.residual(p, qup, quc, qdp, ql, hi)[0] <= 1e-9 * Q
+        f, qmc = orc.residual(p, qup, quc, qdp, ql, h)
+        assert abs(f) <= 1e-9 * max(Q, 1.0)
+        assert abs(q - qmc) <= 1e-12 * abs(qmc) or qmc < 0
+
+
+def test_converged_root_is_the_smallest_on_a_dense_scan():
+    rng = np.random.default_rng(12)
+    for _ in range(60):
+        p = _rand_params(rng)
+        Q = 10 ** rng.uniform(0, 3)
+        q, h, v = orc.solve_converged(p, Q, Q * 1.05, Q * 0.95, 0.2, 1.0)
+        grid = np.geomspace(0.01, h, 400)[:-1]
+        f = np.array([orc.residual(p, Q, Q * 1.05, Q * 0.95, 0.2, x)[0] for x in grid])
+        assert np.all(f > 0), "a smaller root exists below the converged depth"
+
+
+def test_normal_depth_inverts_manning():
+    rng = np.random.default_rng(13)
+    for _ in range(300):
+        p = _rand_params(rng)
+        Q = 10 ** rng.uniform(-3, 4)
+        h = orc.normal_depth(p, Q)
+        assert h > 0
+        assert abs(orc.manning_flow(p, h) - Q) <= 1e-9 * Q
