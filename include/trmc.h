@@ -108,6 +108,8 @@ const char *trmc_last_error(void);
 int  trmc_version(void);                       /* major*10000 + minor*100 + patch */
 int  trmc_device_count(int *count);            /* gfx950 devices visible */
 void trmc_default_options(trmc_options *opt);
+/* Waits for all work the library queued on `device` (hipDeviceSynchronize). */
+int  trmc_device_synchronize(int device);
 
 /* ------------------------------------------------------ drop-in routines */
 

@@ -112,6 +112,7 @@ def _declare(L: C.CDLL) -> None:
         "trmc_version": (I, []),
         "trmc_device_count": (I, [pI]),
         "trmc_default_options": (None, [_p(Options)]),
+        "trmc_device_synchronize": (I, [I]),
         "trmc_wavefront_f32in_f64": (I, [_p(ChParamsF32)] + topo + [pF, pF, pF, pF, pD, pD, I, D]),
         "trmc_wavefront_f64": (I, [_p(ChParamsF64)] + topo + [pD, pD, pD, pD, pD, pD, I, D, _p(Options)]),
         "trmc_reach_parallel_f64": (I, [_p(ChParamsF64), pD, pI, pI, pD, pD, pD, pD, pD, pD, D, D, I]),
@@ -395,6 +396,10 @@ def default_options(**kw) -> Options:
             raise TrmcError(ERR_ARG, f"unknown option {k}")
         setattr(o, k, v)
     return o
+
+
+def synchronize(device: int = 0) -> None:
+    _check(lib().trmc_device_synchronize(device))
 
 
 def device_count() -> int:

@@ -206,6 +206,13 @@ void trmc_default_options(trmc_options *opt)
     opt->device = 0;
 }
 
+int trmc_device_synchronize(int device)
+{
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipDeviceSynchronize());
+    return TRMC_OK;
+}
+
 // ------------------------------------------------------------ host plans
 struct trmc_plan { Plan p; };
 

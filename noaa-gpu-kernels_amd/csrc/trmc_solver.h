@@ -2,11 +2,11 @@
 //
 // REFERENCE rule: the reference's secant (wavefront_mc.cu:215-337, identical
 // arithmetic in troute_mc_final.cu:42-139) with its operation order kept term
-// for term, so that with -ffp-contract=off the only difference from the host
-// routine is the last-ulp behaviour of pow (ocml vs the host libm).  Derived
-// per-segment constants are computed once per segment by the planner with the
-// same IEEE operations the reference performs inline (1/cs, sqrt(s0)/n,
-// sqrt(1+z*z), bankfull depth), so they are bit-identical to the inline ones.
+// for term (-ffp-contract=off), and every pow replaced by the correctly
+// rounded pow23_53 below.  Derived per-segment constants are computed once per
+// segment by the planner with the same IEEE operations the reference performs
+// inline (1/cs, sqrt(s0)/n, sqrt(1+z*z), bankfull depth), so they are
+// bit-identical to the inline ones.
 //
 // CONVERGED rule: the fixed point of the same iteration (h_0 == h, Qj_0 == 0,
 // hence X0 == 0.5) gives the single-valued residual
@@ -17,7 +17,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include "trmc_device.h"
+
+// Device functions are also host-callable so a CPU build of this header can
+// exercise the same solver logic in tests.
+#define TRMC_HD __host__ __device__ __forceinline__
 
 namespace trmc {
 
@@ -25,7 +31,7 @@ struct Seg {
     double dx, bw, twcc, n_ch, ncc, s0, z, bfd, ks, sq, sqs0;
 };
 
-__device__ __forceinline__ Seg load_seg(const double *__restrict__ sc, size_t stride, int s)
+TRMC_HD Seg load_seg(const double *__restrict__ sc, size_t stride, int s)
 {
     Seg c;
     c.dx = sc[SC_DX * stride + s];
@@ -42,11 +48,64 @@ __device__ __forceinline__ Seg load_seg(const double *__restrict__ sc, size_t st
     return c;
 }
 
+// Correctly rounded x^y for the two exponents the MC arithmetic uses,
+// y = double(2/3) and y = double(5/3).  The reference calls the host libm's
+// pow (wavefront_mc.cu:251-252, :270, :334); matching it to the last ulp is
+// what keeps the loose-tolerance secant on the same path, so both powers are
+// computed to ~2^-100 relative before one final rounding:
+//   * x = m 2^(3k) with m in [1, 8);
+//   * c = cbrt(m) refined by one Newton step carried in double-double
+//     (error ~ eps(cbrt)^2 ~ 2^-102);
+//   * m^(2/3) = c^2 and m^(5/3) = m c^2, both as double-double;
+//   * the exponents are not 2/3 and 5/3 exactly: 3 double(2/3) = 2 - 2^-53 and
+//     3 double(5/3) = 5 + 2^-52, so x^double(2/3) = x^(2/3) exp(-L) and
+//     x^double(5/3) = x^(5/3) exp(2L) with L = ln(x) 2^-53 / 3 (|L| < 2^-44),
+//     applied to second order;
+//   * one rounding of the double-double sum, then an exact 2^(2k) / 2^(5k)
+//     scale.
+// Results are exact roundings for every x whose powers are normal numbers,
+// except when the true value lies within ~2^-100 relative of a rounding
+// midpoint.  One cube root and one log serve both powers.
+struct Pow2353 { double p23, p53; };
+
+TRMC_HD Pow2353 pow23_53(double x)
+{
+    Pow2353 r;
+    if (!(x > 0.0) || x == HUGE_VAL) {          // 0, negative, NaN, +inf
+        double v = (x == 0.0) ? 0.0 : (x == HUGE_VAL ? HUGE_VAL : NAN);
+        r.p23 = v;
+        r.p53 = v;
+        return r;
+    }
+    int e = 0;
+    const double f = frexp(x, &e);                // x = f 2^e, f in [0.5, 1)
+    const int e1 = e - 1;
+    const int k = (e1 >= 0) ? e1 / 3 : -((2 - e1) / 3);   // floor(e1 / 3)
+    const double m = ldexp(2.0 * f, e1 - 3 * k);  // [1, 8)
+    const double c0 = cbrt(m);
+    const double s_hi = c0 * c0, s_lo = fma(c0, c0, -s_hi);
+    const double t_hi = s_hi * c0, t_lo = fma(s_hi, c0, -t_hi) + s_lo * c0;
+    const double rho = (t_hi - m) + t_lo;         // c0^3 - m
+    const double corr = rho / (3.0 * s_hi);
+    const double c_hi = c0 - corr;
+    const double c_lo = (c0 - c_hi) - corr;
+    const double p_hi = c_hi * c_hi;
+    const double p_lo = fma(c_hi, c_hi, -p_hi) + 2.0 * c_hi * c_lo;
+    const double q_hi = m * p_hi;
+    const double q_lo = fma(m, p_hi, -q_hi) + m * p_lo;
+    const double L = ldexp(log(x) / 3.0, -53);
+    const double e23 = -L + 0.5 * L * L;          // exp(-L) - 1
+    const double e53 = 2.0 * L + 2.0 * L * L;     // exp(2L) - 1
+    r.p23 = ldexp(p_hi + (p_lo + p_hi * e23), 2 * k);
+    r.p53 = ldexp(q_hi + (q_lo + q_hi * e53), 5 * k);
+    return r;
+}
+
 struct Geo { double twl, R, A, AC, WP, WPC; };
 struct Coef { double c1, c2, c3, c4; };
 
 // compound trapezoid geometry, wavefront_mc.cu:200-213
-__device__ __forceinline__ Geo geometry(double h, const Seg &c)
+TRMC_HD Geo geometry(double h, const Seg &c)
 {
     Geo g;
     g.twl = c.bw + 2.0 * c.z * h;
@@ -62,31 +121,31 @@ __device__ __forceinline__ Geo geometry(double h, const Seg &c)
 }
 
 // kinematic celerity from R^(2/3), R^(5/3), wavefront_mc.cu:249-254
-__device__ __forceinline__ double celerity(double h, double r23, double r53, const Seg &c)
+TRMC_HD double celerity(double h, double r23, double r53, const Seg &c)
 {
     if (!(h > 0.0)) return 0.0;
     return fmax(0.0, c.ks * ((5.0 / 3.0) * r23 - (2.0 / 3.0) * r53 *
                              (2.0 * c.sq / (c.bw + 2.0 * h * c.z))));
 }
 
-__device__ __forceinline__ double storage_km(double ck, double dt, const Seg &c)
+TRMC_HD double storage_km(double ck, double dt, const Seg &c)
 {
     return (ck > 0.0) ? fmax(dt, c.dx / ck) : dt;
 }
 
-__device__ __forceinline__ double weight_width(double h, const Geo &g, const Seg &c)
+TRMC_HD double weight_width(double h, const Geo &g, const Seg &c)
 {
     return (h > c.bfd && c.twcc > 0.0) ? c.twcc : g.twl;
 }
 
-__device__ __forceinline__ double weighting(double ck, double twu, double q, double lo, const Seg &c)
+TRMC_HD double weighting(double ck, double twu, double q, double lo, const Seg &c)
 {
     if (ck > 0.0 && twu * c.s0 * ck * c.dx > 0.0)
         return fmin(0.5, fmax(lo, 0.5 * (1.0 - q / (2.0 * twu * c.s0 * ck * c.dx))));
     return 0.5;
 }
 
-__device__ __forceinline__ Coef muskingum(double km, double x, double dt, double ql)
+TRMC_HD Coef muskingum(double km, double x, double dt, double ql)
 {
     Coef m;
     double D = km * (1.0 - x) + dt / 2.0;
@@ -99,14 +158,14 @@ __device__ __forceinline__ Coef muskingum(double km, double x, double dt, double
 }
 
 // Manning flow with the R^(2/3) already at hand, wavefront_mc.cu:267-270
-__device__ __forceinline__ double manning(const Geo &g, double r23, const Seg &c)
+TRMC_HD double manning(const Geo &g, double r23, const Seg &c)
 {
     if (!(g.WP + g.WPC > 0.0)) return 0.0;
     return (1.0 / (((g.WP * c.n_ch) + (g.WPC * c.ncc)) / (g.WP + g.WPC))) * (g.A + g.AC) * r23 * c.sqs0;
 }
 
 // wavefront_mc.cu:321-328
-__device__ __forceinline__ double outflow(const Coef &m, double qup, double quc, double qdp)
+TRMC_HD double outflow(const Coef &m, double qup, double quc, double qdp)
 {
     double routed = m.c1 * qup + m.c2 * quc + m.c3 * qdp;
     double qmc = routed + m.c4;
@@ -118,16 +177,16 @@ __device__ __forceinline__ double outflow(const Coef &m, double qup, double quc,
 }
 
 // wavefront_mc.cu:330-336
-__device__ __forceinline__ double velocity(double h, const Seg &c)
+TRMC_HD double velocity(double h, const Seg &c)
 {
     double twl = c.bw + 2.0 * c.z * h;
     double Rv = (h * (c.bw + twl) / 2.0) /
                 (c.bw + 2.0 * sqrt(((twl - c.bw) / 2.0) * ((twl - c.bw) / 2.0) + h * h));
     if (Rv < 0.0) Rv = 0.0;
-    return (1.0 / c.n_ch) * pow(Rv, 2.0 / 3.0) * c.sqs0;
+    return (1.0 / c.n_ch) * pow23_53(Rv).p23 * c.sqs0;
 }
 
-__device__ __forceinline__ bool seg_invalid(const Seg &c)
+TRMC_HD bool seg_invalid(const Seg &c)
 {
     return c.n_ch <= 0.0 || c.s0 <= 0.0 || c.z <= 0.0 || c.bw <= 0.0;
 }
@@ -139,7 +198,7 @@ struct SolveOut {
 };
 
 // REFERENCE stopping rule, wavefront_mc.cu:215-337.
-__device__ __forceinline__ SolveOut solve_reference(double dt, double qup, double quc, double qdp,
+TRMC_HD SolveOut solve_reference(double dt, double qup, double quc, double qdp,
                                                     double ql, const Seg &c, double depthp)
 {
     SolveOut o;
@@ -161,8 +220,8 @@ __device__ __forceinline__ SolveOut solve_reference(double dt, double qup, doubl
         int iter = 0;
         while (rerror > 0.01 && aerror >= mindepth && iter <= maxiter) {
             Geo g0 = geometry(h_0, c);
-            double r23_0 = pow(g0.R, 2.0 / 3.0);
-            double r53_0 = pow(g0.R, 5.0 / 3.0);
+            const Pow2353 w0 = pow23_53(g0.R);
+            const double r23_0 = w0.p23, r53_0 = w0.p53;
             double ck0 = celerity(h_0, r23_0, r53_0, c);
             double km0 = storage_km(ck0, dt, c);
             double x0 = weighting(ck0, weight_width(h_0, g0, c), Qj_0, 0.0, c);
@@ -171,8 +230,8 @@ __device__ __forceinline__ SolveOut solve_reference(double dt, double qup, doubl
             Qj_0 = q0 - manning(g0, r23_0, c);
 
             Geo g1 = geometry(h, c);
-            double r23_1 = pow(g1.R, 2.0 / 3.0);
-            double r53_1 = pow(g1.R, 5.0 / 3.0);
+            const Pow2353 w1 = pow23_53(g1.R);
+            const double r23_1 = w1.p23, r53_1 = w1.p53;
             double ck1 = celerity(h, r23_1, r53_1, c);
             double km1 = storage_km(ck1, dt, c);
             double X = weighting(ck1, weight_width(h, g1, c), q0, 0.25, c);
@@ -206,12 +265,12 @@ __device__ __forceinline__ SolveOut solve_reference(double dt, double qup, doubl
 // ------------------------------------------------------------- CONVERGED
 // F(h) with the X0 = 0.5 coefficients feeding the weighting X(h); also returns
 // the final coefficients at h for the outflow rule.
-__device__ __forceinline__ double residual(double h, double dt, double qup, double quc, double qdp,
+TRMC_HD double residual(double h, double dt, double qup, double quc, double qdp,
                                            double ql, const Seg &c, Coef &C)
 {
     Geo g = geometry(h, c);
-    double r23 = pow(g.R, 2.0 / 3.0);
-    double r53 = pow(g.R, 5.0 / 3.0);
+    const Pow2353 w = pow23_53(g.R);
+    const double r23 = w.p23, r53 = w.p53;
     double ck = celerity(h, r23, r53, c);
     double km = storage_km(ck, dt, c);
     Coef m0 = muskingum(km, 0.5, dt, ql);
@@ -231,7 +290,7 @@ struct RootState {
 // Bracket-safeguarded secant for the smallest root of F on [mindepth, hmax]
 // starting from (ha, hb).  Returns the last evaluated depth (its coefficients
 // in C) and counts evaluations in *evals.
-__device__ __forceinline__ double secant_root(double ha, double hb, double hmax, double dt,
+TRMC_HD double secant_root(double ha, double hb, double hmax, double dt,
                                               double qup, double quc, double qdp, double ql,
                                               const Seg &c, double tol, int max_eval,
                                               Coef &C, int *evals, bool *capped)
@@ -276,7 +335,7 @@ __device__ __forceinline__ double secant_root(double ha, double hb, double hmax,
     return h;
 }
 
-__device__ __forceinline__ SolveOut solve_converged(double dt, double qup, double quc, double qdp,
+TRMC_HD SolveOut solve_converged(double dt, double qup, double quc, double qdp,
                                                     double ql, const Seg &c, double ha, double hb,
                                                     double tol, int max_eval)
 {

@@ -14,6 +14,35 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* The pow of the MC arithmetic.  Default: the host libm's pow, as the
+ * reference calls it (bit-for-bit the reference's results).  With
+ * ORC_CR_POW: the correctly rounded pow, computed in quad precision and
+ * rounded once (libquadmath) -- an implementation independent of the GPU
+ * library's, used to check that the GPU is the same algorithm to the bit. */
+#ifdef ORC_CR_POW
+#include <quadmath.h>
+/* Ziv's two-step rounding: the x87 extended pow (64-bit significand) decides
+ * the rounding unless its value lies within 2^-60 relative of a double
+ * rounding boundary; then the quad-precision pow (113-bit) decides. */
+static double orc_pow(double x, double y)
+{
+    long double r = powl((long double)x, (long double)y);
+    double d = (double)r;
+    if (isfinite(d) && d > 0.0) {
+        long double up = (long double)nextafter(d, INFINITY) - (long double)d;
+        long double dn = (long double)d - (long double)nextafter(d, 0.0);
+        long double dist = r >= (long double)d ? up / 2 - (r - (long double)d)
+                                               : dn / 2 - ((long double)d - r);
+        if (dist > fabsl(r) * 0x1p-60L) return d;
+    }
+    return (double)powq((__float128)x, (__float128)y);
+}
+#else
+static double orc_pow(double x, double y) { return pow(x, y); }
+#endif
+
+double orc_pow_fn(double x, double y) { return orc_pow(x, y); }
+
 /* Per-segment constants derived once, wavefront_mc.cu:221-225. */
 typedef struct {
     double dx, bw, tw, twcc, n_ch, ncc, s0;
@@ -61,8 +90,8 @@ static void geometry(double h, const chan_t *c, geo_t *g)
 static double celerity(double h, const geo_t *g, const chan_t *c)
 {
     if (!(h > 0.0)) return 0.0;
-    double r23 = pow(g->R, 2.0 / 3.0);
-    double r53 = pow(g->R, 5.0 / 3.0);
+    double r23 = orc_pow(g->R, 2.0 / 3.0);
+    double r53 = orc_pow(g->R, 5.0 / 3.0);
     return fmax(0.0, (sqrt(c->s0) / c->n_ch) *
                 ((5.0 / 3.0) * r23 - (2.0 / 3.0) * r53 *
                  (2.0 * sqrt(1.0 + c->z * c->z) / (c->bw + 2.0 * h * c->z))));
@@ -105,7 +134,7 @@ static double manning(const geo_t *g, const chan_t *c)
 {
     if (!(g->WP + g->WPC > 0.0)) return 0.0;
     return (1.0 / (((g->WP * c->n_ch) + (g->WPC * c->ncc)) / (g->WP + g->WPC)))
-           * (g->A + g->AC) * pow(g->R, 2.0 / 3.0) * sqrt(c->s0);
+           * (g->A + g->AC) * orc_pow(g->R, 2.0 / 3.0) * sqrt(c->s0);
 }
 
 /* Outflow from the final coefficients, wavefront_mc.cu:321-328. */
@@ -127,7 +156,7 @@ static double velocity(double h, const chan_t *c)
     double Rv = (h * (c->bw + twl) / 2.0) /
                 (c->bw + 2.0 * sqrt(((twl - c->bw) / 2.0) * ((twl - c->bw) / 2.0) + h * h));
     if (Rv < 0.0) Rv = 0.0;
-    return (1.0 / c->n_ch) * pow(Rv, 2.0 / 3.0) * sqrt(c->s0);
+    return (1.0 / c->n_ch) * orc_pow(Rv, 2.0 / 3.0) * sqrt(c->s0);
 }
 
 double orc_mc_solve(double dt, double qup, double quc, double qdp, double ql,

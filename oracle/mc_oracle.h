@@ -26,6 +26,11 @@
  *   orc_normal_depth        depth h with Manning flow Qmn(h) == Q (training
  *                           targets for the learned initial guess)
  *
+ * Two builds of the same source: liborc.so calls the host libm's pow, as the
+ * reference does (bit-for-bit the reference); liborc_cr.so (-DORC_CR_POW)
+ * uses a correctly rounded pow from quad precision, the arithmetic the GPU
+ * library implements with its own correctly rounded pow.
+ *
  * Parity pinning (how each function is checked, see tests/test_oracle.py):
  *   - orc_mc_solve, orc_wavefront_f32in, orc_reach_parallel: bit-for-bit against
  *     the reference's own host routines compiled from the reference sources into
@@ -103,6 +108,10 @@ double orc_mc_solve_converged(double dt, double qup, double quc, double qdp, dou
 /* Manning normal depth: h >= 0 with Qmn(h) == Q (Q > 0). */
 double orc_normal_depth(double Q, const orc_chparams_f64 *p);
 double orc_manning_flow(double h, const orc_chparams_f64 *p);
+
+/* The pow the build uses: libm pow (liborc.so) or correctly rounded pow
+ * (liborc_cr.so, built with -DORC_CR_POW). */
+double orc_pow_fn(double x, double y);
 
 /* Error statistics of wavefront_mc.cu:822-858: relative error over entries
  * with |truth| > 1e-3.  out[0..8] = max_abs, p50, p90, p99, p99.9, max_rel,

@@ -30,17 +30,19 @@ class P32(C.Structure):
 
 
 MODE_REFERENCE, MODE_CONVERGED = 0, 1
-_orc = None
+# "libm": pow of the host libm, as the reference (liborc.so);
+# "cr":   correctly rounded pow (liborc_cr.so), the GPU library's arithmetic.
+VARIANTS = {"libm": "liborc.so", "cr": "liborc_cr.so"}
+_orc = {}
 
 
 def _ptr(a, t):
     return a.ctypes.data_as(C.POINTER(t))
 
 
-def lib() -> C.CDLL:
-    global _orc
-    if _orc is None:
-        path = os.path.join(HERE, "liborc.so")
+def lib(variant: str = "libm") -> C.CDLL:
+    if variant not in _orc:
+        path = os.path.join(HERE, VARIANTS[variant])
         if not os.path.exists(path):
             raise RuntimeError(f"{path} not built (make -C oracle)")
         L = C.CDLL(path)
@@ -64,8 +66,10 @@ def lib() -> C.CDLL:
         L.orc_manning_flow.argtypes = [D, pP64]
         L.orc_compute_stats.restype = None
         L.orc_compute_stats.argtypes = [pD, pD, I, pD]
-        _orc = L
-    return _orc
+        L.orc_pow_fn.restype = D
+        L.orc_pow_fn.argtypes = [D, D]
+        _orc[variant] = L
+    return _orc[variant]
 
 
 def _topo_args(topo):
@@ -84,15 +88,15 @@ def as_topo(t) -> dict:
     return {k: getattr(t, k) for k in keys}
 
 
-def mc_solve(params8, qup, quc, qdp, ql, depthp, dt=300.0):
+def mc_solve(params8, qup, quc, qdp, ql, depthp, dt=300.0, variant="libm"):
     """One segment: returns (qdc, depth, velocity, iterations)."""
     p = P64(*map(float, params8))
     d, v, it = D(), D(), I()
-    q = lib().orc_mc_solve(dt, qup, quc, qdp, ql, C.byref(p), depthp, C.byref(d), C.byref(v), C.byref(it))
+    q = lib(variant).orc_mc_solve(dt, qup, quc, qdp, ql, C.byref(p), depthp, C.byref(d), C.byref(v), C.byref(it))
     return q, d.value, v.value, it.value
 
 
-def mc_solve_many(params, inputs, dt=300.0, mode=MODE_REFERENCE):
+def mc_solve_many(params, inputs, dt=300.0, mode=MODE_REFERENCE, variant="libm"):
     """params [n, 8], inputs [n, 5] (qup, quc, qdp, ql, depthp) -> out [n, 3], iters [n]."""
     params = np.asarray(params, np.float64)
     inputs = np.asarray(inputs, np.float64)
@@ -103,15 +107,15 @@ def mc_solve_many(params, inputs, dt=300.0, mode=MODE_REFERENCE):
         d, v, it = D(), D(), I()
         x = inputs[i]
         if mode == MODE_REFERENCE:
-            q = lib().orc_mc_solve(dt, x[0], x[1], x[2], x[3], C.byref(p), x[4], C.byref(d), C.byref(v), C.byref(it))
+            q = lib(variant).orc_mc_solve(dt, x[0], x[1], x[2], x[3], C.byref(p), x[4], C.byref(d), C.byref(v), C.byref(it))
         else:
-            q = lib().orc_mc_solve_converged(dt, x[0], x[1], x[2], x[3], C.byref(p), x[4], C.byref(d), C.byref(v))
+            q = lib(variant).orc_mc_solve_converged(dt, x[0], x[1], x[2], x[3], C.byref(p), x[4], C.byref(d), C.byref(v))
         out[i] = (q, d.value, v.value)
         its[i] = it.value
     return out, its
 
 
-def wavefront_f32in(topo, params32, qlat32, qdp0, dp0, reach_last0, n_timesteps, dt=300.0):
+def wavefront_f32in(topo, params32, qlat32, qdp0, dp0, reach_last0, n_timesteps, dt=300.0, variant="libm"):
     """cpu_wavefront_fp64 restated: returns (qdc_out [N], reach_last_out [R], iterations)."""
     t = as_topo(topo)
     keep, targs = _topo_args(t)
@@ -121,14 +125,14 @@ def wavefront_f32in(topo, params32, qlat32, qdp0, dp0, reach_last0, n_timesteps,
     qo = np.zeros(int(t["n_segments"]))
     ro = np.zeros(int(t["n_reaches"]))
     its = LL(0)
-    lib().orc_wavefront_f32in(_ptr(p, P32), *targs, _ptr(q, C.c_float), _ptr(q0, C.c_float),
+    lib(variant).orc_wavefront_f32in(_ptr(p, P32), *targs, _ptr(q, C.c_float), _ptr(q0, C.c_float),
                               _ptr(d0, C.c_float), _ptr(r0, C.c_float), _ptr(qo, D), _ptr(ro, D),
                               int(n_timesteps), float(dt), C.byref(its))
     return qo, ro, its.value
 
 
 def wavefront_f64(topo, params, qlat, qdp0, dp0, reach_last0, n_timesteps, dt=300.0,
-                  mode=MODE_REFERENCE, history=False):
+                  mode=MODE_REFERENCE, history=False, variant="libm"):
     """All-double DAG routing.  Returns dict with qdc, depth, vel, reach_last,
     iterations and (history=True) reach_hist [T, R]."""
     t = as_topo(topo)
@@ -140,7 +144,7 @@ def wavefront_f64(topo, params, qlat, qdp0, dp0, reach_last0, n_timesteps, dt=30
     qo, do, vo, ro = np.zeros(N), np.zeros(N), np.zeros(N), np.zeros(R)
     hist = np.zeros((int(n_timesteps), R)) if history else None
     its = LL(0)
-    lib().orc_wavefront_f64(_ptr(p, P64), *targs, _ptr(q, D), _ptr(q0, D), _ptr(d0, D), _ptr(r0, D),
+    lib(variant).orc_wavefront_f64(_ptr(p, P64), *targs, _ptr(q, D), _ptr(q0, D), _ptr(d0, D), _ptr(r0, D),
                             _ptr(qo, D), _ptr(ro, D), _ptr(do, D), _ptr(vo, D),
                             _ptr(hist, D) if history else pD(), int(n_timesteps), float(dt), int(mode),
                             C.byref(its))
@@ -150,39 +154,43 @@ def wavefront_f64(topo, params, qlat, qdp0, dp0, reach_last0, n_timesteps, dt=30
     return out
 
 
-def reach_parallel(params, ql, rs, rl, qu, qd, dp, uq=10.0, dt=300.0):
+def reach_parallel(params, ql, rs, rl, qu, qd, dp, uq=10.0, dt=300.0, variant="libm"):
     p = np.ascontiguousarray(params, np.float64)
     ql, qu, qd, dp = (np.ascontiguousarray(x, np.float64) for x in (ql, qu, qd, dp))
     rs, rl = np.ascontiguousarray(rs, np.int32), np.ascontiguousarray(rl, np.int32)
     n = len(p)
     qo, vo, dpo = np.zeros(n), np.zeros(n), np.zeros(n)
-    lib().orc_reach_parallel(_ptr(p, P64), _ptr(ql, D), _ptr(rs, I), _ptr(rl, I), _ptr(qu, D), _ptr(qd, D),
+    lib(variant).orc_reach_parallel(_ptr(p, P64), _ptr(ql, D), _ptr(rs, I), _ptr(rl, I), _ptr(qu, D), _ptr(qd, D),
                              _ptr(dp, D), _ptr(qo, D), _ptr(vo, D), _ptr(dpo, D), uq, dt, len(rs), None)
     return qo, vo, dpo
 
 
-def residual(params8, qup, quc, qdp, ql, h, dt=300.0):
+def residual(params8, qup, quc, qdp, ql, h, dt=300.0, variant="libm"):
     p = P64(*map(float, params8))
     qmc = D()
-    f = lib().orc_mc_residual(dt, qup, quc, qdp, ql, C.byref(p), h, C.byref(qmc))
+    f = lib(variant).orc_mc_residual(dt, qup, quc, qdp, ql, C.byref(p), h, C.byref(qmc))
     return f, qmc.value
 
 
-def solve_converged(params8, qup, quc, qdp, ql, depthp, dt=300.0):
+def solve_converged(params8, qup, quc, qdp, ql, depthp, dt=300.0, variant="libm"):
     p = P64(*map(float, params8))
     d, v = D(), D()
-    q = lib().orc_mc_solve_converged(dt, qup, quc, qdp, ql, C.byref(p), depthp, C.byref(d), C.byref(v))
+    q = lib(variant).orc_mc_solve_converged(dt, qup, quc, qdp, ql, C.byref(p), depthp, C.byref(d), C.byref(v))
     return q, d.value, v.value
 
 
-def normal_depth(params8, Q):
+def normal_depth(params8, Q, variant="libm"):
     p = P64(*map(float, params8))
-    return lib().orc_normal_depth(Q, C.byref(p))
+    return lib(variant).orc_normal_depth(Q, C.byref(p))
 
 
-def manning_flow(params8, h):
+def manning_flow(params8, h, variant="libm"):
     p = P64(*map(float, params8))
-    return lib().orc_manning_flow(h, C.byref(p))
+    return lib(variant).orc_manning_flow(h, C.byref(p))
+
+
+def pow_fn(x, y, variant="libm"):
+    return lib(variant).orc_pow_fn(float(x), float(y))
 
 
 def compute_stats(pred, truth):

@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 from conftest import golden, load_oracle, tiny_cases
+from crpow import cr_pow, sample_inputs
 
 orc = load_oracle()
 
@@ -257,3 +258,31 @@ def test_normal_depth_inverts_manning():
         h = orc.normal_depth(p, Q)
         assert h > 0
         assert abs(orc.manning_flow(p, h) - Q) <= 1e-9 * Q
+
+
+# -------------------------------------------------- the two pow variants
+def test_libm_variant_calls_the_host_pow():
+    for x in sample_inputs(200, 1):
+        for y in (2.0 / 3.0, 5.0 / 3.0):
+            try:
+                want = math.pow(x, y)            # CPython calls the C library pow
+            except OverflowError:
+                want = math.inf
+            assert orc.pow_fn(x, y) == want
+
+
+def test_cr_variant_pow_is_correctly_rounded():
+    xs = sample_inputs(4000, 2)
+    for y in (2.0 / 3.0, 5.0 / 3.0):
+        bad = [x for x in xs if orc.pow_fn(x, y, "cr") != cr_pow(x, y)]
+        assert not bad, bad[:5]
+
+
+def test_cr_variant_differs_from_reference_only_through_pow():
+    """Same inputs, only the pow changed: all but path-sensitive cases agree
+    closely; the median relative difference is at rounding level."""
+    z = golden("mc_solve_cases.npz")
+    a, _ = orc.mc_solve_many(z["params"], z["inputs"], float(z["dt"]))
+    b, _ = orc.mc_solve_many(z["params"], z["inputs"], float(z["dt"]), variant="cr")
+    rel = np.abs(a - b) / np.maximum(np.abs(a), 1e-12)
+    assert np.median(rel) < 1e-15 and np.mean(rel < 1e-12) > 0.99

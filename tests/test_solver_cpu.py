@@ -1,0 +1,89 @@
+"""The library's segment solver (trmc_solver.h), compiled for the host by
+tests/native, against the oracle on the CPU.  REFERENCE rule: bit-for-bit
+(same libm on both sides).  CONVERGED rule: the safeguarded secant lands on
+the oracle's smallest root (bisection) to 1e-9 in flow."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, golden, load_oracle
+from crpow import cr_pow, sample_inputs
+
+orc = load_oracle()
+SHIM = os.path.join(ROOT, "tests", "native", "libsolver_shim.so")
+
+
+@pytest.fixture(scope="module")
+def shim():
+    if not os.path.exists(SHIM):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "native")], check=True)
+    L = C.CDLL(SHIM)
+    pD = C.POINTER(C.c_double)
+    L.shim_solve.restype = C.c_int
+    L.shim_solve.argtypes = [C.c_int, pD, pD, C.c_double, C.c_double, C.c_int, pD]
+    L.shim_pow23_53.restype = None
+    L.shim_pow23_53.argtypes = [pD, pD, C.c_int]
+    return L
+
+
+def segc_row(p):
+    """The 11 per-segment constants, computed as the library's planner does."""
+    dx, bw, tw, twcc, n, ncc, cs, s0 = map(float, p)
+    z = 1.0 if cs == 0.0 else 1.0 / cs
+    bfd = bw / 0.00001 if bw > tw else (bw / (2.0 * z) if bw == tw else (tw - bw) / (2.0 * z))
+    return np.array([dx, bw, twcc, n, ncc, s0, z, bfd, np.sqrt(s0) / n, np.sqrt(1.0 + z * z), np.sqrt(s0)])
+
+
+def run(shim, mode, p, x, tol=1e-13, max_eval=60):
+    seg = segc_row(p)
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros(3)
+    r = shim.shim_solve(mode, seg.ctypes.data_as(C.POINTER(C.c_double)), x.ctypes.data_as(C.POINTER(C.c_double)),
+                        300.0, tol, max_eval, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out, r & 0xFFFFFF, r >> 24
+
+
+def test_pow23_53_is_correctly_rounded(shim):
+    xs = np.ascontiguousarray(sample_inputs(6000, 3))
+    out = np.zeros(2 * xs.size)
+    shim.shim_pow23_53(xs.ctypes.data_as(C.POINTER(C.c_double)), out.ctypes.data_as(C.POINTER(C.c_double)), xs.size)
+    out = out.reshape(-1, 2)
+    for i, x in enumerate(xs):
+        assert out[i, 0] == cr_pow(x, 2.0 / 3.0), x
+        assert out[i, 1] == cr_pow(x, 5.0 / 3.0), x
+    specials = np.array([0.0, -1.0, np.inf, np.nan])
+    o = np.zeros(8)
+    shim.shim_pow23_53(specials.ctypes.data_as(C.POINTER(C.c_double)), o.ctypes.data_as(C.POINTER(C.c_double)), 4)
+    assert o[0] == 0.0 and o[1] == 0.0 and np.isnan(o[2]) and o[4] == np.inf and np.isnan(o[6])
+
+
+def test_reference_rule_bitwise_against_cr_oracle(shim):
+    z = golden("mc_solve_cases.npz")
+    want, _ = orc.mc_solve_many(z["params"], z["inputs"], float(z["dt"]), variant="cr")
+    for p, x, e in zip(z["params"], z["inputs"], want):
+        out, _, _ = run(shim, 0, p, x)
+        assert out.view(np.uint64).tolist() == e.view(np.uint64).tolist(), (p, x, out, e)
+
+
+def test_converged_rule_finds_the_oracle_root(shim):
+    rng = np.random.default_rng(31)
+    evals = []
+    for i in range(3000):
+        p = [rng.uniform(500, 5000), rng.uniform(1, 50), 0, 0, rng.uniform(0.03, 0.08), 0,
+             rng.uniform(1, 3), 10 ** rng.uniform(-4, -2)]
+        p[2] = p[1] + rng.uniform(2, 20)
+        p[3] = p[2] + rng.uniform(5, 100)
+        p[5] = p[4] * rng.uniform(1.2, 2)
+        Q = 10 ** rng.uniform(-2, 4)
+        x = [Q * rng.uniform(0.7, 1.3), Q * rng.uniform(0.7, 1.3), Q * rng.uniform(0.7, 1.3),
+             rng.uniform(0, 1), 10 ** rng.uniform(-2, 1.3)]
+        out, ev, flags = run(shim, 1, p, x)
+        q, h, v = orc.solve_converged(p, *x, variant="cr")
+        assert flags == 0, (i, p, x)
+        assert abs(out[0] - q) <= 1e-9 * max(abs(q), 1e-3), (i, out, (q, h, v))
+        assert abs(out[1] - h) <= 1e-9 * max(h, 1e-2), (i, out, (q, h, v))
+        evals.append(ev)
+    assert np.mean(evals) < 15
