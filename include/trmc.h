@@ -68,10 +68,16 @@ typedef struct { double dx, bw, tw, twcc, n_ch, ncc, cs, s0; } trmc_chparams_f64
  *           device-scope barrier between super-levels (L + T - 1 barriers).
  *  LEVEL:   one launch per (timestep, level), as the reference's
  *           kernel_level_wavefront loop; the per-step launch sequence is
- *           captured once in a hipGraph and replayed T times. */
-#define TRMC_SCHED_AUTO   0
-#define TRMC_SCHED_SKEWED 1
-#define TRMC_SCHED_LEVEL  2
+ *           captured once in a hipGraph and replayed T times.
+ *  DATAFLOW: one persistent kernel without grid barriers: each (unit, step)
+ *           starts as soon as its upstream outflows for steps t and t-1 are
+ *           in device memory (a full outflow history, (T+1) x units x
+ *           members doubles).  Confluences of at most 4 upstream reaches.
+ * AUTO picks DATAFLOW when the network allows it, else SKEWED. */
+#define TRMC_SCHED_AUTO     0
+#define TRMC_SCHED_SKEWED   1
+#define TRMC_SCHED_LEVEL    2
+#define TRMC_SCHED_DATAFLOW 3
 
 typedef struct {
     int    mode;            /* TRMC_MODE_* (default REFERENCE) */
@@ -86,6 +92,7 @@ typedef struct {
     int    grid_blocks;     /* persistent grid size; 0 = fill the device */
     int    dynamic_queue;   /* SKEWED: 1 = atomic work queue per super-level */
     int    device;          /* HIP device ordinal */
+    int    trace;           /* SKEWED: record per-super-level time and max iterations */
 } trmc_options;
 
 typedef struct {
@@ -221,6 +228,12 @@ int trmc_network_get_state_f64(const trmc_network *net, double *qdc_seg,
 /* With record_history: last-segment outflow of every reach after every step
  * of the last run, [n_members][n_timesteps][n_reaches]. */
 int trmc_network_get_history_f64(const trmc_network *net, double *reach_hist);
+
+/* With trace: for each super-level of the last SKEWED run, the wall time from
+ * the previous barrier's release to this one's (microseconds, block 0's view)
+ * and the largest secant iteration count of any item in it.  Returns the
+ * number of super-levels written (at most n). */
+int trmc_network_get_trace(const trmc_network *net, double *us, int *max_iters, int n);
 
 /* Shape facts of the prepared schedule, as trmc_plan_info with the
  * network's member count. */

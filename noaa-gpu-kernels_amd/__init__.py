@@ -34,7 +34,7 @@ OK = 0
 ERR_ARG, ERR_TOPOLOGY, ERR_HIP, ERR_NO_DEVICE, ERR_TIMEOUT = -1, -2, -3, -4, -5
 ERR_NOMEM, ERR_IO, ERR_STATE, ERR_NOT_CONVERGED = -6, -7, -8, -9
 MODE_REFERENCE, MODE_CONVERGED, MODE_LEARNED = 0, 1, 2
-SCHED_AUTO, SCHED_SKEWED, SCHED_LEVEL = 0, 1, 2
+SCHED_AUTO, SCHED_SKEWED, SCHED_LEVEL, SCHED_DATAFLOW = 0, 1, 2, 3
 
 PARAM_FIELDS = ("dx", "bw", "tw", "twcc", "n_ch", "ncc", "cs", "s0")
 
@@ -61,6 +61,7 @@ class Options(C.Structure):
         ("block_size", C.c_int), ("n_members", C.c_int), ("record_history", C.c_int),
         ("tol", C.c_double), ("max_iter", C.c_int), ("learned_budget", C.c_int),
         ("grid_blocks", C.c_int), ("dynamic_queue", C.c_int), ("device", C.c_int),
+        ("trace", C.c_int),
     ]
 
 
@@ -133,6 +134,7 @@ def _declare(L: C.CDLL) -> None:
         "trmc_network_get_state_f64": (I, [V, pD, pD, pD, pD]),
         "trmc_network_get_history_f64": (I, [V, pD]),
         "trmc_network_info": (I, [V, I, _p(C.c_int64), I]),
+        "trmc_network_get_trace": (I, [V, pD, pI, I]),
         "trmc_load_topo": (I, [C.c_char_p, _p(Topo)]),
         "trmc_save_topo": (I, [C.c_char_p, _p(Topo)]),
         "trmc_free_topo": (None, [_p(Topo)]),
@@ -552,6 +554,13 @@ class Network:
         out = np.zeros((self.n_members, n_timesteps, self.topo.n_reaches))
         _check(lib().trmc_network_get_history_f64(self._h, _ptr(out, C.c_double)))
         return out
+
+    def trace(self, n: int):
+        """Per-super-level (microseconds, max secant iterations) of the last run."""
+        us = np.zeros(n)
+        it = np.zeros(n, np.int32)
+        m = _check(lib().trmc_network_get_trace(self._h, _ptr(us, C.c_double), _ptr(it, C.c_int), n))
+        return us[:m], it[:m]
 
     def info(self, n_timesteps: int) -> dict:
         info = np.zeros(8, np.int64)

@@ -143,6 +143,10 @@ struct trmc_network {
     double *d_segc = nullptr, *d_qlat = nullptr, *d_q = nullptr, *d_d = nullptr, *d_v = nullptr,
            *d_ring = nullptr, *d_hist = nullptr;
     double *d_q0 = nullptr, *d_d0 = nullptr, *d_ring0 = nullptr;   // initial state for reset
+    unsigned long long *d_trace_t = nullptr, *d_flow_hist = nullptr;
+    size_t flow_hist_n = 0;
+    unsigned *d_trace_it = nullptr;
+    int trace_n = 0;
     float *d_mlp = nullptr;
     unsigned *d_bar = nullptr;
     int *d_status = nullptr, *d_t = nullptr;
@@ -159,6 +163,7 @@ struct trmc_network {
         dev_free(d_segc); dev_free(d_qlat); dev_free(d_q); dev_free(d_d); dev_free(d_v);
         dev_free(d_ring); dev_free(d_hist); dev_free(d_mlp); dev_free(d_bar); dev_free(d_status);
         dev_free(d_t); dev_free(d_counters); dev_free(d_q0); dev_free(d_d0); dev_free(d_ring0);
+        dev_free(d_trace_t); dev_free(d_trace_it); dev_free(d_flow_hist);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -204,6 +209,7 @@ void trmc_default_options(trmc_options *opt)
     opt->grid_blocks = 0;
     opt->dynamic_queue = 0;
     opt->device = 0;
+    opt->trace = 0;
 }
 
 int trmc_device_synchronize(int device)
@@ -543,8 +549,49 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
     HIPCHK(launch_reset((unsigned *)net->d_status, 1, s));
     RouteArgs a = make_args(net, T, dt);
     const int mode = o.mode == TRMC_MODE_REFERENCE ? 0 : 1;
-    const int sched = o.schedule == TRMC_SCHED_AUTO ? TRMC_SCHED_SKEWED : o.schedule;
-    if (sched == TRMC_SCHED_SKEWED) {
+    int sched = o.schedule;
+    if (sched == TRMC_SCHED_AUTO)
+        sched = P.max_upstream <= kDataflowMaxUp ? TRMC_SCHED_DATAFLOW : TRMC_SCHED_SKEWED;
+    if (sched == TRMC_SCHED_DATAFLOW && P.max_upstream > kDataflowMaxUp)
+        return fail(TRMC_ERR_ARG, "DATAFLOW schedule supports confluences of at most %d reaches (network has %d)",
+                    kDataflowMaxUp, P.max_upstream);
+    if (sched == TRMC_SCHED_DATAFLOW) {
+        const size_t um = (size_t)P.n_units * net->M;
+        const size_t need = um * ((size_t)T + 1);
+        if (need > net->flow_hist_n) {
+            if ((rc = dev_alloc(&net->d_flow_hist, need))) return rc;
+            net->flow_hist_n = need;
+        }
+        int bpcu = 0;
+        HIPCHK(route_dataflow_occupancy(mode, o.block_size, &bpcu));
+        if (bpcu <= 0) return fail(TRMC_ERR_HIP, "dataflow kernel cannot be resident");
+        long long resident = (long long)bpcu * net->cu_count;
+        long long need_blocks = ((long long)um + o.block_size - 1) / o.block_size;
+        long long grid = std::max(1LL, std::min(need_blocks, resident));
+        if (o.grid_blocks > 0) grid = std::min<long long>(o.grid_blocks, resident);
+        // every lane owns one member of a unit sequence: nthreads % M == 0
+        while (grid > 1 && ((long long)grid * o.block_size) % net->M != 0) --grid;
+        if (((long long)grid * o.block_size) % net->M != 0)
+            return fail(TRMC_ERR_ARG, "n_members %d does not divide the dataflow grid", net->M);
+        a.flow_hist = net->d_flow_hist;
+        HIPCHK(launch_hist_init(net->d_flow_hist, net->d_ring, net->ring_shift % P.ring, (long long)um, T, s));
+        HIPCHK(hipEventRecord(net->ev0, s));
+        HIPCHK(launch_route_dataflow(a, mode, (int)grid, o.block_size, s));
+        HIPCHK(hipEventRecord(net->ev1, s));
+        HIPCHK(launch_hist_finish(net->d_flow_hist, net->d_ring, (T + net->ring_shift) % P.ring, (long long)um, T, s));
+        if (o.record_history) {
+            int *d_rlu = nullptr;
+            if ((rc = dev_upload(&d_rlu, P.reach_last_unit.data(), P.reach_last_unit.size(), s))) return rc;
+            hipError_t e = launch_hist_extract(net->d_flow_hist, d_rlu, P.n_reaches, P.n_units, net->M, T,
+                                               net->d_hist, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            (void)hipFree(d_rlu);
+            if (e != hipSuccess) return fail(TRMC_ERR_HIP, "history extract: %s", hipGetErrorString(e));
+        }
+        st.launches = 1;
+        st.super_levels = P.n_levels + T - 1;
+        st.grid_blocks = (int)grid;
+    } else if (sched == TRMC_SCHED_SKEWED) {
         int bpcu = 0;
         HIPCHK(route_skewed_occupancy(mode, o.block_size, &bpcu));
         if (bpcu <= 0) return fail(TRMC_ERR_HIP, "routing kernel cannot be resident");
@@ -553,6 +600,15 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
         long long grid = std::max(1LL, std::min(need, resident));
         if (o.grid_blocks > 0) grid = std::min<long long>(o.grid_blocks, resident);
         HIPCHK(launch_reset(net->d_bar, kBarWords, s));
+        if (o.trace) {
+            const int ns = a.k_end - a.k_begin;
+            if ((rc = dev_alloc(&net->d_trace_t, (size_t)ns + 1))) return rc;
+            if ((rc = dev_alloc(&net->d_trace_it, (size_t)ns))) return rc;
+            HIPCHK(launch_reset(net->d_trace_it, ns, s));
+            a.trace_t = net->d_trace_t;
+            a.trace_it = net->d_trace_it;
+            net->trace_n = ns;
+        }
         HIPCHK(hipEventRecord(net->ev0, s));
         HIPCHK(launch_route_skewed(a, mode, (int)grid, o.block_size, true, s));
         HIPCHK(hipEventRecord(net->ev1, s));
@@ -679,6 +735,26 @@ int trmc_network_get_history_f64(const trmc_network *net, double *reach_hist)
     size_t n = (size_t)net->M * net->hist_T * net->plan.n_reaches;
     if (n) HIPCHK(hipMemcpy(reach_hist, net->d_hist, n * sizeof(double), hipMemcpyDeviceToHost));
     return TRMC_OK;
+}
+
+int trmc_network_get_trace(const trmc_network *net, double *us, int *max_iters, int n)
+{
+    if (!net || n < 0) return fail(TRMC_ERR_ARG, "bad argument");
+    if (!net->opt.trace || !net->d_trace_t || net->trace_n == 0) return fail(TRMC_ERR_STATE, "no trace recorded");
+    HIPCHK(hipSetDevice(net->opt.device));
+    const int ns = net->trace_n;
+    std::vector<unsigned long long> t((size_t)ns + 1);
+    std::vector<unsigned> it((size_t)ns);
+    HIPCHK(hipMemcpy(t.data(), net->d_trace_t, t.size() * sizeof(t[0]), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(it.data(), net->d_trace_it, it.size() * sizeof(it[0]), hipMemcpyDeviceToHost));
+    const int m = std::min(n, ns);
+    unsigned long long prev = t[(size_t)ns];
+    for (int k = 0; k < m; ++k) {
+        if (us) us[k] = (double)(t[k] - prev) * 1000.0 / net->wall_khz;
+        if (max_iters) max_iters[k] = (int)it[k];
+        prev = t[k];
+    }
+    return m;
 }
 
 int trmc_network_info(const trmc_network *net, int n_timesteps, int64_t *info, int n_info)

@@ -33,6 +33,15 @@ enum CounterSlot {
 // status codes the kernels can raise
 enum { KSTATUS_OK = 0, KSTATUS_TIMEOUT = 1 };
 
+// Bit pattern of a not-yet-written outflow in the dataflow history: a
+// signaling NaN payload.  Arithmetic only produces quiet NaNs and outflows
+// are always computed values, so no routed result can carry it.
+static constexpr unsigned long long kPending = 0x7FF4A5A5A5A5A5A5ull;
+
+// Upstream links per unit the dataflow kernel holds in registers; networks
+// with wider confluences use the barrier schedule.
+static constexpr int kDataflowMaxUp = 4;
+
 struct RouteArgs {
     int n_units, n_levels, n_steps, n_members, ring, ring_shift, n_reaches;
     int n_dev_segs;           // stride of the per-segment constant table
@@ -54,6 +63,9 @@ struct RouteArgs {
     int *status;
     unsigned long long *counters;
     const int *t_dev;         // LEVEL schedule: current local step
+    unsigned long long *trace_t;  // [super-levels] block 0 barrier release times, or null
+    unsigned *trace_it;           // [super-levels] max iterations of an item, or null
+    unsigned long long *flow_hist;  // DATAFLOW: [n_steps+1][n_units][M] outflow bits
 };
 
 // launchers (trmc_kernels.hip); all asynchronous on `stream`
@@ -63,6 +75,16 @@ hipError_t launch_route_skewed(const RouteArgs &a, int mode, int grid, int block
 hipError_t launch_route_level(const RouteArgs &a, int mode, int level, long long items, int block,
                               hipStream_t stream);
 hipError_t launch_step_advance(int *t_dev, hipStream_t stream);
+hipError_t launch_route_dataflow(const RouteArgs &a, int mode, int grid, int block, hipStream_t stream);
+hipError_t route_dataflow_occupancy(int mode, int block, int *blocks_per_cu);
+// history slot 0 <- ring slot `src_slot`; slots 1..T <- kPending
+hipError_t launch_hist_init(unsigned long long *hist, const double *ring, int src_slot, long long um,
+                            int T, hipStream_t stream);
+// ring slot `dst_slot` <- history slot T; reach outflows -> hist_out when non-null
+hipError_t launch_hist_finish(const unsigned long long *hist, double *ring, int dst_slot, long long um,
+                              int T, hipStream_t stream);
+hipError_t launch_hist_extract(const unsigned long long *hist, const int *reach_last_unit, int R, int U,
+                               int M, int T, double *out, hipStream_t stream);
 hipError_t route_skewed_occupancy(int mode, int block, int *blocks_per_cu);
 hipError_t launch_reach_parallel(const double *segc, int n_seg, const double *ql,
                                  const int *rs, const int *rl, const double *qu,

@@ -30,7 +30,8 @@ __device__ __forceinline__ SolveOut solve_one(const RouteArgs &a, double qup, do
 // reference's CSR order starting from 0.0 (wavefront_mc.cu:523-529); an
 // internal unit boundary passes its single upstream value directly.
 template <int MODE>
-__device__ __forceinline__ void route_item(const RouteArgs &a, int u, int m, int t, Acc &acc)
+__device__ __forceinline__ void route_item(const RouteArgs &a, int u, int m, int t, Acc &acc,
+                                          unsigned *max_it = nullptr)
 {
     const int M = a.n_members;
     const size_t UM = (size_t)a.n_units * (size_t)M;
@@ -71,6 +72,7 @@ __device__ __forceinline__ void route_item(const RouteArgs &a, int u, int m, int
         quc_next = o.q;
         last = o.q;
         acc.iters += (unsigned long long)o.iters;
+        if (max_it && (unsigned)o.iters > *max_it) *max_it = (unsigned)o.iters;
         acc.capped += (unsigned long long)(o.flags & 1);
         acc.fallback += (unsigned long long)((o.flags >> 1) & 1);
     }
@@ -157,6 +159,8 @@ template <int MODE>
 __global__ void __launch_bounds__(256) k_route_skewed(RouteArgs a)
 {
     const long long t_start = (long long)wall_clock64();
+    if (a.trace_t && blockIdx.x == 0 && threadIdx.x == 0 && a.k_end > a.k_begin)
+        a.trace_t[a.k_end - a.k_begin] = (unsigned long long)t_start;   // start stamp after the last slot
     const long long nthreads = (long long)gridDim.x * blockDim.x;
     const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int M = a.n_members, T = a.n_steps, L = a.n_levels;
@@ -170,18 +174,203 @@ __global__ void __launch_bounds__(256) k_route_skewed(RouteArgs a)
             const long long hi = (long long)a.level_ptr[hi_l + 1] * M;
             long long off = gtid - lo % nthreads;
             if (off < 0) off += nthreads;
+            unsigned max_it = 0;
             for (long long i = lo + off; i < hi; i += nthreads) {
                 const int u = (int)(i / M);
                 const int m = (int)(i - (long long)u * M);
-                route_item<MODE>(a, u, m, k - a.unit_level[u], acc);
+                route_item<MODE>(a, u, m, k - a.unit_level[u], acc, a.trace_it ? &max_it : nullptr);
             }
+            if (a.trace_it && max_it) atomicMax(&a.trace_it[k - a.k_begin], max_it);
         }
         if (k + 1 < a.k_end) {
             if (!grid_sync(a, epoch, wait)) break;
             ++epoch;
         }
+        if (a.trace_t && blockIdx.x == 0 && threadIdx.x == 0)
+            a.trace_t[k - a.k_begin] = (unsigned long long)wall_clock64();
     }
     flush_acc(a, acc, wait, (long long)wall_clock64() - t_start);
+}
+
+// One unit, one member, one step of the dataflow schedule: upstream outflows
+// come from the full history (slot t+1 holds step t, slot 0 the initial
+// value), already checked present by the caller.
+template <int MODE>
+__device__ __forceinline__ void route_item_hist(const RouteArgs &a, int u, int m, int t,
+                                               const double *up_prev, const double *up_cur, int n_up,
+                                               bool internal, Acc &acc)
+{
+    const int M = a.n_members;
+    double qup_in, quc_in;
+    if (internal) {
+        qup_in = up_prev[0];
+        quc_in = up_cur[0];
+    } else {
+        qup_in = 0.0;
+        quc_in = 0.0;
+#pragma unroll
+        for (int k = 0; k < kDataflowMaxUp; ++k) {
+            if (k < n_up) {
+                qup_in += up_prev[k];
+                quc_in += up_cur[k];
+            }
+        }
+    }
+    const int s0 = a.unit_seg[u], s1 = a.unit_seg[u + 1];
+    const double *ql_row = a.qlat + (size_t)(a.t_base + t) * (size_t)a.n_dev_segs * M;
+    double last = 0.0, qup_next = 0.0, quc_next = 0.0;
+    for (int s = s0; s < s1; ++s) {
+        const size_t sm = (size_t)s * M + m;
+        const double qup = (s == s0) ? qup_in : qup_next;
+        const double quc = (s == s0) ? quc_in : quc_next;
+        const double qdp = a.q[sm];
+        const double dp = a.d[sm];
+        const double ql = ql_row[sm];
+        const Seg c = load_seg(a.segc, (size_t)a.n_dev_segs, s);
+        const SolveOut o = solve_one<MODE>(a, qup, quc, qdp, ql, c, dp);
+        a.q[sm] = o.q;
+        a.d[sm] = o.depth;
+        a.v[sm] = o.vel;
+        qup_next = qdp;
+        quc_next = o.q;
+        last = o.q;
+        acc.iters += (unsigned long long)o.iters;
+        acc.capped += (unsigned long long)(o.flags & 1);
+        acc.fallback += (unsigned long long)((o.flags >> 1) & 1);
+    }
+    const size_t slot = (size_t)(t + 1) * (size_t)a.n_units * M + (size_t)u * M + m;
+    __hip_atomic_store(&a.flow_hist[slot], (unsigned long long)__double_as_longlong(last),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Dataflow sweep without grid barriers.  Lane g owns member g % M of units
+// u_j = g / M + j S (S = nthreads / M, level-sorted) and walks its items in
+// super-level order k = level(u) + t.  An item starts when the outflows of
+// its upstream units for steps t and t-1 are present in the history (not
+// kPending); the value is its own ready flag, so no fences are needed: the
+// item reads nothing else another lane wrote.  Every awaited item has a
+// smaller k, so by induction on k every wait ends while all lanes are
+// resident.  Lanes advance independently; a slow solve delays only the
+// items downstream of it.
+static constexpr int kMaxUp = kDataflowMaxUp;   // upstream links per unit kept in registers
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_route_dataflow(RouteArgs a)
+{
+    const long long t_start = (long long)wall_clock64();
+    const int M = a.n_members, T = a.n_steps, U = a.n_units;
+    const long long nthreads = (long long)gridDim.x * blockDim.x;
+    const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long S = nthreads / M;                  // unit stride (nthreads % M == 0)
+    const int m = (int)(gtid % M);
+    const long long base = gtid / M;
+    const size_t UM = (size_t)U * M;
+    Acc acc = {0, 0, 0};
+    long long wait_ticks = 0;
+    // active window [j_lo, j_hi] of this lane's unit sequence
+    long long j_lo = 0, j_hi = -1;
+    int k = 0;
+    const long long n_own = (base < U && gtid < S * M) ? (U - 1 - base) / S + 1 : 0;
+    bool done = (n_own == 0 || T <= 0);
+    // current item state
+    long long j = 0;           // unit index in the sequence being worked at super-level k
+    bool have_item = false;
+    int u = 0, t = 0, n_up = 0;
+    bool internal = false;
+    int ups[kMaxUp];
+    long long spin_start = 0;
+    while (true) {
+        if (!done && !have_item) {
+            // find the next item: (k, j) in increasing k, then j
+            for (;;) {
+                while (j_hi + 1 < n_own && a.unit_level[base + (j_hi + 1) * S] <= k) ++j_hi;
+                while (j_lo <= j_hi && a.unit_level[base + j_lo * S] + T - 1 < k) ++j_lo;
+                if (j < j_lo) j = j_lo;
+                if (j_lo <= j_hi && j <= j_hi) break;
+                if (j_lo > j_hi) {                      // nothing active: jump to the next unit's level
+                    if (j_hi + 1 >= n_own) { done = true; break; }
+                    k = a.unit_level[base + (j_hi + 1) * S];
+                } else {
+                    ++k;                                 // window exhausted at this k
+                }
+                j = j_lo;
+            }
+            if (!done) {
+                u = (int)(base + j * S);
+                t = k - a.unit_level[u];
+                const int b = a.up_ptr[u];
+                n_up = a.up_ptr[u + 1] - b;
+                internal = (a.unit_flags[u] & 1) != 0;
+#pragma unroll
+                for (int q = 0; q < kMaxUp; ++q) ups[q] = (q < n_up) ? a.up_idx[b + q] : 0;
+                have_item = true;
+                spin_start = (long long)wall_clock64();
+            }
+        }
+        bool ready = false;
+        double up_prev[kMaxUp], up_cur[kMaxUp];
+        if (have_item) {
+            ready = true;
+#pragma unroll
+            for (int q = 0; q < kMaxUp; ++q) {
+                up_prev[q] = 0.0;
+                up_cur[q] = 0.0;
+                if (q < n_up) {
+                    const size_t v = (size_t)ups[q] * M + m;
+                    const unsigned long long bp = __hip_atomic_load(&a.flow_hist[(size_t)t * UM + v],
+                                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const unsigned long long bc = __hip_atomic_load(&a.flow_hist[(size_t)(t + 1) * UM + v],
+                                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ready = ready && bp != kPending && bc != kPending;
+                    up_prev[q] = __longlong_as_double((long long)bp);
+                    up_cur[q] = __longlong_as_double((long long)bc);
+                }
+            }
+        }
+        if (ready) {
+            wait_ticks += (long long)wall_clock64() - spin_start;
+            route_item_hist<MODE>(a, u, m, t, up_prev, up_cur, n_up, internal, acc);
+            have_item = false;
+            ++j;                                         // next unit of this super-level
+        }
+        const bool busy = !done;
+        if (!__any(busy)) break;
+        if (!__any(ready)) {
+            if (have_item && (long long)wall_clock64() - spin_start > a.timeout_ticks)
+                __hip_atomic_store(a.status, (int)KSTATUS_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != KSTATUS_OK) break;
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    flush_acc(a, acc, wait_ticks, (long long)wall_clock64() - t_start);
+}
+
+__global__ void k_hist_init(unsigned long long *hist, const double *ring, int src_slot, long long um, int T)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = um * (long long)(T + 1);
+    if (i >= total) return;
+    if (i < um) hist[i] = (unsigned long long)__double_as_longlong(ring[(size_t)src_slot * um + i]);
+    else hist[i] = kPending;
+}
+
+__global__ void k_hist_finish(const unsigned long long *hist, double *ring, int dst_slot, long long um, int T)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= um) return;
+    ring[(size_t)dst_slot * um + i] = __longlong_as_double((long long)hist[(size_t)T * um + i]);
+}
+
+__global__ void k_hist_extract(const unsigned long long *hist, const int *reach_last_unit, int R, int U,
+                               int M, int T, double *out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)M * T * R;
+    if (i >= total) return;
+    const int r = (int)(i % R);
+    const long long mt = i / R;
+    const int t = (int)(mt % T), m = (int)(mt / T);
+    out[i] = __longlong_as_double((long long)hist[((size_t)(t + 1) * U + reach_last_unit[r]) * M + m]);
 }
 
 // One level of one step (reference-style schedule, graph-captured per step).
@@ -325,6 +514,56 @@ hipError_t launch_route_level(const RouteArgs &a, int mode, int level, long long
         hipLaunchKernelGGL(k_route_level<0>, grid, dim3(block), 0, stream, a, level);
     else
         hipLaunchKernelGGL(k_route_level<1>, grid, dim3(block), 0, stream, a, level);
+    return hipGetLastError();
+}
+
+template <int MODE>
+static hipError_t launch_dataflow_t(const RouteArgs &a, int grid, int block, hipStream_t stream)
+{
+    RouteArgs copy = a;
+    void *args[] = {&copy};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_route_dataflow<MODE>), dim3(grid),
+                                      dim3(block), args, 0, stream);
+}
+
+hipError_t launch_route_dataflow(const RouteArgs &a, int mode, int grid, int block, hipStream_t stream)
+{
+    return mode == 0 ? launch_dataflow_t<0>(a, grid, block, stream) : launch_dataflow_t<1>(a, grid, block, stream);
+}
+
+hipError_t route_dataflow_occupancy(int mode, int block, int *blocks_per_cu)
+{
+    return mode == 0
+               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow<0>, block, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow<1>, block, 0);
+}
+
+hipError_t launch_hist_init(unsigned long long *hist, const double *ring, int src_slot, long long um, int T,
+                            hipStream_t stream)
+{
+    const long long total = um * (long long)(T + 1);
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_hist_init, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, hist, ring,
+                       src_slot, um, T);
+    return hipGetLastError();
+}
+
+hipError_t launch_hist_finish(const unsigned long long *hist, double *ring, int dst_slot, long long um, int T,
+                              hipStream_t stream)
+{
+    if (um <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_hist_finish, dim3((unsigned)((um + 255) / 256)), dim3(256), 0, stream, hist, ring,
+                       dst_slot, um, T);
+    return hipGetLastError();
+}
+
+hipError_t launch_hist_extract(const unsigned long long *hist, const int *reach_last_unit, int R, int U, int M,
+                               int T, double *out, hipStream_t stream)
+{
+    const long long total = (long long)M * T * R;
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_hist_extract, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, hist,
+                       reach_last_unit, R, U, M, T, out);
     return hipGetLastError();
 }
 

@@ -197,9 +197,32 @@ struct SolveOut {
     int flags;   // bit0: hit the iteration cap (CONVERGED), bit1: fell back (LEARNED)
 };
 
-// REFERENCE stopping rule, wavefront_mc.cu:215-337.
-TRMC_HD SolveOut solve_reference(double dt, double qup, double quc, double qdp,
-                                                    double ql, const Seg &c, double depthp)
+// Everything of one secant point that depends only on the depth: geometry,
+// R^(2/3), R^(5/3), celerity, storage constant, weighting width, Manning flow.
+struct Point {
+    double h, r23, ck, km, twu, qmn;
+};
+
+TRMC_HD Point eval_point(double h, double dt, const Seg &c)
+{
+    Point p;
+    p.h = h;
+    const Geo g = geometry(h, c);
+    const Pow2353 w = pow23_53(g.R);
+    p.r23 = w.p23;
+    p.ck = celerity(h, w.p23, w.p53, c);
+    p.km = storage_km(p.ck, dt, c);
+    p.twu = weight_width(h, g, c);
+    p.qmn = manning(g, w.p23, c);
+    return p;
+}
+
+// REFERENCE stopping rule, wavefront_mc.cu:215-337.  Each iteration evaluates
+// the secant points h_0 and h; the next iteration's h_0 is max(0, h) == h, so
+// the depth-only part of that point is carried over instead of recomputed
+// (same inputs, same operations: the values are identical).
+TRMC_HD SolveOut solve_reference(double dt, double qup, double quc, double qdp, double ql,
+                                 const Seg &c, double depthp)
 {
     SolveOut o;
     o.iters = 0;
@@ -215,30 +238,24 @@ TRMC_HD SolveOut solve_reference(double dt, double qup, double quc, double qdp,
     Coef C = {0.0, 0.0, 0.0, 0.0};
     double Qj_0 = 0.0;
     int maxiter = 100, total = 0;
+    Point p0;
+    bool have0 = false;
     for (int attempt = 0; attempt < 5; ++attempt) {
         double aerror = 0.01, rerror = 1.0;
         int iter = 0;
         while (rerror > 0.01 && aerror >= mindepth && iter <= maxiter) {
-            Geo g0 = geometry(h_0, c);
-            const Pow2353 w0 = pow23_53(g0.R);
-            const double r23_0 = w0.p23, r53_0 = w0.p53;
-            double ck0 = celerity(h_0, r23_0, r53_0, c);
-            double km0 = storage_km(ck0, dt, c);
-            double x0 = weighting(ck0, weight_width(h_0, g0, c), Qj_0, 0.0, c);
-            Coef m0 = muskingum(km0, x0, dt, ql);
-            double q0 = m0.c1 * qup + m0.c2 * quc + m0.c3 * qdp + m0.c4;
-            Qj_0 = q0 - manning(g0, r23_0, c);
+            if (!have0 || !(p0.h == h_0)) p0 = eval_point(h_0, dt, c);
+            const double x0 = weighting(p0.ck, p0.twu, Qj_0, 0.0, c);
+            const Coef m0 = muskingum(p0.km, x0, dt, ql);
+            const double q0 = m0.c1 * qup + m0.c2 * quc + m0.c3 * qdp + m0.c4;
+            Qj_0 = q0 - p0.qmn;
 
-            Geo g1 = geometry(h, c);
-            const Pow2353 w1 = pow23_53(g1.R);
-            const double r23_1 = w1.p23, r53_1 = w1.p53;
-            double ck1 = celerity(h, r23_1, r53_1, c);
-            double km1 = storage_km(ck1, dt, c);
-            double X = weighting(ck1, weight_width(h, g1, c), q0, 0.25, c);
-            C = muskingum(km1, X, dt, ql);
-            double routed = C.c1 * qup + C.c2 * quc + C.c3 * qdp;
+            const Point p1 = eval_point(h, dt, c);
+            const double X = weighting(p1.ck, p1.twu, q0, 0.25, c);
+            C = muskingum(p1.km, X, dt, ql);
+            const double routed = C.c1 * qup + C.c2 * quc + C.c3 * qdp;
             if (C.c4 < 0.0 && fabs(C.c4) > routed) C.c4 = -routed;
-            double Qj = (routed + C.c4) - manning(g1, r23_1, c);
+            const double Qj = (routed + C.c4) - p1.qmn;
 
             double h_1 = (Qj_0 - Qj != 0.0) ? h - (Qj * (h_0 - h)) / (Qj_0 - Qj) : h;
             if (h_1 < 0.0) h_1 = h;
@@ -246,6 +263,8 @@ TRMC_HD SolveOut solve_reference(double dt, double qup, double quc, double qdp,
             else { rerror = 0.0; aerror = 0.9; }
             h_0 = fmax(0.0, h);
             h = fmax(0.0, h_1);
+            p0 = p1;
+            have0 = true;
             ++iter;
             if (h < mindepth) break;
         }

@@ -102,9 +102,11 @@ def shreve():
     return w, T, ref, cr
 
 
-@pytest.mark.parametrize("schedule,units", [(trmc.SCHED_SKEWED, 1), (trmc.SCHED_SKEWED, 2),
-                                            (trmc.SCHED_SKEWED, 7), (trmc.SCHED_SKEWED, 64),
-                                            (trmc.SCHED_LEVEL, 1), (trmc.SCHED_LEVEL, 64)])
+@pytest.mark.parametrize("schedule,units", [(trmc.SCHED_DATAFLOW, 1), (trmc.SCHED_DATAFLOW, 3),
+                                            (trmc.SCHED_DATAFLOW, 64), (trmc.SCHED_SKEWED, 1),
+                                            (trmc.SCHED_SKEWED, 2), (trmc.SCHED_SKEWED, 7),
+                                            (trmc.SCHED_SKEWED, 64), (trmc.SCHED_LEVEL, 1),
+                                            (trmc.SCHED_LEVEL, 64)])
 def test_every_timestep_matches_reference(gpu, shreve, schedule, units):
     w, T, ref, cr = shreve
     _log(f"shreve 40k x {T} schedule={schedule} units={units}")
@@ -127,8 +129,8 @@ def test_every_timestep_matches_reference(gpu, shreve, schedule, units):
 def test_results_do_not_depend_on_schedule_bitwise(gpu, shreve):
     w, T, _, _ = shreve
     outs = []
-    for sched, units, block in ((trmc.SCHED_SKEWED, 1, 256), (trmc.SCHED_SKEWED, 5, 64),
-                                (trmc.SCHED_LEVEL, 3, 128)):
+    for sched, units, block in ((trmc.SCHED_DATAFLOW, 1, 256), (trmc.SCHED_SKEWED, 5, 64),
+                                (trmc.SCHED_LEVEL, 3, 128), (trmc.SCHED_DATAFLOW, 2, 64)):
         with trmc.Network(w.topo, schedule=sched, unit_segments=units, block_size=block) as net:
             net.set_params(w.params)
             net.set_state(w.qdp0, w.dp0, w.reach_last0)
@@ -157,10 +159,11 @@ def test_chained_runs_and_reset(gpu, shreve):
     check(a[0][0], cr["qdc"], "chained qdc")
 
 
-def test_ensemble_members_route_independently(gpu):
+@pytest.mark.parametrize("schedule", [trmc.SCHED_DATAFLOW, trmc.SCHED_SKEWED])
+def test_ensemble_members_route_independently(gpu, schedule):
     T, M = 24, 3
     w = trmc.make_workload(2, 12000, T, n_reaches=3600, n_members=M, seed=5)
-    with trmc.Network(w.topo, n_members=M) as net:
+    with trmc.Network(w.topo, n_members=M, schedule=schedule) as net:
         net.set_params(w.params)
         net.set_state(w.qdp0, w.dp0, w.reach_last0)
         net.set_forcing(w.qlat)
@@ -227,7 +230,7 @@ def test_full_size_network(gpu):
         net.run(T)
         states.append(net.state())
     _log(f"full-size 288 steps: {st['kernel_ms']:.1f} ms")
-    with trmc.Network(w.topo, unit_segments=4) as net:
+    with trmc.Network(w.topo, unit_segments=4, schedule=trmc.SCHED_SKEWED) as net:
         net.set_params(w.params)
         net.set_state(w.qdp0, w.dp0, w.reach_last0)
         net.set_forcing(w.qlat)
