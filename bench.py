@@ -91,10 +91,11 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="shreve-1m", choices=sorted(WORKLOADS))
-    ap.add_argument("--unit-segments", type=int, default=1)
-    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--unit-segments", type=int, default=4)
+    ap.add_argument("--block", type=int, default=64)
+    ap.add_argument("--batch-iters", type=int, default=8)
     ap.add_argument("--grid", type=int, default=0)
-    ap.add_argument("--schedule", type=int, default=1)
+    ap.add_argument("--schedule", type=int, default=0, help="0 auto, 1 skewed, 2 level, 3 dataflow")
     ap.add_argument("--cpu-steps", type=int, default=24, help="timesteps in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -111,7 +112,8 @@ def main():
         f"{w.topo.n_levels} levels, {T} steps (generated in {time.perf_counter() - t0:.1f} s)")
 
     net = trmc.Network(w.topo, schedule=args.schedule, unit_segments=args.unit_segments,
-                       block_size=args.block, grid_blocks=args.grid, n_members=M)
+                       block_size=args.block, grid_blocks=args.grid, n_members=M,
+                       batch_iters=args.batch_iters)
     net.set_params(w.params)
     net.set_state(w.qdp0, w.dp0, w.reach_last0)
     net.set_forcing(w.qlat)
@@ -161,10 +163,11 @@ def main():
                         f"{w.topo.n_reaches} reaches, {w.topo.n_levels} reach levels, {T} steps, dt=300 s",
             "segments": w.topo.n_segments, "reaches": w.topo.n_reaches, "reach_levels": w.topo.n_levels,
             "timesteps": T, "members": M, "seed": seed, "stopping_rule": "reference (1% / 1 cm)",
-            "schedule": "time-skewed persistent" if args.schedule == 1 else "level launches, hipGraph per step",
+            "schedule": {0: "auto (dataflow)", 1: "time-skewed persistent, device barrier",
+                         2: "level launches, hipGraph per step", 3: "dataflow persistent"}[args.schedule],
             "unit_segments": args.unit_segments, "unit_levels": info["n_levels"],
             "super_levels": info["super_levels"], "grid_blocks": stats.get("grid_blocks"),
-            "block": args.block, "parallelism": "1 GPU",
+            "block": args.block, "batch_iters": args.batch_iters, "parallelism": "1 GPU",
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

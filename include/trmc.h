@@ -82,17 +82,20 @@ typedef struct { double dx, bw, tw, twcc, n_ch, ncc, cs, s0; } trmc_chparams_f64
 typedef struct {
     int    mode;            /* TRMC_MODE_* (default REFERENCE) */
     int    schedule;        /* TRMC_SCHED_* (default AUTO) */
-    int    unit_segments;   /* max segments per work unit; 0 = auto (1) */
-    int    block_size;      /* threads per workgroup; 0 = auto (256) */
+    int    unit_segments;   /* max segments per work unit; 0 = auto (4) */
+    int    block_size;      /* threads per workgroup; 0 = auto (64) */
     int    n_members;       /* ensemble members sharing params (>= 1) */
     int    record_history;  /* keep every step's last-segment outflow per reach */
     double tol;             /* CONVERGED/LEARNED relative depth tolerance (default 1e-13) */
     int    max_iter;        /* CONVERGED/LEARNED iteration cap (default 60) */
     int    learned_budget;  /* LEARNED: evaluations before falling back (default 12) */
     int    grid_blocks;     /* persistent grid size; 0 = fill the device */
-    int    dynamic_queue;   /* SKEWED: 1 = atomic work queue per super-level */
+    int    dataflow_variant;/* DATAFLOW: 0 = secant iterations interleaved across lanes
+                               (REFERENCE mode), 1 = one whole item per lane pass */
     int    device;          /* HIP device ordinal */
     int    trace;           /* SKEWED: record per-super-level time and max iterations */
+    int    batch_iters;     /* DATAFLOW interleaved: secant iterations between scheduling
+                               phases (0 = default 8) */
 } trmc_options;
 
 typedef struct {

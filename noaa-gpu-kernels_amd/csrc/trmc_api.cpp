@@ -139,7 +139,8 @@ struct trmc_network {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int *d_level_ptr = nullptr, *d_unit_level = nullptr, *d_unit_seg = nullptr, *d_up_ptr = nullptr,
-        *d_up_idx = nullptr, *d_unit_flags = nullptr, *d_unit_reach = nullptr, *d_seg_of_dev = nullptr;
+        *d_up_idx = nullptr, *d_unit_flags = nullptr, *d_unit_reach = nullptr, *d_seg_of_dev = nullptr,
+        *d_unit_rec = nullptr;
     double *d_segc = nullptr, *d_qlat = nullptr, *d_q = nullptr, *d_d = nullptr, *d_v = nullptr,
            *d_ring = nullptr, *d_hist = nullptr;
     double *d_q0 = nullptr, *d_d0 = nullptr, *d_ring0 = nullptr;   // initial state for reset
@@ -160,6 +161,7 @@ struct trmc_network {
     {
         dev_free(d_level_ptr); dev_free(d_unit_level); dev_free(d_unit_seg); dev_free(d_up_ptr);
         dev_free(d_up_idx); dev_free(d_unit_flags); dev_free(d_unit_reach); dev_free(d_seg_of_dev);
+        dev_free(d_unit_rec);
         dev_free(d_segc); dev_free(d_qlat); dev_free(d_q); dev_free(d_d); dev_free(d_v);
         dev_free(d_ring); dev_free(d_hist); dev_free(d_mlp); dev_free(d_bar); dev_free(d_status);
         dev_free(d_t); dev_free(d_counters); dev_free(d_q0); dev_free(d_d0); dev_free(d_ring0);
@@ -207,9 +209,10 @@ void trmc_default_options(trmc_options *opt)
     opt->max_iter = 60;
     opt->learned_budget = 12;
     opt->grid_blocks = 0;
-    opt->dynamic_queue = 0;
+    opt->dataflow_variant = 0;
     opt->device = 0;
     opt->trace = 0;
+    opt->batch_iters = 0;
 }
 
 int trmc_device_synchronize(int device)
@@ -283,10 +286,10 @@ int trmc_network_create(const int *reach_seg_start, const int *reach_seg_len,
     if (opt) net->opt = *opt; else trmc_default_options(&net->opt);
     trmc_options &o = net->opt;
     if (o.n_members < 1) return fail(TRMC_ERR_ARG, "n_members must be >= 1");
-    if (o.block_size <= 0) o.block_size = 256;
+    if (o.block_size <= 0) o.block_size = 64;
     if (o.block_size % 64 != 0 || o.block_size > 256)
         return fail(TRMC_ERR_ARG, "block_size must be a multiple of 64, at most 256");
-    if (o.unit_segments <= 0) o.unit_segments = 1;
+    if (o.unit_segments <= 0) o.unit_segments = 4;   // measured best on the Shreve workloads
     if (o.mode < TRMC_MODE_REFERENCE || o.mode > TRMC_MODE_LEARNED)
         return fail(TRMC_ERR_ARG, "unknown mode %d", o.mode);
     if (o.mode == TRMC_MODE_LEARNED)
@@ -318,6 +321,8 @@ int trmc_network_create(const int *reach_seg_start, const int *reach_seg_len,
     if ((rc = dev_upload(&net->d_unit_flags, P.unit_flags.data(), P.unit_flags.size(), s))) return rc;
     if ((rc = dev_upload(&net->d_unit_reach, P.unit_reach.data(), P.unit_reach.size(), s))) return rc;
     if ((rc = dev_upload(&net->d_seg_of_dev, P.seg_of_dev.data(), P.seg_of_dev.size(), s))) return rc;
+    if (!P.unit_rec.empty() && (rc = dev_upload(&net->d_unit_rec, P.unit_rec.data(), P.unit_rec.size(), s)))
+        return rc;
     const size_t nd = (size_t)P.n_dev_segments * net->M;
     if ((rc = dev_alloc(&net->d_q, nd))) return rc;
     if ((rc = dev_alloc(&net->d_d, nd))) return rc;
@@ -513,6 +518,8 @@ static RouteArgs make_args(trmc_network *net, int T, double dt)
     a.status = net->d_status;
     a.counters = net->d_counters;
     a.t_dev = net->d_t;
+    a.unit_rec = reinterpret_cast<const int4 *>(net->d_unit_rec);
+    a.batch_iters = net->opt.batch_iters > 0 ? net->opt.batch_iters : 8;
     return a;
 }
 
@@ -562,8 +569,10 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
             if ((rc = dev_alloc(&net->d_flow_hist, need))) return rc;
             net->flow_hist_n = need;
         }
+        const bool il = (mode == 0 && o.dataflow_variant == 0);
         int bpcu = 0;
-        HIPCHK(route_dataflow_occupancy(mode, o.block_size, &bpcu));
+        if (il) HIPCHK(route_dataflow_il_occupancy(o.block_size, &bpcu));
+        else HIPCHK(route_dataflow_occupancy(mode, o.block_size, &bpcu));
         if (bpcu <= 0) return fail(TRMC_ERR_HIP, "dataflow kernel cannot be resident");
         long long resident = (long long)bpcu * net->cu_count;
         long long need_blocks = ((long long)um + o.block_size - 1) / o.block_size;
@@ -576,7 +585,8 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
         a.flow_hist = net->d_flow_hist;
         HIPCHK(launch_hist_init(net->d_flow_hist, net->d_ring, net->ring_shift % P.ring, (long long)um, T, s));
         HIPCHK(hipEventRecord(net->ev0, s));
-        HIPCHK(launch_route_dataflow(a, mode, (int)grid, o.block_size, s));
+        if (il) HIPCHK(launch_route_dataflow_il(a, (int)grid, o.block_size, s));
+        else HIPCHK(launch_route_dataflow(a, mode, (int)grid, o.block_size, s));
         HIPCHK(hipEventRecord(net->ev1, s));
         HIPCHK(launch_hist_finish(net->d_flow_hist, net->d_ring, (T + net->ring_shift) % P.ring, (long long)um, T, s));
         if (o.record_history) {

@@ -66,6 +66,8 @@ struct RouteArgs {
     unsigned long long *trace_t;  // [super-levels] block 0 barrier release times, or null
     unsigned *trace_it;           // [super-levels] max iterations of an item, or null
     unsigned long long *flow_hist;  // DATAFLOW: [n_steps+1][n_units][M] outflow bits
+    const int4 *unit_rec;           // DATAFLOW: [n_units][2] packed unit records
+    int batch_iters;                // DATAFLOW interleaved: secant iterations per batch
 };
 
 // launchers (trmc_kernels.hip); all asynchronous on `stream`
@@ -77,6 +79,9 @@ hipError_t launch_route_level(const RouteArgs &a, int mode, int level, long long
 hipError_t launch_step_advance(int *t_dev, hipStream_t stream);
 hipError_t launch_route_dataflow(const RouteArgs &a, int mode, int grid, int block, hipStream_t stream);
 hipError_t route_dataflow_occupancy(int mode, int block, int *blocks_per_cu);
+// REFERENCE rule with secant iterations interleaved across lanes
+hipError_t launch_route_dataflow_il(const RouteArgs &a, int grid, int block, hipStream_t stream);
+hipError_t route_dataflow_il_occupancy(int block, int *blocks_per_cu);
 // history slot 0 <- ring slot `src_slot`; slots 1..T <- kPending
 hipError_t launch_hist_init(unsigned long long *hist, const double *ring, int src_slot, long long um,
                             int T, hipStream_t stream);

@@ -283,13 +283,13 @@ __global__ void __launch_bounds__(256) k_route_dataflow(RouteArgs a)
         if (!done && !have_item) {
             // find the next item: (k, j) in increasing k, then j
             for (;;) {
-                while (j_hi + 1 < n_own && a.unit_level[base + (j_hi + 1) * S] <= k) ++j_hi;
-                while (j_lo <= j_hi && a.unit_level[base + j_lo * S] + T - 1 < k) ++j_lo;
+                while (j_hi + 1 < n_own && a.unit_rec[2 * (base + (j_hi + 1) * S)].x <= k) ++j_hi;
+                while (j_lo <= j_hi && a.unit_rec[2 * (base + j_lo * S)].x + T - 1 < k) ++j_lo;
                 if (j < j_lo) j = j_lo;
                 if (j_lo <= j_hi && j <= j_hi) break;
                 if (j_lo > j_hi) {                      // nothing active: jump to the next unit's level
                     if (j_hi + 1 >= n_own) { done = true; break; }
-                    k = a.unit_level[base + (j_hi + 1) * S];
+                    k = a.unit_rec[2 * (base + (j_hi + 1) * S)].x;
                 } else {
                     ++k;                                 // window exhausted at this k
                 }
@@ -297,12 +297,15 @@ __global__ void __launch_bounds__(256) k_route_dataflow(RouteArgs a)
             }
             if (!done) {
                 u = (int)(base + j * S);
-                t = k - a.unit_level[u];
-                const int b = a.up_ptr[u];
-                n_up = a.up_ptr[u + 1] - b;
-                internal = (a.unit_flags[u] & 1) != 0;
-#pragma unroll
-                for (int q = 0; q < kMaxUp; ++q) ups[q] = (q < n_up) ? a.up_idx[b + q] : 0;
+                const int4 r0 = a.unit_rec[2 * (size_t)u];
+                const int4 r1 = a.unit_rec[2 * (size_t)u + 1];
+                t = k - r0.x;
+                n_up = r0.w >> 8;
+                internal = (r0.w & 1) != 0;
+                ups[0] = r1.x;
+                ups[1] = r1.y;
+                ups[2] = r1.z;
+                ups[3] = r1.w;
                 have_item = true;
                 spin_start = (long long)wall_clock64();
             }
@@ -337,6 +340,179 @@ __global__ void __launch_bounds__(256) k_route_dataflow(RouteArgs a)
         if (!__any(busy)) break;
         if (!__any(ready)) {
             if (have_item && (long long)wall_clock64() - spin_start > a.timeout_ticks)
+                __hip_atomic_store(a.status, (int)KSTATUS_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != KSTATUS_OK) break;
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    flush_acc(a, acc, wait_ticks, (long long)wall_clock64() - t_start);
+}
+
+// Dataflow sweep with secant iterations interleaved across the lanes of a
+// wave.  Same ownership, ordering and readiness test as k_route_dataflow, but
+// a lane's work is a state machine, and each trip of the lane loop is
+//   (1) a scheduling phase: lanes without a running solve look for their next
+//       item, poll its upstream outflows, and load the next segment (all the
+//       memory reads of the loop happen here), then
+//   (2) a solve phase: up to batch_iters secant iterations for every lane
+//       with a running solve, touching registers only.  A lane whose solve
+//       ends stores its segment results (and, at the unit's last segment,
+//       publishes the outflow) at once and idles until the next phase (1).
+// A slow solve therefore holds its own lane only, not the other 63 lanes of
+// its wave.  Each solve is SecantRef, identical to solve_reference.
+__global__ void __launch_bounds__(256) k_route_dataflow_il(RouteArgs a)
+{
+    const long long t_start = (long long)wall_clock64();
+    const int M = a.n_members, T = a.n_steps, U = a.n_units;
+    const long long nthreads = (long long)gridDim.x * blockDim.x;
+    const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long S = nthreads / M;
+    const int m = (int)(gtid % M);
+    const long long base = gtid / M;
+    const size_t UM = (size_t)U * M;
+    const size_t nd = (size_t)a.n_dev_segs;
+    Acc acc = {0, 0, 0};
+    long long wait_ticks = 0;
+    long long j_lo = 0, j_hi = -1, j = 0;
+    int k = 0;
+    const long long n_own = (base < U && gtid < S * M) ? (U - 1 - base) / S + 1 : 0;
+    // PH_NEXT: the unit's next segment needs opening (loads) in phase (1)
+    enum { PH_DONE = 0, PH_NEED, PH_WAIT, PH_NEXT, PH_SOLVE };
+    int phase = (n_own == 0 || T <= 0) ? PH_DONE : PH_NEED;
+    int u = 0, t = 0, n_up = 0, s = 0, s0 = 0, s1 = 0;
+    bool internal = false;
+    int4 ups = make_int4(0, 0, 0, 0);
+    double qup_in = 0.0, quc_in = 0.0, qup_next = 0.0, quc_next = 0.0, last = 0.0, cur_qdp = 0.0;
+    const double *ql_row = a.qlat;
+    Seg c;
+    SecantRef sr;
+    long long spin_start = 0;
+
+    auto publish = [&]() {
+        const size_t slot = (size_t)(t + 1) * UM + (size_t)u * M + m;
+        __hip_atomic_store(&a.flow_hist[slot], (unsigned long long)__double_as_longlong(last),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    // store a finished segment and move to the next one (stores only)
+    auto close_segment = [&](const SolveOut &o, double qdp) {
+        const size_t sm = (size_t)s * M + m;
+        a.q[sm] = o.q;
+        a.d[sm] = o.depth;
+        a.v[sm] = o.vel;
+        acc.iters += (unsigned long long)o.iters;
+        qup_next = qdp;
+        quc_next = o.q;
+        last = o.q;
+        ++s;
+        if (s >= s1) {
+            publish();
+            ++j;
+            phase = PH_NEED;
+        } else {
+            phase = PH_NEXT;
+        }
+    };
+    // load segment s and start its solve; guard-trivial segments close at once
+    auto open_segments = [&]() {
+        while (phase == PH_NEXT) {
+            const size_t sm = (size_t)s * M + m;
+            const double qup = (s == s0) ? qup_in : qup_next;
+            const double quc = (s == s0) ? quc_in : quc_next;
+            const double qdp = a.q[sm];
+            c = load_seg(a.segc, nd, s);
+            if (!sr.start(a.dt, qup, quc, qdp, ql_row[sm], c, a.d[sm])) {
+                cur_qdp = qdp;
+                phase = PH_SOLVE;
+            } else {
+                close_segment(sr.finish(c), qdp);
+            }
+        }
+    };
+
+    while (true) {
+        bool progressed = false;
+        // ---------------------------------------------- (1) scheduling phase
+        if (phase == PH_NEXT) {
+            open_segments();
+            progressed = true;
+        }
+        if (phase == PH_NEED) {
+            for (;;) {
+                while (j_hi + 1 < n_own && a.unit_rec[2 * (base + (j_hi + 1) * S)].x <= k) ++j_hi;
+                while (j_lo <= j_hi && a.unit_rec[2 * (base + j_lo * S)].x + T - 1 < k) ++j_lo;
+                if (j < j_lo) j = j_lo;
+                if (j_lo <= j_hi && j <= j_hi) break;
+                if (j_lo > j_hi) {
+                    if (j_hi + 1 >= n_own) { phase = PH_DONE; break; }
+                    k = a.unit_rec[2 * (base + (j_hi + 1) * S)].x;
+                } else {
+                    ++k;
+                }
+                j = j_lo;
+            }
+            if (phase == PH_NEED) {
+                u = (int)(base + j * S);
+                const int4 r0 = a.unit_rec[2 * (size_t)u];
+                ups = a.unit_rec[2 * (size_t)u + 1];
+                t = k - r0.x;
+                s0 = r0.y;
+                s1 = r0.z;
+                internal = (r0.w & 1) != 0;
+                n_up = r0.w >> 8;
+                phase = PH_WAIT;
+                spin_start = (long long)wall_clock64();
+            }
+        }
+        if (phase == PH_WAIT) {
+            bool ready = true;
+            double sp = 0.0, sc = 0.0, first_p = 0.0, first_c = 0.0;
+            const int upv[4] = {ups.x, ups.y, ups.z, ups.w};
+#pragma unroll
+            for (int q = 0; q < kMaxUp; ++q) {
+                if (q < n_up) {
+                    const size_t v = (size_t)upv[q] * M + m;
+                    const unsigned long long bp = __hip_atomic_load(&a.flow_hist[(size_t)t * UM + v],
+                                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const unsigned long long bc = __hip_atomic_load(&a.flow_hist[(size_t)(t + 1) * UM + v],
+                                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ready = ready && bp != kPending && bc != kPending;
+                    const double vp = __longlong_as_double((long long)bp);
+                    const double vc = __longlong_as_double((long long)bc);
+                    // fixed-order sums starting from 0.0 (wavefront_mc.cu:523-529)
+                    sp += vp;
+                    sc += vc;
+                    if (q == 0) { first_p = vp; first_c = vc; }
+                }
+            }
+            if (ready) {
+                wait_ticks += (long long)wall_clock64() - spin_start;
+                qup_in = internal ? first_p : sp;
+                quc_in = internal ? first_c : sc;
+                s = s0;
+                last = 0.0;
+                ql_row = a.qlat + (size_t)(a.t_base + t) * nd * M;
+                progressed = true;
+                if (s >= s1) {                       // unit without segments: outflow 0
+                    publish();
+                    ++j;
+                    phase = PH_NEED;
+                } else {
+                    phase = PH_NEXT;
+                    open_segments();
+                }
+            }
+        }
+        // -------------------------------------- (2) solve phase, registers only
+        for (int r = 0; r < a.batch_iters; ++r) {
+            if (!__any(phase == PH_SOLVE)) break;
+            if (phase == PH_SOLVE) {
+                progressed = true;
+                if (sr.step(c)) close_segment(sr.finish(c), cur_qdp);
+            }
+        }
+        if (!__any(phase != PH_DONE)) break;
+        if (!__any(progressed)) {
+            if (phase == PH_WAIT && (long long)wall_clock64() - spin_start > a.timeout_ticks)
                 __hip_atomic_store(a.status, (int)KSTATUS_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (__hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != KSTATUS_OK) break;
             __builtin_amdgcn_s_sleep(1);
@@ -524,6 +700,19 @@ static hipError_t launch_dataflow_t(const RouteArgs &a, int grid, int block, hip
     void *args[] = {&copy};
     return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_route_dataflow<MODE>), dim3(grid),
                                       dim3(block), args, 0, stream);
+}
+
+hipError_t launch_route_dataflow_il(const RouteArgs &a, int grid, int block, hipStream_t stream)
+{
+    RouteArgs copy = a;
+    void *args[] = {&copy};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_route_dataflow_il), dim3(grid),
+                                      dim3(block), args, 0, stream);
+}
+
+hipError_t route_dataflow_il_occupancy(int block, int *blocks_per_cu)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il, block, 0);
 }
 
 hipError_t launch_route_dataflow(const RouteArgs &a, int mode, int grid, int block, hipStream_t stream)

@@ -185,6 +185,18 @@ int build_plan(const int *reach_seg_start, const int *reach_seg_len,
         P.unit_seg[(size_t)du + 1] = (int)P.seg_of_dev.size();
     }
     P.n_dev_segments = (int)P.seg_of_dev.size();
+    if (P.max_upstream <= 4) {
+        P.unit_rec.assign((size_t)U * 8, 0);
+        for (int du = 0; du < U; ++du) {
+            int *r = &P.unit_rec[(size_t)du * 8];
+            const int nu = P.up_ptr[(size_t)du + 1] - P.up_ptr[du];
+            r[0] = P.unit_level[du];
+            r[1] = P.unit_seg[du];
+            r[2] = P.unit_seg[(size_t)du + 1];
+            r[3] = (P.unit_flags[du] & 0xFF) | (nu << 8);
+            for (int q = 0; q < nu; ++q) r[4 + q] = P.up_idx[(size_t)P.up_ptr[du] + q];
+        }
+    }
     P.reach_last_unit.resize(n_reaches);
     for (int r = 0; r < n_reaches; ++r) P.reach_last_unit[r] = dev_of_unit[last_unit[r]];
     return 0;

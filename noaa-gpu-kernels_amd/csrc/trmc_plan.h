@@ -43,6 +43,10 @@ struct Plan {
     std::vector<int> dev_of_seg;     // [N] device segment or -1 (segment in no reach)
     std::vector<int> reach_last_unit;// [R] device unit holding each reach's last segment
     int n_dev_segments = 0;          // covered segments
+    // packed per-unit record for the dataflow kernels, 8 ints per unit:
+    // level, first segment, end segment, flags | n_up << 8, up[0..3]
+    // (upstream entries beyond n_up are 0; empty unless max_upstream <= 4)
+    std::vector<int> unit_rec;
 };
 
 // Validates the reference layout and builds the plan.  Returns 0 or a

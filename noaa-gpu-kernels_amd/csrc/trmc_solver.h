@@ -281,6 +281,94 @@ TRMC_HD SolveOut solve_reference(double dt, double qup, double quc, double qdp, 
     return o;
 }
 
+// The REFERENCE solve as a resumable state machine: start() does the guards
+// and the initial bracket, step() one secant iteration plus the loop/attempt
+// bookkeeping of wavefront_mc.cu:246-318, finish() the outputs.  Driving
+// start(); while (!step()); finish() performs exactly the operations of
+// solve_reference in the same order, so the results are identical; a kernel
+// can interleave the iterations of many solves.
+struct SecantRef {
+    Point p0;
+    Coef C;
+    double qup, quc, qdp, ql, dt;
+    double h, h_0, Qj_0, rerror, aerror;
+    int iter, maxiter, attempt, total;
+    bool have0;
+    bool trivial;     // guard hit: outputs are zero
+
+    // returns true when no iteration is needed (guard)
+    TRMC_HD bool start(double dt_, double qup_, double quc_, double qdp_, double ql_, const Seg &c,
+                       double depthp)
+    {
+        dt = dt_; qup = qup_; quc = quc_; qdp = qdp_; ql = ql_;
+        total = 0;
+        trivial = seg_invalid(c) || (ql <= 0.0 && qup <= 0.0 && quc <= 0.0 && qdp <= 0.0);
+        if (trivial) return true;
+        const double d0 = fmax(depthp, 0.0);
+        h = d0 * 1.33 + 0.01;
+        h_0 = d0 * 0.67;
+        C.c1 = 0.0; C.c2 = 0.0; C.c3 = 0.0; C.c4 = 0.0;
+        Qj_0 = 0.0;
+        maxiter = 100;
+        attempt = 0;
+        iter = 0;
+        aerror = 0.01;
+        rerror = 1.0;
+        have0 = false;
+        return false;
+    }
+
+    // one iteration; returns true when the solve has ended
+    TRMC_HD bool step(const Seg &c)
+    {
+        const double mindepth = 0.01;
+        if (!have0 || !(p0.h == h_0)) p0 = eval_point(h_0, dt, c);
+        const double x0 = weighting(p0.ck, p0.twu, Qj_0, 0.0, c);
+        const Coef m0 = muskingum(p0.km, x0, dt, ql);
+        const double q0 = m0.c1 * qup + m0.c2 * quc + m0.c3 * qdp + m0.c4;
+        Qj_0 = q0 - p0.qmn;
+        const Point p1 = eval_point(h, dt, c);
+        const double X = weighting(p1.ck, p1.twu, q0, 0.25, c);
+        C = muskingum(p1.km, X, dt, ql);
+        const double routed = C.c1 * qup + C.c2 * quc + C.c3 * qdp;
+        if (C.c4 < 0.0 && fabs(C.c4) > routed) C.c4 = -routed;
+        const double Qj = (routed + C.c4) - p1.qmn;
+        double h_1 = (Qj_0 - Qj != 0.0) ? h - (Qj * (h_0 - h)) / (Qj_0 - Qj) : h;
+        if (h_1 < 0.0) h_1 = h;
+        if (h > 0.0) { rerror = fabs((h_1 - h) / h); aerror = fabs(h_1 - h); }
+        else { rerror = 0.0; aerror = 0.9; }
+        h_0 = fmax(0.0, h);
+        h = fmax(0.0, h_1);
+        p0 = p1;
+        have0 = true;
+        ++iter;
+        const bool more = !(h < mindepth) && rerror > 0.01 && aerror >= mindepth && iter <= maxiter;
+        if (more) return false;
+        total += iter;                       // attempt over
+        if (iter < maxiter) return true;
+        h *= 1.33;
+        h_0 *= 0.67;
+        maxiter += 25;
+        if (++attempt == 5) return true;
+        iter = 0;                            // the next attempt always iterates at least once
+        aerror = 0.01;
+        rerror = 1.0;
+        return false;
+    }
+
+    TRMC_HD SolveOut finish(const Seg &c) const
+    {
+        SolveOut o;
+        o.flags = 0;
+        o.iters = total;
+        if (trivial) { o.q = 0.0; o.depth = 0.0; o.vel = 0.0; return o; }
+        o.vel = velocity(h, c);
+        o.depth = h;
+        o.q = outflow(C, qup, quc, qdp);
+        return o;
+    }
+};
+
 // ------------------------------------------------------------- CONVERGED
 // F(h) with the X0 = 0.5 coefficients feeding the weighting X(h); also returns
 // the final coefficients at h for the outflow rule.

@@ -31,3 +31,19 @@ extern "C" void shim_pow23_53(const double *x, double *out, int n)
         out[2 * i + 1] = p.p53;
     }
 }
+
+// The resumable REFERENCE solve driven to completion, for comparison with
+// solve_reference.
+extern "C" int shim_solve_resumable(const double *segc, const double *in, double dt, double *out)
+{
+    Seg c = load_seg(segc, 1, 0);
+    SecantRef s;
+    if (!s.start(dt, in[0], in[1], in[2], in[3], c, in[4]))
+        while (!s.step(c)) {
+        }
+    SolveOut o = s.finish(c);
+    out[0] = o.q;
+    out[1] = o.depth;
+    out[2] = o.vel;
+    return o.iters;
+}

@@ -102,15 +102,16 @@ def shreve():
     return w, T, ref, cr
 
 
-@pytest.mark.parametrize("schedule,units", [(trmc.SCHED_DATAFLOW, 1), (trmc.SCHED_DATAFLOW, 3),
-                                            (trmc.SCHED_DATAFLOW, 64), (trmc.SCHED_SKEWED, 1),
-                                            (trmc.SCHED_SKEWED, 2), (trmc.SCHED_SKEWED, 7),
-                                            (trmc.SCHED_SKEWED, 64), (trmc.SCHED_LEVEL, 1),
-                                            (trmc.SCHED_LEVEL, 64)])
-def test_every_timestep_matches_reference(gpu, shreve, schedule, units):
+@pytest.mark.parametrize("schedule,units,variant", [
+    (trmc.SCHED_DATAFLOW, 1, 0), (trmc.SCHED_DATAFLOW, 3, 0), (trmc.SCHED_DATAFLOW, 64, 0),
+    (trmc.SCHED_DATAFLOW, 1, 1), (trmc.SCHED_DATAFLOW, 4, 1),
+    (trmc.SCHED_SKEWED, 1, 0), (trmc.SCHED_SKEWED, 2, 0), (trmc.SCHED_SKEWED, 7, 0),
+    (trmc.SCHED_SKEWED, 64, 0), (trmc.SCHED_LEVEL, 1, 0), (trmc.SCHED_LEVEL, 64, 0)])
+def test_every_timestep_matches_reference(gpu, shreve, schedule, units, variant):
     w, T, ref, cr = shreve
-    _log(f"shreve 40k x {T} schedule={schedule} units={units}")
-    with trmc.Network(w.topo, schedule=schedule, unit_segments=units, record_history=1) as net:
+    _log(f"shreve 40k x {T} schedule={schedule} units={units} variant={variant}")
+    with trmc.Network(w.topo, schedule=schedule, unit_segments=units, record_history=1,
+                      dataflow_variant=variant) as net:
         net.set_params(w.params)
         net.set_state(w.qdp0, w.dp0, w.reach_last0)
         net.set_forcing(w.qlat)
@@ -129,9 +130,11 @@ def test_every_timestep_matches_reference(gpu, shreve, schedule, units):
 def test_results_do_not_depend_on_schedule_bitwise(gpu, shreve):
     w, T, _, _ = shreve
     outs = []
-    for sched, units, block in ((trmc.SCHED_DATAFLOW, 1, 256), (trmc.SCHED_SKEWED, 5, 64),
-                                (trmc.SCHED_LEVEL, 3, 128), (trmc.SCHED_DATAFLOW, 2, 64)):
-        with trmc.Network(w.topo, schedule=sched, unit_segments=units, block_size=block) as net:
+    for sched, units, block, var in ((trmc.SCHED_DATAFLOW, 1, 256, 0), (trmc.SCHED_SKEWED, 5, 64, 0),
+                                     (trmc.SCHED_LEVEL, 3, 128, 0), (trmc.SCHED_DATAFLOW, 2, 64, 1),
+                                     (trmc.SCHED_DATAFLOW, 6, 128, 0)):
+        with trmc.Network(w.topo, schedule=sched, unit_segments=units, block_size=block,
+                          dataflow_variant=var) as net:
             net.set_params(w.params)
             net.set_state(w.qdp0, w.dp0, w.reach_last0)
             net.set_forcing(w.qlat)

@@ -24,6 +24,8 @@ def shim():
     pD = C.POINTER(C.c_double)
     L.shim_solve.restype = C.c_int
     L.shim_solve.argtypes = [C.c_int, pD, pD, C.c_double, C.c_double, C.c_int, pD]
+    L.shim_solve_resumable.restype = C.c_int
+    L.shim_solve_resumable.argtypes = [pD, pD, C.c_double, pD]
     L.shim_pow23_53.restype = None
     L.shim_pow23_53.argtypes = [pD, pD, C.c_int]
     return L
@@ -87,3 +89,16 @@ def test_converged_rule_finds_the_oracle_root(shim):
         assert abs(out[1] - h) <= 1e-9 * max(h, 1e-2), (i, out, (q, h, v))
         evals.append(ev)
     assert np.mean(evals) < 15
+
+
+def test_resumable_reference_solve_is_identical(shim):
+    """The state-machine form the interleaving kernel uses equals the loop form."""
+    z = golden("mc_solve_cases.npz")
+    pD = C.POINTER(C.c_double)
+    for p, x in zip(z["params"], z["inputs"]):
+        a, ia, _ = run(shim, 0, p, x)
+        seg = segc_row(p)
+        xi = np.ascontiguousarray(x, np.float64)
+        b = np.zeros(3)
+        ib = shim.shim_solve_resumable(seg.ctypes.data_as(pD), xi.ctypes.data_as(pD), 300.0, b.ctypes.data_as(pD))
+        assert a.view(np.uint64).tolist() == b.view(np.uint64).tolist() and ia == ib, (p, x)
