@@ -242,6 +242,38 @@ int trmc_network_get_trace(const trmc_network *net, double *us, int *max_iters, 
  * network's member count. */
 int trmc_network_info(const trmc_network *net, int n_timesteps, int64_t *info, int n_info);
 
+/* ------------------------------------------------------ learned guess */
+/* [new] 5->32->32->1 tanh MLP proposing the initial secant depth from
+ * (log Q, Manning n, bottom width, side slope, log S0).  FP32 weights;
+ * trained on the GPU by the library's own forward/backward/Adam kernels on
+ * depth pairs from the FP64 Manning normal-depth solver; evaluated with MFMA
+ * inside the routing kernel.  Output y = log(h_normal / h_wide) with
+ * h_wide = (Q n / (bw sqrt S0))^(3/5). */
+#define TRMC_MLP_NPARAM 1281
+
+/* Generate n training pairs on the device (kept for trmc_mlp_train):
+ * normalised features [n][5], target y [n], and optionally the raw draws
+ * [n][9] = Q, dx, bw, tw, twcc, n_ch, ncc, cs, s0 (host arrays, each may be
+ * NULL). */
+int trmc_mlp_make_dataset(int n, uint64_t seed, float *features, float *target, double *raw);
+
+/* Glorot-uniform initial weights (zero biases) from a seed. */
+void trmc_mlp_init_weights(uint64_t seed, float *weights);
+
+/* Adam training over the last generated set: weights in/out, batch a
+ * multiple of 64, learning rate lr (decayed x0.3 after 60 % and 85 % of the
+ * epochs), loss_curve[epochs] (may be NULL) = mean 0.5 (y - target)^2. */
+int trmc_mlp_train(float *weights, int epochs, int batch, float lr, uint64_t seed, float *loss_curve);
+
+/* Forward and backward on the GPU for n samples (n multiple of 64):
+ * pred [n], grad [TRMC_MLP_NPARAM] = d(mean 0.5 (pred - target)^2)/dW. */
+int trmc_mlp_forward_backward(const float *weights, const float *features, const float *target, int n,
+                              float *pred, float *grad);
+
+/* Inference for n samples with the routing kernel's MFMA wave forward
+ * (use_mfma = 1) or the scalar forward (0). */
+int trmc_mlp_predict(const float *weights, const float *features, int n, float *out, int use_mfma);
+
 /* ------------------------------------------------- formats and generator */
 /* Binary formats of wavefront/gen_network.py:10-37 (little-endian int32 /
  * float32).  Arrays are malloc'd by the library; free with trmc_free. */

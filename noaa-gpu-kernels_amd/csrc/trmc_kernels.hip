@@ -22,8 +22,7 @@ __device__ __forceinline__ SolveOut solve_one(const RouteArgs &a, double qup, do
                                               double ql, const Seg &c, double dp)
 {
     if (MODE == 0) return solve_reference(a.dt, qup, quc, qdp, ql, c, dp);
-    const double d0 = fmax(dp, 0.0);
-    return solve_converged(a.dt, qup, quc, qdp, ql, c, d0 * 0.67, d0 * 1.33 + 0.01, a.tol, a.max_eval);
+    return solve_converged(a.dt, qup, quc, qdp, ql, c, dp, a.tol, a.max_eval);
 }
 
 // One unit, one member, one step.  Upstream reach outflows are summed in the
@@ -624,8 +623,7 @@ __global__ void __launch_bounds__(256) k_solve_batch(const double *__restrict__ 
     if (mode == 0) {
         o = solve_reference(dt, x[0], x[1], x[2], x[3], c, x[4]);
     } else {
-        const double d0 = fmax(x[4], 0.0);
-        o = solve_converged(dt, x[0], x[1], x[2], x[3], c, d0 * 0.67, d0 * 1.33 + 0.01, tol, max_eval);
+        o = solve_converged(dt, x[0], x[1], x[2], x[3], c, x[4], tol, max_eval);
     }
     out[(size_t)i * 3 + 0] = o.q;
     out[(size_t)i * 3 + 1] = o.depth;

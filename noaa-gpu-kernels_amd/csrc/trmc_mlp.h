@@ -1,0 +1,172 @@
+// 5 -> 32 -> 32 -> 1 tanh MLP proposing the initial depth of the converged
+// secant, FP32.
+//
+// Inputs (normalised in mlp_features): log10 Q, Manning n, bottom width,
+// side slope cs, log10 S0.  Output y: the log ratio of the Manning normal
+// depth to the wide-channel estimate h_wide = (Q n / (bw sqrt S0))^(3/5), so
+// the guess is h_wide * exp(y).
+//
+// Weight layout (TRMC_MLP_NPARAM = 1281 floats):
+//   W1[32][5] at 0, b1[32] at 160, W2[32][32] at 192 (W2[o][i]), b2[32] at 1216,
+//   W3[32] at 1248, b3 at 1280.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace trmc {
+
+constexpr int kMlpIn = 5, kMlpHid = 32;
+constexpr int kOffW1 = 0, kOffB1 = 160, kOffW2 = 192, kOffB2 = 1216, kOffW3 = 1248, kOffB3 = 1280;
+constexpr int kMlpParams = 1281;
+
+__host__ __device__ inline void mlp_features(float Q, float n, float bw, float cs, float s0, float *f)
+{
+    f[0] = (log10f(Q) - 0.5f) * 0.5f;
+    f[1] = (n - 0.055f) * 40.0f;
+    f[2] = (bw - 25.5f) * (1.0f / 24.5f);
+    f[3] = cs - 2.0f;
+    f[4] = log10f(s0) + 3.0f;
+}
+
+// wide-channel Manning depth, the scale the network corrects
+__host__ __device__ inline float mlp_wide_depth(float Q, float n, float bw, float s0)
+{
+    return powf(Q * n / (bw * sqrtf(s0)), 0.6f);
+}
+
+__device__ __forceinline__ float fast_tanh(float x)
+{
+    // 1 - 2 / (1 + e^{2x}); saturates correctly for large |x|
+    return 1.0f - __fdividef(2.0f, 1.0f + __expf(2.0f * x));
+}
+
+// Scalar forward for one sample (training kernels); a1, a2 receive the
+// hidden activations.
+__device__ __forceinline__ float mlp_forward_scalar(const float *W, const float *x, float *a1, float *a2)
+{
+#pragma unroll
+    for (int h = 0; h < kMlpHid; ++h) {
+        float z = W[kOffB1 + h];
+#pragma unroll
+        for (int f = 0; f < kMlpIn; ++f) z += W[kOffW1 + h * kMlpIn + f] * x[f];
+        a1[h] = fast_tanh(z);
+    }
+#pragma unroll
+    for (int o = 0; o < kMlpHid; ++o) {
+        float z = W[kOffB2 + o];
+#pragma unroll
+        for (int i = 0; i < kMlpHid; ++i) z += W[kOffW2 + o * kMlpHid + i] * a1[i];
+        a2[o] = fast_tanh(z);
+    }
+    float y = W[kOffB3];
+#pragma unroll
+    for (int h = 0; h < kMlpHid; ++h) y += W[kOffW3 + h] * a2[h];
+    return y;
+}
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// Per-lane weight fragments for the MFMA forward (v_mfma_f32_16x16x4f32).
+// Operand layouts of that instruction for lane l, r = 0..3:
+//   A (16x4): A[l % 16][l / 16];   B (4x16): B[l / 16][l % 16];
+//   D (16x16): D[4 (l / 16) + r][l % 16].
+// The network is evaluated transposed, hidden units as rows and the wave's
+// 64 samples as columns (4 column tiles of 16):
+//   layer 1: H1^T (32x64) = W1 (32x8, features padded to 8) X^T (8x64);
+//            A fragment w1[mi][kk] = W1[16 mi + l%16][4 kk + l/16]
+//   layer 2: H2^T = W2 H1^T, summing over k-steps kappa = (mi, r): lane l's
+//            D register r of tile (mi, nj) is exactly the B operand
+//            H1^T[16 mi + 4 (l/16) + r][16 nj + l%16], so the A fragment is
+//            w2[mo][mi][r] = W2[16 mo + l%16][16 mi + 4 (l/16) + r]
+//   layer 3: y = W3 . H2^T + b3, partial dot per lane, summed over the four
+//            lane groups that hold the same sample column.
+struct MlpFrag {
+    float w1[2][2];
+    float w2[2][2][4];
+    float b1[2][4];     // bias of D rows 16 mi + 4 (l/16) + r
+    float b2[2][4];
+    float w3[2][4];
+    float b3;
+};
+
+__device__ __forceinline__ void mlp_load_frag(const float *W, MlpFrag &F)
+{
+    const int l = threadIdx.x & 63, li = l & 15, lg = l >> 4;
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi) {
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+            const int f = 4 * kk + lg;
+            F.w1[mi][kk] = (f < kMlpIn) ? W[kOffW1 + (16 * mi + li) * kMlpIn + f] : 0.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * mi + 4 * lg + r;
+            F.b1[mi][r] = W[kOffB1 + row];
+            F.b2[mi][r] = W[kOffB2 + row];
+            F.w3[mi][r] = W[kOffW3 + row];
+        }
+    }
+#pragma unroll
+    for (int mo = 0; mo < 2; ++mo)
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                F.w2[mo][mi][r] = W[kOffW2 + (16 * mo + li) * kMlpHid + 16 * mi + 4 * lg + r];
+    F.b3 = W[kOffB3];
+}
+
+// Wave-wide forward: every lane passes its sample's 5 features and gets its
+// own output.  Must be called by all 64 lanes of the wave together.
+__device__ __forceinline__ float mlp_forward_wave(const MlpFrag &F, const float *x)
+{
+    const int l = threadIdx.x & 63, li = l & 15, lg = l >> 4;
+    floatx4 h1[2][4];
+    // ---- layer 1
+#pragma unroll
+    for (int nj = 0; nj < 4; ++nj) {
+        const int src = 16 * nj + li;
+        float xs[8];
+#pragma unroll
+        for (int f = 0; f < kMlpIn; ++f) xs[f] = __shfl(x[f], src, 64);
+        xs[5] = xs[6] = xs[7] = 0.0f;
+        // B operand of k-step kk: feature 4 kk + lg of sample src
+        const float b0 = (lg == 0) ? xs[0] : (lg == 1) ? xs[1] : (lg == 2) ? xs[2] : xs[3];
+        const float b1v = (lg == 0) ? xs[4] : 0.0f;
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi) {
+            floatx4 acc = {F.b1[mi][0], F.b1[mi][1], F.b1[mi][2], F.b1[mi][3]};
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(F.w1[mi][0], b0, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(F.w1[mi][1], b1v, acc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[r] = fast_tanh(acc[r]);
+            h1[mi][nj] = acc;
+        }
+    }
+    // ---- layers 2 and 3
+    float part[4];
+#pragma unroll
+    for (int nj = 0; nj < 4; ++nj) {
+        float p = 0.0f;
+#pragma unroll
+        for (int mo = 0; mo < 2; ++mo) {
+            floatx4 acc = {F.b2[mo][0], F.b2[mo][1], F.b2[mo][2], F.b2[mo][3]};
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(F.w2[mo][mi][r], h1[mi][nj][r], acc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) p += F.w3[mo][r] * fast_tanh(acc[r]);
+        }
+        // sum the four lane groups holding sample column 16 nj + li
+        p += __shfl_xor(p, 16, 64);
+        p += __shfl_xor(p, 32, 64);
+        part[nj] = p;
+    }
+    const float y = (lg == 0) ? part[0] : (lg == 1) ? part[1] : (lg == 2) ? part[2] : part[3];
+    return y + F.b3;
+}
+
+}  // namespace trmc

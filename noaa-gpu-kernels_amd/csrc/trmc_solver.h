@@ -11,8 +11,9 @@
 // CONVERGED rule: the fixed point of the same iteration (h_0 == h, Qj_0 == 0,
 // hence X0 == 0.5) gives the single-valued residual
 //     F(h) = Qmc(h) - Qmn(h)
-// whose smallest root on [mindepth, inf) is found by a bracket-safeguarded
-// secant to a relative depth tolerance.
+// whose leftmost downcrossing on [mindepth, inf) (the smallest h where F goes
+// from > 0 to <= 0; mindepth when F is never positive) is found by a
+// bracket-safeguarded secant to a relative depth tolerance.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -389,94 +390,301 @@ TRMC_HD double residual(double h, double dt, double qup, double quc, double qdp,
     return (routed + C.c4) - manning(g, r23, c);
 }
 
-struct RootState {
-    double lo_pos;   // largest depth seen with F > 0 (below the root), or -1
-    double hi_neg;   // smallest depth seen with F <= 0 (above the root), or +inf
-};
+// Resumable CONVERGED / LEARNED solve.  Every step() is one evaluation of
+// F, so interleaved lanes advance at a uniform cost per step.
+//   root:         the leftmost downcrossing of F on [mindepth, inf) -- the
+//                 smallest h where F passes from > 0 to <= 0 -- or mindepth
+//                 when F is never positive (orc_mc_solve_converged).
+//   start points: CONVERGED -- the reference's bracket (0.67 d, 1.33 d + 0.01);
+//                 LEARNED   -- F at the previous depth d first, then the MLP's
+//                 normal depth for the flow Qmc(d) (need_guess / set_guess).
+//   iteration:    secant on the last two points, kept inside the downcrossing
+//                 bracket [lo_pos, hi_neg] seen so far (bisection when the
+//                 step leaves it, expansion while no F <= 0 lies above a
+//                 positive point, mindepth while every F <= 0).
+//   stop:         |step| <= tol h, F == 0, or bracket width <= tol h.
+//   F(mindepth) <= 0: a geometric scan (x2, visiting bankfull) for a positive
+//                 point; none up to hmax leaves the root at mindepth.
+//   bankfull:     a root above bankfull is kept only if F(bankfull) > 0 or
+//                 F(mindepth) <= 0; otherwise [mindepth, bankfull] is solved.
+//   fallback:     LEARNED solves that have not stopped after `budget`
+//                 evaluations restart from the CONVERGED start points.
+struct SecantConv {
+    double qup, quc, qdp, ql, dt, tol, dp;
+    double ha, fa, hb, fb, lo_pos, hi_neg, hmax;
+    double h_cur, q_est;
+    double root_h, scan_h;
+    Coef C_cur, root_C;
+    int evals, max_eval, budget, stage;
+    bool learned, need_guess, fell_back, capped, trivial, inbank, have_root, scan_bp, neg_seen;
 
-// Bracket-safeguarded secant for the smallest root of F on [mindepth, hmax]
-// starting from (ha, hb).  Returns the last evaluated depth (its coefficients
-// in C) and counts evaluations in *evals.
-TRMC_HD double secant_root(double ha, double hb, double hmax, double dt,
-                                              double qup, double quc, double qdp, double ql,
-                                              const Seg &c, double tol, int max_eval,
-                                              Coef &C, int *evals, bool *capped)
-{
-    const double lo = 0.01;
-    RootState b = {-1.0, hmax};
-    ha = fmin(fmax(ha, lo), hmax);
-    hb = fmin(fmax(hb, lo), hmax);
-    if (hb == ha) hb = (ha < hmax) ? fmin(hmax, ha * 1.01 + 1e-6) : ha * 0.99;
-    Coef Ca;
-    double fa = residual(ha, dt, qup, quc, qdp, ql, c, Ca);
-    int n = 1;
-    if (fa > 0.0) b.lo_pos = ha; else b.hi_neg = fmin(b.hi_neg, ha);
-    double fb = residual(hb, dt, qup, quc, qdp, ql, c, C);
-    ++n;
-    if (fb > 0.0) b.lo_pos = fmax(b.lo_pos, hb); else b.hi_neg = fmin(b.hi_neg, hb);
-    double h = hb;
-    *capped = false;
-    for (;;) {
-        // F(mindepth) <= 0: the root is clamped at mindepth
-        if (b.hi_neg == lo) { h = lo; if (ha != lo && hb != lo) { residual(lo, dt, qup, quc, qdp, ql, c, C); ++n; } break; }
-        double den = fa - fb;
-        double hn = (den != 0.0) ? hb - fb * (ha - hb) / den : 0.5 * (ha + hb);
-        double left = fmax(b.lo_pos, lo);
-        bool bracketed = b.lo_pos > 0.0 && b.hi_neg < hmax;
-        if (!(hn == hn) || hn <= left || hn >= b.hi_neg) {
-            if (bracketed) hn = 0.5 * (left + b.hi_neg);
-            else if (b.lo_pos < 0.0) hn = lo;                 // all F <= 0: try mindepth
-            else hn = fmin(hmax, fmax(hb, ha) * 2.0);         // all F > 0: expand upward
-        }
-        if (n >= max_eval) { *capped = true; break; }
-        double fn = residual(hn, dt, qup, quc, qdp, ql, c, C);
-        ++n;
-        h = hn;
-        if (fn > 0.0) b.lo_pos = fmax(b.lo_pos, hn); else b.hi_neg = fmin(b.hi_neg, hn);
-        double step = fabs(hn - hb);
-        ha = hb; fa = fb; hb = hn; fb = fn;
-        if (fn == 0.0 || step <= tol * hn) break;
-        if (b.lo_pos > 0.0 && b.hi_neg < hmax && (b.hi_neg - b.lo_pos) <= tol * b.hi_neg) break;
+    // stage: 0 eval ha, 1 eval hb, 2 secant, 3 bankfull check, 4 settle at
+    // mindepth, 5 done, 6 scan upward from mindepth, 7 re-evaluate hi_neg
+    TRMC_HD bool start(double dt_, double qup_, double quc_, double qdp_, double ql_, const Seg &c, double depthp,
+                       double tol_, int max_eval_, bool learned_, int budget_)
+    {
+        dt = dt_; qup = qup_; quc = quc_; qdp = qdp_; ql = ql_; tol = tol_;
+        max_eval = max_eval_; budget = budget_;
+        evals = 0;
+        learned = learned_;
+        need_guess = false;
+        fell_back = false;
+        capped = false;
+        inbank = false;
+        have_root = false;
+        trivial = seg_invalid(c) || (ql <= 0.0 && qup <= 0.0 && quc <= 0.0 && qdp <= 0.0);
+        if (trivial) return true;
+        dp = fmax(depthp, 0.0);
+        hmax = 1.0e4;
+        begin_plain_or_learned();
+        return false;
     }
-    *evals = n;
-    return h;
-}
 
-TRMC_HD SolveOut solve_converged(double dt, double qup, double quc, double qdp,
-                                                    double ql, const Seg &c, double ha, double hb,
-                                                    double tol, int max_eval)
-{
-    SolveOut o;
-    o.iters = 0;
-    o.flags = 0;
-    if (seg_invalid(c) || (ql <= 0.0 && qup <= 0.0 && quc <= 0.0 && qdp <= 0.0)) {
-        o.q = 0.0; o.depth = 0.0; o.vel = 0.0;
+    TRMC_HD void begin_plain_or_learned()
+    {
+        lo_pos = -1.0;
+        hi_neg = hmax;
+        neg_seen = false;
+        if (learned) {
+            ha = fmin(fmax(dp, 0.01), hmax);
+            hb = ha;                          // set by set_guess
+        } else {
+            ha = fmin(fmax(dp * 0.67, 0.01), hmax);
+            hb = fmin(fmax(dp * 1.33 + 0.01, 0.01), hmax);
+            if (hb == ha) hb = (ha < hmax) ? fmin(hmax, ha * 1.01 + 1e-6) : ha * 0.99;
+        }
+        stage = 0;
+    }
+
+    TRMC_HD void set_guess(double h)
+    {
+        if (!(h == h) || h <= 0.0) h = ha * 1.5;
+        hb = fmin(fmax(h, 0.01), hmax);
+        if (hb == ha) hb = (ha < hmax) ? fmin(hmax, ha * 1.01 + 1e-6) : ha * 0.99;
+        need_guess = false;
+    }
+
+    // Keeps [lo_pos, hi_neg] around a downcrossing (F(lo_pos) > 0 >= F(hi_neg),
+    // lo_pos < hi_neg; lo_pos < 0 while no positive point lies below hi_neg,
+    // neg_seen once hi_neg is an evaluated point rather than hmax).
+    // F need not be monotone: a positive point above a bracket is ignored, a
+    // non-positive point below lo_pos (an upcrossing lies between) too.  Once
+    // F(mindepth) <= 0 is the lowest point (hi_neg == 0.01), the first
+    // positive point above it opens an upward search.
+    TRMC_HD double eval(double h, const Seg &c)
+    {
+        ++evals;
+        h_cur = h;
+        const double f = residual(h, dt, qup, quc, qdp, ql, c, C_cur);
+        if (f > 0.0) {
+            if (h < hi_neg) lo_pos = fmax(lo_pos, h);
+            else if (lo_pos < 0.0 && hi_neg == 0.01) { lo_pos = h; hi_neg = hmax; neg_seen = false; }
+        } else if (h <= hi_neg && (h > lo_pos || lo_pos < 0.0)) {
+            hi_neg = h;
+            neg_seen = true;
+        }
+        return f;
+    }
+
+    TRMC_HD void accept_root(const Seg &c)
+    {
+        root_h = h_cur;
+        root_C = C_cur;
+        have_root = true;
+        stage = (!inbank && root_h > c.bfd && c.twcc > 0.0 && c.bfd > 0.01) ? 3 : 5;
+    }
+
+    // returns true when the solve is finished (MLP-guess requests leave it running)
+    TRMC_HD bool step(const Seg &c)
+    {
+        if (stage == 5) return true;
+        if (evals >= max_eval) {
+            capped = true;
+            if (!have_root) { root_h = h_cur; root_C = C_cur; have_root = true; }
+            stage = 5;
+            return true;
+        }
+        if (learned && !need_guess && stage <= 2 && evals >= budget) {
+            learned = false;                  // guess did not pay off: reference bracket
+            fell_back = true;
+            begin_plain_or_learned();
+        }
+        switch (stage) {
+        case 0:
+            fa = eval(ha, c);
+            if (learned) {
+                q_est = C_cur.c1 * qup + C_cur.c2 * quc + C_cur.c3 * qdp + C_cur.c4;
+                need_guess = true;
+            }
+            stage = 1;
+            return false;
+        case 1:
+            if (need_guess) return false;     // waiting for the wave's MLP call
+            fb = eval(hb, c);
+            stage = 2;
+            if (fb == 0.0 && lo_pos > 0.0) accept_root(c);
+            return stage == 5;
+        case 2: {
+            if (lo_pos < 0.0 && hi_neg == 0.01) {   // F(mindepth) <= 0, nothing positive above yet
+                scan_h = 0.01;
+                scan_bp = !(c.twcc > 0.0 && c.bfd > 0.01);
+                stage = 6;
+                return false;
+            }
+            const double den = fa - fb;
+            double hn = (den != 0.0) ? hb - fb * (ha - hb) / den : 0.5 * (ha + hb);
+            const double left = fmax(lo_pos, 0.01);
+            const bool bracketed = lo_pos > 0.0 && neg_seen;
+            if (!(hn == hn) || hn <= left || hn >= hi_neg) {
+                if (bracketed) hn = 0.5 * (left + hi_neg);
+                else if (lo_pos < 0.0) hn = 0.01;
+                else hn = fmin(hmax, fmax(ha, hb) * 2.0);
+            }
+            const double fn = eval(hn, c);
+            const double stepsz = fabs(hn - hb);
+            ha = hb; fa = fb; hb = hn; fb = fn;
+            // a converged step from either side of the bracket: points with
+            // F <= 0 only approach from above (hn < hi_neg), so this is a
+            // downcrossing too
+            if ((lo_pos > 0.0 || neg_seen) &&
+                (fn == 0.0 || stepsz <= tol * hn || (lo_pos > 0.0 && neg_seen && (hi_neg - lo_pos) <= tol * hi_neg))) {
+                if (!(h_cur == hi_neg || h_cur == lo_pos)) {
+                    stage = 7;                // last point outside the bracket: take its right end
+                    return false;
+                }
+                accept_root(c);
+            }
+            return stage == 5;
+        }
+        case 3: {                             // a downcrossing below bankfull?
+            const double rh = root_h;
+            const Coef rC = root_C;
+            const double fbf = eval(c.bfd, c);
+            inbank = true;
+            if (fbf > 0.0) {                  // no upcrossing between: keep the root
+                root_h = rh;
+                root_C = rC;
+                stage = 5;
+                return true;
+            }
+            // solve [mindepth, bankfull] unless F(mindepth) <= 0
+            hmax = c.bfd;
+            lo_pos = -1.0;
+            hi_neg = c.bfd;
+            neg_seen = true;
+            ha = 0.01;
+            hb = c.bfd;
+            fb = fbf;
+            fa = eval(ha, c);
+            if (!(fa > 0.0)) {
+                root_h = rh;
+                root_C = rC;
+                stage = 5;
+                return true;
+            }
+            stage = 2;
+            return false;
+        }
+        case 4:
+            eval(0.01, c);
+            root_h = 0.01;
+            root_C = C_cur;
+            have_root = true;
+            stage = 5;
+            return true;
+        case 6: {                             // upward scan for F > 0
+            double hs = scan_h * 2.0;
+            if (!scan_bp && hs >= c.bfd) { hs = c.bfd; scan_bp = true; }
+            if (hs > hmax) { stage = 4; return false; }
+            const double fs = eval(hs, c);
+            // F(scan_h) <= 0; its value only steers a secant step below
+            // lo_pos = hs, which the bracket turns into an expansion
+            ha = scan_h; fa = -1.0;
+            hb = hs; fb = fs;
+            scan_h = hs;
+            if (fs > 0.0) stage = 2;          // lo_pos = hs: expand to a non-positive point
+            return false;
+        }
+        case 7:
+            eval(hi_neg, c);
+            accept_root(c);
+            return stage == 5;
+        }
+        return true;
+    }
+
+    TRMC_HD SolveOut finish(const Seg &c) const
+    {
+        SolveOut o;
+        o.iters = evals;
+        o.flags = (capped ? 1 : 0) | (fell_back ? 2 : 0);
+        if (trivial) { o.q = 0.0; o.depth = 0.0; o.vel = 0.0; return o; }
+        o.depth = root_h;
+        o.vel = velocity(root_h, c);
+        o.q = outflow(root_C, qup, quc, qdp);
         return o;
     }
-    const double hmax = 1.0e4;
-    Coef C;
-    int evals = 0;
-    bool capped = false;
-    double h = secant_root(ha, hb, hmax, dt, qup, quc, qdp, ql, c, tol, max_eval, C, &evals, &capped);
-    // the smallest root lies in the main channel when F(bankfull) <= 0
-    if (h > c.bfd && c.twcc > 0.0 && c.bfd > 0.01) {
-        Coef Cb;
-        double fbf = residual(c.bfd, dt, qup, quc, qdp, ql, c, Cb);
-        ++evals;
-        if (!(fbf > 0.0)) {
-            int e2 = 0;
-            bool cap2 = false;
-            h = secant_root(0.01, c.bfd, c.bfd, dt, qup, quc, qdp, ql, c, tol, max_eval, C, &e2, &cap2);
-            evals += e2;
-            capped = capped || cap2;
+};
+
+// Whole CONVERGED solve from the reference bracket (no MLP).
+TRMC_HD SolveOut solve_converged(double dt, double qup, double quc, double qdp, double ql, const Seg &c,
+                                 double depthp, double tol, int max_eval)
+{
+    SecantConv s;
+    if (!s.start(dt, qup, quc, qdp, ql, c, depthp, tol, max_eval, false, 0))
+        while (!s.step(c)) {
         }
+    return s.finish(c);
+}
+
+// ------------------------------------------------------------- normal depth
+// Leftmost downcrossing of f on [lo, inf): geometric scan (x1.25 from lo,
+// visiting the breakpoint bp) for the first f > 0, then on for the first
+// f <= 0, then bisection to a two-ulp bracket; lo when f is never positive
+// below 1e4 -- the scan the CPU oracle uses (leftmost_downcrossing in mc_oracle.c).
+template <class F>
+TRMC_HD double leftmost_downcrossing(const F &f, double lo, double bp)
+{
+    double x = lo;
+    bool used_bp = !(bp > lo);
+    int k = 0;
+    while (!(f(x) > 0.0)) {
+        if (++k > 400) return lo;
+        double nx = x * 1.25;
+        if (!used_bp && nx >= bp) { nx = bp; used_bp = true; }
+        if (nx > 1.0e4) return lo;
+        x = nx;
     }
-    o.iters = evals;
-    o.flags = capped ? 1 : 0;
-    o.depth = h;
-    o.vel = velocity(h, c);
-    o.q = outflow(C, qup, quc, qdp);
-    return o;
+    double a = x, b = x;
+    for (;;) {
+        if (++k > 800) return b;
+        double nb = b * 1.25;
+        if (!used_bp && nb >= bp) { nb = bp; used_bp = true; }
+        if (!(f(nb) > 0.0)) { b = nb; break; }
+        a = nb;
+        b = nb;
+    }
+    for (int i = 0; i < 200; ++i) {
+        const double m = a + (b - a) / 2.0;
+        if (!(m > a && m < b)) break;
+        if (f(m) > 0.0) a = m; else b = m;
+    }
+    return a + (b - a) / 2.0;
+}
+
+// Manning normal depth: the leftmost downcrossing of Q - Qmn(h) from 1e-6 m
+// (orc_normal_depth).
+TRMC_HD double manning_at(double h, const Seg &c)
+{
+    const Geo g = geometry(h, c);
+    return manning(g, pow23_53(g.R).p23, c);
+}
+
+TRMC_HD double normal_depth(double Q, const Seg &c)
+{
+    if (!(Q > 0.0) || seg_invalid(c)) return 0.0;
+    const double bp = (c.twcc > 0.0) ? c.bfd : -1.0;
+    return leftmost_downcrossing([&](double h) { return Q - manning_at(h, c); }, 1e-6, bp);
 }
 
 }  // namespace trmc

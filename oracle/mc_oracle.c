@@ -393,10 +393,12 @@ void orc_reach_parallel(const orc_chparams_f64 *p, const double *ql,
  *   X(h)   = weighting with lo = 0.25 from the routed flow of the X0 = 0.5
  *            coefficients at the same h.
  *
- * The converged depth h* is the smallest root of F on [mindepth, inf).  The
- * in-bank and overbank branches of the compound section are each scanned, so
- * the root below bankfull is found first when one exists; h* = mindepth when
- * F(mindepth) <= 0.  Outputs then follow wavefront_mc.cu:321-336 at h*.      */
+ * The converged depth h* is the leftmost downcrossing of F on [mindepth, inf):
+ * the smallest h where F passes from > 0 to <= 0 (F may be non-monotone, e.g.
+ * F(mindepth) <= 0 on a sharply rising hydrograph, or several crossings near
+ * bankfull).  h* = mindepth when F is never positive.  The scan visits
+ * bankfull, so an in-bank crossing is found before an overbank one.
+ * Outputs then follow wavefront_mc.cu:321-336 at h*.                        */
 
 static double residual_at(double h, double dt, double qup, double quc, double qdp,
                           double ql, const chan_t *c, mcoef_t *Cout)
@@ -428,27 +430,39 @@ double orc_mc_residual(double dt, double qup, double quc, double qdp, double ql,
     return f;
 }
 
-/* Smallest root of f on [lo, inf): geometric scan (x1.25) that also visits
- * the breakpoint bp (bankfull), then bisection to a 2-ulp bracket.  Returns lo
- * when f(lo) <= 0. */
+/* Leftmost downcrossing of f on [lo, inf): the smallest h where f changes
+ * from > 0 (just below) to <= 0 (at h).  Geometric scan (x1.25 from lo,
+ * visiting the breakpoint bp = bankfull) for the first f > 0, then on for the
+ * first f <= 0, then bisection to a 2-ulp bracket.  Returns lo when f is
+ * never positive on the scan. */
 typedef double (*scalar_fn)(double h, const void *ctx);
 
-static double smallest_root(scalar_fn f, const void *ctx, double lo, double bp)
+static double leftmost_downcrossing(scalar_fn f, const void *ctx, double lo, double bp)
 {
-    double a = lo, fa = f(a, ctx);
-    if (!(fa > 0.0)) return lo;
-    double b = a, fb = fa;
+    double x = lo, fx = f(x, ctx);
     int used_bp = !(bp > lo);
-    for (int k = 0; k < 400; ++k) {
+    int k = 0;
+    /* find the first point with f > 0 */
+    while (!(fx > 0.0)) {
+        if (++k > 400) return lo;
+        double nx = x * 1.25;
+        if (!used_bp && nx >= bp) { nx = bp; used_bp = 1; }
+        if (nx > 1.0e4) return lo;
+        x = nx;
+        fx = f(x, ctx);
+    }
+    /* then the first point after it with f <= 0 */
+    double a = x, b = x;
+    for (;;) {
+        if (++k > 800) return b;
         double nb = b * 1.25;
         if (!used_bp && nb >= bp) { nb = bp; used_bp = 1; }
         double fnb = f(nb, ctx);
-        if (!(fnb > 0.0)) { b = nb; fb = fnb; break; }
-        a = nb; fa = fnb; b = nb;
-        if (k == 399) return b;   /* no sign change found: stays at the last probe */
+        if (!(fnb > 0.0)) { b = nb; break; }
+        a = nb;
+        b = nb;
     }
-    (void)fa; (void)fb;
-    for (int k = 0; k < 200; ++k) {
+    for (int i = 0; i < 200; ++i) {
         double m = a + (b - a) / 2.0;
         if (!(m > a && m < b)) break;
         if (f(m, ctx) > 0.0) a = m; else b = m;
@@ -478,7 +492,7 @@ double orc_mc_solve_converged(double dt, double qup, double quc, double qdp, dou
     }
     route_ctx x = {dt, qup, quc, qdp, ql, &c};
     double bp = (c.twcc > 0.0) ? c.bfd : -1.0;
-    double h = smallest_root(route_residual_fn, &x, 0.01, bp);
+    double h = leftmost_downcrossing(route_residual_fn, &x, 0.01, bp);
     residual_at(h, dt, qup, quc, qdp, ql, &c, &C);
     *velc = velocity(h, &c);
     *depthc = h;
@@ -511,7 +525,7 @@ double orc_normal_depth(double Q, const orc_chparams_f64 *p)
     if (!(Q > 0.0) || chan_invalid(&c)) return 0.0;
     nd_ctx x = {Q, &c};
     double bp = (c.twcc > 0.0) ? c.bfd : -1.0;
-    return smallest_root(normal_residual_fn, &x, 1e-6, bp);
+    return leftmost_downcrossing(normal_residual_fn, &x, 1e-6, bp);
 }
 
 /* ------------------------------------------------------------------------ */

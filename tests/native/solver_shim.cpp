@@ -13,10 +13,8 @@ extern "C" int shim_solve(int mode, const double *segc, const double *in, double
 {
     // segc: the 11 per-segment constants in SegSlot order
     Seg c = load_seg(segc, 1, 0);
-    const double d0 = in[4] > 0.0 ? in[4] : 0.0;
     SolveOut o = mode == 0 ? solve_reference(dt, in[0], in[1], in[2], in[3], c, in[4])
-                           : solve_converged(dt, in[0], in[1], in[2], in[3], c, d0 * 0.67,
-                                             d0 * 1.33 + 0.01, tol, max_eval);
+                           : solve_converged(dt, in[0], in[1], in[2], in[3], c, in[4], tol, max_eval);
     out[0] = o.q;
     out[1] = o.depth;
     out[2] = o.vel;
@@ -46,4 +44,25 @@ extern "C" int shim_solve_resumable(const double *segc, const double *in, double
     out[1] = o.depth;
     out[2] = o.vel;
     return o.iters;
+}
+
+// LEARNED solve with the network's guess replaced by `guess` (the routing
+// kernel supplies it from the MLP when need_guess is raised).
+extern "C" int shim_solve_learned(const double *segc, const double *in, double dt, double tol, int max_eval,
+                                  int budget, double guess, double *out)
+{
+    Seg c = load_seg(segc, 1, 0);
+    SecantConv s;
+    if (!s.start(dt, in[0], in[1], in[2], in[3], c, in[4], tol, max_eval, true, budget)) {
+        for (;;) {
+            const bool done = s.step(c);
+            if (s.need_guess) s.set_guess(guess);
+            if (done) break;
+        }
+    }
+    SolveOut o = s.finish(c);
+    out[0] = o.q;
+    out[1] = o.depth;
+    out[2] = o.vel;
+    return o.iters | (o.flags << 24);
 }

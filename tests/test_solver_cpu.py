@@ -1,7 +1,7 @@
 """The library's segment solver (trmc_solver.h), compiled for the host by
 tests/native, against the oracle on the CPU.  REFERENCE rule: bit-for-bit
 (same libm on both sides).  CONVERGED rule: the safeguarded secant lands on
-the oracle's smallest root (bisection) to 1e-9 in flow."""
+the oracle's root (leftmost downcrossing of F, bisection) to 1e-9 in flow."""
 import ctypes as C
 import os
 import subprocess
@@ -26,6 +26,8 @@ def shim():
     L.shim_solve.argtypes = [C.c_int, pD, pD, C.c_double, C.c_double, C.c_int, pD]
     L.shim_solve_resumable.restype = C.c_int
     L.shim_solve_resumable.argtypes = [pD, pD, C.c_double, pD]
+    L.shim_solve_learned.restype = C.c_int
+    L.shim_solve_learned.argtypes = [pD, pD, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, pD]
     L.shim_pow23_53.restype = None
     L.shim_pow23_53.argtypes = [pD, pD, C.c_int]
     return L
@@ -102,3 +104,34 @@ def test_resumable_reference_solve_is_identical(shim):
         b = np.zeros(3)
         ib = shim.shim_solve_resumable(seg.ctypes.data_as(pD), xi.ctypes.data_as(pD), 300.0, b.ctypes.data_as(pD))
         assert a.view(np.uint64).tolist() == b.view(np.uint64).tolist() and ia == ib, (p, x)
+
+
+def test_learned_start_converges_to_the_same_root(shim):
+    """A guess near the root saves evaluations; a wild guess still ends on the
+    oracle's root (safeguards or the fallback to the reference bracket)."""
+    rng = np.random.default_rng(41)
+    pD = C.POINTER(C.c_double)
+    saved = []
+    for i in range(1500):
+        p = [rng.uniform(500, 5000), rng.uniform(1, 50), 0, 0, rng.uniform(0.03, 0.08), 0,
+             rng.uniform(1, 3), 10 ** rng.uniform(-4, -2)]
+        p[2] = p[1] + rng.uniform(2, 20)
+        p[3] = p[2] + rng.uniform(5, 100)
+        p[5] = p[4] * rng.uniform(1.2, 2)
+        Q = 10 ** rng.uniform(-2, 4)
+        x = [Q * rng.uniform(0.9, 1.1), Q * rng.uniform(0.9, 1.1), Q * rng.uniform(0.9, 1.1), rng.uniform(0, 1), 0.0]
+        q, h, v = orc.solve_converged(p, *x, variant="cr")
+        x[4] = h * rng.uniform(0.9, 1.1)                   # previous depth near the new one
+        seg = segc_row(p)
+        xi = np.ascontiguousarray(x, np.float64)
+        plain, ev_plain, fl = run(shim, 1, p, x)
+        for guess, tag in ((h * (1 + rng.normal(0, 0.01)), "good"), (h * 50.0, "wild"), (-1.0, "invalid")):
+            out = np.zeros(3)
+            r = shim.shim_solve_learned(seg.ctypes.data_as(pD), xi.ctypes.data_as(pD), 300.0, 1e-13, 60, 12,
+                                        guess, out.ctypes.data_as(pD))
+            ev, flags = r & 0xFFFFFF, r >> 24
+            assert not (flags & 1), (i, tag)
+            assert abs(out[0] - q) <= 1e-9 * max(abs(q), 1e-3), (i, tag, out, q)
+            if tag == "good":
+                saved.append(ev_plain - ev)
+    assert np.mean(saved) > 1.0
