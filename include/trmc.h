@@ -90,8 +90,8 @@ typedef struct {
     int    max_iter;        /* CONVERGED/LEARNED iteration cap (default 60) */
     int    learned_budget;  /* LEARNED: evaluations before falling back (default 12) */
     int    grid_blocks;     /* persistent grid size; 0 = fill the device */
-    int    dataflow_variant;/* DATAFLOW: 0 = secant iterations interleaved across lanes
-                               (REFERENCE mode), 1 = one whole item per lane pass */
+    int    dataflow_variant;/* DATAFLOW: 0 = secant iterations interleaved across lanes,
+                               1 = one whole item per lane pass */
     int    device;          /* HIP device ordinal */
     int    trace;           /* SKEWED: record per-super-level time and max iterations */
     int    batch_iters;     /* DATAFLOW interleaved: secant iterations between scheduling
@@ -211,6 +211,10 @@ int trmc_network_set_state_f64(trmc_network *net, const double *qdp0_seg,
 /* Lateral inflow [n_members][n_timesteps][n_segments] (double or float). */
 int trmc_network_set_forcing_f64(trmc_network *net, const double *qlat, int n_timesteps);
 int trmc_network_set_forcing_f32(trmc_network *net, const float *qlat, int n_timesteps);
+
+/* [new] Weights of the LEARNED depth guess (TRMC_MLP_NPARAM floats, layout of
+ * trmc_mlp_train), copied to the device; required before a LEARNED run. */
+int trmc_network_set_mlp(trmc_network *net, const float *weights);
 
 /* Route n_timesteps steps (<= the forcing's) from the current state; the final
  * state becomes the new current state (calls chain: run(T1) then run(T2) ==

@@ -148,6 +148,12 @@ def _declare(L: C.CDLL) -> None:
         "trmc_generate_forcing": (I, [I, I, I, C.c_uint64, pD]),
         "trmc_generate_state": (I, [I, _p(ChParamsF64), D, pD, pD]),
         "trmc_compute_stats": (I, [pD, pD, I, pD]),
+        "trmc_network_set_mlp": (I, [V, pF]),
+        "trmc_mlp_make_dataset": (I, [I, C.c_uint64, pF, pF, pD]),
+        "trmc_mlp_init_weights": (None, [C.c_uint64, pF]),
+        "trmc_mlp_train": (I, [pF, I, I, C.c_float, C.c_uint64, pF]),
+        "trmc_mlp_forward_backward": (I, [pF, pF, pF, I, pF, pF]),
+        "trmc_mlp_predict": (I, [pF, pF, I, pF, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -469,6 +475,54 @@ def mc_solve(params, inputs, dt: float = 300.0, mode: int = MODE_REFERENCE):
     return out, it
 
 
+# ------------------------------------------------------------ learned guess
+MLP_NPARAM = 1281
+
+
+def mlp_make_dataset(n: int, seed: int = 1, raw: bool = False):
+    """Training pairs generated on the GPU (kept there for mlp_train).
+    Returns (features [n, 5] f32, target [n] f32, raw [n, 9] f64 or None)."""
+    X = np.zeros((n, 5), np.float32)
+    Y = np.zeros(n, np.float32)
+    R = np.zeros((n, 9), np.float64) if raw else None
+    _check(lib().trmc_mlp_make_dataset(int(n), int(seed), _ptr(X, C.c_float), _ptr(Y, C.c_float),
+                                       _ptr(R, C.c_double) if raw else C.POINTER(C.c_double)()))
+    return X, Y, R
+
+
+def mlp_init_weights(seed: int = 1) -> np.ndarray:
+    W = np.zeros(MLP_NPARAM, np.float32)
+    lib().trmc_mlp_init_weights(int(seed), _ptr(W, C.c_float))
+    return W
+
+
+def mlp_train(weights, epochs: int = 30, batch: int = 512, lr: float = 3e-3, seed: int = 1):
+    """Adam on the last generated training set.  Returns (weights, loss per epoch)."""
+    W = _f32(np.array(weights, np.float32, copy=True))
+    loss = np.zeros(int(epochs), np.float32)
+    _check(lib().trmc_mlp_train(_ptr(W, C.c_float), int(epochs), int(batch), float(lr), int(seed),
+                                _ptr(loss, C.c_float)))
+    return W, loss
+
+
+def mlp_forward_backward(weights, features, target):
+    W, X, Y = _f32(weights), _f32(features), _f32(target)
+    n = len(Y)
+    pred = np.zeros(n, np.float32)
+    grad = np.zeros(MLP_NPARAM, np.float32)
+    _check(lib().trmc_mlp_forward_backward(_ptr(W, C.c_float), _ptr(X, C.c_float), _ptr(Y, C.c_float), n,
+                                           _ptr(pred, C.c_float), _ptr(grad, C.c_float)))
+    return pred, grad
+
+
+def mlp_predict(weights, features, use_mfma: bool = True) -> np.ndarray:
+    W, X = _f32(weights), _f32(np.asarray(features, np.float32).reshape(-1, 5))
+    out = np.zeros(len(X), np.float32)
+    _check(lib().trmc_mlp_predict(_ptr(W, C.c_float), _ptr(X, C.c_float), len(X), _ptr(out, C.c_float),
+                                  int(bool(use_mfma))))
+    return out
+
+
 class Network:
     """Device-resident network (trmc_network_*): topology, parameters, forcing
     and state stay in HBM across runs."""
@@ -530,6 +584,13 @@ class Network:
         else:
             q = _f64(q)
             _check(lib().trmc_network_set_forcing_f64(self._h, _ptr(q, C.c_double), q.shape[1]))
+
+    def set_mlp(self, weights) -> None:
+        """Weights of the LEARNED depth guess (MLP_NPARAM float32)."""
+        W = _f32(weights)
+        if W.size != MLP_NPARAM:
+            raise TrmcError(ERR_ARG, f"need {MLP_NPARAM} weights")
+        _check(lib().trmc_network_set_mlp(self._h, _ptr(W, C.c_float)))
 
     def run(self, n_timesteps: int, dt: float = 300.0) -> dict:
         st = RunStats()

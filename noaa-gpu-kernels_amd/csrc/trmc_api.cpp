@@ -292,10 +292,9 @@ int trmc_network_create(const int *reach_seg_start, const int *reach_seg_len,
     if (o.unit_segments <= 0) o.unit_segments = 4;   // measured best on the Shreve workloads
     if (o.mode < TRMC_MODE_REFERENCE || o.mode > TRMC_MODE_LEARNED)
         return fail(TRMC_ERR_ARG, "unknown mode %d", o.mode);
-    if (o.mode == TRMC_MODE_LEARNED)
-        return fail(TRMC_ERR_ARG, "LEARNED mode is not available in this build");
     if (o.tol <= 0.0) o.tol = 1e-13;
     if (o.max_iter <= 0) o.max_iter = 60;
+    if (o.learned_budget <= 0) o.learned_budget = 12;
     net->M = o.n_members;
 
     std::string err;
@@ -475,6 +474,18 @@ int trmc_network_set_forcing_f32(trmc_network *net, const float *qlat, int n_tim
     return set_forcing_impl(net, qlat, n_timesteps);
 }
 
+int trmc_network_set_mlp(trmc_network *net, const float *weights)
+{
+    if (!net || !weights) return fail(TRMC_ERR_ARG, "null network or weights");
+    for (int i = 0; i < TRMC_MLP_NPARAM; ++i)
+        if (!std::isfinite(weights[i])) return fail(TRMC_ERR_ARG, "weight %d is not finite", i);
+    HIPCHK(hipSetDevice(net->opt.device));
+    int rc = dev_upload(&net->d_mlp, weights, (size_t)TRMC_MLP_NPARAM, net->stream);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(net->stream));
+    return TRMC_OK;
+}
+
 }  // extern "C"
 
 static RouteArgs make_args(trmc_network *net, int T, double dt)
@@ -555,7 +566,9 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
     HIPCHK(launch_reset((unsigned *)net->d_counters, 2 * CNT_COUNT, s));
     HIPCHK(launch_reset((unsigned *)net->d_status, 1, s));
     RouteArgs a = make_args(net, T, dt);
-    const int mode = o.mode == TRMC_MODE_REFERENCE ? 0 : 1;
+    const int mode = o.mode;
+    if (mode == TRMC_MODE_LEARNED && !net->d_mlp)
+        return fail(TRMC_ERR_STATE, "LEARNED mode needs weights (trmc_network_set_mlp)");
     int sched = o.schedule;
     if (sched == TRMC_SCHED_AUTO)
         sched = P.max_upstream <= kDataflowMaxUp ? TRMC_SCHED_DATAFLOW : TRMC_SCHED_SKEWED;
@@ -569,9 +582,9 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
             if ((rc = dev_alloc(&net->d_flow_hist, need))) return rc;
             net->flow_hist_n = need;
         }
-        const bool il = (mode == 0 && o.dataflow_variant == 0);
+        const bool il = (o.dataflow_variant == 0);
         int bpcu = 0;
-        if (il) HIPCHK(route_dataflow_il_occupancy(o.block_size, &bpcu));
+        if (il) HIPCHK(route_dataflow_il_occupancy(mode, o.block_size, &bpcu));
         else HIPCHK(route_dataflow_occupancy(mode, o.block_size, &bpcu));
         if (bpcu <= 0) return fail(TRMC_ERR_HIP, "dataflow kernel cannot be resident");
         long long resident = (long long)bpcu * net->cu_count;
@@ -585,7 +598,7 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
         a.flow_hist = net->d_flow_hist;
         HIPCHK(launch_hist_init(net->d_flow_hist, net->d_ring, net->ring_shift % P.ring, (long long)um, T, s));
         HIPCHK(hipEventRecord(net->ev0, s));
-        if (il) HIPCHK(launch_route_dataflow_il(a, (int)grid, o.block_size, s));
+        if (il) HIPCHK(launch_route_dataflow_il(a, mode, (int)grid, o.block_size, s));
         else HIPCHK(launch_route_dataflow(a, mode, (int)grid, o.block_size, s));
         HIPCHK(hipEventRecord(net->ev1, s));
         HIPCHK(launch_hist_finish(net->d_flow_hist, net->d_ring, (T + net->ring_shift) % P.ring, (long long)um, T, s));

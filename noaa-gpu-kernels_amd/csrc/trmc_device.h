@@ -79,9 +79,9 @@ hipError_t launch_route_level(const RouteArgs &a, int mode, int level, long long
 hipError_t launch_step_advance(int *t_dev, hipStream_t stream);
 hipError_t launch_route_dataflow(const RouteArgs &a, int mode, int grid, int block, hipStream_t stream);
 hipError_t route_dataflow_occupancy(int mode, int block, int *blocks_per_cu);
-// REFERENCE rule with secant iterations interleaved across lanes
-hipError_t launch_route_dataflow_il(const RouteArgs &a, int grid, int block, hipStream_t stream);
-hipError_t route_dataflow_il_occupancy(int block, int *blocks_per_cu);
+// secant iterations interleaved across lanes (modes 0, 1, 2)
+hipError_t launch_route_dataflow_il(const RouteArgs &a, int mode, int grid, int block, hipStream_t stream);
+hipError_t route_dataflow_il_occupancy(int mode, int block, int *blocks_per_cu);
 // history slot 0 <- ring slot `src_slot`; slots 1..T <- kPending
 hipError_t launch_hist_init(unsigned long long *hist, const double *ring, int src_slot, long long um,
                             int T, hipStream_t stream);

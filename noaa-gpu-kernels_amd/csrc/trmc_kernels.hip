@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include "trmc_device.h"
+#include "trmc_mlp.h"
 #include "trmc_solver.h"
 
 namespace trmc {
@@ -22,7 +23,13 @@ __device__ __forceinline__ SolveOut solve_one(const RouteArgs &a, double qup, do
                                               double ql, const Seg &c, double dp)
 {
     if (MODE == 0) return solve_reference(a.dt, qup, quc, qdp, ql, c, dp);
-    return solve_converged(a.dt, qup, quc, qdp, ql, c, dp, a.tol, a.max_eval);
+    if (MODE == 1) return solve_converged(a.dt, qup, quc, qdp, ql, c, dp, a.tol, a.max_eval);
+    const ConvCfg k = {a.dt, a.tol, a.max_eval, a.learned_budget};
+    SecantConv s;                                // LEARNED: scalar MLP guess
+    if (!s.start(qup, quc, qdp, ql, c, dp, true))
+        while (!s.step(c, k))
+            if (s.need_guess) s.set_guess(mlp_guess_scalar(a.mlp, s.q_cur, c));
+    return s.finish(c);
 }
 
 // One unit, one member, one step.  Upstream reach outflows are summed in the
@@ -358,9 +365,22 @@ __global__ void __launch_bounds__(256) k_route_dataflow(RouteArgs a)
 //       ends stores its segment results (and, at the unit's last segment,
 //       publishes the outflow) at once and idles until the next phase (1).
 // A slow solve therefore holds its own lane only, not the other 63 lanes of
-// its wave.  Each solve is SecantRef, identical to solve_reference.
+// its wave.  REFERENCE solves are SecantRef, identical to solve_reference;
+// CONVERGED and LEARNED solves are SecantConv, one residual evaluation per
+// iteration.  LEARNED: a lane that opens a segment evaluates F at the
+// previous depth there, and one MFMA forward of the wave after the
+// scheduling phase gives every such lane its guess (weights staged in LDS).
+template <int MODE> struct IlSolver { typedef SecantConv type; };
+template <> struct IlSolver<0> { typedef SecantRef type; };
+
+template <int MODE>
 __global__ void __launch_bounds__(256) k_route_dataflow_il(RouteArgs a)
 {
+    __shared__ float sW[MODE == 2 ? kMlpParams : 1];
+    if constexpr (MODE == 2) {
+        for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) sW[i] = a.mlp[i];
+        __syncthreads();
+    }
     const long long t_start = (long long)wall_clock64();
     const int M = a.n_members, T = a.n_steps, U = a.n_units;
     const long long nthreads = (long long)gridDim.x * blockDim.x;
@@ -384,8 +404,20 @@ __global__ void __launch_bounds__(256) k_route_dataflow_il(RouteArgs a)
     double qup_in = 0.0, quc_in = 0.0, qup_next = 0.0, quc_next = 0.0, last = 0.0, cur_qdp = 0.0;
     const double *ql_row = a.qlat;
     Seg c;
-    SecantRef sr;
+    typename IlSolver<MODE>::type sr;
+    const ConvCfg kc = {a.dt, a.tol, a.max_eval, a.learned_budget};
     long long spin_start = 0;
+
+    // true when the segment needs no iteration (guard)
+    auto start_solve = [&](double qup, double quc, double qdp, double ql, double dp) -> bool {
+        if constexpr (MODE == 0) {
+            return sr.start(a.dt, qup, quc, qdp, ql, c, dp);
+        } else {
+            if (sr.start(qup, quc, qdp, ql, c, dp, MODE == 2)) return true;
+            if constexpr (MODE == 2) sr.step(c, kc);   // F at the previous depth: the flow to guess for
+            return false;
+        }
+    };
 
     auto publish = [&]() {
         const size_t slot = (size_t)(t + 1) * UM + (size_t)u * M + m;
@@ -399,6 +431,8 @@ __global__ void __launch_bounds__(256) k_route_dataflow_il(RouteArgs a)
         a.d[sm] = o.depth;
         a.v[sm] = o.vel;
         acc.iters += (unsigned long long)o.iters;
+        acc.capped += (unsigned long long)(o.flags & 1);
+        acc.fallback += (unsigned long long)((o.flags >> 1) & 1);
         qup_next = qdp;
         quc_next = o.q;
         last = o.q;
@@ -419,7 +453,7 @@ __global__ void __launch_bounds__(256) k_route_dataflow_il(RouteArgs a)
             const double quc = (s == s0) ? quc_in : quc_next;
             const double qdp = a.q[sm];
             c = load_seg(a.segc, nd, s);
-            if (!sr.start(a.dt, qup, quc, qdp, ql_row[sm], c, a.d[sm])) {
+            if (!start_solve(qup, quc, qdp, ql_row[sm], a.d[sm])) {
                 cur_qdp = qdp;
                 phase = PH_SOLVE;
             } else {
@@ -501,12 +535,25 @@ __global__ void __launch_bounds__(256) k_route_dataflow_il(RouteArgs a)
                 }
             }
         }
+        // ------------------------------- LEARNED: guesses for the opened solves
+        if constexpr (MODE == 2) {
+            const bool want = phase == PH_SOLVE && sr.need_guess;
+            if (__any(want)) {                       // wave-uniform: all 64 lanes run the forward
+                float x[kMlpIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, hw = 1.0f;
+                if (want) mlp_guess_inputs(sr.q_cur, c, x, hw);
+                const float y = mlp_forward_wave(sW, x);
+                if (want) sr.set_guess((double)(hw * __expf(y)));
+            }
+        }
         // -------------------------------------- (2) solve phase, registers only
         for (int r = 0; r < a.batch_iters; ++r) {
             if (!__any(phase == PH_SOLVE)) break;
             if (phase == PH_SOLVE) {
                 progressed = true;
-                if (sr.step(c)) close_segment(sr.finish(c), cur_qdp);
+                bool fin;
+                if constexpr (MODE == 0) fin = sr.step(c);
+                else fin = sr.step(c, kc);
+                if (fin) close_segment(sr.finish(c), cur_qdp);
             }
         }
         if (!__any(phase != PH_DONE)) break;
@@ -668,15 +715,16 @@ static hipError_t launch_skewed_t(const RouteArgs &a, int grid, int block, bool 
 hipError_t launch_route_skewed(const RouteArgs &a, int mode, int grid, int block, bool cooperative,
                                hipStream_t stream)
 {
-    return mode == 0 ? launch_skewed_t<0>(a, grid, block, cooperative, stream)
-                     : launch_skewed_t<1>(a, grid, block, cooperative, stream);
+    return mode == 0   ? launch_skewed_t<0>(a, grid, block, cooperative, stream)
+           : mode == 1 ? launch_skewed_t<1>(a, grid, block, cooperative, stream)
+                       : launch_skewed_t<2>(a, grid, block, cooperative, stream);
 }
 
 hipError_t route_skewed_occupancy(int mode, int block, int *blocks_per_cu)
 {
-    return mode == 0
-               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_skewed<0>, block, 0)
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_skewed<1>, block, 0);
+    return mode == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_skewed<0>, block, 0)
+           : mode == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_skewed<1>, block, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_skewed<2>, block, 0);
 }
 
 hipError_t launch_route_level(const RouteArgs &a, int mode, int level, long long items, int block,
@@ -686,8 +734,10 @@ hipError_t launch_route_level(const RouteArgs &a, int mode, int level, long long
     const dim3 grid((unsigned)((items + block - 1) / block));
     if (mode == 0)
         hipLaunchKernelGGL(k_route_level<0>, grid, dim3(block), 0, stream, a, level);
-    else
+    else if (mode == 1)
         hipLaunchKernelGGL(k_route_level<1>, grid, dim3(block), 0, stream, a, level);
+    else
+        hipLaunchKernelGGL(k_route_level<2>, grid, dim3(block), 0, stream, a, level);
     return hipGetLastError();
 }
 
@@ -700,29 +750,42 @@ static hipError_t launch_dataflow_t(const RouteArgs &a, int grid, int block, hip
                                       dim3(block), args, 0, stream);
 }
 
-hipError_t launch_route_dataflow_il(const RouteArgs &a, int grid, int block, hipStream_t stream)
+template <int MODE>
+static hipError_t launch_dataflow_il_t(const RouteArgs &a, int grid, int block, hipStream_t stream)
 {
     RouteArgs copy = a;
     void *args[] = {&copy};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_route_dataflow_il), dim3(grid),
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_route_dataflow_il<MODE>), dim3(grid),
                                       dim3(block), args, 0, stream);
 }
 
-hipError_t route_dataflow_il_occupancy(int block, int *blocks_per_cu)
+hipError_t launch_route_dataflow_il(const RouteArgs &a, int mode, int grid, int block, hipStream_t stream)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il, block, 0);
+    return mode == 0   ? launch_dataflow_il_t<0>(a, grid, block, stream)
+           : mode == 1 ? launch_dataflow_il_t<1>(a, grid, block, stream)
+                       : launch_dataflow_il_t<2>(a, grid, block, stream);
+}
+
+hipError_t route_dataflow_il_occupancy(int mode, int block, int *blocks_per_cu)
+{
+    return mode == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<0>, block, 0)
+           : mode == 1
+               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<1>, block, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<2>, block, 0);
 }
 
 hipError_t launch_route_dataflow(const RouteArgs &a, int mode, int grid, int block, hipStream_t stream)
 {
-    return mode == 0 ? launch_dataflow_t<0>(a, grid, block, stream) : launch_dataflow_t<1>(a, grid, block, stream);
+    return mode == 0   ? launch_dataflow_t<0>(a, grid, block, stream)
+           : mode == 1 ? launch_dataflow_t<1>(a, grid, block, stream)
+                       : launch_dataflow_t<2>(a, grid, block, stream);
 }
 
 hipError_t route_dataflow_occupancy(int mode, int block, int *blocks_per_cu)
 {
-    return mode == 0
-               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow<0>, block, 0)
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow<1>, block, 0);
+    return mode == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow<0>, block, 0)
+           : mode == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow<1>, block, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow<2>, block, 0);
 }
 
 hipError_t launch_hist_init(unsigned long long *hist, const double *ring, int src_slot, long long um, int T,

@@ -13,6 +13,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "trmc_solver.h"
+
 namespace trmc {
 
 constexpr int kMlpIn = 5, kMlpHid = 32;
@@ -66,107 +68,89 @@ __device__ __forceinline__ float mlp_forward_scalar(const float *W, const float 
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-// Per-lane weight fragments for the MFMA forward (v_mfma_f32_16x16x4f32).
+// Wave-wide forward with v_mfma_f32_16x16x4f32.  Every lane passes its
+// sample's 5 features and gets its own output; must be called by all 64
+// lanes of the wave together.  W may point to LDS or global memory.
+//
 // Operand layouts of that instruction for lane l, r = 0..3:
 //   A (16x4): A[l % 16][l / 16];   B (4x16): B[l / 16][l % 16];
 //   D (16x16): D[4 (l / 16) + r][l % 16].
 // The network is evaluated transposed, hidden units as rows and the wave's
-// 64 samples as columns (4 column tiles of 16):
-//   layer 1: H1^T (32x64) = W1 (32x8, features padded to 8) X^T (8x64);
-//            A fragment w1[mi][kk] = W1[16 mi + l%16][4 kk + l/16]
-//   layer 2: H2^T = W2 H1^T, summing over k-steps kappa = (mi, r): lane l's
-//            D register r of tile (mi, nj) is exactly the B operand
-//            H1^T[16 mi + 4 (l/16) + r][16 nj + l%16], so the A fragment is
-//            w2[mo][mi][r] = W2[16 mo + l%16][16 mi + 4 (l/16) + r]
+// 64 samples as 4 column tiles nj of 16; one tile at a time:
+//   layer 1: H1^T (32x16) = W1 (32x8, features padded to 8) X^T (8x16):
+//            A = W1[16 mi + l%16][4 kk + l/16], B = feature 4 kk + l/16 of
+//            sample 16 nj + l%16 (shuffled from its lane)
+//   layer 2: H2^T = W2 H1^T, with k-steps kappa = (mi, r): lane l's D
+//            register r of row tile mi is exactly the B operand
+//            H1^T[16 mi + 4 (l/16) + r][l%16], so A = W2[16 mo + l%16][16 mi + 4 (l/16) + r]
 //   layer 3: y = W3 . H2^T + b3, partial dot per lane, summed over the four
 //            lane groups that hold the same sample column.
-struct MlpFrag {
-    float w1[2][2];
-    float w2[2][2][4];
-    float b1[2][4];     // bias of D rows 16 mi + 4 (l/16) + r
-    float b2[2][4];
-    float w3[2][4];
-    float b3;
-};
-
-__device__ __forceinline__ void mlp_load_frag(const float *W, MlpFrag &F)
+__device__ __forceinline__ float mlp_forward_wave(const float *W, const float *x)
 {
     const int l = threadIdx.x & 63, li = l & 15, lg = l >> 4;
-#pragma unroll
-    for (int mi = 0; mi < 2; ++mi) {
-#pragma unroll
-        for (int kk = 0; kk < 2; ++kk) {
-            const int f = 4 * kk + lg;
-            F.w1[mi][kk] = (f < kMlpIn) ? W[kOffW1 + (16 * mi + li) * kMlpIn + f] : 0.0f;
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = 16 * mi + 4 * lg + r;
-            F.b1[mi][r] = W[kOffB1 + row];
-            F.b2[mi][r] = W[kOffB2 + row];
-            F.w3[mi][r] = W[kOffW3 + row];
-        }
-    }
-#pragma unroll
-    for (int mo = 0; mo < 2; ++mo)
-#pragma unroll
-        for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                F.w2[mo][mi][r] = W[kOffW2 + (16 * mo + li) * kMlpHid + 16 * mi + 4 * lg + r];
-    F.b3 = W[kOffB3];
-}
-
-// Wave-wide forward: every lane passes its sample's 5 features and gets its
-// own output.  Must be called by all 64 lanes of the wave together.
-__device__ __forceinline__ float mlp_forward_wave(const MlpFrag &F, const float *x)
-{
-    const int l = threadIdx.x & 63, li = l & 15, lg = l >> 4;
-    floatx4 h1[2][4];
-    // ---- layer 1
+    float mine = 0.0f;
 #pragma unroll
     for (int nj = 0; nj < 4; ++nj) {
         const int src = 16 * nj + li;
-        float xs[8];
+        float xs[kMlpIn];
 #pragma unroll
         for (int f = 0; f < kMlpIn; ++f) xs[f] = __shfl(x[f], src, 64);
-        xs[5] = xs[6] = xs[7] = 0.0f;
-        // B operand of k-step kk: feature 4 kk + lg of sample src
         const float b0 = (lg == 0) ? xs[0] : (lg == 1) ? xs[1] : (lg == 2) ? xs[2] : xs[3];
         const float b1v = (lg == 0) ? xs[4] : 0.0f;
+        floatx4 h1[2];
 #pragma unroll
         for (int mi = 0; mi < 2; ++mi) {
-            floatx4 acc = {F.b1[mi][0], F.b1[mi][1], F.b1[mi][2], F.b1[mi][3]};
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(F.w1[mi][0], b0, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(F.w1[mi][1], b1v, acc, 0, 0, 0);
+            const int row = 16 * mi + 4 * lg;
+            floatx4 acc = {W[kOffB1 + row], W[kOffB1 + row + 1], W[kOffB1 + row + 2], W[kOffB1 + row + 3]};
+            const float a0 = W[kOffW1 + (16 * mi + li) * kMlpIn + lg];
+            const float a1 = (lg == 0) ? W[kOffW1 + (16 * mi + li) * kMlpIn + 4] : 0.0f;
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1v, acc, 0, 0, 0);
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[r] = fast_tanh(acc[r]);
-            h1[mi][nj] = acc;
+            h1[mi] = acc;
         }
-    }
-    // ---- layers 2 and 3
-    float part[4];
-#pragma unroll
-    for (int nj = 0; nj < 4; ++nj) {
         float p = 0.0f;
 #pragma unroll
         for (int mo = 0; mo < 2; ++mo) {
-            floatx4 acc = {F.b2[mo][0], F.b2[mo][1], F.b2[mo][2], F.b2[mo][3]};
+            const int row = 16 * mo + 4 * lg;
+            floatx4 acc = {W[kOffB2 + row], W[kOffB2 + row + 1], W[kOffB2 + row + 2], W[kOffB2 + row + 3]};
 #pragma unroll
             for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(F.w2[mo][mi][r], h1[mi][nj][r], acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                        W[kOffW2 + (16 * mo + li) * kMlpHid + 16 * mi + 4 * lg + r], h1[mi][r], acc, 0, 0, 0);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) p += F.w3[mo][r] * fast_tanh(acc[r]);
+            for (int r = 0; r < 4; ++r) p += W[kOffW3 + row + r] * fast_tanh(acc[r]);
         }
-        // sum the four lane groups holding sample column 16 nj + li
         p += __shfl_xor(p, 16, 64);
         p += __shfl_xor(p, 32, 64);
-        part[nj] = p;
+        if (lg == nj) mine = p;
     }
-    const float y = (lg == 0) ? part[0] : (lg == 1) ? part[1] : (lg == 2) ? part[2] : part[3];
-    return y + F.b3;
+    return mine + W[kOffB3];
+}
+
+// Network inputs for the depth guess of a LEARNED solve waiting in stage 1:
+// the flow q_est = Qmc at the previous depth and the segment's channel
+// parameters; hw receives the wide-channel depth the output scales.
+__device__ __forceinline__ void mlp_guess_inputs(double q_est, const Seg &c, float *x, float &hw)
+{
+    const float Q = (float)fmax(q_est, 1e-6);
+    const float n = (float)c.n_ch, bw = (float)c.bw, s0 = (float)c.s0;
+    const float cs = (float)(1.0 / c.z);
+    mlp_features(Q, n, bw, cs, s0, x);
+    hw = mlp_wide_depth(Q, n, bw, s0);
+}
+
+// One lane's guess with the scalar forward (schedules that solve one
+// segment per lane at a time); W in global memory.
+__device__ __forceinline__ double mlp_guess_scalar(const float *W, double q_est, const Seg &c)
+{
+    float x[kMlpIn], hw, a1[kMlpHid], a2[kMlpHid];
+    mlp_guess_inputs(q_est, c, x, hw);
+    const float y = mlp_forward_scalar(W, x, a1, a2);
+    return (double)(hw * __expf(y));
 }
 
 }  // namespace trmc

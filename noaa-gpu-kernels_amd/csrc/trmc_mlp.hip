@@ -184,12 +184,13 @@ __global__ void k_mlp_reduce_adam(float *W, float *m, float *v, const float *par
 // ------------------------------------------------------------ inference
 __global__ void __launch_bounds__(64) k_mlp_predict_mfma(const float *W, const float *X, long long n, float *out)
 {
-    MlpFrag F;
-    mlp_load_frag(W, F);
+    __shared__ float sW[kMlpParams];
+    for (int c = threadIdx.x; c < kMlpParams; c += 64) sW[c] = W[c];
+    __syncthreads();
     const long long i = (long long)blockIdx.x * 64 + threadIdx.x;
     float x[kMlpIn];
     for (int f = 0; f < kMlpIn; ++f) x[f] = (i < n) ? X[(size_t)i * kMlpIn + f] : 0.0f;
-    const float y = mlp_forward_wave(F, x);       // all 64 lanes participate
+    const float y = mlp_forward_wave(sW, x);      // all 64 lanes participate
     if (i < n) out[i] = y;
 }
 

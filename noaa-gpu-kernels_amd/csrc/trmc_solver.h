@@ -409,22 +409,30 @@ TRMC_HD double residual(double h, double dt, double qup, double quc, double qdp,
 //                 F(mindepth) <= 0; otherwise [mindepth, bankfull] is solved.
 //   fallback:     LEARNED solves that have not stopped after `budget`
 //                 evaluations restart from the CONVERGED start points.
+// Launch-uniform settings of a SecantConv solve (kept out of the per-lane
+// state so they stay in scalar registers).
+struct ConvCfg {
+    double dt, tol;
+    int max_eval, budget;
+};
+
 struct SecantConv {
-    double qup, quc, qdp, ql, dt, tol, dp;
-    double ha, fa, hb, fb, lo_pos, hi_neg, hmax;
-    double h_cur, q_est;
-    double root_h, scan_h;
-    Coef C_cur, root_C;
-    int evals, max_eval, budget, stage;
+    double qup, quc, qdp, ql, dp;
+    double ha, fa, hb, fb, lo_pos, hi_neg;
+    double h_cur, q_cur;     // last evaluated depth and the outflow its coefficients give
+    double root_h, root_q;
+    int evals, stage;
     bool learned, need_guess, fell_back, capped, trivial, inbank, have_root, scan_bp, neg_seen;
+
+    // upper end of the search: bankfull while the in-bank interval is solved
+    TRMC_HD double hmax(const Seg &c) const { return inbank ? c.bfd : 1.0e4; }
 
     // stage: 0 eval ha, 1 eval hb, 2 secant, 3 bankfull check, 4 settle at
     // mindepth, 5 done, 6 scan upward from mindepth, 7 re-evaluate hi_neg
-    TRMC_HD bool start(double dt_, double qup_, double quc_, double qdp_, double ql_, const Seg &c, double depthp,
-                       double tol_, int max_eval_, bool learned_, int budget_)
+    TRMC_HD bool start(double qup_, double quc_, double qdp_, double ql_, const Seg &c, double depthp,
+                       bool learned_)
     {
-        dt = dt_; qup = qup_; quc = quc_; qdp = qdp_; ql = ql_; tol = tol_;
-        max_eval = max_eval_; budget = budget_;
+        qup = qup_; quc = quc_; qdp = qdp_; ql = ql_;
         evals = 0;
         learned = learned_;
         need_guess = false;
@@ -435,32 +443,34 @@ struct SecantConv {
         trivial = seg_invalid(c) || (ql <= 0.0 && qup <= 0.0 && quc <= 0.0 && qdp <= 0.0);
         if (trivial) return true;
         dp = fmax(depthp, 0.0);
-        hmax = 1.0e4;
-        begin_plain_or_learned();
+        begin_plain_or_learned(c);
         return false;
     }
 
-    TRMC_HD void begin_plain_or_learned()
+    TRMC_HD void begin_plain_or_learned(const Seg &c)
     {
+        const double hm = hmax(c);
         lo_pos = -1.0;
-        hi_neg = hmax;
+        hi_neg = hm;
         neg_seen = false;
         if (learned) {
-            ha = fmin(fmax(dp, 0.01), hmax);
+            ha = fmin(fmax(dp, 0.01), hm);
             hb = ha;                          // set by set_guess
         } else {
-            ha = fmin(fmax(dp * 0.67, 0.01), hmax);
-            hb = fmin(fmax(dp * 1.33 + 0.01, 0.01), hmax);
-            if (hb == ha) hb = (ha < hmax) ? fmin(hmax, ha * 1.01 + 1e-6) : ha * 0.99;
+            ha = fmin(fmax(dp * 0.67, 0.01), hm);
+            hb = fmin(fmax(dp * 1.33 + 0.01, 0.01), hm);
+            if (hb == ha) hb = (ha < hm) ? fmin(hm, ha * 1.01 + 1e-6) : ha * 0.99;
         }
         stage = 0;
     }
 
+    // the guess for stage 1 (LEARNED); q_cur is then the flow at the previous depth
     TRMC_HD void set_guess(double h)
     {
         if (!(h == h) || h <= 0.0) h = ha * 1.5;
-        hb = fmin(fmax(h, 0.01), hmax);
-        if (hb == ha) hb = (ha < hmax) ? fmin(hmax, ha * 1.01 + 1e-6) : ha * 0.99;
+        const double hm = 1.0e4;              // a guess is only asked for before any bankfull check
+        hb = fmin(fmax(h, 0.01), hm);
+        if (hb == ha) hb = (ha < hm) ? fmin(hm, ha * 1.01 + 1e-6) : ha * 0.99;
         need_guess = false;
     }
 
@@ -471,14 +481,16 @@ struct SecantConv {
     // non-positive point below lo_pos (an upcrossing lies between) too.  Once
     // F(mindepth) <= 0 is the lowest point (hi_neg == 0.01), the first
     // positive point above it opens an upward search.
-    TRMC_HD double eval(double h, const Seg &c)
+    TRMC_HD double eval(double h, const Seg &c, const ConvCfg &k)
     {
         ++evals;
         h_cur = h;
-        const double f = residual(h, dt, qup, quc, qdp, ql, c, C_cur);
+        Coef C;
+        const double f = residual(h, k.dt, qup, quc, qdp, ql, c, C);
+        q_cur = outflow(C, qup, quc, qdp);
         if (f > 0.0) {
             if (h < hi_neg) lo_pos = fmax(lo_pos, h);
-            else if (lo_pos < 0.0 && hi_neg == 0.01) { lo_pos = h; hi_neg = hmax; neg_seen = false; }
+            else if (lo_pos < 0.0 && hi_neg == 0.01) { lo_pos = h; hi_neg = hmax(c); neg_seen = false; }
         } else if (h <= hi_neg && (h > lo_pos || lo_pos < 0.0)) {
             hi_neg = h;
             neg_seen = true;
@@ -489,48 +501,46 @@ struct SecantConv {
     TRMC_HD void accept_root(const Seg &c)
     {
         root_h = h_cur;
-        root_C = C_cur;
+        root_q = q_cur;
         have_root = true;
         stage = (!inbank && root_h > c.bfd && c.twcc > 0.0 && c.bfd > 0.01) ? 3 : 5;
     }
 
     // returns true when the solve is finished (MLP-guess requests leave it running)
-    TRMC_HD bool step(const Seg &c)
+    TRMC_HD bool step(const Seg &c, const ConvCfg &k)
     {
         if (stage == 5) return true;
-        if (evals >= max_eval) {
+        if (evals >= k.max_eval) {
             capped = true;
-            if (!have_root) { root_h = h_cur; root_C = C_cur; have_root = true; }
+            if (!have_root) { root_h = h_cur; root_q = q_cur; have_root = true; }
             stage = 5;
             return true;
         }
-        if (learned && !need_guess && stage <= 2 && evals >= budget) {
+        if (learned && !need_guess && stage <= 2 && evals >= k.budget) {
             learned = false;                  // guess did not pay off: reference bracket
             fell_back = true;
-            begin_plain_or_learned();
+            begin_plain_or_learned(c);
         }
         switch (stage) {
         case 0:
-            fa = eval(ha, c);
-            if (learned) {
-                q_est = C_cur.c1 * qup + C_cur.c2 * quc + C_cur.c3 * qdp + C_cur.c4;
-                need_guess = true;
-            }
+            fa = eval(ha, c, k);
+            if (learned) need_guess = true;
             stage = 1;
             return false;
         case 1:
             if (need_guess) return false;     // waiting for the wave's MLP call
-            fb = eval(hb, c);
+            fb = eval(hb, c, k);
             stage = 2;
             if (fb == 0.0 && lo_pos > 0.0) accept_root(c);
             return stage == 5;
         case 2: {
             if (lo_pos < 0.0 && hi_neg == 0.01) {   // F(mindepth) <= 0, nothing positive above yet
-                scan_h = 0.01;
+                hb = 0.01;                    // scan position
                 scan_bp = !(c.twcc > 0.0 && c.bfd > 0.01);
                 stage = 6;
                 return false;
             }
+            const double hm = hmax(c);
             const double den = fa - fb;
             double hn = (den != 0.0) ? hb - fb * (ha - hb) / den : 0.5 * (ha + hb);
             const double left = fmax(lo_pos, 0.01);
@@ -538,16 +548,16 @@ struct SecantConv {
             if (!(hn == hn) || hn <= left || hn >= hi_neg) {
                 if (bracketed) hn = 0.5 * (left + hi_neg);
                 else if (lo_pos < 0.0) hn = 0.01;
-                else hn = fmin(hmax, fmax(ha, hb) * 2.0);
+                else hn = fmin(hm, fmax(ha, hb) * 2.0);
             }
-            const double fn = eval(hn, c);
+            const double fn = eval(hn, c, k);
             const double stepsz = fabs(hn - hb);
             ha = hb; fa = fb; hb = hn; fb = fn;
             // a converged step from either side of the bracket: points with
             // F <= 0 only approach from above (hn < hi_neg), so this is a
             // downcrossing too
             if ((lo_pos > 0.0 || neg_seen) &&
-                (fn == 0.0 || stepsz <= tol * hn || (lo_pos > 0.0 && neg_seen && (hi_neg - lo_pos) <= tol * hi_neg))) {
+                (fn == 0.0 || stepsz <= k.tol * hn || (lo_pos > 0.0 && neg_seen && (hi_neg - lo_pos) <= k.tol * hi_neg))) {
                 if (!(h_cur == hi_neg || h_cur == lo_pos)) {
                     stage = 7;                // last point outside the bracket: take its right end
                     return false;
@@ -557,28 +567,26 @@ struct SecantConv {
             return stage == 5;
         }
         case 3: {                             // a downcrossing below bankfull?
-            const double rh = root_h;
-            const Coef rC = root_C;
-            const double fbf = eval(c.bfd, c);
+            const double rh = root_h, rq = root_q;
+            const double fbf = eval(c.bfd, c, k);
             inbank = true;
             if (fbf > 0.0) {                  // no upcrossing between: keep the root
                 root_h = rh;
-                root_C = rC;
+                root_q = rq;
                 stage = 5;
                 return true;
             }
             // solve [mindepth, bankfull] unless F(mindepth) <= 0
-            hmax = c.bfd;
             lo_pos = -1.0;
             hi_neg = c.bfd;
             neg_seen = true;
             ha = 0.01;
             hb = c.bfd;
             fb = fbf;
-            fa = eval(ha, c);
+            fa = eval(ha, c, k);
             if (!(fa > 0.0)) {
                 root_h = rh;
-                root_C = rC;
+                root_q = rq;
                 stage = 5;
                 return true;
             }
@@ -586,27 +594,26 @@ struct SecantConv {
             return false;
         }
         case 4:
-            eval(0.01, c);
+            eval(0.01, c, k);
             root_h = 0.01;
-            root_C = C_cur;
+            root_q = q_cur;
             have_root = true;
             stage = 5;
             return true;
-        case 6: {                             // upward scan for F > 0
-            double hs = scan_h * 2.0;
+        case 6: {                             // upward scan (x2, visiting bankfull) for F > 0
+            double hs = hb * 2.0;
             if (!scan_bp && hs >= c.bfd) { hs = c.bfd; scan_bp = true; }
-            if (hs > hmax) { stage = 4; return false; }
-            const double fs = eval(hs, c);
-            // F(scan_h) <= 0; its value only steers a secant step below
+            if (hs > hmax(c)) { stage = 4; return false; }
+            const double fs = eval(hs, c, k);
+            // F(hb) <= 0; its value only steers a secant step below
             // lo_pos = hs, which the bracket turns into an expansion
-            ha = scan_h; fa = -1.0;
+            ha = hb; fa = -1.0;
             hb = hs; fb = fs;
-            scan_h = hs;
-            if (fs > 0.0) stage = 2;          // lo_pos = hs: expand to a non-positive point
+            if (fs > 0.0) stage = 2;
             return false;
         }
         case 7:
-            eval(hi_neg, c);
+            eval(hi_neg, c, k);
             accept_root(c);
             return stage == 5;
         }
@@ -621,7 +628,7 @@ struct SecantConv {
         if (trivial) { o.q = 0.0; o.depth = 0.0; o.vel = 0.0; return o; }
         o.depth = root_h;
         o.vel = velocity(root_h, c);
-        o.q = outflow(root_C, qup, quc, qdp);
+        o.q = root_q;
         return o;
     }
 };
@@ -630,9 +637,10 @@ struct SecantConv {
 TRMC_HD SolveOut solve_converged(double dt, double qup, double quc, double qdp, double ql, const Seg &c,
                                  double depthp, double tol, int max_eval)
 {
+    const ConvCfg k = {dt, tol, max_eval, 0};
     SecantConv s;
-    if (!s.start(dt, qup, quc, qdp, ql, c, depthp, tol, max_eval, false, 0))
-        while (!s.step(c)) {
+    if (!s.start(qup, quc, qdp, ql, c, depthp, false))
+        while (!s.step(c, k)) {
         }
     return s.finish(c);
 }

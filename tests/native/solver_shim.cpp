@@ -52,10 +52,11 @@ extern "C" int shim_solve_learned(const double *segc, const double *in, double d
                                   int budget, double guess, double *out)
 {
     Seg c = load_seg(segc, 1, 0);
+    const ConvCfg k = {dt, tol, max_eval, budget};
     SecantConv s;
-    if (!s.start(dt, in[0], in[1], in[2], in[3], c, in[4], tol, max_eval, true, budget)) {
+    if (!s.start(in[0], in[1], in[2], in[3], c, in[4], true)) {
         for (;;) {
-            const bool done = s.step(c);
+            const bool done = s.step(c, k);
             if (s.need_guess) s.set_guess(guess);
             if (done) break;
         }

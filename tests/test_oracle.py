@@ -239,15 +239,37 @@ def test_converged_root_is_a_sign_change_of_the_residual():
         assert abs(q - qmc) <= 1e-12 * abs(qmc) or qmc < 0
 
 
-def test_converged_root_is_the_smallest_on_a_dense_scan():
+def _bankfull(p):
+    dx, bw, tw, twcc, n, ncc, cs, s0 = map(float, p)
+    z = 1.0 if cs == 0.0 else 1.0 / cs
+    return bw / 0.00001 if bw > tw else (bw / (2.0 * z) if bw == tw else (tw - bw) / (2.0 * z))
+
+
+def test_converged_root_is_the_leftmost_downcrossing_on_a_dense_scan():
+    """In bank, no point of a dense scan below h* has F > 0 followed by F <= 0,
+    and F is positive just below h*; h* = 1 cm only when F is never positive.
+    (Just above bankfull F can dip in a band narrower than the oracle's x1.25
+    scan; the scan visits bankfull itself and steps over such a band, so the
+    dense check stops at bankfull.)  Half the cases are sharply rising
+    hydrographs, where F(1 cm) <= 0."""
     rng = np.random.default_rng(12)
-    for _ in range(60):
+    rising = 0
+    for i in range(80):
         p = _rand_params(rng)
-        Q = 10 ** rng.uniform(0, 3)
-        q, h, v = orc.solve_converged(p, Q, Q * 1.05, Q * 0.95, 0.2, 1.0)
-        grid = np.geomspace(0.01, h, 400)[:-1]
-        f = np.array([orc.residual(p, Q, Q * 1.05, Q * 0.95, 0.2, x)[0] for x in grid])
-        assert np.all(f > 0), "a smaller root exists below the converged depth"
+        Q = 10 ** rng.uniform(-1, 3)
+        x = (Q, Q * 1.05, Q * 0.95, 0.2) if i % 2 else (Q * 4.0, Q * 6.0, Q * 0.05, 0.05)
+        q, h, v = orc.solve_converged(p, *x, 1.0)
+        top = min(h, _bankfull(p)) if h > 0.01 else 1e3
+        grid = np.geomspace(0.01, top, 600)[:-1]
+        f = np.array([orc.residual(p, *x, g)[0] for g in grid])
+        rising += f[0] <= 0
+        down = (f[:-1] > 0) & (f[1:] <= 0)
+        assert not down.any(), "a downcrossing exists below the converged depth"
+        if h > 0.01:
+            assert orc.residual(p, *x, h * (1 - 1e-9))[0] > 0
+        else:
+            assert np.all(f <= 0)
+    assert rising >= 5, "the rising cases should reach F(1 cm) <= 0"
 
 
 def test_normal_depth_inverts_manning():
