@@ -1,10 +1,18 @@
 """Benchmark: FP64 Muskingum-Cunge routing of the 1,000,000-segment Shreve-style
-river tree over 288 steps (dt = 300 s) on one MI355X.
+river tree over 288 steps (dt = 300 s) on one MI355X, with the learned depth
+guess against the plain secant, both converged to 1e-13 in depth (1e-9 bar).
 
 A "step" of this benchmark is one full routing pass: reset to the initial state
 (device-to-device copies) and route all 288 timesteps of lateral inflow that
 already sit in HBM.  The metric is segment-steps per second
-(1e6 segments x 288 timesteps per pass / seconds per pass).
+(1e6 segments x 288 timesteps per pass / seconds per pass) of the LEARNED mode
+(MLP guess, FP32 on MFMA inside the routing kernel, FP64 secant refinement).
+The same passes are timed for
+  plain       CONVERGED mode: the same FP64 secant from the reference's bracket
+              (0.67 d, 1.33 d + 0.01); speedup_vs_plain = learned / plain
+  reference   the drop-in REFERENCE rule (the reference's 1 % / 1 cm stop)
+The MLP is trained first (not timed) by the library's HIP kernels on pairs
+from its FP64 normal-depth solver.
 
 Prints one JSON line with the driver's fields plus
   roofline      HBM roofline of the routing kernel (112 algorithmic bytes per
@@ -13,9 +21,9 @@ Prints one JSON line with the driver's fields plus
   cpu_baseline  the reference's own FP64 host routine cpu_wavefront_fp64
                 (oracle/_ref, built from the reference sources) timed on one
                 host core over the first 24 steps of the same workload
-  parity        GPU vs that CPU run over those 24 steps: max relative error and
-                the fraction within 1e-12 (the GPU uses a correctly rounded pow,
-                the reference the host libm's; see tests/test_gpu_parity.py)
+  parity        the reference-rule GPU pass vs that CPU run over those 24
+                steps, and the learned pass's first step re-solved per segment
+                by the FP64 oracle (converged rule) on a sample of segments
 """
 import argparse
 import importlib.util
@@ -85,6 +93,99 @@ def cpu_baseline(w, T_sample, dt):
     return rec, (qo, ro)
 
 
+MODES = {"learned": 2, "plain": 1, "reference": 0}
+
+
+def time_mode(trmc, w, T, dt, mode, args, weights=None):
+    """Warm-up then timed passes of one mode; returns (record, network)."""
+    M = w.qlat.shape[0]
+    net = trmc.Network(w.topo, mode=mode, schedule=args.schedule, unit_segments=args.unit_segments,
+                       block_size=args.block, grid_blocks=args.grid, n_members=M,
+                       batch_iters=args.batch_iters)
+    net.set_params(w.params)
+    net.set_state(w.qdp0, w.dp0, w.reach_last0)
+    net.set_forcing(w.qlat)
+    if weights is not None:
+        net.set_mlp(weights)
+    for _ in range(args.warmup):
+        net.reset()
+        net.run(T, dt)
+    trmc.synchronize()
+    kernel_ms, iters, fallbacks, capped, stats = [], 0, 0, 0, {}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        net.reset()
+        stats = net.run(T, dt)
+        kernel_ms.append(stats["kernel_ms"])
+        iters += stats["secant_iters"]
+        fallbacks += stats["fallbacks"]
+        capped += stats["not_converged"]
+    trmc.synchronize()
+    secs = time.perf_counter() - t0
+    seg_steps = w.topo.n_segments * T * M
+    rec = {"value": seg_steps * args.steps / secs, "ms_per_step": 1e3 * secs / args.steps,
+           "kernel_ms": float(np.mean(kernel_ms)),
+           "evaluations_per_seg_step": iters / (seg_steps * args.steps),
+           "fallbacks": fallbacks // args.steps, "not_converged": capped // args.steps,
+           "grid_blocks": stats.get("grid_blocks")}
+    log(f"[bench] mode {mode}: {rec['ms_per_step']:.2f} ms/pass, kernel {rec['kernel_ms']:.2f} ms, "
+        f"{rec['value'] / 1e9:.3f} G seg-steps/s, evaluations/seg-step {rec['evaluations_per_seg_step']:.3f}, "
+        f"fallbacks {rec['fallbacks']}, capped {rec['not_converged']}")
+    return rec, net
+
+
+def train_mlp(trmc, args):
+    t0 = time.perf_counter()
+    trmc.mlp_make_dataset(args.train_pairs, seed=7)
+    t1 = time.perf_counter()
+    W, loss = trmc.mlp_train(trmc.mlp_init_weights(3), epochs=args.train_epochs, batch=args.train_batch,
+                             lr=args.train_lr)
+    t2 = time.perf_counter()
+    Xh, Yh, _ = trmc.mlp_make_dataset(1 << 16, seed=99)
+    rmse = float(np.sqrt(np.mean((trmc.mlp_predict(W, Xh) - Yh) ** 2)))
+    rec = {"pairs": args.train_pairs, "epochs": args.train_epochs, "batch": args.train_batch,
+           "dataset_s": t1 - t0, "train_s": t2 - t1, "loss_first": float(loss[0]), "loss_last": float(loss[-1]),
+           "heldout_rmse_log_depth": rmse}
+    log(f"[bench] mlp: {args.train_pairs} pairs in {t1 - t0:.2f} s, {args.train_epochs} epochs in "
+        f"{t2 - t1:.2f} s, loss {loss[0]:.4g} -> {loss[-1]:.4g}, held-out rmse {rmse:.4f}")
+    return W, rec
+
+
+def oracle_sample_check(orc, w, net, dt, n_sample):
+    """Re-solve a sample of segments of the first step with the FP64 oracle
+    (converged rule) from the GPU's own inputs; returns the max relative flow
+    and depth errors."""
+    topo = w.topo
+    net.reset()
+    net.run(1, dt)
+    q1, d1, _, rl1 = net.state()
+    q1, d1, rl1 = q1[0], d1[0], rl1[0]
+    q0, d0, rl0 = w.qdp0, w.dp0, w.reach_last0
+    starts, lens = np.asarray(topo.reach_seg_start), np.asarray(topo.reach_seg_len)
+    offs = np.arange(int(lens.sum())) - np.repeat(np.cumsum(lens) - lens, lens)
+    seg_reach = np.full(topo.n_segments, -1)
+    seg_reach[np.repeat(starts, lens) + offs] = np.repeat(np.arange(topo.n_reaches), lens)
+    first = np.zeros(topo.n_segments, bool)
+    first[starts[lens > 0]] = True
+    rng = np.random.default_rng(5)
+    covered = np.nonzero(seg_reach >= 0)[0]
+    idx = np.sort(rng.choice(covered, min(n_sample, len(covered)), replace=False))
+    worst_q = worst_d = 0.0
+    for s in idx:
+        if first[s]:
+            r = seg_reach[s]
+            ups = topo.reach_up_idx[topo.reach_up_ptr[r]:topo.reach_up_ptr[r + 1]]
+            qup = float(np.sum(rl0[ups])) if len(ups) else 0.0
+            quc = float(np.sum(rl1[ups])) if len(ups) else 0.0
+        else:
+            qup, quc = float(q0[s - 1]), float(q1[s - 1])
+        q, h, _ = orc.solve_converged(w.params[s], qup, quc, float(q0[s]), float(w.qlat[0][0][s]),
+                                      float(d0[s]), dt, variant="cr")
+        worst_q = max(worst_q, abs(q1[s] - q) / max(abs(q), 1e-300) if q != 0 else abs(q1[s]))
+        worst_d = max(worst_d, abs(d1[s] - h) / max(abs(h), 1e-300) if h != 0 else abs(d1[s]))
+    return worst_q, worst_d, len(idx)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +199,15 @@ def main():
     ap.add_argument("--schedule", type=int, default=0, help="0 auto, 1 skewed, 2 level, 3 dataflow")
     ap.add_argument("--cpu-steps", type=int, default=24, help="timesteps in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", default="learned", choices=sorted(MODES),
+                    help="mode of the headline value; the other two are timed beside it")
+    ap.add_argument("--only", action="store_true", help="time the headline mode only")
+    ap.add_argument("--train-pairs", type=int, default=2_000_000)
+    ap.add_argument("--train-epochs", type=int, default=8)
+    ap.add_argument("--train-batch", type=int, default=1024)
+    ap.add_argument("--train-lr", type=float, default=3e-3)
+    ap.add_argument("--oracle-sample", type=int, default=20000,
+                    help="segments of the first learned step re-solved by the FP64 oracle")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build routes on one GPU only; run with --gpus 1"}))
@@ -111,33 +221,30 @@ def main():
     log(f"[bench] workload {args.workload}: {w.topo.n_segments} segments, {w.topo.n_reaches} reaches, "
         f"{w.topo.n_levels} levels, {T} steps (generated in {time.perf_counter() - t0:.1f} s)")
 
-    net = trmc.Network(w.topo, schedule=args.schedule, unit_segments=args.unit_segments,
-                       block_size=args.block, grid_blocks=args.grid, n_members=M,
-                       batch_iters=args.batch_iters)
-    net.set_params(w.params)
-    net.set_state(w.qdp0, w.dp0, w.reach_last0)
-    net.set_forcing(w.qlat)
-    info = net.info(T)
-    for _ in range(args.warmup):
-        net.reset()
-        net.run(T, dt)
-    trmc.synchronize()
-    kernel_ms, iters, stats = [], 0, {}
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        net.reset()
-        stats = net.run(T, dt)
-        kernel_ms.append(stats["kernel_ms"])
-        iters += stats["secant_iters"]
-    trmc.synchronize()
-    secs = time.perf_counter() - t0
-    ms_per_step = 1e3 * secs / args.steps
     seg_steps = w.topo.n_segments * T * M
-    value = seg_steps * args.steps / secs
-    k_avg_s = float(np.mean(kernel_ms)) / 1e3
+    weights, mlp_rec = None, None
+    order = [args.mode] + ([] if args.only else [m for m in ("learned", "plain", "reference") if m != args.mode])
+    if "learned" in order:
+        weights, mlp_rec = train_mlp(trmc, args)
+    recs, nets, finals = {}, {}, {}
+    for name in order:
+        recs[name], nets[name] = time_mode(trmc, w, T, dt, MODES[name], args,
+                                           weights if name == "learned" else None)
+        finals[name] = nets[name].state()
+        if name not in ("learned", "reference"):
+            nets.pop(name).close()
+    head = recs[args.mode]
+    info = nets[args.mode].info(T)
+    value, ms_per_step = head["value"], head["ms_per_step"]
+    k_avg_s = head["kernel_ms"] / 1e3
     achieved = BYTES_PER_SEG_STEP * seg_steps / k_avg_s / 1e9
-    log(f"[bench] {ms_per_step:.2f} ms/pass, kernel {1e3 * k_avg_s:.2f} ms, {value / 1e9:.3f} G seg-steps/s, "
-        f"iters/seg-step {iters / (seg_steps * args.steps):.3f}, barrier wait {stats['barrier_wait_frac']:.3f}")
+    if "learned" in finals and "plain" in finals:
+        # both converge to 1e-13 in depth: their final states after 288 steps agree to the 1e-9 bar
+        a_, b_ = finals["learned"], finals["plain"]
+        rel = lambda x, y: float(np.max(np.abs(x - y) / np.maximum(np.abs(y), 1e-3)))
+        recs["learned"]["max_rel_vs_plain"] = {"qdc": rel(a_[0], b_[0]), "depth": rel(a_[1], b_[1]),
+                                               "reach_last": rel(a_[3], b_[3])}
+        log(f"[bench] learned vs plain after {T} steps: {recs['learned']['max_rel_vs_plain']}")
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
@@ -162,11 +269,14 @@ def main():
             "workload": f"{args.workload}: FP64 MC routing, {w.topo.n_segments} segments, "
                         f"{w.topo.n_reaches} reaches, {w.topo.n_levels} reach levels, {T} steps, dt=300 s",
             "segments": w.topo.n_segments, "reaches": w.topo.n_reaches, "reach_levels": w.topo.n_levels,
-            "timesteps": T, "members": M, "seed": seed, "stopping_rule": "reference (1% / 1 cm)",
+            "timesteps": T, "members": M, "seed": seed,
+            "mode": {"learned": "LEARNED: MLP depth guess (FP32, MFMA, in-kernel) + FP64 secant to 1e-13",
+                     "plain": "CONVERGED: FP64 secant from the reference bracket to 1e-13",
+                     "reference": "REFERENCE rule (1% / 1 cm stop), drop-in"}[args.mode],
             "schedule": {0: "auto (dataflow)", 1: "time-skewed persistent, device barrier",
                          2: "level launches, hipGraph per step", 3: "dataflow persistent"}[args.schedule],
             "unit_segments": args.unit_segments, "unit_levels": info["n_levels"],
-            "super_levels": info["super_levels"], "grid_blocks": stats.get("grid_blocks"),
+            "super_levels": info["super_levels"], "grid_blocks": head.get("grid_blocks"),
             "block": args.block, "batch_iters": args.batch_iters, "parallelism": "1 GPU",
         },
         "roofline": {
@@ -174,17 +284,29 @@ def main():
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
         },
         "kernel_ms": 1e3 * k_avg_s,
-        "secant_iters_per_seg_step": iters / (seg_steps * args.steps),
-        "barrier_wait_frac": stats.get("barrier_wait_frac"),
+        "evaluations_per_seg_step": head["evaluations_per_seg_step"],
+        "modes": recs,
     }
+    if "learned" in recs and "plain" in recs:
+        out["speedup_vs_plain"] = recs["learned"]["value"] / recs["plain"]["value"]
+        log(f"[bench] learned vs plain secant: {out['speedup_vs_plain']:.3f}x")
+    if mlp_rec is not None:
+        out["mlp"] = mlp_rec
     if not args.no_cpu:
         Tc = min(args.cpu_steps, T)
         log(f"[bench] cpu baseline: {Tc} steps on one core")
         rec, (qo, ro) = cpu_baseline(w, Tc, dt)
         out["cpu_baseline"] = rec
-        net.reset()
-        net.run(Tc, dt)
-        q, _, _, rl = net.state()
+        rnet = nets.get("reference")
+        if rnet is None:
+            rnet = trmc.Network(w.topo, unit_segments=args.unit_segments, block_size=args.block)
+            rnet.set_params(w.params)
+            rnet.set_state(w.qdp0, w.dp0, w.reach_last0)
+            rnet.set_forcing(w.qlat)
+            nets["reference"] = rnet
+        rnet.reset()
+        rnet.run(Tc, dt)
+        q, _, _, rl = rnet.state()
         rel_q = np.abs(q[0] - qo) / np.maximum(np.abs(qo), 1e-300)
         rel_r = np.abs(rl[0] - ro) / np.maximum(np.abs(ro), 1e-300)
         out["parity"] = {"steps": Tc, "against": rec["kind"],
@@ -193,7 +315,16 @@ def main():
                          "bitwise_equal_frac": float(np.mean(q[0].view(np.uint64) == qo.view(np.uint64)))}
         out["speedup_vs_cpu"] = value / rec["value"]
         log(f"[bench] cpu {rec['value'] / 1e6:.2f} M seg-steps/s; parity {out['parity']}")
-    net.close()
+        if "learned" in nets and args.oracle_sample > 0:
+            orc = _load("trmc_oracle", os.path.join(ROOT, "oracle", "oracle.py"))
+            t0 = time.perf_counter()
+            eq, ed, n = oracle_sample_check(orc, w, nets["learned"], dt, args.oracle_sample)
+            out["parity"]["learned_step1_oracle_sample"] = {
+                "segments": n, "max_rel_qdc": eq, "max_rel_depth": ed, "bar": 1e-9}
+            log(f"[bench] learned step 1 vs FP64 oracle on {n} segments: max rel q {eq:.3e}, "
+                f"depth {ed:.3e} ({time.perf_counter() - t0:.1f} s)")
+    for n_ in nets.values():
+        n_.close()
     print(json.dumps(out), flush=True)
 
 

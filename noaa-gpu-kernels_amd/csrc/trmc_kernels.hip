@@ -28,7 +28,8 @@ __device__ __forceinline__ SolveOut solve_one(const RouteArgs &a, double qup, do
     SecantConv s;                                // LEARNED: scalar MLP guess
     if (!s.start(qup, quc, qdp, ql, c, dp, true))
         while (!s.step(c, k))
-            if (s.need_guess) s.set_guess(mlp_guess_scalar(a.mlp, s.q_cur, c));
+            if (s.need_guess)
+                s.set_guess(mlp_guess_scalar(a.mlp, s.guess_anchor(), s.guess_flow_new(), s.guess_flow_old(), c));
     return s.finish(c);
 }
 
@@ -368,8 +369,9 @@ __global__ void __launch_bounds__(256) k_route_dataflow(RouteArgs a)
 // its wave.  REFERENCE solves are SecantRef, identical to solve_reference;
 // CONVERGED and LEARNED solves are SecantConv, one residual evaluation per
 // iteration.  LEARNED: a lane that opens a segment evaluates F at the
-// previous depth there, and one MFMA forward of the wave after the
-// scheduling phase gives every such lane its guess (weights staged in LDS).
+// previous depth there, and after the scheduling phase two MFMA forwards of
+// the wave (the routed flow and the anchor's Manning flow) give every such
+// lane its guess (weights staged in LDS).
 template <int MODE> struct IlSolver { typedef SecantConv type; };
 template <> struct IlSolver<0> { typedef SecantRef type; };
 
@@ -539,10 +541,15 @@ __global__ void __launch_bounds__(256) k_route_dataflow_il(RouteArgs a)
         if constexpr (MODE == 2) {
             const bool want = phase == PH_SOLVE && sr.need_guess;
             if (__any(want)) {                       // wave-uniform: all 64 lanes run the forward
-                float x[kMlpIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, hw = 1.0f;
-                if (want) mlp_guess_inputs(sr.q_cur, c, x, hw);
-                const float y = mlp_forward_wave(sW, x);
-                if (want) sr.set_guess((double)(hw * __expf(y)));
+                float xn[kMlpIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, xo[kMlpIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                if (want) {
+                    mlp_segment_inputs(sr.guess_flow_new(), c, xn);
+                    mlp_segment_inputs(sr.guess_flow_old(), c, xo);
+                }
+                const float yn = mlp_forward_wave(sW, xn);
+                const float yo = mlp_forward_wave(sW, xo);
+                if (want)
+                    sr.set_guess(mlp_guess_from(sr.guess_anchor(), sr.guess_flow_new(), sr.guess_flow_old(), yn, yo));
             }
         }
         // -------------------------------------- (2) solve phase, registers only

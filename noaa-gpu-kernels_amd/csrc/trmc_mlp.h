@@ -131,26 +131,35 @@ __device__ __forceinline__ float mlp_forward_wave(const float *W, const float *x
     return mine + W[kOffB3];
 }
 
-// Network inputs for the depth guess of a LEARNED solve waiting in stage 1:
-// the flow q_est = Qmc at the previous depth and the segment's channel
-// parameters; hw receives the wide-channel depth the output scales.
-__device__ __forceinline__ void mlp_guess_inputs(double q_est, const Seg &c, float *x, float &hw)
+// Network inputs for flow Q at a segment (normalised features of
+// mlp_features; cs = 1 / z).
+__device__ __forceinline__ void mlp_segment_inputs(double Q, const Seg &c, float *x)
 {
-    const float Q = (float)fmax(q_est, 1e-6);
-    const float n = (float)c.n_ch, bw = (float)c.bw, s0 = (float)c.s0;
-    const float cs = (float)(1.0 / c.z);
-    mlp_features(Q, n, bw, cs, s0, x);
-    hw = mlp_wide_depth(Q, n, bw, s0);
+    mlp_features((float)fmax(Q, 1e-6), (float)c.n_ch, (float)c.bw, (float)(1.0 / c.z), (float)c.s0, x);
+}
+
+// The LEARNED guess: from the anchor depth d, whose Manning flow is q_old,
+// along the network's rating curve h(Q) = h_wide(Q) e^{y(Q)} to the routed
+// flow q_new:  h = d (q_new / q_old)^(3/5) e^{y_new - y_old}.  The ratio
+// cancels the per-segment bias of h(Q) (the floodplain geometry is not among
+// the network's inputs).
+__device__ __forceinline__ double mlp_guess_from(double d, double q_new, double q_old, float y_new, float y_old)
+{
+    const float r = (float)(fmax(q_new, 1e-6) / fmax(q_old, 1e-6));
+    return d * (double)__expf(0.6f * __logf(r) + (y_new - y_old));
 }
 
 // One lane's guess with the scalar forward (schedules that solve one
 // segment per lane at a time); W in global memory.
-__device__ __forceinline__ double mlp_guess_scalar(const float *W, double q_est, const Seg &c)
+__device__ __forceinline__ double mlp_guess_scalar(const float *W, double d, double q_new, double q_old,
+                                                   const Seg &c)
 {
-    float x[kMlpIn], hw, a1[kMlpHid], a2[kMlpHid];
-    mlp_guess_inputs(q_est, c, x, hw);
-    const float y = mlp_forward_scalar(W, x, a1, a2);
-    return (double)(hw * __expf(y));
+    float xn[kMlpIn], xo[kMlpIn], a1[kMlpHid], a2[kMlpHid];
+    mlp_segment_inputs(q_new, c, xn);
+    mlp_segment_inputs(q_old, c, xo);
+    const float yn = mlp_forward_scalar(W, xn, a1, a2);
+    const float yo = mlp_forward_scalar(W, xo, a1, a2);
+    return mlp_guess_from(d, q_new, q_old, yn, yo);
 }
 
 }  // namespace trmc

@@ -372,9 +372,9 @@ struct SecantRef {
 
 // ------------------------------------------------------------- CONVERGED
 // F(h) with the X0 = 0.5 coefficients feeding the weighting X(h); also returns
-// the final coefficients at h for the outflow rule.
+// the final coefficients at h for the outflow rule and the Manning flow Qmn(h).
 TRMC_HD double residual(double h, double dt, double qup, double quc, double qdp,
-                                           double ql, const Seg &c, Coef &C)
+                                           double ql, const Seg &c, Coef &C, double &qmn)
 {
     Geo g = geometry(h, c);
     const Pow2353 w = pow23_53(g.R);
@@ -387,7 +387,8 @@ TRMC_HD double residual(double h, double dt, double qup, double quc, double qdp,
     C = muskingum(km, X, dt, ql);
     double routed = C.c1 * qup + C.c2 * quc + C.c3 * qdp;
     if (C.c4 < 0.0 && fabs(C.c4) > routed) C.c4 = -routed;
-    return (routed + C.c4) - manning(g, r23, c);
+    qmn = manning(g, r23, c);
+    return (routed + C.c4) - qmn;
 }
 
 // Resumable CONVERGED / LEARNED solve.  Every step() is one evaluation of
@@ -396,8 +397,9 @@ TRMC_HD double residual(double h, double dt, double qup, double quc, double qdp,
 //                 smallest h where F passes from > 0 to <= 0 -- or mindepth
 //                 when F is never positive (orc_mc_solve_converged).
 //   start points: CONVERGED -- the reference's bracket (0.67 d, 1.33 d + 0.01);
-//                 LEARNED   -- F at the previous depth d first, then the MLP's
-//                 normal depth for the flow Qmc(d) (need_guess / set_guess).
+//                 LEARNED   -- F at the previous depth d first; then a guess
+//                 along the MLP's rating curve from d (the Manning flow
+//                 Qmn(d)) to the flow Qmc(d) (need_guess / guess_* / set_guess).
 //   iteration:    secant on the last two points, kept inside the downcrossing
 //                 bracket [lo_pos, hi_neg] seen so far (bisection when the
 //                 step leaves it, expansion while no F <= 0 lies above a
@@ -420,6 +422,7 @@ struct SecantConv {
     double qup, quc, qdp, ql, dp;
     double ha, fa, hb, fb, lo_pos, hi_neg;
     double h_cur, q_cur;     // last evaluated depth and the outflow its coefficients give
+    double qmn_cur;          // Manning flow Qmn(h_cur)
     double root_h, root_q;
     int evals, stage;
     bool learned, need_guess, fell_back, capped, trivial, inbank, have_root, scan_bp, neg_seen;
@@ -464,7 +467,14 @@ struct SecantConv {
         stage = 0;
     }
 
-    // the guess for stage 1 (LEARNED); q_cur is then the flow at the previous depth
+    // What a LEARNED guess is made from, valid while need_guess: the anchor
+    // depth d (evaluated first), the flow the routing gives there (Qmc(d),
+    // the outflow rule) and the Manning flow Qmn(d) of depth d.
+    TRMC_HD double guess_anchor() const { return ha; }
+    TRMC_HD double guess_flow_new() const { return q_cur; }
+    TRMC_HD double guess_flow_old() const { return qmn_cur; }
+
+    // the guess for stage 1 (LEARNED)
     TRMC_HD void set_guess(double h)
     {
         if (!(h == h) || h <= 0.0) h = ha * 1.5;
@@ -486,8 +496,10 @@ struct SecantConv {
         ++evals;
         h_cur = h;
         Coef C;
-        const double f = residual(h, k.dt, qup, quc, qdp, ql, c, C);
+        double qmn;
+        const double f = residual(h, k.dt, qup, quc, qdp, ql, c, C, qmn);
         q_cur = outflow(C, qup, quc, qdp);
+        qmn_cur = qmn;
         if (f > 0.0) {
             if (h < hi_neg) lo_pos = fmax(lo_pos, h);
             else if (lo_pos < 0.0 && hi_neg == 0.01) { lo_pos = h; hi_neg = hmax(c); neg_seen = false; }
@@ -543,6 +555,14 @@ struct SecantConv {
             const double hm = hmax(c);
             const double den = fa - fb;
             double hn = (den != 0.0) ? hb - fb * (ha - hb) / den : 0.5 * (ha + hb);
+            // the secant puts a downcrossing within tol of the point just
+            // evaluated (toward lower h when F <= 0 there, higher when F > 0):
+            // that point is the root (its step may round to nothing)
+            if (h_cur == hb && hn == hn && fabs(hn - hb) <= k.tol * hb && (fb > 0.0 ? hn >= hb : hn <= hb) &&
+                (lo_pos > 0.0 || neg_seen)) {
+                accept_root(c);
+                return stage == 5;
+            }
             const double left = fmax(lo_pos, 0.01);
             const bool bracketed = lo_pos > 0.0 && neg_seen;
             if (!(hn == hn) || hn <= left || hn >= hi_neg) {

@@ -67,3 +67,19 @@ extern "C" int shim_solve_learned(const double *segc, const double *in, double d
     out[2] = o.vel;
     return o.iters | (o.flags << 24);
 }
+
+// What a LEARNED solve hands to the network after its first evaluation:
+// out = {anchor depth, routed flow there, Manning flow of the anchor};
+// returns 1 when the solve needs no guess (guard).
+extern "C" int shim_learned_anchor(const double *segc, const double *in, double dt, double *out)
+{
+    Seg c = load_seg(segc, 1, 0);
+    const ConvCfg k = {dt, 1e-13, 60, 12};
+    SecantConv s;
+    if (s.start(in[0], in[1], in[2], in[3], c, in[4], true)) return 1;
+    s.step(c, k);
+    out[0] = s.guess_anchor();
+    out[1] = s.guess_flow_new();
+    out[2] = s.guess_flow_old();
+    return s.need_guess ? 0 : 1;
+}

@@ -96,7 +96,7 @@ def test_learned_with_untrained_weights_still_converges(gpu, case):
     """A useless guess costs evaluations (and falls back), never accuracy."""
     w, T, want = case
     W = trmc.mlp_init_weights(1)
-    W[1280] = 8.0                                   # guesses e^8 times the wide-channel depth
+    W[1248:1280] *= 30.0                            # a rating curve that swings by e^+-30
     (q, d, v, rl), hist, st = _run(w, T, trmc.MODE_LEARNED, W)
     close(q[0], want["qdc"], "qdc")
     close(hist[0], want["reach_hist"], "reach outflow per step")

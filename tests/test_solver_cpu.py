@@ -28,6 +28,8 @@ def shim():
     L.shim_solve_resumable.argtypes = [pD, pD, C.c_double, pD]
     L.shim_solve_learned.restype = C.c_int
     L.shim_solve_learned.argtypes = [pD, pD, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, pD]
+    L.shim_learned_anchor.restype = C.c_int
+    L.shim_learned_anchor.argtypes = [pD, pD, C.c_double, pD]
     L.shim_pow23_53.restype = None
     L.shim_pow23_53.argtypes = [pD, pD, C.c_int]
     return L
@@ -135,3 +137,46 @@ def test_learned_start_converges_to_the_same_root(shim):
             if tag == "good":
                 saved.append(ev_plain - ev)
     assert np.mean(saved) > 1.0
+
+
+def test_learned_guess_cuts_evaluations(shim):
+    """The LEARNED start: F at the previous depth d, then a guess along the
+    network's rating curve from Qmn(d) to Qmc(d).  With a network trained (in
+    float64 numpy, on the oracle's normal depths) the solves land on the
+    oracle's roots with fewer residual evaluations than the reference bracket.
+    Inputs look like routing steps: flows within +-20 % of each other, previous
+    depth the normal depth of a flow within +-20 %."""
+    import mlp_ref as R
+    Qt, Pt = R.draw_training_set(60000, seed=4)
+    ht = np.array([orc.normal_depth(Pt[i], Qt[i]) for i in range(len(Qt))])
+    Xt = R.features(Qt, Pt[:, 4], Pt[:, 1], Pt[:, 6], Pt[:, 7]).astype(np.float64)
+    Yt = np.log(ht / R.wide_depth(Qt, Pt[:, 4], Pt[:, 1], Pt[:, 7]))
+    W = R.train_adam(Xt, Yt, epochs=12)
+    Wz = np.zeros(R.NPARAM)
+    rng = np.random.default_rng(8)
+    Qs, Ps = R.draw_training_set(1500, seed=5)
+    pD = C.POINTER(C.c_double)
+    ev = {"plain": [], "learned": [], "zero network": []}
+    for i in range(len(Qs)):
+        Q, p = Qs[i] * 0.1 + 0.05, Ps[i]
+        x = np.array([Q * rng.uniform(0.8, 1.2), Q * rng.uniform(0.8, 1.2), Q * rng.uniform(0.8, 1.2),
+                      rng.uniform(0, 0.5), orc.normal_depth(p, Q * rng.uniform(0.8, 1.2))])
+        want_q, want_h, _ = orc.solve_converged(p, *x, variant="cr")
+        out, n_plain, _ = run(shim, 1, p, x)
+        ev["plain"].append(n_plain)
+        seg = segc_row(p)
+        anc = np.zeros(3)
+        assert shim.shim_learned_anchor(seg.ctypes.data_as(pD), x.ctypes.data_as(pD), 300.0,
+                                        anc.ctypes.data_as(pD)) == 0
+        d, q_new, q_old = anc
+        for name, Wn in (("learned", W), ("zero network", Wz)):
+            g = d * np.exp(R.rating_log_depth(Wn, q_new, p)[0] - R.rating_log_depth(Wn, q_old, p)[0])
+            o = np.zeros(3)
+            r = shim.shim_solve_learned(seg.ctypes.data_as(pD), x.ctypes.data_as(pD), 300.0, 1e-13, 60, 12,
+                                        float(g), o.ctypes.data_as(pD))
+            assert abs(o[0] - want_q) <= 1e-9 * max(abs(want_q), 1e-3), (i, name, o, want_q)
+            assert abs(o[1] - want_h) <= 1e-9 * max(want_h, 1e-2), (i, name, o, want_h)
+            ev[name].append(r & 0xFFFFFF)
+    means = {k: float(np.mean(v)) for k, v in ev.items()}
+    print("mean residual evaluations per solve:", means)
+    assert means["learned"] < 0.8 * means["plain"]
