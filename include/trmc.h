@@ -96,6 +96,10 @@ typedef struct {
     int    trace;           /* SKEWED: record per-super-level time and max iterations */
     int    batch_iters;     /* DATAFLOW interleaved: secant iterations between scheduling
                                phases (0 = default 8) */
+    int    guess_model;     /* LEARNED: 0 = the network's rating curve (default),
+                               1 = the wide-channel power law h ~ Q^(3/5) alone
+                               (ablation: the same anchored guess without the
+                               network) */
 } trmc_options;
 
 typedef struct {

@@ -61,7 +61,7 @@ class Options(C.Structure):
         ("block_size", C.c_int), ("n_members", C.c_int), ("record_history", C.c_int),
         ("tol", C.c_double), ("max_iter", C.c_int), ("learned_budget", C.c_int),
         ("grid_blocks", C.c_int), ("dataflow_variant", C.c_int), ("device", C.c_int),
-        ("trace", C.c_int), ("batch_iters", C.c_int),
+        ("trace", C.c_int), ("batch_iters", C.c_int), ("guess_model", C.c_int),
     ]
 
 

@@ -213,6 +213,7 @@ void trmc_default_options(trmc_options *opt)
     opt->device = 0;
     opt->trace = 0;
     opt->batch_iters = 0;
+    opt->guess_model = 0;
 }
 
 int trmc_device_synchronize(int device)
@@ -531,6 +532,7 @@ static RouteArgs make_args(trmc_network *net, int T, double dt)
     a.t_dev = net->d_t;
     a.unit_rec = reinterpret_cast<const int4 *>(net->d_unit_rec);
     a.batch_iters = net->opt.batch_iters > 0 ? net->opt.batch_iters : 8;
+    a.guess_model = net->opt.guess_model;
     return a;
 }
 
@@ -567,7 +569,7 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
     HIPCHK(launch_reset((unsigned *)net->d_status, 1, s));
     RouteArgs a = make_args(net, T, dt);
     const int mode = o.mode;
-    if (mode == TRMC_MODE_LEARNED && !net->d_mlp)
+    if (mode == TRMC_MODE_LEARNED && o.guess_model == 0 && !net->d_mlp)
         return fail(TRMC_ERR_STATE, "LEARNED mode needs weights (trmc_network_set_mlp)");
     int sched = o.schedule;
     if (sched == TRMC_SCHED_AUTO)

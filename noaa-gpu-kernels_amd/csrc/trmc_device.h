@@ -68,6 +68,7 @@ struct RouteArgs {
     unsigned long long *flow_hist;  // DATAFLOW: [n_steps+1][n_units][M] outflow bits
     const int4 *unit_rec;           // DATAFLOW: [n_units][2] packed unit records
     int batch_iters;                // DATAFLOW interleaved: secant iterations per batch
+    int guess_model;                // LEARNED: 0 network, 1 power law alone
 };
 
 // launchers (trmc_kernels.hip); all asynchronous on `stream`

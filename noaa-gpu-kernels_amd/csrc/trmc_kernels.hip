@@ -29,7 +29,9 @@ __device__ __forceinline__ SolveOut solve_one(const RouteArgs &a, double qup, do
     if (!s.start(qup, quc, qdp, ql, c, dp, true))
         while (!s.step(c, k))
             if (s.need_guess)
-                s.set_guess(mlp_guess_scalar(a.mlp, s.guess_anchor(), s.guess_flow_new(), s.guess_flow_old(), c));
+                s.set_guess(a.guess_model == 1
+                                ? mlp_guess_from(s.guess_anchor(), s.guess_flow_new(), s.guess_flow_old(), 0.0f, 0.0f)
+                                : mlp_guess_scalar(a.mlp, s.guess_anchor(), s.guess_flow_new(), s.guess_flow_old(), c));
     return s.finish(c);
 }
 
@@ -540,7 +542,11 @@ __global__ void __launch_bounds__(256) k_route_dataflow_il(RouteArgs a)
         // ------------------------------- LEARNED: guesses for the opened solves
         if constexpr (MODE == 2) {
             const bool want = phase == PH_SOLVE && sr.need_guess;
-            if (__any(want)) {                       // wave-uniform: all 64 lanes run the forward
+            if (a.guess_model == 1) {                // power law alone: no network
+                if (want)
+                    sr.set_guess(mlp_guess_from(sr.guess_anchor(), sr.guess_flow_new(), sr.guess_flow_old(), 0.0f,
+                                                0.0f));
+            } else if (__any(want)) {                // wave-uniform: all 64 lanes run the forward
                 float xn[kMlpIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, xo[kMlpIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
                 if (want) {
                     mlp_segment_inputs(sr.guess_flow_new(), c, xn);

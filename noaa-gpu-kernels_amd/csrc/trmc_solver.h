@@ -404,7 +404,8 @@ TRMC_HD double residual(double h, double dt, double qup, double quc, double qdp,
 //                 bracket [lo_pos, hi_neg] seen so far (bisection when the
 //                 step leaves it, expansion while no F <= 0 lies above a
 //                 positive point, mindepth while every F <= 0).
-//   stop:         |step| <= tol h, F == 0, or bracket width <= tol h.
+//   stop:         |step| <= tol h, F zero to within its rounding, or
+//                 bracket width <= tol h.
 //   F(mindepth) <= 0: a geometric scan (x2, visiting bankfull) for a positive
 //                 point; none up to hmax leaves the root at mindepth.
 //   bankfull:     a root above bankfull is kept only if F(bankfull) > 0 or
@@ -424,6 +425,7 @@ struct SecantConv {
     double h_cur, q_cur;     // last evaluated depth and the outflow its coefficients give
     double qmn_cur;          // Manning flow Qmn(h_cur)
     double root_h, root_q;
+    double s_prev, s_prev2;  // the last two step sizes taken while bracketed
     int evals, stage;
     bool learned, need_guess, fell_back, capped, trivial, inbank, have_root, scan_bp, neg_seen;
 
@@ -456,6 +458,7 @@ struct SecantConv {
         lo_pos = -1.0;
         hi_neg = hm;
         neg_seen = false;
+        s_prev = s_prev2 = 1.0e30;
         if (learned) {
             ha = fmin(fmax(dp, 0.01), hm);
             hb = ha;                          // set by set_guess
@@ -510,6 +513,14 @@ struct SecantConv {
         return f;
     }
 
+    // F at the last point is zero to within its own rounding: the routed sum
+    // C1 qup + C2 quc + C3 qdp + C4 and Qmn carry errors of a few ulps of
+    // their terms, so no point can be told apart from a root beyond that.
+    TRMC_HD bool f_at_noise(double f) const
+    {
+        return fabs(f) <= 1.6e-14 * (fabs(qup) + fabs(quc) + fabs(qdp) + qmn_cur);
+    }
+
     TRMC_HD void accept_root(const Seg &c)
     {
         root_h = h_cur;
@@ -543,7 +554,7 @@ struct SecantConv {
             if (need_guess) return false;     // waiting for the wave's MLP call
             fb = eval(hb, c, k);
             stage = 2;
-            if (fb == 0.0 && lo_pos > 0.0) accept_root(c);
+            if (f_at_noise(fb) && (lo_pos > 0.0 || neg_seen)) accept_root(c);
             return stage == 5;
         case 2: {
             if (lo_pos < 0.0 && hi_neg == 0.01) {   // F(mindepth) <= 0, nothing positive above yet
@@ -565,10 +576,20 @@ struct SecantConv {
             }
             const double left = fmax(lo_pos, 0.01);
             const bool bracketed = lo_pos > 0.0 && neg_seen;
-            if (!(hn == hn) || hn <= left || hn >= hi_neg) {
-                if (bracketed) hn = 0.5 * (left + hi_neg);
+            // Once bracketed, every step must be under half the step before
+            // the last one, else it bisects the bracket (geometrically while
+            // it spans more than a factor 4), after Brent: a secant pinned to
+            // one end of a strongly curved F then still converges.
+            bool bisect = false;
+            if (bracketed && hn == hn && fabs(hn - hb) > 0.5 * s_prev2) bisect = true;
+            if (bisect || !(hn == hn) || hn <= left || hn >= hi_neg) {
+                if (bracketed) hn = (hi_neg > 4.0 * left) ? sqrt(left * hi_neg) : 0.5 * (left + hi_neg);
                 else if (lo_pos < 0.0) hn = 0.01;
                 else hn = fmin(hm, fmax(ha, hb) * 2.0);
+            }
+            if (bracketed) {
+                s_prev2 = s_prev;
+                s_prev = fabs(hn - hb);
             }
             const double fn = eval(hn, c, k);
             const double stepsz = fabs(hn - hb);
@@ -577,7 +598,8 @@ struct SecantConv {
             // F <= 0 only approach from above (hn < hi_neg), so this is a
             // downcrossing too
             if ((lo_pos > 0.0 || neg_seen) &&
-                (fn == 0.0 || stepsz <= k.tol * hn || (lo_pos > 0.0 && neg_seen && (hi_neg - lo_pos) <= k.tol * hi_neg))) {
+                (f_at_noise(fn) || stepsz <= k.tol * hn ||
+                 (lo_pos > 0.0 && neg_seen && (hi_neg - lo_pos) <= k.tol * hi_neg))) {
                 if (!(h_cur == hi_neg || h_cur == lo_pos)) {
                     stage = 7;                // last point outside the bracket: take its right end
                     return false;
@@ -600,6 +622,7 @@ struct SecantConv {
             lo_pos = -1.0;
             hi_neg = c.bfd;
             neg_seen = true;
+            s_prev = s_prev2 = 1.0e30;
             ha = 0.01;
             hb = c.bfd;
             fb = fbf;
