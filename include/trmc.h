@@ -53,11 +53,16 @@ typedef struct { double dx, bw, tw, twcc, n_ch, ncc, cs, s0; } trmc_chparams_f64
  *  REFERENCE: the reference's rule, 1 % relative or 1 cm absolute depth change,
  *             up to 5 widening attempts (wavefront_mc.cu:242-318).  Bit-for-bit
  *             drop-in semantics.
- *  CONVERGED: [new] the same MC residual solved to a tight tolerance from the
- *             reference's initial bracket (0.67 d, 1.33 d + 0.01).
- *  LEARNED:   [new] as CONVERGED, but the initial depth comes from the on-device
- *             MLP (trmc_mlp_*); a segment whose guess does not converge within
- *             the learned budget restarts from the reference bracket. */
+ *  CONVERGED: [new] the fixed point of the reference iteration (X0 = 0.5),
+ *             F(h) = Qmc(h) - Qmn(h), solved to a tight tolerance from the
+ *             reference's initial bracket (0.67 d, 1.33 d + 0.01).  The depth
+ *             is the leftmost downcrossing of F on [1 cm, inf) (1 cm when F is
+ *             never positive).
+ *  LEARNED:   [new] as CONVERGED, same root, but started from F at the
+ *             previous depth d and a guess along the on-device MLP's rating
+ *             curve from Qmn(d) to the routed flow Qmc(d) (trmc_mlp_*); a
+ *             segment not converged within learned_budget evaluations
+ *             restarts from the reference bracket. */
 #define TRMC_MODE_REFERENCE 0
 #define TRMC_MODE_CONVERGED 1
 #define TRMC_MODE_LEARNED   2
@@ -115,6 +120,8 @@ typedef struct {
     int    grid_blocks;      /* persistent grid actually used */
     int    block_size;
     double barrier_wait_frac;/* fraction of block-time spent waiting in barriers */
+    long long mlp_calls;     /* LEARNED: wave-wide network forwards (64 samples each) */
+    double mlp_frac;         /* LEARNED: fraction of wave-time spent in them */
 } trmc_run_stats;
 
 /* ---------------------------------------------------------------- basics */
@@ -279,7 +286,9 @@ int trmc_mlp_forward_backward(const float *weights, const float *features, const
                               float *pred, float *grad);
 
 /* Inference for n samples with the routing kernel's MFMA wave forward
- * (use_mfma = 1) or the scalar forward (0). */
+ * (use_mfma = 1), the scalar forward (0), or the routing kernel's packed form
+ * (2: sample pairs (2i, 2i+1) of the lanes i with i % 3 != 1, up to 32 pairs
+ * per wave forward; out is NaN for the other lanes' pairs; n even). */
 int trmc_mlp_predict(const float *weights, const float *features, int n, float *out, int use_mfma);
 
 /* ------------------------------------------------- formats and generator */

@@ -71,7 +71,7 @@ class RunStats(C.Structure):
         ("secant_iters", C.c_longlong), ("fallbacks", C.c_longlong),
         ("not_converged", C.c_longlong), ("launches", C.c_int), ("barriers", C.c_int),
         ("super_levels", C.c_int), ("grid_blocks", C.c_int), ("block_size", C.c_int),
-        ("barrier_wait_frac", C.c_double),
+        ("barrier_wait_frac", C.c_double), ("mlp_calls", C.c_longlong), ("mlp_frac", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -515,11 +515,13 @@ def mlp_forward_backward(weights, features, target):
     return pred, grad
 
 
-def mlp_predict(weights, features, use_mfma: bool = True) -> np.ndarray:
+def mlp_predict(weights, features, use_mfma=True) -> np.ndarray:
+    """use_mfma: True/1 MFMA wave forward, False/0 scalar, 2 the routing
+    kernel's packed pairs (NaN for the pairs of lanes i with i % 3 == 1)."""
     W, X = _f32(weights), _f32(np.asarray(features, np.float32).reshape(-1, 5))
     out = np.zeros(len(X), np.float32)
     _check(lib().trmc_mlp_predict(_ptr(W, C.c_float), _ptr(X, C.c_float), len(X), _ptr(out, C.c_float),
-                                  int(bool(use_mfma))))
+                                  int(use_mfma)))
     return out
 
 

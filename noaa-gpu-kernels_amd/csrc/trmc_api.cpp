@@ -691,6 +691,10 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
     st.not_converged = (long long)cnt[CNT_CAPPED];
     st.fallbacks = (long long)cnt[CNT_FALLBACK];
     st.barrier_wait_frac = cnt[CNT_BLOCK_TICKS] ? (double)cnt[CNT_WAIT_TICKS] / (double)cnt[CNT_BLOCK_TICKS] : 0.0;
+    st.mlp_calls = (long long)cnt[CNT_MLP_CALLS];
+    st.mlp_frac = cnt[CNT_BLOCK_TICKS]
+                      ? (double)cnt[CNT_MLP_TICKS] / ((double)cnt[CNT_BLOCK_TICKS] * (o.block_size / 64))
+                      : 0.0;
     st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     if (stats) *stats = st;
     return TRMC_OK;

@@ -27,6 +27,8 @@ enum CounterSlot {
     CNT_FALLBACK,       // LEARNED solves restarted from the reference bracket
     CNT_WAIT_TICKS,     // barrier wait, wall-clock ticks summed over blocks
     CNT_BLOCK_TICKS,    // total wall-clock ticks summed over blocks
+    CNT_MLP_CALLS,      // LEARNED: wave-wide network forwards
+    CNT_MLP_TICKS,      // LEARNED: wall-clock ticks inside them, summed over waves
     CNT_COUNT
 };
 

@@ -16,6 +16,7 @@ namespace trmc {
 
 struct Acc {
     unsigned long long iters, capped, fallback;
+    unsigned long long mlp_calls, mlp_ticks;   // wave-uniform
 };
 
 template <int MODE>
@@ -106,6 +107,8 @@ __device__ __forceinline__ void flush_acc(const RouteArgs &a, const Acc &acc, lo
         if (it) atomicAdd(&a.counters[CNT_ITERS], it);
         if (cp) atomicAdd(&a.counters[CNT_CAPPED], cp);
         if (fb) atomicAdd(&a.counters[CNT_FALLBACK], fb);
+        if (acc.mlp_calls) atomicAdd(&a.counters[CNT_MLP_CALLS], acc.mlp_calls);
+        if (acc.mlp_ticks) atomicAdd(&a.counters[CNT_MLP_TICKS], acc.mlp_ticks);
     }
     if (threadIdx.x == 0) {
         atomicAdd(&a.counters[CNT_WAIT_TICKS], (unsigned long long)wait);
@@ -173,7 +176,7 @@ __global__ void __launch_bounds__(256) k_route_skewed(RouteArgs a)
     const long long nthreads = (long long)gridDim.x * blockDim.x;
     const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int M = a.n_members, T = a.n_steps, L = a.n_levels;
-    Acc acc = {0, 0, 0};
+    Acc acc = {0, 0, 0, 0, 0};
     long long wait = 0;
     unsigned epoch = 0;
     for (int k = a.k_begin; k < a.k_end; ++k) {
@@ -274,7 +277,7 @@ __global__ void __launch_bounds__(256) k_route_dataflow(RouteArgs a)
     const int m = (int)(gtid % M);
     const long long base = gtid / M;
     const size_t UM = (size_t)U * M;
-    Acc acc = {0, 0, 0};
+    Acc acc = {0, 0, 0, 0, 0};
     long long wait_ticks = 0;
     // active window [j_lo, j_hi] of this lane's unit sequence
     long long j_lo = 0, j_hi = -1;
@@ -371,14 +374,15 @@ __global__ void __launch_bounds__(256) k_route_dataflow(RouteArgs a)
 // its wave.  REFERENCE solves are SecantRef, identical to solve_reference;
 // CONVERGED and LEARNED solves are SecantConv, one residual evaluation per
 // iteration.  LEARNED: a lane that opens a segment evaluates F at the
-// previous depth there, and after the scheduling phase two MFMA forwards of
-// the wave (the routed flow and the anchor's Manning flow) give every such
-// lane its guess (weights staged in LDS).
+// previous depth there; after the scheduling phase MFMA forwards of the
+// wave, each packing the two samples (the routed flow and the anchor's
+// Manning flow) of up to 32 such lanes, give every such lane its guess
+// (weights staged in LDS).
 template <int MODE> struct IlSolver { typedef SecantConv type; };
 template <> struct IlSolver<0> { typedef SecantRef type; };
 
 template <int MODE>
-__global__ void __launch_bounds__(256) k_route_dataflow_il(RouteArgs a)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_route_dataflow_il(RouteArgs a)
 {
     __shared__ float sW[MODE == 2 ? kMlpParams : 1];
     if constexpr (MODE == 2) {
@@ -394,7 +398,7 @@ __global__ void __launch_bounds__(256) k_route_dataflow_il(RouteArgs a)
     const long long base = gtid / M;
     const size_t UM = (size_t)U * M;
     const size_t nd = (size_t)a.n_dev_segs;
-    Acc acc = {0, 0, 0};
+    Acc acc = {0, 0, 0, 0, 0};
     long long wait_ticks = 0;
     long long j_lo = 0, j_hi = -1, j = 0;
     int k = 0;
@@ -547,15 +551,18 @@ __global__ void __launch_bounds__(256) k_route_dataflow_il(RouteArgs a)
                     sr.set_guess(mlp_guess_from(sr.guess_anchor(), sr.guess_flow_new(), sr.guess_flow_old(), 0.0f,
                                                 0.0f));
             } else if (__any(want)) {                // wave-uniform: all 64 lanes run the forward
+                const long long m0 = (long long)wall_clock64();
                 float xn[kMlpIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, xo[kMlpIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
                 if (want) {
                     mlp_segment_inputs(sr.guess_flow_new(), c, xn);
                     mlp_segment_inputs(sr.guess_flow_old(), c, xo);
                 }
-                const float yn = mlp_forward_wave(sW, xn);
-                const float yo = mlp_forward_wave(sW, xo);
+                float yn = 0.0f, yo = 0.0f;
+                const int calls = mlp_wave_pairs(sW, want, xn, xo, yn, yo);
                 if (want)
                     sr.set_guess(mlp_guess_from(sr.guess_anchor(), sr.guess_flow_new(), sr.guess_flow_old(), yn, yo));
+                acc.mlp_calls += (unsigned long long)calls;
+                acc.mlp_ticks += (unsigned long long)((long long)wall_clock64() - m0);
             }
         }
         // -------------------------------------- (2) solve phase, registers only
@@ -616,7 +623,7 @@ __global__ void __launch_bounds__(256) k_route_level(RouteArgs a, int level)
     const long long lo = (long long)a.level_ptr[level] * M;
     const long long hi = (long long)a.level_ptr[level + 1] * M;
     const long long i = lo + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    Acc acc = {0, 0, 0};
+    Acc acc = {0, 0, 0, 0, 0};
     if (i < hi) {
         const int u = (int)(i / M);
         const int m = (int)(i - (long long)u * M);

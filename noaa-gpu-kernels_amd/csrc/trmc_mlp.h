@@ -149,6 +149,57 @@ __device__ __forceinline__ double mlp_guess_from(double d, double q_new, double 
     return d * (double)__expf(0.6f * __logf(r) + (y_new - y_old));
 }
 
+// Bit index of the n-th (0-based) set bit of m, for n < popcount(m):
+// binary search on popcounts of the low half of the remaining window.
+__device__ __forceinline__ int nth_set_bit(unsigned long long m, int n)
+{
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const int cnt = __popcll((m >> pos) & ((1ull << w) - 1ull));
+        if (n >= cnt) {
+            n -= cnt;
+            pos += w;
+        }
+    }
+    return pos;
+}
+
+// Network outputs for the lanes with `want`, two samples per lane (xn, xo):
+// lanes 2j and 2j + 1 of a forward evaluate the new and the old sample of the
+// j-th wanting lane, so one wave forward serves 32 wanting lanes.  Must be called by all
+// 64 lanes together; returns the number of forwards run.
+__device__ __forceinline__ int mlp_wave_pairs(const float *W, bool want, const float *xn, const float *xo,
+                                              float &yn, float &yo)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned long long mask = __ballot(want);
+    const int nwant = __popcll(mask);
+    const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+    int calls = 0;
+    for (int base = 0; base < nwant; base += 32) {    // wave-uniform
+        const int j = base + (lane >> 1);
+        const int src = (j < nwant) ? nth_set_bit(mask, j) : lane;
+        float x[kMlpIn];
+#pragma unroll
+        for (int f = 0; f < kMlpIn; ++f) {
+            const float vn = __shfl(xn[f], src, 64), vo = __shfl(xo[f], src, 64);
+            x[f] = (lane & 1) ? vo : vn;
+        }
+        const float y = mlp_forward_wave(W, x);
+        const int r = rank - base;
+        const bool mine = want && r >= 0 && r < 32;
+        const int at = mine ? 2 * r : 0;
+        const float y0 = __shfl(y, at, 64), y1 = __shfl(y, at + 1, 64);
+        if (mine) {
+            yn = y0;
+            yo = y1;
+        }
+        ++calls;
+    }
+    return calls;
+}
+
 // One lane's guess with the scalar forward (schedules that solve one
 // segment per lane at a time); W in global memory.
 __device__ __forceinline__ double mlp_guess_scalar(const float *W, double d, double q_new, double q_old,

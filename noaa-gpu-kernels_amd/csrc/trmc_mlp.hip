@@ -194,6 +194,29 @@ __global__ void __launch_bounds__(64) k_mlp_predict_mfma(const float *W, const f
     if (i < n) out[i] = y;
 }
 
+// The routing kernel's packed form: lane i of the grid holds the sample pair
+// (X[2i], X[2i+1]) and wants outputs when i % 3 != 1; results go to
+// out[2i], out[2i+1] for wanting lanes, NaN for the others.
+__global__ void __launch_bounds__(64) k_mlp_predict_pairs(const float *W, const float *X, long long n, float *out)
+{
+    __shared__ float sW[kMlpParams];
+    for (int c = threadIdx.x; c < kMlpParams; c += 64) sW[c] = W[c];
+    __syncthreads();
+    const long long i = (long long)blockIdx.x * 64 + threadIdx.x;
+    const bool want = (2 * i + 1 < n) && (i % 3 != 1);
+    float xn[kMlpIn], xo[kMlpIn];
+    for (int f = 0; f < kMlpIn; ++f) {
+        xn[f] = want ? X[(size_t)(2 * i) * kMlpIn + f] : 0.0f;
+        xo[f] = want ? X[(size_t)(2 * i + 1) * kMlpIn + f] : 0.0f;
+    }
+    float yn = NAN, yo = NAN;
+    mlp_wave_pairs(sW, want, xn, xo, yn, yo);
+    if (2 * i + 1 < n) {
+        out[2 * i] = want ? yn : NAN;
+        out[2 * i + 1] = want ? yo : NAN;
+    }
+}
+
 __global__ void __launch_bounds__(64) k_mlp_predict_scalar(const float *W, const float *X, long long n, float *out)
 {
     const long long i = (long long)blockIdx.x * 64 + threadIdx.x;
@@ -394,6 +417,7 @@ int trmc_mlp_train(float *weights, int epochs, int batch, float lr, uint64_t see
 int trmc_mlp_predict(const float *weights, const float *features, int n, float *out, int use_mfma)
 {
     if (!weights || !features || !out || n <= 0) return fail(TRMC_ERR_ARG, "bad argument");
+    if (use_mfma == 2 && n % 2 != 0) return fail(TRMC_ERR_ARG, "the packed form takes sample pairs (n even)");
     int rc = ensure_device();
     if (rc) return rc;
     float *dW = nullptr, *dX = nullptr, *dO = nullptr;
@@ -404,8 +428,13 @@ int trmc_mlp_predict(const float *weights, const float *features, int n, float *
     if (e == hipSuccess) e = hipMemcpy(dX, features, (size_t)n * kMlpIn * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
         const dim3 grid((unsigned)((n + 63) / 64));
-        if (use_mfma) hipLaunchKernelGGL(k_mlp_predict_mfma, grid, dim3(64), 0, nullptr, dW, dX, (long long)n, dO);
-        else hipLaunchKernelGGL(k_mlp_predict_scalar, grid, dim3(64), 0, nullptr, dW, dX, (long long)n, dO);
+        if (use_mfma == 2)
+            hipLaunchKernelGGL(k_mlp_predict_pairs, dim3((unsigned)((n / 2 + 63) / 64)), dim3(64), 0, nullptr,
+                               dW, dX, (long long)n, dO);
+        else if (use_mfma)
+            hipLaunchKernelGGL(k_mlp_predict_mfma, grid, dim3(64), 0, nullptr, dW, dX, (long long)n, dO);
+        else
+            hipLaunchKernelGGL(k_mlp_predict_scalar, grid, dim3(64), 0, nullptr, dW, dX, (long long)n, dO);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipDeviceSynchronize();

@@ -102,6 +102,24 @@ TRMC_HD Pow2353 pow23_53(double x)
     return r;
 }
 
+// x^(2/3) and x^(5/3) from one cube root, each within a few ulps (no
+// correctly rounded finish): the CONVERGED/LEARNED residual, whose root is
+// wanted to 1e-9, not bit for bit.
+TRMC_HD Pow2353 pow23_53_faithful(double x)
+{
+    Pow2353 r;
+    if (!(x > 0.0) || x == HUGE_VAL) {
+        double v = (x == 0.0) ? 0.0 : (x == HUGE_VAL ? HUGE_VAL : NAN);
+        r.p23 = v;
+        r.p53 = v;
+        return r;
+    }
+    const double cr = cbrt(x);
+    r.p23 = cr * cr;
+    r.p53 = r.p23 * x;
+    return r;
+}
+
 struct Geo { double twl, R, A, AC, WP, WPC; };
 struct Coef { double c1, c2, c3, c4; };
 
@@ -377,7 +395,7 @@ TRMC_HD double residual(double h, double dt, double qup, double quc, double qdp,
                                            double ql, const Seg &c, Coef &C, double &qmn)
 {
     Geo g = geometry(h, c);
-    const Pow2353 w = pow23_53(g.R);
+    const Pow2353 w = pow23_53_faithful(g.R);
     const double r23 = w.p23, r53 = w.p53;
     double ck = celerity(h, r23, r53, c);
     double km = storage_km(ck, dt, c);
@@ -422,6 +440,7 @@ struct ConvCfg {
 struct SecantConv {
     double qup, quc, qdp, ql, dp;
     double ha, fa, hb, fb, lo_pos, hi_neg;
+    double qa, qb;           // outflow at ha and hb
     double h_cur, q_cur;     // last evaluated depth and the outflow its coefficients give
     double qmn_cur;          // Manning flow Qmn(h_cur)
     double root_h, root_q;
@@ -521,13 +540,15 @@ struct SecantConv {
         return fabs(f) <= 1.6e-14 * (fabs(qup) + fabs(quc) + fabs(qdp) + qmn_cur);
     }
 
-    TRMC_HD void accept_root(const Seg &c)
+    TRMC_HD void accept_at(double h, double q, const Seg &c)
     {
-        root_h = h_cur;
-        root_q = q_cur;
+        root_h = h;
+        root_q = q;
         have_root = true;
         stage = (!inbank && root_h > c.bfd && c.twcc > 0.0 && c.bfd > 0.01) ? 3 : 5;
     }
+
+    TRMC_HD void accept_root(const Seg &c) { accept_at(h_cur, q_cur, c); }
 
     // returns true when the solve is finished (MLP-guess requests leave it running)
     TRMC_HD bool step(const Seg &c, const ConvCfg &k)
@@ -547,18 +568,21 @@ struct SecantConv {
         switch (stage) {
         case 0:
             fa = eval(ha, c, k);
+            qa = q_cur;
             if (learned) need_guess = true;
             stage = 1;
             return false;
         case 1:
             if (need_guess) return false;     // waiting for the wave's MLP call
             fb = eval(hb, c, k);
+            qb = q_cur;
             stage = 2;
             if (f_at_noise(fb) && (lo_pos > 0.0 || neg_seen)) accept_root(c);
             return stage == 5;
         case 2: {
             if (lo_pos < 0.0 && hi_neg == 0.01) {   // F(mindepth) <= 0, nothing positive above yet
-                hb = 0.01;                    // scan position
+                hb = 0.01;                    // scan position (its outflow is not needed:
+                qb = q_cur;                   // the scan's secant steps are expansions)
                 scan_bp = !(c.twcc > 0.0 && c.bfd > 0.01);
                 stage = 6;
                 return false;
@@ -566,12 +590,18 @@ struct SecantConv {
             const double hm = hmax(c);
             const double den = fa - fb;
             double hn = (den != 0.0) ? hb - fb * (ha - hb) / den : 0.5 * (ha + hb);
-            // the secant puts a downcrossing within tol of the point just
-            // evaluated (toward lower h when F <= 0 there, higher when F > 0):
-            // that point is the root (its step may round to nothing)
-            if (h_cur == hb && hn == hn && fabs(hn - hb) <= k.tol * hb && (fb > 0.0 ? hn >= hb : hn <= hb) &&
-                (lo_pos > 0.0 || neg_seen)) {
-                accept_root(c);
+            // Converged without another evaluation: the secant step points
+            // to a downcrossing next to the point just evaluated (toward lower
+            // h when F <= 0 there, higher when F > 0), inside the bracket, and
+            // the secant error estimate |hn - hb| |hn - ha| / h^2 is within
+            // tol.  The root is the prediction hn; its outflow is the secant
+            // interpolation of the outflows at ha and hb (an error of the
+            // same second order).
+            if (h_cur == hb && hn == hn && (fb > 0.0 ? hn >= hb : hn <= hb) && (lo_pos > 0.0 || neg_seen) &&
+                !(hn <= lo_pos) && !(neg_seen && hn > hi_neg) && fabs(hn - hb) <= 1e-6 * hb &&
+                fabs(hn - hb) * fabs(hn - ha) <= k.tol * hb * hb) {
+                const double q = (ha != hb) ? qb + (qa - qb) * ((hn - hb) / (ha - hb)) : qb;
+                accept_at(hn, q, c);
                 return stage == 5;
             }
             const double left = fmax(lo_pos, 0.01);
@@ -594,6 +624,7 @@ struct SecantConv {
             const double fn = eval(hn, c, k);
             const double stepsz = fabs(hn - hb);
             ha = hb; fa = fb; hb = hn; fb = fn;
+            qa = qb; qb = q_cur;
             // a converged step from either side of the bracket: points with
             // F <= 0 only approach from above (hn < hi_neg), so this is a
             // downcrossing too
@@ -611,6 +642,7 @@ struct SecantConv {
         case 3: {                             // a downcrossing below bankfull?
             const double rh = root_h, rq = root_q;
             const double fbf = eval(c.bfd, c, k);
+            const double qbf = q_cur;
             inbank = true;
             if (fbf > 0.0) {                  // no upcrossing between: keep the root
                 root_h = rh;
@@ -626,7 +658,9 @@ struct SecantConv {
             ha = 0.01;
             hb = c.bfd;
             fb = fbf;
+            qb = qbf;
             fa = eval(ha, c, k);
+            qa = q_cur;
             if (!(fa > 0.0)) {
                 root_h = rh;
                 root_q = rq;
@@ -652,6 +686,7 @@ struct SecantConv {
             // lo_pos = hs, which the bracket turns into an expansion
             ha = hb; fa = -1.0;
             hb = hs; fb = fs;
+            qa = qb; qb = q_cur;
             if (fs > 0.0) stage = 2;
             return false;
         }

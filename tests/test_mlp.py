@@ -89,6 +89,21 @@ def test_predict_mfma_and_scalar_match_reference(gpu, n):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("pairs", [1, 40, 1000])
+def test_packed_pairs_match_reference(gpu, pairs):
+    """The routing kernel's packing: wanting lanes' sample pairs gathered two
+    per lane pair into as few forwards as possible, results shuffled back."""
+    W = _rand_weights(31 + pairs)
+    X = np.random.default_rng(pairs).uniform(-2.5, 2.5, (2 * pairs, 5)).astype(np.float32)
+    got = trmc.mlp_predict(W, X, use_mfma=2)
+    want = R.forward(W, X)[0]
+    lane = np.repeat(np.arange(pairs), 2)
+    on = lane % 3 != 1
+    assert np.all(np.isnan(got[~on]))
+    assert np.abs(got[on] - want[on]).max() <= 1e-4 * (1 + np.abs(want).max())
+
+
+@pytest.mark.gpu
 def test_forward_backward_matches_reference(gpu):
     W = _rand_weights(21, 0.3)
     rng = np.random.default_rng(22)
