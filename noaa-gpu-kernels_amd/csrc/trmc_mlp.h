@@ -38,8 +38,10 @@ __host__ __device__ inline float mlp_wide_depth(float Q, float n, float bw, floa
 
 __device__ __forceinline__ float fast_tanh(float x)
 {
-    // 1 - 2 / (1 + e^{2x}); saturates correctly for large |x|
-    return 1.0f - __fdividef(2.0f, 1.0f + __expf(2.0f * x));
+    // 1 - 2 / (1 + e^{2x}) with the hardware exp2 and reciprocal (about one
+    // ulp each): e^{2x} = 2^{2x / ln 2}.  Saturates to +-1 (exp2 -> inf or 0).
+    const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
+    return fmaf(-2.0f, __builtin_amdgcn_rcpf(1.0f + e), 1.0f);
 }
 
 // Scalar forward for one sample (training kernels); a1, a2 receive the
@@ -122,7 +124,7 @@ __device__ __forceinline__ float mlp_forward_wave(const float *W, const float *x
                     acc = __builtin_amdgcn_mfma_f32_16x16x4f32(
                         W[kOffW2 + (16 * mo + li) * kMlpHid + 16 * mi + 4 * lg + r], h1[mi][r], acc, 0, 0, 0);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) p += W[kOffW3 + row + r] * fast_tanh(acc[r]);
+            for (int r = 0; r < 4; ++r) p = fmaf(W[kOffW3 + row + r], fast_tanh(acc[r]), p);
         }
         p += __shfl_xor(p, 16, 64);
         p += __shfl_xor(p, 32, 64);
