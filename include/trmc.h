@@ -223,6 +223,12 @@ int trmc_network_set_state_f64(trmc_network *net, const double *qdp0_seg,
 int trmc_network_set_forcing_f64(trmc_network *net, const double *qlat, int n_timesteps);
 int trmc_network_set_forcing_f32(trmc_network *net, const float *qlat, int n_timesteps);
 
+/* [new] Synthetic lateral inflow generated on the device, in place of
+ * set_forcing: exactly the values trmc_generate_forcing(n_segments,
+ * n_timesteps, n_members, seed) returns, without the host array or the copy
+ * (a 1M-segment x 16-member x 288-step forcing is 37 GB). */
+int trmc_network_generate_forcing(trmc_network *net, int n_timesteps, uint64_t seed);
+
 /* [new] Weights of the LEARNED depth guess (TRMC_MLP_NPARAM floats, layout of
  * trmc_mlp_train), copied to the device; required before a LEARNED run. */
 int trmc_network_set_mlp(trmc_network *net, const float *weights);
@@ -256,6 +262,21 @@ int trmc_network_get_trace(const trmc_network *net, double *us, int *max_iters, 
 /* Shape facts of the prepared schedule, as trmc_plan_info with the
  * network's member count. */
 int trmc_network_info(const trmc_network *net, int n_timesteps, int64_t *info, int n_info);
+
+/* [new] Autotuner for the schedule of a network (or a union of networks):
+ * times candidate unit sizes (2, 4, 8 segments), interleaving batch sizes
+ * (6, 8, 12) and workgroup sizes (64, 128) on the first n_probe steps of the
+ * forcing -- qlat [n_members][n_probe][n_segments], or generated on the
+ * device from forcing_seed when qlat is NULL -- from the given state, and
+ * returns the options with the fastest kernel time (base's other fields
+ * kept) and that time per step.  qdp0/dp0 are [n_members][n_segments]
+ * (base->n_members); mlp_weights is needed for LEARNED only. */
+int trmc_autotune(const int *reach_seg_start, const int *reach_seg_len, const int *reach_up_ptr,
+                  const int *reach_up_idx, const int *level_ptr, const int *level_reach, int n_reaches,
+                  int n_levels, int n_segments, const trmc_chparams_f64 *params, const double *qdp0_seg,
+                  const double *dp0_seg, const double *reach_last0, const double *qlat, uint64_t forcing_seed,
+                  int n_probe, double dt, const float *mlp_weights, const trmc_options *base, trmc_options *best,
+                  double *best_ms_per_step);
 
 /* ------------------------------------------------------ learned guess */
 /* [new] 5->32->32->1 tanh MLP proposing the initial secant depth from
@@ -301,6 +322,12 @@ typedef struct {
 } trmc_topo;
 
 int  trmc_load_topo(const char *path, trmc_topo *out);
+/* [new] Disjoint union of n networks for routing them in one launch:
+ * network i's segments and reaches are numbered after network i-1's, level L
+ * of the union lists level L of every network.  seg_offset / reach_offset
+ * [n + 1] (may be NULL) receive where each network starts.  Free with
+ * trmc_free_topo. */
+int  trmc_topo_concat(const trmc_topo *nets, int n, trmc_topo *out, int *seg_offset, int *reach_offset);
 int  trmc_save_topo(const char *path, const trmc_topo *t);
 void trmc_free_topo(trmc_topo *t);
 int  trmc_load_params(const char *path, int n_segments, trmc_chparams_f32 *out);

@@ -149,6 +149,10 @@ def _declare(L: C.CDLL) -> None:
         "trmc_generate_state": (I, [I, _p(ChParamsF64), D, pD, pD]),
         "trmc_compute_stats": (I, [pD, pD, I, pD]),
         "trmc_network_set_mlp": (I, [V, pF]),
+        "trmc_network_generate_forcing": (I, [V, I, C.c_uint64]),
+        "trmc_topo_concat": (I, [_p(Topo), I, _p(Topo), pI, pI]),
+        "trmc_autotune": (I, topo + [_p(ChParamsF64), pD, pD, pD, pD, C.c_uint64, I, D, pF, _p(Options),
+                                     _p(Options), pD]),
         "trmc_mlp_make_dataset": (I, [I, C.c_uint64, pF, pF, pD]),
         "trmc_mlp_init_weights": (None, [C.c_uint64, pF]),
         "trmc_mlp_train": (I, [pF, I, I, C.c_float, C.c_uint64, pF]),
@@ -335,6 +339,46 @@ def generate_state(params: np.ndarray, q0: float = 1.0):
     return qdp0, dp0
 
 
+def concat_topologies(topos):
+    """Disjoint union of networks (trmc_topo_concat) for one launch.
+    Returns (Topology, segment offsets [n+1], reach offsets [n+1])."""
+    n = len(topos)
+    arr = (Topo * max(n, 1))()
+    keep = []
+    for i, t in enumerate(topos):
+        c = t.to_c()
+        keep.append(c)
+        arr[i] = c
+    out = Topo()
+    so = np.zeros(n + 1, np.int32)
+    ro = np.zeros(n + 1, np.int32)
+    _check(lib().trmc_topo_concat(arr, n, C.byref(out), _ptr(so, C.c_int), _ptr(ro, C.c_int)))
+    try:
+        topo = Topology.from_c(out)
+    finally:
+        lib().trmc_free_topo(C.byref(out))
+    return topo, so, ro
+
+
+def batch_sizes(n_networks: int, lo: int, hi: int, seed: int) -> np.ndarray:
+    """Network sizes log-uniform in [lo, hi] (seeded)."""
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), n_networks)).astype(np.int64)
+
+
+def make_batch_workload(sizes, n_timesteps: int, n_members: int = 1, seed: int = 42, forcing: bool = True):
+    """Independent Shreve-style networks (kind 2, network i seeded seed + i,
+    about 0.3 reaches per segment) routed as one union network.  Parameters
+    and state for the union; lateral inflow [M, T, N] on the host unless
+    forcing=False (then use Network.generate_forcing with seed ^ 0xA1A1)."""
+    nets = [generate_network(2, int(n), int(0.3 * n), seed + i) for i, n in enumerate(sizes)]
+    topo, so, ro = concat_topologies(nets)
+    params = generate_params(topo.n_segments, seed ^ 0x5A5A)
+    qdp0, dp0 = generate_state(params)
+    qlat = generate_forcing(topo.n_segments, n_timesteps, n_members, seed ^ 0xA1A1) if forcing else None
+    return Workload(topo, params, qlat, qdp0, dp0, reach_last0_from(topo, qdp0)), so, ro
+
+
 def reach_last0_from(topo: Topology, qdp0: np.ndarray) -> np.ndarray:
     """reach_last0[r] = qdp0 of reach r's last segment (wavefront_mc.cu:676-680)."""
     last = topo.reach_seg_start + topo.reach_seg_len - 1
@@ -352,11 +396,13 @@ class Workload:
 
 
 def make_workload(kind: int, n_segments: int, n_timesteps: int, n_reaches: int = 0,
-                  n_members: int = 1, seed: int = 42) -> Workload:
-    """Seeded synthetic network, parameters, lateral inflow and initial state."""
+                  n_members: int = 1, seed: int = 42, forcing: bool = True) -> Workload:
+    """Seeded synthetic network, parameters, lateral inflow and initial state
+    (forcing=False: no host lateral inflow; Network.generate_forcing with
+    seed ^ 0xA1A1 makes the same values on the device)."""
     topo = generate_network(kind, n_segments, n_reaches, seed)
     params = generate_params(topo.n_segments, seed ^ 0x5A5A)
-    qlat = generate_forcing(topo.n_segments, n_timesteps, n_members, seed ^ 0xA1A1)
+    qlat = generate_forcing(topo.n_segments, n_timesteps, n_members, seed ^ 0xA1A1) if forcing else None
     qdp0, dp0 = generate_state(params, 1.0)
     return Workload(topo, params, qlat, qdp0, dp0, reach_last0_from(topo, qdp0))
 
@@ -525,6 +571,31 @@ def mlp_predict(weights, features, use_mfma=True) -> np.ndarray:
     return out
 
 
+def autotune(topo: Topology, params, qdp0, dp0, reach_last0=None, n_members: int = 1, n_probe: int = 16,
+             qlat=None, forcing_seed: int = 0, dt: float = 300.0, mlp_weights=None, **options):
+    """trmc_autotune: the fastest (unit_segments, batch_iters, block_size)
+    on the first n_probe steps.  Returns (options dict, ms per step)."""
+    base = default_options(n_members=n_members, **options)
+    p = params_f64(params)
+    M, N = int(n_members), topo.n_segments
+    q = _f64(np.broadcast_to(np.asarray(qdp0, np.float64), (M, N)))
+    d = _f64(np.broadcast_to(np.asarray(dp0, np.float64), (M, N)))
+    rl = None if reach_last0 is None else _f64(np.broadcast_to(np.asarray(reach_last0, np.float64),
+                                                                (M, topo.n_reaches)))
+    ql = None if qlat is None else _f64(np.asarray(qlat)[:, :n_probe])
+    W = None if mlp_weights is None else _f32(mlp_weights)
+    best = Options()
+    ms = C.c_double()
+    _check(lib().trmc_autotune(*topo.c_args(), _ptr(p, ChParamsF64), _ptr(q, C.c_double), _ptr(d, C.c_double),
+                               _ptr(rl, C.c_double) if rl is not None else C.POINTER(C.c_double)(),
+                               _ptr(ql, C.c_double) if ql is not None else C.POINTER(C.c_double)(),
+                               int(forcing_seed), int(n_probe), float(dt),
+                               _ptr(W, C.c_float) if W is not None else C.POINTER(C.c_float)(),
+                               C.byref(base), C.byref(best), C.byref(ms)))
+    keys = [f[0] for f in Options._fields_]
+    return {k: getattr(best, k) for k in keys}, ms.value
+
+
 class Network:
     """Device-resident network (trmc_network_*): topology, parameters, forcing
     and state stay in HBM across runs."""
@@ -586,6 +657,11 @@ class Network:
         else:
             q = _f64(q)
             _check(lib().trmc_network_set_forcing_f64(self._h, _ptr(q, C.c_double), q.shape[1]))
+
+    def generate_forcing(self, n_timesteps: int, seed: int) -> None:
+        """Synthetic lateral inflow generated on the device: the values
+        generate_forcing(n_segments, n_timesteps, n_members, seed) gives."""
+        _check(lib().trmc_network_generate_forcing(self._h, int(n_timesteps), int(seed)))
 
     def set_mlp(self, weights) -> None:
         """Weights of the LEARNED depth guess (MLP_NPARAM float32)."""

@@ -475,6 +475,21 @@ int trmc_network_set_forcing_f32(trmc_network *net, const float *qlat, int n_tim
     return set_forcing_impl(net, qlat, n_timesteps);
 }
 
+int trmc_network_generate_forcing(trmc_network *net, int T, uint64_t seed)
+{
+    if (!net || T < 0) return fail(TRMC_ERR_ARG, "bad forcing arguments");
+    HIPCHK(hipSetDevice(net->opt.device));
+    const Plan &P = net->plan;
+    int rc = dev_alloc(&net->d_qlat, (size_t)T * P.n_dev_segments * net->M);
+    if (rc) return rc;
+    HIPCHK(launch_gen_forcing(net->d_qlat, net->d_seg_of_dev, P.n_dev_segments, P.n_segments, net->M, T,
+                              (unsigned long long)seed, net->stream));
+    HIPCHK(hipStreamSynchronize(net->stream));
+    net->forcing_T = T;
+    net->t_cursor = 0;
+    return TRMC_OK;
+}
+
 int trmc_network_set_mlp(trmc_network *net, const float *weights)
 {
     if (!net || !weights) return fail(TRMC_ERR_ARG, "null network or weights");
@@ -939,6 +954,61 @@ int trmc_mc_solve_f64(const trmc_chparams_f64 *params, const double *in, double 
     return TRMC_OK;
 }
 
+// ------------------------------------------------------------- autotuner
+// Candidate schedules are timed on the first n_probe steps (after one warm-up
+// run of the same steps): unit size (one network each), then batch size and
+// workgroup size on that network.  The fastest kernel time wins.
+int trmc_autotune(const int *reach_seg_start, const int *reach_seg_len, const int *reach_up_ptr,
+                  const int *reach_up_idx, const int *level_ptr, const int *level_reach, int n_reaches,
+                  int n_levels, int n_segments, const trmc_chparams_f64 *params, const double *qdp0_seg,
+                  const double *dp0_seg, const double *reach_last0, const double *qlat, uint64_t forcing_seed,
+                  int n_probe, double dt, const float *mlp_weights, const trmc_options *base, trmc_options *best,
+                  double *best_ms_per_step)
+{
+    if (!params || !qdp0_seg || !dp0_seg || !best || n_probe <= 0) return fail(TRMC_ERR_ARG, "bad argument");
+    trmc_options o0;
+    if (base) o0 = *base; else trmc_default_options(&o0);
+    static const int kUnits[] = {2, 4, 8};
+    static const int kBatch[] = {6, 8, 12};
+    static const int kBlock[] = {64, 128};
+    double best_ms = 1e300;
+    trmc_options winner = o0;
+    for (int us : kUnits) {
+        trmc_options o = o0;
+        o.unit_segments = us;
+        o.record_history = 0;
+        trmc_network *net = nullptr;
+        int rc = trmc_network_create(reach_seg_start, reach_seg_len, reach_up_ptr, reach_up_idx, level_ptr,
+                                     level_reach, n_reaches, n_levels, n_segments, &o, &net);
+        if (!rc) rc = trmc_network_set_params_f64(net, params);
+        if (!rc) rc = trmc_network_set_state_f64(net, qdp0_seg, dp0_seg, reach_last0);
+        if (!rc) rc = qlat ? trmc_network_set_forcing_f64(net, qlat, n_probe)
+                           : trmc_network_generate_forcing(net, n_probe, forcing_seed);
+        if (!rc && o.mode == TRMC_MODE_LEARNED && mlp_weights) rc = trmc_network_set_mlp(net, mlp_weights);
+        for (int bi : kBatch) {
+            for (int blk : kBlock) {
+                if (rc) break;
+                net->opt.batch_iters = bi;
+                net->opt.block_size = blk;
+                trmc_run_stats st;
+                rc = trmc_network_run(net, n_probe, dt, &st);
+                if (!rc) rc = trmc_network_reset(net);
+                if (!rc) rc = trmc_network_run(net, n_probe, dt, &st);
+                if (!rc) rc = trmc_network_reset(net);
+                if (!rc && st.kernel_ms < best_ms) {
+                    best_ms = st.kernel_ms;
+                    winner = net->opt;
+                }
+            }
+        }
+        if (net) trmc_network_destroy(net);
+        if (rc) return rc;
+    }
+    *best = winner;
+    if (best_ms_per_step) *best_ms_per_step = best_ms / n_probe;
+    return TRMC_OK;
+}
+
 // ------------------------------------------------------ formats, generator
 int trmc_load_topo(const char *path, trmc_topo *out)
 {
@@ -954,6 +1024,17 @@ int trmc_save_topo(const char *path, const trmc_topo *t)
     if (!path || !t) return fail(TRMC_ERR_ARG, "null argument");
     int rc = save_topo(path, t);
     if (rc) return fail(rc, "cannot write %s", path);
+    return TRMC_OK;
+}
+
+int trmc_topo_concat(const trmc_topo *nets, int n, trmc_topo *out, int *seg_offset, int *reach_offset)
+{
+    if (!out || n < 0 || (n > 0 && !nets)) return fail(TRMC_ERR_ARG, "bad argument");
+    int rc = concat_topo(nets, n, out, seg_offset, reach_offset);
+    if (rc) {
+        trmc_free_topo(out);
+        return fail(rc, rc == TRMC_ERR_NOMEM ? "out of host memory" : "inconsistent or oversized topology");
+    }
     return TRMC_OK;
 }
 

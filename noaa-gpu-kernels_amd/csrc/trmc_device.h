@@ -74,6 +74,8 @@ struct RouteArgs {
 };
 
 // launchers (trmc_kernels.hip); all asynchronous on `stream`
+hipError_t launch_gen_forcing(double *dst, const int *seg_of_dev, int nd, int N, int M, int T,
+                              unsigned long long seed, hipStream_t stream);
 hipError_t launch_reset(unsigned *bar, int nwords, hipStream_t stream);
 hipError_t launch_route_skewed(const RouteArgs &a, int mode, int grid, int block,
                                bool cooperative, hipStream_t stream);

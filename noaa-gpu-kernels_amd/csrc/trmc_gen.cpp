@@ -16,66 +16,17 @@
 #include <vector>
 
 #include "trmc.h"
+#include "trmc_rng.h"
 #include "trmc_util.h"
 
 namespace trmc {
 
-// ---------------------------------------------------------------- random
-static inline uint64_t mix64(uint64_t z)
-{
-    z += 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-
-static inline uint64_t stream_key(uint64_t seed, uint64_t stream)
-{
-    return mix64(seed * 0x2545F4914F6CDD1Dull ^ mix64(stream + 0x1234567ull));
-}
-
-// uniform in (0, 1): 53 random bits, never exactly 0 or 1
-static inline double uniform_at(uint64_t key, uint64_t i)
-{
-    uint64_t x = mix64(key ^ mix64(i));
-    return ((double)(x >> 11) + 0.5) * 0x1.0p-53;
-}
-
-// natural log from basic operations: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
-// log m = 2 atanh(s), s = (m-1)/(m+1), odd series to s^21.
-static double det_log(double x)
-{
-    int e = 0;
-    double m = std::frexp(x, &e);            // m in [0.5, 1)
-    if (m < 0.70710678118654752440) { m *= 2.0; e -= 1; }
-    double s = (m - 1.0) / (m + 1.0), s2 = s * s, term = s, sum = 0.0;
-    for (int k = 1; k <= 21; k += 2) { sum += term / (double)k; term *= s2; }
-    return 2.0 * sum + (double)e * 0.69314718055994530942;
-}
-
-// exp from basic operations: x = k ln2 + r, |r| <= ln2/2, Taylor to r^17.
-static double det_exp(double x)
-{
-    if (x < -700.0) return 0.0;
-    if (x > 700.0) return HUGE_VAL;
-    double kf = std::floor(x / 0.69314718055994530942 + 0.5);
-    double r = (x - kf * 0.693145751953125) - kf * 1.42860682030941723212e-6;
-    double term = 1.0, sum = 1.0;
-    for (int k = 1; k <= 17; ++k) { term *= r / (double)k; sum += term; }
-    return std::ldexp(sum, (int)kf);
-}
-
-// standard normal, Marsaglia polar method on sub-draws of one counter
-static double normal_at(uint64_t key, uint64_t i)
-{
-    uint64_t sub = mix64(key + 0x51ED27ull) ^ mix64(i);
-    for (uint64_t a = 0;; ++a) {
-        double u = 2.0 * uniform_at(sub, 2 * a) - 1.0;
-        double v = 2.0 * uniform_at(sub, 2 * a + 1) - 1.0;
-        double s = u * u + v * v;
-        if (s > 0.0 && s < 1.0) return u * std::sqrt(-2.0 * det_log(s) / s);
-    }
-}
+using rng::det_exp;
+using rng::det_log;
+using rng::mix64;
+using rng::normal_at;
+using rng::stream_key;
+using rng::uniform_at;
 
 static inline double f32r(double x) { return (double)(float)x; }
 
@@ -286,16 +237,14 @@ int generate_params(int n, uint64_t seed, trmc_chparams_f64 *out)
 int generate_forcing(int n, int T, int M, uint64_t seed, double *qlat)
 {
     if (!qlat || n < 0 || T < 0 || M < 1) return TRMC_ERR_ARG;
-    const double mu = det_log(0.1) - 0.5;
     const size_t per_member = (size_t)n * (size_t)T;
     const size_t total = per_member * (size_t)M;
     unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     if (total < (size_t)1 << 16) nthreads = 1;
     auto work = [&](size_t b, size_t e) {
         for (size_t i = b; i < e; ++i) {
-            size_t m = i / per_member, j = i - m * per_member;
-            uint64_t key = stream_key(seed, 1000 + m);
-            qlat[i] = f32r(det_exp(mu + normal_at(key, j)));
+            const size_t m = i / per_member, j = i - m * per_member;
+            qlat[i] = rng::forcing_at(seed, (int)m, j);
         }
     };
     std::vector<std::thread> pool;
@@ -362,6 +311,78 @@ int load_topo(const char *path, trmc_topo *t, std::string &err)
         trmc_free_topo(t);
         err = std::string("short read in ") + path;
         return rc == TRMC_OK ? TRMC_ERR_IO : rc;
+    }
+    return TRMC_OK;
+}
+
+// Disjoint union: network i's segments and reaches follow network i-1's;
+// level L of the union lists level L of every network, in network order.
+int concat_topo(const trmc_topo *nets, int n, trmc_topo *out, int *seg_offset, int *reach_offset)
+{
+    long long R = 0, N = 0, nnz = 0;
+    int L = 0, max_up = 0;
+    for (int i = 0; i < n; ++i) {
+        const trmc_topo &t = nets[i];
+        if (t.n_reaches < 0 || t.n_segments < 0 || t.n_levels < 0) return TRMC_ERR_ARG;
+        if (t.n_reaches > 0 && (!t.reach_seg_start || !t.reach_seg_len || !t.reach_up_ptr || !t.level_ptr ||
+                                !t.level_reach))
+            return TRMC_ERR_ARG;
+        R += t.n_reaches;
+        N += t.n_segments;
+        nnz += t.n_reaches > 0 ? t.reach_up_ptr[t.n_reaches] : 0;
+        L = std::max(L, t.n_levels);
+        max_up = std::max(max_up, t.max_upstream);
+    }
+    if (R > INT32_MAX / 2 || N > INT32_MAX / 2 || nnz > INT32_MAX / 2) return TRMC_ERR_ARG;
+    *out = trmc_topo{};
+    out->n_reaches = (int)R;
+    out->n_segments = (int)N;
+    out->n_levels = L;
+    out->max_upstream = max_up;
+    auto alloc = [](size_t k) { return (int *)std::malloc(sizeof(int) * (k + 1)); };
+    out->reach_seg_start = alloc(R);
+    out->reach_seg_len = alloc(R);
+    out->reach_level = alloc(R);
+    out->reach_n_up = alloc(R);
+    out->reach_up_ptr = alloc(R + 1);
+    out->reach_up_idx = alloc(nnz);
+    out->level_ptr = alloc(L + 1);
+    out->level_reach = alloc(R);
+    if (!out->reach_seg_start || !out->reach_seg_len || !out->reach_level || !out->reach_n_up ||
+        !out->reach_up_ptr || !out->reach_up_idx || !out->level_ptr || !out->level_reach)
+        return TRMC_ERR_NOMEM;
+    int r0 = 0, s0 = 0, k0 = 0;
+    out->reach_up_ptr[0] = 0;
+    for (int i = 0; i < n; ++i) {
+        const trmc_topo &t = nets[i];
+        if (seg_offset) seg_offset[i] = s0;
+        if (reach_offset) reach_offset[i] = r0;
+        for (int r = 0; r < t.n_reaches; ++r) {
+            out->reach_seg_start[r0 + r] = t.reach_seg_start[r] + s0;
+            out->reach_seg_len[r0 + r] = t.reach_seg_len[r];
+            out->reach_level[r0 + r] = t.reach_level ? t.reach_level[r] : 0;
+            out->reach_n_up[r0 + r] = t.reach_up_ptr[r + 1] - t.reach_up_ptr[r];
+            for (int k = t.reach_up_ptr[r]; k < t.reach_up_ptr[r + 1]; ++k)
+                out->reach_up_idx[k0 + k] = t.reach_up_idx[k] + r0;
+            out->reach_up_ptr[r0 + r + 1] = k0 + t.reach_up_ptr[r + 1];
+        }
+        k0 += t.n_reaches > 0 ? t.reach_up_ptr[t.n_reaches] : 0;
+        r0 += t.n_reaches;
+        s0 += t.n_segments;
+    }
+    if (seg_offset) seg_offset[n] = s0;
+    if (reach_offset) reach_offset[n] = r0;
+    int w = 0;
+    out->level_ptr[0] = 0;
+    for (int lv = 0; lv < L; ++lv) {
+        int rb = 0;
+        for (int i = 0; i < n; ++i) {
+            const trmc_topo &t = nets[i];
+            if (lv < t.n_levels)
+                for (int k = t.level_ptr[lv]; k < t.level_ptr[lv + 1]; ++k) out->level_reach[w++] = t.level_reach[k] + rb;
+            rb += t.n_reaches;
+        }
+        out->level_ptr[lv + 1] = w;
     }
     return TRMC_OK;
 }

@@ -8,8 +8,11 @@
 // (wavefront_mc.cu:388 reads qdp_prev_seg[idx-1]).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "trmc_device.h"
 #include "trmc_mlp.h"
+#include "trmc_rng.h"
 #include "trmc_solver.h"
 
 namespace trmc {
@@ -710,7 +713,34 @@ __global__ void k_scatter_forcing(const TSrc *__restrict__ src, int n_src, long 
     dst[(size_t)i * M + m] = (double)src[(size_t)t * n_src + seg_of_dev[ds]];
 }
 
+// Synthetic lateral inflow straight into the forcing layout
+// [t][device segment][member]: the values trmc_generate_forcing gives for
+// draw t * N + segment of member m.
+__global__ void k_gen_forcing(double *__restrict__ dst, const int *__restrict__ seg_of_dev, int nd, int N, int M,
+                              long long total, unsigned long long seed)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const int m = (int)(i % M);
+        const long long r = i / M;
+        const int ds = (int)(r % nd);
+        const long long t = r / nd;
+        dst[i] = rng::forcing_at(seed, m, (unsigned long long)t * (unsigned long long)N + seg_of_dev[ds]);
+    }
+}
+
 // ----------------------------------------------------------------- launchers
+
+hipError_t launch_gen_forcing(double *dst, const int *seg_of_dev, int nd, int N, int M, int T,
+                              unsigned long long seed, hipStream_t stream)
+{
+    const long long total = (long long)T * nd * M;
+    if (total <= 0) return hipSuccess;
+    const long long blocks = std::min<long long>((total + 255) / 256, 1 << 20);
+    hipLaunchKernelGGL(k_gen_forcing, dim3((unsigned)blocks), dim3(256), 0, stream, dst, seg_of_dev, nd, N, M, total,
+                       seed);
+    return hipGetLastError();
+}
 
 hipError_t launch_reset(unsigned *bar, int nwords, hipStream_t stream)
 {

@@ -21,5 +21,6 @@ int generate_state(int n, const trmc_chparams_f64 *p, double q0, double *qdp0, d
 int load_topo(const char *path, trmc_topo *t, std::string &err);
 int save_topo(const char *path, const trmc_topo *t);
 int compute_stats(const double *pred, const double *truth, int n, double *out);
+int concat_topo(const trmc_topo *nets, int n, trmc_topo *out, int *seg_offset, int *reach_offset);
 
 }  // namespace trmc
