@@ -297,8 +297,8 @@ def main():
     for name in order:
         recs[name], nets[name] = time_mode(trmc, w, M, T, dt, name, opts, fseed, args, weights)
         finals[name] = nets[name].state()
-        if name not in ("learned", "reference", head):
-            nets.pop(name).close()
+        if name not in ("learned", head):
+            nets.pop(name).close()     # keep one full forcing in HBM at a time
     hrec = recs[head]
     info = nets[head].info(T)
     value, ms_per_step = hrec["value"], hrec["ms_per_step"]
@@ -371,7 +371,7 @@ def main():
         out["cpu_baseline"] = rec
         rnet = nets.get("reference")
         if rnet is None:
-            rnet = new_net(trmc, w, M, 0, opts, fseed, T)
+            rnet = new_net(trmc, w, M, 0, opts, fseed, Tc)
             nets["reference"] = rnet
         rnet.reset()
         rnet.run(Tc, dt)
