@@ -79,36 +79,58 @@ def bytes_per_seg_step(M):
     return 48.0 + 64.0 / M
 
 
+def host_forcing(N, T, M, seed):
+    """Lateral inflow [M, T, N] float32 for ensembles: lognormal, mean 0.1
+    m3/s, sigma_ln 1, member m drawn from numpy's generator seeded (seed, m)
+    (member threads in parallel)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    out = np.empty((M, T, N), np.float32)
+    mu = np.float32(np.log(0.1) - 0.5)
+
+    def fill(m):
+        rng = np.random.default_rng([seed, m])
+        for t in range(T):
+            x = rng.standard_normal(N, dtype=np.float32)
+            np.exp(x + mu, out=out[m, t])
+
+    with ThreadPoolExecutor(max(1, min(M, 16))) as ex:
+        list(ex.map(fill, range(M)))
+    return out
+
+
 def make(trmc, spec):
-    """(workload, description, forcing seed).  Lateral inflow is generated on
-    the device (Network.generate_forcing), so no host forcing is built."""
+    """(workload, description).  One member: the library's seeded generator
+    (float64); ensembles and batches: host_forcing (float32)."""
     if spec["kind"] == "batch":
         sizes = trmc.batch_sizes(spec["n"], spec["r"][0], spec["r"][1], spec["seed"])
         w, so, ro = trmc.make_batch_workload(sizes, spec["T"], spec["M"], seed=spec["seed"], forcing=False)
         desc = (f"{spec['n']} independent Shreve networks ({int(sizes.min())}-{int(sizes.max())} segments, "
                 f"log-uniform) x {spec['M']} members, one launch")
     else:
-        w = trmc.make_workload(spec["kind"], spec["n"], spec["T"], n_reaches=spec["r"], n_members=spec["M"],
-                               seed=spec["seed"], forcing=False)
+        w = trmc.make_workload(spec["kind"], spec["n"], spec["T"], n_reaches=spec["r"], n_members=1,
+                               seed=spec["seed"], forcing=spec["M"] == 1)
         desc = "Shreve-style tree" if spec["kind"] == 2 else "binary-confluence tree"
         if spec["M"] > 1:
             desc += f" x {spec['M']} members"
-    return w, desc, spec["seed"] ^ 0xA1A1
+    if w.qlat is None:
+        w.qlat = host_forcing(w.topo.n_segments, spec["T"], spec["M"], spec["seed"] ^ 0xA1A1)
+    return w, desc
 
 
-def new_net(trmc, w, M, mode, opts, fseed, T, weights=None, guess_model=0):
+def new_net(trmc, w, M, mode, opts, T, weights=None, guess_model=0):
     net = trmc.Network(w.topo, mode=mode, n_members=M, guess_model=guess_model, **opts)
     net.set_params(w.params)
     net.set_state(w.qdp0, w.dp0, w.reach_last0)
-    net.generate_forcing(T, fseed)
+    net.set_forcing(w.qlat if T == w.qlat.shape[1] else np.ascontiguousarray(w.qlat[:, :T]))
     if weights is not None:
         net.set_mlp(weights)
     return net
 
 
-def time_mode(trmc, w, M, T, dt, name, opts, fseed, args, weights=None):
+def time_mode(trmc, w, M, T, dt, name, opts, args, weights=None):
     """Warm-up then timed passes of one mode; returns (record, network)."""
-    net = new_net(trmc, w, M, MODES[name], opts, fseed, T, weights if name == "learned" else None,
+    net = new_net(trmc, w, M, MODES[name], opts, T, weights if name == "learned" else None,
                   guess_model=1 if name == "powerlaw" else 0)
     for _ in range(args.warmup):
         net.reset()
@@ -159,7 +181,7 @@ def train_mlp(trmc, args):
     return W, rec
 
 
-def cpu_baseline(trmc, w, T_sample, dt, fseed):
+def cpu_baseline(trmc, w, T_sample, dt):
     """Times the reference routine (or the oracle restatement when the
     reference build is absent) on one core over member 0's first T_sample
     steps; returns (record, outputs, oracle module)."""
@@ -168,7 +190,7 @@ def cpu_baseline(trmc, w, T_sample, dt, fseed):
     topo = orc.as_topo(w.topo)
     N, R = w.topo.n_segments, w.topo.n_reaches
     p32 = np.ascontiguousarray(w.params, np.float32)
-    q32 = np.ascontiguousarray(trmc.generate_forcing(N, T_sample, 1, fseed)[0], np.float32)
+    q32 = np.ascontiguousarray(w.qlat[0][:T_sample], np.float32)
     q0, d0, r0 = (np.ascontiguousarray(x, np.float32) for x in (w.qdp0, w.dp0, w.reach_last0))
     t0 = time.perf_counter()
     if ref is not None:
@@ -193,7 +215,7 @@ def cpu_baseline(trmc, w, T_sample, dt, fseed):
     return rec, (qo, ro), orc
 
 
-def oracle_sample_check(orc, trmc, w, net, dt, n_sample, fseed):
+def oracle_sample_check(orc, w, net, dt, n_sample):
     """Re-solve a sample of member 0's segments in the first step with the
     FP64 oracle (converged rule) from the GPU's own inputs; returns the max
     relative flow and depth errors."""
@@ -203,7 +225,7 @@ def oracle_sample_check(orc, trmc, w, net, dt, n_sample, fseed):
     q1, d1, _, rl1 = net.state()
     q1, d1, rl1 = q1[0], d1[0], rl1[0]
     q0, d0, rl0 = w.qdp0, w.dp0, w.reach_last0
-    ql = trmc.generate_forcing(topo.n_segments, 1, 1, fseed)[0][0]
+    ql = np.asarray(w.qlat[0][0], np.float64)
     starts, lens = np.asarray(topo.reach_seg_start), np.asarray(topo.reach_seg_len)
     offs = np.arange(int(lens.sum())) - np.repeat(np.cumsum(lens) - lens, lens)
     seg_reach = np.full(topo.n_segments, -1)
@@ -263,7 +285,7 @@ def main():
     head = args.mode or spec["mode"]
     dt = 300.0
     t0 = time.perf_counter()
-    w, desc, fseed = make(trmc, spec)
+    w, desc = make(trmc, spec)
     log(f"[bench] workload {args.workload}: {w.topo.n_segments} segments, {w.topo.n_reaches} reaches, "
         f"{w.topo.n_levels} levels, {M} members, {T} steps (generated in {time.perf_counter() - t0:.1f} s)")
     seg_steps = w.topo.n_segments * T * M
@@ -285,7 +307,8 @@ def main():
     if n_probe > 0:
         t1 = time.perf_counter()
         best, ms = trmc.autotune(w.topo, w.params, w.qdp0, w.dp0, w.reach_last0, n_members=M, n_probe=n_probe,
-                                 forcing_seed=fseed, mlp_weights=weights, mode=MODES[head], schedule=args.schedule)
+                                 qlat=w.qlat[:, :n_probe], mlp_weights=weights, mode=MODES[head],
+                                 schedule=args.schedule)
         for k in ("unit_segments", "block_size", "batch_iters"):
             opts[k] = int(best[k])
         tuned = {"probe_steps": n_probe, "ms_per_probe_step": ms, "seconds": time.perf_counter() - t1,
@@ -295,7 +318,7 @@ def main():
 
     recs, nets, finals = {}, {}, {}
     for name in order:
-        recs[name], nets[name] = time_mode(trmc, w, M, T, dt, name, opts, fseed, args, weights)
+        recs[name], nets[name] = time_mode(trmc, w, M, T, dt, name, opts, args, weights)
         finals[name] = nets[name].state()
         if name not in ("learned", head):
             nets.pop(name).close()     # keep one full forcing in HBM at a time
@@ -331,7 +354,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded generator; lateral inflow generated on and resident in HBM)",
+        "data": "synthetic (seeded generators; lateral inflow resident in HBM)",
         "config": {
             "workload": f"{args.workload}: FP64 MC routing, {desc}, {w.topo.n_segments} segments, "
                         f"{w.topo.n_reaches} reaches, {w.topo.n_levels} reach levels, {T} steps, dt=300 s",
@@ -367,11 +390,11 @@ def main():
     if not args.no_cpu:
         Tc = min(args.cpu_steps or spec["cpu_steps"], T)
         log(f"[bench] cpu baseline: {Tc} steps on one core")
-        rec, (qo, ro), orc = cpu_baseline(trmc, w, Tc, dt, fseed)
+        rec, (qo, ro), orc = cpu_baseline(trmc, w, Tc, dt)
         out["cpu_baseline"] = rec
         rnet = nets.get("reference")
         if rnet is None:
-            rnet = new_net(trmc, w, M, 0, opts, fseed, Tc)
+            rnet = new_net(trmc, w, M, 0, opts, Tc)
             nets["reference"] = rnet
         rnet.reset()
         rnet.run(Tc, dt)
@@ -386,7 +409,7 @@ def main():
         log(f"[bench] cpu {rec['value'] / 1e6:.2f} M seg-steps/s; parity {out['parity']}")
         if "learned" in nets and args.oracle_sample > 0:
             t0 = time.perf_counter()
-            eq, ed, n = oracle_sample_check(orc, trmc, w, nets["learned"], dt, args.oracle_sample, fseed)
+            eq, ed, n = oracle_sample_check(orc, w, nets["learned"], dt, args.oracle_sample)
             out["parity"]["learned_step1_oracle_sample"] = {
                 "segments": n, "max_rel_qdc": eq, "max_rel_depth": ed, "bar": 1e-9}
             log(f"[bench] learned step 1 vs FP64 oracle on {n} segments: max rel q {eq:.3e}, "
