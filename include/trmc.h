@@ -226,8 +226,7 @@ int trmc_network_set_forcing_f32(trmc_network *net, const float *qlat, int n_tim
 /* [new] Synthetic lateral inflow generated on the device, in place of
  * set_forcing: exactly the values trmc_generate_forcing(n_segments,
  * n_timesteps, n_members, seed) returns, without the host array or the copy
- * (a 1M-segment x 16-member x 288-step forcing is 37 GB).  Currently switched
- * off (returns TRMC_ERR_STATE). */
+ * (a 1M-segment x 16-member x 288-step forcing is 37 GB). */
 int trmc_network_generate_forcing(trmc_network *net, int n_timesteps, uint64_t seed);
 
 /* [new] Weights of the LEARNED depth guess (TRMC_MLP_NPARAM floats, layout of

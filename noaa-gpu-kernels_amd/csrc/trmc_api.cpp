@@ -296,6 +296,7 @@ int trmc_network_create(const int *reach_seg_start, const int *reach_seg_len,
     if (o.tol <= 0.0) o.tol = 1e-13;
     if (o.max_iter <= 0) o.max_iter = 60;
     if (o.learned_budget <= 0) o.learned_budget = 12;
+    if (o.guess_model != 0 && o.guess_model != 1) return fail(TRMC_ERR_ARG, "guess_model must be 0 or 1");
     net->M = o.n_members;
 
     std::string err;
@@ -475,15 +476,9 @@ int trmc_network_set_forcing_f32(trmc_network *net, const float *qlat, int n_tim
     return set_forcing_impl(net, qlat, n_timesteps);
 }
 
-static constexpr bool kDeviceForcingEnabled = false;
-
 int trmc_network_generate_forcing(trmc_network *net, int T, uint64_t seed)
 {
     if (!net || T < 0) return fail(TRMC_ERR_ARG, "bad forcing arguments");
-    // Switched off: the one run of a 1M-segment LEARNED pass with forcing from
-    // this path ended in a device memory fault whose cause is not yet found.
-    if (!kDeviceForcingEnabled)
-        return fail(TRMC_ERR_STATE, "device forcing generation is switched off; use set_forcing");
     HIPCHK(hipSetDevice(net->opt.device));
     const Plan &P = net->plan;
     int rc = dev_alloc(&net->d_qlat, (size_t)T * P.n_dev_segments * net->M);

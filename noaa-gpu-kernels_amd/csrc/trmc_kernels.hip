@@ -33,7 +33,7 @@ __device__ __forceinline__ SolveOut solve_one(const RouteArgs &a, double qup, do
     if (!s.start(qup, quc, qdp, ql, c, dp, true))
         while (!s.step(c, k))
             if (s.need_guess)
-                s.set_guess(a.guess_model == 1
+                s.set_guess(a.guess_model != 0
                                 ? mlp_guess_from(s.guess_anchor(), s.guess_flow_new(), s.guess_flow_old(), 0.0f, 0.0f)
                                 : mlp_guess_scalar(a.mlp, s.guess_anchor(), s.guess_flow_new(), s.guess_flow_old(), c));
     return s.finish(c);
@@ -389,7 +389,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 {
     __shared__ float sW[MODE == 2 ? kMlpParams : 1];
     if constexpr (MODE == 2) {
-        for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) sW[i] = a.mlp[i];
+        // the weights exist only for the network guess (guess_model 0)
+        if (a.guess_model == 0)
+            for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) sW[i] = a.mlp[i];
         __syncthreads();
     }
     const long long t_start = (long long)wall_clock64();
@@ -549,7 +551,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         // ------------------------------- LEARNED: guesses for the opened solves
         if constexpr (MODE == 2) {
             const bool want = phase == PH_SOLVE && sr.need_guess;
-            if (a.guess_model == 1) {                // power law alone: no network
+            if (a.guess_model != 0) {                // power law alone: no network
                 if (want)
                     sr.set_guess(mlp_guess_from(sr.guess_anchor(), sr.guess_flow_new(), sr.guess_flow_old(), 0.0f,
                                                 0.0f));

@@ -188,3 +188,11 @@ def test_golden_topology_plans():
     tv = TopoView({k: z[k] for k in z.files})
     for C in (1, 3, 64):
         check_plan(tv.t, trmc.plan(tv.t, C, 288), C)
+
+
+def test_network_rejects_unknown_guess_model():
+    trmc = load_trmc()
+    t = trmc.generate_network(1, 50, 0, 1)
+    with pytest.raises(trmc.TrmcError) as e:
+        trmc.Network(t, mode=trmc.MODE_LEARNED, guess_model=2)
+    assert e.value.code == trmc.ERR_ARG

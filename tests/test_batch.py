@@ -68,8 +68,6 @@ def test_batch_sizes_are_log_uniform_and_seeded():
 
 
 @pytest.mark.gpu
-@pytest.mark.skip(reason="device forcing generation is switched off: a 1M-segment LEARNED run with it ended in a "
-                         "device memory fault whose cause is not yet found")
 def test_device_forcing_equals_host_forcing(gpu):
     T, M = 5, 3
     w = trmc.make_workload(2, 3000, T, n_reaches=900, n_members=M, seed=4)

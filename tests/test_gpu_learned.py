@@ -104,6 +104,16 @@ def test_learned_with_untrained_weights_still_converges(gpu, case):
          f"fallbacks {st['fallbacks']}")
 
 
+def test_power_law_guess_runs_without_weights(gpu, case):
+    """guess_model 1 (the anchored guess with the exponent 3/5 alone) needs
+    no network: no weights are set, and the kernel must not read them."""
+    w, T, want = case
+    (q, d, v, rl), hist, st = _run(w, T, trmc.MODE_LEARNED, None, guess_model=1)
+    close(q[0], want["qdc"], "qdc")
+    close(hist[0], want["reach_hist"], "reach outflow per step")
+    assert st["mlp_calls"] == 0
+
+
 def test_learned_needs_weights(gpu, case):
     w, T, _ = case
     with trmc.Network(w.topo, mode=trmc.MODE_LEARNED) as net:
