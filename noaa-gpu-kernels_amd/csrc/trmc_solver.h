@@ -424,8 +424,9 @@ TRMC_HD double residual(double h, double dt, double qup, double quc, double qdp,
 //                 positive point, mindepth while every F <= 0).
 //   stop:         |step| <= tol h, F zero to within its rounding, or
 //                 bracket width <= tol h.
-//   F(mindepth) <= 0: a geometric scan (x2, visiting bankfull) for a positive
-//                 point; none up to hmax leaves the root at mindepth.
+//   F(mindepth) <= 0: the oracle's geometric scan (x1.25, visiting bankfull)
+//                 for a positive point, then for the next non-positive one;
+//                 none positive leaves the root at mindepth.
 //   bankfull:     a root above bankfull is kept only if F(bankfull) > 0 or
 //                 F(mindepth) <= 0; otherwise [mindepth, bankfull] is solved.
 //   fallback:     LEARNED solves that have not stopped after `budget`
@@ -581,8 +582,9 @@ struct SecantConv {
             return stage == 5;
         case 2: {
             if (lo_pos < 0.0 && hi_neg == 0.01) {   // F(mindepth) <= 0, nothing positive above yet
-                hb = 0.01;                    // scan position (its outflow is not needed:
-                qb = q_cur;                   // the scan's secant steps are expansions)
+                hb = 0.01;                    // scan position; fb, qb stand in for F, outflow there
+                fb = -1.0;                    // (used only once the scan has replaced both points)
+                qb = q_cur;
                 scan_bp = !(c.twcc > 0.0 && c.bfd > 0.01);
                 stage = 6;
                 return false;
@@ -677,17 +679,27 @@ struct SecantConv {
             have_root = true;
             stage = 5;
             return true;
-        case 6: {                             // upward scan (x2, visiting bankfull) for F > 0
-            double hs = hb * 2.0;
+        case 6: {
+            // The oracle's scan (leftmost_downcrossing in mc_oracle.c): x1.25
+            // from mindepth, visiting bankfull, first to a point with F > 0,
+            // then on to the first point with F <= 0; the secant then solves
+            // that bracket.  The search for F > 0 stops early once past
+            // bankfull the Manning flow exceeds every flow the routing can
+            // produce (|qup| + |quc| + |qdp| + 2 |ql|, as |C1|, |C2|, |C3| <= 1
+            // and C4 <= 2 ql): beyond that F < 0 while Qmn keeps rising.
+            double hs = hb * 1.25;
             if (!scan_bp && hs >= c.bfd) { hs = c.bfd; scan_bp = true; }
             if (hs > hmax(c)) { stage = 4; return false; }
             const double fs = eval(hs, c, k);
-            // F(hb) <= 0; its value only steers a secant step below
-            // lo_pos = hs, which the bracket turns into an expansion
-            ha = hb; fa = -1.0;
+            ha = hb; fa = fb;
             hb = hs; fb = fs;
             qa = qb; qb = q_cur;
-            if (fs > 0.0) stage = 2;
+            if (lo_pos > 0.0) {                 // past a positive point: F <= 0 closes the bracket
+                if (!(fs > 0.0)) stage = 2;
+            } else if (!(fs > 0.0) && (!(c.twcc > 0.0 && c.bfd > 0.01) || hs >= 1.25 * c.bfd) &&
+                       qmn_cur > fabs(qup) + fabs(quc) + fabs(qdp) + 2.0 * fabs(ql)) {
+                stage = 4;
+            }
             return false;
         }
         case 7:
