@@ -541,6 +541,26 @@ struct SecantConv {
         return fabs(f) <= 1.6e-14 * (fabs(qup) + fabs(quc) + fabs(qdp) + qmn_cur);
     }
 
+    // A lower bound of Qmn on [h, inf).  In bank Qmn rises with h (area and
+    // hydraulic radius both do), so below bankfull it is Qmn(h) for the rest
+    // of the bank.  Above bankfull, with o = h - bfd: the area A_b + twcc o
+    // rises, R = (A_b + twcc o) / (WP_b + twcc + 2 o) is monotone in o (a
+    // ratio of linear functions), so R >= min(R(o), twcc / 2), and the
+    // blended roughness is at most max(n, ncc).
+    TRMC_HD double qmn_never_below(double h, const Seg &c) const
+    {
+        const bool overbank = c.twcc > 0.0 && c.bfd > 0.01;
+        const double nmax = fmax(c.n_ch, c.ncc);
+        auto bound_above = [&](double o) {
+            const double ab = (c.bw + c.bfd * c.z) * c.bfd, wpb = c.bw + 2.0 * c.bfd * c.sq;
+            const double a = ab + c.twcc * o, r = a / (wpb + c.twcc + 2.0 * o);
+            return a * pow23_53_faithful(fmin(r, 0.5 * c.twcc)).p23 * c.sqs0 / nmax;
+        };
+        if (!overbank) return qmn_cur;              // in bank for every depth
+        if (h <= c.bfd) return fmin(qmn_cur, bound_above(0.0));
+        return bound_above(h - c.bfd);
+    }
+
     TRMC_HD void accept_at(double h, double q, const Seg &c)
     {
         root_h = h;
@@ -561,8 +581,11 @@ struct SecantConv {
             stage = 5;
             return true;
         }
-        if (learned && !need_guess && stage <= 2 && evals >= k.budget) {
-            learned = false;                  // guess did not pay off: reference bracket
+        // a guess that has not led to a bracket (or to the mindepth scan)
+        // within the budget: restart from the reference bracket
+        if (learned && !need_guess && stage <= 2 && evals >= k.budget && !(lo_pos > 0.0 && neg_seen) &&
+            hi_neg != 0.01) {
+            learned = false;
             fell_back = true;
             begin_plain_or_learned(c);
         }
@@ -612,17 +635,16 @@ struct SecantConv {
             // the last one, else it bisects the bracket (geometrically while
             // it spans more than a factor 4), after Brent: a secant pinned to
             // one end of a strongly curved F then still converges.
-            bool bisect = false;
-            if (bracketed && hn == hn && fabs(hn - hb) > 0.5 * s_prev2) bisect = true;
+            // The same rule without a bracket: a one-sided secant that creeps
+            // (every F <= 0, or every F > 0) jumps to mindepth or expands.
+            const bool bisect = hn == hn && fabs(hn - hb) > 0.5 * s_prev2;
             if (bisect || !(hn == hn) || hn <= left || hn >= hi_neg) {
                 if (bracketed) hn = (hi_neg > 4.0 * left) ? sqrt(left * hi_neg) : 0.5 * (left + hi_neg);
                 else if (lo_pos < 0.0) hn = 0.01;
                 else hn = fmin(hm, fmax(ha, hb) * 2.0);
             }
-            if (bracketed) {
-                s_prev2 = s_prev;
-                s_prev = fabs(hn - hb);
-            }
+            s_prev2 = s_prev;
+            s_prev = fabs(hn - hb);
             const double fn = eval(hn, c, k);
             const double stepsz = fabs(hn - hb);
             ha = hb; fa = fb; hb = hn; fb = fn;
@@ -683,10 +705,11 @@ struct SecantConv {
             // The oracle's scan (leftmost_downcrossing in mc_oracle.c): x1.25
             // from mindepth, visiting bankfull, first to a point with F > 0,
             // then on to the first point with F <= 0; the secant then solves
-            // that bracket.  The search for F > 0 stops early once past
-            // bankfull the Manning flow exceeds every flow the routing can
-            // produce (|qup| + |quc| + |qdp| + 2 |ql|, as |C1|, |C2|, |C3| <= 1
-            // and C4 <= 2 ql): beyond that F < 0 while Qmn keeps rising.
+            // that bracket.  The search for F > 0 stops early once Qmn on
+            // [h, inf) provably exceeds every flow the routing can produce,
+            // |qup| + |quc| + |qdp| + 2 |ql| (|C1|, |C2|, |C3| <= 1, C4 <= 2 ql
+            // since D >= dt / 2): F < 0 there, so the oracle finds nothing
+            // above either.
             double hs = hb * 1.25;
             if (!scan_bp && hs >= c.bfd) { hs = c.bfd; scan_bp = true; }
             if (hs > hmax(c)) { stage = 4; return false; }
@@ -696,8 +719,7 @@ struct SecantConv {
             qa = qb; qb = q_cur;
             if (lo_pos > 0.0) {                 // past a positive point: F <= 0 closes the bracket
                 if (!(fs > 0.0)) stage = 2;
-            } else if (!(fs > 0.0) && (!(c.twcc > 0.0 && c.bfd > 0.01) || hs >= 1.25 * c.bfd) &&
-                       qmn_cur > fabs(qup) + fabs(quc) + fabs(qdp) + 2.0 * fabs(ql)) {
+            } else if (!(fs > 0.0) && qmn_never_below(hs, c) > fabs(qup) + fabs(quc) + fabs(qdp) + 2.0 * fabs(ql)) {
                 stage = 4;
             }
             return false;
