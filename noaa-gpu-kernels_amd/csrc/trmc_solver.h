@@ -447,13 +447,17 @@ struct SecantConv {
     double root_h, root_q;
     double s_prev, s_prev2;  // the last two step sizes taken while bracketed
     int evals, stage;
+    double thr;              // F within +-thr is rounding noise (f_at_noise)
     bool learned, need_guess, fell_back, capped, trivial, inbank, have_root, scan_bp, neg_seen;
+    bool lo_clear, hi_clear; // F(lo_pos) > thr, F(hi_neg) < -thr
+    bool conf_below;         // stage 8 looks below the estimate for F > thr
 
     // upper end of the search: bankfull while the in-bank interval is solved
     TRMC_HD double hmax(const Seg &c) const { return inbank ? c.bfd : 1.0e4; }
 
     // stage: 0 eval ha, 1 eval hb, 2 secant, 3 bankfull check, 4 settle at
-    // mindepth, 5 done, 6 scan upward from mindepth, 7 re-evaluate hi_neg
+    // mindepth, 5 done, 6 scan upward from mindepth, 7 re-evaluate hi_neg,
+    // 8 confirm a one-sided root estimate
     TRMC_HD bool start(double qup_, double quc_, double qdp_, double ql_, const Seg &c, double depthp,
                        bool learned_)
     {
@@ -468,6 +472,7 @@ struct SecantConv {
         trivial = seg_invalid(c) || (ql <= 0.0 && qup <= 0.0 && quc <= 0.0 && qdp <= 0.0);
         if (trivial) return true;
         dp = fmax(depthp, 0.0);
+        thr = 1.6e-14 * (fabs(qup) + fabs(quc) + fabs(qdp) + 2.0 * fabs(ql));
         begin_plain_or_learned(c);
         return false;
     }
@@ -478,6 +483,7 @@ struct SecantConv {
         lo_pos = -1.0;
         hi_neg = hm;
         neg_seen = false;
+        lo_clear = hi_clear = false;
         s_prev = s_prev2 = 1.0e30;
         if (learned) {
             ha = fmin(fmax(dp, 0.01), hm);
@@ -524,10 +530,15 @@ struct SecantConv {
         q_cur = outflow(C, qup, quc, qdp);
         qmn_cur = qmn;
         if (f > 0.0) {
-            if (h < hi_neg) lo_pos = fmax(lo_pos, h);
-            else if (lo_pos < 0.0 && hi_neg == 0.01) { lo_pos = h; hi_neg = hmax(c); neg_seen = false; }
+            if (h < hi_neg) {
+                if (h >= lo_pos) { lo_pos = h; lo_clear = f > thr; }
+            } else if (lo_pos < 0.0 && hi_neg == 0.01) {
+                lo_pos = h; lo_clear = f > thr;
+                hi_neg = hmax(c); neg_seen = false;
+            }
         } else if (h <= hi_neg && (h > lo_pos || lo_pos < 0.0)) {
             hi_neg = h;
+            hi_clear = f < -thr;
             neg_seen = true;
         }
         return f;
@@ -536,10 +547,7 @@ struct SecantConv {
     // F at the last point is zero to within its own rounding: the routed sum
     // C1 qup + C2 quc + C3 qdp + C4 and Qmn carry errors of a few ulps of
     // their terms, so no point can be told apart from a root beyond that.
-    TRMC_HD bool f_at_noise(double f) const
-    {
-        return fabs(f) <= 1.6e-14 * (fabs(qup) + fabs(quc) + fabs(qdp) + qmn_cur);
-    }
+    TRMC_HD bool f_at_noise(double f) const { return fabs(f) <= thr; }
 
     // A lower bound of Qmn on [h, inf).  In bank Qmn rises with h (area and
     // hydraulic radius both do), so below bankfull it is Qmn(h) for the rest
@@ -571,6 +579,27 @@ struct SecantConv {
 
     TRMC_HD void accept_root(const Seg &c) { accept_at(h_cur, q_cur, c); }
 
+    // A converged estimate r (outflow q) is a root only where F changes sign
+    // from > 0 to <= 0: accepted inside a two-sided bracket; with one side
+    // only, first confirmed by one evaluation just across r (stage 8) -- a
+    // secant that creeps onto a point where F only touches zero must not
+    // take it.  Returns false when r lies outside a two-sided bracket.
+    TRMC_HD bool converge_at(double r, double q, const Seg &c)
+    {
+        const bool two = lo_pos > 0.0 && neg_seen;
+        if (two && !(r >= lo_pos && r <= hi_neg)) return false;
+        if (two && lo_clear && hi_clear) {
+            accept_at(r, q, c);
+            return true;
+        }
+        // a side known only at noise level (or not at all): look across r
+        conf_below = !(lo_pos > 0.0 && lo_clear);
+        root_h = r;                           // pending until confirmed
+        root_q = q;
+        stage = 8;
+        return true;
+    }
+
     // returns true when the solve is finished (MLP-guess requests leave it running)
     TRMC_HD bool step(const Seg &c, const ConvCfg &k)
     {
@@ -601,7 +630,7 @@ struct SecantConv {
             fb = eval(hb, c, k);
             qb = q_cur;
             stage = 2;
-            if (f_at_noise(fb) && (lo_pos > 0.0 || neg_seen)) accept_root(c);
+            if (f_at_noise(fb) && (lo_pos > 0.0 || neg_seen)) converge_at(hb, qb, c);
             return stage == 5;
         case 2: {
             if (lo_pos < 0.0 && hi_neg == 0.01) {   // F(mindepth) <= 0, nothing positive above yet
@@ -626,8 +655,7 @@ struct SecantConv {
                 !(hn <= lo_pos) && !(neg_seen && hn > hi_neg) && fabs(hn - hb) <= 1e-6 * hb &&
                 fabs(hn - hb) * fabs(hn - ha) <= k.tol * hb * hb) {
                 const double q = (ha != hb) ? qb + (qa - qb) * ((hn - hb) / (ha - hb)) : qb;
-                accept_at(hn, q, c);
-                return stage == 5;
+                if (converge_at(hn, q, c)) return stage == 5;
             }
             const double left = fmax(lo_pos, 0.01);
             const bool bracketed = lo_pos > 0.0 && neg_seen;
@@ -655,11 +683,11 @@ struct SecantConv {
             if ((lo_pos > 0.0 || neg_seen) &&
                 (f_at_noise(fn) || stepsz <= k.tol * hn ||
                  (lo_pos > 0.0 && neg_seen && (hi_neg - lo_pos) <= k.tol * hi_neg))) {
-                if (!(h_cur == hi_neg || h_cur == lo_pos)) {
+                if (lo_pos > 0.0 && neg_seen && !(h_cur == hi_neg || h_cur == lo_pos)) {
                     stage = 7;                // last point outside the bracket: take its right end
                     return false;
                 }
-                accept_root(c);
+                converge_at(h_cur, q_cur, c);
             }
             return stage == 5;
         }
@@ -678,6 +706,8 @@ struct SecantConv {
             lo_pos = -1.0;
             hi_neg = c.bfd;
             neg_seen = true;
+            lo_clear = false;
+            hi_clear = fbf < -thr;
             s_prev = s_prev2 = 1.0e30;
             ha = 0.01;
             hb = c.bfd;
@@ -728,6 +758,19 @@ struct SecantConv {
             eval(hi_neg, c, k);
             accept_root(c);
             return stage == 5;
+        case 8: {                             // confirm a one-sided estimate
+            const double r = root_h, q = root_q;
+            const double hc = conf_below ? r * (1.0 - 1e-10) : r * (1.0 + 1e-10);
+            const double fc = eval(hc, c, k);
+            ha = hb; fa = fb; hb = hc; fb = fc;
+            qa = qb; qb = q_cur;
+            if (conf_below ? fc > thr : fc < -thr) {
+                accept_at(r, q, c);
+                return stage == 5;
+            }
+            stage = 2;                        // no clear sign change there: keep searching
+            return false;
+        }
         }
         return true;
     }
