@@ -336,6 +336,10 @@ def main():
             "segments_beyond_1e-12": int((rel > 1e-12).sum()), "segments": int(rel.size)}
         log(f"[bench] learned vs plain after {T} steps: {recs['learned']['vs_plain_after_all_steps']}")
 
+    # plain streaming copy of 1 GiB: the achievable HBM rate on this box, and
+    # the known-size reference the traffic counters are scaled by
+    copy_gbps = trmc.stream_copy(1 << 30, reps=5)
+    log(f"[bench] 1 GiB device copy: {copy_gbps:.0f} GB/s")
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tpath):
@@ -372,7 +376,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "copy_reference_gbps": copy_gbps,
         },
         "kernel_ms": 1e3 * k_avg_s,
         "evaluations_per_seg_step": hrec["evaluations_per_seg_step"],

@@ -263,6 +263,11 @@ int trmc_network_get_trace(const trmc_network *net, double *us, int *max_iters, 
  * network's member count. */
 int trmc_network_info(const trmc_network *net, int n_timesteps, int64_t *info, int n_info);
 
+/* [new] Streaming device copy of `bytes` (read + write), reps times after
+ * one warm-up; *gbps = 2 bytes reps / time.  The reference for achievable
+ * HBM bandwidth and for scaling memory-traffic counters. */
+int trmc_stream_copy(int device, size_t bytes, int reps, double *gbps);
+
 /* [new] Autotuner for the schedule of a network (or a union of networks):
  * times candidate unit sizes (2, 4, 8 segments), interleaving batch sizes
  * (6, 8, 12) and workgroup sizes (64, 128) on the first n_probe steps of the

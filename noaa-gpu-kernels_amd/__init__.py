@@ -150,6 +150,7 @@ def _declare(L: C.CDLL) -> None:
         "trmc_compute_stats": (I, [pD, pD, I, pD]),
         "trmc_network_set_mlp": (I, [V, pF]),
         "trmc_network_generate_forcing": (I, [V, I, C.c_uint64]),
+        "trmc_stream_copy": (I, [I, C.c_size_t, I, pD]),
         "trmc_topo_concat": (I, [_p(Topo), I, _p(Topo), pI, pI]),
         "trmc_autotune": (I, topo + [_p(ChParamsF64), pD, pD, pD, pD, C.c_uint64, I, D, pF, _p(Options),
                                      _p(Options), pD]),
@@ -450,6 +451,13 @@ def default_options(**kw) -> Options:
             raise TrmcError(ERR_ARG, f"unknown option {k}")
         setattr(o, k, v)
     return o
+
+
+def stream_copy(n_bytes: int, reps: int = 5, device: int = 0) -> float:
+    """GB/s of a plain device copy of n_bytes (read + write counted)."""
+    g = C.c_double()
+    _check(lib().trmc_stream_copy(int(device), int(n_bytes), int(reps), C.byref(g)))
+    return g.value
 
 
 def synchronize(device: int = 0) -> None:

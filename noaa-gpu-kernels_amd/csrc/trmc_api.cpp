@@ -955,6 +955,42 @@ int trmc_mc_solve_f64(const trmc_chparams_f64 *params, const double *in, double 
     return TRMC_OK;
 }
 
+// -------------------------------------------------------- copy reference
+int trmc_stream_copy(int device, size_t bytes, int reps, double *gbps)
+{
+    if (bytes < 16 || reps <= 0) return fail(TRMC_ERR_ARG, "need bytes >= 16 and reps > 0");
+    int cu = 0;
+    double khz = 0.0;
+    int rc = check_device(device, &cu, &khz);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(device));
+    bytes &= ~(size_t)15;
+    void *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess) return fail(TRMC_ERR_NOMEM, "hipMalloc of %zu bytes failed", bytes);
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(TRMC_ERR_NOMEM, "hipMalloc of %zu bytes failed", bytes);
+    }
+    hipEvent_t e0, e1;
+    hipError_t e = hipMemset(a, 0, bytes);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = launch_stream_copy(a, b, bytes, nullptr);      // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_stream_copy(a, b, bytes, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(TRMC_ERR_HIP, "stream copy: %s", hipGetErrorString(e));
+    if (gbps) *gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    return TRMC_OK;
+}
+
 // ------------------------------------------------------------- autotuner
 // Candidate schedules are timed on the first n_probe steps (after one warm-up
 // run of the same steps): unit size (one network each), then batch size and

@@ -74,6 +74,7 @@ struct RouteArgs {
 };
 
 // launchers (trmc_kernels.hip); all asynchronous on `stream`
+hipError_t launch_stream_copy(const void *src, void *dst, size_t bytes, hipStream_t stream);
 hipError_t launch_gen_forcing(double *dst, const int *seg_of_dev, int nd, int N, int M, int T,
                               unsigned long long seed, hipStream_t stream);
 hipError_t launch_reset(unsigned *bar, int nwords, hipStream_t stream);

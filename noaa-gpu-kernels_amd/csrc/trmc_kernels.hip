@@ -731,7 +731,24 @@ __global__ void k_gen_forcing(double *__restrict__ dst, const int *__restrict__ 
     }
 }
 
+// Plain streaming copy (16-byte vectors, grid-stride): the reference point
+// for HBM bandwidth and for scaling the memory-traffic counters.
+__global__ void __launch_bounds__(256) k_stream_copy(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                    long long n)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 // ----------------------------------------------------------------- launchers
+
+hipError_t launch_stream_copy(const void *src, void *dst, size_t bytes, hipStream_t stream)
+{
+    const long long n = (long long)(bytes / sizeof(double2));
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stream_copy, dim3(4096), dim3(256), 0, stream, (const double2 *)src, (double2 *)dst, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_gen_forcing(double *dst, const int *seg_of_dev, int nd, int N, int M, int T,
                               unsigned long long seed, hipStream_t stream)
