@@ -447,17 +447,13 @@ struct SecantConv {
     double root_h, root_q;
     double s_prev, s_prev2;  // the last two step sizes taken while bracketed
     int evals, stage;
-    double thr;              // F within +-thr is rounding noise (f_at_noise)
     bool learned, need_guess, fell_back, capped, trivial, inbank, have_root, scan_bp, neg_seen;
-    bool lo_clear, hi_clear; // F(lo_pos) > thr, F(hi_neg) < -thr
-    bool conf_below;         // stage 8 looks below the estimate for F > thr
 
     // upper end of the search: bankfull while the in-bank interval is solved
     TRMC_HD double hmax(const Seg &c) const { return inbank ? c.bfd : 1.0e4; }
 
     // stage: 0 eval ha, 1 eval hb, 2 secant, 3 bankfull check, 4 settle at
-    // mindepth, 5 done, 6 scan upward from mindepth, 7 re-evaluate hi_neg,
-    // 8 confirm a one-sided root estimate, 9 F(mindepth) for the in-bank search
+    // mindepth, 5 done, 6 scan upward from mindepth, 7 re-evaluate hi_neg
     TRMC_HD bool start(double qup_, double quc_, double qdp_, double ql_, const Seg &c, double depthp,
                        bool learned_)
     {
@@ -472,7 +468,6 @@ struct SecantConv {
         trivial = seg_invalid(c) || (ql <= 0.0 && qup <= 0.0 && quc <= 0.0 && qdp <= 0.0);
         if (trivial) return true;
         dp = fmax(depthp, 0.0);
-        thr = 1.6e-14 * (fabs(qup) + fabs(quc) + fabs(qdp) + 2.0 * fabs(ql));
         begin_plain_or_learned(c);
         return false;
     }
@@ -483,7 +478,6 @@ struct SecantConv {
         lo_pos = -1.0;
         hi_neg = hm;
         neg_seen = false;
-        lo_clear = hi_clear = false;
         s_prev = s_prev2 = 1.0e30;
         if (learned) {
             ha = fmin(fmax(dp, 0.01), hm);
@@ -530,15 +524,10 @@ struct SecantConv {
         q_cur = outflow(C, qup, quc, qdp);
         qmn_cur = qmn;
         if (f > 0.0) {
-            if (h < hi_neg) {
-                if (h >= lo_pos) { lo_pos = h; lo_clear = f > thr; }
-            } else if (lo_pos < 0.0 && hi_neg == 0.01) {
-                lo_pos = h; lo_clear = f > thr;
-                hi_neg = hmax(c); neg_seen = false;
-            }
+            if (h < hi_neg) lo_pos = fmax(lo_pos, h);
+            else if (lo_pos < 0.0 && hi_neg == 0.01) { lo_pos = h; hi_neg = hmax(c); neg_seen = false; }
         } else if (h <= hi_neg && (h > lo_pos || lo_pos < 0.0)) {
             hi_neg = h;
-            hi_clear = f < -thr;
             neg_seen = true;
         }
         return f;
@@ -547,7 +536,10 @@ struct SecantConv {
     // F at the last point is zero to within its own rounding: the routed sum
     // C1 qup + C2 quc + C3 qdp + C4 and Qmn carry errors of a few ulps of
     // their terms, so no point can be told apart from a root beyond that.
-    TRMC_HD bool f_at_noise(double f) const { return fabs(f) <= thr; }
+    TRMC_HD bool f_at_noise(double f) const
+    {
+        return fabs(f) <= 1.6e-14 * (fabs(qup) + fabs(quc) + fabs(qdp) + qmn_cur);
+    }
 
     // A lower bound of Qmn on [h, inf).  In bank Qmn rises with h (area and
     // hydraulic radius both do), so below bankfull it is Qmn(h) for the rest
@@ -579,32 +571,7 @@ struct SecantConv {
 
     TRMC_HD void accept_root(const Seg &c) { accept_at(h_cur, q_cur, c); }
 
-    // A converged estimate r (outflow q) is a root only where F changes sign
-    // from > 0 to <= 0: accepted inside a two-sided bracket; with one side
-    // only, first confirmed by one evaluation just across r (stage 8) -- a
-    // secant that creeps onto a point where F only touches zero must not
-    // take it.  Returns false when r lies outside a two-sided bracket.
-    TRMC_HD bool converge_at(double r, double q, const Seg &c)
-    {
-        const bool two = lo_pos > 0.0 && neg_seen;
-        if (two && !(r >= lo_pos && r <= hi_neg)) return false;
-        if (two && lo_clear && hi_clear) {
-            accept_at(r, q, c);
-            return true;
-        }
-        // a side known only at noise level (or not at all): look across r
-        conf_below = !(lo_pos > 0.0 && lo_clear);
-        root_h = r;                           // pending until confirmed
-        root_q = q;
-        stage = 8;
-        return true;
-    }
-
     // returns true when the solve is finished (MLP-guess requests leave it running)
-    // One iteration: pick the stage's next point (lane-divergent but cheap),
-    // evaluate F there at one shared call site (so lanes of a wave in
-    // different stages evaluate together), then consume the value.  Returns
-    // true when the solve is finished (an MLP-guess request leaves it running).
     TRMC_HD bool step(const Seg &c, const ConvCfg &k)
     {
         if (stage == 5) return true;
@@ -622,195 +589,147 @@ struct SecantConv {
             fell_back = true;
             begin_plain_or_learned(c);
         }
-        if (stage == 1 && need_guess) return false;   // waiting for the wave's MLP call
-        double h = 0.0;
-        if (!next_point(c, k, h)) return stage == 5;
-        const double f = eval(h, c, k);
-        consume(f, c, k);
-        return stage == 5;
-    }
-
-    // the point the current stage evaluates next; false when the stage ends
-    // the solve (or waits) without an evaluation
-    TRMC_HD bool next_point(const Seg &c, const ConvCfg &k, double &h)
-    {
-        for (;;) {
-            switch (stage) {
-            case 0: h = ha; return true;
-            case 1: h = hb; return true;
-            case 2: {
-                if (lo_pos < 0.0 && hi_neg == 0.01) {   // F(mindepth) <= 0, nothing positive above yet
-                    hb = 0.01;                // scan position; fb, qb stand in for F, outflow there
-                    fb = -1.0;                // (used only once the scan has replaced both points)
-                    qb = q_cur;
-                    scan_bp = !(c.twcc > 0.0 && c.bfd > 0.01);
-                    stage = 6;
-                    continue;
-                }
-                const double hm = hmax(c);
-                const double den = fa - fb;
-                double hn = (den != 0.0) ? hb - fb * (ha - hb) / den : 0.5 * (ha + hb);
-                // Converged without another evaluation: the secant step points
-                // to a downcrossing next to the point just evaluated (toward
-                // lower h when F <= 0 there, higher when F > 0), inside the
-                // bracket, and the secant error estimate |hn - hb| |hn - ha| /
-                // h^2 is within tol.  The root is the prediction hn; its outflow
-                // is the secant interpolation of the outflows at ha and hb (an
-                // error of the same second order).
-                if (h_cur == hb && hn == hn && (fb > 0.0 ? hn >= hb : hn <= hb) && (lo_pos > 0.0 || neg_seen) &&
-                    !(hn <= lo_pos) && !(neg_seen && hn > hi_neg) && fabs(hn - hb) <= 1e-6 * hb &&
-                    fabs(hn - hb) * fabs(hn - ha) <= k.tol * hb * hb) {
-                    const double q = (ha != hb) ? qb + (qa - qb) * ((hn - hb) / (ha - hb)) : qb;
-                    if (converge_at(hn, q, c)) {
-                        if (stage == 5) return false;
-                        continue;             // stage 8: confirm
-                    }
-                }
-                const double left = fmax(lo_pos, 0.01);
-                const bool bracketed = lo_pos > 0.0 && neg_seen;
-                // Once bracketed, every step must be under half the step before
-                // the last one, else it bisects the bracket (geometrically while
-                // it spans more than a factor 4), after Brent: a secant pinned to
-                // one end of a strongly curved F then still converges.  The same
-                // rule without a bracket: a one-sided secant that creeps (every
-                // F <= 0, or every F > 0) jumps to mindepth or expands.
-                const bool bisect = hn == hn && fabs(hn - hb) > 0.5 * s_prev2;
-                if (bisect || !(hn == hn) || hn <= left || hn >= hi_neg) {
-                    if (bracketed) hn = (hi_neg > 4.0 * left) ? sqrt(left * hi_neg) : 0.5 * (left + hi_neg);
-                    else if (lo_pos < 0.0) hn = 0.01;
-                    else hn = fmin(hm, fmax(ha, hb) * 2.0);
-                }
-                s_prev2 = s_prev;
-                s_prev = fabs(hn - hb);
-                h = hn;
-                return true;
-            }
-            case 3: h = c.bfd; return true;   // a downcrossing below bankfull?
-            case 4: h = 0.01; return true;    // settle at mindepth
-            case 6: {
-                // The oracle's scan (leftmost_downcrossing in mc_oracle.c): x1.25
-                // from mindepth, visiting bankfull, first to a point with F > 0,
-                // then on to the first point with F <= 0; the secant then solves
-                // that bracket.
-                double hs = hb * 1.25;
-                if (!scan_bp && hs >= c.bfd) { hs = c.bfd; scan_bp = true; }
-                if (hs > hmax(c)) { stage = 4; continue; }
-                h = hs;
-                return true;
-            }
-            case 7: h = hi_neg; return true;  // the bracket's right end, for its outflow
-            case 8:                           // confirm a one-sided estimate across it
-                h = conf_below ? root_h * (1.0 - 1e-10) : root_h * (1.0 + 1e-10);
-                return true;
-            case 9: h = 0.01; return true;    // F(mindepth) before the in-bank search
-            default: return false;
-            }
-        }
-    }
-
-    // use F(h_cur) = f according to the stage that asked for it
-    TRMC_HD void consume(double f, const Seg &c, const ConvCfg &k)
-    {
         switch (stage) {
         case 0:
-            fa = f;
+            fa = eval(ha, c, k);
             qa = q_cur;
             if (learned) need_guess = true;
             stage = 1;
-            return;
+            return false;
         case 1:
-            fb = f;
+            if (need_guess) return false;     // waiting for the wave's MLP call
+            fb = eval(hb, c, k);
             qb = q_cur;
             stage = 2;
-            if (f_at_noise(fb) && (lo_pos > 0.0 || neg_seen)) converge_at(hb, qb, c);
-            return;
+            if (f_at_noise(fb) && (lo_pos > 0.0 || neg_seen)) accept_root(c);
+            return stage == 5;
         case 2: {
-            const double hn = h_cur;
+            if (lo_pos < 0.0 && hi_neg == 0.01) {   // F(mindepth) <= 0, nothing positive above yet
+                hb = 0.01;                    // scan position; fb, qb stand in for F, outflow there
+                fb = -1.0;                    // (used only once the scan has replaced both points)
+                qb = q_cur;
+                scan_bp = !(c.twcc > 0.0 && c.bfd > 0.01);
+                stage = 6;
+                return false;
+            }
+            const double hm = hmax(c);
+            const double den = fa - fb;
+            double hn = (den != 0.0) ? hb - fb * (ha - hb) / den : 0.5 * (ha + hb);
+            // Converged without another evaluation: the secant step points
+            // to a downcrossing next to the point just evaluated (toward lower
+            // h when F <= 0 there, higher when F > 0), inside the bracket, and
+            // the secant error estimate |hn - hb| |hn - ha| / h^2 is within
+            // tol.  The root is the prediction hn; its outflow is the secant
+            // interpolation of the outflows at ha and hb (an error of the
+            // same second order).
+            if (h_cur == hb && hn == hn && (fb > 0.0 ? hn >= hb : hn <= hb) && (lo_pos > 0.0 || neg_seen) &&
+                !(hn <= lo_pos) && !(neg_seen && hn > hi_neg) && fabs(hn - hb) <= 1e-6 * hb &&
+                fabs(hn - hb) * fabs(hn - ha) <= k.tol * hb * hb) {
+                const double q = (ha != hb) ? qb + (qa - qb) * ((hn - hb) / (ha - hb)) : qb;
+                accept_at(hn, q, c);
+                return stage == 5;
+            }
+            const double left = fmax(lo_pos, 0.01);
+            const bool bracketed = lo_pos > 0.0 && neg_seen;
+            // Once bracketed, every step must be under half the step before
+            // the last one, else it bisects the bracket (geometrically while
+            // it spans more than a factor 4), after Brent: a secant pinned to
+            // one end of a strongly curved F then still converges.
+            // The same rule without a bracket: a one-sided secant that creeps
+            // (every F <= 0, or every F > 0) jumps to mindepth or expands.
+            const bool bisect = hn == hn && fabs(hn - hb) > 0.5 * s_prev2;
+            if (bisect || !(hn == hn) || hn <= left || hn >= hi_neg) {
+                if (bracketed) hn = (hi_neg > 4.0 * left) ? sqrt(left * hi_neg) : 0.5 * (left + hi_neg);
+                else if (lo_pos < 0.0) hn = 0.01;
+                else hn = fmin(hm, fmax(ha, hb) * 2.0);
+            }
+            s_prev2 = s_prev;
+            s_prev = fabs(hn - hb);
+            const double fn = eval(hn, c, k);
             const double stepsz = fabs(hn - hb);
-            ha = hb; fa = fb; hb = hn; fb = f;
+            ha = hb; fa = fb; hb = hn; fb = fn;
             qa = qb; qb = q_cur;
             // a converged step from either side of the bracket: points with
             // F <= 0 only approach from above (hn < hi_neg), so this is a
             // downcrossing too
             if ((lo_pos > 0.0 || neg_seen) &&
-                (f_at_noise(f) || stepsz <= k.tol * hn ||
+                (f_at_noise(fn) || stepsz <= k.tol * hn ||
                  (lo_pos > 0.0 && neg_seen && (hi_neg - lo_pos) <= k.tol * hi_neg))) {
-                if (lo_pos > 0.0 && neg_seen && !(h_cur == hi_neg || h_cur == lo_pos)) {
+                if (!(h_cur == hi_neg || h_cur == lo_pos)) {
                     stage = 7;                // last point outside the bracket: take its right end
-                    return;
+                    return false;
                 }
-                converge_at(h_cur, q_cur, c);
+                accept_root(c);
             }
-            return;
+            return stage == 5;
         }
-        case 3:
-            // root_h / root_q hold the overbank root found
+        case 3: {                             // a downcrossing below bankfull?
+            const double rh = root_h, rq = root_q;
+            const double fbf = eval(c.bfd, c, k);
+            const double qbf = q_cur;
             inbank = true;
-            if (f > 0.0) {                    // no upcrossing between: keep the root
+            if (fbf > 0.0) {                  // no upcrossing between: keep the root
+                root_h = rh;
+                root_q = rq;
                 stage = 5;
-                return;
+                return true;
             }
             // solve [mindepth, bankfull] unless F(mindepth) <= 0
             lo_pos = -1.0;
             hi_neg = c.bfd;
             neg_seen = true;
-            lo_clear = false;
-            hi_clear = f < -thr;
             s_prev = s_prev2 = 1.0e30;
-            hb = c.bfd;
-            fb = f;
-            qb = q_cur;
-            stage = 9;
-            return;
-        case 9:
             ha = 0.01;
-            fa = f;
+            hb = c.bfd;
+            fb = fbf;
+            qb = qbf;
+            fa = eval(ha, c, k);
             qa = q_cur;
-            if (!(f > 0.0)) {                 // keep the overbank root
+            if (!(fa > 0.0)) {
+                root_h = rh;
+                root_q = rq;
                 stage = 5;
-                return;
+                return true;
             }
             stage = 2;
-            return;
+            return false;
+        }
         case 4:
+            eval(0.01, c, k);
             root_h = 0.01;
             root_q = q_cur;
             have_root = true;
             stage = 5;
-            return;
+            return true;
         case 6: {
-            const double hs = h_cur;
+            // The oracle's scan (leftmost_downcrossing in mc_oracle.c): x1.25
+            // from mindepth, visiting bankfull, first to a point with F > 0,
+            // then on to the first point with F <= 0; the secant then solves
+            // that bracket.  The search for F > 0 stops early once Qmn on
+            // [h, inf) provably exceeds every flow the routing can produce,
+            // |qup| + |quc| + |qdp| + 2 |ql| (|C1|, |C2|, |C3| <= 1, C4 <= 2 ql
+            // since D >= dt / 2): F < 0 there, so the oracle finds nothing
+            // above either.
+            double hs = hb * 1.25;
+            if (!scan_bp && hs >= c.bfd) { hs = c.bfd; scan_bp = true; }
+            if (hs > hmax(c)) { stage = 4; return false; }
+            const double fs = eval(hs, c, k);
             ha = hb; fa = fb;
-            hb = hs; fb = f;
+            hb = hs; fb = fs;
             qa = qb; qb = q_cur;
-            // The search for F > 0 stops early once Qmn on [h, inf) provably
-            // exceeds every flow the routing can produce, |qup| + |quc| + |qdp|
-            // + 2 |ql| (|C1|, |C2|, |C3| <= 1, C4 <= 2 ql since D >= dt / 2):
-            // F < 0 there, so the oracle finds nothing above either.
-            if (lo_pos > 0.0) {               // past a positive point: F <= 0 closes the bracket
-                if (!(f > 0.0)) stage = 2;
-            } else if (!(f > 0.0) && qmn_never_below(hs, c) > fabs(qup) + fabs(quc) + fabs(qdp) + 2.0 * fabs(ql)) {
+            if (lo_pos > 0.0) {                 // past a positive point: F <= 0 closes the bracket
+                if (!(fs > 0.0)) stage = 2;
+            } else if (!(fs > 0.0) && qmn_never_below(hs, c) > fabs(qup) + fabs(quc) + fabs(qdp) + 2.0 * fabs(ql)) {
                 stage = 4;
             }
-            return;
+            return false;
         }
         case 7:
+            eval(hi_neg, c, k);
             accept_root(c);
-            return;
-        case 8: {
-            const double r = root_h, q = root_q;
-            ha = hb; fa = fb; hb = h_cur; fb = f;
-            qa = qb; qb = q_cur;
-            if (conf_below ? f > thr : f < -thr) {
-                accept_at(r, q, c);
-                return;
-            }
-            stage = 2;                        // no clear sign change there: keep searching
-            return;
+            return stage == 5;
         }
-        default:
-            return;
-        }
+        return true;
     }
 
     TRMC_HD SolveOut finish(const Seg &c) const
