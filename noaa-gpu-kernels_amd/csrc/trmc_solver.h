@@ -102,9 +102,13 @@ TRMC_HD Pow2353 pow23_53(double x)
     return r;
 }
 
-// x^(2/3) and x^(5/3) from one cube root, each within a few ulps (no
-// correctly rounded finish): the CONVERGED/LEARNED residual, whose root is
-// wanted to 1e-9, not bit for bit.
+// x^(2/3) and x^(5/3) to within a few ulps (no correctly rounded finish): the
+// CONVERGED/LEARNED residual and velocity, whose root is wanted to 1e-9, not
+// bit for bit.  On the device, for x in [2^-100, 2^100]: s = x^(-1/3) from
+// the FP32 hardware log2/exp2 (relative error < 3e-6), then two Newton steps
+// s <- s + (s/3)(1 - x s^3), each of which squares the relative error
+// (e -> -2 e^2: 3e-6 -> 2e-11 -> 1e-21); x^(2/3) = x s, x^(5/3) = x^(2/3) x.
+// Elsewhere (and in host builds) through cbrt.
 TRMC_HD Pow2353 pow23_53_faithful(double x)
 {
     Pow2353 r;
@@ -114,10 +118,41 @@ TRMC_HD Pow2353 pow23_53_faithful(double x)
         r.p53 = v;
         return r;
     }
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (x >= 0x1p-100 && x <= 0x1p100) {
+        double s = (double)__builtin_amdgcn_exp2f(-(1.0f / 3.0f) * __builtin_amdgcn_logf((float)x));
+        double e = fma(-x * s, s * s, 1.0);
+        s = fma(s * (1.0 / 3.0), e, s);
+        e = fma(-x * s, s * s, 1.0);
+        s = fma(s * (1.0 / 3.0), e, s);
+        r.p23 = x * s;
+        r.p53 = r.p23 * x;
+        return r;
+    }
+#endif
     const double cr = cbrt(x);
     r.p23 = cr * cr;
     r.p53 = r.p23 * x;
     return r;
+}
+
+// 1/x for the CONVERGED/LEARNED residual, within about an ulp.  On the device,
+// for x in [2^-1000, 2^1000]: the hardware reciprocal estimate refined by two
+// Newton steps r <- r + r (1 - x r) (each squares the relative error); other
+// x (and host builds) by division.
+TRMC_HD double recip(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double ax = fabs(x);
+    if (ax >= 0x1p-1000 && ax <= 0x1p1000) {
+        double r = __builtin_amdgcn_rcp(x);
+        double e = fma(-x, r, 1.0);
+        r = fma(r, e, r);
+        e = fma(-x, r, 1.0);
+        return fma(r, e, r);
+    }
+#endif
+    return 1.0 / x;
 }
 
 struct Geo { double twl, R, A, AC, WP, WPC; };
@@ -195,14 +230,16 @@ TRMC_HD double outflow(const Coef &m, double qup, double quc, double qdp)
     return qmc;
 }
 
-// wavefront_mc.cu:330-336
+// wavefront_mc.cu:330-336; FAITHFUL (the CONVERGED/LEARNED output) takes
+// R^(2/3) within a few ulps instead of correctly rounded
+template <bool FAITHFUL = false>
 TRMC_HD double velocity(double h, const Seg &c)
 {
     double twl = c.bw + 2.0 * c.z * h;
     double Rv = (h * (c.bw + twl) / 2.0) /
                 (c.bw + 2.0 * sqrt(((twl - c.bw) / 2.0) * ((twl - c.bw) / 2.0) + h * h));
     if (Rv < 0.0) Rv = 0.0;
-    return (1.0 / c.n_ch) * pow23_53(Rv).p23 * c.sqs0;
+    return (1.0 / c.n_ch) * (FAITHFUL ? pow23_53_faithful(Rv) : pow23_53(Rv)).p23 * c.sqs0;
 }
 
 TRMC_HD bool seg_invalid(const Seg &c)
@@ -391,21 +428,41 @@ struct SecantRef {
 // ------------------------------------------------------------- CONVERGED
 // F(h) with the X0 = 0.5 coefficients feeding the weighting X(h); also returns
 // the final coefficients at h for the outflow rule and the Manning flow Qmn(h).
+// The reference's formulas (wavefront_mc.cu:249-328) with each shared
+// denominator inverted once (recip) and the powers from pow23_53_faithful:
+// F moves by a few ulps of its terms, its root by far less than the 1e-9
+// the CONVERGED/LEARNED outputs are held to.
 TRMC_HD double residual(double h, double dt, double qup, double quc, double qdp,
                                            double ql, const Seg &c, Coef &C, double &qmn)
 {
-    Geo g = geometry(h, c);
+    const Geo g = geometry(h, c);
     const Pow2353 w = pow23_53_faithful(g.R);
     const double r23 = w.p23, r53 = w.p53;
-    double ck = celerity(h, r23, r53, c);
-    double km = storage_km(ck, dt, c);
-    Coef m0 = muskingum(km, 0.5, dt, ql);
-    double q0 = m0.c1 * qup + m0.c2 * quc + m0.c3 * qdp + m0.c4;
-    double X = weighting(ck, weight_width(h, g, c), q0, 0.25, c);
-    C = muskingum(km, X, dt, ql);
-    double routed = C.c1 * qup + C.c2 * quc + C.c3 * qdp;
+    // celerity (wavefront_mc.cu:249-254) and storage constant
+    double ck = 0.0;
+    if (h > 0.0)
+        ck = fmax(0.0, c.ks * ((5.0 / 3.0) * r23 - (2.0 / 3.0) * r53 * (2.0 * c.sq * recip(c.bw + 2.0 * h * c.z))));
+    const double km = (ck > 0.0) ? fmax(dt, c.dx * recip(ck)) : dt;
+    const double hdt = 0.5 * dt;
+    // X0 = 0.5 routing, then the weighting X from it (lower limit 0.25)
+    const double k5 = 0.5 * km;
+    const double q0 = ((k5 + hdt) * qup + (hdt - k5) * quc + (k5 - hdt) * qdp + ql * dt) * recip(k5 + hdt);
+    const double twu = weight_width(h, g, c);
+    const double den = twu * c.s0 * ck * c.dx;
+    double X = 0.5;
+    if (ck > 0.0 && den > 0.0) X = fmin(0.5, fmax(0.25, 0.5 * (1.0 - q0 * recip(2.0 * den))));
+    double D = km * (1.0 - X) + hdt;
+    if (D == 0.0) D = 1.0;
+    const double iD = recip(D);
+    C.c1 = (km * X + hdt) * iD;
+    C.c2 = (hdt - km * X) * iD;
+    C.c3 = (km * (1.0 - X) - hdt) * iD;
+    C.c4 = (ql * dt) * iD;
+    const double routed = C.c1 * qup + C.c2 * quc + C.c3 * qdp;
     if (C.c4 < 0.0 && fabs(C.c4) > routed) C.c4 = -routed;
-    qmn = manning(g, r23, c);
+    // Manning flow (wavefront_mc.cu:267-270)
+    const double wp = g.WP + g.WPC;
+    qmn = (wp > 0.0) ? (g.A + g.AC) * r23 * c.sqs0 * wp * recip(g.WP * c.n_ch + g.WPC * c.ncc) : 0.0;
     return (routed + C.c4) - qmn;
 }
 
@@ -739,7 +796,7 @@ struct SecantConv {
         o.flags = (capped ? 1 : 0) | (fell_back ? 2 : 0);
         if (trivial) { o.q = 0.0; o.depth = 0.0; o.vel = 0.0; return o; }
         o.depth = root_h;
-        o.vel = velocity(root_h, c);
+        o.vel = velocity<true>(root_h, c);
         o.q = root_q;
         return o;
     }

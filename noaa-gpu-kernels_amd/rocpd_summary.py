@@ -5,7 +5,10 @@ schema); its --stats CSV is not always written (for instance when the tool
 itself exits abnormally after the program finished).  This prints the same
 per-kernel statistics from the database:
 
-    python noaa-gpu-kernels_amd/rocpd_summary.py gpurun_out/prof/run_results.db
+    python noaa-gpu-kernels_amd/rocpd_summary.py gpurun_out/prof/run_results.db [--dispatches NAME]
+
+--dispatches NAME also lists every dispatch of the kernels whose name contains
+NAME in launch order (one template instance can serve several modes of a run).
 """
 import sqlite3
 import sys
@@ -24,5 +27,18 @@ def summarize(path: str) -> str:
     return "\n".join(out)
 
 
+def dispatches(path: str, name: str) -> str:
+    db = sqlite3.connect(path)
+    rows = db.execute("select name, start, duration from kernels where name like ? order by start",
+                      (f"%{name}%",)).fetchall()
+    out = [f"{'#':>4} {'duration_us':>14}  kernel"]
+    for i, (n, _, d) in enumerate(rows):
+        out.append(f"{i:>4} {d / 1e3:>14.1f}  {n[:110]}")
+    return "\n".join(out)
+
+
 if __name__ == "__main__":
     print(summarize(sys.argv[1]))
+    if len(sys.argv) > 3 and sys.argv[2] == "--dispatches":
+        print()
+        print(dispatches(sys.argv[1], sys.argv[3]))
