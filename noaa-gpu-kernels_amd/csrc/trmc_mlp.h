@@ -87,12 +87,15 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 //            H1^T[16 mi + 4 (l/16) + r][l%16], so A = W2[16 mo + l%16][16 mi + 4 (l/16) + r]
 //   layer 3: y = W3 . H2^T + b3, partial dot per lane, summed over the four
 //            lane groups that hold the same sample column.
-__device__ __forceinline__ float mlp_forward_wave(const float *W, const float *x)
+// Only the first `tiles` column tiles (wave-uniform, 1..4) are evaluated:
+// lanes 16 tiles .. 63 get an unspecified output.
+__device__ __forceinline__ float mlp_forward_wave(const float *W, const float *x, int tiles = 4)
 {
     const int l = threadIdx.x & 63, li = l & 15, lg = l >> 4;
     float mine = 0.0f;
 #pragma unroll
     for (int nj = 0; nj < 4; ++nj) {
+        if (nj >= tiles) break;
         const int src = 16 * nj + li;
         float xs[kMlpIn];
 #pragma unroll
@@ -188,7 +191,9 @@ __device__ __forceinline__ int mlp_wave_pairs(const float *W, bool want, const f
             const float vn = __shfl(xn[f], src, 64), vo = __shfl(xo[f], src, 64);
             x[f] = (lane & 1) ? vo : vn;
         }
-        const float y = mlp_forward_wave(W, x);
+        // samples sit in lanes 0 .. 2 cnt - 1: only their tiles are run
+        const int cnt = (nwant - base < 32) ? nwant - base : 32;
+        const float y = mlp_forward_wave(W, x, (2 * cnt + 15) >> 4);
         const int r = rank - base;
         const bool mine = want && r >= 0 && r < 32;
         const int at = mine ? 2 * r : 0;
