@@ -510,8 +510,7 @@ struct SecantConv {
     TRMC_HD double hmax(const Seg &c) const { return inbank ? c.bfd : 1.0e4; }
 
     // stage: 0 eval ha, 1 eval hb, 2 secant, 3 bankfull check, 4 settle at
-    // mindepth, 5 done, 6 scan upward from mindepth, 7 re-evaluate hi_neg,
-    // 9 F(mindepth) for the in-bank search after a bankfull check
+    // mindepth, 5 done, 6 scan upward from mindepth, 7 re-evaluate hi_neg
     TRMC_HD bool start(double qup_, double quc_, double qdp_, double ql_, const Seg &c, double depthp,
                        bool learned_)
     {
@@ -629,11 +628,7 @@ struct SecantConv {
 
     TRMC_HD void accept_root(const Seg &c) { accept_at(h_cur, q_cur, c); }
 
-    // One step: the stage's transitions that need no evaluation, then exactly
-    // one evaluation of F, at one call site shared by every stage (lanes of a
-    // wave in different stages evaluate together), then its use by the stage.
-    // Returns true when the solve is finished (an MLP-guess request leaves it
-    // running).
+    // returns true when the solve is finished (MLP-guess requests leave it running)
     TRMC_HD bool step(const Seg &c, const ConvCfg &k)
     {
         if (stage == 5) return true;
@@ -651,152 +646,147 @@ struct SecantConv {
             fell_back = true;
             begin_plain_or_learned(c);
         }
-        if (need_guess) return false;         // stage 1 waits for the wave's MLP call
-        double h = 0.0;
-        if (stage == 2) {
+        switch (stage) {
+        case 0:
+            fa = eval(ha, c, k);
+            qa = q_cur;
+            if (learned) need_guess = true;
+            stage = 1;
+            return false;
+        case 1:
+            if (need_guess) return false;     // waiting for the wave's MLP call
+            fb = eval(hb, c, k);
+            qb = q_cur;
+            stage = 2;
+            if (f_at_noise(fb) && (lo_pos > 0.0 || neg_seen)) accept_root(c);
+            return stage == 5;
+        case 2: {
             if (lo_pos < 0.0 && hi_neg == 0.01) {   // F(mindepth) <= 0, nothing positive above yet
                 hb = 0.01;                    // scan position; fb, qb stand in for F, outflow there
                 fb = -1.0;                    // (used only once the scan has replaced both points)
                 qb = q_cur;
                 scan_bp = !(c.twcc > 0.0 && c.bfd > 0.01);
                 stage = 6;
-            } else {
-                const double hm = hmax(c);
-                const double den = fa - fb;
-                double hn = (den != 0.0) ? hb - fb * (ha - hb) / den : 0.5 * (ha + hb);
-                // Converged without another evaluation: the secant step points
-                // to a downcrossing next to the point just evaluated (toward
-                // lower h when F <= 0 there, higher when F > 0), inside the
-                // bracket, and the secant error estimate |hn - hb| |hn - ha| /
-                // h^2 is within tol.  The root is the prediction hn; its outflow
-                // is the secant interpolation of the outflows at ha and hb (an
-                // error of the same second order).
-                if (h_cur == hb && hn == hn && (fb > 0.0 ? hn >= hb : hn <= hb) && (lo_pos > 0.0 || neg_seen) &&
-                    !(hn <= lo_pos) && !(neg_seen && hn > hi_neg) && fabs(hn - hb) <= 1e-6 * hb &&
-                    fabs(hn - hb) * fabs(hn - ha) <= k.tol * hb * hb) {
-                    const double q = (ha != hb) ? qb + (qa - qb) * ((hn - hb) / (ha - hb)) : qb;
-                    accept_at(hn, q, c);
-                    if (stage == 5) return true;
-                } else {
-                    const double left = fmax(lo_pos, 0.01);
-                    const bool bracketed = lo_pos > 0.0 && neg_seen;
-                    // Once bracketed, every step must be under half the step
-                    // before the last one, else it bisects the bracket
-                    // (geometrically while it spans more than a factor 4), after
-                    // Brent: a secant pinned to one end of a strongly curved F
-                    // then still converges.  The same rule without a bracket: a
-                    // one-sided secant that creeps (every F <= 0, or every
-                    // F > 0) jumps to mindepth or expands.
-                    const bool bisect = hn == hn && fabs(hn - hb) > 0.5 * s_prev2;
-                    if (bisect || !(hn == hn) || hn <= left || hn >= hi_neg) {
-                        if (bracketed) hn = (hi_neg > 4.0 * left) ? sqrt(left * hi_neg) : 0.5 * (left + hi_neg);
-                        else if (lo_pos < 0.0) hn = 0.01;
-                        else hn = fmin(hm, fmax(ha, hb) * 2.0);
-                    }
-                    s_prev2 = s_prev;
-                    s_prev = fabs(hn - hb);
-                    h = hn;
-                }
+                return false;
             }
-        }
-        if (stage == 6) {
-            // The oracle's scan (leftmost_downcrossing in mc_oracle.c): x1.25
-            // from mindepth, visiting bankfull, first to a point with F > 0,
-            // then on to the first point with F <= 0; the secant then solves
-            // that bracket.
-            double hs = hb * 1.25;
-            if (!scan_bp && hs >= c.bfd) { hs = c.bfd; scan_bp = true; }
-            if (hs > hmax(c)) stage = 4;
-            else h = hs;
-        }
-        switch (stage) {
-        case 0: h = ha; break;
-        case 1: h = hb; break;
-        case 3: h = c.bfd; break;             // a downcrossing below bankfull?
-        case 4: h = 0.01; break;              // settle at mindepth
-        case 7: h = hi_neg; break;            // the bracket's right end, for its outflow
-        case 9: h = 0.01; break;              // F(mindepth) for the in-bank search
-        default: break;                       // 2, 6: set above
-        }
-        const double f = eval(h, c, k);
-        switch (stage) {
-        case 0:
-            fa = f;
-            qa = q_cur;
-            if (learned) need_guess = true;
-            stage = 1;
-            break;
-        case 1:
-            fb = f;
-            qb = q_cur;
-            stage = 2;
-            if (f_at_noise(fb) && (lo_pos > 0.0 || neg_seen)) accept_root(c);
-            break;
-        case 2: {
-            const double stepsz = fabs(h - hb);
-            ha = hb; fa = fb; hb = h; fb = f;
+            const double hm = hmax(c);
+            const double den = fa - fb;
+            double hn = (den != 0.0) ? hb - fb * (ha - hb) / den : 0.5 * (ha + hb);
+            // Converged without another evaluation: the secant step points
+            // to a downcrossing next to the point just evaluated (toward lower
+            // h when F <= 0 there, higher when F > 0), inside the bracket, and
+            // the secant error estimate |hn - hb| |hn - ha| / h^2 is within
+            // tol.  The root is the prediction hn; its outflow is the secant
+            // interpolation of the outflows at ha and hb (an error of the
+            // same second order).
+            if (h_cur == hb && hn == hn && (fb > 0.0 ? hn >= hb : hn <= hb) && (lo_pos > 0.0 || neg_seen) &&
+                !(hn <= lo_pos) && !(neg_seen && hn > hi_neg) && fabs(hn - hb) <= 1e-6 * hb &&
+                fabs(hn - hb) * fabs(hn - ha) <= k.tol * hb * hb) {
+                const double q = (ha != hb) ? qb + (qa - qb) * ((hn - hb) / (ha - hb)) : qb;
+                accept_at(hn, q, c);
+                return stage == 5;
+            }
+            const double left = fmax(lo_pos, 0.01);
+            const bool bracketed = lo_pos > 0.0 && neg_seen;
+            // Once bracketed, every step must be under half the step before
+            // the last one, else it bisects the bracket (geometrically while
+            // it spans more than a factor 4), after Brent: a secant pinned to
+            // one end of a strongly curved F then still converges.
+            // The same rule without a bracket: a one-sided secant that creeps
+            // (every F <= 0, or every F > 0) jumps to mindepth or expands.
+            const bool bisect = hn == hn && fabs(hn - hb) > 0.5 * s_prev2;
+            if (bisect || !(hn == hn) || hn <= left || hn >= hi_neg) {
+                if (bracketed) hn = (hi_neg > 4.0 * left) ? sqrt(left * hi_neg) : 0.5 * (left + hi_neg);
+                else if (lo_pos < 0.0) hn = 0.01;
+                else hn = fmin(hm, fmax(ha, hb) * 2.0);
+            }
+            s_prev2 = s_prev;
+            s_prev = fabs(hn - hb);
+            const double fn = eval(hn, c, k);
+            const double stepsz = fabs(hn - hb);
+            ha = hb; fa = fb; hb = hn; fb = fn;
             qa = qb; qb = q_cur;
             // a converged step from either side of the bracket: points with
-            // F <= 0 only approach from above (h < hi_neg), so this is a
+            // F <= 0 only approach from above (hn < hi_neg), so this is a
             // downcrossing too
             if ((lo_pos > 0.0 || neg_seen) &&
-                (f_at_noise(f) || stepsz <= k.tol * h ||
+                (f_at_noise(fn) || stepsz <= k.tol * hn ||
                  (lo_pos > 0.0 && neg_seen && (hi_neg - lo_pos) <= k.tol * hi_neg))) {
-                if (!(h_cur == hi_neg || h_cur == lo_pos)) stage = 7;   // outside the bracket: take its right end
-                else accept_root(c);
+                if (!(h_cur == hi_neg || h_cur == lo_pos)) {
+                    stage = 7;                // last point outside the bracket: take its right end
+                    return false;
+                }
+                accept_root(c);
             }
-            break;
+            return stage == 5;
         }
-        case 3:
-            // root_h / root_q hold the overbank root
+        case 3: {                             // a downcrossing below bankfull?
+            const double rh = root_h, rq = root_q;
+            const double fbf = eval(c.bfd, c, k);
+            const double qbf = q_cur;
             inbank = true;
-            if (f > 0.0) {                    // no upcrossing between: keep the root
+            if (fbf > 0.0) {                  // no upcrossing between: keep the root
+                root_h = rh;
+                root_q = rq;
                 stage = 5;
-                break;
+                return true;
             }
             // solve [mindepth, bankfull] unless F(mindepth) <= 0
             lo_pos = -1.0;
             hi_neg = c.bfd;
             neg_seen = true;
             s_prev = s_prev2 = 1.0e30;
-            hb = c.bfd;
-            fb = f;
-            qb = q_cur;
-            stage = 9;
-            break;
-        case 9:
             ha = 0.01;
-            fa = f;
+            hb = c.bfd;
+            fb = fbf;
+            qb = qbf;
+            fa = eval(ha, c, k);
             qa = q_cur;
-            stage = (f > 0.0) ? 2 : 5;        // F(mindepth) <= 0: keep the overbank root
-            break;
+            if (!(fa > 0.0)) {
+                root_h = rh;
+                root_q = rq;
+                stage = 5;
+                return true;
+            }
+            stage = 2;
+            return false;
+        }
         case 4:
+            eval(0.01, c, k);
             root_h = 0.01;
             root_q = q_cur;
             have_root = true;
             stage = 5;
-            break;
-        case 6:
+            return true;
+        case 6: {
+            // The oracle's scan (leftmost_downcrossing in mc_oracle.c): x1.25
+            // from mindepth, visiting bankfull, first to a point with F > 0,
+            // then on to the first point with F <= 0; the secant then solves
+            // that bracket.  The search for F > 0 stops early once Qmn on
+            // [h, inf) provably exceeds every flow the routing can produce,
+            // |qup| + |quc| + |qdp| + 2 |ql| (|C1|, |C2|, |C3| <= 1, C4 <= 2 ql
+            // since D >= dt / 2): F < 0 there, so the oracle finds nothing
+            // above either.
+            double hs = hb * 1.25;
+            if (!scan_bp && hs >= c.bfd) { hs = c.bfd; scan_bp = true; }
+            if (hs > hmax(c)) { stage = 4; return false; }
+            const double fs = eval(hs, c, k);
             ha = hb; fa = fb;
-            hb = h; fb = f;
+            hb = hs; fb = fs;
             qa = qb; qb = q_cur;
-            // The search for F > 0 stops early once Qmn on [h, inf) provably
-            // exceeds every flow the routing can produce, |qup| + |quc| + |qdp|
-            // + 2 |ql| (|C1|, |C2|, |C3| <= 1, C4 <= 2 ql since D >= dt / 2):
-            // F < 0 there, so the oracle finds nothing above either.
-            if (lo_pos > 0.0) {               // past a positive point: F <= 0 closes the bracket
-                if (!(f > 0.0)) stage = 2;
-            } else if (!(f > 0.0) && qmn_never_below(h, c) > fabs(qup) + fabs(quc) + fabs(qdp) + 2.0 * fabs(ql)) {
+            if (lo_pos > 0.0) {                 // past a positive point: F <= 0 closes the bracket
+                if (!(fs > 0.0)) stage = 2;
+            } else if (!(fs > 0.0) && qmn_never_below(hs, c) > fabs(qup) + fabs(quc) + fabs(qdp) + 2.0 * fabs(ql)) {
                 stage = 4;
             }
-            break;
-        case 7:
-            accept_root(c);
-            break;
-        default:
-            break;
+            return false;
         }
-        return stage == 5;
+        case 7:
+            eval(hi_neg, c, k);
+            accept_root(c);
+            return stage == 5;
+        }
+        return true;
     }
 
     TRMC_HD SolveOut finish(const Seg &c) const
