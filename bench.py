@@ -409,6 +409,22 @@ def main():
                          "max_rel_qdc": float(rel_q.max()), "max_rel_reach_last": float(rel_r.max()),
                          "frac_within_1e-12": float(np.mean(np.concatenate([rel_q, rel_r]) <= 1e-12)),
                          "bitwise_equal_frac": float(np.mean(q[0].view(np.uint64) == qo.view(np.uint64)))}
+        # the same steps on the oracle with a correctly rounded pow: the GPU
+        # REFERENCE rule computes exactly that, so it must agree bit for bit;
+        # the reference's own glibc pow differs in about 1 call in 1,000
+        t0 = time.perf_counter()
+        topo_o = orc.as_topo(w.topo)
+        qcr, rcr, _ = orc.wavefront_f32in(topo_o, np.ascontiguousarray(w.params, np.float32),
+                                          np.ascontiguousarray(w.qlat[0][:Tc], np.float32), w.qdp0, w.dp0,
+                                          w.reach_last0, Tc, dt, variant="cr")
+        rel_cq = np.abs(q[0] - qcr) / np.maximum(np.abs(qcr), 1e-300)
+        rel_cr = np.abs(rl[0] - rcr) / np.maximum(np.abs(rcr), 1e-300)
+        out["parity"]["vs_oracle_cr_pow"] = {
+            "max_rel_qdc": float(rel_cq.max()), "max_rel_reach_last": float(rel_cr.max()),
+            "bitwise_equal_frac": float(np.mean(np.concatenate([q[0].view(np.uint64) == qcr.view(np.uint64),
+                                                                rl[0].view(np.uint64) == rcr.view(np.uint64)]))),
+            "reference_vs_cr_pow_max_rel": float(np.max(np.abs(qo - qcr) / np.maximum(np.abs(qcr), 1e-300))),
+            "seconds": time.perf_counter() - t0}
         out["speedup_vs_cpu"] = value / rec["value"]
         log(f"[bench] cpu {rec['value'] / 1e6:.2f} M seg-steps/s; parity {out['parity']}")
         if "learned" in nets and args.oracle_sample > 0:
@@ -418,6 +434,22 @@ def main():
                 "segments": n, "max_rel_qdc": eq, "max_rel_depth": ed, "bar": 1e-9}
             log(f"[bench] learned step 1 vs FP64 oracle on {n} segments: max rel q {eq:.3e}, "
                 f"depth {ed:.3e} ({time.perf_counter() - t0:.1f} s)")
+    # the same pass with the lateral inflow uploaded from host memory first
+    # (the reference times its per-step host-to-device copies); reported beside
+    # the resident-input value, never as it
+    hnet = nets[head]
+    trmc.synchronize()
+    t0 = time.perf_counter()
+    hnet.set_forcing(w.qlat)
+    t1 = time.perf_counter()
+    hnet.reset()
+    hnet.run(T, dt)
+    trmc.synchronize()
+    t2 = time.perf_counter()
+    out["pcie_inclusive"] = {"value": seg_steps / (t2 - t0), "unit": "segment-steps/s",
+                             "upload_s": t1 - t0, "upload_bytes": int(w.qlat.nbytes),
+                             "host_dtype": str(w.qlat.dtype), "pass_s": t2 - t1}
+    log(f"[bench] with forcing upload: {out['pcie_inclusive']}")
     for n_ in nets.values():
         n_.close()
     print(json.dumps(out), flush=True)
