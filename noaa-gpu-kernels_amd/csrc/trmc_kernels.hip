@@ -412,7 +412,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     enum { PH_DONE = 0, PH_NEED, PH_WAIT, PH_NEXT, PH_SOLVE };
     int phase = (n_own == 0 || T <= 0) ? PH_DONE : PH_NEED;
     int u = 0, t = 0, n_up = 0, s = 0, s0 = 0, s1 = 0;
-    bool internal = false;
+    bool internal = false, crit = false;
     int4 ups = make_int4(0, 0, 0, 0);
     double qup_in = 0.0, quc_in = 0.0, qup_next = 0.0, quc_next = 0.0, last = 0.0, cur_qdp = 0.0;
     const double *ql_row = a.qlat;
@@ -504,6 +504,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                 s0 = r0.y;
                 s1 = r0.z;
                 internal = (r0.w & 1) != 0;
+                crit = (r0.w & 2) != 0;
                 n_up = r0.w >> 8;
                 phase = PH_WAIT;
                 spin_start = (long long)wall_clock64();
@@ -571,6 +572,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             }
         }
         // -------------------------------------- (2) solve phase, registers only
+        // a wave solving a unit on the longest dependency chain issues first
+        if (__any(crit && phase == PH_SOLVE)) __builtin_amdgcn_s_setprio(2);
+        else __builtin_amdgcn_s_setprio(0);
         for (int r = 0; r < a.batch_iters; ++r) {
             if (!__any(phase == PH_SOLVE)) break;
             if (phase == PH_SOLVE) {

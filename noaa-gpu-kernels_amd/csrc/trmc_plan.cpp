@@ -193,7 +193,12 @@ int build_plan(const int *reach_seg_start, const int *reach_seg_len,
             r[0] = P.unit_level[du];
             r[1] = P.unit_seg[du];
             r[2] = P.unit_seg[(size_t)du + 1];
-            r[3] = (P.unit_flags[du] & 0xFF) | (nu << 8);
+            // bit 1: the unit lies on a longest dependency chain (ASAP level ==
+            // ALAP level, no slack); the dataflow kernel raises the issue
+            // priority of a wave while it solves such a unit
+            const int u = unit_of_dev[du];
+            const bool crit = alap[u] == asap[u];
+            r[3] = (P.unit_flags[du] & 0xFF) | (crit ? 2 : 0) | (nu << 8);
             for (int q = 0; q < nu; ++q) r[4 + q] = P.up_idx[(size_t)P.up_ptr[du] + q];
         }
     }

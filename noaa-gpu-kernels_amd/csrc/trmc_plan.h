@@ -45,6 +45,8 @@ struct Plan {
     int n_dev_segments = 0;          // covered segments
     // packed per-unit record for the dataflow kernels, 8 ints per unit:
     // level, first segment, end segment, flags | n_up << 8, up[0..3]
+    // (flags bit 0: single-upstream internal boundary; bit 1: no slack, the
+    // unit lies on a longest dependency chain)
     // (upstream entries beyond n_up are 0; empty unless max_upstream <= 4)
     std::vector<int> unit_rec;
 };
