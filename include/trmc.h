@@ -194,7 +194,11 @@ int  trmc_plan_info(const trmc_plan *plan, int n_timesteps, int n_members,
                     int64_t *info, int n_info);
 /* Copies plan array `which` into out[n] and returns its length:
  * 0 level_ptr, 1 unit_level, 2 unit_seg, 3 up_ptr, 4 up_idx, 5 unit_flags,
- * 6 unit_reach, 7 unit_init, 8 seg_of_dev, 9 reach_last_unit. */
+ * 6 unit_reach, 7 unit_init, 8 seg_of_dev, 9 reach_last_unit,
+ * 10 unit_rec (8 ints per unit: level, first and end segment,
+ *    flags | n_up << 8 with flags bit 0 internal boundary and bit 1 no slack
+ *    (ASAP level == ALAP level), up to 4 upstream units; empty when a unit
+ *    has more than 4 upstream units). */
 int  trmc_plan_export(const trmc_plan *plan, int which, int *out, int n);
 
 /* ----------------------------------------------- resident network handle */

@@ -419,7 +419,7 @@ def compute_stats(pred: np.ndarray, truth: np.ndarray) -> dict:
 
 # ---------------------------------------------------------------- plan
 PLAN_ARRAYS = ("level_ptr", "unit_level", "unit_seg", "up_ptr", "up_idx", "unit_flags",
-               "unit_reach", "unit_init", "seg_of_dev", "reach_last_unit")
+               "unit_reach", "unit_init", "seg_of_dev", "reach_last_unit", "unit_rec")
 
 
 def plan(topo: Topology, unit_segments: int = 1, n_timesteps: int = 1, n_members: int = 1) -> dict:
@@ -432,7 +432,7 @@ def plan(topo: Topology, unit_segments: int = 1, n_timesteps: int = 1, n_members
         keys = ("n_units", "n_levels", "ring", "super_levels", "max_items", "n_dev_segments",
                 "max_span", "n_links")
         out = dict(zip(keys, info.tolist()))
-        cap = int(max(topo.n_segments, topo.n_reaches, out["n_units"], out["n_links"]) + out["n_levels"] + 2)
+        cap = int(max(topo.n_segments, topo.n_reaches, 8 * out["n_units"], out["n_links"]) + out["n_levels"] + 2)
         buf = np.zeros(cap, np.int32)
         for i, name in enumerate(PLAN_ARRAYS):
             n = _check(lib().trmc_plan_export(h, i, _ptr(buf, C.c_int), cap))

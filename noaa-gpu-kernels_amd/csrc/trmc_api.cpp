@@ -265,7 +265,7 @@ int trmc_plan_export(const trmc_plan *h, int which, int *out, int n)
     const Plan &p = h->p;
     const std::vector<int> *src[] = {&p.level_ptr, &p.unit_level, &p.unit_seg, &p.up_ptr, &p.up_idx,
                                      &p.unit_flags, &p.unit_reach, &p.unit_init, &p.seg_of_dev,
-                                     &p.reach_last_unit};
+                                     &p.reach_last_unit, &p.unit_rec};
     if (which < 0 || which >= (int)(sizeof(src) / sizeof(src[0])))
         return fail(TRMC_ERR_ARG, "unknown plan array %d", which);
     const std::vector<int> &v = *src[which];

@@ -103,6 +103,23 @@ def check_plan(topo, p, C):
     has_down = down_min < L
     assert np.all(p["unit_level"][has_down] == down_min[has_down] - 1)
     assert np.all(p["unit_level"][~has_down] == L - 1)
+    # packed records restate the arrays; bit 1 marks the units without slack
+    # (longest path from a headwater, ASAP, equal to the ALAP level)
+    rec = p["unit_rec"].reshape(-1, 8)
+    assert rec.shape[0] == U
+    assert np.array_equal(rec[:, 0], p["unit_level"])
+    assert np.array_equal(rec[:, 1], seg[:-1]) and np.array_equal(rec[:, 2], seg[1:])
+    assert np.array_equal(rec[:, 3] >> 8, n_up) and np.array_equal(rec[:, 3] & 1, internal)
+    asap = np.zeros(U, np.int64)
+    for u in range(U):                                # level-sorted: upstreams come first
+        ups = p["up_idx"][p["up_ptr"][u]:p["up_ptr"][u + 1]]
+        if len(ups):
+            asap[u] = asap[ups].max() + 1
+        assert np.array_equal(rec[u, 4:4 + len(ups)], ups)
+    crit = (rec[:, 3] >> 1) & 1
+    assert np.array_equal(crit == 1, asap == p["unit_level"])
+    # a longest chain runs through every level
+    assert set(p["unit_level"][crit == 1].tolist()) == set(range(L))
 
 
 @pytest.mark.parametrize("kind,n,R", [(1, 1000, 0), (2, 30000, 9000)])
