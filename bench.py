@@ -268,7 +268,7 @@ def main():
                     help="probe steps for the library autotuner (0 off; default: on for batch workloads)")
     ap.add_argument("--cpu-steps", type=int, default=0, help="timesteps in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--train-pairs", type=int, default=2_000_000)
+    ap.add_argument("--train-pairs", type=int, default=10_000_000)
     ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--train-batch", type=int, default=1024)
     ap.add_argument("--train-lr", type=float, default=3e-3)
