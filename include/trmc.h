@@ -62,10 +62,15 @@ typedef struct { double dx, bw, tw, twcc, n_ch, ncc, cs, s0; } trmc_chparams_f64
  *             previous depth d and a guess along the on-device MLP's rating
  *             curve from Qmn(d) to the routed flow Qmc(d) (trmc_mlp_*); a
  *             segment not converged within learned_budget evaluations
- *             restarts from the reference bracket. */
+ *             restarts from the reference bracket.
+ *  LINEAR:    the linearized MC of wavefront/linear_mc.cu:130-182 with fixed
+ *             per-segment coefficients (trmc_network_set_linear_coeffs):
+ *             Q = C1 Qup_new + C2 Qup_old + C3 Q_old + C4/ql ql, no secant;
+ *             DATAFLOW schedule only.  Depth and velocity are left 0. */
 #define TRMC_MODE_REFERENCE 0
 #define TRMC_MODE_CONVERGED 1
 #define TRMC_MODE_LEARNED   2
+#define TRMC_MODE_LINEAR    3
 
 /* Schedules of the level-ordered DAG sweep.
  *  AUTO:    pick from the autotuner's table (or SKEWED).
@@ -168,6 +173,26 @@ int trmc_reach_parallel_f64(const trmc_chparams_f64 *p, const double *ql,
                             double *qo, double *vo, double *dpo,
                             double uq, double dt, int nr);
 
+/* cpu_linear_mc_fp64 (wavefront/linear_mc.cu:238-289), called from main at
+ * :561-567: the linearized MC over n_timesteps with float coefficients C1, C2,
+ * C3, C4/ql per segment, float lateral inflow [T][N] and initial flow Q0 [N];
+ * Q_final [N] = every segment's flow after the last step, FP64 on the GPU.
+ * Same argument list; returns a status.  Every reach needs at least one
+ * segment (the reference writes a reach's first segment unconditionally). */
+int trmc_linear_mc_f64(const float *C1, const float *C2, const float *C3, const float *C4oq,
+                       const int *reach_seg_start, const int *reach_seg_len,
+                       const int *reach_up_ptr, const int *reach_up_idx,
+                       const int *level_ptr, const int *level_reach,
+                       int n_reaches, int n_levels, int n_segments,
+                       const float *qlat_ts, const float *Q0, double *Q_final,
+                       int n_timesteps);
+
+/* compute_mc_coeffs / kernel_precompute_coeffs (wavefront/linear_mc.cu:57-113):
+ * the linearized MC coefficients at half bankfull depth, float, on the GPU
+ * (each power is the float rounding of the double pow). */
+int trmc_linear_coeffs_f32(const trmc_chparams_f32 *params, int n, float dt,
+                           float *C1, float *C2, float *C3, float *C4oq);
+
 /* mc_solve_d (wavefront_mc.cu:215-220) over n independent segment inputs on
  * the GPU.  in = [n][5] {qup, quc, qdp, ql, depthp}; out = [n][3]
  * {qdc, depth, velocity}; iters (may be NULL) = [n] secant iterations.
@@ -236,6 +261,12 @@ int trmc_network_generate_forcing(trmc_network *net, int n_timesteps, uint64_t s
 /* [new] Weights of the LEARNED depth guess (TRMC_MLP_NPARAM floats, layout of
  * trmc_mlp_train), copied to the device; required before a LEARNED run. */
 int trmc_network_set_mlp(trmc_network *net, const float *weights);
+
+/* [new] Coefficients of the LINEAR mode, one float per segment each, in the
+ * caller's segment order (as trmc_linear_coeffs_f32 returns them); required
+ * before a LINEAR run. */
+int trmc_network_set_linear_coeffs(trmc_network *net, const float *C1, const float *C2,
+                                   const float *C3, const float *C4oq);
 
 /* Route n_timesteps steps (<= the forcing's) from the current state; the final
  * state becomes the new current state (calls chain: run(T1) then run(T2) ==

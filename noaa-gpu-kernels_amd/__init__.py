@@ -33,7 +33,7 @@ LIB_PATH = os.path.join(HERE, "libtrmc.so")
 OK = 0
 ERR_ARG, ERR_TOPOLOGY, ERR_HIP, ERR_NO_DEVICE, ERR_TIMEOUT = -1, -2, -3, -4, -5
 ERR_NOMEM, ERR_IO, ERR_STATE, ERR_NOT_CONVERGED = -6, -7, -8, -9
-MODE_REFERENCE, MODE_CONVERGED, MODE_LEARNED = 0, 1, 2
+MODE_REFERENCE, MODE_CONVERGED, MODE_LEARNED, MODE_LINEAR = 0, 1, 2, 3
 SCHED_AUTO, SCHED_SKEWED, SCHED_LEVEL, SCHED_DATAFLOW = 0, 1, 2, 3
 
 PARAM_FIELDS = ("dx", "bw", "tw", "twcc", "n_ch", "ncc", "cs", "s0")
@@ -149,6 +149,9 @@ def _declare(L: C.CDLL) -> None:
         "trmc_generate_state": (I, [I, _p(ChParamsF64), D, pD, pD]),
         "trmc_compute_stats": (I, [pD, pD, I, pD]),
         "trmc_network_set_mlp": (I, [V, pF]),
+        "trmc_network_set_linear_coeffs": (I, [V, pF, pF, pF, pF]),
+        "trmc_linear_coeffs_f32": (I, [_p(ChParamsF32), I, C.c_float, pF, pF, pF, pF]),
+        "trmc_linear_mc_f64": (I, [pF] * 4 + topo + [pF, pF, pD, I]),
         "trmc_network_generate_forcing": (I, [V, I, C.c_uint64]),
         "trmc_stream_copy": (I, [I, C.c_size_t, I, pD]),
         "trmc_topo_concat": (I, [_p(Topo), I, _p(Topo), pI, pI]),
@@ -486,6 +489,28 @@ def wavefront_fp64(params32, topo: Topology, qlat_ts, qdp0_seg, dp0_seg, reach_l
     return qout, rout
 
 
+def linear_coeffs(params32, dt: float = 300.0):
+    """compute_mc_coeffs on the GPU (linear_mc.cu:57-113): (C1, C2, C3, C4/ql),
+    float32 [N] each."""
+    p = params_f32(params32)
+    n = len(p)
+    out = [np.zeros(n, np.float32) for _ in range(4)]
+    _check(lib().trmc_linear_coeffs_f32(_ptr(p, ChParamsF32), n, float(dt), *[_ptr(a, C.c_float) for a in out]))
+    return tuple(out)
+
+
+def linear_mc(coeffs, topo: Topology, qlat_ts, Q0, n_timesteps: int):
+    """Drop-in for cpu_linear_mc_fp64 (linear_mc.cu:238-289): float32
+    coefficients, forcing [T, N] and initial flow; FP64 routing on the GPU.
+    Returns Q_final [N] float64."""
+    cs = [_f32(c) for c in coeffs]
+    q, q0 = _f32(qlat_ts), _f32(Q0)
+    out = np.zeros(topo.n_segments, np.float64)
+    _check(lib().trmc_linear_mc_f64(*[_ptr(c, C.c_float) for c in cs], *topo.c_args(), _ptr(q, C.c_float),
+                                    _ptr(q0, C.c_float), _ptr(out, C.c_double), int(n_timesteps)))
+    return out
+
+
 def wavefront_f64(params, topo: Topology, qlat_ts, qdp0_seg, dp0_seg, reach_last0,
                   n_timesteps: int, dt: float = 300.0, **options):
     """FP64-end-to-end variant of the drop-in; options as trmc_options fields."""
@@ -670,6 +695,13 @@ class Network:
         """Synthetic lateral inflow generated on the device: the values
         generate_forcing(n_segments, n_timesteps, n_members, seed) gives."""
         _check(lib().trmc_network_generate_forcing(self._h, int(n_timesteps), int(seed)))
+
+    def set_linear_coeffs(self, coeffs) -> None:
+        """LINEAR mode coefficients (C1, C2, C3, C4/ql), float32 [N] each."""
+        cs = [_f32(c) for c in coeffs]
+        if any(len(c) != self.topo.n_segments for c in cs):
+            raise TrmcError(ERR_ARG, "coefficients must have one value per segment")
+        _check(lib().trmc_network_set_linear_coeffs(self._h, *[_ptr(c, C.c_float) for c in cs]))
 
     def set_mlp(self, weights) -> None:
         """Weights of the LEARNED depth guess (MLP_NPARAM float32)."""

@@ -149,12 +149,13 @@ struct trmc_network {
     unsigned *d_trace_it = nullptr;
     int trace_n = 0;
     float *d_mlp = nullptr;
+    double *d_lin = nullptr;             // LINEAR coefficients [4][n_dev_segs]
     unsigned *d_bar = nullptr;
     int *d_status = nullptr, *d_t = nullptr;
     unsigned long long *d_counters = nullptr;
     int forcing_T = 0, t_cursor = 0, ring_shift = 0, hist_T = 0;
     long long steps_done = 0;
-    bool have_params = false, have_state = false;
+    bool have_params = false, have_state = false, have_lin = false;
     std::vector<double> init_q, init_d;  // [M][N] host copies (segments outside every reach)
 
     ~trmc_network()
@@ -165,7 +166,7 @@ struct trmc_network {
         dev_free(d_segc); dev_free(d_qlat); dev_free(d_q); dev_free(d_d); dev_free(d_v);
         dev_free(d_ring); dev_free(d_hist); dev_free(d_mlp); dev_free(d_bar); dev_free(d_status);
         dev_free(d_t); dev_free(d_counters); dev_free(d_q0); dev_free(d_d0); dev_free(d_ring0);
-        dev_free(d_trace_t); dev_free(d_trace_it); dev_free(d_flow_hist);
+        dev_free(d_trace_t); dev_free(d_trace_it); dev_free(d_flow_hist); dev_free(d_lin);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -291,7 +292,7 @@ int trmc_network_create(const int *reach_seg_start, const int *reach_seg_len,
     if (o.block_size % 64 != 0 || o.block_size > 256)
         return fail(TRMC_ERR_ARG, "block_size must be a multiple of 64, at most 256");
     if (o.unit_segments <= 0) o.unit_segments = 4;   // measured best on the Shreve workloads
-    if (o.mode < TRMC_MODE_REFERENCE || o.mode > TRMC_MODE_LEARNED)
+    if (o.mode < TRMC_MODE_REFERENCE || o.mode > TRMC_MODE_LINEAR)
         return fail(TRMC_ERR_ARG, "unknown mode %d", o.mode);
     if (o.tol <= 0.0) o.tol = 1e-13;
     if (o.max_iter <= 0) o.max_iter = 60;
@@ -491,6 +492,26 @@ int trmc_network_generate_forcing(trmc_network *net, int T, uint64_t seed)
     return TRMC_OK;
 }
 
+int trmc_network_set_linear_coeffs(trmc_network *net, const float *C1, const float *C2, const float *C3,
+                                   const float *C4oq)
+{
+    if (!net) return fail(TRMC_ERR_ARG, "null network");
+    const Plan &P = net->plan;
+    const int nd = P.n_dev_segments;
+    if (nd > 0 && (!C1 || !C2 || !C3 || !C4oq)) return fail(TRMC_ERR_ARG, "null coefficient array");
+    HIPCHK(hipSetDevice(net->opt.device));
+    // float coefficients widen exactly (linear_mc.cu:247-249), device order
+    std::vector<double> lin((size_t)4 * std::max(nd, 1), 0.0);
+    const float *src[4] = {C1, C2, C3, C4oq};
+    for (int k = 0; k < 4; ++k)
+        for (int ds = 0; ds < nd; ++ds) lin[(size_t)k * nd + ds] = (double)src[k][P.seg_of_dev[ds]];
+    int rc = dev_upload(&net->d_lin, lin.data(), lin.size(), net->stream);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(net->stream));
+    net->have_lin = true;
+    return TRMC_OK;
+}
+
 int trmc_network_set_mlp(trmc_network *net, const float *weights)
 {
     if (!net || !weights) return fail(TRMC_ERR_ARG, "null network or weights");
@@ -542,6 +563,7 @@ static RouteArgs make_args(trmc_network *net, int T, double dt)
     a.ring_buf = net->d_ring;
     a.hist_out = net->opt.record_history ? net->d_hist : nullptr;
     a.mlp = net->d_mlp;
+    a.lin = net->d_lin;
     a.bar = net->d_bar;
     a.status = net->d_status;
     a.counters = net->d_counters;
@@ -558,7 +580,12 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
 {
     if (!net) return fail(TRMC_ERR_ARG, "null network");
     if (T < 0) return fail(TRMC_ERR_ARG, "negative n_timesteps");
-    if (!net->have_params || !net->have_state) return fail(TRMC_ERR_STATE, "params and state must be set");
+    if (net->opt.mode == TRMC_MODE_LINEAR) {
+        if (!net->have_lin || !net->have_state)
+            return fail(TRMC_ERR_STATE, "LINEAR mode needs coefficients (trmc_network_set_linear_coeffs) and state");
+    } else if (!net->have_params || !net->have_state) {
+        return fail(TRMC_ERR_STATE, "params and state must be set");
+    }
     if (net->t_cursor + T > net->forcing_T)
         return fail(TRMC_ERR_STATE, "forcing has %d steps left, %d requested",
                     net->forcing_T - net->t_cursor, T);
@@ -590,6 +617,8 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
     int sched = o.schedule;
     if (sched == TRMC_SCHED_AUTO)
         sched = P.max_upstream <= kDataflowMaxUp ? TRMC_SCHED_DATAFLOW : TRMC_SCHED_SKEWED;
+    if (mode == TRMC_MODE_LINEAR && (sched != TRMC_SCHED_DATAFLOW || o.dataflow_variant != 0))
+        return fail(TRMC_ERR_ARG, "LINEAR mode runs on the interleaved DATAFLOW schedule");
     if (sched == TRMC_SCHED_DATAFLOW && P.max_upstream > kDataflowMaxUp)
         return fail(TRMC_ERR_ARG, "DATAFLOW schedule supports confluences of at most %d reaches (network has %d)",
                     kDataflowMaxUp, P.max_upstream);
@@ -862,6 +891,64 @@ int trmc_wavefront_f32in_f64(const trmc_chparams_f32 *params, const int *reach_s
     if ((rc = trmc_network_set_forcing_f32(net, qlat_ts, n_timesteps))) return rc;
     if ((rc = trmc_network_run(net, n_timesteps, dt, nullptr))) return rc;
     return trmc_network_get_state_f64(net, qdc_seg_out_d, nullptr, nullptr, reach_last_out_d);
+}
+
+int trmc_linear_coeffs_f32(const trmc_chparams_f32 *params, int n, float dt, float *C1, float *C2, float *C3,
+                           float *C4oq)
+{
+    if (n < 0) return fail(TRMC_ERR_ARG, "negative n");
+    if (n == 0) return TRMC_OK;
+    if (!params || !C1 || !C2 || !C3 || !C4oq) return fail(TRMC_ERR_ARG, "null array");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(TRMC_ERR_NO_DEVICE, "no gfx950 device");
+    static_assert(sizeof(trmc_chparams_f32) == 8 * sizeof(float), "ChParams layout");
+    float *d_p = nullptr, *d_out = nullptr;
+    int rc = dev_upload(&d_p, reinterpret_cast<const float *>(params), (size_t)n * 8, nullptr);
+    if (!rc) rc = dev_alloc(&d_out, (size_t)4 * n);
+    hipError_t e = hipSuccess;
+    if (!rc) e = launch_linear_coeffs(d_p, n, dt, d_out, nullptr);
+    std::vector<float> out;
+    if (!rc && e == hipSuccess) {
+        out.resize((size_t)4 * n);
+        e = hipMemcpy(out.data(), d_out, out.size() * sizeof(float), hipMemcpyDeviceToHost);
+    }
+    dev_free(d_p);
+    dev_free(d_out);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(TRMC_ERR_HIP, "linear coefficients: %s", hipGetErrorString(e));
+    float *dst[4] = {C1, C2, C3, C4oq};
+    for (int k = 0; k < 4; ++k) std::copy(out.begin() + (size_t)k * n, out.begin() + (size_t)(k + 1) * n, dst[k]);
+    return TRMC_OK;
+}
+
+int trmc_linear_mc_f64(const float *C1, const float *C2, const float *C3, const float *C4oq,
+                       const int *reach_seg_start, const int *reach_seg_len, const int *reach_up_ptr,
+                       const int *reach_up_idx, const int *level_ptr, const int *level_reach, int n_reaches,
+                       int n_levels, int n_segments, const float *qlat_ts, const float *Q0, double *Q_final,
+                       int n_timesteps)
+{
+    if (n_timesteps < 0 || n_segments < 0 || n_reaches < 0) return fail(TRMC_ERR_ARG, "negative size");
+    if (n_segments > 0 && (!Q0 || !Q_final || !C1 || !C2 || !C3 || !C4oq)) return fail(TRMC_ERR_ARG, "null array");
+    for (int r = 0; r < n_reaches; ++r)
+        if (reach_seg_len && reach_seg_len[r] < 1)
+            return fail(TRMC_ERR_TOPOLOGY, "reach %d has no segment (the linear routine writes its first one)", r);
+    trmc_options o;
+    trmc_default_options(&o);
+    o.mode = TRMC_MODE_LINEAR;
+    trmc_network *net = nullptr;
+    int rc = trmc_network_create(reach_seg_start, reach_seg_len, reach_up_ptr, reach_up_idx, level_ptr, level_reach,
+                                 n_reaches, n_levels, n_segments, &o, &net);
+    if (rc) return rc;
+    std::unique_ptr<trmc_network, void (*)(trmc_network *)> guard(net, trmc_network_destroy);
+    // every segment starts at Q0; an upstream reach's previous-step outflow is
+    // the flow of its last segment (linear_mc.cu:255-256, :266)
+    std::vector<double> q0(Q0, Q0 + n_segments), d0((size_t)n_segments, 0.0), rl0((size_t)n_reaches, 0.0);
+    for (int r = 0; r < n_reaches; ++r) rl0[r] = (double)Q0[reach_seg_start[r] + reach_seg_len[r] - 1];
+    if ((rc = trmc_network_set_linear_coeffs(net, C1, C2, C3, C4oq))) return rc;
+    if ((rc = trmc_network_set_state_f64(net, q0.data(), d0.data(), rl0.data()))) return rc;
+    if ((rc = trmc_network_set_forcing_f32(net, qlat_ts, n_timesteps))) return rc;
+    if ((rc = trmc_network_run(net, n_timesteps, 300.0, nullptr))) return rc;
+    return trmc_network_get_state_f64(net, Q_final, nullptr, nullptr, nullptr);
 }
 
 int trmc_reach_parallel_f64(const trmc_chparams_f64 *p, const double *ql, const int *rs,

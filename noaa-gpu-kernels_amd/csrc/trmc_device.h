@@ -71,10 +71,13 @@ struct RouteArgs {
     const int4 *unit_rec;           // DATAFLOW: [n_units][2] packed unit records
     int batch_iters;                // DATAFLOW interleaved: secant iterations per batch
     int guess_model;                // LEARNED: 0 network, 1 power law alone
+    const double *lin;              // LINEAR: [4][n_dev_segs] C1, C2, C3, C4/ql
 };
 
 // launchers (trmc_kernels.hip); all asynchronous on `stream`
 hipError_t launch_stream_copy(const void *src, void *dst, size_t bytes, hipStream_t stream);
+// params: n ChParams of 8 floats (dx, bw, tw, twcc, n_ch, ncc, cs, s0); out [4][n]
+hipError_t launch_linear_coeffs(const float *params, int n, float dt, float *out, hipStream_t stream);
 hipError_t launch_gen_forcing(double *dst, const int *seg_of_dev, int nd, int N, int M, int T,
                               unsigned long long seed, hipStream_t stream);
 hipError_t launch_reset(unsigned *bar, int nwords, hipStream_t stream);

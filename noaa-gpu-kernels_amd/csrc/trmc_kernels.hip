@@ -383,6 +383,7 @@ __global__ void __launch_bounds__(256) k_route_dataflow(RouteArgs a)
 // (weights staged in LDS).
 template <int MODE> struct IlSolver { typedef SecantConv type; };
 template <> struct IlSolver<0> { typedef SecantRef type; };
+template <> struct IlSolver<3> { typedef SecantRef type; };   // LINEAR: no iteration (unused)
 
 template <int MODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_route_dataflow_il(RouteArgs a)
@@ -423,7 +424,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 
     // true when the segment needs no iteration (guard)
     auto start_solve = [&](double qup, double quc, double qdp, double ql, double dp) -> bool {
-        if constexpr (MODE == 0) {
+        if constexpr (MODE == 0 || MODE == 3) {
             return sr.start(a.dt, qup, quc, qdp, ql, c, dp);
         } else {
             if (sr.start(qup, quc, qdp, ql, c, dp, MODE == 2)) return true;
@@ -465,6 +466,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             const double qup = (s == s0) ? qup_in : qup_next;
             const double quc = (s == s0) ? quc_in : quc_next;
             const double qdp = a.q[sm];
+            if constexpr (MODE == 3) {
+                // linearized MC, linear_mc.cu:171-180: fixed coefficients, terms
+                // in the reference's order C1 new + C2 old + C3 Q_old + C4/ql ql
+                SolveOut o;
+                o.q = a.lin[s] * quc + a.lin[nd + s] * qup + a.lin[2 * nd + s] * qdp + a.lin[3 * nd + s] * ql_row[sm];
+                o.depth = 0.0;
+                o.vel = 0.0;
+                o.iters = 0;
+                o.flags = 0;
+                close_segment(o, qdp);
+                continue;
+            }
             c = load_seg(a.segc, nd, s);
             if (!start_solve(qup, quc, qdp, ql_row[sm], a.d[sm])) {
                 cur_qdp = qdp;
@@ -580,7 +593,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             if (phase == PH_SOLVE) {
                 progressed = true;
                 bool fin;
-                if constexpr (MODE == 0) fin = sr.step(c);
+                if constexpr (MODE == 0 || MODE == 3) fin = sr.step(c);
                 else fin = sr.step(c, kc);
                 if (fin) close_segment(sr.finish(c), cur_qdp);
             }
@@ -836,7 +849,8 @@ hipError_t launch_route_dataflow_il(const RouteArgs &a, int mode, int grid, int 
 {
     return mode == 0   ? launch_dataflow_il_t<0>(a, grid, block, stream)
            : mode == 1 ? launch_dataflow_il_t<1>(a, grid, block, stream)
-                       : launch_dataflow_il_t<2>(a, grid, block, stream);
+           : mode == 2 ? launch_dataflow_il_t<2>(a, grid, block, stream)
+                       : launch_dataflow_il_t<3>(a, grid, block, stream);
 }
 
 hipError_t route_dataflow_il_occupancy(int mode, int block, int *blocks_per_cu)
@@ -844,7 +858,53 @@ hipError_t route_dataflow_il_occupancy(int mode, int block, int *blocks_per_cu)
     return mode == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<0>, block, 0)
            : mode == 1
                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<1>, block, 0)
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<2>, block, 0);
+           : mode == 2
+               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<2>, block, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<3>, block, 0);
+}
+
+// Linearized MC coefficients, linear_mc.cu:57-100 (compute_mc_coeffs): float
+// arithmetic at the reference depth h = max(0.5 bankfull, 0.1); each power is
+// the float rounding of the double pow (correctly rounded division and square
+// root are the defaults for float).  out [4][n]: C1, C2, C3, C4/ql.
+__global__ void k_linear_coeffs(const float *__restrict__ params, int n, float dt, float *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *p = params + (size_t)i * 8;
+    const float dx = p[0], bw = p[1], tw = p[2], twcc = p[3], n_ch = p[4], cs = p[6], s0 = p[7];
+    const float z = (cs == 0.0f) ? 1.0f : 1.0f / cs;
+    float bfd;
+    if (bw > tw) bfd = bw / 0.00001f;
+    else if (bw == tw) bfd = bw / (2.0f * z);
+    else bfd = (tw - bw) / (2.0f * z);
+    const float h = fmaxf(0.5f * bfd, 0.1f);
+    const float twl = bw + 2.0f * z * h;
+    const float A = (bw + h * z) * h;
+    const float WP = bw + 2.0f * h * sqrtf(1.0f + z * z);
+    const float R = (WP > 0.0f) ? A / WP : 0.0f;
+    const float r23 = (float)pow((double)R, (double)(2.0f / 3.0f));
+    const float r53 = (float)pow((double)R, (double)(5.0f / 3.0f));
+    const float Ck = fmaxf(1e-6f, (sqrtf(s0) / n_ch) * ((5.0f / 3.0f) * r23 - (2.0f / 3.0f) * r53 *
+                                                        (2.0f * sqrtf(1.0f + z * z) / (bw + 2.0f * h * z))));
+    const float Km = fmaxf(dt, dx / Ck);
+    const float tw_use = (h > bfd && twcc > 0.0f) ? twcc : twl;
+    float X = 0.25f;
+    if (Ck > 0.0f && tw_use * s0 * Ck * dx > 0.0f)
+        X = fminf(0.5f, fmaxf(0.0f, 0.5f * (1.0f - 0.0f / (2.0f * tw_use * s0 * Ck * dx))));
+    float D = Km * (1.0f - X) + dt / 2.0f;
+    if (D == 0.0f) D = 1.0f;
+    out[i] = (Km * X + dt / 2.0f) / D;
+    out[(size_t)n + i] = (dt / 2.0f - Km * X) / D;
+    out[2 * (size_t)n + i] = (Km * (1.0f - X) - dt / 2.0f) / D;
+    out[3 * (size_t)n + i] = dt / D;
+}
+
+hipError_t launch_linear_coeffs(const float *params, int n, float dt, float *out, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_linear_coeffs, dim3((n + 255) / 256), dim3(256), 0, stream, params, n, dt, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_route_dataflow(const RouteArgs &a, int mode, int grid, int block, hipStream_t stream)

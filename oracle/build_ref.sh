@@ -15,11 +15,14 @@
 #   troute_mc_final.cu        : 20-23 (includes, skipping cuda_runtime.h at 24)
 #                               26-28 (ChParams, double)
 #                               167-252 (cpu_mc), 254-274 (gen)
+#   wavefront/linear_mc.cu    : 27-35 (std includes, skipping cuda_runtime.h at 36)
+#                               238-289 (cpu_linear_mc_fp64)
 set -euo pipefail
 REF=${TROUTE_REF_DIR:-/root/reference/water-prediction/t-route}
 HERE=$(cd "$(dirname "$0")" && pwd)
 OUT="$HERE/_ref"
-if [ ! -f "$REF/wavefront/wavefront_mc.cu" ] || [ ! -f "$REF/troute_mc_final.cu" ]; then
+if [ ! -f "$REF/wavefront/wavefront_mc.cu" ] || [ ! -f "$REF/troute_mc_final.cu" ] || \
+   [ ! -f "$REF/wavefront/linear_mc.cu" ]; then
     echo "build_ref: reference sources not present; oracle/_ref not rebuilt"
     exit 0
 fi
@@ -29,15 +32,18 @@ trap 'rm -rf "$TMP"' EXIT
   cat "$HERE/ref_driver_wavefront.inc"; } > "$TMP/ref_wavefront.cpp"
 { sed -n '20,23p;26,28p;167,252p;254,274p' "$REF/troute_mc_final.cu"
   cat "$HERE/ref_driver_mcfinal.inc"; } > "$TMP/ref_mcfinal.cpp"
+{ sed -n '27,35p;238,289p' "$REF/wavefront/linear_mc.cu"
+  cat "$HERE/ref_driver_linear.inc"; } > "$TMP/ref_linear.cpp"
 # the cut must land on the routines it names (guards against line drift)
 grep -q "static inline double mc_solve_d(" "$TMP/ref_wavefront.cpp"
 grep -q "static void cpu_wavefront_fp64(" "$TMP/ref_wavefront.cpp"
 grep -q "^void cpu_mc(" "$TMP/ref_mcfinal.cpp"
 grep -q "^void gen(" "$TMP/ref_mcfinal.cpp"
-if grep -q "cuda" "$TMP/ref_wavefront.cpp" "$TMP/ref_mcfinal.cpp"; then
+grep -q "static void cpu_linear_mc_fp64(" "$TMP/ref_linear.cpp"
+if grep -q "cuda" "$TMP/ref_wavefront.cpp" "$TMP/ref_mcfinal.cpp" "$TMP/ref_linear.cpp"; then
     echo "build_ref: extracted ranges still reference CUDA" >&2; exit 1
 fi
 mkdir -p "$OUT"
 g++ -O2 -ffp-contract=off -fPIC -shared -std=c++17 -w \
-    -o "$OUT/libtroute_ref.so" "$TMP/ref_wavefront.cpp" "$TMP/ref_mcfinal.cpp"
+    -o "$OUT/libtroute_ref.so" "$TMP/ref_wavefront.cpp" "$TMP/ref_mcfinal.cpp" "$TMP/ref_linear.cpp"
 echo "build_ref: built $OUT/libtroute_ref.so"

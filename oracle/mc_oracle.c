@@ -574,3 +574,101 @@ void orc_compute_stats(const double *pred, const double *truth, int n, double *o
     out[8] = (double)nans;
     free(rel);
 }
+
+
+/* ------------------------------------------------------------ linearized MC */
+
+/* x^y in float as the float rounding of the double pow. */
+static float powf_from_double(float x, double y)
+{
+    return (float)pow((double)x, y);
+}
+
+/* wavefront/linear_mc.cu:57-100: coefficients at the reference depth
+ * h = max(0.5 bankfull, 0.1), weighting X from the Cunge formula with Qj = 0.
+ * The reference evaluates 2.0f / 3.0f and 5.0f / 3.0f in float. */
+void orc_linear_coeffs(const orc_chparams_f32 *params, int n, float dt,
+                       float *C1, float *C2, float *C3, float *C4oq)
+{
+    for (int i = 0; i < n; ++i) {
+        const orc_chparams_f32 *p = &params[i];
+        float z = (p->cs == 0.0f) ? 1.0f : 1.0f / p->cs;
+        float bfd;
+        if (p->bw > p->tw) bfd = p->bw / 0.00001f;
+        else if (p->bw == p->tw) bfd = p->bw / (2.0f * z);
+        else bfd = (p->tw - p->bw) / (2.0f * z);
+        float h = fmaxf(0.5f * bfd, 0.1f);
+        float twl = p->bw + 2.0f * z * h;
+        float A = (p->bw + h * z) * h;
+        float WP = p->bw + 2.0f * h * sqrtf(1.0f + z * z);
+        float R = (WP > 0.0f) ? A / WP : 0.0f;
+        float r23 = powf_from_double(R, (double)(2.0f / 3.0f));
+        float r53 = powf_from_double(R, (double)(5.0f / 3.0f));
+        float Ck = fmaxf(1e-6f, (sqrtf(p->s0) / p->n_ch) *
+                                    ((5.0f / 3.0f) * r23 - (2.0f / 3.0f) * r53 *
+                                     (2.0f * sqrtf(1.0f + z * z) / (p->bw + 2.0f * h * z))));
+        float Km = fmaxf(dt, p->dx / Ck);
+        float tw_use = (h > bfd && p->twcc > 0.0f) ? p->twcc : twl;
+        float X = 0.25f;
+        if (Ck > 0.0f && tw_use * p->s0 * Ck * p->dx > 0.0f)
+            X = fminf(0.5f, fmaxf(0.0f, 0.5f * (1.0f - 0.0f / (2.0f * tw_use * p->s0 * Ck * p->dx))));
+        float D = Km * (1.0f - X) + dt / 2.0f;
+        if (D == 0.0f) D = 1.0f;
+        C1[i] = (Km * X + dt / 2.0f) / D;
+        C2[i] = (dt / 2.0f - Km * X) / D;
+        C3[i] = (Km * (1.0f - X) - dt / 2.0f) / D;
+        C4oq[i] = dt / D;
+    }
+}
+
+/* wavefront/linear_mc.cu:238-289: level order, reaches in level_reach
+ * order; a reach's first segment takes the sums (from 0.0, CSR order) of its
+ * upstream reaches' last-segment flows, new and old; later segments take
+ * the previous segment's.  Terms in the reference's order:
+ * C1 new + C2 old + C3 Q_old + C4/ql qlat. */
+void orc_linear_mc_fp64(const float *C1f, const float *C2f, const float *C3f, const float *C4f,
+                        const int *reach_seg_start, const int *reach_seg_len,
+                        const int *reach_up_ptr, const int *reach_up_idx,
+                        const int *level_ptr, const int *level_reach,
+                        int n_reaches, int n_levels, int n_segments,
+                        const float *qlat_ts, const float *Q0, double *Q_final,
+                        int n_timesteps)
+{
+    (void)n_reaches;
+    double *Qp = (double *)malloc(sizeof(double) * (size_t)(n_segments > 0 ? n_segments : 1));
+    double *Qn = (double *)calloc((size_t)(n_segments > 0 ? n_segments : 1), sizeof(double));
+    for (int i = 0; i < n_segments; ++i) Qp[i] = Q0[i];
+    for (int t = 0; t < n_timesteps; ++t) {
+        const float *qlat = qlat_ts + (size_t)t * n_segments;
+        for (int L = 0; L < n_levels; ++L) {
+            for (int kk = level_ptr[L]; kk < level_ptr[L + 1]; ++kk) {
+                int r = level_reach[kk];
+                int s0 = reach_seg_start[r], sl = reach_seg_len[r];
+                double sum_new = 0.0, sum_old = 0.0;
+                for (int k = reach_up_ptr[r]; k < reach_up_ptr[r + 1]; ++k) {
+                    int u = reach_up_idx[k];
+                    int last_u = reach_seg_start[u] + reach_seg_len[u] - 1;
+                    sum_new += Qn[last_u];
+                    sum_old += Qp[last_u];
+                }
+                if (sl <= 0) continue;
+                int idx = s0;
+                Qn[idx] = (double)C1f[idx] * sum_new + (double)C2f[idx] * sum_old +
+                          (double)C3f[idx] * Qp[idx] + (double)C4f[idx] * (double)qlat[idx];
+                double prev_new = Qn[idx], prev_old = Qp[idx];
+                for (int i = 1; i < sl; ++i) {
+                    idx = s0 + i;
+                    Qn[idx] = (double)C1f[idx] * prev_new + (double)C2f[idx] * prev_old +
+                              (double)C3f[idx] * Qp[idx] + (double)C4f[idx] * (double)qlat[idx];
+                    prev_new = Qn[idx];
+                    prev_old = Qp[idx];
+                }
+            }
+        }
+        double *tmp = Qp; Qp = Qn; Qn = tmp;
+        memset(Qn, 0, sizeof(double) * (size_t)n_segments);
+    }
+    for (int i = 0; i < n_segments; ++i) Q_final[i] = Qp[i];
+    free(Qp);
+    free(Qn);
+}

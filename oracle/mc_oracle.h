@@ -26,6 +26,14 @@
  *   orc_normal_depth        depth h with Manning flow Qmn(h) == Q (training
  *                           targets for the learned initial guess)
  *
+ * Linearized MC (fixed coefficients, no secant):
+ *   orc_linear_coeffs       wavefront/linear_mc.cu:57-100 (compute_mc_coeffs),
+ *                           float arithmetic; each powf is taken as the float
+ *                           rounding of the double pow (the reference's device
+ *                           powf is not correctly rounded, so no CPU code can
+ *                           reproduce its coefficients bit for bit)
+ *   orc_linear_mc_fp64      wavefront/linear_mc.cu:238-289 (cpu_linear_mc_fp64)
+ *
  * Two builds of the same source: liborc.so calls the host libm's pow, as the
  * reference does (bit-for-bit the reference); liborc_cr.so (-DORC_CR_POW)
  * uses a correctly rounded pow from quad precision, the arithmetic the GPU
@@ -39,6 +47,13 @@
  *     state, guards, hand-traced tiny networks).
  *   - orc_wavefront_f64: identical to orc_wavefront_f32in on float-representable
  *     inputs (checked), so it inherits that pin.
+ *   - orc_linear_mc_fp64: bit-for-bit against the reference's own
+ *     cpu_linear_mc_fp64 compiled into oracle/_ref, plus the steady state.
+ *   - orc_linear_coeffs: compute_mc_coeffs is a __host__ __device__ function
+ *     that cannot be compiled here without CUDA headers; pinned by what its
+ *     arithmetic fixes: X = 0.5 for valid channels, hence C1 == 1 and
+ *     C3 == -C2 exactly, C1 + C2 + C3 == 1, C4/ql == dt / D, and the
+ *     celerity against a double-precision closed form.
  *   - orc_mc_solve_converged / orc_normal_depth: residual |F(h*)| at the root,
  *     brute-force dense scan on small cases, the sign change bracketing the root,
  *     and the steady-state closed form.  No reference counterpart exists, so
@@ -104,6 +119,20 @@ double orc_mc_residual(double dt, double qup, double quc, double qdp, double ql,
 double orc_mc_solve_converged(double dt, double qup, double quc, double qdp, double ql,
                               const orc_chparams_f64 *p, double depthp,
                               double *depthc, double *velc);
+
+/* Linearized MC coefficients at half bankfull depth, float (n segments). */
+void orc_linear_coeffs(const orc_chparams_f32 *params, int n, float dt,
+                       float *C1, float *C2, float *C3, float *C4oq);
+
+/* FP64 linearized MC over T steps from float coefficients, forcing and
+ * initial flow; Q_final = flow of every segment after the last step. */
+void orc_linear_mc_fp64(const float *C1, const float *C2, const float *C3, const float *C4oq,
+                        const int *reach_seg_start, const int *reach_seg_len,
+                        const int *reach_up_ptr, const int *reach_up_idx,
+                        const int *level_ptr, const int *level_reach,
+                        int n_reaches, int n_levels, int n_segments,
+                        const float *qlat_ts, const float *Q0, double *Q_final,
+                        int n_timesteps);
 
 /* Manning normal depth: h >= 0 with Qmn(h) == Q (Q > 0). */
 double orc_normal_depth(double Q, const orc_chparams_f64 *p);

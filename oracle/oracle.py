@@ -68,6 +68,10 @@ def lib(variant: str = "libm") -> C.CDLL:
         L.orc_compute_stats.argtypes = [pD, pD, I, pD]
         L.orc_pow_fn.restype = D
         L.orc_pow_fn.argtypes = [D, D]
+        L.orc_linear_coeffs.restype = None
+        L.orc_linear_coeffs.argtypes = [pP32, I, C.c_float] + [pF] * 4
+        L.orc_linear_mc_fp64.restype = None
+        L.orc_linear_mc_fp64.argtypes = [pF] * 4 + topo + [pF, pF, pD, I]
         _orc[variant] = L
     return _orc[variant]
 
@@ -214,4 +218,31 @@ def load_ref():
     ref.ref_cpu_wavefront_fp64.argtypes = [C.c_void_p] + [pI] * 6 + [I, I, I] + [pF] * 4 + [pD, pD, I, D]
     ref.ref_cpu_mc.restype = None
     ref.ref_cpu_mc.argtypes = [C.c_void_p, pD, pI, pI, pD, pD, pD, pD, pD, pD, D, D, I]
+    if hasattr(ref, "ref_cpu_linear_mc_fp64"):
+        ref.ref_cpu_linear_mc_fp64.restype = None
+        ref.ref_cpu_linear_mc_fp64.argtypes = [pF] * 4 + [pI] * 6 + [I, I, I] + [pF, pF, pD, I]
     return ref
+
+
+def linear_coeffs(params32, dt=300.0):
+    """compute_mc_coeffs restated (linear_mc.cu:57-100): (C1, C2, C3, C4/ql), float [n] each."""
+    p = np.ascontiguousarray(params32, np.float32).reshape(-1, 8)
+    n = len(p)
+    out = [np.zeros(n, np.float32) for _ in range(4)]
+    lib().orc_linear_coeffs(p.ctypes.data_as(C.POINTER(P32)), n, float(dt), *[_ptr(a, C.c_float) for a in out])
+    return tuple(out)
+
+
+def linear_mc_fp64(coeffs, topo, qlat32, Q0, n_timesteps, reference=False):
+    """cpu_linear_mc_fp64 restated (linear_mc.cu:238-289); reference=True runs the
+    reference's own routine from oracle/_ref instead.  Returns Q_final [N] (double)."""
+    t = as_topo(topo)
+    keep, targs = _topo_args(t)
+    cs = [np.ascontiguousarray(c, np.float32) for c in coeffs]
+    q = np.ascontiguousarray(qlat32, np.float32)
+    q0 = np.ascontiguousarray(Q0, np.float32)
+    out = np.zeros(int(t["n_segments"]))
+    fn = load_ref().ref_cpu_linear_mc_fp64 if reference else lib().orc_linear_mc_fp64
+    fn(*[_ptr(c, C.c_float) for c in cs], *targs, _ptr(q, C.c_float), _ptr(q0, C.c_float), _ptr(out, D),
+       int(n_timesteps))
+    return out
