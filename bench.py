@@ -23,6 +23,7 @@ Other workloads (--workload):
                  headline mode REFERENCE (plain FP64 MC)
   shreve-1m-x16  the 1M network x 16 members, LEARNED, one launch
   tree-1k        the reference's own 1,000-segment binary tree
+  linear-1m      the linearized MC (wavefront/linear_mc.cu) on the 1M network
 
 Prints one JSON line with the driver's fields plus
   roofline      HBM roofline of the routing kernel (112 algorithmic bytes per
@@ -67,6 +68,7 @@ WORKLOADS = {
     "shreve-1m-x16": dict(kind=2, n=1_000_000, r=300_000, T=288, M=16, seed=42, mode="learned", cpu_steps=24),
     "batch-64x8": dict(kind="batch", n=64, r=(10_000, 200_000), T=288, M=8, seed=3, mode="reference", cpu_steps=5),
     "tree-1k": dict(kind=1, n=1000, r=0, T=288, M=1, seed=42, mode="learned", cpu_steps=288),
+    "linear-1m": dict(kind=2, n=1_000_000, r=300_000, T=288, M=1, seed=42, mode="linear", cpu_steps=288),
 }
 MODES = {"learned": 2, "plain": 1, "reference": 0, "powerlaw": 2}
 
@@ -251,6 +253,77 @@ def oracle_sample_check(orc, w, net, dt, n_sample):
     return worst_q, worst_d, len(idx)
 
 
+def bench_linear(trmc, spec, args):
+    """The linearized MC (wavefront/linear_mc.cu) on the same 1M network:
+    coefficients from the library's kernel, then LINEAR passes (reset + 288
+    steps, inputs resident).  56 algorithmic bytes per segment-step: four
+    coefficients 32, lateral inflow 8, previous flow 8, new flow 8.  The CPU
+    baseline is the reference's cpu_linear_mc_fp64 over all steps, fed the
+    same coefficients, which the GPU must match bit for bit."""
+    T, dt = spec["T"], 300.0
+    w = trmc.make_workload(spec["kind"], spec["n"], T, n_reaches=spec["r"], seed=spec["seed"])
+    topo = w.topo
+    p32 = np.ascontiguousarray(w.params, np.float32)
+    q0 = np.asarray(w.qdp0, np.float64)
+    last = np.array([q0[s + n - 1] for s, n in zip(topo.reach_seg_start, topo.reach_seg_len)])
+    t0 = time.perf_counter()
+    coeffs = trmc.linear_coeffs(p32, dt)
+    coeff_s = time.perf_counter() - t0
+    opts = dict(unit_segments=args.unit_segments, block_size=args.block, grid_blocks=args.grid,
+                batch_iters=args.batch_iters)
+    net = trmc.Network(topo, mode=trmc.MODE_LINEAR, **opts)
+    net.set_linear_coeffs(coeffs)
+    net.set_state(q0, np.zeros_like(q0), last)
+    net.set_forcing(w.qlat)
+    for _ in range(args.warmup):
+        net.reset()
+        net.run(T, dt)
+    trmc.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        net.reset()
+        kms.append(net.run(T, dt)["kernel_ms"])
+    trmc.synchronize()
+    secs = (time.perf_counter() - t0) / args.steps
+    seg_steps = topo.n_segments * T
+    q_gpu = net.state()[0][0].copy()
+    k_s = float(np.mean(kms)) / 1e3
+    achieved = 56.0 * seg_steps / k_s / 1e9
+    log(f"[bench] linear: {1e3 * secs:.2f} ms/pass, kernel {1e3 * k_s:.2f} ms, {seg_steps / secs / 1e9:.3f} G seg-steps/s")
+    out = {
+        "metric": "segment-steps/sec, 1M-seg FP64 linearized MC (linear_mc.cu)",
+        "value": seg_steps / secs, "unit": "segment-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * secs, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generators; lateral inflow resident in HBM)",
+        "config": {"workload": f"linear-1m: FP64 linearized MC, Shreve-style tree, {topo.n_segments} segments, "
+                               f"{topo.n_reaches} reaches, {T} steps, coefficients at half bankfull",
+                   "segments": topo.n_segments, "reaches": topo.n_reaches, "timesteps": T, "members": 1,
+                   "seed": spec["seed"], "mode": "LINEAR", "parallelism": "1 GPU",
+                   "coefficients_s": coeff_s},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "kernel_ms": 1e3 * k_s,
+    }
+    net.close()
+    if not args.no_cpu:
+        orc = _load("trmc_oracle", os.path.join(ROOT, "oracle", "oracle.py"))
+        use_ref = orc.load_ref() is not None and hasattr(orc.load_ref(), "ref_cpu_linear_mc_fp64")
+        q32 = np.ascontiguousarray(w.qlat[0], np.float32)
+        t0 = time.perf_counter()
+        q_cpu = orc.linear_mc_fp64(coeffs, topo, q32, np.asarray(w.qdp0, np.float32), T, reference=use_ref)
+        cs = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": seg_steps / cs, "unit": "segment-steps/s", "cores": 1,
+                               "kind": "reference" if use_ref else "port",
+                               "sample": f"all {T} steps of all {topo.n_segments} segments, one thread ({cs:.1f} s)"}
+        out["parity"] = {"against": "reference" if use_ref else "port", "steps": T,
+                         "bitwise_equal_frac": float(np.mean(q_gpu.view(np.uint64) == q_cpu.view(np.uint64))),
+                         "max_rel": float(np.max(np.abs(q_gpu - q_cpu) / np.maximum(np.abs(q_cpu), 1e-300)))}
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+        log(f"[bench] linear cpu {out['cpu_baseline']['value'] / 1e6:.1f} M seg-steps/s; parity {out['parity']}")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -281,6 +354,9 @@ def main():
 
     trmc = _load("noaa_gpu_kernels_amd", os.path.join(ROOT, "noaa-gpu-kernels_amd", "__init__.py"))
     spec = WORKLOADS[args.workload]
+    if spec["mode"] == "linear":
+        bench_linear(trmc, spec, args)
+        return
     T, M = spec["T"], spec["M"]
     head = args.mode or spec["mode"]
     dt = 300.0

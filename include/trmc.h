@@ -66,7 +66,7 @@ typedef struct { double dx, bw, tw, twcc, n_ch, ncc, cs, s0; } trmc_chparams_f64
  *  LINEAR:    the linearized MC of wavefront/linear_mc.cu:130-182 with fixed
  *             per-segment coefficients (trmc_network_set_linear_coeffs):
  *             Q = C1 Qup_new + C2 Qup_old + C3 Q_old + C4/ql ql, no secant;
- *             DATAFLOW schedule only.  Depth and velocity are left 0. */
+ *             DATAFLOW schedule only.  Depth and velocity are not updated. */
 #define TRMC_MODE_REFERENCE 0
 #define TRMC_MODE_CONVERGED 1
 #define TRMC_MODE_LEARNED   2

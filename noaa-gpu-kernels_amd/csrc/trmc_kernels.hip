@@ -442,8 +442,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     auto close_segment = [&](const SolveOut &o, double qdp) {
         const size_t sm = (size_t)s * M + m;
         a.q[sm] = o.q;
-        a.d[sm] = o.depth;
-        a.v[sm] = o.vel;
+        if constexpr (MODE != 3) {            // LINEAR carries flow only
+            a.d[sm] = o.depth;
+            a.v[sm] = o.vel;
+        }
         acc.iters += (unsigned long long)o.iters;
         acc.capped += (unsigned long long)(o.flags & 1);
         acc.fallback += (unsigned long long)((o.flags >> 1) & 1);
