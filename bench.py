@@ -24,6 +24,7 @@ Other workloads (--workload):
   shreve-1m-x16  the 1M network x 16 members, LEARNED, one launch
   tree-1k        the reference's own 1,000-segment binary tree
   linear-1m      the linearized MC (wavefront/linear_mc.cu) on the 1M network
+  diffusive-10k  the diffusive-wave tridiagonal step on 10,000 reaches
 
 Prints one JSON line with the driver's fields plus
   roofline      HBM roofline of the routing kernel (112 algorithmic bytes per
@@ -69,6 +70,7 @@ WORKLOADS = {
     "batch-64x8": dict(kind="batch", n=64, r=(10_000, 200_000), T=288, M=8, seed=3, mode="reference", cpu_steps=5),
     "tree-1k": dict(kind=1, n=1000, r=0, T=288, M=1, seed=42, mode="learned", cpu_steps=288),
     "linear-1m": dict(kind=2, n=1_000_000, r=300_000, T=288, M=1, seed=42, mode="linear", cpu_steps=288),
+    "diffusive-10k": dict(kind="diffusive", n=10_000, r=0, T=1, M=1, seed=42, mode="diffusive", cpu_steps=1),
 }
 MODES = {"learned": 2, "plain": 1, "reference": 0, "powerlaw": 2}
 
@@ -324,6 +326,60 @@ def bench_linear(trmc, spec, args):
     print(json.dumps(out), flush=True)
 
 
+def bench_diffusive(trmc, spec, args):
+    """The diffusive-wave step (owp_extended_kernels.cu:23-135) on the
+    reference's largest benchmark size, 10,000 reaches of 20-99 nodes: inputs
+    resident, 200 timed launches.  16 algorithmic bytes per node: spacing,
+    diffusivity and previous discharge read, new discharge written.  The CPU
+    baseline is the reference's cpu_diffusive_tridiag on one core; the GPU
+    must match it bit for bit."""
+    n, reps = spec["n"], 200
+    rng = np.random.default_rng(spec["seed"])
+    rp = np.zeros(n, trmc.DIFF_REACH)
+    rp["ncomp"] = rng.integers(20, 100, n)
+    rp["theta"] = 0.6
+    rp["dtini"] = 60.0
+    size = n * trmc.DIFF_MAX_NODES
+    dx, df, ce, qp = ((lo + w * rng.random(size)).astype(np.float32)
+                      for lo, w in ((100, 500), (10, 100), (0.5, 2), (1, 50)))
+    t0 = time.perf_counter()
+    out, kms = trmc.diffusive_tridiag(rp, dx, df, ce, qp, reps=reps)
+    wall = time.perf_counter() - t0
+    nodes = int(rp["ncomp"].sum())
+    achieved = 16.0 * nodes / (kms * 1e-3) / 1e9
+    log(f"[bench] diffusive: {n} reaches, {nodes} nodes, kernel {1e3 * kms:.1f} us per step")
+    res = {
+        "metric": "node-steps/sec, diffusive-wave tridiagonal step (owp_extended_kernels.cu)",
+        "value": nodes / (kms * 1e-3), "unit": "node-steps/s", "n_gpus": 1, "steps": reps, "warmup": 1,
+        "ms_per_step": kms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded, the reference generator's distributions; inputs resident)",
+        "config": {"workload": f"diffusive-10k: {n} reaches, {nodes} nodes, theta 0.6, dt 60 s", "reaches": n,
+                   "nodes": nodes, "parallelism": "1 GPU", "call_wall_s": wall},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "kernel_ms": kms,
+    }
+    if not args.no_cpu:
+        orc = _load("trmc_oracle", os.path.join(ROOT, "oracle", "oracle.py"))
+        use_ref = orc.load_ref() is not None and hasattr(orc.load_ref(), "ref_cpu_diffusive_tridiag")
+        t0 = time.perf_counter()
+        k = 0
+        while True:
+            want = orc.diffusive_tridiag(rp, dx, df, ce, qp, reference=use_ref)
+            k += 1
+            if time.perf_counter() - t0 > 2.0:
+                break
+        cs = (time.perf_counter() - t0) / k
+        res["cpu_baseline"] = {"value": nodes / cs, "unit": "node-steps/s", "cores": 1,
+                               "kind": "reference" if use_ref else "port",
+                               "sample": f"the whole step, {k} repetitions ({cs * 1e3:.1f} ms each), one thread"}
+        res["parity"] = {"against": "reference" if use_ref else "port",
+                         "bitwise_equal_frac": float(np.mean(out.view(np.uint32) == want.view(np.uint32)))}
+        res["speedup_vs_cpu"] = res["value"] / res["cpu_baseline"]["value"]
+        log(f"[bench] diffusive cpu {res['cpu_baseline']['value'] / 1e6:.1f} M node-steps/s; parity {res['parity']}")
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -356,6 +412,9 @@ def main():
     spec = WORKLOADS[args.workload]
     if spec["mode"] == "linear":
         bench_linear(trmc, spec, args)
+        return
+    if spec["mode"] == "diffusive":
+        bench_diffusive(trmc, spec, args)
         return
     T, M = spec["T"], spec["M"]
     head = args.mode or spec["mode"]

@@ -193,6 +193,26 @@ int trmc_linear_mc_f64(const float *C1, const float *C2, const float *C3, const 
 int trmc_linear_coeffs_f32(const trmc_chparams_f32 *params, int n, float dt,
                            float *C1, float *C2, float *C3, float *C4oq);
 
+/* Diffusive-wave step (owp_extended_kernels.cu:23-135): one implicit
+ * (theta-weighted) diffusion step per reach, solved by forward elimination and
+ * back substitution, FP32.  Node i of reach j is element
+ * j * TRMC_DIFF_MAX_NODES + i of every node array. */
+#define TRMC_DIFF_MAX_NODES 500
+typedef struct { int ncomp; float theta; float dtini; } trmc_diff_reach;   /* DiffReachParams, :28-32 */
+
+/* cpu_diffusive_tridiag / kernel_diffusive_tridiag (owp_extended_kernels.cu:35-94,
+ * :97-158), same argument list; 1 <= ncomp <= TRMC_DIFF_MAX_NODES per reach.
+ * celerity is accepted and unused (the reference computes a Courant number
+ * from it that it never uses).  Entries beyond a reach's ncomp are untouched. */
+int trmc_diffusive_tridiag_f32(const trmc_diff_reach *rp, const float *dx, const float *diffusivity,
+                               const float *celerity, const float *qp_prev, float *qp_out, int nreach);
+
+/* [new] The same step with the inputs resident: one launch, then reps timed
+ * launches (HIP events); *avg_kernel_ms = their average. */
+int trmc_diffusive_tridiag_timed_f32(const trmc_diff_reach *rp, const float *dx, const float *diffusivity,
+                                     const float *celerity, const float *qp_prev, float *qp_out, int nreach,
+                                     int reps, double *avg_kernel_ms);
+
 /* mc_solve_d (wavefront_mc.cu:215-220) over n independent segment inputs on
  * the GPU.  in = [n][5] {qup, quc, qdp, ql, depthp}; out = [n][3]
  * {qdc, depth, velocity}; iters (may be NULL) = [n] secant iterations.

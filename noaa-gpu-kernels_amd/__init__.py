@@ -152,6 +152,8 @@ def _declare(L: C.CDLL) -> None:
         "trmc_network_set_linear_coeffs": (I, [V, pF, pF, pF, pF]),
         "trmc_linear_coeffs_f32": (I, [_p(ChParamsF32), I, C.c_float, pF, pF, pF, pF]),
         "trmc_linear_mc_f64": (I, [pF] * 4 + topo + [pF, pF, pD, I]),
+        "trmc_diffusive_tridiag_f32": (I, [V] + [pF] * 5 + [I]),
+        "trmc_diffusive_tridiag_timed_f32": (I, [V] + [pF] * 5 + [I, I, pD]),
         "trmc_network_generate_forcing": (I, [V, I, C.c_uint64]),
         "trmc_stream_copy": (I, [I, C.c_size_t, I, pD]),
         "trmc_topo_concat": (I, [_p(Topo), I, _p(Topo), pI, pI]),
@@ -508,6 +510,29 @@ def linear_mc(coeffs, topo: Topology, qlat_ts, Q0, n_timesteps: int):
     out = np.zeros(topo.n_segments, np.float64)
     _check(lib().trmc_linear_mc_f64(*[_ptr(c, C.c_float) for c in cs], *topo.c_args(), _ptr(q, C.c_float),
                                     _ptr(q0, C.c_float), _ptr(out, C.c_double), int(n_timesteps)))
+    return out
+
+
+DIFF_MAX_NODES = 500
+DIFF_REACH = np.dtype([("ncomp", np.int32), ("theta", np.float32), ("dtini", np.float32)])
+
+
+def diffusive_tridiag(rp, dx, diffusivity, celerity, qp_prev, qp_out=None, reps: int = 0):
+    """Drop-in for cpu_diffusive_tridiag (owp_extended_kernels.cu:35-94) on the
+    GPU.  rp: DIFF_REACH [nreach]; node arrays float32 [nreach * 500].  Returns
+    qp_out (entries beyond ncomp as given, zeros by default); with reps > 0
+    returns (qp_out, average kernel ms over reps resident launches)."""
+    rp = np.ascontiguousarray(rp, DIFF_REACH)
+    arrs = [_f32(a) for a in (dx, diffusivity, celerity, qp_prev)]
+    out = np.zeros_like(arrs[3]) if qp_out is None else np.array(qp_out, np.float32, copy=True)
+    if any(a.size != len(rp) * DIFF_MAX_NODES for a in arrs + [out]):
+        raise TrmcError(ERR_ARG, "node arrays must hold nreach * 500 floats")
+    ptrs = [_ptr(a, C.c_float) for a in arrs] + [_ptr(out, C.c_float)]
+    if reps > 0:
+        ms = C.c_double()
+        _check(lib().trmc_diffusive_tridiag_timed_f32(rp.ctypes.data, *ptrs, len(rp), int(reps), C.byref(ms)))
+        return out, ms.value
+    _check(lib().trmc_diffusive_tridiag_f32(rp.ctypes.data, *ptrs, len(rp)))
     return out
 
 

@@ -893,6 +893,71 @@ int trmc_wavefront_f32in_f64(const trmc_chparams_f32 *params, const int *reach_s
     return trmc_network_get_state_f64(net, qdc_seg_out_d, nullptr, nullptr, reach_last_out_d);
 }
 
+static int diffusive_run(const trmc_diff_reach *rp, const float *dx, const float *diffusivity,
+                         const float *qp_prev, float *qp_out, int nreach, int reps, double *avg_ms)
+{
+    if (nreach < 0) return fail(TRMC_ERR_ARG, "negative nreach");
+    if (nreach == 0) return TRMC_OK;
+    if (!rp || !dx || !diffusivity || !qp_prev || !qp_out) return fail(TRMC_ERR_ARG, "null array");
+    for (int j = 0; j < nreach; ++j)
+        if (rp[j].ncomp < 1 || rp[j].ncomp > TRMC_DIFF_MAX_NODES)
+            return fail(TRMC_ERR_ARG, "reach %d has ncomp %d (1..%d)", j, rp[j].ncomp, TRMC_DIFF_MAX_NODES);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(TRMC_ERR_NO_DEVICE, "no gfx950 device");
+    static_assert(sizeof(trmc_diff_reach) == sizeof(DiffReach), "reach record layout");
+    const size_t nn = (size_t)nreach * TRMC_DIFF_MAX_NODES;
+    DiffReach *d_rp = nullptr;
+    float *d_dx = nullptr, *d_df = nullptr, *d_qp = nullptr, *d_out = nullptr, *d_ws = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = dev_upload(&d_rp, reinterpret_cast<const DiffReach *>(rp), (size_t)nreach, nullptr);
+    if (!rc) rc = dev_upload(&d_dx, dx, nn, nullptr);
+    if (!rc) rc = dev_upload(&d_df, diffusivity, nn, nullptr);
+    if (!rc) rc = dev_upload(&d_qp, qp_prev, nn, nullptr);
+    if (!rc) rc = dev_alloc(&d_out, nn);
+    if (!rc) rc = dev_alloc(&d_ws, nn);
+    hipError_t e = hipSuccess;
+    float ms = 0.0f;
+    if (!rc) {
+        // entries beyond a reach's ncomp keep the caller's values, as in place
+        e = hipMemcpy(d_out, qp_out, nn * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        if (e == hipSuccess) e = launch_diffusive_tridiag(d_rp, d_dx, d_df, d_qp, d_out, d_ws, nreach, nullptr);
+        if (e == hipSuccess && reps > 0) {
+            e = hipEventRecord(e0, nullptr);
+            for (int r = 0; r < reps && e == hipSuccess; ++r)
+                e = launch_diffusive_tridiag(d_rp, d_dx, d_df, d_qp, d_out, d_ws, nreach, nullptr);
+            if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        }
+        if (e == hipSuccess) e = hipMemcpy(qp_out, d_out, nn * sizeof(float), hipMemcpyDeviceToHost);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dev_free(d_rp); dev_free(d_dx); dev_free(d_df); dev_free(d_qp); dev_free(d_out); dev_free(d_ws);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(TRMC_ERR_HIP, "diffusive step: %s", hipGetErrorString(e));
+    if (avg_ms) *avg_ms = reps > 0 ? (double)ms / reps : 0.0;
+    return TRMC_OK;
+}
+
+int trmc_diffusive_tridiag_f32(const trmc_diff_reach *rp, const float *dx, const float *diffusivity,
+                               const float *celerity, const float *qp_prev, float *qp_out, int nreach)
+{
+    (void)celerity;   // computed into an unused Courant number by the reference
+    return diffusive_run(rp, dx, diffusivity, qp_prev, qp_out, nreach, 0, nullptr);
+}
+
+int trmc_diffusive_tridiag_timed_f32(const trmc_diff_reach *rp, const float *dx, const float *diffusivity,
+                                     const float *celerity, const float *qp_prev, float *qp_out, int nreach,
+                                     int reps, double *avg_kernel_ms)
+{
+    (void)celerity;
+    if (reps < 1 || !avg_kernel_ms) return fail(TRMC_ERR_ARG, "reps >= 1 and avg_kernel_ms required");
+    return diffusive_run(rp, dx, diffusivity, qp_prev, qp_out, nreach, reps, avg_kernel_ms);
+}
+
 int trmc_linear_coeffs_f32(const trmc_chparams_f32 *params, int n, float dt, float *C1, float *C2, float *C3,
                            float *C4oq)
 {

@@ -76,6 +76,11 @@ struct RouteArgs {
 
 // launchers (trmc_kernels.hip); all asynchronous on `stream`
 hipError_t launch_stream_copy(const void *src, void *dst, size_t bytes, hipStream_t stream);
+// diffusive wave: one reach's computational nodes and step parameters
+struct DiffReach { int ncomp; float theta, dtini; };
+// node arrays [nreach][500] floats; ws >= 500 * nreach floats
+hipError_t launch_diffusive_tridiag(const DiffReach *rp, const float *dx, const float *diffusivity,
+                                    const float *qp_prev, float *qp_out, float *ws, int nreach, hipStream_t stream);
 // params: n ChParams of 8 floats (dx, bw, tw, twcc, n_ch, ncc, cs, s0); out [4][n]
 hipError_t launch_linear_coeffs(const float *params, int n, float dt, float *out, hipStream_t stream);
 hipError_t launch_gen_forcing(double *dst, const int *seg_of_dev, int nd, int N, int M, int T,

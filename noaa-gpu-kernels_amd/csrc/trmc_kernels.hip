@@ -902,6 +902,69 @@ __global__ void k_linear_coeffs(const float *__restrict__ params, int n, float d
     out[3 * (size_t)n + i] = dt / D;
 }
 
+// Diffusive-wave step, owp_extended_kernels.cu:35-94 (cpu_diffusive_tridiag):
+// one lane per reach, the reference's float operations in its order.  Node i of
+// reach j is at j * kDiffMaxNodes + i in the node arrays; the elimination
+// factors eei go to a node-major workspace (ws[i * nreach + j]: lanes of a
+// wave touch adjacent words), the factors ffi to qp_out itself, which the back
+// substitution then overwrites in place.
+constexpr int kDiffMaxNodes = 500;
+
+__global__ void __launch_bounds__(256) k_diffusive_tridiag(const DiffReach *__restrict__ rp,
+                                                           const float *__restrict__ dx,
+                                                           const float *__restrict__ diffusivity,
+                                                           const float *__restrict__ qp_prev,
+                                                           float *__restrict__ qp_out, float *__restrict__ ws,
+                                                           int nreach)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nreach) return;
+    const int nc = rp[j].ncomp;
+    const float theta = rp[j].theta, dtini = rp[j].dtini;
+    const size_t base = (size_t)j * kDiffMaxNodes;
+    const float *q = qp_prev + base;
+    float *out = qp_out + base;
+    float e = 1.0f, f = 0.0f;                 // eei[0], ffi[0]
+    ws[j] = e;
+    out[0] = f;
+    float q_im1 = q[0], q_i = (nc > 1) ? q[1] : q[0];
+    for (int i = 1; i < nc; i++) {
+        float dxi = dx[base + i - 1];
+        if (dxi < 1.0f) dxi = 1.0f;
+        const float alpha = 1.0f;
+        float diff_i = diffusivity[base + i];
+        if (diff_i < 0.01f) diff_i = 0.01f;
+        const float ppi = -theta * diff_i * dtini / (dxi * dxi) * 2.0f / (alpha * (alpha + 1.0f)) * alpha;
+        const float qqi = 1.0f - ppi * (alpha + 1.0f) / alpha;
+        const float rri = ppi / alpha;
+        const float q_ip1 = (i < nc - 1) ? q[i + 1] : q_i;
+        const float ssi = q_i + dtini * diff_i * (1.0f - theta) * (q_im1 - 2.0f * q_i + q_ip1) / (dxi * dxi);
+        float denom = ppi * e + qqi;
+        if (fabsf(denom) < 1e-20f) denom = 1e-20f;
+        e = -rri / denom;
+        f = (ssi - ppi * f) / denom;
+        ws[(size_t)i * nreach + j] = e;
+        out[i] = f;
+        q_im1 = q_i;
+        q_i = q_ip1;
+    }
+    float qn = e * q[nc - 1] + f;             // downstream ghost node: qp_prev[nc - 1]
+    out[nc - 1] = qn;
+    for (int i = nc - 2; i >= 0; i--) {
+        qn = ws[(size_t)i * nreach + j] * qn + out[i];
+        out[i] = qn;
+    }
+}
+
+hipError_t launch_diffusive_tridiag(const DiffReach *rp, const float *dx, const float *diffusivity,
+                                    const float *qp_prev, float *qp_out, float *ws, int nreach, hipStream_t stream)
+{
+    if (nreach <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_diffusive_tridiag, dim3((nreach + 63) / 64), dim3(64), 0, stream, rp, dx, diffusivity,
+                       qp_prev, qp_out, ws, nreach);
+    return hipGetLastError();
+}
+
 hipError_t launch_linear_coeffs(const float *params, int n, float dt, float *out, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;

@@ -672,3 +672,45 @@ void orc_linear_mc_fp64(const float *C1f, const float *C2f, const float *C3f, co
     free(Qp);
     free(Qn);
 }
+
+
+/* ------------------------------------------------------------ diffusive wave */
+
+/* owp_extended_kernels.cu:35-94: per reach, forward elimination of the
+ * implicit (theta-weighted) diffusion system with the upstream boundary
+ * eei[0] = 1, ffi[0] = 0, then back substitution from the downstream ghost
+ * node qp_prev[nc - 1].  Float arithmetic in the reference's order; the
+ * reference's alpha == 1 is kept as a variable so the expressions match. */
+void orc_diffusive_tridiag(const orc_diff_reach *rp, const float *dx, const float *diffusivity,
+                           const float *celerity, const float *qp_prev, float *qp_out, int nreach)
+{
+    float eei[ORC_DIFF_MAX_NODES], ffi[ORC_DIFF_MAX_NODES];
+    (void)celerity;   /* the reference's Courant clamp is computed and not used */
+    for (int j = 0; j < nreach; j++) {
+        const int nc = rp[j].ncomp;
+        const float theta = rp[j].theta, dtini = rp[j].dtini;
+        const size_t base = (size_t)j * ORC_DIFF_MAX_NODES;
+        eei[0] = 1.0f;
+        ffi[0] = 0.0f;
+        for (int i = 1; i < nc; i++) {
+            float dxi = dx[base + i - 1];
+            if (dxi < 1.0f) dxi = 1.0f;
+            const float alpha = 1.0f;
+            float diff_i = diffusivity[base + i];
+            if (diff_i < 0.01f) diff_i = 0.01f;
+            const float ppi = -theta * diff_i * dtini / (dxi * dxi) * 2.0f / (alpha * (alpha + 1.0f)) * alpha;
+            const float qqi = 1.0f - ppi * (alpha + 1.0f) / alpha;
+            const float rri = ppi / alpha;
+            const float next = (i < nc - 1) ? qp_prev[base + i + 1] : qp_prev[base + i];
+            const float ssi = qp_prev[base + i] + dtini * diff_i * (1.0f - theta) *
+                                                      (qp_prev[base + i - 1] - 2.0f * qp_prev[base + i] + next) /
+                                                      (dxi * dxi);
+            float denom = ppi * eei[i - 1] + qqi;
+            if (fabsf(denom) < 1e-20f) denom = 1e-20f;
+            eei[i] = -rri / denom;
+            ffi[i] = (ssi - ppi * ffi[i - 1]) / denom;
+        }
+        qp_out[base + nc - 1] = eei[nc - 1] * qp_prev[base + nc - 1] + ffi[nc - 1];
+        for (int i = nc - 2; i >= 0; i--) qp_out[base + i] = eei[i] * qp_out[base + i + 1] + ffi[i];
+    }
+}

@@ -34,6 +34,10 @@
  *                           reproduce its coefficients bit for bit)
  *   orc_linear_mc_fp64      wavefront/linear_mc.cu:238-289 (cpu_linear_mc_fp64)
  *
+ * Diffusive wave (one implicit step per reach, FP32):
+ *   orc_diffusive_tridiag   water-prediction/owp_extended_kernels.cu:35-94
+ *                           (cpu_diffusive_tridiag)
+ *
  * Two builds of the same source: liborc.so calls the host libm's pow, as the
  * reference does (bit-for-bit the reference); liborc_cr.so (-DORC_CR_POW)
  * uses a correctly rounded pow from quad precision, the arithmetic the GPU
@@ -54,6 +58,10 @@
  *     arithmetic fixes: X = 0.5 for valid channels, hence C1 == 1 and
  *     C3 == -C2 exactly, C1 + C2 + C3 == 1, C4/ql == dt / D, and the
  *     celerity against a double-precision closed form.
+ *   - orc_diffusive_tridiag: bit-for-bit against the reference's own
+ *     cpu_diffusive_tridiag compiled into oracle/_ref, on inputs from the
+ *     reference's own gen_diff_data; and its output solves the tridiagonal
+ *     system it eliminates (residual in double).
  *   - orc_mc_solve_converged / orc_normal_depth: residual |F(h*)| at the root,
  *     brute-force dense scan on small cases, the sign change bracketing the root,
  *     and the steady-state closed form.  No reference counterpart exists, so
@@ -133,6 +141,14 @@ void orc_linear_mc_fp64(const float *C1, const float *C2, const float *C3, const
                         int n_reaches, int n_levels, int n_segments,
                         const float *qlat_ts, const float *Q0, double *Q_final,
                         int n_timesteps);
+
+/* Diffusive-wave step: reach j has rp[j] = {ncomp, theta, dtini} and node i
+ * at j * ORC_DIFF_MAX_NODES + i of dx, diffusivity, celerity (unused by the
+ * step), qp_prev and qp_out; 1 <= ncomp <= ORC_DIFF_MAX_NODES. */
+#define ORC_DIFF_MAX_NODES 500
+typedef struct { int ncomp; float theta, dtini; } orc_diff_reach;
+void orc_diffusive_tridiag(const orc_diff_reach *rp, const float *dx, const float *diffusivity,
+                           const float *celerity, const float *qp_prev, float *qp_out, int nreach);
 
 /* Manning normal depth: h >= 0 with Qmn(h) == Q (Q > 0). */
 double orc_normal_depth(double Q, const orc_chparams_f64 *p);

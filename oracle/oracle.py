@@ -70,6 +70,8 @@ def lib(variant: str = "libm") -> C.CDLL:
         L.orc_pow_fn.argtypes = [D, D]
         L.orc_linear_coeffs.restype = None
         L.orc_linear_coeffs.argtypes = [pP32, I, C.c_float] + [pF] * 4
+        L.orc_diffusive_tridiag.restype = None
+        L.orc_diffusive_tridiag.argtypes = [C.c_void_p] + [pF] * 5 + [I]
         L.orc_linear_mc_fp64.restype = None
         L.orc_linear_mc_fp64.argtypes = [pF] * 4 + topo + [pF, pF, pD, I]
         _orc[variant] = L
@@ -218,6 +220,11 @@ def load_ref():
     ref.ref_cpu_wavefront_fp64.argtypes = [C.c_void_p] + [pI] * 6 + [I, I, I] + [pF] * 4 + [pD, pD, I, D]
     ref.ref_cpu_mc.restype = None
     ref.ref_cpu_mc.argtypes = [C.c_void_p, pD, pI, pI, pD, pD, pD, pD, pD, pD, D, D, I]
+    if hasattr(ref, "ref_cpu_diffusive_tridiag"):
+        ref.ref_cpu_diffusive_tridiag.restype = None
+        ref.ref_cpu_diffusive_tridiag.argtypes = [C.c_void_p] + [pF] * 5 + [I]
+        ref.ref_gen_diff_data.restype = None
+        ref.ref_gen_diff_data.argtypes = [C.c_void_p] + [pF] * 4 + [I, C.c_uint]
     if hasattr(ref, "ref_cpu_linear_mc_fp64"):
         ref.ref_cpu_linear_mc_fp64.restype = None
         ref.ref_cpu_linear_mc_fp64.argtypes = [pF] * 4 + [pI] * 6 + [I, I, I] + [pF, pF, pD, I]
@@ -246,3 +253,28 @@ def linear_mc_fp64(coeffs, topo, qlat32, Q0, n_timesteps, reference=False):
     fn(*[_ptr(c, C.c_float) for c in cs], *targs, _ptr(q, C.c_float), _ptr(q0, C.c_float), _ptr(out, D),
        int(n_timesteps))
     return out
+
+
+DIFF_MAX_NODES = 500
+DIFF_REACH = np.dtype([("ncomp", np.int32), ("theta", np.float32), ("dtini", np.float32)])
+
+
+def diffusive_tridiag(rp, dx, diffusivity, celerity, qp_prev, reference=False):
+    """cpu_diffusive_tridiag restated (owp_extended_kernels.cu:35-94); reference=True
+    runs the reference's own routine (oracle/_ref).  rp: DIFF_REACH [nreach];
+    node arrays float32 [nreach * 500].  Returns qp_out float32."""
+    rp = np.ascontiguousarray(rp, DIFF_REACH)
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (dx, diffusivity, celerity, qp_prev)]
+    out = np.zeros_like(arrs[3])
+    fn = load_ref().ref_cpu_diffusive_tridiag if reference else lib().orc_diffusive_tridiag
+    fn(rp.ctypes.data, *[_ptr(a, C.c_float) for a in arrs], _ptr(out, C.c_float), len(rp))
+    return out
+
+
+def gen_diff_data_reference(nreach, seed=42):
+    """The reference's own gen_diff_data (owp_extended_kernels.cu:347-362, glibc
+    rand) from oracle/_ref: (rp, dx, diffusivity, celerity, qp)."""
+    rp = np.zeros(nreach, DIFF_REACH)
+    arrs = [np.zeros(nreach * DIFF_MAX_NODES, np.float32) for _ in range(4)]
+    load_ref().ref_gen_diff_data(rp.ctypes.data, *[_ptr(a, C.c_float) for a in arrs], nreach, seed)
+    return (rp, *arrs)
