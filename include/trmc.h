@@ -213,6 +213,23 @@ int trmc_diffusive_tridiag_timed_f32(const trmc_diff_reach *rp, const float *dx,
                                      const float *celerity, const float *qp_prev, float *qp_out, int nreach,
                                      int reps, double *avg_kernel_ms);
 
+/* Forcing ingest (data_parsers/parse_qlat_fast_cpp.cpp:76-224): lateral inflow
+ * from a directory of per-nexus CSVs "nex-<id>_output.csv" (rows "timestep,
+ * time, value"), pivoted to qlat [n_timesteps][n_reaches] float with reach r
+ * taking the file of nexus topo_to_wbid[r]; host threads (n_threads <= 0: all
+ * cores).  Unnamed reaches and timesteps stay 0; a reach named by two files
+ * takes the later one in name order.  counts (may be NULL) = {files read,
+ * files skipped: no number, unknown id or unreadable}. */
+int trmc_ingest_nex_csv(const char *csv_dir, const int64_t *topo_to_wbid, int n_reaches,
+                        int n_timesteps, int n_threads, float *qlat, int64_t *counts);
+
+/* The program's whole path: topo_to_wbid from a .npy (int64) file, then
+ * forcings.bin = int32 T, float qlat [T][R], qup0 [R] = 0, qdp0 [R] = 0,
+ * dp0 [R] = 0.5. */
+int trmc_ingest_nex_csv_to_forcings(const char *csv_dir, const char *wbid_npy_path,
+                                    const char *out_path, int n_timesteps, int n_threads,
+                                    int64_t *counts);
+
 /* mc_solve_d (wavefront_mc.cu:215-220) over n independent segment inputs on
  * the GPU.  in = [n][5] {qup, quc, qdp, ql, depthp}; out = [n][3]
  * {qdc, depth, velocity}; iters (may be NULL) = [n] secant iterations.

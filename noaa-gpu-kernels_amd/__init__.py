@@ -153,6 +153,8 @@ def _declare(L: C.CDLL) -> None:
         "trmc_linear_coeffs_f32": (I, [_p(ChParamsF32), I, C.c_float, pF, pF, pF, pF]),
         "trmc_linear_mc_f64": (I, [pF] * 4 + topo + [pF, pF, pD, I]),
         "trmc_diffusive_tridiag_f32": (I, [V] + [pF] * 5 + [I]),
+        "trmc_ingest_nex_csv": (I, [C.c_char_p, C.POINTER(C.c_int64), I, I, I, pF, C.POINTER(C.c_int64)]),
+        "trmc_ingest_nex_csv_to_forcings": (I, [C.c_char_p] * 3 + [I, I, C.POINTER(C.c_int64)]),
         "trmc_diffusive_tridiag_timed_f32": (I, [V] + [pF] * 5 + [I, I, pD]),
         "trmc_network_generate_forcing": (I, [V, I, C.c_uint64]),
         "trmc_stream_copy": (I, [I, C.c_size_t, I, pD]),
@@ -511,6 +513,28 @@ def linear_mc(coeffs, topo: Topology, qlat_ts, Q0, n_timesteps: int):
     _check(lib().trmc_linear_mc_f64(*[_ptr(c, C.c_float) for c in cs], *topo.c_args(), _ptr(q, C.c_float),
                                     _ptr(q0, C.c_float), _ptr(out, C.c_double), int(n_timesteps)))
     return out
+
+
+def ingest_nex_csv(csv_dir: str, topo_to_wbid, n_timesteps: int, n_threads: int = 0):
+    """Lateral inflow [T, R] float32 from per-nexus CSVs (parse_qlat_fast_cpp.cpp);
+    returns (qlat, files read, files skipped)."""
+    w = np.ascontiguousarray(topo_to_wbid, np.int64)
+    q = np.zeros((int(n_timesteps), len(w)), np.float32)
+    cnt = np.zeros(2, np.int64)
+    _check(lib().trmc_ingest_nex_csv(os.fsencode(csv_dir), w.ctypes.data_as(C.POINTER(C.c_int64)), len(w),
+                                     int(n_timesteps), int(n_threads), _ptr(q, C.c_float),
+                                     cnt.ctypes.data_as(C.POINTER(C.c_int64))))
+    return q, int(cnt[0]), int(cnt[1])
+
+
+def ingest_nex_csv_to_forcings(csv_dir: str, wbid_npy: str, out_path: str, n_timesteps: int = 24,
+                               n_threads: int = 0):
+    """The parse_qlat_fast program's path: writes forcings.bin; returns (files read, skipped)."""
+    cnt = np.zeros(2, np.int64)
+    _check(lib().trmc_ingest_nex_csv_to_forcings(os.fsencode(csv_dir), os.fsencode(wbid_npy), os.fsencode(out_path),
+                                                 int(n_timesteps), int(n_threads),
+                                                 cnt.ctypes.data_as(C.POINTER(C.c_int64))))
+    return int(cnt[0]), int(cnt[1])
 
 
 DIFF_MAX_NODES = 500

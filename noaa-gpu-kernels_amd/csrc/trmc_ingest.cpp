@@ -1,0 +1,185 @@
+// Forcing ingest: per-nexus lateral-inflow CSVs (nex-<id>_output.csv, rows
+// "timestep, time, value") pivoted into the [timestep][reach] float tensor
+// of forcings.bin, after data_parsers/parse_qlat_fast_cpp.cpp:76-224.  Host
+// C++: files are independent, so threads take contiguous chunks of the file
+// list and write disjoint reach columns.
+#include <algorithm>
+#include <atomic>
+#include <cerrno>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <filesystem>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "trmc.h"
+#include "trmc_util.h"
+
+namespace fs = std::filesystem;
+
+namespace {
+
+// "nex-1328591_output.csv" -> 1328591 (digits between the first '-' and the
+// next '_' or '.'), -1 when absent or not a number (parse_qlat_fast_cpp.cpp:43-58)
+long long nexus_id(const std::string &name)
+{
+    const size_t dash = name.find('-');
+    if (dash == std::string::npos) return -1;
+    size_t end = name.find('_', dash + 1);
+    if (end == std::string::npos) {
+        end = name.find('.', dash + 1);
+        if (end == std::string::npos) return -1;
+    }
+    const std::string num = name.substr(dash + 1, end - dash - 1);
+    // std::stoi semantics: optional leading space/sign, digits, stop at the first
+    // non-digit; nothing parsed or out of int range is -1
+    const char *p = num.c_str();
+    char *stop = nullptr;
+    errno = 0;
+    const long v = std::strtol(p, &stop, 10);
+    if (stop == p || errno == ERANGE || v < INT32_MIN || v > INT32_MAX) return -1;
+    return (long long)v;
+}
+
+// one row: the timestep before the first comma (atoi), the value after the
+// last comma (atof, rounded to float) (parse_qlat_fast_cpp.cpp:62-74)
+bool parse_row(const char *line, size_t len, int &ts, float &val)
+{
+    size_t i = 0;
+    while (i < len && line[i] != ',') ++i;
+    if (i == len) return false;
+    ts = std::atoi(line);
+    size_t j = len;
+    while (j > 0 && line[j - 1] != ',') --j;
+    if (j == 0 || j >= len) return false;
+    val = (float)std::atof(line + j);
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int trmc_ingest_nex_csv(const char *csv_dir, const int64_t *topo_to_wbid, int n_reaches, int n_timesteps,
+                        int n_threads, float *qlat, int64_t *counts)
+{
+    if (!csv_dir || (!topo_to_wbid && n_reaches > 0) || (!qlat && n_reaches > 0 && n_timesteps > 0))
+        return trmc::fail(TRMC_ERR_ARG, "null argument");
+    if (n_reaches < 0 || n_timesteps < 0) return trmc::fail(TRMC_ERR_ARG, "negative size");
+    std::unordered_map<int64_t, int> topo_of;
+    topo_of.reserve((size_t)n_reaches * 2);
+    for (int i = 0; i < n_reaches; ++i) topo_of[topo_to_wbid[i]] = i;   // a repeated id keeps its last reach
+    std::vector<std::string> paths;
+    std::error_code ec;
+    for (fs::directory_iterator it(csv_dir, ec), end; !ec && it != end; it.increment(ec)) {
+        if (!it->is_regular_file(ec)) continue;
+        const std::string name = it->path().filename().string();
+        if (name.rfind("nex-", 0) != 0 || name.find("_output.csv") == std::string::npos) continue;
+        paths.push_back(it->path().string());
+    }
+    if (ec) return trmc::fail(TRMC_ERR_IO, "cannot list %s: %s", csv_dir, ec.message().c_str());
+    // a fixed order, so a reach named by two files gets the same one every run
+    std::sort(paths.begin(), paths.end());
+    const size_t R = (size_t)n_reaches;
+    std::fill(qlat, qlat + (size_t)n_timesteps * R, 0.0f);
+    if (n_threads <= 0) n_threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<long long> matched{0}, missing{0};
+    const size_t per = (paths.size() + n_threads - 1) / std::max(1, n_threads);
+    // one owner per reach column: the last file (in sorted order) naming it
+    std::vector<int> owner(R, -1);
+    std::vector<int> reach_of(paths.size(), -1);
+    for (size_t pi = 0; pi < paths.size(); ++pi) {
+        const long long id = nexus_id(fs::path(paths[pi]).filename().string());
+        if (id < 0) continue;
+        auto f = topo_of.find((int64_t)id);
+        if (f == topo_of.end()) continue;
+        reach_of[pi] = f->second;
+        owner[f->second] = (int)pi;
+    }
+    auto work = [&](size_t b, size_t e) {
+        std::vector<char> buf(4096);
+        for (size_t pi = b; pi < e; ++pi) {
+            const int r = reach_of[pi];
+            if (r < 0) { missing.fetch_add(1); continue; }
+            FILE *fp = std::fopen(paths[pi].c_str(), "rb");
+            if (!fp) { missing.fetch_add(1); continue; }
+            std::fseek(fp, 0, SEEK_END);
+            const long size = std::ftell(fp);
+            std::fseek(fp, 0, SEEK_SET);
+            if (size < 0) { std::fclose(fp); missing.fetch_add(1); continue; }
+            if ((long)buf.size() < size + 1) buf.resize((size_t)size + 1);
+            const size_t got = std::fread(buf.data(), 1, (size_t)size, fp);
+            buf[got] = 0;
+            std::fclose(fp);
+            matched.fetch_add(1);
+            if (owner[r] != (int)pi) continue;   // a later file names the same reach
+            size_t i = 0;
+            while (i < got) {
+                const size_t ls = i;
+                while (i < got && buf[i] != '\n') ++i;
+                const size_t ll = i - ls;
+                if (i < got) ++i;
+                if (ll < 5) continue;
+                int ts;
+                float val;
+                if (!parse_row(buf.data() + ls, ll, ts, val)) continue;
+                if (ts >= 0 && ts < n_timesteps) qlat[(size_t)ts * R + r] = val;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < n_threads; ++t) {
+        const size_t b = (size_t)t * per, e = std::min(paths.size(), b + per);
+        if (b >= e) break;
+        th.emplace_back(work, b, e);
+    }
+    for (auto &t : th) t.join();
+    if (counts) {
+        counts[0] = matched.load();
+        counts[1] = missing.load();
+    }
+    return TRMC_OK;
+}
+
+int trmc_ingest_nex_csv_to_forcings(const char *csv_dir, const char *wbid_npy_path, const char *out_path,
+                                    int n_timesteps, int n_threads, int64_t *counts)
+{
+    if (!csv_dir || !wbid_npy_path || !out_path) return trmc::fail(TRMC_ERR_ARG, "null path");
+    // topo_to_wbid.npy: int64 array after the .npy header (magic 6, version 2,
+    // little-endian header length 2, header) (parse_qlat_fast_cpp.cpp:90-110)
+    FILE *f = std::fopen(wbid_npy_path, "rb");
+    if (!f) return trmc::fail(TRMC_ERR_IO, "cannot open %s", wbid_npy_path);
+    unsigned char pre[10];
+    if (std::fread(pre, 1, 10, f) != 10 || std::memcmp(pre, "\x93NUMPY", 6) != 0) {
+        std::fclose(f);
+        return trmc::fail(TRMC_ERR_IO, "%s is not a .npy file", wbid_npy_path);
+    }
+    const long header_end = 10 + (long)(pre[8] | (pre[9] << 8));
+    std::fseek(f, 0, SEEK_END);
+    const long total = std::ftell(f);
+    const size_t R = total > header_end ? (size_t)(total - header_end) / 8 : 0;
+    std::vector<int64_t> wbid(R);
+    std::fseek(f, header_end, SEEK_SET);
+    const size_t got = R ? std::fread(wbid.data(), 8, R, f) : 0;
+    std::fclose(f);
+    if (got != R) return trmc::fail(TRMC_ERR_IO, "short read in %s", wbid_npy_path);
+    std::vector<float> qlat((size_t)std::max(n_timesteps, 0) * R);
+    int rc = trmc_ingest_nex_csv(csv_dir, wbid.data(), (int)R, n_timesteps, n_threads, qlat.data(), counts);
+    if (rc) return rc;
+    // forcings.bin: int32 T, qlat [T][R], qup0 [R] = 0, qdp0 [R] = 0, dp0 [R] = 0.5
+    FILE *o = std::fopen(out_path, "wb");
+    if (!o) return trmc::fail(TRMC_ERR_IO, "cannot write %s", out_path);
+    const int32_t nt = n_timesteps;
+    std::vector<float> zeros(R, 0.0f), half(R, 0.5f);
+    bool ok = std::fwrite(&nt, 4, 1, o) == 1 && std::fwrite(qlat.data(), 4, qlat.size(), o) == qlat.size() &&
+              std::fwrite(zeros.data(), 4, R, o) == R && std::fwrite(zeros.data(), 4, R, o) == R &&
+              std::fwrite(half.data(), 4, R, o) == R;
+    ok = (std::fclose(o) == 0) && ok;
+    return ok ? TRMC_OK : trmc::fail(TRMC_ERR_IO, "write failed: %s", out_path);
+}
+
+}  // extern "C"

@@ -17,6 +17,8 @@
 #                               167-252 (cpu_mc), 254-274 (gen)
 #   wavefront/linear_mc.cu    : 27-35 (std includes, skipping cuda_runtime.h at 36)
 #                               238-289 (cpu_linear_mc_fp64)
+#   wavefront/data_parsers/parse_qlat_fast_cpp.cpp: the whole program (no CUDA),
+#                               built as oracle/_ref/parse_qlat_fast
 #   ../owp_extended_kernels.cu: 13-16 (std includes, skipping cuda_runtime.h at 17)
 #                               19-20 (MAX_NODES), 28-32 (DiffReachParams)
 #                               35-94 (cpu_diffusive_tridiag), 347-362 (gen_diff_data)
@@ -55,3 +57,9 @@ g++ -O2 -ffp-contract=off -fPIC -shared -std=c++17 -w \
     -o "$OUT/libtroute_ref.so" "$TMP/ref_wavefront.cpp" "$TMP/ref_mcfinal.cpp" "$TMP/ref_linear.cpp" \
     "$TMP/ref_diffusive.cpp"
 echo "build_ref: built $OUT/libtroute_ref.so"
+# the forcing-ingest program compiles whole from its own file (std C++17 only)
+QLAT="$REF/wavefront/data_parsers/parse_qlat_fast_cpp.cpp"
+if [ -f "$QLAT" ]; then
+    g++ -O2 -std=c++17 -pthread -w -o "$OUT/parse_qlat_fast" "$QLAT"
+    echo "build_ref: built $OUT/parse_qlat_fast"
+fi
