@@ -418,6 +418,23 @@ int  trmc_save_forcings(const char *path, int n_reaches, int n_segments, int n_t
                         const float *dp0);
 void trmc_free(void *p);
 
+/* Hydrofabric network builder (data_parsers/parse_gpkg.py:43-270): the
+ * flowpaths table's rows whose id parses, wb_id[n_rows] and to_id[n_rows]
+ * (waterbody "wb-<k>" draining through nexus "nex-<m>", which feeds waterbody m;
+ * to_id < 0 for a NULL, unparsable or terminal toid), and the
+ * flowpath-attributes rows whose link parses, link[n_attr] with vals[n_attr][8]
+ * = {Length_m, n, nCC, BtmWdth, TopWdth, TopWdthCC, ChSlp, So} (NaN = NULL).
+ * The first row of a repeated id is kept; reaches are ordered by Kahn's sort
+ * with a stack of ready reaches; one segment per reach; params start at
+ * {1000, 20, 25, 60, 0.06, 0.12, 2, 0.001} and take each attribute row in
+ * table order under the reference's clamps.  Fills *topo (free with
+ * trmc_free_topo) and returns *params [n] and *topo_to_wbid [n] allocated by
+ * the library (free with trmc_free).  A cycle is TRMC_ERR_TOPOLOGY. */
+int trmc_build_flowpath_network(const int64_t *wb_id, const int64_t *to_id, int n_rows,
+                                const int64_t *link, const double *vals, int n_attr,
+                                trmc_topo *topo, trmc_chparams_f32 **params,
+                                int64_t **topo_to_wbid);
+
 /* [new] Seeded synthetic networks for the benchmark workloads.
  *  kind 1: binary-confluence tree, reaches of U{1..40} segments.
  *  kind 2: Shreve-style random tree: uniform random binary topology grown by

@@ -143,6 +143,8 @@ def _declare(L: C.CDLL) -> None:
         "trmc_load_forcings": (I, [C.c_char_p, I, I, pI, pF, pF, pF, pF]),
         "trmc_save_forcings": (I, [C.c_char_p, I, I, I, pF, pF, pF, pF]),
         "trmc_free": (None, [V]),
+        "trmc_build_flowpath_network": (I, [_p(C.c_int64), _p(C.c_int64), I, _p(C.c_int64), pD, I, _p(Topo),
+                                            _p(_p(ChParamsF32)), _p(_p(C.c_int64))]),
         "trmc_generate_network": (I, [I, I, I, C.c_uint64, _p(Topo)]),
         "trmc_generate_params": (I, [I, C.c_uint64, _p(ChParamsF64)]),
         "trmc_generate_forcing": (I, [I, I, I, C.c_uint64, pD]),
@@ -535,6 +537,89 @@ def ingest_nex_csv_to_forcings(csv_dir: str, wbid_npy: str, out_path: str, n_tim
                                                  int(n_timesteps), int(n_threads),
                                                  cnt.ctypes.data_as(C.POINTER(C.c_int64))))
     return int(cnt[0]), int(cnt[1])
+
+
+HYDROFABRIC_ATTRS = ("Length_m", "n", "nCC", "BtmWdth", "TopWdth", "TopWdthCC", "ChSlp", "So")
+
+
+def hydrofabric_id(s):
+    """The number after the first '-' of "wb-12", "nex-12", "tnx-12" as Python's
+    int() reads it, or None (parse_gpkg.py:30-40)."""
+    if s is None:
+        return None
+    parts = s.split("-", 1)
+    if len(parts) != 2:
+        return None
+    try:
+        return int(parts[1])
+    except ValueError:
+        return None
+
+
+def build_flowpath_network(ids, toids, links, vals):
+    """Network from the flowpaths rows (text ids, None for NULL) and the
+    flowpath-attributes rows (link text, vals [rows, 8] float with NaN for NULL
+    in HYDROFABRIC_ATTRS order), by the rules of parse_gpkg.py:43-270.
+    Returns (Topology, params [n, 8] float32, topo_to_wbid [n] int64)."""
+    wid, tid = [], []
+    for i, t in zip(ids, toids):
+        w = hydrofabric_id(i)
+        if w is None:
+            continue
+        k = hydrofabric_id(t)
+        wid.append(w)
+        tid.append(-1 if k is None or k < 0 else k)
+    keep = [j for j, l in enumerate(links) if hydrofabric_id(l) is not None]
+    lid = np.array([hydrofabric_id(links[j]) for j in keep], np.int64)
+    av = np.ascontiguousarray(np.asarray(vals, np.float64).reshape(-1, 8)[keep], np.float64)
+    wid, tid = np.array(wid, np.int64), np.array(tid, np.int64)
+    t = Topo()
+    pp = C.POINTER(ChParamsF32)()
+    pw = C.POINTER(C.c_int64)()
+    i64 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int64))
+    _check(lib().trmc_build_flowpath_network(i64(wid), i64(tid), len(wid), i64(lid), _ptr(av, C.c_double), len(lid),
+                                             C.byref(t), C.byref(pp), C.byref(pw)))
+    try:
+        n = t.n_reaches
+        topo = Topology.from_c(t)
+        params = np.ctypeslib.as_array(C.cast(pp, C.POINTER(C.c_float)), (n * 8,)).reshape(n, 8).copy() if n else \
+            np.zeros((0, 8), np.float32)
+        wbid = np.ctypeslib.as_array(pw, (n,)).copy() if n else np.zeros(0, np.int64)
+    finally:
+        lib().trmc_free_topo(C.byref(t))
+        lib().trmc_free(pp)
+        lib().trmc_free(pw)
+    return topo, params, wbid
+
+
+def read_hydrofabric(gpkg_path: str):
+    """The two GeoPackage tables parse_gpkg.py reads (:52-53, :183-186) through
+    sqlite3: (ids, toids, links, vals [rows, 8] with NaN for NULL)."""
+    import sqlite3
+    if not os.path.exists(gpkg_path):
+        raise TrmcError(ERR_IO, f"no such GeoPackage: {gpkg_path}")
+    con = sqlite3.connect(f"file:{gpkg_path}?mode=ro", uri=True)
+    try:
+        rows = con.execute("SELECT id, toid FROM flowpaths").fetchall()
+        cols = ", ".join(f'"{c}"' for c in HYDROFABRIC_ATTRS)
+        attrs = con.execute(f'SELECT link, {cols} FROM "flowpath-attributes"').fetchall()
+    finally:
+        con.close()
+    vals = np.array([[np.nan if v is None else float(v) for v in a[1:]] for a in attrs], np.float64).reshape(-1, 8)
+    return [r[0] for r in rows], [r[1] for r in rows], [a[0] for a in attrs], vals
+
+
+def build_hydrofabric(gpkg_path: str, out_dir: str = None):
+    """parse_gpkg.py's whole path: the network of a GeoPackage, and, when
+    out_dir is given, its topo.bin, params.bin and topo_to_wbid.npy there.
+    Returns (Topology, params, topo_to_wbid)."""
+    topo, params, wbid = build_flowpath_network(*read_hydrofabric(gpkg_path))
+    if out_dir is not None:
+        os.makedirs(out_dir, exist_ok=True)
+        save_topo(os.path.join(out_dir, "topo.bin"), topo)
+        save_params(os.path.join(out_dir, "params.bin"), params)
+        np.save(os.path.join(out_dir, "topo_to_wbid.npy"), wbid)
+    return topo, params, wbid
 
 
 DIFF_MAX_NODES = 500
