@@ -25,6 +25,8 @@ Other workloads (--workload):
   tree-1k        the reference's own 1,000-segment binary tree
   linear-1m      the linearized MC (wavefront/linear_mc.cu) on the 1M network
   diffusive-10k  the diffusive-wave tridiagonal step on 10,000 reaches
+  matrix-309k    the matrix form of the linearized MC (linear_mc_matrix*.cu):
+                 309,000 one-segment reaches, 24 steps, tol 1e-6
 
 Prints one JSON line with the driver's fields plus
   roofline      HBM roofline of the routing kernel (112 algorithmic bytes per
@@ -71,6 +73,7 @@ WORKLOADS = {
     "tree-1k": dict(kind=1, n=1000, r=0, T=288, M=1, seed=42, mode="learned", cpu_steps=288),
     "linear-1m": dict(kind=2, n=1_000_000, r=300_000, T=288, M=1, seed=42, mode="linear", cpu_steps=288),
     "diffusive-10k": dict(kind="diffusive", n=10_000, r=0, T=1, M=1, seed=42, mode="diffusive", cpu_steps=1),
+    "matrix-309k": dict(kind=2, n=309_000, r=309_000, T=24, M=1, seed=42, mode="matrix", cpu_steps=24),
 }
 MODES = {"learned": 2, "plain": 1, "reference": 0, "powerlaw": 2}
 
@@ -380,6 +383,85 @@ def bench_diffusive(trmc, spec, args):
     print(json.dumps(res), flush=True)
 
 
+def bench_matrix(trmc, spec, args):
+    """The matrix form of the linearized MC (linear_mc_matrix.cu,
+    linear_mc_matrix_graph.cu) at the reference's real-network size: 309,000
+    one-segment reaches, 24 steps, entries below 1e-6 dropped.  Coefficients
+    (GPU) and propagation matrices (host) are set-up, timed apart; the metric
+    is reach-steps per second of the 24 steps replayed as one HIP graph with
+    every input resident.  Algorithmic bytes per step: 8 per matrix entry
+    of the union of the two matrices (column, P_Q and P_q values), 16 per reach
+    for its row record, and one pass over the state pairs {flow, lateral
+    inflow} read (8) and written (8) and the next step's lateral inflow (4).
+    The CPU baseline is the oracle's sequential float step loop on one core;
+    the GPU must equal the oracle's loop in its own summation order bit for
+    bit, and stay within 1 % of the FP64 level-scheduled recurrence."""
+    n, T, tol = spec["n"], spec["T"], 1e-6
+    orc = _load("trmc_oracle", os.path.join(ROOT, "oracle", "oracle.py"))
+    w = trmc.make_workload(spec["kind"], n, 1, n_reaches=spec["r"], seed=spec["seed"])
+    t = w.topo
+    p32 = np.asarray(w.params, np.float32)
+    rng = np.random.default_rng(spec["seed"])
+    qlat = rng.lognormal(np.log(0.1) - 0.5, 1.0, (T, n)).astype(np.float32)
+    Q0 = np.asarray(w.qdp0, np.float32)
+    t0 = time.perf_counter()
+    c = trmc.matrix_coeffs(p32)
+    t1 = time.perf_counter()
+    PQ, Pq = trmc.matrix_build(t.reach_up_ptr, t.reach_up_idx, c, tol)
+    t2 = time.perf_counter()
+    nnz = (len(PQ[1]), len(Pq[1]))
+    sweep = {}
+    for lanes in (8, 16, 32):
+        _, ms = trmc.matrix_route(PQ, Pq, qlat, Q0, T, lanes_per_row=lanes, reps=args.steps)
+        sweep[lanes] = ms
+        log(f"[bench] matrix: lanes {lanes}: {ms:.3f} ms per {T}-step pass")
+    lanes = 0
+    out, ms = trmc.matrix_route(PQ, Pq, qlat, Q0, T, lanes_per_row=lanes, reps=args.steps)
+    step_ms = ms / T
+    rows = lambda m: np.repeat(np.arange(n, dtype=np.int64), np.diff(m[0]))
+    union = len(np.union1d(rows(PQ) * n + PQ[1], rows(Pq) * n + Pq[1]))
+    step_bytes = 12.0 * union + 16.0 * n + 20.0 * n
+    achieved = step_bytes / (step_ms * 1e-3) / 1e9
+    log(f"[bench] matrix: {n} reaches, nnz {nnz}, {step_ms * 1e3:.1f} us per step, {achieved:.0f} GB/s")
+    res = {
+        "metric": "reach-steps/sec, matrix-form linearized MC (linear_mc_matrix_graph.cu)",
+        "value": n * T / (ms * 1e-3), "unit": "reach-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded Shreve-style tree, one segment per reach; inputs resident)",
+        "config": {"workload": f"matrix-309k: {n} one-segment reaches, {t.n_levels} levels, {T} steps of "
+                               f"dt=300 s, tol {tol:g}", "reaches": n, "levels": t.n_levels, "timesteps": T,
+                   "tol": tol, "nnz_P_Q": nnz[0], "nnz_P_q": nnz[1], "union_entries": union,
+                   "lanes_per_row": "by union row length U (smallest power of two G with 4 G >= U, <= 64)",
+                   "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "ms_per_timestep": step_ms,
+        "lanes_sweep_ms_per_pass": sweep,
+        "setup_s": {"coefficients_gpu": t1 - t0, "matrices_host": t2 - t1},
+    }
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        want = orc.matrix_route(PQ, Pq, qlat, Q0, T, lanes=lanes)
+        cs = time.perf_counter() - t0
+        seq = orc.matrix_route(PQ, Pq, qlat, Q0, T, lanes=1)
+        truth = orc.linear_mc_fp64(c, t, qlat, Q0, T)
+        rel = np.abs(out.astype(np.float64) - truth) / np.abs(truth)
+        res["cpu_baseline"] = {"value": n * T / cs, "unit": "reach-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"the whole {T}-step pass ({cs:.2f} s), oracle step loop, one thread"}
+        res["parity"] = {"against": "port", "bitwise_equal_frac": float(np.mean(out.view(np.uint32) ==
+                                                                             want.view(np.uint32))),
+                         "max_rel_vs_sequential_order": float(np.max(np.abs(out - seq) / np.abs(seq))),
+                         "vs_fp64_recurrence": {"max_rel": float(rel.max()),
+                                                "frac_within_1pct": float(np.mean(rel < 0.01))}}
+        res["speedup_vs_cpu"] = res["value"] / res["cpu_baseline"]["value"]
+        if orc.load_ref() is not None and hasattr(orc.load_ref(), "ref_matrix_build"):
+            t0 = time.perf_counter()
+            orc.matrix_build(t.reach_up_ptr, t.reach_up_idx, c, tol, reference=True)
+            res["setup_s"]["reference_matrices_host"] = time.perf_counter() - t0
+        log(f"[bench] matrix cpu {res['cpu_baseline']['value'] / 1e6:.1f} M reach-steps/s; parity {res['parity']}")
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -415,6 +497,9 @@ def main():
         return
     if spec["mode"] == "diffusive":
         bench_diffusive(trmc, spec, args)
+        return
+    if spec["mode"] == "matrix":
+        bench_matrix(trmc, spec, args)
         return
     T, M = spec["T"], spec["M"]
     head = args.mode or spec["mode"]

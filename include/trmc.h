@@ -193,6 +193,57 @@ int trmc_linear_mc_f64(const float *C1, const float *C2, const float *C3, const 
 int trmc_linear_coeffs_f32(const trmc_chparams_f32 *params, int n, float dt,
                            float *C1, float *C2, float *C3, float *C4oq);
 
+/* Matrix form of the linearized MC (wavefront/linear_mc_matrix.cu,
+ * linear_mc_matrix_graph.cu), one segment per reach, reaches numbered so that
+ * every upstream reach precedes its downstream one.  Each step is
+ *   Q_new = P_Q Q_old + P_q qlat_t
+ * with P_Q = (I - C1 N)^-1 (C2 N + C3 I) and P_q = (I - C1 N)^-1 diag(C4/ql),
+ * N the upstream adjacency; entries below a tolerance are dropped. */
+typedef struct {
+    int    n, nnz;
+    int   *rowptr;   /* [n + 1] */
+    int   *col;      /* [nnz], ascending within a row */
+    float *val;      /* [nnz] */
+} trmc_csr;
+
+/* compute_mc_coeffs_cpu (linear_mc_matrix.cu:58-84): the linearized MC
+ * coefficients at half bankfull depth with the weighting X = 0.25, float, on
+ * the GPU (each power is the float rounding of the double pow). */
+int trmc_matrix_coeffs_f32(const trmc_chparams_f32 *params, int n, float dt,
+                           float *C1, float *C2, float *C3, float *C4oq);
+
+/* build_propagation_matrices (linear_mc_matrix.cu:118-229): P_Q and P_q row
+ * by row in reach order, each row C1 x the sum of its upstream rows (pruned
+ * below tol), plus C2 at each upstream column and C3 (P_Q) or C4/ql (P_q) on
+ * the diagonal, pruned again; the reference's float operations in its order,
+ * on the host.  An upstream index not below its reach is TRMC_ERR_TOPOLOGY.
+ * Fills *P_Q and *P_q (free with trmc_free_csr). */
+int trmc_matrix_build(int n_reaches, const int *reach_up_ptr, const int *reach_up_idx,
+                      const float *C1, const float *C2, const float *C3, const float *C4oq,
+                      float tol, trmc_csr *P_Q, trmc_csr *P_q);
+void trmc_free_csr(trmc_csr *m);
+
+/* The timestep loop of linear_mc_matrix.cu:340-380 and
+ * linear_mc_matrix_graph.cu:355-392: n_timesteps steps from Q0 [n] with
+ * lateral inflow qlat [T][n], both products and the sum fused in one FP32
+ * kernel per step, the steps replayed as one HIP graph; Q_final [n] = the flow
+ * after the last step.  Columns must ascend within each row.  Summation order
+ * (fixed, so results do not depend on scheduling): each row is the union of
+ * its two CSR rows by column (a matrix without an entry there contributes
+ * 0 * value); G lanes share a row, lane l taking union entries l, l + G, ...,
+ * each as s = s + vQ * Q[c] then s = s + vq * ql[c]; the lanes' sums are
+ * added pairwise by xor distance G/2, ..., 1.  lanes_per_row = G for every
+ * row (1, 2, 4, ..., 64), or 0: each row's G from its union length U, the
+ * smallest power of two with 4 G >= U, at most 64. */
+int trmc_matrix_route_f32(const trmc_csr *P_Q, const trmc_csr *P_q, const float *qlat,
+                          const float *Q0, int n_timesteps, int lanes_per_row, float *Q_final);
+/* [new] The same with reps timed replays of the whole graph after one
+ * untimed one, inputs resident; *avg_pass_ms = mean HIP-event time of one
+ * replay (all n_timesteps steps). */
+int trmc_matrix_route_timed_f32(const trmc_csr *P_Q, const trmc_csr *P_q, const float *qlat,
+                                const float *Q0, int n_timesteps, int lanes_per_row, float *Q_final,
+                                int reps, double *avg_pass_ms);
+
 /* Diffusive-wave step (owp_extended_kernels.cu:23-135): one implicit
  * (theta-weighted) diffusion step per reach, solved by forward elimination and
  * back substitution, FP32.  Node i of reach j is element

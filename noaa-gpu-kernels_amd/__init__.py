@@ -55,6 +55,11 @@ class ChParamsF64(C.Structure):
     _fields_ = [(n, C.c_double) for n in PARAM_FIELDS]
 
 
+class Csr(C.Structure):
+    _fields_ = [("n", C.c_int), ("nnz", C.c_int), ("rowptr", C.POINTER(C.c_int)), ("col", C.POINTER(C.c_int)),
+                ("val", C.POINTER(C.c_float))]
+
+
 class Options(C.Structure):
     _fields_ = [
         ("mode", C.c_int), ("schedule", C.c_int), ("unit_segments", C.c_int),
@@ -143,6 +148,11 @@ def _declare(L: C.CDLL) -> None:
         "trmc_load_forcings": (I, [C.c_char_p, I, I, pI, pF, pF, pF, pF]),
         "trmc_save_forcings": (I, [C.c_char_p, I, I, I, pF, pF, pF, pF]),
         "trmc_free": (None, [V]),
+        "trmc_matrix_coeffs_f32": (I, [_p(ChParamsF32), I, C.c_float] + [pF] * 4),
+        "trmc_matrix_build": (I, [I, pI, pI] + [pF] * 4 + [C.c_float, _p(Csr), _p(Csr)]),
+        "trmc_free_csr": (None, [_p(Csr)]),
+        "trmc_matrix_route_f32": (I, [_p(Csr), _p(Csr), pF, pF, I, I, pF]),
+        "trmc_matrix_route_timed_f32": (I, [_p(Csr), _p(Csr), pF, pF, I, I, pF, I, pD]),
         "trmc_build_flowpath_network": (I, [_p(C.c_int64), _p(C.c_int64), I, _p(C.c_int64), pD, I, _p(Topo),
                                             _p(_p(ChParamsF32)), _p(_p(C.c_int64))]),
         "trmc_generate_network": (I, [I, I, I, C.c_uint64, _p(Topo)]),
@@ -537,6 +547,65 @@ def ingest_nex_csv_to_forcings(csv_dir: str, wbid_npy: str, out_path: str, n_tim
                                                  int(n_timesteps), int(n_threads),
                                                  cnt.ctypes.data_as(C.POINTER(C.c_int64))))
     return int(cnt[0]), int(cnt[1])
+
+
+def matrix_coeffs(params, dt: float = 300.0):
+    """compute_mc_coeffs_cpu (linear_mc_matrix.cu:58-84) on the GPU: the
+    linearized MC coefficients with X = 0.25, float [n] each."""
+    p = params_f32(params)
+    n = p.shape[0]
+    out = [np.zeros(n, np.float32) for _ in range(4)]
+    _check(lib().trmc_matrix_coeffs_f32(_ptr(p, ChParamsF32), n, float(dt), *[_ptr(o, C.c_float) for o in out]))
+    return tuple(out)
+
+
+def matrix_build(up_ptr, up_idx, coeffs, tol: float = 1e-6):
+    """build_propagation_matrices (linear_mc_matrix.cu:118-229): (P_Q, P_q), each
+    (rowptr, col, val) numpy arrays."""
+    ptr, idx = _i32(up_ptr), _i32(up_idx)
+    if len(idx) == 0:
+        idx = np.zeros(1, np.int32)
+    cs = [_f32(c) for c in coeffs]
+    mQ, mq = Csr(), Csr()
+    _check(lib().trmc_matrix_build(len(ptr) - 1, _ptr(ptr, C.c_int), _ptr(idx, C.c_int),
+                                   *[_ptr(c, C.c_float) for c in cs], float(tol), C.byref(mQ), C.byref(mq)))
+    try:
+        out = []
+        for m in (mQ, mq):
+            rp = np.ctypeslib.as_array(m.rowptr, (m.n + 1,)).copy()
+            col = np.ctypeslib.as_array(m.col, (m.nnz,)).copy() if m.nnz else np.zeros(0, np.int32)
+            val = np.ctypeslib.as_array(m.val, (m.nnz,)).copy() if m.nnz else np.zeros(0, np.float32)
+            out.append((rp, col, val))
+    finally:
+        lib().trmc_free_csr(C.byref(mQ))
+        lib().trmc_free_csr(C.byref(mq))
+    return tuple(out)
+
+
+def _csr_view(m):
+    rp, col, val = _i32(m[0]), _i32(m[1]), _f32(m[2])
+    keep = (rp, col if len(col) else np.zeros(1, np.int32), val if len(val) else np.zeros(1, np.float32))
+    c = Csr(len(rp) - 1, len(col), _ptr(keep[0], C.c_int), _ptr(keep[1], C.c_int), _ptr(keep[2], C.c_float))
+    return c, keep
+
+
+def matrix_route(P_Q, P_q, qlat, Q0, n_timesteps: int, lanes_per_row: int = 0, reps: int = 0):
+    """The matrix-form time loop on the GPU (linear_mc_matrix_graph.cu:355-392):
+    Q after n_timesteps steps, float [n]; with reps > 0 also the mean time of
+    one whole replay in ms (inputs resident)."""
+    cQ, kQ = _csr_view(P_Q)
+    cq, kq = _csr_view(P_q)
+    q, q0 = _f32(qlat), _f32(Q0)
+    out = np.zeros(cQ.n, np.float32)
+    if reps > 0:
+        ms = C.c_double()
+        _check(lib().trmc_matrix_route_timed_f32(C.byref(cQ), C.byref(cq), _ptr(q, C.c_float), _ptr(q0, C.c_float),
+                                                 int(n_timesteps), int(lanes_per_row), _ptr(out, C.c_float),
+                                                 int(reps), C.byref(ms)))
+        return out, ms.value
+    _check(lib().trmc_matrix_route_f32(C.byref(cQ), C.byref(cq), _ptr(q, C.c_float), _ptr(q0, C.c_float),
+                                       int(n_timesteps), int(lanes_per_row), _ptr(out, C.c_float)))
+    return out
 
 
 HYDROFABRIC_ATTRS = ("Length_m", "n", "nCC", "BtmWdth", "TopWdth", "TopWdthCC", "ChSlp", "So")

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -958,8 +959,8 @@ int trmc_diffusive_tridiag_timed_f32(const trmc_diff_reach *rp, const float *dx,
     return diffusive_run(rp, dx, diffusivity, qp_prev, qp_out, nreach, reps, avg_kernel_ms);
 }
 
-int trmc_linear_coeffs_f32(const trmc_chparams_f32 *params, int n, float dt, float *C1, float *C2, float *C3,
-                           float *C4oq)
+static int coeffs_run(const trmc_chparams_f32 *params, int n, float dt, int fixed_x, float *C1, float *C2,
+                      float *C3, float *C4oq)
 {
     if (n < 0) return fail(TRMC_ERR_ARG, "negative n");
     if (n == 0) return TRMC_OK;
@@ -971,7 +972,7 @@ int trmc_linear_coeffs_f32(const trmc_chparams_f32 *params, int n, float dt, flo
     int rc = dev_upload(&d_p, reinterpret_cast<const float *>(params), (size_t)n * 8, nullptr);
     if (!rc) rc = dev_alloc(&d_out, (size_t)4 * n);
     hipError_t e = hipSuccess;
-    if (!rc) e = launch_linear_coeffs(d_p, n, dt, d_out, nullptr);
+    if (!rc) e = launch_linear_coeffs(d_p, n, dt, fixed_x, d_out, nullptr);
     std::vector<float> out;
     if (!rc && e == hipSuccess) {
         out.resize((size_t)4 * n);
@@ -984,6 +985,188 @@ int trmc_linear_coeffs_f32(const trmc_chparams_f32 *params, int n, float dt, flo
     float *dst[4] = {C1, C2, C3, C4oq};
     for (int k = 0; k < 4; ++k) std::copy(out.begin() + (size_t)k * n, out.begin() + (size_t)(k + 1) * n, dst[k]);
     return TRMC_OK;
+}
+
+int trmc_linear_coeffs_f32(const trmc_chparams_f32 *params, int n, float dt, float *C1, float *C2, float *C3,
+                           float *C4oq)
+{
+    return coeffs_run(params, n, dt, 0, C1, C2, C3, C4oq);
+}
+
+int trmc_matrix_coeffs_f32(const trmc_chparams_f32 *params, int n, float dt, float *C1, float *C2, float *C3,
+                           float *C4oq)
+{
+    return coeffs_run(params, n, dt, 1, C1, C2, C3, C4oq);
+}
+
+// Host checks of one propagation matrix: the kernel gathers x[col] for every
+// entry, so every column must index the vector.
+static int check_csr(const trmc_csr *m, int n, const char *name)
+{
+    if (!m) return fail(TRMC_ERR_ARG, "%s is null", name);
+    if (m->n != n) return fail(TRMC_ERR_ARG, "%s has %d rows, expected %d", name, m->n, n);
+    if (m->nnz < 0 || !m->rowptr || (m->nnz > 0 && (!m->col || !m->val)))
+        return fail(TRMC_ERR_ARG, "%s: null array or negative nnz", name);
+    if (m->rowptr[0] != 0 || m->rowptr[n] != m->nnz)
+        return fail(TRMC_ERR_ARG, "%s: rowptr must run from 0 to nnz", name);
+    for (int r = 0; r < n; ++r)
+        if (m->rowptr[r + 1] < m->rowptr[r]) return fail(TRMC_ERR_ARG, "%s: rowptr decreases at row %d", name, r);
+    for (int r = 0; r < n; ++r)
+        for (int k = m->rowptr[r]; k < m->rowptr[r + 1]; ++k) {
+            if (m->col[k] < 0 || m->col[k] >= n)
+                return fail(TRMC_ERR_ARG, "%s: column %d out of range", name, m->col[k]);
+            if (k > m->rowptr[r] && m->col[k] <= m->col[k - 1])
+                return fail(TRMC_ERR_ARG, "%s: columns of row %d not ascending", name, r);
+        }
+    return TRMC_OK;
+}
+
+static int matrix_run(const trmc_csr *PQ, const trmc_csr *Pq, const float *qlat, const float *Q0, int T,
+                      int lanes, float *Q_final, int reps, double *avg_ms)
+{
+    if (!PQ) return fail(TRMC_ERR_ARG, "P_Q is null");
+    const int n = PQ->n;
+    if (n < 0 || T < 0) return fail(TRMC_ERR_ARG, "negative size");
+    int rc = check_csr(PQ, n, "P_Q");
+    if (!rc) rc = check_csr(Pq, n, "P_q");
+    if (rc) return rc;
+    if (lanes < 0 || lanes > 64 || (lanes & (lanes - 1)))
+        return fail(TRMC_ERR_ARG, "lanes_per_row %d is neither 0 nor a power of two in 1..64", lanes);
+    if (n == 0) return TRMC_OK;
+    if (!Q0 || !Q_final || (T > 0 && !qlat)) return fail(TRMC_ERR_ARG, "null array");
+    if (T == 0) {
+        std::copy(Q0, Q0 + n, Q_final);
+        if (avg_ms) *avg_ms = 0.0;
+        return TRMC_OK;
+    }
+    if ((rc = check_device(0, nullptr, nullptr))) return rc;
+    // the union of each row's two CSR rows: column, {P_Q value, P_q value}
+    std::vector<int> urow((size_t)n + 1), ucol;
+    std::vector<float2> uval;
+    ucol.reserve((size_t)std::max(PQ->nnz, Pq->nnz));
+    uval.reserve((size_t)std::max(PQ->nnz, Pq->nnz));
+    urow[0] = 0;
+    for (int r = 0; r < n; ++r) {
+        int a = PQ->rowptr[r], b = Pq->rowptr[r];
+        const int ae = PQ->rowptr[r + 1], be = Pq->rowptr[r + 1];
+        while (a < ae || b < be) {
+            const int ca = a < ae ? PQ->col[a] : INT_MAX, cb = b < be ? Pq->col[b] : INT_MAX;
+            const int c = std::min(ca, cb);
+            ucol.push_back(c);
+            uval.push_back(make_float2(ca == c ? PQ->val[a++] : 0.0f, cb == c ? Pq->val[b++] : 0.0f));
+        }
+        if (ucol.size() > (size_t)INT_MAX) return fail(TRMC_ERR_ARG, "union of the matrices exceeds 2^31 entries");
+        urow[r + 1] = (int)ucol.size();
+    }
+    if (ucol.empty()) { ucol.push_back(0); uval.push_back(make_float2(0.0f, 0.0f)); }
+    // lanes per row: fixed, or by the row's union length U (smallest power of
+    // two G with 4 G >= U, at most 64); slots hold the rows of larger G first,
+    // in row order within one G; blocks hold one G each
+    std::vector<int> gof((size_t)n), order((size_t)n);
+    for (int r = 0; r < n; ++r) {
+        int G = lanes;
+        if (G == 0) {
+            const int U = urow[r + 1] - urow[r];
+            for (G = 1; G < 64 && G * 4 < U; G *= 2) {}
+        }
+        gof[r] = G;
+        order[r] = r;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return gof[a] > gof[b]; });
+    std::vector<int4> slots((size_t)n), blocks;
+    for (int i = 0; i < n; ++i) slots[i] = make_int4(order[i], urow[order[i]], urow[order[i] + 1], 0);
+    for (int i = 0; i < n;) {
+        const int G = gof[order[i]], per = 256 / G;
+        int k = 0;
+        while (k < per && i + k < n && gof[order[i + k]] == G) ++k;
+        blocks.push_back(make_int4(G, i, k, 0));
+        i += k;
+    }
+    // state pairs {Q, ql of the step about to run}
+    std::vector<float2> z0((size_t)n);
+    for (int r = 0; r < n; ++r) z0[r] = make_float2(Q0[r], qlat[r]);
+    int4 *d_slots = nullptr, *d_blk = nullptr;
+    int *d_col = nullptr;
+    float2 *d_val = nullptr, *d_za = nullptr, *d_zb = nullptr;
+    float *d_ql = nullptr;
+    hipStream_t st = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(TRMC_ERR_HIP, "stream: %s", hipGetErrorString(e));
+    rc = dev_upload(&d_slots, slots.data(), slots.size(), st);
+    if (!rc) rc = dev_upload(&d_blk, blocks.data(), blocks.size(), st);
+    if (!rc) rc = dev_upload(&d_col, ucol.data(), ucol.size(), st);
+    if (!rc) rc = dev_upload(&d_val, uval.data(), uval.size(), st);
+    if (!rc) rc = dev_upload(&d_ql, qlat, (size_t)T * n, st);
+    if (!rc) rc = dev_upload(&d_za, z0.data(), (size_t)n, st);
+    if (!rc) rc = dev_alloc(&d_zb, (size_t)n);
+    std::vector<float2> zf;
+    float ms = 0.0f;
+    if (!rc) {
+        e = hipStreamSynchronize(st);
+        // the T steps as one graph: step t reads one state buffer and writes
+        // the other, with the lateral inflow of step t + 1 (of step 0 after
+        // the last, so a replay starts from well-formed pairs)
+        if (e == hipSuccess) e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            hipError_t le = hipSuccess;
+            for (int t = 0; t < T && le == hipSuccess; ++t) {
+                const float2 *zi = (t & 1) ? d_zb : d_za;
+                float2 *zo = (t & 1) ? d_za : d_zb;
+                le = launch_matrix_step((int)blocks.size(), d_blk, d_slots, d_col, d_val, zi,
+                                        d_ql + (size_t)((t + 1) % T) * n, zo, st);
+            }
+            e = hipStreamEndCapture(st, &graph);
+            if (e == hipSuccess) e = le;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e == hipSuccess) e = hipGraphLaunch(exec, st);
+        if (e == hipSuccess) {
+            zf.resize((size_t)n);
+            e = hipMemcpyAsync(zf.data(), (T & 1) ? d_zb : d_za, (size_t)n * sizeof(float2), hipMemcpyDeviceToHost,
+                               st);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        // timed replays continue from the state the previous one left: the
+        // work of a step does not depend on the values
+        if (e == hipSuccess && reps > 0) {
+            e = hipEventCreate(&e0);
+            if (e == hipSuccess) e = hipEventCreate(&e1);
+            if (e == hipSuccess) e = hipEventRecord(e0, st);
+            for (int r = 0; r < reps && e == hipSuccess; ++r) e = hipGraphLaunch(exec, st);
+            if (e == hipSuccess) e = hipEventRecord(e1, st);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        }
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(st);
+    dev_free(d_slots); dev_free(d_blk); dev_free(d_col); dev_free(d_val); dev_free(d_ql); dev_free(d_za);
+    dev_free(d_zb);
+    (void)hipStreamDestroy(st);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(TRMC_ERR_HIP, "matrix route: %s", hipGetErrorString(e));
+    for (int r = 0; r < n; ++r) Q_final[r] = zf[r].x;
+    if (avg_ms) *avg_ms = reps > 0 ? (double)ms / reps : 0.0;
+    return TRMC_OK;
+}
+
+int trmc_matrix_route_f32(const trmc_csr *P_Q, const trmc_csr *P_q, const float *qlat, const float *Q0,
+                          int n_timesteps, int lanes_per_row, float *Q_final)
+{
+    return matrix_run(P_Q, P_q, qlat, Q0, n_timesteps, lanes_per_row, Q_final, 0, nullptr);
+}
+
+int trmc_matrix_route_timed_f32(const trmc_csr *P_Q, const trmc_csr *P_q, const float *qlat, const float *Q0,
+                                int n_timesteps, int lanes_per_row, float *Q_final, int reps, double *avg_pass_ms)
+{
+    if (reps < 1 || !avg_pass_ms) return fail(TRMC_ERR_ARG, "reps >= 1 and avg_pass_ms required");
+    return matrix_run(P_Q, P_q, qlat, Q0, n_timesteps, lanes_per_row, Q_final, reps, avg_pass_ms);
 }
 
 int trmc_linear_mc_f64(const float *C1, const float *C2, const float *C3, const float *C4oq,

@@ -82,7 +82,13 @@ struct DiffReach { int ncomp; float theta, dtini; };
 hipError_t launch_diffusive_tridiag(const DiffReach *rp, const float *dx, const float *diffusivity,
                                     const float *qp_prev, float *qp_out, float *ws, int nreach, hipStream_t stream);
 // params: n ChParams of 8 floats (dx, bw, tw, twcc, n_ch, ncc, cs, s0); out [4][n]
-hipError_t launch_linear_coeffs(const float *params, int n, float dt, float *out, hipStream_t stream);
+hipError_t launch_linear_coeffs(const float *params, int n, float dt, int fixed_x, float *out, hipStream_t stream);
+// matrix-form linearized MC step over the union rows of P_Q and P_q:
+// col[k], val[k] = {P_Q value, P_q value}; z[c] = {Q, ql}; blocks[b] = {G in
+// 1, 2, 4, ..., 64, first slot, slots <= 256 / G, 0}; slots[i] = {row, first
+// entry, end, 0}; writes z_out[row] = {new Q, ql_next[row]}
+hipError_t launch_matrix_step(int n_blocks, const int4 *blocks, const int4 *slots, const int *col, const float2 *val,
+                              const float2 *z, const float *ql_next, float2 *z_out, hipStream_t stream);
 hipError_t launch_gen_forcing(double *dst, const int *seg_of_dev, int nd, int N, int M, int T,
                               unsigned long long seed, hipStream_t stream);
 hipError_t launch_reset(unsigned *bar, int nwords, hipStream_t stream);

@@ -868,8 +868,11 @@ hipError_t route_dataflow_il_occupancy(int mode, int block, int *blocks_per_cu)
 // Linearized MC coefficients, linear_mc.cu:57-100 (compute_mc_coeffs): float
 // arithmetic at the reference depth h = max(0.5 bankfull, 0.1); each power is
 // the float rounding of the double pow (correctly rounded division and square
-// root are the defaults for float).  out [4][n]: C1, C2, C3, C4/ql.
-__global__ void k_linear_coeffs(const float *__restrict__ params, int n, float dt, float *__restrict__ out)
+// root are the defaults for float).  out [4][n]: C1, C2, C3, C4/ql.  fixed_x:
+// the weighting X = 0.25 of the matrix form (linear_mc_matrix.cu:58-84) in
+// place of the Cunge formula.
+__global__ void k_linear_coeffs(const float *__restrict__ params, int n, float dt, int fixed_x,
+                                float *__restrict__ out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -892,7 +895,7 @@ __global__ void k_linear_coeffs(const float *__restrict__ params, int n, float d
     const float Km = fmaxf(dt, dx / Ck);
     const float tw_use = (h > bfd && twcc > 0.0f) ? twcc : twl;
     float X = 0.25f;
-    if (Ck > 0.0f && tw_use * s0 * Ck * dx > 0.0f)
+    if (!fixed_x && Ck > 0.0f && tw_use * s0 * Ck * dx > 0.0f)
         X = fminf(0.5f, fmaxf(0.0f, 0.5f * (1.0f - 0.0f / (2.0f * tw_use * s0 * Ck * dx))));
     float D = Km * (1.0f - X) + dt / 2.0f;
     if (D == 0.0f) D = 1.0f;
@@ -956,6 +959,89 @@ __global__ void __launch_bounds__(256) k_diffusive_tridiag(const DiffReach *__re
     }
 }
 
+// One step of the matrix-form linearized MC, linear_mc_matrix_graph.cu:153-205
+// (kernel_spmv_fused): y = P_Q x + P_q ql for every row, both products and
+// their sum in one pass.  The two matrices share nearly all of their pattern
+// (a row's columns are the reach and its ancestors), so a row is stored once
+// as the union of its two CSR rows: a column and the pair {P_Q value, P_q
+// value} (0 where a matrix has no entry), and the vector is stored as pairs
+// z[c] = {Q[c], ql[c]}, so one 8-byte gather serves both products.  Per
+// entry, in column order: s = s + vQ * Q[c], then s = s + vq * ql[c] (a
+// product with a missing entry's 0 adds a zero, which leaves a sum started at
+// +0 unchanged).  G lanes share a row: lane l takes union entries l, l + G,
+// ...; the G partial sums are added by xor distance G/2, ..., 1, which every
+// lane of the group computes alike, so the row's value is fixed by its G.
+// Each lane issues kMatrixUnroll entry loads, then their gathers, then adds in
+// order, so the loads of a row overlap.  The step writes z_out[r] = {y[r],
+// ql of the next step at r}.
+//
+// Work layout: the host orders the rows into slots (rows of equal G together,
+// in row order) with slot[i] = {row, first entry, end}; each 256-lane block
+// takes rows of one G (block table {G, first slot, slots}), 256 / G rows each.
+constexpr int kMatrixUnroll = 4;
+
+template <int G>
+__device__ __forceinline__ void matrix_rows(int slot0, int nslots, const int4 *__restrict__ slots,
+                                            const int *__restrict__ col, const float2 *__restrict__ val,
+                                            const float2 *__restrict__ z, const float *__restrict__ ql_next,
+                                            float2 *__restrict__ z_out)
+{
+    const int i = threadIdx.x / G, l = threadIdx.x % G;
+    const bool live = i < nslots;
+    float s = 0.0f;
+    int row = 0;
+    if (live) {
+        const int4 sl = slots[slot0 + i];
+        row = sl.x;
+        for (int k = sl.y + l; k < sl.z; k += G * kMatrixUnroll) {
+            int c[kMatrixUnroll];
+            float2 v[kMatrixUnroll], zc[kMatrixUnroll];
+#pragma unroll
+            for (int u = 0; u < kMatrixUnroll; ++u) {
+                const bool in = k + u * G < sl.z;
+                c[u] = in ? col[k + u * G] : 0;
+                v[u] = in ? val[k + u * G] : make_float2(0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int u = 0; u < kMatrixUnroll; ++u) zc[u] = (k + u * G < sl.z) ? z[c[u]] : make_float2(0.0f, 0.0f);
+#pragma unroll
+            for (int u = 0; u < kMatrixUnroll; ++u)
+                if (k + u * G < sl.z) {
+                    s = s + v[u].x * zc[u].x;
+                    s = s + v[u].y * zc[u].y;
+                }
+        }
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) s = s + __shfl_xor(s, o, G);
+    if (live && l == 0) z_out[row] = make_float2(s, ql_next[row]);
+}
+
+__global__ void __launch_bounds__(256) k_matrix_step(const int4 *__restrict__ blocks, const int4 *__restrict__ slots,
+                                                     const int *__restrict__ col, const float2 *__restrict__ val,
+                                                     const float2 *__restrict__ z, const float *__restrict__ ql_next,
+                                                     float2 *__restrict__ z_out)
+{
+    const int4 bt = blocks[blockIdx.x];   // {G, first slot, slots, 0}
+    switch (bt.x) {
+    case 1: matrix_rows<1>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
+    case 2: matrix_rows<2>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
+    case 4: matrix_rows<4>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
+    case 8: matrix_rows<8>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
+    case 16: matrix_rows<16>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
+    case 32: matrix_rows<32>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
+    default: matrix_rows<64>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
+    }
+}
+
+hipError_t launch_matrix_step(int n_blocks, const int4 *blocks, const int4 *slots, const int *col, const float2 *val,
+                              const float2 *z, const float *ql_next, float2 *z_out, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_matrix_step, dim3(n_blocks), dim3(256), 0, stream, blocks, slots, col, val, z, ql_next,
+                       z_out);
+    return hipGetLastError();
+}
+
 hipError_t launch_diffusive_tridiag(const DiffReach *rp, const float *dx, const float *diffusivity,
                                     const float *qp_prev, float *qp_out, float *ws, int nreach, hipStream_t stream)
 {
@@ -965,10 +1051,10 @@ hipError_t launch_diffusive_tridiag(const DiffReach *rp, const float *dx, const 
     return hipGetLastError();
 }
 
-hipError_t launch_linear_coeffs(const float *params, int n, float dt, float *out, hipStream_t stream)
+hipError_t launch_linear_coeffs(const float *params, int n, float dt, int fixed_x, float *out, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_linear_coeffs, dim3((n + 255) / 256), dim3(256), 0, stream, params, n, dt, out);
+    hipLaunchKernelGGL(k_linear_coeffs, dim3((n + 255) / 256), dim3(256), 0, stream, params, n, dt, fixed_x, out);
     return hipGetLastError();
 }
 

@@ -587,8 +587,8 @@ static float powf_from_double(float x, double y)
 /* wavefront/linear_mc.cu:57-100: coefficients at the reference depth
  * h = max(0.5 bankfull, 0.1), weighting X from the Cunge formula with Qj = 0.
  * The reference evaluates 2.0f / 3.0f and 5.0f / 3.0f in float. */
-void orc_linear_coeffs(const orc_chparams_f32 *params, int n, float dt,
-                       float *C1, float *C2, float *C3, float *C4oq)
+static void linear_coeffs(const orc_chparams_f32 *params, int n, float dt, int fixed_x,
+                          float *C1, float *C2, float *C3, float *C4oq)
 {
     for (int i = 0; i < n; ++i) {
         const orc_chparams_f32 *p = &params[i];
@@ -610,7 +610,7 @@ void orc_linear_coeffs(const orc_chparams_f32 *params, int n, float dt,
         float Km = fmaxf(dt, p->dx / Ck);
         float tw_use = (h > bfd && p->twcc > 0.0f) ? p->twcc : twl;
         float X = 0.25f;
-        if (Ck > 0.0f && tw_use * p->s0 * Ck * p->dx > 0.0f)
+        if (!fixed_x && Ck > 0.0f && tw_use * p->s0 * Ck * p->dx > 0.0f)
             X = fminf(0.5f, fmaxf(0.0f, 0.5f * (1.0f - 0.0f / (2.0f * tw_use * p->s0 * Ck * p->dx))));
         float D = Km * (1.0f - X) + dt / 2.0f;
         if (D == 0.0f) D = 1.0f;
@@ -619,6 +619,190 @@ void orc_linear_coeffs(const orc_chparams_f32 *params, int n, float dt,
         C3[i] = (Km * (1.0f - X) - dt / 2.0f) / D;
         C4oq[i] = dt / D;
     }
+}
+
+void orc_linear_coeffs(const orc_chparams_f32 *params, int n, float dt,
+                       float *C1, float *C2, float *C3, float *C4oq)
+{
+    linear_coeffs(params, n, dt, 0, C1, C2, C3, C4oq);
+}
+
+/* wavefront/linear_mc_matrix.cu:58-84 (compute_mc_coeffs_cpu): the same
+ * arithmetic with the weighting fixed at X = 0.25. */
+void orc_matrix_coeffs(const orc_chparams_f32 *params, int n, float dt,
+                       float *C1, float *C2, float *C3, float *C4oq)
+{
+    linear_coeffs(params, n, dt, 1, C1, C2, C3, C4oq);
+}
+
+/* ---- matrix form: propagation matrices (linear_mc_matrix.cu:118-229) ----
+ * Rows as sorted (column, value) lists.  Row r: the upstream rows merged one
+ * after another into an initially empty list, each scaled by C1[r] (a column
+ * new to the list takes the product, a present one adds it), pruned (drop
+ * |v| < tol), then C2[r] added at each upstream column and C3[r] (P_Q) or
+ * C4/ql[r] (P_q) at column r (inserted when absent), pruned again. */
+typedef struct { int n, cap; int *col; float *val; } orc_row;
+
+static void row_reserve(orc_row *r, int need)
+{
+    if (need <= r->cap) return;
+    int cap = r->cap ? r->cap : 4;
+    while (cap < need) cap *= 2;
+    r->col = (int *)realloc(r->col, sizeof(int) * (size_t)cap);
+    r->val = (float *)realloc(r->val, sizeof(float) * (size_t)cap);
+    r->cap = cap;
+}
+
+/* dst = merge(dst, scale * src), both sorted by column */
+static void row_merge_scaled(orc_row *dst, const orc_row *src, float scale, orc_row *tmp)
+{
+    tmp->n = 0;
+    row_reserve(tmp, dst->n + src->n);
+    int i = 0, j = 0;
+    while (i < dst->n && j < src->n) {
+        if (dst->col[i] < src->col[j]) {
+            tmp->col[tmp->n] = dst->col[i]; tmp->val[tmp->n++] = dst->val[i++];
+        } else if (dst->col[i] > src->col[j]) {
+            tmp->col[tmp->n] = src->col[j]; tmp->val[tmp->n++] = src->val[j] * scale; j++;
+        } else {
+            tmp->col[tmp->n] = dst->col[i]; tmp->val[tmp->n++] = dst->val[i] + src->val[j] * scale;
+            i++; j++;
+        }
+    }
+    while (i < dst->n) { tmp->col[tmp->n] = dst->col[i]; tmp->val[tmp->n++] = dst->val[i++]; }
+    while (j < src->n) { tmp->col[tmp->n] = src->col[j]; tmp->val[tmp->n++] = src->val[j] * scale; j++; }
+    orc_row t = *dst; *dst = *tmp; *tmp = t;
+}
+
+static void row_add_col(orc_row *r, int col, float v)
+{
+    int lo = 0, hi = r->n;
+    while (lo < hi) { int m = (lo + hi) / 2; if (r->col[m] < col) lo = m + 1; else hi = m; }
+    if (lo < r->n && r->col[lo] == col) { r->val[lo] += v; return; }
+    row_reserve(r, r->n + 1);
+    memmove(r->col + lo + 1, r->col + lo, sizeof(int) * (size_t)(r->n - lo));
+    memmove(r->val + lo + 1, r->val + lo, sizeof(float) * (size_t)(r->n - lo));
+    r->col[lo] = col; r->val[lo] = v; r->n++;
+}
+
+static void row_prune(orc_row *r, float tol)
+{
+    int k = 0;
+    for (int i = 0; i < r->n; i++)
+        if (!(fabsf(r->val[i]) < tol)) { r->col[k] = r->col[i]; r->val[k++] = r->val[i]; }
+    r->n = k;
+}
+
+static int rows_to_csr(orc_row *rows, int n, int **rowptr, int **col, float **val)
+{
+    size_t nnz = 0;
+    for (int r = 0; r < n; r++) nnz += (size_t)rows[r].n;
+    *rowptr = (int *)malloc(sizeof(int) * ((size_t)n + 1));
+    *col = (int *)malloc(sizeof(int) * (nnz ? nnz : 1));
+    *val = (float *)malloc(sizeof(float) * (nnz ? nnz : 1));
+    size_t p = 0;
+    (*rowptr)[0] = 0;
+    for (int r = 0; r < n; r++) {
+        memcpy(*col + p, rows[r].col, sizeof(int) * (size_t)rows[r].n);
+        memcpy(*val + p, rows[r].val, sizeof(float) * (size_t)rows[r].n);
+        p += (size_t)rows[r].n;
+        (*rowptr)[r + 1] = (int)p;
+        free(rows[r].col); free(rows[r].val);
+    }
+    free(rows);
+    return (int)nnz;
+}
+
+int orc_matrix_build(int n, const int *up_ptr, const int *up_idx,
+                     const float *C1, const float *C2, const float *C3, const float *C4oq, float tol,
+                     int **rpQ, int **colQ, float **valQ, int **rpq, int **colq, float **valq, int *nnz)
+{
+    orc_row *Q = (orc_row *)calloc((size_t)(n ? n : 1), sizeof(orc_row));
+    orc_row *q = (orc_row *)calloc((size_t)(n ? n : 1), sizeof(orc_row));
+    orc_row tmp = {0, 0, NULL, NULL};
+    for (int r = 0; r < n; r++) {
+        for (int k = up_ptr[r]; k < up_ptr[r + 1]; k++) {
+            const int u = up_idx[k];
+            if (u < 0 || u >= r) return -1;   /* not an earlier reach */
+            row_merge_scaled(&Q[r], &Q[u], C1[r], &tmp);
+            row_merge_scaled(&q[r], &q[u], C1[r], &tmp);
+        }
+        row_prune(&Q[r], tol);
+        row_prune(&q[r], tol);
+        for (int k = up_ptr[r]; k < up_ptr[r + 1]; k++) row_add_col(&Q[r], up_idx[k], C2[r]);
+        row_add_col(&Q[r], r, C3[r]);
+        row_add_col(&q[r], r, C4oq[r]);
+        row_prune(&Q[r], tol);
+        row_prune(&q[r], tol);
+    }
+    free(tmp.col); free(tmp.val);
+    nnz[0] = rows_to_csr(Q, n, rpQ, colQ, valQ);
+    nnz[1] = rows_to_csr(q, n, rpq, colq, valq);
+    return 0;
+}
+
+void orc_free(void *p) { free(p); }
+
+/* Steps of the matrix form, y = P_Q x + P_q ql_t, in float with a fixed
+ * summation order over each row's union of its two CSR rows (columns
+ * ascending; a matrix without an entry at a column contributes 0 * value):
+ * with G lanes per row, partial l (0 <= l < G) takes union entries l, l + G,
+ * ... and adds, per entry, s = s + vQ * x[c] then s = s + vq * ql[c] (no
+ * fused multiply-add); the partials are combined as p[l] = p[l] + p[l ^ o]
+ * for o = G/2, ..., 1 and the row's value is p[0].  G = 1 is the sequential
+ * row sum; G = 0 takes each row's G from its union length U: the smallest
+ * power of two with 4 G >= U, at most 64.  Q [n] holds Q0 on entry and the
+ * flow after T steps on return. */
+void orc_matrix_route(int n, const int *rpQ, const int *colQ, const float *valQ,
+                      const int *rpq, const int *colq, const float *valq,
+                      const float *qlat, float *Q, int T, int Gfix)
+{
+    float *y = (float *)malloc(sizeof(float) * (size_t)(n ? n : 1));
+    int ucap = 64;
+    int *uc = (int *)malloc(sizeof(int) * (size_t)ucap);
+    float *uq = (float *)malloc(sizeof(float) * (size_t)ucap), *uv = (float *)malloc(sizeof(float) * (size_t)ucap);
+    float p[64], p2[64];
+    for (int t = 0; t < T; t++) {
+        const float *ql = qlat + (size_t)t * n;
+        for (int r = 0; r < n; r++) {
+            /* union of the row's entries, ascending columns */
+            int a = rpQ[r], b = rpq[r], U = 0;
+            const int need = (rpQ[r + 1] - a) + (rpq[r + 1] - b);
+            if (need > ucap) {
+                ucap = need;
+                uc = (int *)realloc(uc, sizeof(int) * (size_t)ucap);
+                uq = (float *)realloc(uq, sizeof(float) * (size_t)ucap);
+                uv = (float *)realloc(uv, sizeof(float) * (size_t)ucap);
+            }
+            while (a < rpQ[r + 1] || b < rpq[r + 1]) {
+                const int ca = a < rpQ[r + 1] ? colQ[a] : 0x7fffffff, cb = b < rpq[r + 1] ? colq[b] : 0x7fffffff;
+                const int c = ca < cb ? ca : cb;
+                uc[U] = c;
+                uq[U] = (ca == c) ? valQ[a++] : 0.0f;
+                uv[U] = (cb == c) ? valq[b++] : 0.0f;
+                U++;
+            }
+            int G = Gfix;
+            if (G == 0)
+                for (G = 1; G < 64 && G * 4 < U; G *= 2) {}
+            for (int l = 0; l < G; l++) {
+                float s = 0.0f;
+                for (int k = l; k < U; k += G) {
+                    s = s + uq[k] * Q[uc[k]];
+                    s = s + uv[k] * ql[uc[k]];
+                }
+                p[l] = s;
+            }
+            for (int o = G / 2; o > 0; o >>= 1) {
+                for (int l = 0; l < G; l++) p2[l] = p[l] + p[l ^ o];
+                memcpy(p, p2, sizeof(float) * (size_t)G);
+            }
+            y[r] = p[0];
+        }
+        memcpy(Q, y, sizeof(float) * (size_t)n);
+    }
+    free(uc); free(uq); free(uv);
+    free(y);
 }
 
 /* wavefront/linear_mc.cu:238-289: level order, reaches in level_reach

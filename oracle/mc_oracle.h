@@ -34,6 +34,13 @@
  *                           reproduce its coefficients bit for bit)
  *   orc_linear_mc_fp64      wavefront/linear_mc.cu:238-289 (cpu_linear_mc_fp64)
  *
+ * Matrix form of the linearized MC (FP32):
+ *   orc_matrix_coeffs       wavefront/linear_mc_matrix.cu:58-84 (compute_mc_coeffs_cpu)
+ *   orc_matrix_build        linear_mc_matrix.cu:118-229 (build_propagation_matrices)
+ *   orc_matrix_route        the step loop of linear_mc_matrix.cu:340-380 /
+ *                           linear_mc_matrix_graph.cu:179-205, 355-392, with a
+ *                           stated summation order
+ *
  * Diffusive wave (one implicit step per reach, FP32):
  *   orc_diffusive_tridiag   water-prediction/owp_extended_kernels.cu:35-94
  *                           (cpu_diffusive_tridiag)
@@ -62,6 +69,13 @@
  *     cpu_diffusive_tridiag compiled into oracle/_ref, on inputs from the
  *     reference's own gen_diff_data; and its output solves the tridiagonal
  *     system it eliminates (residual in double).
+ *   - orc_matrix_coeffs / orc_matrix_build: bit-for-bit against the
+ *     reference's own compute_mc_coeffs_cpu and build_propagation_matrices
+ *     compiled into oracle/_ref (coefficients: powf of the host libm there,
+ *     the double pow rounded to float here, equal but for rare double
+ *     rounding).  orc_matrix_route: against the FP64 level-scheduled linear MC
+ *     of the same coefficients (with tol = 0 the matrix form is the same
+ *     recurrence), and its summation orders against one another.
  *   - orc_mc_solve_converged / orc_normal_depth: residual |F(h*)| at the root,
  *     brute-force dense scan on small cases, the sign change bracketing the root,
  *     and the steady-state closed form.  No reference counterpart exists, so
@@ -141,6 +155,23 @@ void orc_linear_mc_fp64(const float *C1, const float *C2, const float *C3, const
                         int n_reaches, int n_levels, int n_segments,
                         const float *qlat_ts, const float *Q0, double *Q_final,
                         int n_timesteps);
+
+/* Matrix form of the linearized MC (wavefront/linear_mc_matrix.cu): X = 0.25
+ * coefficients; P_Q and P_q built row by row with the reference's merge,
+ * prune and insert steps (returns -1 when an upstream index is not below its
+ * reach; arrays malloc'd, free with orc_free; nnz[0] = P_Q's, nnz[1] = P_q's);
+ * T float steps from Q (in place) over each row's union of entries with G
+ * lanes' fixed summation order (G = 0: each row's G from its union length, as
+ * the library's default). */
+void orc_matrix_coeffs(const orc_chparams_f32 *params, int n, float dt,
+                       float *C1, float *C2, float *C3, float *C4oq);
+int orc_matrix_build(int n, const int *up_ptr, const int *up_idx,
+                     const float *C1, const float *C2, const float *C3, const float *C4oq, float tol,
+                     int **rpQ, int **colQ, float **valQ, int **rpq, int **colq, float **valq, int *nnz);
+void orc_free(void *p);
+void orc_matrix_route(int n, const int *rpQ, const int *colQ, const float *valQ,
+                      const int *rpq, const int *colq, const float *valq,
+                      const float *qlat, float *Q, int T, int G);
 
 /* Diffusive-wave step: reach j has rp[j] = {ncomp, theta, dtini} and node i
  * at j * ORC_DIFF_MAX_NODES + i of dx, diffusivity, celerity (unused by the

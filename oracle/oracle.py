@@ -74,6 +74,14 @@ def lib(variant: str = "libm") -> C.CDLL:
         L.orc_diffusive_tridiag.argtypes = [C.c_void_p] + [pF] * 5 + [I]
         L.orc_linear_mc_fp64.restype = None
         L.orc_linear_mc_fp64.argtypes = [pF] * 4 + topo + [pF, pF, pD, I]
+        L.orc_matrix_coeffs.restype = None
+        L.orc_matrix_coeffs.argtypes = [pP32, I, C.c_float] + [pF] * 4
+        L.orc_matrix_build.restype = I
+        L.orc_matrix_build.argtypes = _MATRIX_BUILD_ARGS
+        L.orc_free.restype = None
+        L.orc_free.argtypes = [C.c_void_p]
+        L.orc_matrix_route.restype = None
+        L.orc_matrix_route.argtypes = [I, pI, pI, pF, pI, pI, pF, pF, pF, I, I]
         _orc[variant] = L
     return _orc[variant]
 
@@ -208,6 +216,11 @@ def compute_stats(pred, truth):
     return dict(zip(keys, out.tolist()))
 
 
+_ppI, _ppF = C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.POINTER(C.c_float))
+_MATRIX_BUILD_ARGS = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)] + [C.POINTER(C.c_float)] * 4 + \
+    [C.c_float, _ppI, _ppI, _ppF, _ppI, _ppI, _ppF, C.POINTER(C.c_int)]
+
+
 def load_ref():
     """The reference's own host routines (oracle/_ref), or None if not built."""
     path = os.path.join(HERE, "_ref", "libtroute_ref.so")
@@ -225,6 +238,13 @@ def load_ref():
         ref.ref_cpu_diffusive_tridiag.argtypes = [C.c_void_p] + [pF] * 5 + [I]
         ref.ref_gen_diff_data.restype = None
         ref.ref_gen_diff_data.argtypes = [C.c_void_p] + [pF] * 4 + [I, C.c_uint]
+    if hasattr(ref, "ref_matrix_build"):
+        ref.ref_matrix_coeffs.restype = None
+        ref.ref_matrix_coeffs.argtypes = [C.c_void_p, I, C.c_float] + [pF] * 4
+        ref.ref_matrix_build.restype = None
+        ref.ref_matrix_build.argtypes = _MATRIX_BUILD_ARGS
+        ref.ref_free.restype = None
+        ref.ref_free.argtypes = [C.c_void_p]
     if hasattr(ref, "ref_cpu_linear_mc_fp64"):
         ref.ref_cpu_linear_mc_fp64.restype = None
         ref.ref_cpu_linear_mc_fp64.argtypes = [pF] * 4 + [pI] * 6 + [I, I, I] + [pF, pF, pD, I]
@@ -278,3 +298,63 @@ def gen_diff_data_reference(nreach, seed=42):
     arrs = [np.zeros(nreach * DIFF_MAX_NODES, np.float32) for _ in range(4)]
     load_ref().ref_gen_diff_data(rp.ctypes.data, *[_ptr(a, C.c_float) for a in arrs], nreach, seed)
     return (rp, *arrs)
+
+
+# ------------------------------------------------------------ matrix form
+def matrix_coeffs(params32, dt=300.0, reference=False):
+    """compute_mc_coeffs_cpu restated (linear_mc_matrix.cu:58-84), X = 0.25;
+    reference=True runs the reference's own routine from oracle/_ref."""
+    p = np.ascontiguousarray(params32, np.float32).reshape(-1, 8)
+    n = len(p)
+    out = [np.zeros(n, np.float32) for _ in range(4)]
+    if reference:
+        load_ref().ref_matrix_coeffs(p.ctypes.data, n, float(dt), *[_ptr(a, C.c_float) for a in out])
+    else:
+        lib().orc_matrix_coeffs(p.ctypes.data_as(C.POINTER(P32)), n, float(dt), *[_ptr(a, C.c_float) for a in out])
+    return tuple(out)
+
+
+def matrix_build(up_ptr, up_idx, coeffs, tol, reference=False):
+    """build_propagation_matrices restated (linear_mc_matrix.cu:118-229); reference=True
+    runs the reference's own routine.  Returns (P_Q, P_q), each (rowptr, col, val)."""
+    ptr = np.ascontiguousarray(up_ptr, np.int32)
+    idx = np.ascontiguousarray(up_idx, np.int32)
+    n = len(ptr) - 1
+    cs = [np.ascontiguousarray(c, np.float32) for c in coeffs]
+    outs = [C.POINTER(C.c_int)(), C.POINTER(C.c_int)(), C.POINTER(C.c_float)(),
+            C.POINTER(C.c_int)(), C.POINTER(C.c_int)(), C.POINTER(C.c_float)()]
+    nnz = np.zeros(2, np.int32)
+    args = [n, _ptr(ptr, C.c_int), _ptr(idx if len(idx) else np.zeros(1, np.int32), C.c_int),
+            *[_ptr(c, C.c_float) for c in cs], float(tol), *[C.byref(o) for o in outs], _ptr(nnz, C.c_int)]
+    if reference:
+        R = load_ref()
+        R.ref_matrix_build(*args)
+        free = R.ref_free
+    else:
+        if lib().orc_matrix_build(*args) != 0:
+            raise ValueError("an upstream reach index is not below its reach")
+        free = lib().orc_free
+
+    def take(i, count, dt):
+        a = np.ctypeslib.as_array(outs[i], (count,)).copy() if count else np.zeros(0, dt)
+        free(outs[i])
+        return a
+    PQ = (take(0, n + 1, np.int32), take(1, int(nnz[0]), np.int32), take(2, int(nnz[0]), np.float32))
+    Pq = (take(3, n + 1, np.int32), take(4, int(nnz[1]), np.int32), take(5, int(nnz[1]), np.float32))
+    return PQ, Pq
+
+
+def matrix_route(PQ, Pq, qlat32, Q0, n_timesteps, lanes=1):
+    """The matrix-form steps in float over each row's union of its P_Q and P_q
+    entries, with the summation order of `lanes` lanes per row (1 = sequential;
+    0 = by union length U, the smallest power of two G with 4 G >= U, at most
+    64).  Returns Q after n_timesteps steps, float [n]."""
+    n = len(PQ[0]) - 1
+    arr = [np.ascontiguousarray(a) for a in (*PQ, *Pq)]
+    q = np.ascontiguousarray(qlat32, np.float32)
+    Q = np.array(Q0, np.float32, copy=True)
+    z = lambda a, t: _ptr(a if len(a) else np.zeros(1, a.dtype), t)
+    lib().orc_matrix_route(n, z(arr[0], C.c_int), z(arr[1], C.c_int), z(arr[2], C.c_float),
+                           z(arr[3], C.c_int), z(arr[4], C.c_int), z(arr[5], C.c_float),
+                           z(q.reshape(-1), C.c_float), _ptr(Q, C.c_float), int(n_timesteps), int(lanes))
+    return Q
