@@ -383,6 +383,16 @@ def bench_diffusive(trmc, spec, args):
     print(json.dumps(res), flush=True)
 
 
+def _traffic(workload):
+    """HBM bytes per launch of the workload's kernel from the committed counter
+    summary (two rocprofv3 --pmc runs scaled by a plain copy), or None."""
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
 def bench_matrix(trmc, spec, args):
     """The matrix form of the linearized MC (linear_mc_matrix.cu,
     linear_mc_matrix_graph.cu) at the reference's real-network size: 309,000
@@ -410,8 +420,9 @@ def bench_matrix(trmc, spec, args):
     PQ, Pq = trmc.matrix_build(t.reach_up_ptr, t.reach_up_idx, c, tol)
     t2 = time.perf_counter()
     nnz = (len(PQ[1]), len(Pq[1]))
+    copy_gbps = trmc.stream_copy(1 << 30, reps=3)
     sweep = {}
-    for lanes in (8, 16, 32):
+    for lanes in (() if args.only else (8, 16, 32)):
         _, ms = trmc.matrix_route(PQ, Pq, qlat, Q0, T, lanes_per_row=lanes, reps=args.steps)
         sweep[lanes] = ms
         log(f"[bench] matrix: lanes {lanes}: {ms:.3f} ms per {T}-step pass")
@@ -434,7 +445,8 @@ def bench_matrix(trmc, spec, args):
                    "lanes_per_row": "by union row length U (smallest power of two G with 4 G >= U, <= 64)",
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic("matrix-309k"),
+                     "copy_reference_gbps": copy_gbps},
         "ms_per_timestep": step_ms,
         "lanes_sweep_ms_per_pass": sweep,
         "setup_s": {"coefficients_gpu": t1 - t0, "matrices_host": t2 - t1},
