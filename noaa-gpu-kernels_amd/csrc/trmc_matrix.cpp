@@ -4,11 +4,11 @@
 //   row_Q(r) = prune(C1[r] * sum_u row_Q(u)) + C2[r] at each upstream column
 //              + C3[r] at column r, pruned again
 //   row_q(r) = prune(C1[r] * sum_u row_q(u)) + C4/ql[r] at column r, pruned again
-// with prune dropping |value| < tol.  The reference merges the upstream rows
-// one after another into a sorted list; here one dense accumulator per matrix
-// takes them in the same upstream order, so each column sees the same float
-// operations in the same order (first product alone, later ones added), and a
-// touched list restores the sorted rows.  One-time host work.
+// with prune dropping |value| < tol.  The upstream rows are merged one after
+// another, in CSR order, into a sorted working row (a column new to it takes
+// the product, a present one adds it), as the reference does, but through two
+// scratch rows reused for every merge and written once into the output, so
+// no allocation happens per merge.  One-time host work.
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -26,42 +26,67 @@ struct Rows {
     std::vector<float> val;
 };
 
-// one matrix's accumulator: acc[c] valid while mark[c] == current row
-struct Acc {
-    std::vector<float> acc;
-    std::vector<int> mark, touched;
-    explicit Acc(int n) : acc((size_t)n), mark((size_t)n, -1) {}
-    // value v (a product or a coefficient) onto column c of row r
-    void add(int r, int c, float v)
+// a sorted working row
+struct Work {
+    std::vector<int> col, tcol;
+    std::vector<float> val, tval;
+    void clear() { col.clear(); val.clear(); }
+    // this = merge(this, scale * row k of m)
+    void merge_scaled(const Rows &m, int k, float scale)
     {
-        if (mark[c] == r) {
-            acc[c] = acc[c] + v;
+        const int b = m.rowptr[k], e = m.rowptr[k + 1];
+        if (col.empty()) {
+            for (int j = b; j < e; ++j) {
+                col.push_back(m.col[j]);
+                val.push_back(m.val[j] * scale);
+            }
+            return;
+        }
+        tcol.clear();
+        tval.clear();
+        size_t i = 0;
+        int j = b;
+        while (i < col.size() && j < e) {
+            if (col[i] < m.col[j]) {
+                tcol.push_back(col[i]); tval.push_back(val[i]); ++i;
+            } else if (col[i] > m.col[j]) {
+                tcol.push_back(m.col[j]); tval.push_back(m.val[j] * scale); ++j;
+            } else {
+                tcol.push_back(col[i]); tval.push_back(val[i] + m.val[j] * scale); ++i; ++j;
+            }
+        }
+        for (; i < col.size(); ++i) { tcol.push_back(col[i]); tval.push_back(val[i]); }
+        for (; j < e; ++j) { tcol.push_back(m.col[j]); tval.push_back(m.val[j] * scale); }
+        col.swap(tcol);
+        val.swap(tval);
+    }
+    // value v onto column c (inserted in order when absent)
+    void add(int c, float v)
+    {
+        auto it = std::lower_bound(col.begin(), col.end(), c);
+        const size_t at = (size_t)(it - col.begin());
+        if (it != col.end() && *it == c) {
+            val[at] = val[at] + v;
         } else {
-            mark[c] = r;
-            acc[c] = v;
-            touched.push_back(c);
+            col.insert(it, c);
+            val.insert(val.begin() + (std::ptrdiff_t)at, v);
         }
     }
     // drop |value| < tol (NaN stays, as remove_if over fabs(v) < tol keeps it)
     void prune(float tol)
     {
         size_t k = 0;
-        for (int c : touched) {
-            if (std::fabs(acc[c]) < tol) mark[c] = -1;
-            else touched[k++] = c;
-        }
-        touched.resize(k);
+        for (size_t i = 0; i < col.size(); ++i)
+            if (!(std::fabs(val[i]) < tol)) { col[k] = col[i]; val[k] = val[i]; ++k; }
+        col.resize(k);
+        val.resize(k);
     }
-    bool emit(Rows &m)
+    bool emit(Rows &m) const
     {
-        std::sort(touched.begin(), touched.end());
-        if (m.col.size() + touched.size() > (size_t)INT_MAX) return false;
-        for (int c : touched) {
-            m.col.push_back(c);
-            m.val.push_back(acc[c]);
-        }
+        if (m.col.size() + col.size() > (size_t)INT_MAX) return false;
+        m.col.insert(m.col.end(), col.begin(), col.end());
+        m.val.insert(m.val.end(), val.begin(), val.end());
         m.rowptr.push_back((int)m.col.size());
-        touched.clear();
         return true;
     }
 };
@@ -117,23 +142,24 @@ int trmc_matrix_build(int n, const int *up_ptr, const int *up_idx, const float *
     mq.rowptr.reserve((size_t)n + 1);
     mQ.rowptr.push_back(0);
     mq.rowptr.push_back(0);
-    Acc aQ(n), aq(n);
+    Work wQ, wq;
     for (int r = 0; r < n; ++r) {
         const float c1 = C1[r];
         const int ub = up_ptr[r], ue = up_ptr[r + 1];
+        wQ.clear();
+        wq.clear();
         for (int k = ub; k < ue; ++k) {
-            const int u = up_idx[k];
-            for (int e = mQ.rowptr[u]; e < mQ.rowptr[u + 1]; ++e) aQ.add(r, mQ.col[e], mQ.val[e] * c1);
-            for (int e = mq.rowptr[u]; e < mq.rowptr[u + 1]; ++e) aq.add(r, mq.col[e], mq.val[e] * c1);
+            wQ.merge_scaled(mQ, up_idx[k], c1);
+            wq.merge_scaled(mq, up_idx[k], c1);
         }
-        aQ.prune(tol);
-        aq.prune(tol);
-        for (int k = ub; k < ue; ++k) aQ.add(r, up_idx[k], C2[r]);
-        aQ.add(r, r, C3[r]);
-        aq.add(r, r, C4oq[r]);
-        aQ.prune(tol);
-        aq.prune(tol);
-        if (!aQ.emit(mQ) || !aq.emit(mq))
+        wQ.prune(tol);
+        wq.prune(tol);
+        for (int k = ub; k < ue; ++k) wQ.add(up_idx[k], C2[r]);
+        wQ.add(r, C3[r]);
+        wq.add(r, C4oq[r]);
+        wQ.prune(tol);
+        wq.prune(tol);
+        if (!wQ.emit(mQ) || !wq.emit(mq))
             return trmc::fail(TRMC_ERR_NOMEM, "propagation matrix exceeds 2^31 entries (raise tol)");
     }
     int rc = to_csr(mQ, n, P_Q);
