@@ -58,6 +58,16 @@ def test_builders_match_reference_routine(tol):
     assert _same(trmc.matrix_build(t.reach_up_ptr, t.reach_up_idx, c, tol), ref)
 
 
+def test_threaded_builder_matches_oracle():
+    """100k reaches: the library builds the levels on several host threads;
+    every row is still the same float sequence, so the matrices equal the
+    oracle's (one thread, sorted merges) bit for bit."""
+    t, p, _, _ = _net(100000, seed=12)
+    c = orc.matrix_coeffs(p)
+    assert _same(trmc.matrix_build(t.reach_up_ptr, t.reach_up_idx, c, 1e-6),
+                 orc.matrix_build(t.reach_up_ptr, t.reach_up_idx, c, 1e-6))
+
+
 def test_builders_match_on_hydrofabric_network():
     """The network the hydrofabric builder writes (fan-in up to 12, forest of
     outlets), and the library's builder against the oracle's where the
