@@ -666,6 +666,19 @@ def main():
             "libm_pow_vs_cr_pow_max_rel": float(np.max(np.abs(qr8 - qcr) / np.maximum(np.abs(qcr), 1e-300))),
             "seconds": time.perf_counter() - t0}
         out["speedup_vs_cpu"] = value / rec["value"]
+        # a second, labelled baseline: the oracle restatement (same bits as the
+        # one-core run) with each level's reaches on all host threads
+        nthr = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        qmt, rmt, _ = orc.wavefront_f32in(topo_o, np.ascontiguousarray(w.params, np.float32),
+                                          np.ascontiguousarray(w.qlat[0][:Tc], np.float32), w.qdp0, w.dp0,
+                                          w.reach_last0, Tc, dt, variant="libm", threads=nthr)
+        smt = time.perf_counter() - t0
+        out["cpu_baseline_all_cores"] = {
+            "value": w.topo.n_segments * Tc / smt, "unit": "segment-steps/s", "cores": nthr, "kind": "port",
+            "sample": f"member 0, first {Tc} timesteps, {nthr} threads, level-parallel ({smt:.1f} s)",
+            "equal_to_one_core_frac": float(np.mean(qmt.view(np.uint64) == qo.view(np.uint64)))}
+        log(f"[bench] cpu all cores {out['cpu_baseline_all_cores']['value'] / 1e6:.2f} M seg-steps/s")
         log(f"[bench] cpu {rec['value'] / 1e6:.2f} M seg-steps/s; parity {out['parity']}")
         if "learned" in nets and args.oracle_sample > 0:
             t0 = time.perf_counter()

@@ -237,7 +237,7 @@ static void route_dag(const orc_chparams_f64 *params,
                       const float *qlat_f, const double *qlat_d,
                       double *q_prev, double *d_prev, double *rl_prev,
                       double *v_last, double *reach_hist,
-                      int n_timesteps, double dt, int mode, long long *iter_sum)
+                      int n_timesteps, double dt, int mode, long long *iter_sum, int n_threads)
 {
     double *q_cur = (double *)calloc((size_t)n_segments + 1, sizeof(double));
     double *d_cur = (double *)calloc((size_t)n_segments + 1, sizeof(double));
@@ -247,6 +247,10 @@ static void route_dag(const orc_chparams_f64 *params,
     for (int t = 0; t < n_timesteps; ++t) {
         size_t row = (size_t)t * (size_t)n_segments;
         for (int L = 0; L < n_levels; ++L) {
+            /* the reaches of one level are independent: with n_threads > 1
+             * they are shared out over threads, each reach computed exactly
+             * as on one thread */
+#pragma omp parallel for schedule(dynamic, 32) num_threads(n_threads) if (n_threads > 1) reduction(+ : its)
             for (int ii = level_ptr[L]; ii < level_ptr[L + 1]; ++ii) {
                 int r = level_reach[ii];
                 /* fixed-order upstream sums in CSR order, :523-529 */
@@ -294,15 +298,15 @@ static void route_dag(const orc_chparams_f64 *params,
     free(q_cur); free(d_cur); free(rl_cur);
 }
 
-void orc_wavefront_f32in(const orc_chparams_f32 *params,
-                         const int *reach_seg_start, const int *reach_seg_len,
-                         const int *reach_up_ptr, const int *reach_up_idx,
-                         const int *level_ptr, const int *level_reach,
-                         int n_reaches, int n_levels, int n_segments,
-                         const float *qlat_ts, const float *qdp0_seg,
-                         const float *dp0_seg, const float *reach_last0,
-                         double *qdc_seg_out_d, double *reach_last_out_d,
-                         int n_timesteps, double dt, long long *iter_sum)
+static void wavefront_f32in(const orc_chparams_f32 *params,
+                            const int *reach_seg_start, const int *reach_seg_len,
+                            const int *reach_up_ptr, const int *reach_up_idx,
+                            const int *level_ptr, const int *level_reach,
+                            int n_reaches, int n_levels, int n_segments,
+                            const float *qlat_ts, const float *qdp0_seg,
+                            const float *dp0_seg, const float *reach_last0,
+                            double *qdc_seg_out_d, double *reach_last_out_d,
+                            int n_timesteps, double dt, long long *iter_sum, int n_threads)
 {
     /* float params are widened field by field, :536-538 */
     orc_chparams_f64 *pd = (orc_chparams_f64 *)malloc(((size_t)n_segments + 1) * sizeof(*pd));
@@ -320,10 +324,42 @@ void orc_wavefront_f32in(const orc_chparams_f32 *params,
     route_dag(pd, reach_seg_start, reach_seg_len, reach_up_ptr, reach_up_idx,
               level_ptr, level_reach, n_reaches, n_levels, n_segments,
               qlat_ts, NULL, q, d, rl, NULL, NULL, n_timesteps, dt,
-              ORC_MODE_REFERENCE, iter_sum);
+              ORC_MODE_REFERENCE, iter_sum, n_threads);
     memcpy(qdc_seg_out_d, q, (size_t)n_segments * sizeof(double));
     memcpy(reach_last_out_d, rl, (size_t)n_reaches * sizeof(double));
     free(pd); free(q); free(d); free(rl);
+}
+
+void orc_wavefront_f32in(const orc_chparams_f32 *params,
+                         const int *reach_seg_start, const int *reach_seg_len,
+                         const int *reach_up_ptr, const int *reach_up_idx,
+                         const int *level_ptr, const int *level_reach,
+                         int n_reaches, int n_levels, int n_segments,
+                         const float *qlat_ts, const float *qdp0_seg,
+                         const float *dp0_seg, const float *reach_last0,
+                         double *qdc_seg_out_d, double *reach_last_out_d,
+                         int n_timesteps, double dt, long long *iter_sum)
+{
+    wavefront_f32in(params, reach_seg_start, reach_seg_len, reach_up_ptr, reach_up_idx, level_ptr,
+                    level_reach, n_reaches, n_levels, n_segments, qlat_ts, qdp0_seg, dp0_seg, reach_last0,
+                    qdc_seg_out_d, reach_last_out_d, n_timesteps, dt, iter_sum, 1);
+}
+
+/* The same on n_threads host threads, level by level (a labelled all-core
+ * CPU baseline; results equal the one-thread routine's bit for bit). */
+void orc_wavefront_f32in_mt(const orc_chparams_f32 *params,
+                            const int *reach_seg_start, const int *reach_seg_len,
+                            const int *reach_up_ptr, const int *reach_up_idx,
+                            const int *level_ptr, const int *level_reach,
+                            int n_reaches, int n_levels, int n_segments,
+                            const float *qlat_ts, const float *qdp0_seg,
+                            const float *dp0_seg, const float *reach_last0,
+                            double *qdc_seg_out_d, double *reach_last_out_d,
+                            int n_timesteps, double dt, long long *iter_sum, int n_threads)
+{
+    wavefront_f32in(params, reach_seg_start, reach_seg_len, reach_up_ptr, reach_up_idx, level_ptr,
+                    level_reach, n_reaches, n_levels, n_segments, qlat_ts, qdp0_seg, dp0_seg, reach_last0,
+                    qdc_seg_out_d, reach_last_out_d, n_timesteps, dt, iter_sum, n_threads < 1 ? 1 : n_threads);
 }
 
 void orc_wavefront_f64(const orc_chparams_f64 *params,
@@ -346,7 +382,7 @@ void orc_wavefront_f64(const orc_chparams_f64 *params,
     if (vel_out) memset(vel_out, 0, (size_t)n_segments * sizeof(double));
     route_dag(params, reach_seg_start, reach_seg_len, reach_up_ptr, reach_up_idx,
               level_ptr, level_reach, n_reaches, n_levels, n_segments,
-              NULL, qlat_ts, q, d, rl, vel_out, reach_hist, n_timesteps, dt, mode, iter_sum);
+              NULL, qlat_ts, q, d, rl, vel_out, reach_hist, n_timesteps, dt, mode, iter_sum, 1);
     memcpy(qdc_seg_out, q, (size_t)n_segments * sizeof(double));
     memcpy(reach_last_out, rl, (size_t)n_reaches * sizeof(double));
     if (depth_out) memcpy(depth_out, d, (size_t)n_segments * sizeof(double));

@@ -142,6 +142,18 @@ double orc_mc_solve_converged(double dt, double qup, double quc, double qdp, dou
                               const orc_chparams_f64 *p, double depthp,
                               double *depthc, double *velc);
 
+/* orc_wavefront_f32in on n_threads host threads (OpenMP over the reaches of
+ * one level); bit for bit the one-thread result. */
+void orc_wavefront_f32in_mt(const orc_chparams_f32 *params,
+                            const int *reach_seg_start, const int *reach_seg_len,
+                            const int *reach_up_ptr, const int *reach_up_idx,
+                            const int *level_ptr, const int *level_reach,
+                            int n_reaches, int n_levels, int n_segments,
+                            const float *qlat_ts, const float *qdp0_seg,
+                            const float *dp0_seg, const float *reach_last0,
+                            double *qdc_seg_out_d, double *reach_last_out_d,
+                            int n_timesteps, double dt, long long *iter_sum, int n_threads);
+
 /* Linearized MC coefficients at half bankfull depth, float (n segments). */
 void orc_linear_coeffs(const orc_chparams_f32 *params, int n, float dt,
                        float *C1, float *C2, float *C3, float *C4oq);

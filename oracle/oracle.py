@@ -52,6 +52,8 @@ def lib(variant: str = "libm") -> C.CDLL:
         L.orc_mc_solve.argtypes = [D] * 5 + [pP64, D, pD, pD, pI]
         L.orc_wavefront_f32in.restype = None
         L.orc_wavefront_f32in.argtypes = [pP32] + topo + [pF] * 4 + [pD, pD, I, D, C.POINTER(LL)]
+        L.orc_wavefront_f32in_mt.restype = None
+        L.orc_wavefront_f32in_mt.argtypes = [pP32] + topo + [pF] * 4 + [pD, pD, I, D, C.POINTER(LL), I]
         L.orc_wavefront_f64.restype = None
         L.orc_wavefront_f64.argtypes = [pP64] + topo + [pD] * 4 + [pD] * 5 + [I, D, I, C.POINTER(LL)]
         L.orc_reach_parallel.restype = None
@@ -129,8 +131,10 @@ def mc_solve_many(params, inputs, dt=300.0, mode=MODE_REFERENCE, variant="libm")
     return out, its
 
 
-def wavefront_f32in(topo, params32, qlat32, qdp0, dp0, reach_last0, n_timesteps, dt=300.0, variant="libm"):
-    """cpu_wavefront_fp64 restated: returns (qdc_out [N], reach_last_out [R], iterations)."""
+def wavefront_f32in(topo, params32, qlat32, qdp0, dp0, reach_last0, n_timesteps, dt=300.0, variant="libm",
+                    threads=1):
+    """cpu_wavefront_fp64 restated: returns (qdc_out [N], reach_last_out [R], iterations).
+    threads > 1 shares each level's reaches over host threads (same bits)."""
     t = as_topo(topo)
     keep, targs = _topo_args(t)
     p = np.ascontiguousarray(params32, np.float32)
@@ -139,9 +143,12 @@ def wavefront_f32in(topo, params32, qlat32, qdp0, dp0, reach_last0, n_timesteps,
     qo = np.zeros(int(t["n_segments"]))
     ro = np.zeros(int(t["n_reaches"]))
     its = LL(0)
-    lib(variant).orc_wavefront_f32in(_ptr(p, P32), *targs, _ptr(q, C.c_float), _ptr(q0, C.c_float),
-                              _ptr(d0, C.c_float), _ptr(r0, C.c_float), _ptr(qo, D), _ptr(ro, D),
-                              int(n_timesteps), float(dt), C.byref(its))
+    args = [_ptr(p, P32), *targs, _ptr(q, C.c_float), _ptr(q0, C.c_float), _ptr(d0, C.c_float),
+            _ptr(r0, C.c_float), _ptr(qo, D), _ptr(ro, D), int(n_timesteps), float(dt), C.byref(its)]
+    if threads > 1:
+        lib(variant).orc_wavefront_f32in_mt(*args, int(threads))
+    else:
+        lib(variant).orc_wavefront_f32in(*args)
     return qo, ro, its.value
 
 

@@ -308,3 +308,17 @@ def test_cr_variant_differs_from_reference_only_through_pow():
     b, _ = orc.mc_solve_many(z["params"], z["inputs"], float(z["dt"]), variant="cr")
     rel = np.abs(a - b) / np.maximum(np.abs(a), 1e-12)
     assert np.median(rel) < 1e-15 and np.mean(rel < 1e-12) > 0.99
+
+
+def test_level_parallel_restatement_equals_one_thread():
+    """The all-cores CPU baseline shares each level's reaches over threads;
+    every segment is computed as on one thread, so the outputs and the
+    iteration count are the same bits."""
+    from conftest import load_trmc
+    w = load_trmc().make_workload(2, 50000, 3, n_reaches=15000, seed=8)
+    topo = orc.as_topo(w.topo)
+    p, q = np.asarray(w.params, np.float32), np.asarray(w.qlat[0], np.float32)
+    a = orc.wavefront_f32in(topo, p, q, w.qdp0, w.dp0, w.reach_last0, 3)
+    b = orc.wavefront_f32in(topo, p, q, w.qdp0, w.dp0, w.reach_last0, 3, threads=4)
+    assert np.array_equal(a[0].view(np.uint64), b[0].view(np.uint64))
+    assert np.array_equal(a[1].view(np.uint64), b[1].view(np.uint64)) and a[2] == b[2]
