@@ -468,8 +468,11 @@ def bench_matrix(trmc, spec, args):
         res["speedup_vs_cpu"] = res["value"] / res["cpu_baseline"]["value"]
         if orc.load_ref() is not None and hasattr(orc.load_ref(), "ref_matrix_build"):
             t0 = time.perf_counter()
-            orc.matrix_build(t.reach_up_ptr, t.reach_up_idx, c, tol, reference=True)
+            rQ, rq = orc.matrix_build(t.reach_up_ptr, t.reach_up_idx, c, tol, reference=True)
             res["setup_s"]["reference_matrices_host"] = time.perf_counter() - t0
+            res["parity"]["matrices_equal_reference_builder"] = all(
+                np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
+                for m, r in ((PQ, rQ), (Pq, rq)) for a, b in zip(m, r))
         log(f"[bench] matrix cpu {res['cpu_baseline']['value'] / 1e6:.1f} M reach-steps/s; parity {res['parity']}")
     print(json.dumps(res), flush=True)
 
