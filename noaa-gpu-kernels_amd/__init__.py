@@ -674,7 +674,8 @@ def read_hydrofabric(gpkg_path: str):
         attrs = con.execute(f'SELECT link, {cols} FROM "flowpath-attributes"').fetchall()
     finally:
         con.close()
-    vals = np.array([[np.nan if v is None else float(v) for v in a[1:]] for a in attrs], np.float64).reshape(-1, 8)
+    # numpy reads a NULL (None) as NaN in a float array
+    vals = np.array([a[1:] for a in attrs], np.float64).reshape(-1, 8)
     return [r[0] for r in rows], [r[1] for r in rows], [a[0] for a in attrs], vals
 
 
