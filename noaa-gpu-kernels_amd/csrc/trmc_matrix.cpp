@@ -121,6 +121,7 @@ struct Work {
 class Barrier {
 public:
     explicit Barrier(int n) : n_(n) {}
+    void set_count(int n) { n_ = n; }   // before any thread waits
     void wait()
     {
         const unsigned gen = gen_.load(std::memory_order_acquire);
@@ -133,7 +134,7 @@ public:
     }
 
 private:
-    const int n_;
+    int n_;
     std::atomic<int> count_{0};
     std::atomic<unsigned> gen_{0};
 };
@@ -207,14 +208,21 @@ int trmc_matrix_build(int n, const int *up_ptr, const int *up_idx, const float *
     std::vector<RowRef> rQ((size_t)n), rq((size_t)n);
     std::vector<Arena> aQ((size_t)nt), aq((size_t)nt);
     std::atomic<int> next{0};
+    std::atomic<bool> failed{false};
     Barrier bar(nt);
+    // a thread whose allocation fails stops building rows but still passes
+    // every barrier, so the others finish; the call then reports the failure
+    std::atomic<bool> go{false};
     auto worker = [&](int tid) {
+        while (!go.load(std::memory_order_acquire)) std::this_thread::yield();
         Work wQ, wq;
         for (int L = 0; L < n_levels; ++L) {
             const int b = lptr[L], e = lptr[L + 1];
             for (;;) {
                 const int i0 = b + next.fetch_add(16, std::memory_order_relaxed);
                 if (i0 >= e) break;
+                if (failed.load(std::memory_order_relaxed)) continue;
+                try {
                 for (int i = i0; i < std::min(i0 + 16, e); ++i) {
                     const int r = lrows[i];
                     const float c1 = C1[r];
@@ -235,6 +243,9 @@ int trmc_matrix_build(int n, const int *up_ptr, const int *up_idx, const float *
                     rQ[r] = aQ[tid].put(wQ.col, wQ.val);
                     rq[r] = aq[tid].put(wq.col, wq.val);
                 }
+                } catch (...) {
+                    failed.store(true, std::memory_order_relaxed);
+                }
             }
             bar.wait();                       // every row of level L is written
             if (tid == 0) next.store(0, std::memory_order_relaxed);
@@ -242,11 +253,21 @@ int trmc_matrix_build(int n, const int *up_ptr, const int *up_idx, const float *
         }
     };
     {
+        // the barrier counts the threads that did start
         std::vector<std::thread> pool;
-        for (int t = 1; t < nt; ++t) pool.emplace_back(worker, t);
+        for (int t = 1; t < nt; ++t) {
+            try {
+                pool.emplace_back(worker, t);
+            } catch (...) {
+                break;
+            }
+        }
+        bar.set_count((int)pool.size() + 1);
+        go.store(true, std::memory_order_release);
         worker(0);
         for (auto &th : pool) th.join();
     }
+    if (failed.load()) return trmc::fail(TRMC_ERR_NOMEM, "out of host memory building the matrices");
     int rc = to_csr(rQ, n, P_Q);
     if (!rc) rc = to_csr(rq, n, P_q);
     if (rc) {
