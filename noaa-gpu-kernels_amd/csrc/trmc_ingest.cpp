@@ -131,13 +131,28 @@ int trmc_ingest_nex_csv(const char *csv_dir, const int64_t *topo_to_wbid, int n_
             }
         }
     };
+    // a worker that cannot allocate reports it; a range whose thread cannot
+    // start is read on this thread
+    std::atomic<bool> failed{false};
+    auto work_safe = [&](size_t b, size_t e) {
+        try {
+            work(b, e);
+        } catch (...) {
+            failed.store(true);
+        }
+    };
     std::vector<std::thread> th;
     for (int t = 0; t < n_threads; ++t) {
         const size_t b = (size_t)t * per, e = std::min(paths.size(), b + per);
         if (b >= e) break;
-        th.emplace_back(work, b, e);
+        try {
+            th.emplace_back(work_safe, b, e);
+        } catch (...) {
+            work_safe(b, e);
+        }
     }
     for (auto &t : th) t.join();
+    if (failed.load()) return trmc::fail(TRMC_ERR_NOMEM, "out of host memory reading the forcing files");
     if (counts) {
         counts[0] = matched.load();
         counts[1] = missing.load();
