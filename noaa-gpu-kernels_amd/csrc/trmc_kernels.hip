@@ -381,6 +381,12 @@ __global__ void __launch_bounds__(256) k_route_dataflow(RouteArgs a)
 // wave, each packing the two samples (the routed flow and the anchor's
 // Manning flow) of up to 32 such lanes, give every such lane its guess
 // (weights staged in LDS).
+// LEARNED: a lane whose routed and anchor Manning flows differ by less than
+// this in |ln(q_new / q_old)| (about 20 %) skips the network: its correction
+// e^{y(q_new) - y(q_old)} is then near 1, and the wave's forward would cost
+// more than the evaluations it saves
+constexpr float kNetMinLogRatio = 0.2f;
+
 template <int MODE> struct IlSolver { typedef SecantConv type; };
 template <> struct IlSolver<0> { typedef SecantRef type; };
 template <> struct IlSolver<3> { typedef SecantRef type; };   // LINEAR: no iteration (unused)
@@ -571,19 +577,32 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                 if (want)
                     sr.set_guess(mlp_guess_from(sr.guess_anchor(), sr.guess_flow_new(), sr.guess_flow_old(), 0.0f,
                                                 0.0f));
-            } else if (__any(want)) {                // wave-uniform: all 64 lanes run the forward
-                const long long m0 = (long long)wall_clock64();
-                float xn[kMlpIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, xo[kMlpIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            } else {
+                // the network's factor e^{y(q_new) - y(q_old)} is near 1 when
+                // the flow barely changes: such lanes take the power law alone
+                bool net = false;
                 if (want) {
-                    mlp_segment_inputs(sr.guess_flow_new(), c, xn);
-                    mlp_segment_inputs(sr.guess_flow_old(), c, xo);
+                    const float r = (float)(fmax(sr.guess_flow_new(), 1e-6) / fmax(sr.guess_flow_old(), 1e-6));
+                    net = !(fabsf(__logf(r)) < kNetMinLogRatio);
+                    if (!net)
+                        sr.set_guess(mlp_guess_from(sr.guess_anchor(), sr.guess_flow_new(), sr.guess_flow_old(),
+                                                    0.0f, 0.0f));
                 }
-                float yn = 0.0f, yo = 0.0f;
-                const int calls = mlp_wave_pairs(sW, want, xn, xo, yn, yo);
-                if (want)
-                    sr.set_guess(mlp_guess_from(sr.guess_anchor(), sr.guess_flow_new(), sr.guess_flow_old(), yn, yo));
-                acc.mlp_calls += (unsigned long long)calls;
-                acc.mlp_ticks += (unsigned long long)((long long)wall_clock64() - m0);
+                if (__any(net)) {                    // wave-uniform: all 64 lanes run the forward
+                    const long long m0 = (long long)wall_clock64();
+                    float xn[kMlpIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, xo[kMlpIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                    if (net) {
+                        mlp_segment_inputs(sr.guess_flow_new(), c, xn);
+                        mlp_segment_inputs(sr.guess_flow_old(), c, xo);
+                    }
+                    float yn = 0.0f, yo = 0.0f;
+                    const int calls = mlp_wave_pairs(sW, net, xn, xo, yn, yo);
+                    if (net)
+                        sr.set_guess(
+                            mlp_guess_from(sr.guess_anchor(), sr.guess_flow_new(), sr.guess_flow_old(), yn, yo));
+                    acc.mlp_calls += (unsigned long long)calls;
+                    acc.mlp_ticks += (unsigned long long)((long long)wall_clock64() - m0);
+                }
             }
         }
         // -------------------------------------- (2) solve phase, registers only
