@@ -931,6 +931,7 @@ __global__ void k_linear_coeffs(const float *__restrict__ params, int n, float d
 // wave touch adjacent words), the factors ffi to qp_out itself, which the back
 // substitution then overwrites in place.
 constexpr int kDiffMaxNodes = 500;
+constexpr int kDiffChunk = 8;   // nodes whose inputs are loaded together
 
 __global__ void __launch_bounds__(256) k_diffusive_tridiag(const DiffReach *__restrict__ rp,
                                                            const float *__restrict__ dx,
@@ -950,31 +951,58 @@ __global__ void __launch_bounds__(256) k_diffusive_tridiag(const DiffReach *__re
     ws[j] = e;
     out[0] = f;
     float q_im1 = q[0], q_i = (nc > 1) ? q[1] : q[0];
-    for (int i = 1; i < nc; i++) {
-        float dxi = dx[base + i - 1];
-        if (dxi < 1.0f) dxi = 1.0f;
-        const float alpha = 1.0f;
-        float diff_i = diffusivity[base + i];
-        if (diff_i < 0.01f) diff_i = 0.01f;
-        const float ppi = -theta * diff_i * dtini / (dxi * dxi) * 2.0f / (alpha * (alpha + 1.0f)) * alpha;
-        const float qqi = 1.0f - ppi * (alpha + 1.0f) / alpha;
-        const float rri = ppi / alpha;
-        const float q_ip1 = (i < nc - 1) ? q[i + 1] : q_i;
-        const float ssi = q_i + dtini * diff_i * (1.0f - theta) * (q_im1 - 2.0f * q_i + q_ip1) / (dxi * dxi);
-        float denom = ppi * e + qqi;
-        if (fabsf(denom) < 1e-20f) denom = 1e-20f;
-        e = -rri / denom;
-        f = (ssi - ppi * f) / denom;
-        ws[(size_t)i * nreach + j] = e;
-        out[i] = f;
-        q_im1 = q_i;
-        q_i = q_ip1;
+    // the node inputs do not depend on the elimination: each chunk of
+    // kDiffChunk nodes is loaded at once, then eliminated in order
+    for (int i0 = 1; i0 < nc; i0 += kDiffChunk) {
+        float dxv[kDiffChunk], dfv[kDiffChunk], qv[kDiffChunk];
+#pragma unroll
+        for (int u = 0; u < kDiffChunk; ++u) {
+            const int i = i0 + u;
+            dxv[u] = (i < nc) ? dx[base + i - 1] : 1.0f;
+            dfv[u] = (i < nc) ? diffusivity[base + i] : 1.0f;
+            qv[u] = (i < nc - 1) ? q[i + 1] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kDiffChunk; ++u) {
+            const int i = i0 + u;
+            if (i >= nc) break;
+            float dxi = dxv[u];
+            if (dxi < 1.0f) dxi = 1.0f;
+            const float alpha = 1.0f;
+            float diff_i = dfv[u];
+            if (diff_i < 0.01f) diff_i = 0.01f;
+            const float ppi = -theta * diff_i * dtini / (dxi * dxi) * 2.0f / (alpha * (alpha + 1.0f)) * alpha;
+            const float qqi = 1.0f - ppi * (alpha + 1.0f) / alpha;
+            const float rri = ppi / alpha;
+            const float q_ip1 = (i < nc - 1) ? qv[u] : q_i;
+            const float ssi = q_i + dtini * diff_i * (1.0f - theta) * (q_im1 - 2.0f * q_i + q_ip1) / (dxi * dxi);
+            float denom = ppi * e + qqi;
+            if (fabsf(denom) < 1e-20f) denom = 1e-20f;
+            e = -rri / denom;
+            f = (ssi - ppi * f) / denom;
+            ws[(size_t)i * nreach + j] = e;
+            out[i] = f;
+            q_im1 = q_i;
+            q_i = q_ip1;
+        }
     }
     float qn = e * q[nc - 1] + f;             // downstream ghost node: qp_prev[nc - 1]
     out[nc - 1] = qn;
-    for (int i = nc - 2; i >= 0; i--) {
-        qn = ws[(size_t)i * nreach + j] * qn + out[i];
-        out[i] = qn;
+    for (int i0 = nc - 2; i0 >= 0; i0 -= kDiffChunk) {
+        float ev[kDiffChunk], fv[kDiffChunk];
+#pragma unroll
+        for (int u = 0; u < kDiffChunk; ++u) {
+            const int i = i0 - u;
+            ev[u] = (i >= 0) ? ws[(size_t)i * nreach + j] : 0.0f;
+            fv[u] = (i >= 0) ? out[i] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kDiffChunk; ++u) {
+            const int i = i0 - u;
+            if (i < 0) break;
+            qn = ev[u] * qn + fv[u];
+            out[i] = qn;
+        }
     }
 }
 
