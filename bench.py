@@ -71,7 +71,7 @@ WORKLOADS = {
     "shreve-1m-x16": dict(kind=2, n=1_000_000, r=300_000, T=288, M=16, seed=42, mode="learned", cpu_steps=24),
     "batch-64x8": dict(kind="batch", n=64, r=(10_000, 200_000), T=288, M=8, seed=3, mode="reference", cpu_steps=5),
     "tree-1k": dict(kind=1, n=1000, r=0, T=288, M=1, seed=42, mode="learned", cpu_steps=288),
-    "linear-1m": dict(kind=2, n=1_000_000, r=300_000, T=288, M=1, seed=42, mode="linear", cpu_steps=288),
+    "linear-1m": dict(kind=2, n=1_000_000, r=300_000, T=288, M=1, seed=42, mode="linear", cpu_steps=288, units=8),
     "diffusive-10k": dict(kind="diffusive", n=10_000, r=0, T=1, M=1, seed=42, mode="diffusive", cpu_steps=1),
     "matrix-309k": dict(kind=2, n=309_000, r=309_000, T=24, M=1, seed=42, mode="matrix", cpu_steps=24),
 }
@@ -305,7 +305,7 @@ def bench_linear(trmc, spec, args):
                                f"{topo.n_reaches} reaches, {T} steps, coefficients at half bankfull",
                    "segments": topo.n_segments, "reaches": topo.n_reaches, "timesteps": T, "members": 1,
                    "seed": spec["seed"], "mode": "LINEAR", "parallelism": "1 GPU",
-                   "coefficients_s": coeff_s},
+                   "unit_segments": args.unit_segments, "block": args.block, "coefficients_s": coeff_s},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "kernel_ms": 1e3 * k_s,
@@ -485,7 +485,8 @@ def main():
     ap.add_argument("--workload", default="shreve-1m", choices=sorted(WORKLOADS))
     ap.add_argument("--mode", default=None, choices=sorted(MODES), help="headline mode (workload default)")
     ap.add_argument("--only", action="store_true", help="time the headline mode only")
-    ap.add_argument("--unit-segments", type=int, default=4)
+    ap.add_argument("--unit-segments", type=int, default=None,
+                    help="segments per unit (default: the workload's, 4; 8 for linear-1m)")
     ap.add_argument("--block", type=int, default=64)
     ap.add_argument("--batch-iters", type=int, default=12)
     ap.add_argument("--grid", type=int, default=0)
@@ -507,6 +508,8 @@ def main():
 
     trmc = _load("noaa_gpu_kernels_amd", os.path.join(ROOT, "noaa-gpu-kernels_amd", "__init__.py"))
     spec = WORKLOADS[args.workload]
+    if args.unit_segments is None:
+        args.unit_segments = spec.get("units", 4)
     if spec["mode"] == "linear":
         bench_linear(trmc, spec, args)
         return
