@@ -508,6 +508,9 @@ def main():
 
     trmc = _load("noaa_gpu_kernels_amd", os.path.join(ROOT, "noaa-gpu-kernels_amd", "__init__.py"))
     spec = WORKLOADS[args.workload]
+    # segments per unit: the workload's, else 2 for the rules that solve to
+    # convergence and 4 for the REFERENCE rule (measured best on shreve-1m)
+    units_given = args.unit_segments is not None
     if args.unit_segments is None:
         args.unit_segments = spec.get("units", 4)
     if spec["mode"] == "linear":
@@ -554,9 +557,16 @@ def main():
                  "batch_iters": opts["batch_iters"]}
         log(f"[bench] autotune: {tuned}")
 
+    def mode_opts(name):
+        o = dict(opts)
+        if not units_given and tuned is None and "units" not in spec:
+            o["unit_segments"] = 4 if name == "reference" else 2
+        return o
+
     recs, nets, finals = {}, {}, {}
     for name in order:
-        recs[name], nets[name] = time_mode(trmc, w, M, T, dt, name, opts, args, weights)
+        recs[name], nets[name] = time_mode(trmc, w, M, T, dt, name, mode_opts(name), args, weights)
+        recs[name]["unit_segments"] = mode_opts(name)["unit_segments"]
         finals[name] = nets[name].state()
         if name not in ("learned", head):
             nets.pop(name).close()     # keep one full forcing in HBM at a time
@@ -608,7 +618,7 @@ def main():
                      "reference": "REFERENCE rule (1% / 1 cm stop), drop-in"}[head],
             "schedule": {0: "auto (dataflow)", 1: "time-skewed persistent, device barrier",
                          2: "level launches, hipGraph per step", 3: "dataflow persistent"}[args.schedule],
-            "unit_segments": opts["unit_segments"], "unit_levels": info["n_levels"],
+            "unit_segments": mode_opts(head)["unit_segments"], "unit_levels": info["n_levels"],
             "super_levels": info["super_levels"], "grid_blocks": hrec.get("grid_blocks"),
             "block": opts["block_size"], "batch_iters": opts["batch_iters"], "parallelism": "1 GPU",
         },
@@ -636,7 +646,7 @@ def main():
         out["cpu_baseline"] = rec
         rnet = nets.get("reference")
         if rnet is None:
-            rnet = new_net(trmc, w, M, 0, opts, Tc)
+            rnet = new_net(trmc, w, M, 0, mode_opts("reference"), Tc)
             nets["reference"] = rnet
         rnet.reset()
         rnet.run(Tc, dt)

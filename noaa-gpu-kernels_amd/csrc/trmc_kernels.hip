@@ -630,32 +630,38 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     flush_acc(a, acc, wait_ticks, (long long)wall_clock64() - t_start);
 }
 
+// The element kernels below loop over their range with the grid's stride: a
+// dispatch counts its work-items in 32 bits, so one lane per element would
+// truncate ranges beyond 2^32 (a 1M-unit, 16-member, 288-step history has
+// 4.6e9 entries).
 __global__ void k_hist_init(unsigned long long *hist, const double *ring, int src_slot, long long um, int T)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = um * (long long)(T + 1);
-    if (i >= total) return;
-    if (i < um) hist[i] = (unsigned long long)__double_as_longlong(ring[(size_t)src_slot * um + i]);
-    else hist[i] = kPending;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        if (i < um) hist[i] = (unsigned long long)__double_as_longlong(ring[(size_t)src_slot * um + i]);
+        else hist[i] = kPending;
+    }
 }
 
 __global__ void k_hist_finish(const unsigned long long *hist, double *ring, int dst_slot, long long um, int T)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= um) return;
-    ring[(size_t)dst_slot * um + i] = __longlong_as_double((long long)hist[(size_t)T * um + i]);
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < um; i += stride)
+        ring[(size_t)dst_slot * um + i] = __longlong_as_double((long long)hist[(size_t)T * um + i]);
 }
 
 __global__ void k_hist_extract(const unsigned long long *hist, const int *reach_last_unit, int R, int U,
                                int M, int T, double *out)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = (long long)M * T * R;
-    if (i >= total) return;
-    const int r = (int)(i % R);
-    const long long mt = i / R;
-    const int t = (int)(mt % T), m = (int)(mt / T);
-    out[i] = __longlong_as_double((long long)hist[((size_t)(t + 1) * U + reach_last_unit[r]) * M + m]);
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const int r = (int)(i % R);
+        const long long mt = i / R;
+        const int t = (int)(mt % T), m = (int)(mt / T);
+        out[i] = __longlong_as_double((long long)hist[((size_t)(t + 1) * U + reach_last_unit[r]) * M + m]);
+    }
 }
 
 // One level of one step (reference-style schedule, graph-captured per step).
@@ -746,11 +752,12 @@ __global__ void k_scatter_forcing(const TSrc *__restrict__ src, int n_src, long 
                                   const int *__restrict__ seg_of_dev, int nd, int M, int m,
                                   double *__restrict__ dst)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const long long t = i / nd;
-    const int ds = (int)(i - t * nd);
-    dst[(size_t)i * M + m] = (double)src[(size_t)t * n_src + seg_of_dev[ds]];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const long long t = i / nd;
+        const int ds = (int)(i - t * nd);
+        dst[(size_t)i * M + m] = (double)src[(size_t)t * n_src + seg_of_dev[ds]];
+    }
 }
 
 // Synthetic lateral inflow straight into the forcing layout
@@ -1119,13 +1126,19 @@ hipError_t route_dataflow_occupancy(int mode, int block, int *blocks_per_cu)
                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow<2>, block, 0);
 }
 
+// blocks of 256 for a grid-stride element kernel over n elements: enough to
+// fill the GPU, at most 2^20
+static unsigned element_blocks(long long n)
+{
+    return (unsigned)std::min<long long>((n + 255) / 256, 1LL << 20);
+}
+
 hipError_t launch_hist_init(unsigned long long *hist, const double *ring, int src_slot, long long um, int T,
                             hipStream_t stream)
 {
     const long long total = um * (long long)(T + 1);
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_hist_init, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, hist, ring,
-                       src_slot, um, T);
+    hipLaunchKernelGGL(k_hist_init, dim3(element_blocks(total)), dim3(256), 0, stream, hist, ring, src_slot, um, T);
     return hipGetLastError();
 }
 
@@ -1133,8 +1146,7 @@ hipError_t launch_hist_finish(const unsigned long long *hist, double *ring, int 
                               hipStream_t stream)
 {
     if (um <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_hist_finish, dim3((unsigned)((um + 255) / 256)), dim3(256), 0, stream, hist, ring,
-                       dst_slot, um, T);
+    hipLaunchKernelGGL(k_hist_finish, dim3(element_blocks(um)), dim3(256), 0, stream, hist, ring, dst_slot, um, T);
     return hipGetLastError();
 }
 
@@ -1143,8 +1155,8 @@ hipError_t launch_hist_extract(const unsigned long long *hist, const int *reach_
 {
     const long long total = (long long)M * T * R;
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_hist_extract, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, hist,
-                       reach_last_unit, R, U, M, T, out);
+    hipLaunchKernelGGL(k_hist_extract, dim3(element_blocks(total)), dim3(256), 0, stream, hist, reach_last_unit, R,
+                       U, M, T, out);
     return hipGetLastError();
 }
 
@@ -1179,7 +1191,7 @@ hipError_t launch_scatter_forcing_f64(const double *src, int n_src, int T, const
 {
     const long long total = (long long)T * nd;
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scatter_forcing<double>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+    hipLaunchKernelGGL(k_scatter_forcing<double>, dim3(element_blocks(total)), dim3(256), 0,
                        stream, src, n_src, total, seg_of_dev, nd, M, m, dst);
     return hipGetLastError();
 }
@@ -1189,7 +1201,7 @@ hipError_t launch_scatter_forcing_f32(const float *src, int n_src, int T, const 
 {
     const long long total = (long long)T * nd;
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scatter_forcing<float>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+    hipLaunchKernelGGL(k_scatter_forcing<float>, dim3(element_blocks(total)), dim3(256), 0,
                        stream, src, n_src, total, seg_of_dev, nd, M, m, dst);
     return hipGetLastError();
 }
