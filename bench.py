@@ -68,7 +68,8 @@ def _load(name, path):
 # T timesteps, M members, seed, headline mode, CPU sample steps
 WORKLOADS = {
     "shreve-1m": dict(kind=2, n=1_000_000, r=300_000, T=288, M=1, seed=42, mode="learned", cpu_steps=24),
-    "shreve-1m-x16": dict(kind=2, n=1_000_000, r=300_000, T=288, M=16, seed=42, mode="learned", cpu_steps=24),
+    "shreve-1m-x16": dict(kind=2, n=1_000_000, r=300_000, T=288, M=16, seed=42, mode="learned", cpu_steps=24,
+                          conv_units=1),
     "batch-64x8": dict(kind="batch", n=64, r=(10_000, 200_000), T=288, M=8, seed=3, mode="reference", cpu_steps=5),
     "tree-1k": dict(kind=1, n=1000, r=0, T=288, M=1, seed=42, mode="learned", cpu_steps=288),
     "linear-1m": dict(kind=2, n=1_000_000, r=300_000, T=288, M=1, seed=42, mode="linear", cpu_steps=288, units=8),
@@ -508,8 +509,8 @@ def main():
 
     trmc = _load("noaa_gpu_kernels_amd", os.path.join(ROOT, "noaa-gpu-kernels_amd", "__init__.py"))
     spec = WORKLOADS[args.workload]
-    # segments per unit: the workload's, else 2 for the rules that solve to
-    # convergence and 4 for the REFERENCE rule (measured best on shreve-1m)
+    # segments per unit: the workload's, else (measured best) 2 for the rules
+    # that solve to convergence (1 with 16 members) and 4 for the REFERENCE rule
     units_given = args.unit_segments is not None
     if args.unit_segments is None:
         args.unit_segments = spec.get("units", 4)
@@ -560,7 +561,7 @@ def main():
     def mode_opts(name):
         o = dict(opts)
         if not units_given and tuned is None and "units" not in spec:
-            o["unit_segments"] = 4 if name == "reference" else 2
+            o["unit_segments"] = 4 if name == "reference" else spec.get("conv_units", 2)
         return o
 
     recs, nets, finals = {}, {}, {}
