@@ -92,7 +92,8 @@ typedef struct { double dx, bw, tw, twcc, n_ch, ncc, cs, s0; } trmc_chparams_f64
 typedef struct {
     int    mode;            /* TRMC_MODE_* (default REFERENCE) */
     int    schedule;        /* TRMC_SCHED_* (default AUTO) */
-    int    unit_segments;   /* max segments per work unit; 0 = auto (4) */
+    int    unit_segments;   /* max segments per work unit; 0 = auto (REFERENCE 4,
+                               CONVERGED and LEARNED 2, LINEAR 8) */
     int    block_size;      /* threads per workgroup; 0 = auto (64) */
     int    n_members;       /* ensemble members sharing params (>= 1) */
     int    record_history;  /* keep every step's last-segment outflow per reach */

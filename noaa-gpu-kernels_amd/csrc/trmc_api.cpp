@@ -292,9 +292,13 @@ int trmc_network_create(const int *reach_seg_start, const int *reach_seg_len,
     if (o.block_size <= 0) o.block_size = 64;
     if (o.block_size % 64 != 0 || o.block_size > 256)
         return fail(TRMC_ERR_ARG, "block_size must be a multiple of 64, at most 256");
-    if (o.unit_segments <= 0) o.unit_segments = 4;   // measured best on the Shreve workloads
     if (o.mode < TRMC_MODE_REFERENCE || o.mode > TRMC_MODE_LINEAR)
         return fail(TRMC_ERR_ARG, "unknown mode %d", o.mode);
+    // measured best on the 1M-segment Shreve network: the REFERENCE rule's
+    // short solves want longer units, the converged rules shorter ones (more
+    // items side by side), LINEAR (no solve) longer still
+    if (o.unit_segments <= 0)
+        o.unit_segments = o.mode == TRMC_MODE_REFERENCE ? 4 : o.mode == TRMC_MODE_LINEAR ? 8 : 2;
     if (o.tol <= 0.0) o.tol = 1e-13;
     if (o.max_iter <= 0) o.max_iter = 60;
     if (o.learned_budget <= 0) o.learned_budget = 12;
