@@ -145,6 +145,7 @@ def time_mode(trmc, w, M, T, dt, name, opts, args, weights=None):
         net.run(T, dt)
     trmc.synchronize()
     kernel_ms, iters, fallbacks, capped, mlp_calls, mlp_frac, stats = [], 0, 0, 0, 0, 0.0, {}
+    wait_frac = 0.0
     t0 = time.perf_counter()
     for _ in range(args.steps):
         net.reset()
@@ -155,6 +156,7 @@ def time_mode(trmc, w, M, T, dt, name, opts, args, weights=None):
         capped += stats["not_converged"]
         mlp_calls += stats["mlp_calls"]
         mlp_frac += stats["mlp_frac"]
+        wait_frac += stats["barrier_wait_frac"]
     trmc.synchronize()
     secs = time.perf_counter() - t0
     seg_steps = w.topo.n_segments * T * M
@@ -162,7 +164,11 @@ def time_mode(trmc, w, M, T, dt, name, opts, args, weights=None):
            "kernel_ms": float(np.mean(kernel_ms)),
            "evaluations_per_seg_step": iters / (seg_steps * args.steps),
            "fallbacks": fallbacks // args.steps, "not_converged": capped // args.steps,
-           "grid_blocks": stats.get("grid_blocks")}
+           "grid_blocks": stats.get("grid_blocks"),
+           # the dataflow schedule has no grid barrier: the share of lane time
+           # spent waiting for upstream outflows takes the barrier stall's
+           # place (sampled on the first lane of every block)
+           "upstream_wait_frac": wait_frac / args.steps}
     if name == "learned":
         rec["network_forwards"] = mlp_calls // args.steps
         rec["network_time_frac"] = mlp_frac / args.steps
