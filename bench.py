@@ -27,6 +27,8 @@ Other workloads (--workload):
   diffusive-10k  the diffusive-wave tridiagonal step on 10,000 reaches
   matrix-309k    the matrix form of the linearized MC (linear_mc_matrix*.cu):
                  309,000 one-segment reaches, 24 steps, tol 1e-6
+  matrix-secant-309k  the same with the secant-fitted coefficients of
+                 linear_mc_matrix_secant.cu, tol 1e-7
 
 Prints one JSON line with the driver's fields plus
   roofline      HBM roofline of the routing kernel (112 algorithmic bytes per
@@ -75,6 +77,8 @@ WORKLOADS = {
     "linear-1m": dict(kind=2, n=1_000_000, r=300_000, T=288, M=1, seed=42, mode="linear", cpu_steps=288, units=8),
     "diffusive-10k": dict(kind="diffusive", n=10_000, r=0, T=1, M=1, seed=42, mode="diffusive", cpu_steps=1),
     "matrix-309k": dict(kind=2, n=309_000, r=309_000, T=24, M=1, seed=42, mode="matrix", cpu_steps=24),
+    "matrix-secant-309k": dict(kind=2, n=309_000, r=309_000, T=24, M=1, seed=42, mode="matrix", cpu_steps=24,
+                               coeffs="secant", tol=1e-7),
 }
 MODES = {"learned": 2, "plain": 1, "reference": 0, "powerlaw": 2}
 
@@ -413,7 +417,8 @@ def bench_matrix(trmc, spec, args):
     The CPU baseline is the oracle's sequential float step loop on one core;
     the GPU must equal the oracle's loop in its own summation order bit for
     bit, and stay within 1 % of the FP64 level-scheduled recurrence."""
-    n, T, tol = spec["n"], spec["T"], 1e-6
+    n, T, tol = spec["n"], spec["T"], spec.get("tol", 1e-6)
+    secant = spec.get("coeffs") == "secant"
     orc = _load("trmc_oracle", os.path.join(ROOT, "oracle", "oracle.py"))
     w = trmc.make_workload(spec["kind"], n, 1, n_reaches=spec["r"], seed=spec["seed"])
     t = w.topo
@@ -422,7 +427,7 @@ def bench_matrix(trmc, spec, args):
     qlat = rng.lognormal(np.log(0.1) - 0.5, 1.0, (T, n)).astype(np.float32)
     Q0 = np.asarray(w.qdp0, np.float32)
     t0 = time.perf_counter()
-    c = trmc.matrix_coeffs(p32)
+    c = trmc.matrix_secant_coeffs(p32, t, qlat[0]) if secant else trmc.matrix_coeffs(p32)
     t1 = time.perf_counter()
     PQ, Pq = trmc.matrix_build(t.reach_up_ptr, t.reach_up_idx, c, tol)
     t2 = time.perf_counter()
@@ -446,17 +451,19 @@ def bench_matrix(trmc, spec, args):
         "value": n * T / (ms * 1e-3), "unit": "reach-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": 1, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (seeded Shreve-style tree, one segment per reach; inputs resident)",
-        "config": {"workload": f"matrix-309k: {n} one-segment reaches, {t.n_levels} levels, {T} steps of "
-                               f"dt=300 s, tol {tol:g}", "reaches": n, "levels": t.n_levels, "timesteps": T,
+        "config": {"workload": f"{args.workload}: {n} one-segment reaches, {t.n_levels} levels, {T} steps of "
+                               f"dt=300 s, tol {tol:g}, "
+                               + ("secant-fitted coefficients" if secant else "half-bankfull coefficients (X=0.25)"),
+                   "reaches": n, "levels": t.n_levels, "timesteps": T,
                    "tol": tol, "nnz_P_Q": nnz[0], "nnz_P_q": nnz[1], "union_entries": union,
                    "lanes_per_row": "by union row length U (smallest power of two G with 4 G >= U, <= 64)",
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic("matrix-309k"),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic(args.workload),
                      "copy_reference_gbps": copy_gbps},
         "ms_per_timestep": step_ms,
         "lanes_sweep_ms_per_pass": sweep,
-        "setup_s": {"coefficients_gpu": t1 - t0, "matrices_host": t2 - t1},
+        "setup_s": {("coefficients_host" if secant else "coefficients_gpu"): t1 - t0, "matrices_host": t2 - t1},
     }
     if not args.no_cpu:
         t0 = time.perf_counter()

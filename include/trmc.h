@@ -213,6 +213,21 @@ typedef struct {
 int trmc_matrix_coeffs_f32(const trmc_chparams_f32 *params, int n, float dt,
                            float *C1, float *C2, float *C3, float *C4oq);
 
+/* The secant-fitted coefficients of wavefront/linear_mc_matrix_secant.cu
+ * (mc_secant_solve_cpu :57-195, called from main :358-418): per reach, the
+ * float MC secant on depth at a steady reference flow (upstream reference
+ * flows + own first-step lateral inflow + the mean first-step inflow over all
+ * segments, in level order), its last iterate's C1, C2, C3 and C4/ql; the
+ * half-bankfull X = 0.25 coefficients for invalid channels.  One segment per
+ * reach: params [n_reaches], qlat0 = the first step's lateral inflow
+ * [n_segments >= n_reaches].  Host threads; float operations in the
+ * reference's order. */
+int trmc_matrix_secant_coeffs_f32(const trmc_chparams_f32 *params, int n_reaches,
+                                  const int *reach_up_ptr, const int *reach_up_idx,
+                                  const int *level_ptr, const int *level_reach, int n_levels,
+                                  const float *qlat0, int n_segments, float dt,
+                                  float *C1, float *C2, float *C3, float *C4oq);
+
 /* build_propagation_matrices (linear_mc_matrix.cu:118-229): P_Q and P_q row
  * by row in reach order, each row C1 x the sum of its upstream rows (pruned
  * below tol), plus C2 at each upstream column and C3 (P_Q) or C4/ql (P_q) on

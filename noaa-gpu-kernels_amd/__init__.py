@@ -150,6 +150,7 @@ def _declare(L: C.CDLL) -> None:
         "trmc_free": (None, [V]),
         "trmc_matrix_coeffs_f32": (I, [_p(ChParamsF32), I, C.c_float] + [pF] * 4),
         "trmc_matrix_build": (I, [I, pI, pI] + [pF] * 4 + [C.c_float, _p(Csr), _p(Csr)]),
+        "trmc_matrix_secant_coeffs_f32": (I, [_p(ChParamsF32), I, pI, pI, pI, pI, I, pF, I, C.c_float] + [pF] * 4),
         "trmc_free_csr": (None, [_p(Csr)]),
         "trmc_matrix_route_f32": (I, [_p(Csr), _p(Csr), pF, pF, I, I, pF]),
         "trmc_matrix_route_timed_f32": (I, [_p(Csr), _p(Csr), pF, pF, I, I, pF, I, pD]),
@@ -556,6 +557,23 @@ def matrix_coeffs(params, dt: float = 300.0):
     n = p.shape[0]
     out = [np.zeros(n, np.float32) for _ in range(4)]
     _check(lib().trmc_matrix_coeffs_f32(_ptr(p, ChParamsF32), n, float(dt), *[_ptr(o, C.c_float) for o in out]))
+    return tuple(out)
+
+
+def matrix_secant_coeffs(params, topo: "Topology", qlat0, dt: float = 300.0):
+    """The secant-fitted coefficients of linear_mc_matrix_secant.cu (one segment
+    per reach): (C1, C2, C3, C4/ql), float [n_reaches] each, from the first
+    step's lateral inflow qlat0 [n_segments]."""
+    p = params_f32(params)
+    q = _f32(qlat0)
+    R = topo.n_reaches
+    out = [np.zeros(R, np.float32) for _ in range(4)]
+    ptr, lptr, lrch = _i32(topo.reach_up_ptr), _i32(topo.level_ptr), _i32(topo.level_reach)
+    idx = _i32(topo.reach_up_idx) if len(topo.reach_up_idx) else np.zeros(1, np.int32)
+    _check(lib().trmc_matrix_secant_coeffs_f32(
+        _ptr(p, ChParamsF32), R, _ptr(ptr, C.c_int), _ptr(idx, C.c_int), _ptr(lptr, C.c_int),
+        _ptr(lrch, C.c_int), topo.n_levels, _ptr(q, C.c_float), len(q), float(dt),
+        *[_ptr(o, C.c_float) for o in out]))
     return tuple(out)
 
 

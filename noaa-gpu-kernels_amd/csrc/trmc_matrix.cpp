@@ -278,3 +278,241 @@ int trmc_matrix_build(int n, const int *up_ptr, const int *up_idx, const float *
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Secant-fitted coefficients of the matrix form, after
+// wavefront/linear_mc_matrix_secant.cu: per reach, the float MC secant on
+// depth at a steady reference flow (mc_secant_solve_cpu, :57-195), its last
+// iterate's C1..C3 and C4/ql; the reference flows follow the reach DAG in
+// level order (main, :358-418).  Float operations in the reference's order
+// (the coefficients are the matrices' input and are checked bit for bit);
+// host threads over reaches, one-time set-up.
+namespace {
+
+struct SecGeo {
+    float twl, R, A, AC, WP, WPC;
+};
+
+SecGeo sec_geometry(float h, float bfd, float bw, float twcc, float z)
+{
+    SecGeo g;
+    g.twl = bw + 2.0f * z * h;
+    float hgb = std::fmax(h - bfd, 0.0f), hlb = std::fmin(bfd, h);
+    if (hgb > 0.0f && twcc <= 0.0f) {
+        hgb = 0.0f;
+        hlb = h;
+    }
+    g.A = (bw + hlb * z) * hlb;
+    g.WP = bw + 2.0f * hlb * std::sqrt(1.0f + z * z);
+    g.AC = twcc * hgb;
+    g.WPC = (hgb > 0.0f) ? (twcc + 2.0f * hgb) : 0.0f;
+    g.R = (g.WP + g.WPC > 0.0f) ? (g.A + g.AC) / (g.WP + g.WPC) : 0.0f;
+    return g;
+}
+
+float sec_celerity(float h, const SecGeo &g, const trmc_chparams_f32 &p, float z)
+{
+    if (!(h > 0.0f)) return 0.0f;
+    const float r23 = std::pow(g.R, 2.0f / 3.0f), r53 = std::pow(g.R, 5.0f / 3.0f);
+    return std::fmax(0.0f, (std::sqrt(p.s0) / p.n_ch) *
+                               ((5.0f / 3.0f) * r23 -
+                                (2.0f / 3.0f) * r53 * (2.0f * std::sqrt(1.0f + z * z) / (p.bw + 2.0f * h * z))));
+}
+
+float sec_manning(const SecGeo &g, const trmc_chparams_f32 &p)
+{
+    if (!(g.WP + g.WPC > 0.0f)) return 0.0f;
+    return (1.0f / (((g.WP * p.n_ch) + (g.WPC * p.ncc)) / (g.WP + g.WPC))) * (g.A + g.AC) *
+           std::pow(g.R, 2.0f / 3.0f) * std::sqrt(p.s0);
+}
+
+// half-bankfull coefficients with X = 0.25 (the reference's fallback)
+void sec_fallback(const trmc_chparams_f32 &p, float dt, float z, float bfd, float c[4])
+{
+    const float h = std::fmax(0.5f * bfd, 0.1f);
+    const float A = (p.bw + h * z) * h;
+    const float WP = p.bw + 2.0f * h * std::sqrt(1.0f + z * z);
+    const float R = (WP > 0.0f) ? A / WP : 0.0f;
+    const float r23 = std::pow(R, 2.0f / 3.0f), r53 = std::pow(R, 5.0f / 3.0f);
+    const float Ck = std::fmax(1e-6f, (std::sqrt(p.s0) / p.n_ch) *
+                                          ((5.0f / 3.0f) * r23 - (2.0f / 3.0f) * r53 *
+                                                                     (2.0f * std::sqrt(1.0f + z * z) /
+                                                                      (p.bw + 2.0f * h * z))));
+    const float Km = std::fmax(dt, p.dx / Ck);
+    const float X = 0.25f;
+    float D = Km * (1.0f - X) + dt / 2.0f;
+    if (D == 0.0f) D = 1.0f;
+    c[0] = (Km * X + dt / 2.0f) / D;
+    c[1] = (dt / 2.0f - Km * X) / D;
+    c[2] = (Km * (1.0f - X) - dt / 2.0f) / D;
+    c[3] = dt / D;
+}
+
+// one reach: c = {C1, C2, C3, C4/ql}
+void sec_solve(float dt, float qup, float quc, float qdp, float ql, const trmc_chparams_f32 &p, float depthp,
+               float c[4])
+{
+    const float z = (p.cs == 0.0f) ? 1.0f : 1.0f / p.cs;
+    float bfd;
+    if (p.bw > p.tw) bfd = p.bw / 0.00001f;
+    else if (p.bw == p.tw) bfd = p.bw / (2.0f * z);
+    else bfd = (p.tw - p.bw) / (2.0f * z);
+    if (p.n_ch <= 0.0f || p.s0 <= 0.0f || z <= 0.0f || p.bw <= 0.0f ||
+        (ql <= 0.0f && qup <= 0.0f && quc <= 0.0f && qdp <= 0.0f)) {
+        sec_fallback(p, dt, z, bfd, c);
+        return;
+    }
+    const float mindepth = 0.01f;
+    const float depthc = std::fmax(depthp, 0.0f);
+    float h = depthc * 1.33f + mindepth, h_0 = depthc * 0.67f;
+    float C1 = 0, C2 = 0, C3 = 0, C4 = 0, X = 0.25f, Qj_0 = 0.0f, Qj = 0.0f;
+    int maxiter = 100;
+    for (int att = 0; att < 5; att++) {
+        float aerror = 0.01f, rerror = 1.0f;
+        int iter = 0;
+        while (rerror > 0.01f && aerror >= mindepth && iter <= maxiter) {
+            // first point h_0: its X uses the previous iteration's Qj_0
+            const SecGeo g0 = sec_geometry(h_0, bfd, p.bw, p.twcc, z);
+            const float Ck0 = sec_celerity(h_0, g0, p, z);
+            const float Km0 = (Ck0 > 0.0f) ? std::fmax(dt, p.dx / Ck0) : dt;
+            const float twu0 = (h_0 > bfd && p.twcc > 0.0f) ? p.twcc : g0.twl;
+            const float X0 = (Ck0 > 0.0f && twu0 * p.s0 * Ck0 * p.dx > 0.0f)
+                                 ? std::fmin(0.5f, std::fmax(0.0f, 0.5f * (1.0f - Qj_0 / (2.0f * twu0 * p.s0 * Ck0 *
+                                                                                          p.dx))))
+                                 : 0.5f;
+            float D0 = Km0 * (1.0f - X0) + dt / 2.0f;
+            if (D0 == 0.0f) D0 = 1.0f;
+            const float c1_0 = (Km0 * X0 + dt / 2.0f) / D0;
+            const float c2_0 = (dt / 2.0f - Km0 * X0) / D0;
+            const float c3_0 = (Km0 * (1.0f - X0) - dt / 2.0f) / D0;
+            const float c4_0 = (ql * dt) / D0;
+            const float q0 = c1_0 * qup + c2_0 * quc + c3_0 * qdp + c4_0;
+            Qj_0 = q0 - sec_manning(g0, p);
+            // second point h
+            const SecGeo g1 = sec_geometry(h, bfd, p.bw, p.twcc, z);
+            const float Ck1 = sec_celerity(h, g1, p, z);
+            const float Km1 = (Ck1 > 0.0f) ? std::fmax(dt, p.dx / Ck1) : dt;
+            const float twu1 = (h > bfd && p.twcc > 0.0f) ? p.twcc : g1.twl;
+            X = (Ck1 > 0.0f && twu1 * p.s0 * Ck1 * p.dx > 0.0f)
+                    ? std::fmin(0.5f, std::fmax(0.25f, 0.5f * (1.0f - q0 / (2.0f * twu1 * p.s0 * Ck1 * p.dx))))
+                    : 0.5f;
+            float D1 = Km1 * (1.0f - X) + dt / 2.0f;
+            if (D1 == 0.0f) D1 = 1.0f;
+            C1 = (Km1 * X + dt / 2.0f) / D1;
+            C2 = (dt / 2.0f - Km1 * X) / D1;
+            C3 = (Km1 * (1.0f - X) - dt / 2.0f) / D1;
+            C4 = (ql * dt) / D1;
+            if (C4 < 0.0f && std::fabs(C4) > C1 * qup + C2 * quc + C3 * qdp) C4 = -(C1 * qup + C2 * quc + C3 * qdp);
+            Qj = (C1 * qup + C2 * quc + C3 * qdp + C4) - sec_manning(g1, p);
+            float h_1 = (Qj_0 - Qj != 0.0f) ? h - (Qj * (h_0 - h)) / (Qj_0 - Qj) : h;
+            if (h_1 < 0.0f) h_1 = h;
+            if (h > 0.0f) {
+                rerror = std::fabs((h_1 - h) / h);
+                aerror = std::fabs(h_1 - h);
+            } else {
+                rerror = 0.0f;
+                aerror = 0.9f;
+            }
+            h_0 = std::fmax(0.0f, h);
+            h = std::fmax(0.0f, h_1);
+            iter++;
+            if (h < mindepth) break;
+        }
+        if (iter < maxiter) break;
+        h *= 1.33f;
+        h_0 *= 0.67f;
+        maxiter += 25;
+    }
+    c[0] = C1;
+    c[1] = C2;
+    c[2] = C3;
+    if (std::fabs(ql) > 1e-12f) {
+        c[3] = C4 / ql;
+    } else {
+        float fb[4];
+        sec_fallback(p, dt, z, bfd, fb);
+        c[3] = fb[3];
+    }
+}
+
+}  // namespace
+
+extern "C" int trmc_matrix_secant_coeffs_f32(const trmc_chparams_f32 *params, int n_reaches, const int *up_ptr,
+                                             const int *up_idx, const int *level_ptr, const int *level_reach,
+                                             int n_levels, const float *qlat0, int n_segments, float dt, float *C1,
+                                             float *C2, float *C3, float *C4oq)
+{
+    if (n_reaches < 0 || n_levels < 0 || n_segments < 0) return trmc::fail(TRMC_ERR_ARG, "negative size");
+    if (n_reaches == 0) return TRMC_OK;
+    if (!params || !up_ptr || !level_ptr || !level_reach || !qlat0 || !C1 || !C2 || !C3 || !C4oq)
+        return trmc::fail(TRMC_ERR_ARG, "null array");
+    if (n_segments < n_reaches) return trmc::fail(TRMC_ERR_ARG, "qlat0 needs a value per reach (n_segments >= n_reaches)");
+    if (level_ptr[0] != 0 || level_ptr[n_levels] != n_reaches)
+        return trmc::fail(TRMC_ERR_TOPOLOGY, "level_ptr must run from 0 to n_reaches");
+    // every reach once, its upstream reaches in earlier levels
+    std::vector<int> lev((size_t)n_reaches, -1);
+    for (int L = 0; L < n_levels; ++L) {
+        if (level_ptr[L + 1] < level_ptr[L]) return trmc::fail(TRMC_ERR_TOPOLOGY, "level_ptr decreases at %d", L);
+        for (int i = level_ptr[L]; i < level_ptr[L + 1]; ++i) {
+            const int r = level_reach[i];
+            if (r < 0 || r >= n_reaches || lev[r] >= 0) return trmc::fail(TRMC_ERR_TOPOLOGY, "level_reach entry %d", i);
+            lev[r] = L;
+        }
+    }
+    for (int r = 0; r < n_reaches; ++r)
+        for (int k = up_ptr[r]; k < up_ptr[r + 1]; ++k)
+            if (!up_idx || up_idx[k] < 0 || up_idx[k] >= n_reaches || lev[up_idx[k]] >= lev[r])
+                return trmc::fail(TRMC_ERR_TOPOLOGY, "reach %d: upstream not in an earlier level", r);
+    // mean first-step lateral inflow over all segments, in double
+    double mean = 0.0;
+    for (int i = 0; i < n_segments; ++i) mean += qlat0[i];
+    mean /= n_segments;
+    const float fmean = (float)mean;
+    // steady reference flows: upstream sum + own inflow + the mean, in level order
+    std::vector<float> refq((size_t)n_reaches, 0.0f);
+    for (int L = 0; L < n_levels; ++L)
+        for (int i = level_ptr[L]; i < level_ptr[L + 1]; ++i) {
+            const int r = level_reach[i];
+            float sum_up = 0.0f;
+            for (int k = up_ptr[r]; k < up_ptr[r + 1]; ++k) sum_up += refq[up_idx[k]];
+            refq[r] = sum_up + qlat0[r] + fmean;
+        }
+    auto work = [&](int b, int e) {
+        for (int r = b; r < e; ++r) {
+            const trmc_chparams_f32 &p = params[r];
+            const float qdp = std::fmax(refq[r], 0.1f);
+            float qup = 0.0f;
+            for (int k = up_ptr[r]; k < up_ptr[r + 1]; ++k) qup += refq[up_idx[k]];
+            qup = std::fmax(qup, 0.1f);
+            const float quc = qup;
+            const float ql = std::fmax(qlat0[r], 0.01f);
+            const float depthp = std::fmax(0.1f, std::pow(qdp / (p.tw + 1.0f), 0.6f));
+            float c[4];
+            sec_solve(dt, qup, quc, qdp, ql, p, depthp, c);
+            C1[r] = c[0];
+            C2[r] = c[1];
+            C3[r] = c[2];
+            C4oq[r] = c[3];
+        }
+    };
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = (int)std::min<unsigned>(std::min(hw, 16u), (unsigned)std::max(1, n_reaches / 4096));
+    const int per = (n_reaches + nt - 1) / nt;
+    std::vector<std::thread> pool;
+    int started_to = 0;
+    for (int t = 1; t < nt; ++t) {
+        const int b = t * per, e = std::min(n_reaches, b + per);
+        if (b >= e) break;
+        try {
+            pool.emplace_back(work, b, e);
+            started_to = e;
+        } catch (...) {
+            break;
+        }
+    }
+    work(0, std::min(per, n_reaches));
+    if (started_to < n_reaches && started_to > 0) work(started_to, n_reaches);
+    else if (started_to == 0 && per < n_reaches) work(per, n_reaches);
+    for (auto &th : pool) th.join();
+    return TRMC_OK;
+}

@@ -23,6 +23,9 @@
 #                               and cusparse.h at 36-37), 55 (ChParams, float),
 #                               58-84 (compute_mc_coeffs_cpu), 111-229 (Matrix,
 #                               build_propagation_matrices)
+#   wavefront/linear_mc_matrix_secant.cu: 23-32 (std includes, skipping
+#                               cuda_runtime.h and cusparse.h at 33-34), 52
+#                               (ChParams, float), 57-195 (mc_secant_solve_cpu)
 #   ../owp_extended_kernels.cu: 13-16 (std includes, skipping cuda_runtime.h at 17)
 #                               19-20 (MAX_NODES), 28-32 (DiffReachParams)
 #                               35-94 (cpu_diffusive_tridiag), 347-362 (gen_diff_data)
@@ -32,7 +35,7 @@ HERE=$(cd "$(dirname "$0")" && pwd)
 OUT="$HERE/_ref"
 if [ ! -f "$REF/wavefront/wavefront_mc.cu" ] || [ ! -f "$REF/troute_mc_final.cu" ] || \
    [ ! -f "$REF/wavefront/linear_mc.cu" ] || [ ! -f "$REF/../owp_extended_kernels.cu" ] || \
-   [ ! -f "$REF/wavefront/linear_mc_matrix.cu" ]; then
+   [ ! -f "$REF/wavefront/linear_mc_matrix.cu" ] || [ ! -f "$REF/wavefront/linear_mc_matrix_secant.cu" ]; then
     echo "build_ref: reference sources not present; oracle/_ref not rebuilt"
     exit 0
 fi
@@ -46,6 +49,8 @@ trap 'rm -rf "$TMP"' EXIT
   cat "$HERE/ref_driver_linear.inc"; } > "$TMP/ref_linear.cpp"
 { sed -n '26,35p;55p;58,84p;111,229p' "$REF/wavefront/linear_mc_matrix.cu"
   cat "$HERE/ref_driver_matrix.inc"; } > "$TMP/ref_matrix.cpp"
+{ sed -n '23,32p;52p;57,195p' "$REF/wavefront/linear_mc_matrix_secant.cu"
+  cat "$HERE/ref_driver_matrix_secant.inc"; } > "$TMP/ref_matrix_secant.cpp"
 { sed -n '13,16p;19,20p;28,32p;35,94p;347,362p' "$REF/../owp_extended_kernels.cu"
   cat "$HERE/ref_driver_diffusive.inc"; } > "$TMP/ref_diffusive.cpp"
 # the cut must land on the routines it names (guards against line drift)
@@ -57,15 +62,16 @@ grep -q "static void cpu_linear_mc_fp64(" "$TMP/ref_linear.cpp"
 grep -q "^void cpu_diffusive_tridiag(" "$TMP/ref_diffusive.cpp"
 grep -q "static void compute_mc_coeffs_cpu(" "$TMP/ref_matrix.cpp"
 grep -q "static void build_propagation_matrices(" "$TMP/ref_matrix.cpp"
+grep -q "static void mc_secant_solve_cpu(" "$TMP/ref_matrix_secant.cpp"
 grep -q "^void gen_diff_data(" "$TMP/ref_diffusive.cpp"
 if grep -q "cuda\|cusparse" "$TMP/ref_wavefront.cpp" "$TMP/ref_mcfinal.cpp" "$TMP/ref_linear.cpp" \
-        "$TMP/ref_diffusive.cpp" "$TMP/ref_matrix.cpp"; then
+        "$TMP/ref_diffusive.cpp" "$TMP/ref_matrix.cpp" "$TMP/ref_matrix_secant.cpp"; then
     echo "build_ref: extracted ranges still reference CUDA" >&2; exit 1
 fi
 mkdir -p "$OUT"
 g++ -O2 -ffp-contract=off -fPIC -shared -std=c++17 -w \
     -o "$OUT/libtroute_ref.so" "$TMP/ref_wavefront.cpp" "$TMP/ref_mcfinal.cpp" "$TMP/ref_linear.cpp" \
-    "$TMP/ref_diffusive.cpp" "$TMP/ref_matrix.cpp"
+    "$TMP/ref_diffusive.cpp" "$TMP/ref_matrix.cpp" "$TMP/ref_matrix_secant.cpp"
 echo "build_ref: built $OUT/libtroute_ref.so"
 # the forcing-ingest program compiles whole from its own file (std C++17 only)
 QLAT="$REF/wavefront/data_parsers/parse_qlat_fast_cpp.cpp"

@@ -11,6 +11,7 @@ oracle/build_ref.sh (present only where that build ran).
 from __future__ import annotations
 
 import ctypes as C
+import ctypes.util as ctypes_util
 import os
 
 import numpy as np
@@ -252,6 +253,9 @@ def load_ref():
         ref.ref_matrix_build.argtypes = _MATRIX_BUILD_ARGS
         ref.ref_free.restype = None
         ref.ref_free.argtypes = [C.c_void_p]
+    if hasattr(ref, "ref_matrix_secant_solve"):
+        ref.ref_matrix_secant_solve.restype = None
+        ref.ref_matrix_secant_solve.argtypes = [I, C.c_float] + [pF] * 10
     if hasattr(ref, "ref_cpu_linear_mc_fp64"):
         ref.ref_cpu_linear_mc_fp64.restype = None
         ref.ref_cpu_linear_mc_fp64.argtypes = [pF] * 4 + [pI] * 6 + [I, I, I] + [pF, pF, pD, I]
@@ -365,3 +369,55 @@ def matrix_route(PQ, Pq, qlat32, Q0, n_timesteps, lanes=1):
                            z(arr[3], C.c_int), z(arr[4], C.c_int), z(arr[5], C.c_float),
                            z(q.reshape(-1), C.c_float), _ptr(Q, C.c_float), int(n_timesteps), int(lanes))
     return Q
+
+
+def matrix_secant_inputs(topo, params32, qlat0):
+    """The per-reach secant inputs of linear_mc_matrix_secant.cu main (:358-412)
+    restated in float32: the mean first-step inflow (sequential double sum over
+    all segments), the steady reference flows in level order (upstream sum in
+    CSR order + own inflow + mean), then qup = quc = max(upstream sum, 0.1),
+    qdp = max(reference flow, 0.1), ql = max(own inflow, 0.01) and
+    depthp = max(0.1, (qdp / (tw + 1))^0.6).  Returns (qup, quc, qdp, ql,
+    depthp, reference flows), float32 [n_reaches] each.  Python loops: small
+    networks only."""
+    t = as_topo(topo)
+    f = np.float32
+    q0 = np.ascontiguousarray(qlat0, np.float32)
+    p = np.ascontiguousarray(params32, np.float32).reshape(-1, 8)
+    R, L = int(t["n_reaches"]), int(t["n_levels"])
+    mean = f(np.cumsum(q0.astype(np.float64))[-1] / len(q0))
+    ptr, idx = np.asarray(t["reach_up_ptr"]), np.asarray(t["reach_up_idx"])
+    lp, lr = np.asarray(t["level_ptr"]), np.asarray(t["level_reach"])
+    refq = np.zeros(R, np.float32)
+    upsum = np.zeros(R, np.float32)
+    for lv in range(L):
+        for r in lr[lp[lv]:lp[lv + 1]]:
+            s_ = f(0.0)
+            for k in range(ptr[r], ptr[r + 1]):
+                s_ = f(s_ + refq[idx[k]])
+            upsum[r] = s_
+            refq[r] = f(f(s_ + q0[r]) + mean)
+    qdp = np.fmax(refq, f(0.1))
+    qup = np.fmax(upsum, f(0.1))
+    ql = np.fmax(q0[:R], f(0.01))
+    # the reference's std::pow(float, float) is the C library's powf; numpy's
+    # float32 power rounds differently in about a fifth of cases
+    libm = C.CDLL(ctypes_util.find_library("m"))
+    libm.powf.restype, libm.powf.argtypes = C.c_float, [C.c_float, C.c_float]
+    base = (qdp / (p[:R, 2] + f(1.0))).astype(np.float32)
+    pw = np.array([libm.powf(float(b), 0.6) for b in base], np.float32)
+    depthp = np.fmax(f(0.1), pw)
+    return qup, qup.copy(), qdp, ql, depthp, refq
+
+
+def matrix_secant_solve_reference(inputs, params32, dt=300.0):
+    """The reference's own mc_secant_solve_cpu (oracle/_ref) per reach on the
+    given inputs: (C1, C2, C3, C4/ql)."""
+    qup, quc, qdp, ql, depthp = [np.ascontiguousarray(a, np.float32) for a in inputs[:5]]
+    p = np.ascontiguousarray(params32, np.float32).reshape(-1, 8)
+    n = len(qup)
+    out = [np.zeros(n, np.float32) for _ in range(4)]
+    load_ref().ref_matrix_secant_solve(n, float(dt), *[_ptr(a, C.c_float) for a in (qup, quc, qdp, ql)],
+                                       _ptr(p, C.c_float), _ptr(depthp, C.c_float),
+                                       *[_ptr(o, C.c_float) for o in out])
+    return tuple(out)

@@ -157,6 +157,70 @@ def test_route_rejects_bad_matrices_before_any_device_work():
         trmc.matrix_route(two, Pq, q, Q0, 1)                 # columns of row 0 not ascending
 
 
+def _q0(t, seed):
+    rng = np.random.default_rng(seed)
+    return rng.lognormal(np.log(0.1) - 0.5, 1.0, t.n_segments).astype(np.float32)
+
+
+def _bits(a, b):
+    return all(np.array_equal(x.view(np.uint32), y.view(np.uint32)) for x, y in zip(a, b))
+
+
+def test_secant_coefficients_match_reference_solver():
+    """linear_mc_matrix_secant.cu: the library's coefficients equal the
+    reference's own mc_secant_solve_cpu (oracle/_ref) run on the per-reach
+    inputs of its main, restated in the oracle, bit for bit: a 20k Shreve
+    network and the hydrofabric network."""
+    if orc.load_ref() is None or not hasattr(orc.load_ref(), "ref_matrix_secant_solve"):
+        pytest.skip("reference build (oracle/_ref) not present")
+    t, p, _, _ = _net(20000, seed=3)
+    q0 = _q0(t, 3)
+    lib = trmc.matrix_secant_coeffs(p, t, q0)
+    assert _bits(lib, orc.matrix_secant_solve_reference(orc.matrix_secant_inputs(t, p, q0), p))
+    topo, params = _hydrofabric_net()
+    q0 = _q0(topo, 4)
+    lib = trmc.matrix_secant_coeffs(params, topo, q0)
+    assert _bits(lib, orc.matrix_secant_solve_reference(orc.matrix_secant_inputs(topo, params, q0), params))
+
+
+def test_secant_reference_flows_are_subtree_sums():
+    """The steady reference flow of a reach is the sum, over the reach and
+    every reach upstream of it, of its first-step inflow plus the mean inflow
+    (float sums in the reference, double here)."""
+    t, p, _, _ = _net(5000, seed=5)
+    q0 = _q0(t, 5)
+    refq = orc.matrix_secant_inputs(t, p, q0)[5]
+    mean = q0.astype(np.float64).mean()
+    want = q0.astype(np.float64)[:t.n_reaches] + mean
+    for r in range(t.n_reaches):                       # reaches are numbered upstream first
+        for u in t.reach_up_idx[t.reach_up_ptr[r]:t.reach_up_ptr[r + 1]]:
+            want[r] += want[u]
+    np.testing.assert_allclose(refq, want, rtol=2e-5)
+
+
+def test_secant_falls_back_to_half_bankfull_coefficients():
+    """Invalid channels (zero bottom width) take the X = 0.25 half-bankfull
+    coefficients: those of the reference's compute_mc_coeffs_cpu."""
+    t, p, _, _ = _net(3000, seed=6)
+    p = p.copy()
+    bad = np.arange(0, t.n_reaches, 7)
+    p[bad, 1] = 0.0
+    lib = trmc.matrix_secant_coeffs(p, t, _q0(t, 6))
+    want = orc.matrix_coeffs(p, reference=orc.load_ref() is not None and hasattr(orc.load_ref(), "ref_matrix_coeffs"))
+    for g, w in zip(lib, want):
+        assert np.array_equal(g[bad].view(np.uint32), w[bad].view(np.uint32)) or \
+            np.allclose(g[bad], w[bad], rtol=4e-6)
+
+
+def test_secant_rejects_bad_levels():
+    import dataclasses
+    t, p, _, _ = _net(200, seed=7)
+    lr = np.array(t.level_reach, np.int32).copy()
+    lr[0] = lr[1]                                       # a reach listed twice
+    with pytest.raises(trmc.TrmcError):
+        trmc.matrix_secant_coeffs(p, dataclasses.replace(t, level_reach=lr), _q0(t, 7))
+
+
 # ------------------------------------------------------------------ GPU parity
 @pytest.mark.gpu
 def test_gpu_coefficients_match_oracle(gpu):
@@ -210,3 +274,15 @@ def test_gpu_edge_cases(gpu):
     Q0 = np.full(topo.n_reaches, 0.5, np.float32)
     got = trmc.matrix_route(PQ, Pq, q, Q0, 24, lanes_per_row=16)
     assert np.array_equal(got.view(np.uint32), orc.matrix_route(PQ, Pq, q, Q0, 24, lanes=16).view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_gpu_secant_fitted_matrices_route_bitwise(gpu):
+    """The secant-fitted coefficients through the same builder and step
+    kernel (tol 1e-7, the secant variant's default): every reach equals the
+    oracle's steps bit for bit."""
+    t, p, qlat, Q0 = _net(20000, seed=10, T=24)
+    c = trmc.matrix_secant_coeffs(p, t, qlat[0])
+    PQ, Pq = trmc.matrix_build(t.reach_up_ptr, t.reach_up_idx, c, 1e-7)
+    got = trmc.matrix_route(PQ, Pq, qlat, Q0, 24)
+    assert np.array_equal(got.view(np.uint32), orc.matrix_route(PQ, Pq, qlat, Q0, 24, lanes=0).view(np.uint32))
