@@ -460,9 +460,14 @@ extern "C" int trmc_matrix_secant_coeffs_f32(const trmc_chparams_f32 *params, in
         }
     }
     for (int r = 0; r < n_reaches; ++r)
+        if (lev[r] < 0) return trmc::fail(TRMC_ERR_TOPOLOGY, "reach %d is in no level", r);
+    if (up_ptr[0] != 0) return trmc::fail(TRMC_ERR_TOPOLOGY, "reach_up_ptr[0] = %d", up_ptr[0]);
+    for (int r = 0; r < n_reaches; ++r) {
+        if (up_ptr[r + 1] < up_ptr[r]) return trmc::fail(TRMC_ERR_TOPOLOGY, "reach_up_ptr decreases at %d", r);
         for (int k = up_ptr[r]; k < up_ptr[r + 1]; ++k)
             if (!up_idx || up_idx[k] < 0 || up_idx[k] >= n_reaches || lev[up_idx[k]] >= lev[r])
                 return trmc::fail(TRMC_ERR_TOPOLOGY, "reach %d: upstream not in an earlier level", r);
+    }
     // mean first-step lateral inflow over all segments, in double
     double mean = 0.0;
     for (int i = 0; i < n_segments; ++i) mean += qlat0[i];
