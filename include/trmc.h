@@ -297,6 +297,24 @@ int trmc_ingest_nex_csv_to_forcings(const char *csv_dir, const char *wbid_npy_pa
                                     const char *out_path, int n_timesteps, int n_threads,
                                     int64_t *counts);
 
+/* The same pivot under the parsing rules of the reference's Rust twin
+ * (rust_preprocessor/src/main.rs:28-91, 116-273) or of the C++ program:
+ *  TRMC_INGEST_CPP   parse_qlat_fast_cpp.cpp (atoi / atof, the value rounded
+ *                    from double to float);
+ *  TRMC_INGEST_RUST  names "nex-<i64>_output.csv"; a row's timestep is its
+ *                    leading digits, its value the text after the last comma
+ *                    in Rust's float grammar, correctly rounded to float; a
+ *                    row whose value does not read is skipped.
+ * A reach named by two files takes the later one in name order (the Rust
+ * program's order is the directory's). */
+#define TRMC_INGEST_CPP  0
+#define TRMC_INGEST_RUST 1
+int trmc_ingest_nex_csv_rules(const char *csv_dir, const int64_t *topo_to_wbid, int n_reaches,
+                              int n_timesteps, int n_threads, int rules, float *qlat, int64_t *counts);
+int trmc_ingest_nex_csv_to_forcings_rules(const char *csv_dir, const char *wbid_npy_path,
+                                          const char *out_path, int n_timesteps, int n_threads,
+                                          int rules, int64_t *counts);
+
 /* mc_solve_d (wavefront_mc.cu:215-220) over n independent segment inputs on
  * the GPU.  in = [n][5] {qup, quc, qdp, ql, depthp}; out = [n][3]
  * {qdc, depth, velocity}; iters (may be NULL) = [n] secant iterations.

@@ -168,6 +168,8 @@ def _declare(L: C.CDLL) -> None:
         "trmc_diffusive_tridiag_f32": (I, [V] + [pF] * 5 + [I]),
         "trmc_ingest_nex_csv": (I, [C.c_char_p, C.POINTER(C.c_int64), I, I, I, pF, C.POINTER(C.c_int64)]),
         "trmc_ingest_nex_csv_to_forcings": (I, [C.c_char_p] * 3 + [I, I, C.POINTER(C.c_int64)]),
+        "trmc_ingest_nex_csv_rules": (I, [C.c_char_p, C.POINTER(C.c_int64), I, I, I, I, pF, C.POINTER(C.c_int64)]),
+        "trmc_ingest_nex_csv_to_forcings_rules": (I, [C.c_char_p] * 3 + [I, I, I, C.POINTER(C.c_int64)]),
         "trmc_diffusive_tridiag_timed_f32": (I, [V] + [pF] * 5 + [I, I, pD]),
         "trmc_network_generate_forcing": (I, [V, I, C.c_uint64]),
         "trmc_stream_copy": (I, [I, C.c_size_t, I, pD]),
@@ -528,25 +530,30 @@ def linear_mc(coeffs, topo: Topology, qlat_ts, Q0, n_timesteps: int):
     return out
 
 
-def ingest_nex_csv(csv_dir: str, topo_to_wbid, n_timesteps: int, n_threads: int = 0):
-    """Lateral inflow [T, R] float32 from per-nexus CSVs (parse_qlat_fast_cpp.cpp);
+INGEST_CPP, INGEST_RUST = 0, 1
+
+
+def ingest_nex_csv(csv_dir: str, topo_to_wbid, n_timesteps: int, n_threads: int = 0, rules: int = INGEST_CPP):
+    """Lateral inflow [T, R] float32 from per-nexus CSVs, under the parsing rules
+    of parse_qlat_fast_cpp.cpp (INGEST_CPP) or rust_preprocessor (INGEST_RUST);
     returns (qlat, files read, files skipped)."""
     w = np.ascontiguousarray(topo_to_wbid, np.int64)
     q = np.zeros((int(n_timesteps), len(w)), np.float32)
     cnt = np.zeros(2, np.int64)
-    _check(lib().trmc_ingest_nex_csv(os.fsencode(csv_dir), w.ctypes.data_as(C.POINTER(C.c_int64)), len(w),
-                                     int(n_timesteps), int(n_threads), _ptr(q, C.c_float),
-                                     cnt.ctypes.data_as(C.POINTER(C.c_int64))))
+    _check(lib().trmc_ingest_nex_csv_rules(os.fsencode(csv_dir), w.ctypes.data_as(C.POINTER(C.c_int64)), len(w),
+                                           int(n_timesteps), int(n_threads), int(rules), _ptr(q, C.c_float),
+                                           cnt.ctypes.data_as(C.POINTER(C.c_int64))))
     return q, int(cnt[0]), int(cnt[1])
 
 
 def ingest_nex_csv_to_forcings(csv_dir: str, wbid_npy: str, out_path: str, n_timesteps: int = 24,
-                               n_threads: int = 0):
-    """The parse_qlat_fast program's path: writes forcings.bin; returns (files read, skipped)."""
+                               n_threads: int = 0, rules: int = INGEST_CPP):
+    """The parse_qlat_fast program's path (or the Rust twin's, INGEST_RUST): writes
+    forcings.bin; returns (files read, skipped)."""
     cnt = np.zeros(2, np.int64)
-    _check(lib().trmc_ingest_nex_csv_to_forcings(os.fsencode(csv_dir), os.fsencode(wbid_npy), os.fsencode(out_path),
-                                                 int(n_timesteps), int(n_threads),
-                                                 cnt.ctypes.data_as(C.POINTER(C.c_int64))))
+    _check(lib().trmc_ingest_nex_csv_to_forcings_rules(os.fsencode(csv_dir), os.fsencode(wbid_npy),
+                                                       os.fsencode(out_path), int(n_timesteps), int(n_threads),
+                                                       int(rules), cnt.ctypes.data_as(C.POINTER(C.c_int64))))
     return int(cnt[0]), int(cnt[1])
 
 

@@ -1,8 +1,9 @@
 // Forcing ingest: per-nexus lateral-inflow CSVs (nex-<id>_output.csv, rows
 // "timestep, time, value") pivoted into the [timestep][reach] float tensor
-// of forcings.bin, after data_parsers/parse_qlat_fast_cpp.cpp:76-224.  Host
-// C++: files are independent, so threads take contiguous chunks of the file
-// list and write disjoint reach columns.
+// of forcings.bin, after data_parsers/parse_qlat_fast_cpp.cpp:76-224 or, with
+// TRMC_INGEST_RUST, the parsing rules of rust_preprocessor/src/main.rs:28-91,
+// 116-273.  Host C++: files are independent, so threads take contiguous chunks
+// of the file list and write disjoint reach columns.
 #include <algorithm>
 #include <atomic>
 #include <cerrno>
@@ -60,6 +61,101 @@ bool parse_row(const char *line, size_t len, int &ts, float &val)
     return true;
 }
 
+// Rust rules, file name: "nex-" + a whole signed 64-bit integer up to the
+// first '_' (or '.'); -1 marks no id (strip_wbid_from_filename, main.rs:28-33)
+bool rust_nexus_id(const std::string &name, long long &id)
+{
+    if (name.rfind("nex-", 0) != 0) return false;
+    const std::string rest = name.substr(4);
+    size_t end = rest.find('_');
+    if (end == std::string::npos) end = rest.find('.');
+    if (end == std::string::npos) return false;
+    const std::string num = rest.substr(0, end);
+    size_t k = 0;
+    if (k < num.size() && (num[k] == '+' || num[k] == '-')) ++k;
+    if (k == num.size()) return false;
+    for (size_t j = k; j < num.size(); ++j)
+        if (num[j] < '0' || num[j] > '9') return false;
+    errno = 0;
+    const long long v = std::strtoll(num.c_str(), nullptr, 10);
+    if (errno == ERANGE) return false;
+    id = v;
+    return true;
+}
+
+// Rust's f32 text grammar: [+-] then inf | infinity | nan (any case), or
+// digits with an optional '.' and digits (at least one digit in all) and an
+// optional exponent [eE][+-]digits; the value is the correctly rounded float
+// (strtof on the accepted text)
+bool rust_f32(const char *s, size_t n, float &out)
+{
+    size_t k = 0;
+    if (k < n && (s[k] == '+' || s[k] == '-')) ++k;
+    auto word = [&](const char *w) {
+        const size_t m = std::strlen(w);
+        if (n - k != m) return false;
+        for (size_t j = 0; j < m; ++j)
+            if ((s[k + j] | 0x20) != w[j]) return false;
+        return true;
+    };
+    if (!(word("inf") || word("infinity") || word("nan"))) {
+        size_t digits = 0;
+        while (k < n && s[k] >= '0' && s[k] <= '9') { ++k; ++digits; }
+        if (k < n && s[k] == '.') {
+            ++k;
+            while (k < n && s[k] >= '0' && s[k] <= '9') { ++k; ++digits; }
+        }
+        if (digits == 0) return false;
+        if (k < n && (s[k] == 'e' || s[k] == 'E')) {
+            ++k;
+            if (k < n && (s[k] == '+' || s[k] == '-')) ++k;
+            size_t ed = 0;
+            while (k < n && s[k] >= '0' && s[k] <= '9') { ++k; ++ed; }
+            if (ed == 0) return false;
+        }
+        if (k != n) return false;
+    }
+    char tmp[128];
+    if (n >= sizeof(tmp)) {
+        std::string big(s, n);
+        out = std::strtof(big.c_str(), nullptr);
+    } else {
+        std::memcpy(tmp, s, n);
+        tmp[n] = 0;
+        out = std::strtof(tmp, nullptr);
+    }
+    return true;
+}
+
+// Rust rules, one row: spaces, the timestep's digits (no sign; none or too
+// many for the integer type skip the row), then the text after the last comma
+// with spaces skipped and ASCII whitespace trimmed, which must read as a float
+// (parse_csv_bytes, main.rs:37-91)
+bool parse_row_rust(const char *line, size_t len, unsigned long long &ts, float &val)
+{
+    size_t k = 0;
+    while (k < len && line[k] == ' ') ++k;
+    const size_t t0 = k;
+    unsigned long long v = 0;
+    for (; k < len && line[k] >= '0' && line[k] <= '9'; ++k) {
+        const unsigned d = (unsigned)(line[k] - '0');
+        if (v > (UINT64_MAX - d) / 10) return false;    // beyond a 64-bit usize
+        v = v * 10 + d;
+    }
+    if (k == t0) return false;
+    ts = v;
+    size_t c = len;
+    while (c > 0 && line[c - 1] != ',') --c;
+    if (c == 0) return false;                      // no comma
+    size_t b = c;                                  // first byte after the comma
+    while (b < len && line[b] == ' ') ++b;
+    auto ws = [](char x) { return x == ' ' || x == '\t' || x == '\n' || x == '\v' || x == '\f' || x == '\r'; };
+    size_t e = len;
+    while (e > b && ws(line[e - 1])) --e;
+    while (b < e && ws(line[b])) ++b;
+    return rust_f32(line + b, e - b, val);
+}
+
 }  // namespace
 
 extern "C" {
@@ -67,6 +163,14 @@ extern "C" {
 int trmc_ingest_nex_csv(const char *csv_dir, const int64_t *topo_to_wbid, int n_reaches, int n_timesteps,
                         int n_threads, float *qlat, int64_t *counts)
 {
+    return trmc_ingest_nex_csv_rules(csv_dir, topo_to_wbid, n_reaches, n_timesteps, n_threads, TRMC_INGEST_CPP,
+                                     qlat, counts);
+}
+
+int trmc_ingest_nex_csv_rules(const char *csv_dir, const int64_t *topo_to_wbid, int n_reaches, int n_timesteps,
+                              int n_threads, int rules, float *qlat, int64_t *counts)
+{
+    if (rules != TRMC_INGEST_CPP && rules != TRMC_INGEST_RUST) return trmc::fail(TRMC_ERR_ARG, "unknown rules %d", rules);
     if (!csv_dir || (!topo_to_wbid && n_reaches > 0) || (!qlat && n_reaches > 0 && n_timesteps > 0))
         return trmc::fail(TRMC_ERR_ARG, "null argument");
     if (n_reaches < 0 || n_timesteps < 0) return trmc::fail(TRMC_ERR_ARG, "negative size");
@@ -78,7 +182,12 @@ int trmc_ingest_nex_csv(const char *csv_dir, const int64_t *topo_to_wbid, int n_
     for (fs::directory_iterator it(csv_dir, ec), end; !ec && it != end; it.increment(ec)) {
         if (!it->is_regular_file(ec)) continue;
         const std::string name = it->path().filename().string();
-        if (name.rfind("nex-", 0) != 0 || name.find("_output.csv") == std::string::npos) continue;
+        if (name.rfind("nex-", 0) != 0) continue;
+        if (rules == TRMC_INGEST_RUST) {
+            if (name.size() < 15 || name.compare(name.size() - 11, 11, "_output.csv") != 0) continue;
+        } else if (name.find("_output.csv") == std::string::npos) {
+            continue;
+        }
         paths.push_back(it->path().string());
     }
     if (ec) return trmc::fail(TRMC_ERR_IO, "cannot list %s: %s", csv_dir, ec.message().c_str());
@@ -93,8 +202,14 @@ int trmc_ingest_nex_csv(const char *csv_dir, const int64_t *topo_to_wbid, int n_
     std::vector<int> owner(R, -1);
     std::vector<int> reach_of(paths.size(), -1);
     for (size_t pi = 0; pi < paths.size(); ++pi) {
-        const long long id = nexus_id(fs::path(paths[pi]).filename().string());
-        if (id < 0) continue;
+        const std::string name = fs::path(paths[pi]).filename().string();
+        long long id = -1;
+        if (rules == TRMC_INGEST_RUST) {
+            if (!rust_nexus_id(name, id)) continue;
+        } else {
+            id = nexus_id(name);
+            if (id < 0) continue;
+        }
         auto f = topo_of.find((int64_t)id);
         if (f == topo_of.end()) continue;
         reach_of[pi] = f->second;
@@ -124,10 +239,16 @@ int trmc_ingest_nex_csv(const char *csv_dir, const int64_t *topo_to_wbid, int n_
                 const size_t ll = i - ls;
                 if (i < got) ++i;
                 if (ll < 5) continue;
-                int ts;
                 float val;
-                if (!parse_row(buf.data() + ls, ll, ts, val)) continue;
-                if (ts >= 0 && ts < n_timesteps) qlat[(size_t)ts * R + r] = val;
+                if (rules == TRMC_INGEST_RUST) {
+                    unsigned long long ts;
+                    if (!parse_row_rust(buf.data() + ls, ll, ts, val)) continue;
+                    if (ts < (unsigned long long)n_timesteps) qlat[(size_t)ts * R + r] = val;
+                } else {
+                    int ts;
+                    if (!parse_row(buf.data() + ls, ll, ts, val)) continue;
+                    if (ts >= 0 && ts < n_timesteps) qlat[(size_t)ts * R + r] = val;
+                }
             }
         }
     };
@@ -163,6 +284,13 @@ int trmc_ingest_nex_csv(const char *csv_dir, const int64_t *topo_to_wbid, int n_
 int trmc_ingest_nex_csv_to_forcings(const char *csv_dir, const char *wbid_npy_path, const char *out_path,
                                     int n_timesteps, int n_threads, int64_t *counts)
 {
+    return trmc_ingest_nex_csv_to_forcings_rules(csv_dir, wbid_npy_path, out_path, n_timesteps, n_threads,
+                                                 TRMC_INGEST_CPP, counts);
+}
+
+int trmc_ingest_nex_csv_to_forcings_rules(const char *csv_dir, const char *wbid_npy_path, const char *out_path,
+                                          int n_timesteps, int n_threads, int rules, int64_t *counts)
+{
     if (!csv_dir || !wbid_npy_path || !out_path) return trmc::fail(TRMC_ERR_ARG, "null path");
     // topo_to_wbid.npy: int64 array after the .npy header (magic 6, version 2,
     // little-endian header length 2, header) (parse_qlat_fast_cpp.cpp:90-110)
@@ -183,7 +311,8 @@ int trmc_ingest_nex_csv_to_forcings(const char *csv_dir, const char *wbid_npy_pa
     std::fclose(f);
     if (got != R) return trmc::fail(TRMC_ERR_IO, "short read in %s", wbid_npy_path);
     std::vector<float> qlat((size_t)std::max(n_timesteps, 0) * R);
-    int rc = trmc_ingest_nex_csv(csv_dir, wbid.data(), (int)R, n_timesteps, n_threads, qlat.data(), counts);
+    int rc = trmc_ingest_nex_csv_rules(csv_dir, wbid.data(), (int)R, n_timesteps, n_threads, rules, qlat.data(),
+                                       counts);
     if (rc) return rc;
     // forcings.bin: int32 T, qlat [T][R], qup0 [R] = 0, qdp0 [R] = 0, dp0 [R] = 0.5
     FILE *o = std::fopen(out_path, "wb");

@@ -89,3 +89,61 @@ def test_ingest_result_does_not_depend_on_threads(tmp_path):
 def test_ingest_missing_directory_is_an_error(tmp_path):
     with pytest.raises(trmc.TrmcError):
         trmc.ingest_nex_csv(str(tmp_path / "absent"), np.arange(3), 4)
+
+
+# ------------------------------------------------------ the Rust twin's rules
+def test_rust_rules_match_the_cpp_program_on_well_formed_files(tmp_path):
+    """On files every row of which both programs read (values printed from
+    floats, names nex-<id>_output.csv), the two rule sets give the same
+    forcings.bin bytes, and the C++ rules are the reference program's."""
+    d = str(tmp_path)
+    rng = np.random.default_rng(4)
+    wbid = rng.choice(np.arange(1000, 10_000_000), 120, replace=False).astype(np.int64)
+    np.save(os.path.join(d, "topo_to_wbid.npy"), wbid)
+    for r in range(120):
+        vals = rng.lognormal(np.log(0.1), 1.0, 24).astype(np.float32)
+        with open(os.path.join(d, f"nex-{int(wbid[r])}_output.csv"), "w") as f:
+            f.write("".join(f"{t}, 2017-01-01 {t:02d}:00:00, {float(v):.9g}\n" for t, v in enumerate(vals)))
+    a, b = os.path.join(d, "cpp.bin"), os.path.join(d, "rust.bin")
+    trmc.ingest_nex_csv_to_forcings(d, os.path.join(d, "topo_to_wbid.npy"), a, 24)
+    trmc.ingest_nex_csv_to_forcings(d, os.path.join(d, "topo_to_wbid.npy"), b, 24, rules=trmc.INGEST_RUST)
+    with open(a, "rb") as fa, open(b, "rb") as fb:
+        assert fa.read() == fb.read()
+
+
+def test_rust_rules_row_and_name_cases(tmp_path):
+    """main.rs:28-91: the timestep is the leading digits (spaces first, no
+    sign); the value after the last comma must read in Rust's float grammar
+    (sign, digits with an optional point and exponent, inf / infinity / nan)
+    and is correctly rounded to float; a row that does not read is skipped
+    (the C++ program's atof would store 0 or a prefix); names must end in
+    _output.csv and hold a whole integer."""
+    d = str(tmp_path)
+    wbid = np.array([11, 22, 33, -4, 55], np.int64)
+    rows = {
+        11: ["0, t, 1.5", "  1, t, +2.25e1", "2, t, .5", "3, t, 7.", "4, t, abc", "5, t, 0x1p3",
+             "6, t, 1e", "7, t,   -inf  ", "8, t, 0.1\r", "-9, t, 3.0", "10 t, 4.0", "11, t"],
+        22: ["0, t, 1.0", "1, t, NaN", "2, t, 2.0 extra", "3,t,3.0", "00000000000000000004, t, 4.0"],
+    }
+    for wid, rs in rows.items():
+        with open(os.path.join(d, f"nex-{wid}_output.csv"), "w") as f:
+            f.write("\n".join(rs) + "\n")
+    for name in ("nex-33_output.csv.bak", "nex-3x_output.csv", "nex--4_output.csv", "nex-+55.csv"):
+        with open(os.path.join(d, name), "w") as f:
+            f.write("0, t, 9.0\n")
+    q, read, skipped = trmc.ingest_nex_csv(d, wbid, 12, rules=trmc.INGEST_RUST)
+    f = np.float32
+    want11 = {0: 1.5, 1: 22.5, 2: 0.5, 3: 7.0, 7: -np.inf, 8: f(0.1), 10: 4.0}
+    for t in range(12):
+        if t in want11:
+            assert q[t, 0] == f(want11[t]), t
+        else:
+            assert q[t, 0] == 0.0, t                      # rows 4, 5, 6, 9, 11 skipped
+    assert q[0, 1] == 1.0 and np.isnan(q[1, 1]) and q[2, 1] == 0.0 and q[3, 1] == 3.0 and q[4, 1] == 4.0
+    assert q[0, 3] == 9.0                                 # nex--4: a whole signed integer
+    assert np.all(q[:, 2] == 0) and np.all(q[:, 4] == 0)  # .bak and "nex-+55.csv" are not read
+    # files read: 11, 22, -4; skipped: "nex-3x" (no number)
+    assert read == 3 and skipped == 1
+    # the C++ rules read "4, t, abc" as 0 and "2, t, 2.0 extra" as 2
+    qc = trmc.ingest_nex_csv(d, wbid, 12)[0]
+    assert qc[4, 0] == 0.0 and qc[2, 1] == 2.0
