@@ -177,50 +177,81 @@ int trmc_ingest_nex_csv_rules(const char *csv_dir, const int64_t *topo_to_wbid, 
     std::unordered_map<int64_t, int> topo_of;
     topo_of.reserve((size_t)n_reaches * 2);
     for (int i = 0; i < n_reaches; ++i) topo_of[topo_to_wbid[i]] = i;   // a repeated id keeps its last reach
-    std::vector<std::string> paths;
+    // file names only; paths are joined in the workers
+    std::vector<std::string> names;
     std::error_code ec;
     for (fs::directory_iterator it(csv_dir, ec), end; !ec && it != end; it.increment(ec)) {
         if (!it->is_regular_file(ec)) continue;
-        const std::string name = it->path().filename().string();
+        std::string name = it->path().filename().string();
         if (name.rfind("nex-", 0) != 0) continue;
         if (rules == TRMC_INGEST_RUST) {
             if (name.size() < 15 || name.compare(name.size() - 11, 11, "_output.csv") != 0) continue;
         } else if (name.find("_output.csv") == std::string::npos) {
             continue;
         }
-        paths.push_back(it->path().string());
+        names.push_back(std::move(name));
     }
     if (ec) return trmc::fail(TRMC_ERR_IO, "cannot list %s: %s", csv_dir, ec.message().c_str());
-    // a fixed order, so a reach named by two files gets the same one every run
-    std::sort(paths.begin(), paths.end());
+    const std::string dir = std::string(csv_dir) + "/";
     const size_t R = (size_t)n_reaches;
     std::fill(qlat, qlat + (size_t)n_timesteps * R, 0.0f);
     if (n_threads <= 0) n_threads = (int)std::max(1u, std::thread::hardware_concurrency());
     std::atomic<long long> matched{0}, missing{0};
-    const size_t per = (paths.size() + n_threads - 1) / std::max(1, n_threads);
-    // one owner per reach column: the last file (in sorted order) naming it
-    std::vector<int> owner(R, -1);
-    std::vector<int> reach_of(paths.size(), -1);
-    for (size_t pi = 0; pi < paths.size(); ++pi) {
-        const std::string name = fs::path(paths[pi]).filename().string();
-        long long id = -1;
-        if (rules == TRMC_INGEST_RUST) {
-            if (!rust_nexus_id(name, id)) continue;
-        } else {
-            id = nexus_id(name);
-            if (id < 0) continue;
+    const size_t per = (names.size() + n_threads - 1) / std::max(1, n_threads);
+    // a worker that cannot allocate reports it; a range whose thread cannot
+    // start is run on this thread
+    std::atomic<bool> failed{false};
+    auto run_ranges = [&](auto &&fn) {
+        auto safe = [&](size_t b, size_t e) {
+            try {
+                fn(b, e);
+            } catch (...) {
+                failed.store(true);
+            }
+        };
+        std::vector<std::thread> th;
+        for (int t = 0; t < n_threads; ++t) {
+            const size_t b = (size_t)t * per, e = std::min(names.size(), b + per);
+            if (b >= e) break;
+            try {
+                th.emplace_back(safe, b, e);
+            } catch (...) {
+                safe(b, e);
+            }
         }
-        auto f = topo_of.find((int64_t)id);
-        if (f == topo_of.end()) continue;
-        reach_of[pi] = f->second;
-        owner[f->second] = (int)pi;
+        for (auto &t : th) t.join();
+    };
+    // each file's reach (-1: no number or unknown id), in parallel
+    std::vector<int> reach_of(names.size(), -1);
+    run_ranges([&](size_t b, size_t e) {
+        for (size_t pi = b; pi < e; ++pi) {
+            long long id = -1;
+            if (rules == TRMC_INGEST_RUST) {
+                if (!rust_nexus_id(names[pi], id)) continue;
+            } else {
+                id = nexus_id(names[pi]);
+                if (id < 0) continue;
+            }
+            auto f = topo_of.find((int64_t)id);
+            if (f != topo_of.end()) reach_of[pi] = f->second;
+        }
+    });
+    if (failed.load()) return trmc::fail(TRMC_ERR_NOMEM, "out of host memory reading the forcing files");
+    // one owner per reach column: of the files naming it, the last in name
+    // order (a fixed choice, whatever order the directory lists them in)
+    std::vector<int> owner(R, -1);
+    for (size_t pi = 0; pi < names.size(); ++pi) {
+        const int r = reach_of[pi];
+        if (r >= 0 && (owner[r] < 0 || names[pi] > names[owner[r]])) owner[r] = (int)pi;
     }
     auto work = [&](size_t b, size_t e) {
         std::vector<char> buf(4096);
+        std::string path;
         for (size_t pi = b; pi < e; ++pi) {
             const int r = reach_of[pi];
             if (r < 0) { missing.fetch_add(1); continue; }
-            FILE *fp = std::fopen(paths[pi].c_str(), "rb");
+            path.assign(dir).append(names[pi]);
+            FILE *fp = std::fopen(path.c_str(), "rb");
             if (!fp) { missing.fetch_add(1); continue; }
             std::fseek(fp, 0, SEEK_END);
             const long size = std::ftell(fp);
@@ -252,27 +283,7 @@ int trmc_ingest_nex_csv_rules(const char *csv_dir, const int64_t *topo_to_wbid, 
             }
         }
     };
-    // a worker that cannot allocate reports it; a range whose thread cannot
-    // start is read on this thread
-    std::atomic<bool> failed{false};
-    auto work_safe = [&](size_t b, size_t e) {
-        try {
-            work(b, e);
-        } catch (...) {
-            failed.store(true);
-        }
-    };
-    std::vector<std::thread> th;
-    for (int t = 0; t < n_threads; ++t) {
-        const size_t b = (size_t)t * per, e = std::min(paths.size(), b + per);
-        if (b >= e) break;
-        try {
-            th.emplace_back(work_safe, b, e);
-        } catch (...) {
-            work_safe(b, e);
-        }
-    }
-    for (auto &t : th) t.join();
+    run_ranges(work);
     if (failed.load()) return trmc::fail(TRMC_ERR_NOMEM, "out of host memory reading the forcing files");
     if (counts) {
         counts[0] = matched.load();
