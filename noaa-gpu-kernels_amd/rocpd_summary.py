@@ -5,7 +5,7 @@ schema); its --stats CSV is not always written (for instance when the tool
 itself exits abnormally after the program finished).  This prints the same
 per-kernel statistics from the database:
 
-    python noaa-gpu-kernels_amd/rocpd_summary.py gpurun_out/prof/run_results.db [--dispatches NAME]
+    python noaa-gpu-kernels_amd/rocpd_summary.py prof/run_results.db [--dispatches NAME]
 
 --dispatches NAME also lists every dispatch of the kernels whose name contains
 NAME in launch order (one template instance can serve several modes of a run).
