@@ -147,3 +147,9 @@ def test_rust_rules_row_and_name_cases(tmp_path):
     # the C++ rules read "4, t, abc" as 0 and "2, t, 2.0 extra" as 2
     qc = trmc.ingest_nex_csv(d, wbid, 12)[0]
     assert qc[4, 0] == 0.0 and qc[2, 1] == 2.0
+
+
+def test_unknown_rules_are_an_error(tmp_path):
+    with pytest.raises(trmc.TrmcError) as e:
+        trmc.ingest_nex_csv(str(tmp_path), np.arange(3), 4, rules=7)
+    assert e.value.code == trmc.ERR_ARG

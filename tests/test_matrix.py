@@ -212,6 +212,13 @@ def test_secant_falls_back_to_half_bankfull_coefficients():
             np.allclose(g[bad], w[bad], rtol=4e-6)
 
 
+def test_secant_needs_an_inflow_per_reach():
+    t, p, _, _ = _net(200, seed=8)
+    with pytest.raises(trmc.TrmcError) as e:
+        trmc.matrix_secant_coeffs(p, t, np.zeros(t.n_reaches - 1, np.float32))
+    assert e.value.code == trmc.ERR_ARG
+
+
 def test_secant_rejects_bad_levels():
     import dataclasses
     t, p, _, _ = _net(200, seed=7)
