@@ -1071,28 +1071,34 @@ __device__ __forceinline__ void matrix_rows(int slot0, int nslots, const int4 *_
     if (live && l == 0) z_out[row] = make_float2(s, ql_next[row]);
 }
 
-__global__ void __launch_bounds__(256) k_matrix_step(const int4 *__restrict__ blocks, const int4 *__restrict__ slots,
-                                                     const int *__restrict__ col, const float2 *__restrict__ val,
-                                                     const float2 *__restrict__ z, const float *__restrict__ ql_next,
-                                                     float2 *__restrict__ z_out)
+// The grid loops over the block table (at most 2^20 blocks launched), so the
+// work-item count of one dispatch stays within 32 bits for any table size.
+__global__ void __launch_bounds__(256) k_matrix_step(int n_blocks, const int4 *__restrict__ blocks,
+                                                     const int4 *__restrict__ slots, const int *__restrict__ col,
+                                                     const float2 *__restrict__ val, const float2 *__restrict__ z,
+                                                     const float *__restrict__ ql_next, float2 *__restrict__ z_out)
 {
-    const int4 bt = blocks[blockIdx.x];   // {G, first slot, slots, 0}
-    switch (bt.x) {
-    case 1: matrix_rows<1>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
-    case 2: matrix_rows<2>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
-    case 4: matrix_rows<4>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
-    case 8: matrix_rows<8>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
-    case 16: matrix_rows<16>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
-    case 32: matrix_rows<32>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
-    default: matrix_rows<64>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
+    for (int b = blockIdx.x; b < n_blocks; b += gridDim.x) {
+        const int4 bt = blocks[b];   // {G, first slot, slots, 0}
+        switch (bt.x) {
+        case 1: matrix_rows<1>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
+        case 2: matrix_rows<2>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
+        case 4: matrix_rows<4>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
+        case 8: matrix_rows<8>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
+        case 16: matrix_rows<16>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
+        case 32: matrix_rows<32>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
+        default: matrix_rows<64>(bt.y, bt.z, slots, col, val, z, ql_next, z_out); break;
+        }
     }
 }
 
 hipError_t launch_matrix_step(int n_blocks, const int4 *blocks, const int4 *slots, const int *col, const float2 *val,
                               const float2 *z, const float *ql_next, float2 *z_out, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_matrix_step, dim3(n_blocks), dim3(256), 0, stream, blocks, slots, col, val, z, ql_next,
-                       z_out);
+    if (n_blocks <= 0) return hipSuccess;
+    const int grid = std::min(n_blocks, 1 << 20);
+    hipLaunchKernelGGL(k_matrix_step, dim3(grid), dim3(256), 0, stream, n_blocks, blocks, slots, col, val, z,
+                       ql_next, z_out);
     return hipGetLastError();
 }
 
