@@ -456,7 +456,7 @@ def bench_matrix(trmc, spec, args):
                                + ("secant-fitted coefficients" if secant else "half-bankfull coefficients (X=0.25)"),
                    "reaches": n, "levels": t.n_levels, "timesteps": T,
                    "tol": tol, "nnz_P_Q": nnz[0], "nnz_P_q": nnz[1], "union_entries": union,
-                   "lanes_per_row": "by union row length U (smallest power of two G with 4 G >= U, <= 64)",
+                   "lanes_per_row": "by union row length U (smallest power of two G with 2 G >= U, <= 64)",
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic(args.workload),

@@ -250,7 +250,7 @@ void trmc_free_csr(trmc_csr *m);
  * each as s = s + vQ * Q[c] then s = s + vq * ql[c]; the lanes' sums are
  * added pairwise by xor distance G/2, ..., 1.  lanes_per_row = G for every
  * row (1, 2, 4, ..., 64), or 0: each row's G from its union length U, the
- * smallest power of two with 4 G >= U, at most 64. */
+ * smallest power of two with 2 G >= U, at most 64. */
 int trmc_matrix_route_f32(const trmc_csr *P_Q, const trmc_csr *P_q, const float *qlat,
                           const float *Q0, int n_timesteps, int lanes_per_row, float *Q_final);
 /* [new] The same with reps timed replays of the whole graph after one

@@ -1064,14 +1064,14 @@ static int matrix_run(const trmc_csr *PQ, const trmc_csr *Pq, const float *qlat,
     }
     if (ucol.empty()) { ucol.push_back(0); uval.push_back(make_float2(0.0f, 0.0f)); }
     // lanes per row: fixed, or by the row's union length U (smallest power of
-    // two G with 4 G >= U, at most 64); slots hold the rows of larger G first,
+    // two G with 2 G >= U, at most 64); slots hold the rows of larger G first,
     // in row order within one G; blocks hold one G each
     std::vector<int> gof((size_t)n), order((size_t)n);
     for (int r = 0; r < n; ++r) {
         int G = lanes;
         if (G == 0) {
             const int U = urow[r + 1] - urow[r];
-            for (G = 1; G < 64 && G * 4 < U; G *= 2) {}
+            for (G = 1; G < 64 && G * 2 < U; G *= 2) {}
         }
         gof[r] = G;
         order[r] = r;

@@ -787,7 +787,7 @@ void orc_free(void *p) { free(p); }
  * fused multiply-add); the partials are combined as p[l] = p[l] + p[l ^ o]
  * for o = G/2, ..., 1 and the row's value is p[0].  G = 1 is the sequential
  * row sum; G = 0 takes each row's G from its union length U: the smallest
- * power of two with 4 G >= U, at most 64.  Q [n] holds Q0 on entry and the
+ * power of two with 2 G >= U, at most 64.  Q [n] holds Q0 on entry and the
  * flow after T steps on return. */
 void orc_matrix_route(int n, const int *rpQ, const int *colQ, const float *valQ,
                       const int *rpq, const int *colq, const float *valq,
@@ -820,7 +820,7 @@ void orc_matrix_route(int n, const int *rpQ, const int *colQ, const float *valQ,
             }
             int G = Gfix;
             if (G == 0)
-                for (G = 1; G < 64 && G * 4 < U; G *= 2) {}
+                for (G = 1; G < 64 && G * 2 < U; G *= 2) {}
             for (int l = 0; l < G; l++) {
                 float s = 0.0f;
                 for (int k = l; k < U; k += G) {

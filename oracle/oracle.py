@@ -358,7 +358,7 @@ def matrix_build(up_ptr, up_idx, coeffs, tol, reference=False):
 def matrix_route(PQ, Pq, qlat32, Q0, n_timesteps, lanes=1):
     """The matrix-form steps in float over each row's union of its P_Q and P_q
     entries, with the summation order of `lanes` lanes per row (1 = sequential;
-    0 = by union length U, the smallest power of two G with 4 G >= U, at most
+    0 = by union length U, the smallest power of two G with 2 G >= U, at most
     64).  Returns Q after n_timesteps steps, float [n]."""
     n = len(PQ[0]) - 1
     arr = [np.ascontiguousarray(a) for a in (*PQ, *Pq)]
