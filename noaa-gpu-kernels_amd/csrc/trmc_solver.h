@@ -466,6 +466,323 @@ TRMC_HD double residual(double h, double dt, double qup, double quc, double qdp,
     return (routed + C.c4) - qmn;
 }
 
+// ---------------------------------------------- certified sign of F
+// The solves where F(mindepth) <= 0 (a sharply rising inflow makes the routed
+// flow at small depth negative) may hold a positive band of F narrower than
+// any sample spacing; there the leftmost downcrossing is found by a scan that
+// proves the sign of F on whole intervals.  Two facts make a cheap enclosure:
+//   * for fixed X the routed flow R = C1 qup + C2 quc + C3 qdp + C4 is a ratio
+//     of functions linear in K (positive denominator K (1 - X) + dt/2), and
+//     for fixed K a ratio linear in X: R is monotone in K and in X separately,
+//     so over a box of (K, X) it is extreme at a corner (K = inf as the limit
+//     C1 = -C2 = X / (1 - X), C3 = 1, C4 = 0; the C4 clamp, max(R, 0) when
+//     ql < 0, keeps this);
+//   * within the main channel (h <= bankfull) area, hydraulic radius, top
+//     width and Manning flow rise with h; on the floodplain area and wetted
+//     perimeter are linear in o = h - bankfull, so the hydraulic radius and the
+//     blended roughness are monotone in o, extreme at the ends (the bankfull end
+//     as the limit o -> 0+).
+// The celerity is bounded term by term, K = max(dt, dx / ck) and the weighting
+// X = clamp(0.5 (1 - q0 / (2 twu s0 ck dx)), 0.25, 0.5) from those ranges.
+
+TRMC_HD double routed_corner(double K, bool k_inf, double X, double dt, double qup, double quc, double qdp,
+                             double ql, bool clamp)
+{
+    double c1, c2, c3, c4;
+    if (k_inf) {
+        c1 = X / (1.0 - X); c2 = -c1; c3 = 1.0; c4 = 0.0;
+    } else {
+        const double D = K * (1.0 - X) + 0.5 * dt;
+        c1 = (K * X + 0.5 * dt) / D;
+        c2 = (0.5 * dt - K * X) / D;
+        c3 = (K * (1.0 - X) - 0.5 * dt) / D;
+        c4 = ql * dt / D;
+    }
+    double v = c1 * qup + c2 * quc + c3 * qdp + c4;
+    if (clamp && c4 < 0.0 && v < 0.0) v = 0.0;
+    return v;
+}
+
+struct DepthEnd { double A, R, n_eff, twu; };
+
+TRMC_HD DepthEnd depth_end(double h, bool flood, const Seg &c)
+{
+    DepthEnd e;
+    if (!flood) {
+        e.A = (c.bw + h * c.z) * h;
+        e.R = e.A / (c.bw + 2.0 * h * c.sq);
+        e.n_eff = c.n_ch;
+        e.twu = c.bw + 2.0 * c.z * h;
+    } else {
+        const double o = fmax(h - c.bfd, 0.0);
+        const double Ab = (c.bw + c.bfd * c.z) * c.bfd, Pb = c.bw + 2.0 * c.bfd * c.sq;
+        const double P = Pb + c.twcc + 2.0 * o;
+        e.A = Ab + c.twcc * o;
+        e.R = e.A / P;
+        e.n_eff = (Pb * c.n_ch + (c.twcc + 2.0 * o) * c.ncc) / P;
+        e.twu = c.twcc;
+    }
+    return e;
+}
+
+// Enclosure [lo, hi] of F on [a, b], both in the main channel (flood false)
+// or both at or above bankfull on a floodplain (flood true), widened by 1e-12
+// of the flows for rounding.
+struct Enclosure { double lo, hi; };
+
+TRMC_HD Enclosure f_enclosure(double a, double b, bool flood, double dt, double qup, double quc, double qdp,
+                              double ql, const Seg &c)
+{
+    const DepthEnd ea = depth_end(a, flood, c), eb = depth_end(b, flood, c);
+    const double Rlo = fmin(ea.R, eb.R), Rhi = fmax(ea.R, eb.R);
+    const double nlo = fmin(ea.n_eff, eb.n_eff), nhi = fmax(ea.n_eff, eb.n_eff);
+    const Pow2353 plo = pow23_53_faithful(Rlo), phi = pow23_53_faithful(Rhi);
+    const double qmn_lo = ea.A * plo.p23 * c.sqs0 / nhi;
+    const double qmn_hi = eb.A * phi.p23 * c.sqs0 / nlo;
+    const double wa = 2.0 * c.sq / (c.bw + 2.0 * a * c.z), wb = 2.0 * c.sq / (c.bw + 2.0 * b * c.z);
+    const double ck_lo = fmax(0.0, c.ks * ((5.0 / 3.0) * plo.p23 - (2.0 / 3.0) * phi.p53 * wa));
+    const double ck_hi = fmax(0.0, c.ks * ((5.0 / 3.0) * phi.p23 - (2.0 / 3.0) * plo.p53 * wb));
+    double K_lo = dt, K_hi = dt, X_lo = 0.5, X_hi = 0.5;
+    bool k_inf = false;
+    if (ck_hi > 0.0) {
+        if (!(ck_lo > 0.0)) {                 // the celerity may vanish: K anywhere in [dt, inf]
+            k_inf = true;
+            X_lo = 0.25;
+        } else {
+            K_lo = fmax(dt, c.dx / ck_hi);
+            K_hi = fmax(dt, c.dx / ck_lo);
+            const double q0a = routed_corner(K_lo, false, 0.5, dt, qup, quc, qdp, ql, false);
+            const double q0b = routed_corner(K_hi, false, 0.5, dt, qup, quc, qdp, ql, false);
+            const double q0lo = fmin(q0a, q0b), q0hi = fmax(q0a, q0b);
+            const double dlo = fmin(ea.twu, eb.twu) * c.s0 * ck_lo * c.dx;
+            const double dhi = fmax(ea.twu, eb.twu) * c.s0 * ck_hi * c.dx;
+            if (!(dlo > 0.0)) {
+                X_lo = 0.25;
+            } else {
+                const double rlo = fmin(q0lo / (2.0 * dlo), q0lo / (2.0 * dhi));
+                const double rhi = fmax(q0hi / (2.0 * dlo), q0hi / (2.0 * dhi));
+                X_lo = fmin(0.5, fmax(0.25, 0.5 * (1.0 - rhi)));
+                X_hi = fmin(0.5, fmax(0.25, 0.5 * (1.0 - rlo)));
+            }
+        }
+    }
+    const double r1 = routed_corner(K_lo, false, X_lo, dt, qup, quc, qdp, ql, true);
+    const double r2 = routed_corner(K_lo, false, X_hi, dt, qup, quc, qdp, ql, true);
+    const double r3 = routed_corner(K_hi, k_inf, X_lo, dt, qup, quc, qdp, ql, true);
+    const double r4 = routed_corner(K_hi, k_inf, X_hi, dt, qup, quc, qdp, ql, true);
+    const double slack = 1e-12 * (fabs(qup) + fabs(quc) + fabs(qdp) + 2.0 * fabs(ql) + qmn_hi);
+    Enclosure e;
+    e.lo = fmin(fmin(r1, r2), fmin(r3, r4)) - qmn_hi - slack;
+    e.hi = fmax(fmax(r1, r2), fmax(r3, r4)) - qmn_lo + slack;
+    return e;
+}
+
+// The largest routed flow any depth gives: the corners of K in [dt, inf],
+// X in [0.25, 0.5].
+TRMC_HD double routed_max(double dt, double qup, double quc, double qdp, double ql)
+{
+    const double r1 = routed_corner(dt, false, 0.25, dt, qup, quc, qdp, ql, true);
+    const double r2 = routed_corner(dt, false, 0.5, dt, qup, quc, qdp, ql, true);
+    const double r3 = routed_corner(dt, true, 0.25, dt, qup, quc, qdp, ql, true);
+    const double r4 = routed_corner(dt, true, 0.5, dt, qup, quc, qdp, ql, true);
+    return fmax(fmax(r1, r2), fmax(r3, r4));
+}
+
+// A lower bound of Qmn on [h, inf): Qmn rises in the main channel; on the
+// floodplain it is at least A(h) min(R_h(h), twcc / 2)^(2/3) sqrt(s0) / max(n, ncc)
+// (the hydraulic radius tends to twcc / 2 there).
+TRMC_HD double qmn_floor(double h, const Seg &c)
+{
+    const bool flood = c.twcc > 0.0 && c.bfd > 0.01 && c.bfd < 1.0e4;
+    const DepthEnd e = depth_end(h, false, c);
+    const double in_bank = e.A * pow23_53_faithful(e.R).p23 * c.sqs0 / c.n_ch;
+    if (!flood) return in_bank;
+    const DepthEnd f = depth_end(fmax(h, c.bfd), true, c);
+    const double fl = f.A * pow23_53_faithful(fmin(f.R, 0.5 * c.twcc)).p23 * c.sqs0 / fmax(c.n_ch, c.ncc);
+    return (h >= c.bfd) ? fl : fmin(in_bank, fl);
+}
+
+// F(mindepth) > 0 for certain: the smallest routed flow over every K >= dt and
+// X in [0.25, 0.5] (four corners) exceeds an upper bound of Qmn(0.01) (the
+// main channel's, R_h < h).  Otherwise F(mindepth) is evaluated first.
+TRMC_HD bool f_mindepth_positive(double qup, double quc, double qdp, double ql, const Seg &c)
+{
+    if (!(c.bfd >= 0.01)) return false;
+    const double r = fmin(fmin(0.6 * qup + 0.2 * quc + 0.2 * qdp + 0.8 * ql, qup + ql),
+                          fmin(qdp + (qup - quc) / 3.0, qdp + (qup - quc)));
+    const double qmn_ub = (c.bw + 0.01 * c.z) * 0.01 * 0.046415888336127789 * c.sqs0 / c.n_ch;
+    return r > qmn_ub * (1.0 + 1e-9) + 1e-13 * (fabs(qup) + fabs(quc) + fabs(qdp) + fabs(ql));
+}
+
+// Mean-value form.  The enclosure above bounds the routed flow and Qmn
+// separately, so its width grows like (|R'| + |Qmn'|) (b - a) while F changes
+// like |R' - Qmn'|: where the two nearly cancel (a narrow band of F) it is far
+// too wide.  An enclosure of F' over [a, b] by the chain rule,
+//   F' = R_K K' + R_X X' - Qmn',
+//   R_K = (dt/2)(X d + (1 - X)(qdp - S)) / D^2,  R_X = K (K d + dt (qup - qdp + ql)) / D^2,
+//   d = qup - quc, S = qup + quc - qdp + 2 ql, D = K (1 - X) + dt/2,
+//   K' = -dx ck' / ck^2 (0 where K = dt),  ck' = ks (g_rho rho' + g_w w'),
+//   g_rho = 10/9 rho^(-1/3) (1 - rho w),  g_w = -2/3 rho^(5/3),
+//   X' = -(q0' den - q0 den') / (4 den^2) where the weighting is not clamped,
+//   Qmn' = sqrt(s0) (A' rho^(2/3) / n + A 2/3 rho^(-1/3) rho' / n - A rho^(2/3) n' / n^2),
+// every factor in interval arithmetic (a clamp that may switch inside [a, b]
+// takes the hull with 0), with F at both ends gives bounds that are tight to
+// second order.  None is formed where the celerity may vanish.
+struct Iv { double lo, hi; };
+
+TRMC_HD Iv iv_of(double a, double b) { return Iv{fmin(a, b), fmax(a, b)}; }
+TRMC_HD Iv iv_add(Iv a, Iv b) { return Iv{a.lo + b.lo, a.hi + b.hi}; }
+TRMC_HD Iv iv_sub(Iv a, Iv b) { return Iv{a.lo - b.hi, a.hi - b.lo}; }
+TRMC_HD Iv iv_mul(Iv a, Iv b)
+{
+    const double p1 = a.lo * b.lo, p2 = a.lo * b.hi, p3 = a.hi * b.lo, p4 = a.hi * b.hi;
+    return Iv{fmin(fmin(p1, p2), fmin(p3, p4)), fmax(fmax(p1, p2), fmax(p3, p4))};
+}
+TRMC_HD Iv iv_scale(Iv a, double k) { return iv_of(a.lo * k, a.hi * k); }
+TRMC_HD Iv iv_inv(Iv a) { return Iv{1.0 / a.hi, 1.0 / a.lo}; }     // 0 not in a
+TRMC_HD Iv iv_hull0(Iv a) { return Iv{fmin(a.lo, 0.0), fmax(a.hi, 0.0)}; }
+
+// R_K over a (K, X) box (numerator linear in X; D > 0)
+TRMC_HD Iv routed_dk(Iv K, Iv X, double hdt, double d, double qdp_minus_S)
+{
+    const Iv one_x = iv_sub(Iv{1.0, 1.0}, X);
+    const Iv D = iv_add(iv_mul(K, one_x), Iv{hdt, hdt});
+    const Iv num = iv_scale(iv_add(iv_scale(X, d), iv_scale(one_x, qdp_minus_S)), hdt);
+    return iv_mul(num, iv_inv(iv_mul(D, D)));
+}
+
+TRMC_HD bool fprime_enclosure(double a, double b, bool flood, double dt, double qup, double quc, double qdp,
+                              double ql, const Seg &c, Iv &out)
+{
+    const double hdt = 0.5 * dt;
+    Iv A, Ad, rho, rhod, n_eff, n_effd, twu, twud;
+    if (!flood) {
+        const double Pa = c.bw + 2.0 * a * c.sq, Pb = c.bw + 2.0 * b * c.sq;
+        const double Aa = (c.bw + a * c.z) * a, Ab = (c.bw + b * c.z) * b;
+        A = Iv{Aa, Ab};
+        Ad = Iv{c.bw + 2.0 * c.z * a, c.bw + 2.0 * c.z * b};
+        rho = Iv{Aa / Pa, Ab / Pb};
+        // rho' = (bw^2 + 2 z h bw + 2 z h^2 sqrt(1+z^2)) / P^2, numerator rising
+        const double na = c.bw * c.bw + 2.0 * c.z * a * c.bw + 2.0 * c.z * a * a * c.sq;
+        const double nb = c.bw * c.bw + 2.0 * c.z * b * c.bw + 2.0 * c.z * b * b * c.sq;
+        rhod = Iv{na / (Pb * Pb), nb / (Pa * Pa)};
+        n_eff = Iv{c.n_ch, c.n_ch};
+        n_effd = Iv{0.0, 0.0};
+        twu = Iv{c.bw + 2.0 * c.z * a, c.bw + 2.0 * c.z * b};
+        twud = Iv{2.0 * c.z, 2.0 * c.z};
+    } else {
+        const double oa = fmax(a - c.bfd, 0.0), ob = fmax(b - c.bfd, 0.0);
+        const double Abf = (c.bw + c.bfd * c.z) * c.bfd, P0 = c.bw + 2.0 * c.bfd * c.sq;
+        const double Pa = P0 + c.twcc + 2.0 * oa, Pb = P0 + c.twcc + 2.0 * ob;
+        A = Iv{Abf + c.twcc * oa, Abf + c.twcc * ob};
+        Ad = Iv{c.twcc, c.twcc};
+        rho = iv_of((Abf + c.twcc * oa) / Pa, (Abf + c.twcc * ob) / Pb);
+        const double c0 = c.twcc * (P0 + c.twcc) - 2.0 * Abf;   // rho' = c0 / P^2
+        rhod = iv_of(c0 / (Pa * Pa), c0 / (Pb * Pb));
+        n_eff = iv_of((P0 * c.n_ch + (c.twcc + 2.0 * oa) * c.ncc) / Pa, (P0 * c.n_ch + (c.twcc + 2.0 * ob) * c.ncc) / Pb);
+        const double c1 = 2.0 * P0 * (c.ncc - c.n_ch);           // n' = c1 / P^2
+        n_effd = iv_of(c1 / (Pa * Pa), c1 / (Pb * Pb));
+        twu = Iv{c.twcc, c.twcc};
+        twud = Iv{0.0, 0.0};
+    }
+    const Pow2353 pl = pow23_53_faithful(rho.lo), ph = pow23_53_faithful(rho.hi);
+    const Iv r23 = Iv{pl.p23, ph.p23}, r53 = Iv{pl.p53, ph.p53};
+    const Iv rm13o = Iv{ph.p23 / rho.hi, pl.p23 / rho.lo};      // rho^(-1/3) = rho^(2/3) / rho, falling
+    const Iv inv_n = iv_inv(n_eff);
+    const Iv q1 = iv_mul(iv_mul(Ad, r23), inv_n);
+    const Iv q2 = iv_mul(iv_mul(iv_mul(A, rm13o), rhod), iv_scale(inv_n, 2.0 / 3.0));
+    const Iv q3 = iv_mul(iv_mul(iv_mul(A, r23), n_effd), iv_mul(inv_n, inv_n));
+    const Iv qmnd = iv_scale(iv_sub(iv_add(q1, q2), q3), c.sqs0);
+    const double ea = c.bw + 2.0 * a * c.z, eb = c.bw + 2.0 * b * c.z;
+    const Iv wv = Iv{2.0 * c.sq / eb, 2.0 * c.sq / ea};
+    const Iv wd = Iv{-4.0 * c.sq * c.z / (ea * ea), -4.0 * c.sq * c.z / (eb * eb)};
+    const Iv g = iv_sub(iv_scale(r23, 5.0 / 3.0), iv_mul(iv_scale(r53, 2.0 / 3.0), wv));
+    if (!(g.lo > 0.0)) return false;
+    const Iv ck = iv_scale(g, c.ks);
+    const Iv g_rho = iv_mul(iv_scale(rm13o, 10.0 / 9.0), iv_sub(Iv{1.0, 1.0}, iv_mul(rho, wv)));
+    const Iv ckd = iv_scale(iv_add(iv_mul(g_rho, rhod), iv_mul(iv_scale(r53, -2.0 / 3.0), wd)), c.ks);
+    Iv K, Kd;
+    const Iv kd_branch = iv_scale(iv_mul(ckd, iv_inv(iv_mul(ck, ck))), -c.dx);
+    if (c.dx / ck.hi >= dt) { K = Iv{c.dx / ck.hi, c.dx / ck.lo}; Kd = kd_branch; }
+    else if (c.dx / ck.lo <= dt) { K = Iv{dt, dt}; Kd = Iv{0.0, 0.0}; }
+    else { K = Iv{dt, c.dx / ck.lo}; Kd = iv_hull0(kd_branch); }
+    const double d = qup - quc, qdp_minus_S = qdp - (qup + quc - qdp + 2.0 * ql);
+    const Iv half = Iv{0.5, 0.5};
+    const Iv q0 = iv_of(routed_corner(K.lo, false, 0.5, dt, qup, quc, qdp, ql, false),
+                        routed_corner(K.hi, false, 0.5, dt, qup, quc, qdp, ql, false));
+    const Iv q0d = iv_mul(routed_dk(K, half, hdt, d, qdp_minus_S), Kd);
+    const Iv den = iv_scale(iv_mul(twu, ck), c.s0 * c.dx);
+    const Iv dend = iv_scale(iv_add(iv_mul(twud, ck), iv_mul(twu, ckd)), c.s0 * c.dx);
+    const Iv u = iv_sub(half, iv_mul(q0, iv_inv(iv_scale(den, 4.0))));
+    const Iv ud = iv_scale(iv_mul(iv_sub(iv_mul(q0d, den), iv_mul(q0, dend)), iv_inv(iv_mul(den, den))), -0.25);
+    const Iv X = Iv{fmin(0.5, fmax(0.25, u.lo)), fmin(0.5, fmax(0.25, u.hi))};
+    Iv Xd;
+    if (u.lo > 0.25 && u.hi < 0.5) Xd = ud;
+    else if (u.lo >= 0.5 || u.hi <= 0.25) Xd = Iv{0.0, 0.0};
+    else Xd = iv_hull0(ud);
+    const Iv Dx = iv_add(iv_mul(K, iv_sub(Iv{1.0, 1.0}, X)), Iv{hdt, hdt});
+    const double e = dt * (qup - qdp + ql);
+    const Iv RX = iv_mul(iv_mul(K, iv_add(iv_scale(K, d), Iv{e, e})), iv_inv(iv_mul(Dx, Dx)));
+    Iv Rd = iv_add(iv_mul(routed_dk(K, X, hdt, d, qdp_minus_S), Kd), iv_mul(RX, Xd));
+    if (ql < 0.0) Rd = iv_hull0(Rd);                          // the C4 clamp may switch inside
+    Iv Fd = iv_sub(Rd, qmnd);
+    const double mag = fmax(fabs(Fd.lo), fabs(Fd.hi));
+    Fd.lo -= 1e-9 * mag;
+    Fd.hi += 1e-9 * mag;
+    out = Fd;
+    return fabs(Fd.lo) < HUGE_VAL && fabs(Fd.hi) < HUGE_VAL;     // false for inf and NaN
+}
+
+// Bounds of F on [a, b] (width w) from F(a) = fa (when have_fa), F(b) = fb
+// and F' in [sl, sh]: with t = h - a, F(h) <= min(fa + sh t, fb - sl (w - t))
+// and F(h) >= max(fa + sl t, fb - sh (w - t)); a minimum of two lines is
+// concave, a maximum convex, so the extremes lie at t = 0, w or the crossing.
+TRMC_HD Enclosure slope_bounds(double w, double fa, bool have_fa, double fb, double sl, double sh)
+{
+    double u0 = fb - sl * w, uw = fb, l0 = fb - sh * w, lw = fb;
+    if (have_fa) {
+        u0 = fmin(u0, fa); uw = fmin(uw, fa + sh * w);
+        l0 = fmax(l0, fa); lw = fmax(lw, fa + sl * w);
+    }
+    Enclosure e;
+    e.hi = fmax(u0, uw);
+    e.lo = fmin(l0, lw);
+    if (have_fa && sh > sl) {
+        const double t = (fb - sl * w - fa) / (sh - sl);
+        if (t > 0.0 && t < w) e.hi = fmax(e.hi, fa + sh * t);
+        const double t2 = (fb - sh * w - fa) / (sl - sh);
+        if (t2 > 0.0 && t2 < w) e.lo = fmin(e.lo, fa + sl * t2);
+    }
+    return e;
+}
+
+// F keeps its sign (pos: F > 0, else F <= 0) on [a, b]: the direct enclosure,
+// then (only if that fails) the slope bounds.
+TRMC_HD bool sign_holds(double a, double b, bool flood, double fa, bool have_fa, double fb, bool pos, double dt,
+                        double qup, double quc, double qdp, double ql, const Seg &c)
+{
+    const Enclosure en = f_enclosure(a, b, flood, dt, qup, quc, qdp, ql, c);
+    if (pos ? en.lo > 0.0 : !(en.hi > 0.0)) return true;
+    Iv Fd;
+    if (!fprime_enclosure(a, b, flood, dt, qup, quc, qdp, ql, c, Fd)) return false;
+    const Enclosure sb = slope_bounds(b - a, fa, have_fa, fb, Fd.lo, Fd.hi);
+    const double slack = 1e-12 * (fabs(qup) + fabs(quc) + fabs(qdp) + 2.0 * fabs(ql) + fabs(fb));
+    return pos ? sb.lo - slack > 0.0 : !(sb.hi + slack > 0.0);
+}
+
+// The scan's next step after [a, b] (width w) held its sign: where F moves
+// toward 0, 0.95 of the distance at which the average slope over [a, b]
+// would reach it (a Newton step that stops short); else four times w.
+TRMC_HD double scan_next_step(double w, double fa, double fb)
+{
+    const double toward = fabs(fb) < fabs(fa) ? (fabs(fa) - fabs(fb)) / w : 0.0;
+    return (toward > 0.0) ? fmin(4.0 * w, 0.95 * fabs(fb) / toward) : 4.0 * w;
+}
+
+// relative depth resolution of the certified scan, and its step limit
+constexpr double kScanRes = 1e-6;
+constexpr int kScanCap = 400;
+
 // Resumable CONVERGED / LEARNED solve.  Every step() is one evaluation of
 // F, so interleaved lanes advance at a uniform cost per step.
 //   root:         the leftmost downcrossing of F on [mindepth, inf) -- the
@@ -481,9 +798,13 @@ TRMC_HD double residual(double h, double dt, double qup, double quc, double qdp,
 //                 positive point, mindepth while every F <= 0).
 //   stop:         |step| <= tol h, F zero to within its rounding, or
 //                 bracket width <= tol h.
-//   F(mindepth) <= 0: the oracle's geometric scan (x1.25, visiting bankfull)
-//                 for a positive point, then for the next non-positive one;
-//                 none positive leaves the root at mindepth.
+//   F(mindepth) <= 0: possible only when the routed flow can fall to Qmn at
+//                 1 cm (f_mindepth_positive false, under 1 % of solves); then
+//                 F(mindepth) is evaluated first, and if it is <= 0 a certified
+//                 scan (f_enclosure on whole intervals) finds the first
+//                 positive point and the first non-positive one after it,
+//                 whose bracket the secant solves; none positive leaves the
+//                 root at mindepth.
 //   bankfull:     a root above bankfull is kept only if F(bankfull) > 0 or
 //                 F(mindepth) <= 0; otherwise [mindepth, bankfull] is solved.
 //   fallback:     LEARNED solves that have not stopped after `budget`
@@ -503,14 +824,17 @@ struct SecantConv {
     double qmn_cur;          // Manning flow Qmn(h_cur)
     double root_h, root_q;
     double s_prev, s_prev2;  // the last two step sizes taken while bracketed
-    int evals, stage;
-    bool learned, need_guess, fell_back, capped, trivial, inbank, have_root, scan_bp, neg_seen;
+    int evals, stage, scan_n;
+    bool learned, need_guess, fell_back, capped, trivial, inbank, have_root, neg_seen;
+    bool spos;               // certified scan: F > 0 at its position
+    bool certified;          // the bracket came from the certified scan (no bankfull check)
 
     // upper end of the search: bankfull while the in-bank interval is solved
     TRMC_HD double hmax(const Seg &c) const { return inbank ? c.bfd : 1.0e4; }
 
     // stage: 0 eval ha, 1 eval hb, 2 secant, 3 bankfull check, 4 settle at
-    // mindepth, 5 done, 6 scan upward from mindepth, 7 re-evaluate hi_neg
+    // mindepth, 5 done, 7 re-evaluate hi_neg, 8 F(mindepth) first,
+    // 9 certified scan
     TRMC_HD bool start(double qup_, double quc_, double qdp_, double ql_, const Seg &c, double depthp,
                        bool learned_)
     {
@@ -525,7 +849,10 @@ struct SecantConv {
         trivial = seg_invalid(c) || (ql <= 0.0 && qup <= 0.0 && quc <= 0.0 && qdp <= 0.0);
         if (trivial) return true;
         dp = fmax(depthp, 0.0);
+        scan_n = 0;
+        certified = false;
         begin_plain_or_learned(c);
+        if (!f_mindepth_positive(qup, quc, qdp, ql, c)) stage = 8;
         return false;
     }
 
@@ -598,24 +925,18 @@ struct SecantConv {
         return fabs(f) <= 1.6e-14 * (fabs(qup) + fabs(quc) + fabs(qdp) + qmn_cur);
     }
 
-    // A lower bound of Qmn on [h, inf).  In bank Qmn rises with h (area and
-    // hydraulic radius both do), so below bankfull it is Qmn(h) for the rest
-    // of the bank.  Above bankfull, with o = h - bfd: the area A_b + twcc o
-    // rises, R = (A_b + twcc o) / (WP_b + twcc + 2 o) is monotone in o (a
-    // ratio of linear functions), so R >= min(R(o), twcc / 2), and the
-    // blended roughness is at most max(n, ncc).
-    TRMC_HD double qmn_never_below(double h, const Seg &c) const
+    // The certified scan keeps its position a (where the sign of F is known)
+    // in ha, F(a) in fa, the outflow there in qa and its step in hb: the
+    // secant's points are unused until the scan hands it a bracket.  Entered
+    // right after F(a) was evaluated.
+    TRMC_HD void begin_scan(double a, bool pos)
     {
-        const bool overbank = c.twcc > 0.0 && c.bfd > 0.01;
-        const double nmax = fmax(c.n_ch, c.ncc);
-        auto bound_above = [&](double o) {
-            const double ab = (c.bw + c.bfd * c.z) * c.bfd, wpb = c.bw + 2.0 * c.bfd * c.sq;
-            const double a = ab + c.twcc * o, r = a / (wpb + c.twcc + 2.0 * o);
-            return a * pow23_53_faithful(fmin(r, 0.5 * c.twcc)).p23 * c.sqs0 / nmax;
-        };
-        if (!overbank) return qmn_cur;              // in bank for every depth
-        if (h <= c.bfd) return fmin(qmn_cur, bound_above(0.0));
-        return bound_above(h - c.bfd);
+        ha = a;
+        fa = -1.0;
+        qa = q_cur;
+        hb = a;
+        spos = pos;
+        stage = 9;
     }
 
     TRMC_HD void accept_at(double h, double q, const Seg &c)
@@ -623,7 +944,7 @@ struct SecantConv {
         root_h = h;
         root_q = q;
         have_root = true;
-        stage = (!inbank && root_h > c.bfd && c.twcc > 0.0 && c.bfd > 0.01) ? 3 : 5;
+        stage = (!inbank && !certified && root_h > c.bfd && c.twcc > 0.0 && c.bfd > 0.01) ? 3 : 5;
     }
 
     TRMC_HD void accept_root(const Seg &c) { accept_at(h_cur, q_cur, c); }
@@ -632,7 +953,8 @@ struct SecantConv {
     TRMC_HD bool step(const Seg &c, const ConvCfg &k)
     {
         if (stage == 5) return true;
-        if (evals >= k.max_eval) {
+        // the cap counts evaluations outside the certified scan (which has its own)
+        if (stage != 9 && evals - scan_n >= k.max_eval) {
             capped = true;
             if (!have_root) { root_h = h_cur; root_q = q_cur; have_root = true; }
             stage = 5;
@@ -662,11 +984,7 @@ struct SecantConv {
             return stage == 5;
         case 2: {
             if (lo_pos < 0.0 && hi_neg == 0.01) {   // F(mindepth) <= 0, nothing positive above yet
-                hb = 0.01;                    // scan position; fb, qb stand in for F, outflow there
-                fb = -1.0;                    // (used only once the scan has replaced both points)
-                qb = q_cur;
-                scan_bp = !(c.twcc > 0.0 && c.bfd > 0.01);
-                stage = 6;
+                stage = 8;                    // F(mindepth) again, then the certified scan
                 return false;
             }
             const double hm = hmax(c);
@@ -679,7 +997,11 @@ struct SecantConv {
             // tol.  The root is the prediction hn; its outflow is the secant
             // interpolation of the outflows at ha and hb (an error of the
             // same second order).
-            if (h_cur == hb && hn == hn && (fb > 0.0 ? hn >= hb : hn <= hb) && (lo_pos > 0.0 || neg_seen) &&
+            // F jumps at bankfull on a floodplain, where the error estimate
+            // means nothing: a bracket across it (and the certified scan's,
+            // which may end there) converges by steps instead.
+            const bool smooth = !certified && !(c.twcc > 0.0 && fmin(ha, hb) <= c.bfd && fmax(ha, hb) > c.bfd);
+            if (smooth && h_cur == hb && hn == hn && (fb > 0.0 ? hn >= hb : hn <= hb) && (lo_pos > 0.0 || neg_seen) &&
                 !(hn <= lo_pos) && !(neg_seen && hn > hi_neg) && fabs(hn - hb) <= 1e-6 * hb &&
                 fabs(hn - hb) * fabs(hn - ha) <= k.tol * hb * hb) {
                 const double q = (ha != hb) ? qb + (qa - qb) * ((hn - hb) / (ha - hb)) : qb;
@@ -758,33 +1080,83 @@ struct SecantConv {
             have_root = true;
             stage = 5;
             return true;
-        case 6: {
-            // The oracle's scan (leftmost_downcrossing in mc_oracle.c): x1.25
-            // from mindepth, visiting bankfull, first to a point with F > 0,
-            // then on to the first point with F <= 0; the secant then solves
-            // that bracket.  The search for F > 0 stops early once Qmn on
-            // [h, inf) provably exceeds every flow the routing can produce,
-            // |qup| + |quc| + |qdp| + 2 |ql| (|C1|, |C2|, |C3| <= 1, C4 <= 2 ql
-            // since D >= dt / 2): F < 0 there, so the oracle finds nothing
-            // above either.
-            double hs = hb * 1.25;
-            if (!scan_bp && hs >= c.bfd) { hs = c.bfd; scan_bp = true; }
-            if (hs > hmax(c)) { stage = 4; return false; }
-            const double fs = eval(hs, c, k);
-            ha = hb; fa = fb;
-            hb = hs; fb = fs;
-            qa = qb; qb = q_cur;
-            if (lo_pos > 0.0) {                 // past a positive point: F <= 0 closes the bracket
-                if (!(fs > 0.0)) stage = 2;
-            } else if (!(fs > 0.0) && qmn_never_below(hs, c) > fabs(qup) + fabs(quc) + fabs(qdp) + 2.0 * fabs(ql)) {
-                stage = 4;
-            }
-            return false;
-        }
         case 7:
             eval(hi_neg, c, k);
             accept_root(c);
             return stage == 5;
+        case 8: {                             // F(mindepth) first: it may be <= 0
+            const double f = eval(0.01, c, k);
+            if (f > 0.0) {
+                stage = 0;
+            } else {
+                begin_scan(0.01, false);
+                fa = f;
+            }
+            return false;
+        }
+        case 9: {
+            // Certified scan (mirrors the oracle's) from a = ha, where F has
+            // the sign spos and the value fa, with step hb: F(b) at b = a + hb
+            // (not crossing bankfull); the other sign there means a sign
+            // change in (a, b]: at the resolution kScanRes a, b is the first
+            // point of that sign, else b moves back toward the secant estimate
+            // of the crossing; the same sign and the sign shown on all of
+            // [a, b] (sign_holds) move a to b with a Newton-like next step;
+            // otherwise the step halves, and at the resolution a moves on.
+            // Looking for F > 0 the scan ends at mindepth once Qmn on [a, inf)
+            // exceeds every routed flow; the first F <= 0 after a positive
+            // point closes the bracket the secant then solves.
+            if (++scan_n > kScanCap) {
+                capped = true;
+                if (spos) hi_neg = ha;
+                stage = spos ? 7 : 4;
+                return false;
+            }
+            const double a = ha, w = hb;
+            if (!spos) {
+                const double rmax = routed_max(k.dt, qup, quc, qdp, ql);
+                if (a >= 1.0e4 || qmn_floor(a, c) > rmax + 1e-12 * (fabs(rmax) + 1.0)) { stage = 4; return false; }
+            } else if (a >= 1.0e4) {
+                hi_neg = 1.0e4;
+                stage = 7;
+                return false;
+            }
+            const bool has_flood = c.twcc > 0.0 && c.bfd > 0.01 && c.bfd < 1.0e4;
+            double b = fmin(a + w, 1.0e4);
+            if (has_flood && a < c.bfd && b > c.bfd) b = c.bfd;
+            const bool flood = has_flood && a >= c.bfd;
+            const bool have_fa = !(flood && a <= c.bfd);
+            const double fbv = eval(b, c, k);
+            if (spos ? !(fbv > 0.0) : fbv > 0.0) {          // a sign change in (a, b]
+                if (b - a <= kScanRes * a) {
+                    if (spos) {                              // the bracket [a, b]
+                        lo_pos = a;
+                        hi_neg = b;
+                        neg_seen = true;
+                        certified = true;
+                        hb = b; fb = fbv; qb = q_cur;
+                        s_prev = s_prev2 = 1.0e30;
+                        stage = 2;
+                        return false;
+                    }
+                    spos = true;
+                    ha = b; fa = fbv; qa = q_cur;
+                    return false;
+                }
+                const double t = (have_fa && fa != fbv) ? (b - a) * fa / (fa - fbv) : 0.5 * (b - a);
+                hb = fmax(0.5 * kScanRes * a, fmin(0.5 * (b - a), 0.98 * t));
+                return false;
+            }
+            if (sign_holds(a, b, flood, fa, have_fa, fbv, spos, k.dt, qup, quc, qdp, ql, c)) {
+                hb = fmax(kScanRes * b, have_fa ? scan_next_step(b - a, fa, fbv) : 4.0 * (b - a));
+                ha = b; fa = fbv; qa = q_cur;
+                return false;
+            }
+            if (b - a > kScanRes * a) { hb = 0.5 * (b - a); return false; }
+            ha = b; fa = fbv; qa = q_cur;                    // at the resolution: move on
+            hb = 2.0 * w;
+            return false;
+        }
         }
         return true;
     }
@@ -792,7 +1164,7 @@ struct SecantConv {
     TRMC_HD SolveOut finish(const Seg &c) const
     {
         SolveOut o;
-        o.iters = evals;
+        o.iters = evals + scan_n;
         o.flags = (capped ? 1 : 0) | (fell_back ? 2 : 0);
         if (trivial) { o.q = 0.0; o.depth = 0.0; o.vel = 0.0; return o; }
         o.depth = root_h;

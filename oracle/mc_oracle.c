@@ -430,11 +430,17 @@ void orc_reach_parallel(const orc_chparams_f64 *p, const double *ql,
  *            coefficients at the same h.
  *
  * The converged depth h* is the leftmost downcrossing of F on [mindepth, inf):
- * the smallest h where F passes from > 0 to <= 0 (F may be non-monotone, e.g.
- * F(mindepth) <= 0 on a sharply rising hydrograph, or several crossings near
- * bankfull).  h* = mindepth when F is never positive.  The scan visits
- * bankfull, so an in-bank crossing is found before an overbank one.
- * Outputs then follow wavefront_mc.cu:321-336 at h*.                        */
+ * the smallest h where F passes from > 0 (just below) to <= 0 (at h); h* =
+ * mindepth when F is positive nowhere.  F need not be monotone: F(mindepth)
+ * <= 0 on a sharply rising hydrograph (the routed flow at small depth is
+ * negative), with a positive band above it, and the floodplain's jump in
+ * wetted perimeter at bankfull can make F positive again above an in-bank
+ * root.  A search over sample points can step over a narrow band, so h* is
+ * found by a CERTIFIED scan: every interval the scan passes over is shown to
+ * keep the sign of F by an enclosure of F on the whole interval
+ * (f_enclosure below), not by samples.  Only features narrower than
+ * CERT_RES relative depth (where the scan falls back to point values) are
+ * below its resolution.  Outputs then follow wavefront_mc.cu:321-336 at h*. */
 
 static double residual_at(double h, double dt, double qup, double quc, double qdp,
                           double ql, const chan_t *c, mcoef_t *Cout)
@@ -506,12 +512,522 @@ static double leftmost_downcrossing(scalar_fn f, const void *ctx, double lo, dou
     return a + (b - a) / 2.0;
 }
 
+/* ---- enclosure of F on an interval --------------------------------------
+ * F depends on h through the geometry and through the storage constant K and
+ * weighting X.  For fixed X the routed flow R = C1 qup + C2 quc + C3 qdp + C4
+ * is a ratio of functions linear in K (denominator K (1 - X) + dt/2 > 0),
+ * and for fixed K a ratio of functions linear in X, so R is monotone in each
+ * of K and X separately: over a box of (K, X) its extremes are at corners
+ * (K -> inf included as the limit C1 = X/(1-X) = -C2, C3 = 1, C4 = 0).  The
+ * C4 clamp turns R into max(R, 0) when ql < 0, which keeps that property.
+ *
+ * On [a, b] inside one regime (in the main channel, or on the floodplain
+ * above bankfull) every depth function used is monotone or a difference of
+ * monotone terms:
+ *   main channel: area A, hydraulic radius R_h (dR_h/dh > 0 for a trapezoid),
+ *                 top width, Manning flow Qmn all rise with h;
+ *   floodplain:   A = A_b + twcc o and wetted perimeter P = P_b + twcc + 2 o
+ *                 are linear in o = h - bfd, so R_h = A / P and the blended
+ *                 roughness (P_b n + (twcc + 2 o) ncc) / P are monotone in o,
+ *                 extremes at the ends (the end at bankfull is the limit from
+ *                 above, o -> 0+); Qmn >= A(a) min R_h^(2/3) sqrt(s0) / max n;
+ *   celerity:     5/3 R_h^(2/3) - 2/3 R_h^(5/3) w, w = 2 sqrt(1+z^2)/(bw+2hz)
+ *                 falling in h: bounded term by term;
+ *   K = max(dt, dx/ck) from the celerity range (K spans [dt, inf] when the
+ *                 celerity can be 0); q0 = R(K, 0.5) from the K range; X =
+ *                 clamp(0.5 (1 - q0 / (2 twu s0 ck dx)), 0.25, 0.5) from the
+ *                 ranges of q0 and of the denominator.
+ * The enclosure is [min R - max Qmn, max R - min Qmn], widened by 1e-12 of
+ * the flows for rounding.  It is loose on wide intervals and tight as they
+ * shrink, which is all the scan below needs. */
 typedef struct { double dt, qup, quc, qdp, ql; const chan_t *c; } route_ctx;
+
+static double routed_kx(const route_ctx *x, double K, double X, int clamp)
+{
+    double c1, c2, c3, c4;
+    if (isinf(K)) {
+        c1 = X / (1.0 - X); c2 = -c1; c3 = 1.0; c4 = 0.0;
+    } else {
+        double D = K * (1.0 - X) + x->dt / 2.0;
+        c1 = (K * X + x->dt / 2.0) / D;
+        c2 = (x->dt / 2.0 - K * X) / D;
+        c3 = (K * (1.0 - X) - x->dt / 2.0) / D;
+        c4 = x->ql * x->dt / D;
+    }
+    double v = c1 * x->qup + c2 * x->quc + c3 * x->qdp + c4;
+    if (clamp && c4 < 0.0 && v < 0.0) v = 0.0;
+    return v;
+}
+
+typedef struct { double A, R, n_eff, twu; } hbox_t;   /* depth functions at one end */
+
+static hbox_t depth_end(double h, int flood, const chan_t *c)
+{
+    hbox_t e;
+    double sq = sqrt(1.0 + c->z * c->z);
+    if (!flood) {
+        e.A = (c->bw + h * c->z) * h;
+        e.R = e.A / (c->bw + 2.0 * h * sq);
+        e.n_eff = c->n_ch;
+        e.twu = c->bw + 2.0 * c->z * h;
+    } else {
+        double o = fmax(h - c->bfd, 0.0);
+        double Ab = (c->bw + c->bfd * c->z) * c->bfd, Pb = c->bw + 2.0 * c->bfd * sq;
+        double P = Pb + c->twcc + 2.0 * o;
+        e.A = Ab + c->twcc * o;
+        e.R = e.A / P;
+        e.n_eff = (Pb * c->n_ch + (c->twcc + 2.0 * o) * c->ncc) / P;
+        e.twu = c->twcc;
+    }
+    return e;
+}
+
+/* Enclosure [*lo, *hi] of F on [a, b]; flood = the floodplain regime
+ * (a >= bfd, twcc > 0), else the main channel (b <= bfd or no floodplain). */
+static void f_enclosure(const route_ctx *x, double a, double b, int flood, double *lo, double *hi)
+{
+    const chan_t *c = x->c;
+    const double dt = x->dt;
+    hbox_t ea = depth_end(a, flood, c), eb = depth_end(b, flood, c);
+    double Rlo = fmin(ea.R, eb.R), Rhi = fmax(ea.R, eb.R);
+    double nlo = fmin(ea.n_eff, eb.n_eff), nhi = fmax(ea.n_eff, eb.n_eff);
+    double sqs0 = sqrt(c->s0), ks = sqs0 / c->n_ch, sq = sqrt(1.0 + c->z * c->z);
+    /* Manning flow */
+    double qmn_lo = ea.A * orc_pow(Rlo, 2.0 / 3.0) * sqs0 / nhi;
+    double qmn_hi = eb.A * orc_pow(Rhi, 2.0 / 3.0) * sqs0 / nlo;
+    /* celerity */
+    double wa = 2.0 * sq / (c->bw + 2.0 * a * c->z), wb = 2.0 * sq / (c->bw + 2.0 * b * c->z);
+    double ck_lo = fmax(0.0, ks * ((5.0 / 3.0) * orc_pow(Rlo, 2.0 / 3.0) - (2.0 / 3.0) * orc_pow(Rhi, 5.0 / 3.0) * wa));
+    double ck_hi = fmax(0.0, ks * ((5.0 / 3.0) * orc_pow(Rhi, 2.0 / 3.0) - (2.0 / 3.0) * orc_pow(Rlo, 5.0 / 3.0) * wb));
+    /* storage constant and weighting */
+    double K_lo, K_hi, X_lo, X_hi;
+    if (!(ck_hi > 0.0)) {                   /* ck == 0: K = dt, X = 0.5 */
+        K_lo = K_hi = dt; X_lo = X_hi = 0.5;
+    } else if (!(ck_lo > 0.0)) {            /* ck may be 0 or tiny */
+        K_lo = dt; K_hi = INFINITY; X_lo = 0.25; X_hi = 0.5;
+    } else {
+        K_lo = fmax(dt, c->dx / ck_hi);
+        K_hi = fmax(dt, c->dx / ck_lo);
+        double q0a = routed_kx(x, K_lo, 0.5, 0), q0b = routed_kx(x, K_hi, 0.5, 0);
+        double q0lo = fmin(q0a, q0b), q0hi = fmax(q0a, q0b);
+        double twlo = fmin(ea.twu, eb.twu), twhi = fmax(ea.twu, eb.twu);
+        double dlo = twlo * c->s0 * ck_lo * c->dx, dhi = twhi * c->s0 * ck_hi * c->dx;
+        if (!(dlo > 0.0)) {
+            X_lo = 0.25; X_hi = 0.5;
+        } else {
+            double rlo = fmin(q0lo / (2.0 * dlo), q0lo / (2.0 * dhi));
+            double rhi = fmax(q0hi / (2.0 * dlo), q0hi / (2.0 * dhi));
+            X_lo = fmin(0.5, fmax(0.25, 0.5 * (1.0 - rhi)));
+            X_hi = fmin(0.5, fmax(0.25, 0.5 * (1.0 - rlo)));
+        }
+    }
+    double r1 = routed_kx(x, K_lo, X_lo, 1), r2 = routed_kx(x, K_lo, X_hi, 1);
+    double r3 = routed_kx(x, K_hi, X_lo, 1), r4 = routed_kx(x, K_hi, X_hi, 1);
+    double Rt_lo = fmin(fmin(r1, r2), fmin(r3, r4)), Rt_hi = fmax(fmax(r1, r2), fmax(r3, r4));
+    double slack = 1e-12 * (fabs(x->qup) + fabs(x->quc) + fabs(x->qdp) + 2.0 * fabs(x->ql) + qmn_hi);
+    *lo = Rt_lo - qmn_hi - slack;
+    *hi = Rt_hi - qmn_lo + slack;
+}
+
+/* ---- mean-value enclosure --------------------------------------------------
+ * The direct enclosure bounds the routed flow and Qmn separately, so its
+ * width grows like (|R'| + |Qmn'|) (b - a) while F changes like |R' - Qmn'|:
+ * where the two nearly cancel (a narrow band of F) it is far too wide.  The
+ * mean-value form F(h) in F(m) + F'([a, b]) [a - m, b - m], m the midpoint,
+ * is tight to second order.  F' = R_K K' + R_X X' - Qmn' by the chain rule,
+ * every factor bounded by interval arithmetic over [a, b]:
+ *   R_K = (dt/2) (X d + (1 - X)(qdp - S)) / D^2,  R_X = K (K d + dt (qup - qdp + ql)) / D^2,
+ *     d = qup - quc, S = qup + quc - qdp + 2 ql, D = K (1 - X) + dt/2;
+ *   K' = -dx ck' / ck^2 where dx / ck > dt (0 where K = dt),
+ *   ck' = ks (g_rho rho' + g_w w'), g_rho = 10/9 rho^(-1/3) (1 - rho w), g_w = -2/3 rho^(5/3),
+ *   X' = -(q0' den - q0 den') / (4 den^2) where the weighting is not clamped,
+ *   Qmn' = sqrt(s0) (A' rho^(2/3) / n + A 2/3 rho^(-1/3) rho' / n - A rho^(2/3) n' / n^2);
+ * at a kink (a clamp that may switch inside [a, b]) the range takes the hull
+ * with 0, which keeps the mean-value bound (F is continuous within a regime).
+ * Returns 0 where the celerity may vanish (K unbounded); the direct enclosure
+ * alone is used there. */
+typedef struct { double lo, hi; } iv_t;
+
+static iv_t iv(double a, double b) { iv_t r = {fmin(a, b), fmax(a, b)}; return r; }
+static iv_t iv_add(iv_t a, iv_t b) { return iv(a.lo + b.lo, a.hi + b.hi); }
+static iv_t iv_sub(iv_t a, iv_t b) { return iv(a.lo - b.hi, a.hi - b.lo); }
+static iv_t iv_mul(iv_t a, iv_t b)
+{
+    double p1 = a.lo * b.lo, p2 = a.lo * b.hi, p3 = a.hi * b.lo, p4 = a.hi * b.hi;
+    iv_t r = {fmin(fmin(p1, p2), fmin(p3, p4)), fmax(fmax(p1, p2), fmax(p3, p4))};
+    return r;
+}
+static iv_t iv_scale(iv_t a, double k) { return iv(a.lo * k, a.hi * k); }
+static iv_t iv_inv(iv_t a) { return iv(1.0 / a.hi, 1.0 / a.lo); }   /* 0 not in a */
+static iv_t iv_hull0(iv_t a) { return iv(fmin(a.lo, 0.0), fmax(a.hi, 0.0)); }
+static iv_t iv_pow(iv_t a, double y) { return iv(orc_pow(a.lo, y), orc_pow(a.hi, y)); }   /* a > 0 */
+
+static int fprime_enclosure(const route_ctx *x, double a, double b, int flood, iv_t *out)
+{
+    const chan_t *c = x->c;
+    const double dt = x->dt, hdt = 0.5 * dt;
+    const double sq = sqrt(1.0 + c->z * c->z), sqs0 = sqrt(c->s0), ks = sqs0 / c->n_ch;
+    iv_t A, Ad, rho, rhod, n_eff, n_effd, twu, twud;
+    if (!flood) {
+        A = iv((c->bw + a * c->z) * a, (c->bw + b * c->z) * b);
+        Ad = iv(c->bw + 2.0 * c->z * a, c->bw + 2.0 * c->z * b);
+        double Pa = c->bw + 2.0 * a * sq, Pb_ = c->bw + 2.0 * b * sq;
+        rho = iv(((c->bw + a * c->z) * a) / Pa, ((c->bw + b * c->z) * b) / Pb_);
+        /* rho' = (bw^2 + 2 z h bw + 2 z h^2 sqrt(1+z^2)) / P^2, numerator rising */
+        double na = c->bw * c->bw + 2.0 * c->z * a * c->bw + 2.0 * c->z * a * a * sq;
+        double nb = c->bw * c->bw + 2.0 * c->z * b * c->bw + 2.0 * c->z * b * b * sq;
+        rhod = iv(na / (Pb_ * Pb_), nb / (Pa * Pa));
+        n_eff = iv(c->n_ch, c->n_ch);
+        n_effd = iv(0.0, 0.0);
+        twu = iv(c->bw + 2.0 * c->z * a, c->bw + 2.0 * c->z * b);
+        twud = iv(2.0 * c->z, 2.0 * c->z);
+    } else {
+        double oa = fmax(a - c->bfd, 0.0), ob = fmax(b - c->bfd, 0.0);
+        double Ab = (c->bw + c->bfd * c->z) * c->bfd, Pb0 = c->bw + 2.0 * c->bfd * sq;
+        double Pa = Pb0 + c->twcc + 2.0 * oa, Pb_ = Pb0 + c->twcc + 2.0 * ob;
+        A = iv(Ab + c->twcc * oa, Ab + c->twcc * ob);
+        Ad = iv(c->twcc, c->twcc);
+        rho = iv((Ab + c->twcc * oa) / Pa, (Ab + c->twcc * ob) / Pb_);
+        double c0 = c->twcc * (Pb0 + c->twcc) - 2.0 * Ab;          /* rho' = c0 / P^2 */
+        rhod = iv(c0 / (Pa * Pa), c0 / (Pb_ * Pb_));
+        n_eff = iv((Pb0 * c->n_ch + (c->twcc + 2.0 * oa) * c->ncc) / Pa,
+                   (Pb0 * c->n_ch + (c->twcc + 2.0 * ob) * c->ncc) / Pb_);
+        double c1 = 2.0 * Pb0 * (c->ncc - c->n_ch);                  /* n' = c1 / P^2 */
+        n_effd = iv(c1 / (Pa * Pa), c1 / (Pb_ * Pb_));
+        twu = iv(c->twcc, c->twcc);
+        twud = iv(0.0, 0.0);
+    }
+    iv_t r23 = iv_pow(rho, 2.0 / 3.0), r53 = iv_pow(rho, 5.0 / 3.0), rm13 = iv_pow(rho, -1.0 / 3.0);
+    iv_t inv_n = iv_inv(n_eff);
+    /* Qmn' */
+    iv_t q1 = iv_mul(iv_mul(Ad, r23), inv_n);
+    iv_t q2 = iv_mul(iv_mul(iv_mul(A, rm13), rhod), iv_scale(inv_n, 2.0 / 3.0));
+    iv_t q3 = iv_mul(iv_mul(iv_mul(A, r23), n_effd), iv_mul(inv_n, inv_n));
+    iv_t qmnd = iv_scale(iv_sub(iv_add(q1, q2), q3), sqs0);
+    /* celerity and its derivative */
+    double wa = 2.0 * sq / (c->bw + 2.0 * a * c->z), wb = 2.0 * sq / (c->bw + 2.0 * b * c->z);
+    iv_t wv = iv(wb, wa);
+    double ea = c->bw + 2.0 * a * c->z, eb = c->bw + 2.0 * b * c->z;
+    iv_t wd = iv(-4.0 * sq * c->z / (ea * ea), -4.0 * sq * c->z / (eb * eb));
+    iv_t g = iv_sub(iv_scale(r23, 5.0 / 3.0), iv_mul(iv_scale(r53, 2.0 / 3.0), wv));
+    if (!(g.lo > 0.0)) return 0;
+    iv_t ck = iv_scale(g, ks);
+    iv_t g_rho = iv_mul(iv_scale(rm13, 10.0 / 9.0), iv_sub(iv(1.0, 1.0), iv_mul(rho, wv)));
+    iv_t ckd = iv_scale(iv_add(iv_mul(g_rho, rhod), iv_mul(iv_scale(r53, -2.0 / 3.0), wd)), ks);
+    /* storage constant */
+    iv_t K, Kd;
+    iv_t kd_branch = iv_scale(iv_mul(ckd, iv_inv(iv_mul(ck, ck))), -c->dx);
+    if (c->dx / ck.hi >= dt) { K = iv(c->dx / ck.hi, c->dx / ck.lo); Kd = kd_branch; }
+    else if (c->dx / ck.lo <= dt) { K = iv(dt, dt); Kd = iv(0.0, 0.0); }
+    else { K = iv(dt, c->dx / ck.lo); Kd = iv_hull0(kd_branch); }
+    const double d = x->qup - x->quc, S = x->qup + x->quc - x->qdp + 2.0 * x->ql;
+    /* routed flow partials over a (K, X) box */
+#define RK(Kv, Xv) iv_mul(iv_scale(iv_add(iv_scale(Xv, d), iv_scale(iv_sub(iv(1.0, 1.0), Xv), x->qdp - S)), hdt), \
+                          iv_inv(iv_mul(iv_add(iv_mul(Kv, iv_sub(iv(1.0, 1.0), Xv)), iv(hdt, hdt)), \
+                                        iv_add(iv_mul(Kv, iv_sub(iv(1.0, 1.0), Xv)), iv(hdt, hdt)))))
+    /* weighting */
+    iv_t half = iv(0.5, 0.5);
+    double q0a = routed_kx(x, K.lo, 0.5, 0), q0b = routed_kx(x, K.hi, 0.5, 0);
+    iv_t q0 = iv(q0a, q0b);
+    iv_t q0d = iv_mul(RK(K, half), Kd);
+    iv_t den = iv_scale(iv_mul(twu, ck), c->s0 * c->dx);
+    iv_t dend = iv_scale(iv_add(iv_mul(twud, ck), iv_mul(twu, ckd)), c->s0 * c->dx);
+    iv_t u = iv_sub(half, iv_mul(q0, iv_inv(iv_scale(den, 4.0))));
+    iv_t ud = iv_scale(iv_mul(iv_sub(iv_mul(q0d, den), iv_mul(q0, dend)), iv_inv(iv_mul(den, den))), -0.25);
+    iv_t X = iv(fmin(0.5, fmax(0.25, u.lo)), fmin(0.5, fmax(0.25, u.hi)));
+    iv_t Xd;
+    if (u.lo > 0.25 && u.hi < 0.5) Xd = ud;
+    else if (u.lo >= 0.5 || u.hi <= 0.25) Xd = iv(0.0, 0.0);
+    else Xd = iv_hull0(ud);
+    iv_t Dx = iv_add(iv_mul(K, iv_sub(iv(1.0, 1.0), X)), iv(hdt, hdt));
+    iv_t RX = iv_mul(iv_mul(K, iv_add(iv_scale(K, d), iv(dt * (x->qup - x->qdp + x->ql), dt * (x->qup - x->qdp + x->ql)))),
+                     iv_inv(iv_mul(Dx, Dx)));
+    iv_t Rd = iv_add(iv_mul(RK(K, X), Kd), iv_mul(RX, Xd));
+#undef RK
+    if (x->ql < 0.0) Rd = iv_hull0(Rd);        /* the C4 clamp may switch inside */
+    iv_t Fd = iv_sub(Rd, qmnd);
+    double mag = fmax(fabs(Fd.lo), fabs(Fd.hi));
+    Fd.lo -= 1e-9 * mag + 1e-300;
+    Fd.hi += 1e-9 * mag + 1e-300;
+    *out = Fd;
+    return isfinite(Fd.lo) && isfinite(Fd.hi);
+}
+
+/* The direct enclosure intersected with the mean-value one. */
+static void f_enclosure_mv(const route_ctx *x, double a, double b, int flood, double *lo, double *hi)
+{
+    f_enclosure(x, a, b, flood, lo, hi);
+    iv_t Fd;
+    if (fprime_enclosure(x, a, b, flood, &Fd)) {
+        double m = a + 0.5 * (b - a);
+        double fm = residual_at(m, x->dt, x->qup, x->quc, x->qdp, x->ql, x->c, NULL);
+        double r = 0.5 * (b - a) * fmax(fabs(Fd.lo), fabs(Fd.hi));
+        double slack = 1e-12 * (fabs(x->qup) + fabs(x->quc) + fabs(x->qdp) + 2.0 * fabs(x->ql) + fabs(fm));
+        *lo = fmax(*lo, fm - r - slack);
+        *hi = fmin(*hi, fm + r + slack);
+    }
+}
+
+/* The largest routed flow any depth can give: the four corners of
+ * K in [dt, inf], X in [0.25, 0.5]. */
+static double routed_max(const route_ctx *x)
+{
+    double r1 = routed_kx(x, x->dt, 0.25, 1), r2 = routed_kx(x, x->dt, 0.5, 1);
+    double r3 = routed_kx(x, INFINITY, 0.25, 1), r4 = routed_kx(x, INFINITY, 0.5, 1);
+    return fmax(fmax(r1, r2), fmax(r3, r4));
+}
+
+/* A lower bound of Qmn on [h, inf) (Qmn rises in the main channel; on the
+ * floodplain it is at least A(h) min(R_h(h), twcc/2)^(2/3) sqrt(s0) / max n,
+ * R_h tending to twcc/2). */
+static double qmn_floor_above(double h, const chan_t *c)
+{
+    int flood = c->twcc > 0.0 && c->bfd > 0.01 && c->bfd < 1.0e4;
+    double sqs0 = sqrt(c->s0);
+    double nmax = fmax(c->n_ch, c->ncc);
+    hbox_t fb = depth_end(fmax(h, c->bfd), 1, c);
+    double flood_floor = fb.A * orc_pow(fmin(fb.R, 0.5 * c->twcc), 2.0 / 3.0) * sqs0 / nmax;
+    if (!flood) {
+        hbox_t e = depth_end(h, 0, c);
+        return e.A * orc_pow(e.R, 2.0 / 3.0) * sqs0 / c->n_ch;
+    }
+    if (h >= c->bfd) return flood_floor;
+    hbox_t e = depth_end(h, 0, c);
+    return fmin(e.A * orc_pow(e.R, 2.0 / 3.0) * sqs0 / c->n_ch, flood_floor);
+}
 
 static double route_residual_fn(double h, const void *vctx)
 {
     const route_ctx *x = (const route_ctx *)vctx;
     return residual_at(h, x->dt, x->qup, x->quc, x->qdp, x->ql, x->c, NULL);
+}
+
+#define CERT_RES 1e-6         /* relative depth resolution of the scan */
+#define CERT_HMAX 1.0e4       /* depths searched: [mindepth, CERT_HMAX] */
+
+static long long g_cert_steps;  /* enclosures evaluated (diagnostics, racy under threads) */
+
+typedef struct {
+    const route_ctx *x;
+    int has_flood;
+    double bfd;
+    long long steps;
+} scan_t;
+
+/* Bounds of F on [a, b] from F at its ends and F' in [sl, sh]: for t = h - a
+ * in [0, w], F(h) <= min(F(a) + sh t, F(b) - sl (w - t)) and F(h) >=
+ * max(F(a) + sl t, F(b) - sh (w - t)); a minimum of two lines is concave, a
+ * maximum convex, so their extremes lie at t = 0, t = w or where the lines
+ * cross.  have_fa = 0 drops the left line (an interval that starts at
+ * bankfull on the floodplain, where F(a) is the main channel's value). */
+static void slope_bounds(double w, double fa, int have_fa, double fb, double sl, double sh,
+                         double *lo, double *hi)
+{
+    double u0 = fb - sl * w, uw = fb, l0 = fb - sh * w, lw = fb;
+    if (have_fa) {
+        u0 = fmin(u0, fa); uw = fmin(uw, fa + sh * w);
+        l0 = fmax(l0, fa); lw = fmax(lw, fa + sl * w);
+    }
+    double up = fmax(u0, uw), dn = fmin(l0, lw);
+    if (have_fa && sh > sl) {
+        double t = (fb - sl * w - fa) / (sh - sl);            /* fa + sh t == fb - sl (w - t) */
+        if (t > 0.0 && t < w) up = fmax(up, fa + sh * t);
+        double t2 = (fb - sh * w - fa) / (sl - sh);           /* fa + sl t == fb - sh (w - t) */
+        if (t2 > 0.0 && t2 < w) dn = fmin(dn, fa + sl * t2);
+    }
+    *lo = dn;
+    *hi = up;
+}
+
+/* Does F keep its sign (pos: F > 0, else F <= 0) on [a, b] (not crossing
+ * bankfull)?  fa = F(a) (unused when !have_fa), fb = F(b), both of that sign.
+ * The direct enclosure intersected with the slope bounds; *sl, *sh receive
+ * the slope enclosure (+-inf when none is formed). */
+static int sign_holds(scan_t *S, double a, double b, int flood, double fa, int have_fa, double fb, int pos)
+{
+    const route_ctx *x = S->x;
+    double elo, ehi;
+    ++S->steps;
+    f_enclosure(x, a, b, flood, &elo, &ehi);
+    if (pos ? elo > 0.0 : !(ehi > 0.0)) return 1;
+    iv_t Fd;
+    if (!fprime_enclosure(x, a, b, flood, &Fd)) return 0;
+    double lo, hi;
+    slope_bounds(b - a, fa, have_fa, fb, Fd.lo, Fd.hi, &lo, &hi);
+    double slack = 1e-12 * (fabs(x->qup) + fabs(x->quc) + fabs(x->qdp) + 2.0 * fabs(x->ql) + fabs(fb));
+    return pos ? lo - slack > 0.0 : !(hi + slack > 0.0);
+}
+
+/* The next step after [a, b] (width w) held its sign, F(a) = fa, F(b) = fb:
+ * where F moves toward 0, 0.95 of the distance at which the average slope
+ * over [a, b] would reach it (a Newton step that stops short); otherwise,
+ * or beyond that, four times w. */
+static double next_step(double w, double fa, double fb)
+{
+    double toward = fabs(fb) < fabs(fa) ? (fabs(fa) - fabs(fb)) / w : 0.0;
+    double wn = (toward > 0.0) ? 0.95 * fabs(fb) / toward : 4.0 * w;
+    return fmin(4.0 * w, wn);
+}
+
+/* Root of F in [l, r] with F(l) > 0 >= F(r), to two ulps: regula falsi with
+ * the Illinois modification, bisecting whenever a step fails to halve the
+ * bracket. */
+static double bracket_root(const route_ctx *x, double l, double fl, double r, double fr)
+{
+    int side = 0;
+    double width = r - l;
+    for (int i = 0; i < 400; ++i) {
+        double m = (fr - fl != 0.0) ? (l * fr - r * fl) / (fr - fl) : l + (r - l) / 2.0;
+        if ((i % 3) == 2 && r - l > 0.5 * width) m = l + (r - l) / 2.0;
+        if ((i % 3) == 2) width = r - l;
+        if (!(m > l && m < r)) m = l + (r - l) / 2.0;
+        if (!(m > l && m < r)) break;
+        double fm = route_residual_fn(m, x);
+        if (fm > 0.0) {
+            l = m; fl = fm;
+            if (side == 1) fr *= 0.5;
+            side = 1;
+        } else {
+            r = m; fr = fm;
+            if (side == -1) fl *= 0.5;
+            side = -1;
+        }
+    }
+    return l + (r - l) / 2.0;
+}
+
+/* Generic certified scan from a, where F's sign is pos and its value fa.  It
+ * keeps a point a whose sign is known, evaluates F at b = a + w (never
+ * crossing bankfull), and:
+ *   - F(b) of the other sign: a sign change lies in (a, b]; at the resolution
+ *     (w <= CERT_RES a) b is taken as the first point of the other sign,
+ *     else b moves back toward the secant estimate of the crossing;
+ *   - F(b) of the same sign and the sign shown to hold on all of [a, b]:
+ *     a moves to b, and the next step is 0.9 of the distance the slope bound
+ *     lets F travel toward 0 (at most 4 w);
+ *   - otherwise the step halves, and at the resolution a moves to b.
+ * Looking for F > 0 it stops when Qmn on [a, inf) exceeds every routed flow
+ * (F < 0 beyond: no root, the answer is lo); looking for F <= 0, the first
+ * such point b closes a bracket [a, b] solved by bracket_root. */
+static double scan_generic(scan_t *S, double a, double fa, int pos, double lo)
+{
+    const route_ctx *x = S->x;
+    const double rmax = routed_max(x);
+    double w = a;
+    for (;;) {
+        if (!pos && qmn_floor_above(a, x->c) > rmax + 1e-12 * (fabs(rmax) + 1.0)) return lo;
+        if (a >= CERT_HMAX) return pos ? CERT_HMAX : lo;
+        double b = fmin(a + w, CERT_HMAX);
+        if (S->has_flood && a < S->bfd && b > S->bfd) b = S->bfd;
+        const int flood = S->has_flood && a >= S->bfd;
+        const int have_fa = !(flood && a <= S->bfd);
+        double fb = route_residual_fn(b, x);
+        if (pos ? !(fb > 0.0) : fb > 0.0) {          /* a sign change in (a, b] */
+            if (b - a <= CERT_RES * a) {
+                if (pos) return bracket_root(x, a, fa, b, fb);
+                pos = 1; a = b; fa = fb;
+                continue;
+            }
+            double t = (have_fa && fa != fb) ? (b - a) * fa / (fa - fb) : 0.5 * (b - a);
+            w = fmax(0.5 * CERT_RES * a, fmin(0.5 * (b - a), 0.98 * t));
+            continue;
+        }
+        if (sign_holds(S, a, b, flood, fa, have_fa, fb, pos)) {
+            w = fmax(CERT_RES * b, have_fa ? next_step(b - a, fa, fb) : 4.0 * (b - a));
+            a = b; fa = fb;
+            continue;
+        }
+        if (b - a > CERT_RES * a) { w = 0.5 * (b - a); continue; }
+        a = b; fa = fb;                                /* at the resolution: move on */
+        w = 2.0 * w;
+    }
+}
+
+/* Certified leftmost downcrossing of F on [lo, CERT_HMAX].
+ * F(lo) <= 0: the generic scan (first a positive point, then the first
+ * non-positive one).  F(lo) > 0, the usual case: a candidate root first (F
+ * at lo 2^k, visiting bankfull, up to the first F <= 0; then bracket_root),
+ * then a certificate that F > 0 on [lo, candidate) with steps that are a
+ * fraction of the remaining distance; if a step cannot be certified even at
+ * the resolution, F has another sign change first and the generic scan takes
+ * over from the last certified point. */
+static double certified_leftmost_downcrossing(const route_ctx *x, double lo)
+{
+    const chan_t *c = x->c;
+    scan_t S = {x, c->twcc > 0.0 && c->bfd > lo && c->bfd < CERT_HMAX, c->bfd, 0};
+    double root;
+    double flo = route_residual_fn(lo, x);
+    if (!(flo > 0.0)) {
+        root = scan_generic(&S, lo, flo, 0, lo);
+        g_cert_steps += S.steps;
+        return root;
+    }
+    /* candidate */
+    double p = lo, fp = flo, q = lo, fq = flo;
+    int used_bfd = !S.has_flood;
+    while (fq > 0.0 && q < CERT_HMAX) {
+        p = q; fp = fq;
+        q = fmin(2.0 * q, CERT_HMAX);
+        if (!used_bfd && q >= S.bfd) { q = S.bfd; used_bfd = 1; }
+        fq = route_residual_fn(q, x);
+    }
+    if (fq > 0.0) { g_cert_steps += S.steps; return CERT_HMAX; }
+    double r = bracket_root(x, p, fp, q, fq);
+    /* certificate on [lo, r) */
+    double a = lo, fa = flo, beta = 0.5;
+    for (;;) {
+        double d = r - a;
+        if (d <= CERT_RES * a) break;
+        double b = a + beta * d;
+        if (S.has_flood && a < S.bfd && b > S.bfd) b = S.bfd;
+        const int flood = S.has_flood && a >= S.bfd;
+        double fb = route_residual_fn(b, x);
+        if (!(fb > 0.0)) {                            /* another sign change first */
+            r = scan_generic(&S, a, fa, 1, lo);
+            break;
+        }
+        const int have_fa = !(flood && a <= S.bfd);
+        if (sign_holds(&S, a, b, flood, fa, have_fa, fb, 1)) {
+            double wn = have_fa ? next_step(b - a, fa, fb) : 4.0 * (b - a);
+            a = b;
+            fa = fb;
+            beta = fmin(0.95, fmax(0.05, wn / (r - a)));
+        } else {
+            beta *= 0.5;
+            if (beta * d <= CERT_RES * a) {
+                r = scan_generic(&S, a, fa, 1, lo);
+                break;
+            }
+        }
+    }
+    g_cert_steps += S.steps;
+    return r;
+}
+
+long long orc_cert_steps(void) { return g_cert_steps; }
+
+void orc_residual_enclosure(double dt, double qup, double quc, double qdp, double ql,
+                            const orc_chparams_f64 *p, double a, double b, double *lo, double *hi)
+{
+    chan_t c;
+    chan_init(p, &c);
+    route_ctx x = {dt, qup, quc, qdp, ql, &c};
+    int flood = c.twcc > 0.0 && c.bfd > 0.01 && c.bfd < CERT_HMAX && a >= c.bfd;
+    f_enclosure_mv(&x, a, b, flood, lo, hi);
+}
+
+int orc_residual_slope_enclosure(double dt, double qup, double quc, double qdp, double ql,
+                                 const orc_chparams_f64 *p, double a, double b, double *lo, double *hi)
+{
+    chan_t c;
+    chan_init(p, &c);
+    route_ctx x = {dt, qup, quc, qdp, ql, &c};
+    int flood = c.twcc > 0.0 && c.bfd > 0.01 && c.bfd < CERT_HMAX && a >= c.bfd;
+    iv_t Fd;
+    if (!fprime_enclosure(&x, a, b, flood, &Fd)) return 0;
+    *lo = Fd.lo;
+    *hi = Fd.hi;
+    return 1;
 }
 
 double orc_mc_solve_converged(double dt, double qup, double quc, double qdp, double ql,
@@ -527,8 +1043,7 @@ double orc_mc_solve_converged(double dt, double qup, double quc, double qdp, dou
         return 0.0;
     }
     route_ctx x = {dt, qup, quc, qdp, ql, &c};
-    double bp = (c.twcc > 0.0) ? c.bfd : -1.0;
-    double h = leftmost_downcrossing(route_residual_fn, &x, 0.01, bp);
+    double h = certified_leftmost_downcrossing(&x, 0.01);
     residual_at(h, dt, qup, quc, qdp, ql, &c, &C);
     *velc = velocity(h, &c);
     *depthc = h;

@@ -142,6 +142,20 @@ double orc_mc_solve_converged(double dt, double qup, double quc, double qdp, dou
                               const orc_chparams_f64 *p, double depthp,
                               double *depthc, double *velc);
 
+/* Enclosure [*lo, *hi] of the converged residual F over depths [a, b] (both in
+ * the main channel, or both at or above bankfull on a floodplain); the
+ * certified scan of orc_mc_solve_converged is built on it. */
+void orc_residual_enclosure(double dt, double qup, double quc, double qdp, double ql,
+                            const orc_chparams_f64 *p, double a, double b, double *lo, double *hi);
+
+/* Enclosure [*lo, *hi] of dF/dh over [a, b] (same regimes); returns 0 where
+ * none is formed (the celerity may vanish on [a, b]). */
+int orc_residual_slope_enclosure(double dt, double qup, double quc, double qdp, double ql,
+                                 const orc_chparams_f64 *p, double a, double b, double *lo, double *hi);
+
+/* Enclosures evaluated by certified scans so far (diagnostics). */
+long long orc_cert_steps(void);
+
 /* orc_wavefront_f32in on n_threads host threads (OpenMP over the reaches of
  * one level); bit for bit the one-thread result. */
 void orc_wavefront_f32in_mt(const orc_chparams_f32 *params,

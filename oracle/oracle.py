@@ -63,6 +63,10 @@ def lib(variant: str = "libm") -> C.CDLL:
         L.orc_mc_residual.argtypes = [D] * 5 + [pP64, D, pD]
         L.orc_mc_solve_converged.restype = D
         L.orc_mc_solve_converged.argtypes = [D] * 5 + [pP64, D, pD, pD]
+        L.orc_residual_enclosure.restype = None
+        L.orc_residual_enclosure.argtypes = [D] * 5 + [pP64, D, D, pD, pD]
+        L.orc_residual_slope_enclosure.restype = I
+        L.orc_residual_slope_enclosure.argtypes = [D] * 5 + [pP64, D, D, pD, pD]
         L.orc_normal_depth.restype = D
         L.orc_normal_depth.argtypes = [D, pP64]
         L.orc_manning_flow.restype = D
@@ -192,6 +196,22 @@ def residual(params8, qup, quc, qdp, ql, h, dt=300.0, variant="libm"):
     qmc = D()
     f = lib(variant).orc_mc_residual(dt, qup, quc, qdp, ql, C.byref(p), h, C.byref(qmc))
     return f, qmc.value
+
+
+def residual_enclosure(params8, qup, quc, qdp, ql, a, b, dt=300.0, variant="libm"):
+    """Enclosure (lo, hi) of the converged residual over depths [a, b]."""
+    p = P64(*map(float, params8))
+    lo, hi = D(), D()
+    lib(variant).orc_residual_enclosure(dt, qup, quc, qdp, ql, C.byref(p), a, b, C.byref(lo), C.byref(hi))
+    return lo.value, hi.value
+
+
+def residual_slope_enclosure(params8, qup, quc, qdp, ql, a, b, dt=300.0, variant="libm"):
+    """Enclosure (lo, hi) of dF/dh over [a, b], or None where none is formed."""
+    p = P64(*map(float, params8))
+    lo, hi = D(), D()
+    ok = lib(variant).orc_residual_slope_enclosure(dt, qup, quc, qdp, ql, C.byref(p), a, b, C.byref(lo), C.byref(hi))
+    return (lo.value, hi.value) if ok else None
 
 
 def solve_converged(params8, qup, quc, qdp, ql, depthp, dt=300.0, variant="libm"):

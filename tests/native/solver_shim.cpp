@@ -68,7 +68,7 @@ extern "C" int shim_solve_learned(const double *segc, const double *in, double d
     return o.iters | (o.flags << 24);
 }
 
-// What a LEARNED solve hands to the network after its first evaluation:
+// What a LEARNED solve hands to the network when it asks for a guess:
 // out = {anchor depth, routed flow there, Manning flow of the anchor};
 // returns 1 when the solve needs no guess (guard).
 extern "C" int shim_learned_anchor(const double *segc, const double *in, double dt, double *out)
@@ -77,7 +77,9 @@ extern "C" int shim_learned_anchor(const double *segc, const double *in, double 
     const ConvCfg k = {dt, 1e-13, 60, 12};
     SecantConv s;
     if (s.start(in[0], in[1], in[2], in[3], c, in[4], true)) return 1;
-    s.step(c, k);
+    // F at the previous depth (after F(mindepth) where that may be <= 0)
+    while (!s.need_guess)
+        if (s.step(c, k)) return 1;
     out[0] = s.guess_anchor();
     out[1] = s.guess_flow_new();
     out[2] = s.guess_flow_old();

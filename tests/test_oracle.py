@@ -246,23 +246,32 @@ def _bankfull(p):
 
 
 def test_converged_root_is_the_leftmost_downcrossing_on_a_dense_scan():
-    """In bank, no point of a dense scan below h* has F > 0 followed by F <= 0,
-    and F is positive just below h*; h* = 1 cm only when F is never positive.
-    (Just above bankfull F can dip in a band narrower than the oracle's x1.25
-    scan; the scan visits bankfull itself and steps over such a band, so the
-    dense check stops at bankfull.)  Half the cases are sharply rising
-    hydrographs, where F(1 cm) <= 0."""
+    """No point of a dense scan below h* -- in bank and on the floodplain, with
+    bankfull and the first point above it visited -- has F > 0 followed by
+    F <= 0; F is positive just below h*; h* = 1 cm only when F is positive at
+    no scanned point.  Half the cases are sharply rising hydrographs, where
+    F(1 cm) <= 0; a third of the others have floodplain flows."""
     rng = np.random.default_rng(12)
-    rising = 0
-    for i in range(80):
+    rising = overbank = 0
+    for i in range(90):
         p = _rand_params(rng)
         Q = 10 ** rng.uniform(-1, 3)
-        x = (Q, Q * 1.05, Q * 0.95, 0.2) if i % 2 else (Q * 4.0, Q * 6.0, Q * 0.05, 0.05)
+        if i % 2 == 0:
+            x = (Q * 4.0, Q * 6.0, Q * 0.05, 0.05)
+        elif i % 3 == 0:
+            x = (Q * 30, Q * 31, Q * 29, 0.2)
+        else:
+            x = (Q, Q * 1.05, Q * 0.95, 0.2)
         q, h, v = orc.solve_converged(p, *x, 1.0)
-        top = min(h, _bankfull(p)) if h > 0.01 else 1e3
-        grid = np.geomspace(0.01, top, 600)[:-1]
+        bfd = _bankfull(p)
+        top = h if h > 0.01 else 1e3
+        grid = np.geomspace(0.01, top, 1500)[:-1]
+        if bfd < top:
+            grid = np.unique(np.concatenate([grid, [bfd, bfd * (1 + 1e-12)]]))
+            grid = grid[grid < top]
         f = np.array([orc.residual(p, *x, g)[0] for g in grid])
         rising += f[0] <= 0
+        overbank += h > bfd
         down = (f[:-1] > 0) & (f[1:] <= 0)
         assert not down.any(), "a downcrossing exists below the converged depth"
         if h > 0.01:
@@ -270,6 +279,73 @@ def test_converged_root_is_the_leftmost_downcrossing_on_a_dense_scan():
         else:
             assert np.all(f <= 0)
     assert rising >= 5, "the rising cases should reach F(1 cm) <= 0"
+    assert overbank >= 3, "some roots should lie above bankfull"
+
+
+def test_converged_root_on_cases_a_sample_search_gets_wrong():
+    """tests/golden/converged_cases.npz: residuals whose leftmost downcrossing
+    a search over sample points can miss (a positive band narrower than a x1.25
+    depth grid while F(1 cm) <= 0; roots above bankfull; an in-bank root below
+    a floodplain one; bands just above bankfull), each with its downcrossing
+    from a brute-force scan of 20,000 points per decade.  The oracle's
+    certified search lands on every one; the x1.25 grid search (the search a
+    band can defeat) returns 1 cm on the narrow-band cases."""
+    import sys
+    from conftest import GOLDEN
+    sys.path.insert(0, GOLDEN)
+    import make_converged_cases as M
+    z = golden("converged_cases.npz")
+    dt = float(z["dt"])
+    assert set(z["cat"].tolist()) == {0, 1, 2, 3, 4}
+    for p, x, root, cat in zip(z["params"], z["inputs"], z["root"], z["cat"]):
+        q, h, v = orc.solve_converged(p, *x, 1.0, dt=dt)
+        assert abs(h - root) <= 1e-12 * root, (cat, h, root)
+        if cat == 0:
+            assert M.grid125_root(p, x) == 0.01 < root
+        if cat in (2, 4):
+            assert root > _bankfull(p)
+        if cat == 3:
+            assert root <= _bankfull(p)
+
+
+def test_residual_enclosure_contains_the_residual():
+    """The certified search rests on the enclosure of F over an interval:
+    every sampled F on [a, b] lies inside it, in bank and on the floodplain
+    (from bankfull, the limit from above), for rising hydrographs and negative
+    lateral inflow too; it tightens as the interval shrinks."""
+    rng = np.random.default_rng(21)
+    widths = {}
+    for case in range(400):
+        p = _rand_params(rng)
+        if rng.random() < 0.1:
+            p[3] = 0.0                                   # no floodplain
+        Q = 10 ** rng.uniform(-2, 3)
+        qup, quc, qdp = Q * np.exp(rng.normal(0, 0.5, 3))
+        if rng.random() < 0.3:
+            quc = qup + qdp * rng.uniform(0.5, 3)         # F(1 cm) may be <= 0
+        ql = rng.lognormal(np.log(0.1) - 0.5, 1) * (1 if rng.random() < 0.9 else -1)
+        bfd = _bankfull(p)
+        for k in range(6):
+            rel = 10 ** rng.uniform(-6, 0)
+            flood = p[3] > 0 and rng.random() < 0.5
+            if flood:
+                a = bfd if rng.random() < 0.3 else bfd * (1 + rng.uniform(0, 2))
+                b = a * (1 + rel)
+            else:
+                a = np.exp(rng.uniform(np.log(0.01), np.log(bfd if p[3] > 0 else 50)))
+                b = a * (1 + rel)
+                if p[3] > 0:
+                    b = min(b, bfd)
+                if not b > a:
+                    continue
+            lo, hi = orc.residual_enclosure(p, qup, quc, qdp, ql, a, b)
+            hs = a + (b - a) * rng.random(30)
+            hs = np.concatenate([hs, [b] if not flood else [b], [] if (flood and a == bfd) else [a]])
+            for h in hs:
+                f = orc.residual(p, qup, quc, qdp, ql, h)[0]
+                assert lo <= f <= hi, (case, a, b, h, f, lo, hi)
+            widths.setdefault(rel < 1e-3, []).append((hi - lo) / (abs(qup) + abs(quc) + abs(qdp)))
+    assert np.median(widths[True]) < 1e-2 * np.median(widths[False])
 
 
 def test_normal_depth_inverts_manning():

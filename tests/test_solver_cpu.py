@@ -166,11 +166,12 @@ def test_learned_guess_cuts_evaluations(shim):
         ev["plain"].append(n_plain)
         seg = segc_row(p)
         anc = np.zeros(3)
-        assert shim.shim_learned_anchor(seg.ctypes.data_as(pD), x.ctypes.data_as(pD), 300.0,
+        # a solve whose F(1 cm) <= 0 takes the certified scan and asks for no guess
+        asks = shim.shim_learned_anchor(seg.ctypes.data_as(pD), x.ctypes.data_as(pD), 300.0,
                                         anc.ctypes.data_as(pD)) == 0
         d, q_new, q_old = anc
         for name, Wn in (("learned", W), ("zero network", Wz)):
-            g = d * np.exp(R.rating_log_depth(Wn, q_new, p)[0] - R.rating_log_depth(Wn, q_old, p)[0])
+            g = d * np.exp(R.rating_log_depth(Wn, q_new, p)[0] - R.rating_log_depth(Wn, q_old, p)[0]) if asks else -1.0
             o = np.zeros(3)
             r = shim.shim_solve_learned(seg.ctypes.data_as(pD), x.ctypes.data_as(pD), 300.0, 1e-13, 60, 12,
                                         float(g), o.ctypes.data_as(pD))
@@ -180,3 +181,31 @@ def test_learned_guess_cuts_evaluations(shim):
     means = {k: float(np.mean(v)) for k, v in ev.items()}
     print("mean residual evaluations per solve:", means)
     assert means["learned"] < 0.8 * means["plain"]
+
+
+def test_device_solver_on_cases_a_sample_search_gets_wrong(shim):
+    """The library's CONVERGED and LEARNED solves (host build of the device
+    code) on tests/golden/converged_cases.npz: narrow positive bands while
+    F(1 cm) <= 0 (the solver's corner bound flags these, F(1 cm) is evaluated
+    and the certified scan runs), roots above bankfull and in-bank roots below
+    floodplain ones.  Every start -- the reference bracket, a guess at the
+    root, a guess 50 times too deep, no usable guess -- ends on the
+    brute-force downcrossing."""
+    z = golden("converged_cases.npz")
+    dt = float(z["dt"])
+    pD = C.POINTER(C.c_double)
+    for i, (p, x, root, cat) in enumerate(zip(z["params"], z["inputs"], z["root"], z["cat"])):
+        want_q = orc.solve_converged(p, *x, 1.0, dt=dt)[0]
+        for dp in (root, 1.0, 0.05):
+            xi = np.ascontiguousarray(list(x) + [dp], np.float64)
+            out, ev, flags = run(shim, 1, p, xi)
+            assert not (flags & 1), (i, cat, "capped")
+            assert abs(out[1] - root) <= 1e-9 * root, (i, cat, dp, out, root)
+            assert abs(out[0] - want_q) <= 1e-9 * max(abs(want_q), 1e-3), (i, cat, dp, out, want_q)
+            seg = segc_row(p)
+            for guess in (root * 1.001, root * 50.0, -1.0):
+                o = np.zeros(3)
+                r = shim.shim_solve_learned(seg.ctypes.data_as(pD), xi.ctypes.data_as(pD), dt, 1e-13, 60, 12,
+                                            float(guess), o.ctypes.data_as(pD))
+                assert not ((r >> 24) & 1), (i, cat, "capped")
+                assert abs(o[1] - root) <= 1e-9 * root, (i, cat, dp, guess, o, root)
