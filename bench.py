@@ -503,6 +503,7 @@ def main():
                     help="segments per unit (default: the workload's, 4; 8 for linear-1m)")
     ap.add_argument("--block", type=int, default=64)
     ap.add_argument("--batch-iters", type=int, default=12)
+    ap.add_argument("--tol", type=float, default=1e-13, help="CONVERGED/LEARNED relative depth tolerance")
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--schedule", type=int, default=0, help="0 auto, 1 skewed, 2 level, 3 dataflow")
     ap.add_argument("--autotune", type=int, default=-1,
@@ -556,7 +557,7 @@ def main():
         weights, mlp_rec = train_mlp(trmc, args)
 
     opts = dict(schedule=args.schedule, unit_segments=args.unit_segments, block_size=args.block,
-                grid_blocks=args.grid, batch_iters=args.batch_iters)
+                grid_blocks=args.grid, batch_iters=args.batch_iters, tol=args.tol)
     tuned = None
     n_probe = args.autotune if args.autotune >= 0 else (16 if spec["kind"] == "batch" else 0)
     if n_probe > 0:

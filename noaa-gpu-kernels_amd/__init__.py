@@ -212,6 +212,13 @@ def _ptr(a: np.ndarray, ct):
     return a.ctypes.data_as(C.POINTER(ct))
 
 
+def _need(a: np.ndarray, n: int, what: str, at_least: bool = False) -> None:
+    """The C side reads n values of a: refuse a shorter (or, unless
+    at_least, a differently sized) array before any device work."""
+    if (a.size < n) if at_least else (a.size != n):
+        raise TrmcError(ERR_ARG, f"{what} holds {a.size} values, needs {'at least ' if at_least else ''}{n}")
+
+
 def params_f64(p: np.ndarray) -> np.ndarray:
     """[N, 8] parameter matrix in ChParams field order -> contiguous float64."""
     p = _f64(p)
@@ -492,6 +499,15 @@ def device_count() -> int:
     return n.value
 
 
+def _check_route_inputs(topo, p, q, qd, dp, rl, n_timesteps):
+    N, R = topo.n_segments, topo.n_reaches
+    _need(p, 8 * N, "params")
+    _need(q, int(n_timesteps) * N, "qlat_ts [T, N]", at_least=True)
+    _need(qd, N, "qdp0_seg")
+    _need(dp, N, "dp0_seg")
+    _need(rl, R, "reach_last0")
+
+
 def wavefront_fp64(params32, topo: Topology, qlat_ts, qdp0_seg, dp0_seg, reach_last0,
                    n_timesteps: int, dt: float = 300.0):
     """Drop-in for cpu_wavefront_fp64: float32 inputs, FP64 routing on the GPU.
@@ -499,6 +515,7 @@ def wavefront_fp64(params32, topo: Topology, qlat_ts, qdp0_seg, dp0_seg, reach_l
     Returns (qdc_seg_out_d [N], reach_last_out_d [R]) as float64."""
     p = params_f32(params32)
     q, qd, dp, rl = _f32(qlat_ts), _f32(qdp0_seg), _f32(dp0_seg), _f32(reach_last0)
+    _check_route_inputs(topo, p, q, qd, dp, rl, n_timesteps)
     qout = np.zeros(topo.n_segments, np.float64)
     rout = np.zeros(topo.n_reaches, np.float64)
     _check(lib().trmc_wavefront_f32in_f64(
@@ -524,6 +541,13 @@ def linear_mc(coeffs, topo: Topology, qlat_ts, Q0, n_timesteps: int):
     Returns Q_final [N] float64."""
     cs = [_f32(c) for c in coeffs]
     q, q0 = _f32(qlat_ts), _f32(Q0)
+    N = topo.n_segments
+    if len(cs) != 4:
+        raise TrmcError(ERR_ARG, "coeffs must be (C1, C2, C3, C4/ql)")
+    for i, c in enumerate(cs):
+        _need(c, N, f"coefficient {i + 1}")
+    _need(q, int(n_timesteps) * N, "qlat_ts [T, N]", at_least=True)
+    _need(q0, N, "Q0")
     out = np.zeros(topo.n_segments, np.float64)
     _check(lib().trmc_linear_mc_f64(*[_ptr(c, C.c_float) for c in cs], *topo.c_args(), _ptr(q, C.c_float),
                                     _ptr(q0, C.c_float), _ptr(out, C.c_double), int(n_timesteps)))
@@ -574,6 +598,8 @@ def matrix_secant_coeffs(params, topo: "Topology", qlat0, dt: float = 300.0):
     p = params_f32(params)
     q = _f32(qlat0)
     R = topo.n_reaches
+    _need(p, 8 * R, "params (one segment per reach)")
+    _need(q, topo.n_segments, "qlat0")
     out = [np.zeros(R, np.float32) for _ in range(4)]
     ptr, lptr, lrch = _i32(topo.reach_up_ptr), _i32(topo.level_ptr), _i32(topo.level_reach)
     idx = _i32(topo.reach_up_idx) if len(topo.reach_up_idx) else np.zeros(1, np.int32)
@@ -588,9 +614,15 @@ def matrix_build(up_ptr, up_idx, coeffs, tol: float = 1e-6):
     """build_propagation_matrices (linear_mc_matrix.cu:118-229): (P_Q, P_q), each
     (rowptr, col, val) numpy arrays."""
     ptr, idx = _i32(up_ptr), _i32(up_idx)
+    if ptr.size < 1 or ptr[0] != 0 or int(ptr[-1]) > idx.size or np.any(np.diff(ptr) < 0):
+        raise TrmcError(ERR_ARG, "up_ptr must be a CSR row pointer over up_idx")
     if len(idx) == 0:
         idx = np.zeros(1, np.int32)
     cs = [_f32(c) for c in coeffs]
+    if len(cs) != 4:
+        raise TrmcError(ERR_ARG, "coeffs must be (C1, C2, C3, C4/ql)")
+    for i, c in enumerate(cs):
+        _need(c, len(ptr) - 1, f"coefficient {i + 1}")
     mQ, mq = Csr(), Csr()
     _check(lib().trmc_matrix_build(len(ptr) - 1, _ptr(ptr, C.c_int), _ptr(idx, C.c_int),
                                    *[_ptr(c, C.c_float) for c in cs], float(tol), C.byref(mQ), C.byref(mq)))
@@ -621,6 +653,10 @@ def matrix_route(P_Q, P_q, qlat, Q0, n_timesteps: int, lanes_per_row: int = 0, r
     cQ, kQ = _csr_view(P_Q)
     cq, kq = _csr_view(P_q)
     q, q0 = _f32(qlat), _f32(Q0)
+    if cq.n != cQ.n:
+        raise TrmcError(ERR_ARG, "P_Q and P_q must have the same number of rows")
+    _need(q, int(n_timesteps) * cQ.n, "qlat [T, n]", at_least=True)
+    _need(q0, cQ.n, "Q0")
     out = np.zeros(cQ.n, np.float32)
     if reps > 0:
         ms = C.c_double()
@@ -745,6 +781,7 @@ def wavefront_f64(params, topo: Topology, qlat_ts, qdp0_seg, dp0_seg, reach_last
     """FP64-end-to-end variant of the drop-in; options as trmc_options fields."""
     p = params_f64(params)
     q, qd, dp, rl = _f64(qlat_ts), _f64(qdp0_seg), _f64(dp0_seg), _f64(reach_last0)
+    _check_route_inputs(topo, p, q, qd, dp, rl, n_timesteps)
     qout = np.zeros(topo.n_segments, np.float64)
     rout = np.zeros(topo.n_reaches, np.float64)
     o = default_options(**options)
@@ -761,6 +798,11 @@ def reach_parallel(params, ql, rs, rl, qu, qd, dp, uq: float = 10.0, dt: float =
     ql, qu, qd, dp = _f64(ql), _f64(qu), _f64(qd), _f64(dp)
     rs, rl = _i32(rs), _i32(rl)
     n = p.shape[0]
+    for a, what in ((ql, "ql"), (qu, "qu"), (qd, "qd"), (dp, "dp")):
+        _need(a, n, what)
+    _need(rl, rs.size, "rl")
+    if rs.size and (rs.min() < 0 or rl.min() < 0 or int((rs.astype(np.int64) + rl).max()) > n):
+        raise TrmcError(ERR_ARG, "reach segments lie outside the segment arrays")
     qo, vo, dpo = np.zeros(n), np.zeros(n), np.zeros(n)
     _check(lib().trmc_reach_parallel_f64(
         _ptr(p, ChParamsF64), _ptr(ql, C.c_double), _ptr(rs, C.c_int), _ptr(rl, C.c_int),
@@ -776,6 +818,7 @@ def mc_solve(params, inputs, dt: float = 300.0, mode: int = MODE_REFERENCE):
     p = params_f64(params)
     x = _f64(inputs).reshape(-1, 5)
     n = x.shape[0]
+    _need(p, 8 * n, "params (one row per solve)")
     out = np.zeros((n, 3), np.float64)
     it = np.zeros(n, np.int32)
     _check(lib().trmc_mc_solve_f64(_ptr(p, ChParamsF64), _ptr(x, C.c_double), _ptr(out, C.c_double),

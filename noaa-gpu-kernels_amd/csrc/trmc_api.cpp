@@ -151,6 +151,10 @@ struct trmc_network {
     int trace_n = 0;
     float *d_mlp = nullptr;
     double *d_lin = nullptr;             // LINEAR coefficients [4][n_dev_segs]
+    unsigned long long *d_scanq = nullptr;   // certified-solve queue: control words, ring, answer tickets
+    double *d_scan_data = nullptr;           // requests [lanes][5], answers [lanes][3]
+    size_t scan_lanes = 0;
+    int scan_cap = 0;
     unsigned *d_bar = nullptr;
     int *d_status = nullptr, *d_t = nullptr;
     unsigned long long *d_counters = nullptr;
@@ -168,6 +172,7 @@ struct trmc_network {
         dev_free(d_ring); dev_free(d_hist); dev_free(d_mlp); dev_free(d_bar); dev_free(d_status);
         dev_free(d_t); dev_free(d_counters); dev_free(d_q0); dev_free(d_d0); dev_free(d_ring0);
         dev_free(d_trace_t); dev_free(d_trace_it); dev_free(d_flow_hist); dev_free(d_lin);
+        dev_free(d_scanq); dev_free(d_scan_data);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -640,13 +645,41 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
         else HIPCHK(route_dataflow_occupancy(mode, o.block_size, &bpcu));
         if (bpcu <= 0) return fail(TRMC_ERR_HIP, "dataflow kernel cannot be resident");
         long long resident = (long long)bpcu * net->cu_count;
+        // CONVERGED/LEARNED on the interleaved kernel: the last blocks serve
+        // the certified solves (one per 32 CUs, at least 2), resident with the rest
+        const bool servers = il && (mode == TRMC_MODE_CONVERGED || mode == TRMC_MODE_LEARNED);
+        const long long n_servers = servers ? std::max(2, net->cu_count / 32) : 0;
+        if (resident <= n_servers) return fail(TRMC_ERR_HIP, "dataflow kernel cannot be resident");
         long long need_blocks = ((long long)um + o.block_size - 1) / o.block_size;
-        long long grid = std::max(1LL, std::min(need_blocks, resident));
-        if (o.grid_blocks > 0) grid = std::min<long long>(o.grid_blocks, resident);
+        long long grid = std::max(1LL, std::min(need_blocks, resident - n_servers));
+        if (o.grid_blocks > 0) grid = std::min<long long>(o.grid_blocks, resident - n_servers);
         // every lane owns one member of a unit sequence: nthreads % M == 0
         while (grid > 1 && ((long long)grid * o.block_size) % net->M != 0) --grid;
         if (((long long)grid * o.block_size) % net->M != 0)
             return fail(TRMC_ERR_ARG, "n_members %d does not divide the dataflow grid", net->M);
+        a.n_route_blocks = (int)grid;
+        if (servers) {
+            const size_t lanes = (size_t)grid * o.block_size;
+            if (lanes >= ((size_t)1 << kScanLaneBits))
+                return fail(TRMC_ERR_ARG, "dataflow grid of %zu lanes exceeds the certified-solve queue", lanes);
+            int cap = 1024;
+            while ((size_t)cap < 2 * lanes) cap *= 2;
+            if (lanes > net->scan_lanes || cap > net->scan_cap) {
+                if ((rc = dev_alloc(&net->d_scanq, (size_t)kScanCtlWords + cap + lanes))) return rc;
+                if ((rc = dev_alloc(&net->d_scan_data, lanes * 8))) return rc;
+                net->scan_lanes = lanes;
+                net->scan_cap = cap;
+            }
+            // control words, ring and answer tickets start at 0 in every launch
+            HIPCHK(launch_reset((unsigned *)net->d_scanq, (int)(2 * ((size_t)kScanCtlWords + cap + lanes)), s));
+            a.scan_cap = cap;
+            a.scan_ctl = net->d_scanq;
+            a.scan_ring = net->d_scanq + kScanCtlWords;
+            a.scan_resp_seq = net->d_scanq + kScanCtlWords + cap;
+            a.scan_req = net->d_scan_data;
+            a.scan_resp = net->d_scan_data + lanes * 5;
+            grid += n_servers;
+        }
         a.flow_hist = net->d_flow_hist;
         HIPCHK(launch_hist_init(net->d_flow_hist, net->d_ring, net->ring_shift % P.ring, (long long)um, T, s));
         HIPCHK(hipEventRecord(net->ev0, s));

@@ -72,15 +72,31 @@ struct RouteArgs {
     int batch_iters;                // DATAFLOW interleaved: secant iterations per batch
     int guess_model;                // LEARNED: 0 network, 1 power law alone
     const double *lin;              // LINEAR: [4][n_dev_segs] C1, C2, C3, C4/ql
+    // DATAFLOW interleaved, CONVERGED/LEARNED: blocks >= n_route_blocks are
+    // server waves that run the certified solves routing lanes hand them
+    // (solves whose F(mindepth) <= 0); see k_route_dataflow_il.
+    int n_route_blocks;
+    int scan_cap;                   // ring entries (a power of two)
+    unsigned long long *scan_ctl;   // [0] tickets issued, [32] tickets claimed, [64] routing waves done
+    unsigned long long *scan_ring;  // [scan_cap] (ticket + 1) << 20 | lane, 1 once read, 0 never used
+    unsigned long long *scan_resp_seq;  // [routing lanes] ticket + 1 once answered
+    double *scan_req;               // [routing lanes][5] qup, quc, qdp, ql, device segment
+    double *scan_resp;              // [routing lanes][3] root depth, outflow, evaluations
 };
+
+// scan queue layout in one allocation of 64-bit words
+static constexpr int kScanCtlWords = 96;
+static constexpr int kScanLaneBits = 20;      // routing lanes < 2^20
 
 // launchers (trmc_kernels.hip); all asynchronous on `stream`
 hipError_t launch_stream_copy(const void *src, void *dst, size_t bytes, hipStream_t stream);
 // diffusive wave: one reach's computational nodes and step parameters
 struct DiffReach { int ncomp; float theta, dtini; };
-// node arrays [nreach][500] floats; ws >= 500 * nreach floats
+// node arrays [nreach][500] floats; ws >= 500 * nreach floats; a reach whose
+// ncomp lies outside 1..500 is skipped and counted in *bad (when non-null)
 hipError_t launch_diffusive_tridiag(const DiffReach *rp, const float *dx, const float *diffusivity,
-                                    const float *qp_prev, float *qp_out, float *ws, int nreach, hipStream_t stream);
+                                    const float *qp_prev, float *qp_out, float *ws, int nreach, hipStream_t stream,
+                                    unsigned *bad = nullptr);
 // params: n ChParams of 8 floats (dx, bw, tw, twcc, n_ch, ncc, cs, s0); out [4][n]
 hipError_t launch_linear_coeffs(const float *params, int n, float dt, int fixed_x, float *out, hipStream_t stream);
 // matrix-form linearized MC step over the union rows of P_Q and P_q:

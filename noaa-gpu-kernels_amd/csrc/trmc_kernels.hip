@@ -387,6 +387,97 @@ __global__ void __launch_bounds__(256) k_route_dataflow(RouteArgs a)
 // more than the evaluations it saves
 constexpr float kNetMinLogRatio = 0.2f;
 
+
+// ------------------------------------------------- certified-solve servers
+// A CONVERGED/LEARNED solve whose F(mindepth) <= 0 needs the certified scan
+// (trmc_solver.h), whose interval arithmetic would add its registers to the
+// routing loop if it ran there.  Routing lanes hand such solves to server
+// waves (the last blocks of the same cooperative launch) and wait, as they
+// wait for upstream outflows:
+//   request:  the lane writes its inputs to its own slot, takes a ticket
+//             (scan_ctl[0]) and publishes (ticket + 1) << 20 | lane in ring
+//             entry ticket mod scan_cap with release order, once the entry's
+//             previous occupant has been read (entry 0 or 1);
+//   serve:    a server lane claims the next ticket (scan_ctl[32]), waits for
+//             its entry (acquire), reads the request, marks the entry read
+//             (1), solves, writes the answer and then scan_resp_seq[lane] =
+//             ticket + 1 (release);
+//   end:      every routing wave counts itself in scan_ctl[64] when it leaves
+//             its loop (a routing lane leaves only with its answer), and a
+//             server lane whose ticket has no request ends once all have.
+// Every wait also ends when the launch's status leaves OK.
+__device__ __forceinline__ void scan_post(const RouteArgs &a, long long lane, int seg, const SecantConv &sr,
+                                          unsigned long long &ticket)
+{
+    double *rq = a.scan_req + (size_t)lane * 5;
+    rq[0] = sr.qup;
+    rq[1] = sr.quc;
+    rq[2] = sr.qdp;
+    rq[3] = sr.ql;
+    rq[4] = (double)seg;
+    ticket = __hip_atomic_fetch_add(&a.scan_ctl[0], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long *e = &a.scan_ring[ticket & (unsigned long long)(a.scan_cap - 1)];
+    // the entry's previous occupant (ticket - scan_cap) must have been read
+    const long long t0 = (long long)wall_clock64();
+    while (__hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 1ull) {
+        if ((long long)wall_clock64() - t0 > a.timeout_ticks) {
+            __hip_atomic_store(a.status, (int)KSTATUS_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __hip_atomic_store(e, ((ticket + 1ull) << kScanLaneBits) | (unsigned long long)lane, __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int MODE>
+__device__ void scan_server(const RouteArgs &a)
+{
+    // One loop for waiting and serving: a lane whose request has come serves
+    // it while the wave's other lanes go on waiting (a wait loop of its own
+    // would hold every lane of the wave until all of their requests came).
+    const unsigned long long route_waves = (unsigned long long)a.n_route_blocks * ((blockDim.x + 63) / 64);
+    const ConvCfg kc = {a.dt, a.tol, a.max_eval, a.learned_budget};
+    const unsigned long long mask = (unsigned long long)(a.scan_cap - 1);
+    unsigned long long my =
+        __hip_atomic_fetch_add(&a.scan_ctl[32], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        unsigned long long *e = &a.scan_ring[my & mask];
+        const unsigned long long v = __hip_atomic_load(e, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if ((v >> kScanLaneBits) == my + 1ull) {
+            const long long lane = (long long)(v & ((1ull << kScanLaneBits) - 1ull));
+            const double *rq = a.scan_req + (size_t)lane * 5;
+            const double qup = rq[0], quc = rq[1], qdp = rq[2], ql = rq[3];
+            const int seg = (int)rq[4];
+            __hip_atomic_store(e, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);   // entry read
+            const Seg c = load_seg(a.segc, (size_t)a.n_dev_segs, seg);
+            SecantConv sv;
+            double h = 0.01, q = 0.0;
+            int n = 0;
+            if (!sv.start(qup, quc, qdp, ql, c, 0.0, false)) {
+                sv.stage = 8;                     // F(mindepth) first: the requester found it <= 0
+                while (!sv.step(c, kc)) {
+                }
+                h = sv.root_h;
+                q = sv.root_q;
+                n = sv.evals + sv.scan_n;
+            }
+            double *rs = a.scan_resp + (size_t)lane * 3;
+            rs[0] = h;
+            rs[1] = q;
+            rs[2] = (double)n;
+            __hip_atomic_store(&a.scan_resp_seq[lane], my + 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            my = __hip_atomic_fetch_add(&a.scan_ctl[32], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            continue;
+        }
+        // no request for this ticket yet: none will come once every routing wave is done
+        if (__hip_atomic_load(&a.scan_ctl[64], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= route_waves ||
+            __hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != KSTATUS_OK)
+            return;
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
 template <int MODE> struct IlSolver { typedef SecantConv type; };
 template <> struct IlSolver<0> { typedef SecantRef type; };
 template <> struct IlSolver<3> { typedef SecantRef type; };   // LINEAR: no iteration (unused)
@@ -394,6 +485,12 @@ template <> struct IlSolver<3> { typedef SecantRef type; };   // LINEAR: no iter
 template <int MODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_route_dataflow_il(RouteArgs a)
 {
+    if constexpr (MODE == 1 || MODE == 2) {
+        if ((int)blockIdx.x >= a.n_route_blocks) {   // a server block (whole block: no barrier below is split)
+            scan_server<MODE>(a);
+            return;
+        }
+    }
     __shared__ float sW[MODE == 2 ? kMlpParams : 1];
     if constexpr (MODE == 2) {
         // the weights exist only for the network guess (guess_model 0)
@@ -403,13 +500,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     }
     const long long t_start = (long long)wall_clock64();
     const int M = a.n_members, T = a.n_steps, U = a.n_units;
-    const long long nthreads = (long long)gridDim.x * blockDim.x;
+    const long long nthreads = (long long)a.n_route_blocks * blockDim.x;
     const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long S = nthreads / M;
     const int m = (int)(gtid % M);
     const long long base = gtid / M;
     const size_t UM = (size_t)U * M;
     const size_t nd = (size_t)a.n_dev_segs;
+    unsigned long long scan_ticket = 0;
+    bool scan_posted = false;
+    long long scan_t0 = 0;
     Acc acc = {0, 0, 0, 0, 0};
     long long wait_ticks = 0;
     long long j_lo = 0, j_hi = -1, j = 0;
@@ -434,7 +534,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             return sr.start(a.dt, qup, quc, qdp, ql, c, dp);
         } else {
             if (sr.start(qup, quc, qdp, ql, c, dp, MODE == 2)) return true;
-            if constexpr (MODE == 2) sr.step(c, kc);   // F at the previous depth: the flow to guess for
+            if constexpr (MODE == 2) sr.template step<true>(c, kc);   // F at the previous depth: the flow to guess for
             return false;
         }
     };
@@ -605,17 +705,40 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                 }
             }
         }
+        // ------------- CONVERGED/LEARNED: certified solves handed to the servers
+        if constexpr (MODE == 1 || MODE == 2) {
+            if (phase == PH_SOLVE && sr.awaiting_scan()) {
+                if (!scan_posted) {
+                    scan_post(a, gtid, s, sr, scan_ticket);
+                    scan_posted = true;
+                    scan_t0 = (long long)wall_clock64();
+                } else if (__hip_atomic_load(&a.scan_resp_seq[gtid], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) ==
+                           scan_ticket + 1ull) {
+                    const double *rs = a.scan_resp + (size_t)gtid * 3;
+                    sr.take_scan_result(rs[0], rs[1], (int)rs[2]);
+                    scan_posted = false;
+                    progressed = true;
+                    close_segment(sr.finish(c), cur_qdp);
+                } else if ((long long)wall_clock64() - scan_t0 > a.timeout_ticks) {
+                    __hip_atomic_store(a.status, (int)KSTATUS_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
         // -------------------------------------- (2) solve phase, registers only
         // a wave solving a unit on the longest dependency chain issues first
-        if (__any(crit && phase == PH_SOLVE)) __builtin_amdgcn_s_setprio(2);
+        auto solving = [&]() -> bool {
+            if constexpr (MODE == 1 || MODE == 2) return phase == PH_SOLVE && !sr.awaiting_scan();
+            else return phase == PH_SOLVE;
+        };
+        if (__any(crit && solving())) __builtin_amdgcn_s_setprio(2);
         else __builtin_amdgcn_s_setprio(0);
         for (int r = 0; r < a.batch_iters; ++r) {
-            if (!__any(phase == PH_SOLVE)) break;
-            if (phase == PH_SOLVE) {
+            if (!__any(solving())) break;
+            if (solving()) {
                 progressed = true;
                 bool fin;
                 if constexpr (MODE == 0 || MODE == 3) fin = sr.step(c);
-                else fin = sr.step(c, kc);
+                else fin = sr.template step<true>(c, kc);
                 if (fin) close_segment(sr.finish(c), cur_qdp);
             }
         }
@@ -626,6 +749,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             if (__hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != KSTATUS_OK) break;
             __builtin_amdgcn_s_sleep(1);
         }
+    }
+    if constexpr (MODE == 1 || MODE == 2) {      // this routing wave is done: servers may end
+        if ((threadIdx.x & 63) == 0)
+            __hip_atomic_fetch_add(&a.scan_ctl[64], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     flush_acc(a, acc, wait_ticks, (long long)wall_clock64() - t_start);
 }
@@ -692,7 +819,7 @@ __global__ void k_step_advance(int *t_dev)
 
 __global__ void k_reset(unsigned *w, int n)
 {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) w[i] = 0u;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) w[i] = 0u;
 }
 
 // Independent reaches, one step (troute_mc_final.cu:142-164): one thread per
@@ -808,7 +935,8 @@ hipError_t launch_gen_forcing(double *dst, const int *seg_of_dev, int nd, int N,
 
 hipError_t launch_reset(unsigned *bar, int nwords, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, stream, bar, nwords);
+    const int blocks = std::max(1, std::min(1024, (nwords + 255) / 256));
+    hipLaunchKernelGGL(k_reset, dim3(blocks), dim3(256), 0, stream, bar, nwords);
     return hipGetLastError();
 }
 
@@ -945,11 +1073,15 @@ __global__ void __launch_bounds__(256) k_diffusive_tridiag(const DiffReach *__re
                                                            const float *__restrict__ diffusivity,
                                                            const float *__restrict__ qp_prev,
                                                            float *__restrict__ qp_out, float *__restrict__ ws,
-                                                           int nreach)
+                                                           int nreach, unsigned *bad)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nreach) return;
     const int nc = rp[j].ncomp;
+    if (nc < 1 || nc > kDiffMaxNodes) {       // only device-resident reach records reach here unchecked
+        if (bad) atomicAdd(bad, 1u);
+        return;
+    }
     const float theta = rp[j].theta, dtini = rp[j].dtini;
     const size_t base = (size_t)j * kDiffMaxNodes;
     const float *q = qp_prev + base;
@@ -1103,11 +1235,12 @@ hipError_t launch_matrix_step(int n_blocks, const int4 *blocks, const int4 *slot
 }
 
 hipError_t launch_diffusive_tridiag(const DiffReach *rp, const float *dx, const float *diffusivity,
-                                    const float *qp_prev, float *qp_out, float *ws, int nreach, hipStream_t stream)
+                                    const float *qp_prev, float *qp_out, float *ws, int nreach, hipStream_t stream,
+                                    unsigned *bad)
 {
     if (nreach <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_diffusive_tridiag, dim3((nreach + 63) / 64), dim3(64), 0, stream, rp, dx, diffusivity,
-                       qp_prev, qp_out, ws, nreach);
+                       qp_prev, qp_out, ws, nreach, bad);
     return hipGetLastError();
 }
 

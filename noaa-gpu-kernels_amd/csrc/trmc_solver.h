@@ -834,7 +834,7 @@ struct SecantConv {
 
     // stage: 0 eval ha, 1 eval hb, 2 secant, 3 bankfull check, 4 settle at
     // mindepth, 5 done, 7 re-evaluate hi_neg, 8 F(mindepth) first,
-    // 9 certified scan
+    // 9 certified scan, 11 waiting for a deferred certified solve
     TRMC_HD bool start(double qup_, double quc_, double qdp_, double ql_, const Seg &c, double depthp,
                        bool learned_)
     {
@@ -949,10 +949,29 @@ struct SecantConv {
 
     TRMC_HD void accept_root(const Seg &c) { accept_at(h_cur, q_cur, c); }
 
-    // returns true when the solve is finished (MLP-guess requests leave it running)
+    // A certified solve done elsewhere (kDefer): its root, the outflow there
+    // and the evaluations it took.
+    TRMC_HD void take_scan_result(double h, double q, int n_evals)
+    {
+        root_h = h;
+        root_q = q;
+        have_root = true;
+        evals += n_evals;
+        stage = 5;
+    }
+
+    TRMC_HD bool awaiting_scan() const { return stage == 11; }
+
+    // returns true when the solve is finished (MLP-guess requests leave it
+    // running).  kDefer: a solve whose F(mindepth) <= 0 does not run the
+    // certified scan here but waits in stage 11 for take_scan_result (the
+    // routing kernel hands it to separate waves, which keeps the scan's
+    // registers out of its hot loop).
+    template <bool kDefer = false>
     TRMC_HD bool step(const Seg &c, const ConvCfg &k)
     {
         if (stage == 5) return true;
+        if (stage == 11) return false;           // waiting for a deferred certified solve
         // the cap counts evaluations outside the certified scan (which has its own)
         if (stage != 9 && evals - scan_n >= k.max_eval) {
             capped = true;
@@ -1088,75 +1107,79 @@ struct SecantConv {
             const double f = eval(0.01, c, k);
             if (f > 0.0) {
                 stage = 0;
+            } else if constexpr (kDefer) {
+                stage = 11;
             } else {
                 begin_scan(0.01, false);
                 fa = f;
             }
             return false;
         }
-        case 9: {
-            // Certified scan (mirrors the oracle's) from a = ha, where F has
-            // the sign spos and the value fa, with step hb: F(b) at b = a + hb
-            // (not crossing bankfull); the other sign there means a sign
-            // change in (a, b]: at the resolution kScanRes a, b is the first
-            // point of that sign, else b moves back toward the secant estimate
-            // of the crossing; the same sign and the sign shown on all of
-            // [a, b] (sign_holds) move a to b with a Newton-like next step;
-            // otherwise the step halves, and at the resolution a moves on.
-            // Looking for F > 0 the scan ends at mindepth once Qmn on [a, inf)
-            // exceeds every routed flow; the first F <= 0 after a positive
-            // point closes the bracket the secant then solves.
-            if (++scan_n > kScanCap) {
-                capped = true;
-                if (spos) hi_neg = ha;
-                stage = spos ? 7 : 4;
-                return false;
-            }
-            const double a = ha, w = hb;
-            if (!spos) {
-                const double rmax = routed_max(k.dt, qup, quc, qdp, ql);
-                if (a >= 1.0e4 || qmn_floor(a, c) > rmax + 1e-12 * (fabs(rmax) + 1.0)) { stage = 4; return false; }
-            } else if (a >= 1.0e4) {
-                hi_neg = 1.0e4;
-                stage = 7;
-                return false;
-            }
-            const bool has_flood = c.twcc > 0.0 && c.bfd > 0.01 && c.bfd < 1.0e4;
-            double b = fmin(a + w, 1.0e4);
-            if (has_flood && a < c.bfd && b > c.bfd) b = c.bfd;
-            const bool flood = has_flood && a >= c.bfd;
-            const bool have_fa = !(flood && a <= c.bfd);
-            const double fbv = eval(b, c, k);
-            if (spos ? !(fbv > 0.0) : fbv > 0.0) {          // a sign change in (a, b]
-                if (b - a <= kScanRes * a) {
-                    if (spos) {                              // the bracket [a, b]
-                        lo_pos = a;
-                        hi_neg = b;
-                        neg_seen = true;
-                        certified = true;
-                        hb = b; fb = fbv; qb = q_cur;
-                        s_prev = s_prev2 = 1.0e30;
-                        stage = 2;
+        case 9:
+            if constexpr (!kDefer) {
+                // Certified scan (mirrors the oracle's) from a = ha, where F has
+                // the sign spos and the value fa, with step hb: F(b) at b = a + hb
+                // (not crossing bankfull); the other sign there means a sign
+                // change in (a, b]: at the resolution kScanRes a, b is the first
+                // point of that sign, else b moves back toward the secant estimate
+                // of the crossing; the same sign and the sign shown on all of
+                // [a, b] (sign_holds) move a to b with a Newton-like next step;
+                // otherwise the step halves, and at the resolution a moves on.
+                // Looking for F > 0 the scan ends at mindepth once Qmn on [a, inf)
+                // exceeds every routed flow; the first F <= 0 after a positive
+                // point closes the bracket the secant then solves.
+                if (++scan_n > kScanCap) {
+                    capped = true;
+                    if (spos) hi_neg = ha;
+                    stage = spos ? 7 : 4;
+                    return false;
+                }
+                const double a = ha, w = hb;
+                if (!spos) {
+                    const double rmax = routed_max(k.dt, qup, quc, qdp, ql);
+                    if (a >= 1.0e4 || qmn_floor(a, c) > rmax + 1e-12 * (fabs(rmax) + 1.0)) { stage = 4; return false; }
+                } else if (a >= 1.0e4) {
+                    hi_neg = 1.0e4;
+                    stage = 7;
+                    return false;
+                }
+                const bool has_flood = c.twcc > 0.0 && c.bfd > 0.01 && c.bfd < 1.0e4;
+                double b = fmin(a + w, 1.0e4);
+                if (has_flood && a < c.bfd && b > c.bfd) b = c.bfd;
+                const bool flood = has_flood && a >= c.bfd;
+                const bool have_fa = !(flood && a <= c.bfd);
+                const double fbv = eval(b, c, k);
+                if (spos ? !(fbv > 0.0) : fbv > 0.0) {          // a sign change in (a, b]
+                    if (b - a <= kScanRes * a) {
+                        if (spos) {                              // the bracket [a, b]
+                            lo_pos = a;
+                            hi_neg = b;
+                            neg_seen = true;
+                            certified = true;
+                            hb = b; fb = fbv; qb = q_cur;
+                            s_prev = s_prev2 = 1.0e30;
+                            stage = 2;
+                            return false;
+                        }
+                        spos = true;
+                        ha = b; fa = fbv; qa = q_cur;
                         return false;
                     }
-                    spos = true;
+                    const double t = (have_fa && fa != fbv) ? (b - a) * fa / (fa - fbv) : 0.5 * (b - a);
+                    hb = fmax(0.5 * kScanRes * a, fmin(0.5 * (b - a), 0.98 * t));
+                    return false;
+                }
+                if (sign_holds(a, b, flood, fa, have_fa, fbv, spos, k.dt, qup, quc, qdp, ql, c)) {
+                    hb = fmax(kScanRes * b, have_fa ? scan_next_step(b - a, fa, fbv) : 4.0 * (b - a));
                     ha = b; fa = fbv; qa = q_cur;
                     return false;
                 }
-                const double t = (have_fa && fa != fbv) ? (b - a) * fa / (fa - fbv) : 0.5 * (b - a);
-                hb = fmax(0.5 * kScanRes * a, fmin(0.5 * (b - a), 0.98 * t));
+                if (b - a > kScanRes * a) { hb = 0.5 * (b - a); return false; }
+                ha = b; fa = fbv; qa = q_cur;                    // at the resolution: move on
+                hb = 2.0 * w;
                 return false;
             }
-            if (sign_holds(a, b, flood, fa, have_fa, fbv, spos, k.dt, qup, quc, qdp, ql, c)) {
-                hb = fmax(kScanRes * b, have_fa ? scan_next_step(b - a, fa, fbv) : 4.0 * (b - a));
-                ha = b; fa = fbv; qa = q_cur;
-                return false;
-            }
-            if (b - a > kScanRes * a) { hb = 0.5 * (b - a); return false; }
-            ha = b; fa = fbv; qa = q_cur;                    // at the resolution: move on
-            hb = 2.0 * w;
             return false;
-        }
         }
         return true;
     }

@@ -362,6 +362,33 @@ void orc_wavefront_f32in_mt(const orc_chparams_f32 *params,
                     qdc_seg_out_d, reach_last_out_d, n_timesteps, dt, iter_sum, n_threads < 1 ? 1 : n_threads);
 }
 
+static void wavefront_f64(const orc_chparams_f64 *params,
+                          const int *reach_seg_start, const int *reach_seg_len,
+                          const int *reach_up_ptr, const int *reach_up_idx,
+                          const int *level_ptr, const int *level_reach,
+                          int n_reaches, int n_levels, int n_segments,
+                          const double *qlat_ts, const double *qdp0_seg,
+                          const double *dp0_seg, const double *reach_last0,
+                          double *qdc_seg_out, double *reach_last_out,
+                          double *depth_out, double *vel_out, double *reach_hist,
+                          int n_timesteps, double dt, int mode, long long *iter_sum, int n_threads)
+{
+    double *q = (double *)malloc(((size_t)n_segments + 1) * sizeof(double));
+    double *d = (double *)malloc(((size_t)n_segments + 1) * sizeof(double));
+    double *rl = (double *)malloc(((size_t)n_reaches + 1) * sizeof(double));
+    memcpy(q, qdp0_seg, (size_t)n_segments * sizeof(double));
+    memcpy(d, dp0_seg, (size_t)n_segments * sizeof(double));
+    memcpy(rl, reach_last0, (size_t)n_reaches * sizeof(double));
+    if (vel_out) memset(vel_out, 0, (size_t)n_segments * sizeof(double));
+    route_dag(params, reach_seg_start, reach_seg_len, reach_up_ptr, reach_up_idx,
+              level_ptr, level_reach, n_reaches, n_levels, n_segments,
+              NULL, qlat_ts, q, d, rl, vel_out, reach_hist, n_timesteps, dt, mode, iter_sum, n_threads);
+    memcpy(qdc_seg_out, q, (size_t)n_segments * sizeof(double));
+    memcpy(reach_last_out, rl, (size_t)n_reaches * sizeof(double));
+    if (depth_out) memcpy(depth_out, d, (size_t)n_segments * sizeof(double));
+    free(q); free(d); free(rl);
+}
+
 void orc_wavefront_f64(const orc_chparams_f64 *params,
                        const int *reach_seg_start, const int *reach_seg_len,
                        const int *reach_up_ptr, const int *reach_up_idx,
@@ -373,20 +400,25 @@ void orc_wavefront_f64(const orc_chparams_f64 *params,
                        double *depth_out, double *vel_out, double *reach_hist,
                        int n_timesteps, double dt, int mode, long long *iter_sum)
 {
-    double *q = (double *)malloc(((size_t)n_segments + 1) * sizeof(double));
-    double *d = (double *)malloc(((size_t)n_segments + 1) * sizeof(double));
-    double *rl = (double *)malloc(((size_t)n_reaches + 1) * sizeof(double));
-    memcpy(q, qdp0_seg, (size_t)n_segments * sizeof(double));
-    memcpy(d, dp0_seg, (size_t)n_segments * sizeof(double));
-    memcpy(rl, reach_last0, (size_t)n_reaches * sizeof(double));
-    if (vel_out) memset(vel_out, 0, (size_t)n_segments * sizeof(double));
-    route_dag(params, reach_seg_start, reach_seg_len, reach_up_ptr, reach_up_idx,
-              level_ptr, level_reach, n_reaches, n_levels, n_segments,
-              NULL, qlat_ts, q, d, rl, vel_out, reach_hist, n_timesteps, dt, mode, iter_sum, 1);
-    memcpy(qdc_seg_out, q, (size_t)n_segments * sizeof(double));
-    memcpy(reach_last_out, rl, (size_t)n_reaches * sizeof(double));
-    if (depth_out) memcpy(depth_out, d, (size_t)n_segments * sizeof(double));
-    free(q); free(d); free(rl);
+    wavefront_f64(params, reach_seg_start, reach_seg_len, reach_up_ptr, reach_up_idx, level_ptr, level_reach,
+                  n_reaches, n_levels, n_segments, qlat_ts, qdp0_seg, dp0_seg, reach_last0, qdc_seg_out,
+                  reach_last_out, depth_out, vel_out, reach_hist, n_timesteps, dt, mode, iter_sum, 1);
+}
+
+void orc_wavefront_f64_mt(const orc_chparams_f64 *params,
+                          const int *reach_seg_start, const int *reach_seg_len,
+                          const int *reach_up_ptr, const int *reach_up_idx,
+                          const int *level_ptr, const int *level_reach,
+                          int n_reaches, int n_levels, int n_segments,
+                          const double *qlat_ts, const double *qdp0_seg,
+                          const double *dp0_seg, const double *reach_last0,
+                          double *qdc_seg_out, double *reach_last_out,
+                          double *depth_out, double *vel_out, double *reach_hist,
+                          int n_timesteps, double dt, int mode, long long *iter_sum, int n_threads)
+{
+    wavefront_f64(params, reach_seg_start, reach_seg_len, reach_up_ptr, reach_up_idx, level_ptr, level_reach,
+                  n_reaches, n_levels, n_segments, qlat_ts, qdp0_seg, dp0_seg, reach_last0, qdc_seg_out,
+                  reach_last_out, depth_out, vel_out, reach_hist, n_timesteps, dt, mode, iter_sum, n_threads);
 }
 
 /* troute_mc_final.cu:167-252.  The inlined solve there is the same

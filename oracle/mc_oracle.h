@@ -126,6 +126,19 @@ void orc_wavefront_f64(const orc_chparams_f64 *params,
                        double *depth_out, double *vel_out, double *reach_hist,
                        int n_timesteps, double dt, int mode, long long *iter_sum);
 
+/* orc_wavefront_f64 on n_threads host threads (OpenMP over the reaches of one
+ * level); bit for bit the one-thread result. */
+void orc_wavefront_f64_mt(const orc_chparams_f64 *params,
+                          const int *reach_seg_start, const int *reach_seg_len,
+                          const int *reach_up_ptr, const int *reach_up_idx,
+                          const int *level_ptr, const int *level_reach,
+                          int n_reaches, int n_levels, int n_segments,
+                          const double *qlat_ts, const double *qdp0_seg,
+                          const double *dp0_seg, const double *reach_last0,
+                          double *qdc_seg_out, double *reach_last_out,
+                          double *depth_out, double *vel_out, double *reach_hist,
+                          int n_timesteps, double dt, int mode, long long *iter_sum, int n_threads);
+
 /* cpu_mc: independent reaches, one step, constant upstream inflow uq into
  * segment 0 of every reach. */
 void orc_reach_parallel(const orc_chparams_f64 *p, const double *ql,

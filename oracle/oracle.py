@@ -57,6 +57,8 @@ def lib(variant: str = "libm") -> C.CDLL:
         L.orc_wavefront_f32in_mt.argtypes = [pP32] + topo + [pF] * 4 + [pD, pD, I, D, C.POINTER(LL), I]
         L.orc_wavefront_f64.restype = None
         L.orc_wavefront_f64.argtypes = [pP64] + topo + [pD] * 4 + [pD] * 5 + [I, D, I, C.POINTER(LL)]
+        L.orc_wavefront_f64_mt.restype = None
+        L.orc_wavefront_f64_mt.argtypes = [pP64] + topo + [pD] * 4 + [pD] * 5 + [I, D, I, C.POINTER(LL), I]
         L.orc_reach_parallel.restype = None
         L.orc_reach_parallel.argtypes = [pP64, pD, pI, pI, pD, pD, pD, pD, pD, pD, D, D, I, C.POINTER(LL)]
         L.orc_mc_residual.restype = D
@@ -158,7 +160,7 @@ def wavefront_f32in(topo, params32, qlat32, qdp0, dp0, reach_last0, n_timesteps,
 
 
 def wavefront_f64(topo, params, qlat, qdp0, dp0, reach_last0, n_timesteps, dt=300.0,
-                  mode=MODE_REFERENCE, history=False, variant="libm"):
+                  mode=MODE_REFERENCE, history=False, variant="libm", threads=1):
     """All-double DAG routing.  Returns dict with qdc, depth, vel, reach_last,
     iterations and (history=True) reach_hist [T, R]."""
     t = as_topo(topo)
@@ -170,10 +172,13 @@ def wavefront_f64(topo, params, qlat, qdp0, dp0, reach_last0, n_timesteps, dt=30
     qo, do, vo, ro = np.zeros(N), np.zeros(N), np.zeros(N), np.zeros(R)
     hist = np.zeros((int(n_timesteps), R)) if history else None
     its = LL(0)
-    lib(variant).orc_wavefront_f64(_ptr(p, P64), *targs, _ptr(q, D), _ptr(q0, D), _ptr(d0, D), _ptr(r0, D),
-                            _ptr(qo, D), _ptr(ro, D), _ptr(do, D), _ptr(vo, D),
-                            _ptr(hist, D) if history else pD(), int(n_timesteps), float(dt), int(mode),
-                            C.byref(its))
+    args = [_ptr(p, P64), *targs, _ptr(q, D), _ptr(q0, D), _ptr(d0, D), _ptr(r0, D),
+            _ptr(qo, D), _ptr(ro, D), _ptr(do, D), _ptr(vo, D),
+            _ptr(hist, D) if history else pD(), int(n_timesteps), float(dt), int(mode), C.byref(its)]
+    if threads > 1:
+        lib(variant).orc_wavefront_f64_mt(*args, int(threads))
+    else:
+        lib(variant).orc_wavefront_f64(*args)
     out = dict(qdc=qo, depth=do, vel=vo, reach_last=ro, iterations=its.value)
     if history:
         out["reach_hist"] = hist

@@ -213,3 +213,37 @@ def test_network_rejects_unknown_guess_model():
     with pytest.raises(trmc.TrmcError) as e:
         trmc.Network(t, mode=trmc.MODE_LEARNED, guess_model=2)
     assert e.value.code == trmc.ERR_ARG
+
+
+def test_wrappers_refuse_short_arrays_before_device_work():
+    """Arrays the C side would read past their end are argument errors raised
+    by the wrappers (no device is needed to see it)."""
+    trmc = load_trmc()
+    w = trmc.make_workload(2, 2000, 4, n_reaches=600, seed=3)
+    N, R = w.topo.n_segments, w.topo.n_reaches
+    p32 = w.params.astype(np.float32)
+    q = w.qlat[0]
+    bad_calls = [
+        lambda: trmc.wavefront_fp64(p32, w.topo, q[:3], w.qdp0, w.dp0, w.reach_last0, 4),      # T rows short
+        lambda: trmc.wavefront_fp64(p32, w.topo, q, w.qdp0[:-1], w.dp0, w.reach_last0, 4),
+        lambda: trmc.wavefront_f64(w.params, w.topo, q, w.qdp0, w.dp0, w.reach_last0[:-1], 4),
+        lambda: trmc.linear_mc([np.zeros(N, np.float32)] * 3 + [np.zeros(N - 1, np.float32)], w.topo, q, w.qdp0, 4),
+        lambda: trmc.linear_mc([np.zeros(N, np.float32)] * 4, w.topo, q[:2], w.qdp0, 4),
+        lambda: trmc.matrix_build(w.topo.reach_up_ptr, w.topo.reach_up_idx, [np.zeros(R, np.float32)] * 3 +
+                                  [np.zeros(R - 5, np.float32)]),
+        lambda: trmc.matrix_route((np.array([0, 1, 2]), np.array([0, 1]), np.ones(2, np.float32)),
+                                  (np.array([0, 1, 2]), np.array([0, 1]), np.ones(2, np.float32)),
+                                  np.zeros(5, np.float32), np.zeros(2, np.float32), 3),           # qlat short
+        lambda: trmc.matrix_route((np.array([0, 1, 2]), np.array([0, 1]), np.ones(2, np.float32)),
+                                  (np.array([0, 1, 2]), np.array([0, 1]), np.ones(2, np.float32)),
+                                  np.zeros(6, np.float32), np.zeros(1, np.float32), 3),           # Q0 short
+        lambda: trmc.reach_parallel(w.params[:10], np.zeros(9), [0], [10], np.zeros(10), np.zeros(10),
+                                    np.zeros(10)),
+        lambda: trmc.reach_parallel(w.params[:10], np.zeros(10), [0], [11], np.zeros(10), np.zeros(10),
+                                    np.zeros(10)),
+        lambda: trmc.mc_solve(w.params[:3], np.zeros((4, 5))),
+    ]
+    for i, call in enumerate(bad_calls):
+        with pytest.raises(trmc.TrmcError) as e:
+            call()
+        assert e.value.code == trmc.ERR_ARG, i
