@@ -280,6 +280,22 @@ int trmc_diffusive_tridiag_timed_f32(const trmc_diff_reach *rp, const float *dx,
                                      const float *celerity, const float *qp_prev, float *qp_out, int nreach,
                                      int reps, double *avg_kernel_ms);
 
+/* [new] No counterpart in the reference: for callers that cannot allocate
+ * device memory (serving loops, harnesses that forbid hipMalloc).  The step of
+ * trmc_diffusive_tridiag_f32 with device-resident inputs and outputs (device
+ * pointers, same layout; stream = a hipStream_t, NULL for the default stream).
+ * Its scratch (the elimination factors, a count of invalid reach records)
+ * lives in a static device array of 32.8 MB per GPU, held until the process
+ * ends, so the call allocates nothing.  At most TRMC_DIFF_NOALLOC_MAX_REACHES
+ * reaches; more is TRMC_ERR_ARG before any device work.  Synchronous: returns
+ * after the step; a reach record with ncomp outside 1..TRMC_DIFF_MAX_NODES
+ * leaves that reach's nodes untouched and the call returns TRMC_ERR_ARG.
+ * Calls that overlap on one GPU share its scratch and race. */
+#define TRMC_DIFF_NOALLOC_MAX_REACHES 16384
+int trmc_diffusive_tridiag_f32_noalloc(const trmc_diff_reach *d_rp, const float *d_dx, const float *d_diffusivity,
+                                       const float *d_celerity, const float *d_qp_prev, float *d_qp_out,
+                                       int nreach, void *stream);
+
 /* Forcing ingest (data_parsers/parse_qlat_fast_cpp.cpp:76-224): lateral inflow
  * from a directory of per-nexus CSVs "nex-<id>_output.csv" (rows "timestep,
  * time, value"), pivoted to qlat [n_timesteps][n_reaches] float with reach r

@@ -166,6 +166,7 @@ def _declare(L: C.CDLL) -> None:
         "trmc_linear_coeffs_f32": (I, [_p(ChParamsF32), I, C.c_float, pF, pF, pF, pF]),
         "trmc_linear_mc_f64": (I, [pF] * 4 + topo + [pF, pF, pD, I]),
         "trmc_diffusive_tridiag_f32": (I, [V] + [pF] * 5 + [I]),
+        "trmc_diffusive_tridiag_f32_noalloc": (I, [V] * 6 + [I, V]),
         "trmc_ingest_nex_csv": (I, [C.c_char_p, C.POINTER(C.c_int64), I, I, I, pF, C.POINTER(C.c_int64)]),
         "trmc_ingest_nex_csv_to_forcings": (I, [C.c_char_p] * 3 + [I, I, C.POINTER(C.c_int64)]),
         "trmc_ingest_nex_csv_rules": (I, [C.c_char_p, C.POINTER(C.c_int64), I, I, I, I, pF, C.POINTER(C.c_int64)]),
@@ -774,6 +775,18 @@ def diffusive_tridiag(rp, dx, diffusivity, celerity, qp_prev, qp_out=None, reps:
         return out, ms.value
     _check(lib().trmc_diffusive_tridiag_f32(rp.ctypes.data, *ptrs, len(rp)))
     return out
+
+
+DIFF_NOALLOC_MAX_REACHES = 16384
+
+
+def diffusive_tridiag_noalloc(d_rp: int, d_dx: int, d_diffusivity: int, d_celerity: int, d_qp_prev: int,
+                              d_qp_out: int, nreach: int, stream: int = 0) -> None:
+    """trmc_diffusive_tridiag_f32_noalloc: the diffusive step on device-resident
+    arrays (device addresses as ints, e.g. a torch tensor's data_ptr()),
+    allocating nothing; at most DIFF_NOALLOC_MAX_REACHES reaches."""
+    _check(lib().trmc_diffusive_tridiag_f32_noalloc(d_rp, d_dx, d_diffusivity, d_celerity, d_qp_prev, d_qp_out,
+                                                    int(nreach), stream or None))
 
 
 def wavefront_f64(params, topo: Topology, qlat_ts, qdp0_seg, dp0_seg, reach_last0,
