@@ -149,6 +149,7 @@ def time_mode(trmc, w, M, T, dt, name, opts, args, weights=None):
         net.run(T, dt)
     trmc.synchronize()
     kernel_ms, iters, fallbacks, capped, mlp_calls, mlp_frac, stats = [], 0, 0, 0, 0, 0.0, {}
+    certified = 0
     wait_frac = 0.0
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -159,6 +160,7 @@ def time_mode(trmc, w, M, T, dt, name, opts, args, weights=None):
         fallbacks += stats["fallbacks"]
         capped += stats["not_converged"]
         mlp_calls += stats["mlp_calls"]
+        certified += stats.get("certified_solves", 0)
         mlp_frac += stats["mlp_frac"]
         wait_frac += stats["barrier_wait_frac"]
     trmc.synchronize()
@@ -168,6 +170,7 @@ def time_mode(trmc, w, M, T, dt, name, opts, args, weights=None):
            "kernel_ms": float(np.mean(kernel_ms)),
            "evaluations_per_seg_step": iters / (seg_steps * args.steps),
            "fallbacks": fallbacks // args.steps, "not_converged": capped // args.steps,
+           "certified_solves": certified // args.steps,
            "grid_blocks": stats.get("grid_blocks"),
            # the dataflow schedule has no grid barrier: the share of lane time
            # spent waiting for upstream outflows takes the barrier stall's
@@ -178,7 +181,7 @@ def time_mode(trmc, w, M, T, dt, name, opts, args, weights=None):
         rec["network_time_frac"] = mlp_frac / args.steps
     log(f"[bench] {name}: {rec['ms_per_step']:.2f} ms/pass, kernel {rec['kernel_ms']:.2f} ms, "
         f"{rec['value'] / 1e9:.3f} G seg-steps/s, evaluations/seg-step {rec['evaluations_per_seg_step']:.3f}, "
-        f"fallbacks {rec['fallbacks']}, capped {rec['not_converged']}")
+        f"fallbacks {rec['fallbacks']}, capped {rec['not_converged']}, certified {rec['certified_solves']}")
     return rec, net
 
 

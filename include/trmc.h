@@ -128,6 +128,8 @@ typedef struct {
     double barrier_wait_frac;/* fraction of block-time spent waiting in barriers */
     long long mlp_calls;     /* LEARNED: wave-wide network forwards (64 samples each) */
     double mlp_frac;         /* LEARNED: fraction of wave-time spent in them */
+    long long certified_solves; /* CONVERGED/LEARNED: solves with F(mindepth) <= 0 done by the
+                                   certified scan (on the server waves of the DATAFLOW kernel) */
 } trmc_run_stats;
 
 /* ---------------------------------------------------------------- basics */

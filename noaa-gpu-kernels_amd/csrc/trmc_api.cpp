@@ -646,9 +646,9 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
         if (bpcu <= 0) return fail(TRMC_ERR_HIP, "dataflow kernel cannot be resident");
         long long resident = (long long)bpcu * net->cu_count;
         // CONVERGED/LEARNED on the interleaved kernel: the last blocks serve
-        // the certified solves (one per 32 CUs, at least 2), resident with the rest
+        // the certified solves (one per 8 CUs, at least 2), resident with the rest
         const bool servers = il && (mode == TRMC_MODE_CONVERGED || mode == TRMC_MODE_LEARNED);
-        const long long n_servers = servers ? std::max(2, net->cu_count / 32) : 0;
+        const long long n_servers = servers ? std::max(2, net->cu_count / 8) : 0;
         if (resident <= n_servers) return fail(TRMC_ERR_HIP, "dataflow kernel cannot be resident");
         long long need_blocks = ((long long)um + o.block_size - 1) / o.block_size;
         long long grid = std::max(1LL, std::min(need_blocks, resident - n_servers));
@@ -763,6 +763,15 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
     unsigned long long cnt[CNT_COUNT];
     HIPCHK(hipMemcpy(&status, net->d_status, sizeof(int), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(cnt, net->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    if (a.scan_ctl) {
+        unsigned long long issued = 0;
+        HIPCHK(hipMemcpy(&issued, a.scan_ctl, sizeof(issued), hipMemcpyDeviceToHost));
+        st.certified_solves = (long long)issued;
+        if (issued && std::getenv("TRMC_DEBUG_SCAN"))
+            std::fprintf(stderr, "[trmc] certified solves %llu: request-to-answer %.1f us, server solve %.1f us (mean)\n",
+                         issued, cnt[CNT_SCAN_WAIT_TICKS] / (net->wall_khz * 1e-3) / issued,
+                         cnt[CNT_SCAN_SOLVE_TICKS] / (net->wall_khz * 1e-3) / issued);
+    }
     if (status == KSTATUS_TIMEOUT)
         return fail(TRMC_ERR_TIMEOUT, "persistent routing kernel: barrier wait exceeded 30 s");
     net->t_cursor += T;

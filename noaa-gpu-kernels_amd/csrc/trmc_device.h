@@ -29,6 +29,8 @@ enum CounterSlot {
     CNT_BLOCK_TICKS,    // total wall-clock ticks summed over blocks
     CNT_MLP_CALLS,      // LEARNED: wave-wide network forwards
     CNT_MLP_TICKS,      // LEARNED: wall-clock ticks inside them, summed over waves
+    CNT_SCAN_WAIT_TICKS,   // certified solves: ticks from request to answer, summed over requests
+    CNT_SCAN_SOLVE_TICKS,  // certified solves: ticks a server lane spends solving, summed
     CNT_COUNT
 };
 

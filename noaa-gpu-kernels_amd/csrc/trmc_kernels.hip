@@ -431,7 +431,7 @@ __device__ __forceinline__ void scan_post(const RouteArgs &a, long long lane, in
 }
 
 template <int MODE>
-__device__ void scan_server(const RouteArgs &a)
+__device__ __forceinline__ void scan_server(const RouteArgs &a)
 {
     // One loop for waiting and serving: a lane whose request has come serves
     // it while the wave's other lanes go on waiting (a wait loop of its own
@@ -439,6 +439,9 @@ __device__ void scan_server(const RouteArgs &a)
     const unsigned long long route_waves = (unsigned long long)a.n_route_blocks * ((blockDim.x + 63) / 64);
     const ConvCfg kc = {a.dt, a.tol, a.max_eval, a.learned_budget};
     const unsigned long long mask = (unsigned long long)(a.scan_cap - 1);
+    // a routing lane waits on every answer: serve ahead of the routing waves
+    // that share the CU (idle servers sleep, so they take little from them)
+    __builtin_amdgcn_s_setprio(3);
     unsigned long long my =
         __hip_atomic_fetch_add(&a.scan_ctl[32], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
@@ -450,6 +453,7 @@ __device__ void scan_server(const RouteArgs &a)
             const double qup = rq[0], quc = rq[1], qdp = rq[2], ql = rq[3];
             const int seg = (int)rq[4];
             __hip_atomic_store(e, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);   // entry read
+            const long long ts0 = (long long)wall_clock64();
             const Seg c = load_seg(a.segc, (size_t)a.n_dev_segs, seg);
             SecantConv sv;
             double h = 0.01, q = 0.0;
@@ -467,6 +471,7 @@ __device__ void scan_server(const RouteArgs &a)
             rs[1] = q;
             rs[2] = (double)n;
             __hip_atomic_store(&a.scan_resp_seq[lane], my + 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            atomicAdd(&a.counters[CNT_SCAN_SOLVE_TICKS], (unsigned long long)((long long)wall_clock64() - ts0));
             my = __hip_atomic_fetch_add(&a.scan_ctl[32], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             continue;
         }
@@ -474,7 +479,9 @@ __device__ void scan_server(const RouteArgs &a)
         if (__hip_atomic_load(&a.scan_ctl[64], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= route_waves ||
             __hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != KSTATUS_OK)
             return;
-        __builtin_amdgcn_s_sleep(2);
+        // requests are rare: poll every few microseconds, so that idle
+        // servers take no memory bandwidth or issue slots from the routing
+        __builtin_amdgcn_s_sleep(127);
     }
 }
 
@@ -534,7 +541,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             return sr.start(a.dt, qup, quc, qdp, ql, c, dp);
         } else {
             if (sr.start(qup, quc, qdp, ql, c, dp, MODE == 2)) return true;
-            if constexpr (MODE == 2) sr.template step<true>(c, kc);   // F at the previous depth: the flow to guess for
+            if constexpr (MODE == 2) sr.first_step(c, kc);   // F at the previous depth: the flow to guess for
             return false;
         }
     };
@@ -716,6 +723,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                            scan_ticket + 1ull) {
                     const double *rs = a.scan_resp + (size_t)gtid * 3;
                     sr.take_scan_result(rs[0], rs[1], (int)rs[2]);
+                    atomicAdd(&a.counters[CNT_SCAN_WAIT_TICKS],
+                              (unsigned long long)((long long)wall_clock64() - scan_t0));
                     scan_posted = false;
                     progressed = true;
                     close_segment(sr.finish(c), cur_qdp);

@@ -608,9 +608,11 @@ TRMC_HD double qmn_floor(double h, const Seg &c)
 TRMC_HD bool f_mindepth_positive(double qup, double quc, double qdp, double ql, const Seg &c)
 {
     if (!(c.bfd >= 0.01)) return false;
+    // (products by the constants' roundings differ from the corners by an ulp
+    // or so; the margin below covers that)
     const double r = fmin(fmin(0.6 * qup + 0.2 * quc + 0.2 * qdp + 0.8 * ql, qup + ql),
-                          fmin(qdp + (qup - quc) / 3.0, qdp + (qup - quc)));
-    const double qmn_ub = (c.bw + 0.01 * c.z) * 0.01 * 0.046415888336127789 * c.sqs0 / c.n_ch;
+                          fmin(qdp + (qup - quc) * (1.0 / 3.0), qdp + (qup - quc)));
+    const double qmn_ub = (c.bw + 0.01 * c.z) * (0.01 * 0.046415888336127789) * c.ks;   // ks = sqrt(s0) / n
     return r > qmn_ub * (1.0 + 1e-9) + 1e-13 * (fabs(qup) + fabs(quc) + fabs(qdp) + fabs(ql));
 }
 
@@ -756,13 +758,11 @@ TRMC_HD Enclosure slope_bounds(double w, double fa, bool have_fa, double fb, dou
     return e;
 }
 
-// F keeps its sign (pos: F > 0, else F <= 0) on [a, b]: the direct enclosure,
-// then (only if that fails) the slope bounds.
-TRMC_HD bool sign_holds(double a, double b, bool flood, double fa, bool have_fa, double fb, bool pos, double dt,
-                        double qup, double quc, double qdp, double ql, const Seg &c)
+// F keeps its sign (pos: F > 0, else F <= 0) on [a, b], shown by the slope
+// bounds from F(a) = fa (when have_fa) and F(b) = fb.
+TRMC_HD bool slope_sign_holds(double a, double b, bool flood, double fa, bool have_fa, double fb, bool pos,
+                              double dt, double qup, double quc, double qdp, double ql, const Seg &c)
 {
-    const Enclosure en = f_enclosure(a, b, flood, dt, qup, quc, qdp, ql, c);
-    if (pos ? en.lo > 0.0 : !(en.hi > 0.0)) return true;
     Iv Fd;
     if (!fprime_enclosure(a, b, flood, dt, qup, quc, qdp, ql, c, Fd)) return false;
     const Enclosure sb = slope_bounds(b - a, fa, have_fa, fb, Fd.lo, Fd.hi);
@@ -834,7 +834,8 @@ struct SecantConv {
 
     // stage: 0 eval ha, 1 eval hb, 2 secant, 3 bankfull check, 4 settle at
     // mindepth, 5 done, 7 re-evaluate hi_neg, 8 F(mindepth) first,
-    // 9 certified scan, 11 waiting for a deferred certified solve
+    // 9 certified scan, 10 F at the scan bracket's lower end, 11 waiting
+    // for a deferred certified solve
     TRMC_HD bool start(double qup_, double quc_, double qdp_, double ql_, const Seg &c, double depthp,
                        bool learned_)
     {
@@ -961,6 +962,19 @@ struct SecantConv {
     }
 
     TRMC_HD bool awaiting_scan() const { return stage == 11; }
+
+    // The LEARNED start as the routing kernel opens a segment: F at the
+    // previous depth (stage 0), which raises the guess request; a solve that
+    // starts with F(mindepth) (stage 8) is left to step().  Stage 0 of step()
+    // on its own, so that this call site inlines that stage only.
+    TRMC_HD void first_step(const Seg &c, const ConvCfg &k)
+    {
+        if (stage != 0) return;
+        fa = eval(ha, c, k);
+        qa = q_cur;
+        if (learned) need_guess = true;
+        stage = 1;
+    }
 
     // returns true when the solve is finished (MLP-guess requests leave it
     // running).  kDefer: a solve whose F(mindepth) <= 0 does not run the
@@ -1115,6 +1129,13 @@ struct SecantConv {
             }
             return false;
         }
+        case 10: {                            // F at the scan bracket's lower end, then the secant
+            const double fl = eval(lo_pos, c, k);
+            if (!(fl > 0.0)) { accept_root(c); return stage == 5; }
+            ha = lo_pos; fa = fl; qa = q_cur;
+            stage = 2;
+            return false;
+        }
         case 9:
             if constexpr (!kDefer) {
                 // Certified scan (mirrors the oracle's) from a = ha, where F has
@@ -1147,7 +1168,17 @@ struct SecantConv {
                 double b = fmin(a + w, 1.0e4);
                 if (has_flood && a < c.bfd && b > c.bfd) b = c.bfd;
                 const bool flood = has_flood && a >= c.bfd;
-                const bool have_fa = !(flood && a <= c.bfd);
+                // fa is NaN where a was reached by the enclosure alone (F(a) not evaluated)
+                const bool have_fa = !(flood && a <= c.bfd) && fa == fa;
+                {                                             // the direct enclosure needs no F(b)
+                    const Enclosure en = f_enclosure(a, b, flood, k.dt, qup, quc, qdp, ql, c);
+                    if (spos ? en.lo > 0.0 : !(en.hi > 0.0)) {
+                        ha = b;
+                        fa = NAN;
+                        hb = 2.0 * (b - a);
+                        return false;
+                    }
+                }
                 const double fbv = eval(b, c, k);
                 if (spos ? !(fbv > 0.0) : fbv > 0.0) {          // a sign change in (a, b]
                     if (b - a <= kScanRes * a) {
@@ -1158,19 +1189,19 @@ struct SecantConv {
                             certified = true;
                             hb = b; fb = fbv; qb = q_cur;
                             s_prev = s_prev2 = 1.0e30;
-                            stage = 2;
+                            stage = have_fa ? 2 : 10;            // the secant needs F(a) as well
                             return false;
                         }
                         spos = true;
                         ha = b; fa = fbv; qa = q_cur;
                         return false;
                     }
-                    const double t = (have_fa && fa != fbv) ? (b - a) * fa / (fa - fbv) : 0.5 * (b - a);
+                    const double t = have_fa && fa != fbv ? (b - a) * fa / (fa - fbv) : 0.5 * (b - a);
                     hb = fmax(0.5 * kScanRes * a, fmin(0.5 * (b - a), 0.98 * t));
                     return false;
                 }
-                if (sign_holds(a, b, flood, fa, have_fa, fbv, spos, k.dt, qup, quc, qdp, ql, c)) {
-                    hb = fmax(kScanRes * b, have_fa ? scan_next_step(b - a, fa, fbv) : 4.0 * (b - a));
+                if (slope_sign_holds(a, b, flood, fa, have_fa, fbv, spos, k.dt, qup, quc, qdp, ql, c)) {
+                    hb = fmax(kScanRes * b, have_fa ? scan_next_step(b - a, fa, fbv) : 2.0 * (b - a));
                     ha = b; fa = fbv; qa = q_cur;
                     return false;
                 }
