@@ -13,6 +13,8 @@
  *                                (and the GPU time/level loop of :737-762 with
  *                                kernel_level_wavefront :342-408)
  *   trmc_wavefront_f64        <- same call, every array double (FP64 end to end)
+ *   trmc_wavefront_f32        <- cpu_wavefront, wavefront_mc.cu:413-427 (the FP32 rule:
+ *                                mc_solve :66-193 in float, float in and out)
  *   trmc_reach_parallel_f64   <- cpu_mc, water-prediction/t-route/troute_mc_final.cu:167-171
  *                                and kernel_mc_gpu :142-148
  *   trmc_mc_solve_f64         <- mc_solve_d, wavefront_mc.cu:215-220 (batched)
@@ -66,11 +68,17 @@ typedef struct { double dx, bw, tw, twcc, n_ch, ncc, cs, s0; } trmc_chparams_f64
  *  LINEAR:    the linearized MC of wavefront/linear_mc.cu:130-182 with fixed
  *             per-segment coefficients (trmc_network_set_linear_coeffs):
  *             Q = C1 Qup_new + C2 Qup_old + C3 Q_old + C4/ql ql, no secant;
- *             DATAFLOW schedule only.  Depth and velocity are not updated. */
+ *             DATAFLOW schedule only.  Depth and velocity are not updated.
+ *  REFERENCE_F32: the reference's FP32 solve (wavefront_mc.cu:66-193): the
+ *             REFERENCE rule in float arithmetic, parameters rounded to float,
+ *             float upstream sums (:444-451), every powf correctly rounded;
+ *             flows and depths are float values (held exactly in the double
+ *             state).  Interleaved DATAFLOW schedule only. */
 #define TRMC_MODE_REFERENCE 0
 #define TRMC_MODE_CONVERGED 1
 #define TRMC_MODE_LEARNED   2
 #define TRMC_MODE_LINEAR    3
+#define TRMC_MODE_REFERENCE_F32 4
 
 /* Schedules of the level-ordered DAG sweep.
  *  AUTO:    pick from the autotuner's table (or SKEWED).
@@ -158,6 +166,21 @@ int trmc_wavefront_f32in_f64(const trmc_chparams_f32 *params,
                              int n_timesteps, double dt);
 
 /* Same call with every array double (FP64 end to end).  opt may be NULL. */
+/* cpu_wavefront (wavefront_mc.cu:413-427), same argument list: the FP32
+ * rule end to end (TRMC_MODE_REFERENCE_F32 on the DATAFLOW schedule), final-
+ * step per-segment qdc and per-reach last-segment qdc as float.  Results are
+ * the reference routine's with every powf correctly rounded (the reference's
+ * host libm powf is not, in about 1 of 1,600 calls). */
+int trmc_wavefront_f32(const trmc_chparams_f32 *params,
+                       const int *reach_seg_start, const int *reach_seg_len,
+                       const int *reach_up_ptr, const int *reach_up_idx,
+                       const int *level_ptr, const int *level_reach,
+                       int n_reaches, int n_levels, int n_segments,
+                       const float *qlat_ts, const float *qdp0_seg,
+                       const float *dp0_seg, const float *reach_last0,
+                       float *qdc_seg_out, float *reach_last_out,
+                       int n_timesteps, float dt);
+
 int trmc_wavefront_f64(const trmc_chparams_f64 *params,
                        const int *reach_seg_start, const int *reach_seg_len,
                        const int *reach_up_ptr, const int *reach_up_idx,

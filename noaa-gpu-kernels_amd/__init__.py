@@ -33,7 +33,7 @@ LIB_PATH = os.path.join(HERE, "libtrmc.so")
 OK = 0
 ERR_ARG, ERR_TOPOLOGY, ERR_HIP, ERR_NO_DEVICE, ERR_TIMEOUT = -1, -2, -3, -4, -5
 ERR_NOMEM, ERR_IO, ERR_STATE, ERR_NOT_CONVERGED = -6, -7, -8, -9
-MODE_REFERENCE, MODE_CONVERGED, MODE_LEARNED, MODE_LINEAR = 0, 1, 2, 3
+MODE_REFERENCE, MODE_CONVERGED, MODE_LEARNED, MODE_LINEAR, MODE_REFERENCE_F32 = 0, 1, 2, 3, 4
 SCHED_AUTO, SCHED_SKEWED, SCHED_LEVEL, SCHED_DATAFLOW = 0, 1, 2, 3
 
 PARAM_FIELDS = ("dx", "bw", "tw", "twcc", "n_ch", "ncc", "cs", "s0")
@@ -121,6 +121,7 @@ def _declare(L: C.CDLL) -> None:
         "trmc_default_options": (None, [_p(Options)]),
         "trmc_device_synchronize": (I, [I]),
         "trmc_wavefront_f32in_f64": (I, [_p(ChParamsF32)] + topo + [pF, pF, pF, pF, pD, pD, I, D]),
+        "trmc_wavefront_f32": (I, [_p(ChParamsF32)] + topo + [pF, pF, pF, pF, pF, pF, I, C.c_float]),
         "trmc_wavefront_f64": (I, [_p(ChParamsF64)] + topo + [pD, pD, pD, pD, pD, pD, I, D, _p(Options)]),
         "trmc_reach_parallel_f64": (I, [_p(ChParamsF64), pD, pI, pI, pD, pD, pD, pD, pD, pD, D, D, I]),
         "trmc_mc_solve_f64": (I, [_p(ChParamsF64), pD, pD, pI, I, D, I]),
@@ -523,6 +524,24 @@ def wavefront_fp64(params32, topo: Topology, qlat_ts, qdp0_seg, dp0_seg, reach_l
     _check(lib().trmc_wavefront_f32in_f64(
         _ptr(p, ChParamsF32), *topo.c_args(), _ptr(q, C.c_float), _ptr(qd, C.c_float),
         _ptr(dp, C.c_float), _ptr(rl, C.c_float), _ptr(qout, C.c_double), _ptr(rout, C.c_double),
+        int(n_timesteps), float(dt)))
+    return qout, rout
+
+
+def wavefront_f32(params32, topo: Topology, qlat_ts, qdp0_seg, dp0_seg, reach_last0,
+                  n_timesteps: int, dt: float = 300.0):
+    """Drop-in for cpu_wavefront (wavefront_mc.cu:413-427): the FP32 rule end to
+    end on the GPU (every powf correctly rounded).
+
+    Returns (qdc_seg_out [N], reach_last_out [R]) as float32."""
+    p = params_f32(params32)
+    q, qd, dp, rl = _f32(qlat_ts), _f32(qdp0_seg), _f32(dp0_seg), _f32(reach_last0)
+    _check_route_inputs(topo, p, q, qd, dp, rl, n_timesteps)
+    qout = np.zeros(topo.n_segments, np.float32)
+    rout = np.zeros(topo.n_reaches, np.float32)
+    _check(lib().trmc_wavefront_f32(
+        _ptr(p, ChParamsF32), *topo.c_args(), _ptr(q, C.c_float), _ptr(qd, C.c_float),
+        _ptr(dp, C.c_float), _ptr(rl, C.c_float), _ptr(qout, C.c_float), _ptr(rout, C.c_float),
         int(n_timesteps), float(dt)))
     return qout, rout
 

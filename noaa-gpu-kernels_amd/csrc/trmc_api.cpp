@@ -93,6 +93,39 @@ static void build_segc(const trmc_chparams_f64 *p, const std::vector<int> &seg_o
     }
 }
 
+// The same table for the FP32 REFERENCE rule: the parameters rounded to float
+// (the reference's ChParams are float, wavefront_mc.cu:41-44) and the derived
+// constants in float arithmetic, as its float solve computes them inline
+// (z and bankfull depth :73-76, sqrtf(s0)/n_ch :107, sqrtf(1+z*z) :58,
+// sqrtf(s0) :124).
+static void build_segc32(const trmc_chparams_f64 *p, const std::vector<int> &seg_of_dev,
+                         std::vector<float> &out)
+{
+    const size_t nd = seg_of_dev.size();
+    out.assign((size_t)SC_COUNT * nd, 0.0f);
+    for (size_t ds = 0; ds < nd; ++ds) {
+        const trmc_chparams_f64 &w = p[seg_of_dev[ds]];
+        const float dx = (float)w.dx, bw = (float)w.bw, tw = (float)w.tw, twcc = (float)w.twcc;
+        const float n_ch = (float)w.n_ch, ncc = (float)w.ncc, cs = (float)w.cs, s0 = (float)w.s0;
+        const float z = (cs == 0.0f) ? 1.0f : 1.0f / cs;
+        float bfd;
+        if (bw > tw) bfd = bw / 0.00001f;
+        else if (bw == tw) bfd = bw / (2.0f * z);
+        else bfd = (tw - bw) / (2.0f * z);
+        out[SC_DX * nd + ds] = dx;
+        out[SC_BW * nd + ds] = bw;
+        out[SC_TWCC * nd + ds] = twcc;
+        out[SC_N * nd + ds] = n_ch;
+        out[SC_NCC * nd + ds] = ncc;
+        out[SC_S0 * nd + ds] = s0;
+        out[SC_Z * nd + ds] = z;
+        out[SC_BFD * nd + ds] = bfd;
+        out[SC_KS * nd + ds] = sqrtf(s0) / n_ch;
+        out[SC_SQ * nd + ds] = sqrtf(1.0f + z * z);
+        out[SC_SQS0 * nd + ds] = sqrtf(s0);
+    }
+}
+
 template <typename T>
 static int dev_upload(T **dptr, const T *host, size_t n, hipStream_t s)
 {
@@ -150,6 +183,7 @@ struct trmc_network {
     unsigned *d_trace_it = nullptr;
     int trace_n = 0;
     float *d_mlp = nullptr;
+    float *d_segc32 = nullptr;           // FP32 REFERENCE constants [SC_COUNT][n_dev_segs]
     double *d_lin = nullptr;             // LINEAR coefficients [4][n_dev_segs]
     unsigned long long *d_scanq = nullptr;   // certified-solve queue: control words, ring, answer tickets
     double *d_scan_data = nullptr;           // requests [lanes][5], answers [lanes][3]
@@ -172,7 +206,7 @@ struct trmc_network {
         dev_free(d_ring); dev_free(d_hist); dev_free(d_mlp); dev_free(d_bar); dev_free(d_status);
         dev_free(d_t); dev_free(d_counters); dev_free(d_q0); dev_free(d_d0); dev_free(d_ring0);
         dev_free(d_trace_t); dev_free(d_trace_it); dev_free(d_flow_hist); dev_free(d_lin);
-        dev_free(d_scanq); dev_free(d_scan_data);
+        dev_free(d_scanq); dev_free(d_scan_data); dev_free(d_segc32);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -297,13 +331,15 @@ int trmc_network_create(const int *reach_seg_start, const int *reach_seg_len,
     if (o.block_size <= 0) o.block_size = 64;
     if (o.block_size % 64 != 0 || o.block_size > 256)
         return fail(TRMC_ERR_ARG, "block_size must be a multiple of 64, at most 256");
-    if (o.mode < TRMC_MODE_REFERENCE || o.mode > TRMC_MODE_LINEAR)
+    if (o.mode < TRMC_MODE_REFERENCE || o.mode > TRMC_MODE_REFERENCE_F32)
         return fail(TRMC_ERR_ARG, "unknown mode %d", o.mode);
     // measured best on the 1M-segment Shreve network: the REFERENCE rule's
     // short solves want longer units, the converged rules shorter ones (more
     // items side by side), LINEAR (no solve) longer still
     if (o.unit_segments <= 0)
-        o.unit_segments = o.mode == TRMC_MODE_REFERENCE ? 4 : o.mode == TRMC_MODE_LINEAR ? 8 : 2;
+        o.unit_segments = (o.mode == TRMC_MODE_REFERENCE || o.mode == TRMC_MODE_REFERENCE_F32) ? 4
+                          : o.mode == TRMC_MODE_LINEAR                                          ? 8
+                                                                                                : 2;
     if (o.tol <= 0.0) o.tol = 1e-13;
     if (o.max_iter <= 0) o.max_iter = 60;
     if (o.learned_budget <= 0) o.learned_budget = 12;
@@ -365,6 +401,11 @@ int trmc_network_set_params_f64(trmc_network *net, const trmc_chparams_f64 *para
     build_segc(params, net->plan.seg_of_dev, segc);
     int rc = dev_upload(&net->d_segc, segc.data(), segc.size(), net->stream);
     if (rc) return rc;
+    std::vector<float> segc32;
+    if (net->opt.mode == TRMC_MODE_REFERENCE_F32) {
+        build_segc32(params, net->plan.seg_of_dev, segc32);
+        if ((rc = dev_upload(&net->d_segc32, segc32.data(), segc32.size(), net->stream))) return rc;
+    }
     HIPCHK(hipStreamSynchronize(net->stream));
     net->have_params = true;
     return TRMC_OK;
@@ -566,6 +607,7 @@ static RouteArgs make_args(trmc_network *net, int T, double dt)
     a.unit_flags = net->d_unit_flags;
     a.unit_reach = net->d_unit_reach;
     a.segc = net->d_segc;
+    a.segc32 = net->d_segc32;
     a.qlat = net->d_qlat;
     a.q = net->d_q;
     a.d = net->d_d;
@@ -629,6 +671,8 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
         sched = P.max_upstream <= kDataflowMaxUp ? TRMC_SCHED_DATAFLOW : TRMC_SCHED_SKEWED;
     if (mode == TRMC_MODE_LINEAR && (sched != TRMC_SCHED_DATAFLOW || o.dataflow_variant != 0))
         return fail(TRMC_ERR_ARG, "LINEAR mode runs on the interleaved DATAFLOW schedule");
+    if (mode == TRMC_MODE_REFERENCE_F32 && (sched != TRMC_SCHED_DATAFLOW || o.dataflow_variant != 0))
+        return fail(TRMC_ERR_ARG, "REFERENCE_F32 mode runs on the interleaved DATAFLOW schedule");
     if (sched == TRMC_SCHED_DATAFLOW && P.max_upstream > kDataflowMaxUp)
         return fail(TRMC_ERR_ARG, "DATAFLOW schedule supports confluences of at most %d reaches (network has %d)",
                     kDataflowMaxUp, P.max_upstream);
@@ -938,6 +982,40 @@ int trmc_wavefront_f32in_f64(const trmc_chparams_f32 *params, const int *reach_s
     if ((rc = trmc_network_set_forcing_f32(net, qlat_ts, n_timesteps))) return rc;
     if ((rc = trmc_network_run(net, n_timesteps, dt, nullptr))) return rc;
     return trmc_network_get_state_f64(net, qdc_seg_out_d, nullptr, nullptr, reach_last_out_d);
+}
+
+int trmc_wavefront_f32(const trmc_chparams_f32 *params, const int *reach_seg_start,
+                       const int *reach_seg_len, const int *reach_up_ptr, const int *reach_up_idx,
+                       const int *level_ptr, const int *level_reach, int n_reaches, int n_levels,
+                       int n_segments, const float *qlat_ts, const float *qdp0_seg, const float *dp0_seg,
+                       const float *reach_last0, float *qdc_seg_out, float *reach_last_out, int n_timesteps,
+                       float dt)
+{
+    if (n_timesteps < 0 || n_segments < 0 || n_reaches < 0) return fail(TRMC_ERR_ARG, "negative size");
+    if ((n_segments > 0 && (!qdc_seg_out || !qdp0_seg || !dp0_seg)) ||
+        (n_reaches > 0 && (!reach_last_out || !reach_last0)))
+        return fail(TRMC_ERR_ARG, "null array");
+    trmc_options o;
+    trmc_default_options(&o);
+    o.mode = TRMC_MODE_REFERENCE_F32;
+    o.schedule = TRMC_SCHED_DATAFLOW;
+    trmc_network *net = nullptr;
+    int rc = trmc_network_create(reach_seg_start, reach_seg_len, reach_up_ptr, reach_up_idx,
+                                 level_ptr, level_reach, n_reaches, n_levels, n_segments, &o, &net);
+    if (rc) return rc;
+    std::unique_ptr<trmc_network, void (*)(trmc_network *)> guard(net, trmc_network_destroy);
+    // float values are held exactly in the network's double state
+    std::vector<double> q0(qdp0_seg, qdp0_seg + n_segments), d0(dp0_seg, dp0_seg + n_segments);
+    std::vector<double> rl0(reach_last0, reach_last0 + n_reaches);
+    if ((rc = trmc_network_set_params_f32(net, params))) return rc;
+    if ((rc = trmc_network_set_state_f64(net, q0.data(), d0.data(), rl0.data()))) return rc;
+    if ((rc = trmc_network_set_forcing_f32(net, qlat_ts, n_timesteps))) return rc;
+    if ((rc = trmc_network_run(net, n_timesteps, (double)dt, nullptr))) return rc;
+    std::vector<double> qo((size_t)n_segments), ro((size_t)n_reaches);
+    if ((rc = trmc_network_get_state_f64(net, qo.data(), nullptr, nullptr, ro.data()))) return rc;
+    for (int i = 0; i < n_segments; ++i) qdc_seg_out[i] = (float)qo[i];
+    for (int r = 0; r < n_reaches; ++r) reach_last_out[r] = (float)ro[r];
+    return TRMC_OK;
 }
 
 static int diffusive_run(const trmc_diff_reach *rp, const float *dx, const float *diffusivity,

@@ -58,6 +58,7 @@ struct RouteArgs {
     long long timeout_ticks;  // barrier spin limit, wall-clock ticks
     const int *level_ptr, *unit_level, *unit_seg, *up_ptr, *up_idx, *unit_flags, *unit_reach;
     const double *segc;       // [SC_COUNT][n_dev_segs]
+    const float *segc32;      // the same constants in float arithmetic (FP32 REFERENCE rule)
     const double *qlat;       // [forcing rows][n_dev_segs][M]
     double *q, *d, *v;        // [n_dev_segs][M], updated in place
     double *ring_buf;         // [ring][n_units][M]
@@ -117,7 +118,7 @@ hipError_t launch_route_level(const RouteArgs &a, int mode, int level, long long
 hipError_t launch_step_advance(int *t_dev, hipStream_t stream);
 hipError_t launch_route_dataflow(const RouteArgs &a, int mode, int grid, int block, hipStream_t stream);
 hipError_t route_dataflow_occupancy(int mode, int block, int *blocks_per_cu);
-// secant iterations interleaved across lanes (modes 0, 1, 2)
+// secant iterations interleaved across lanes (every mode)
 hipError_t launch_route_dataflow_il(const RouteArgs &a, int mode, int grid, int block, hipStream_t stream);
 hipError_t route_dataflow_il_occupancy(int mode, int block, int *blocks_per_cu);
 // history slot 0 <- ring slot `src_slot`; slots 1..T <- kPending

@@ -485,9 +485,12 @@ __device__ __forceinline__ void scan_server(const RouteArgs &a)
     }
 }
 
-template <int MODE> struct IlSolver { typedef SecantConv type; };
-template <> struct IlSolver<0> { typedef SecantRef type; };
-template <> struct IlSolver<3> { typedef SecantRef type; };   // LINEAR: no iteration (unused)
+// the solver and the segment constants of each rule (MODE 4: the FP32
+// REFERENCE rule, float arithmetic on float values held in the double state)
+template <int MODE> struct IlSolver { typedef SecantConv type; typedef Seg seg; };
+template <> struct IlSolver<0> { typedef SecantRef type; typedef Seg seg; };
+template <> struct IlSolver<3> { typedef SecantRef type; typedef Seg seg; };   // LINEAR: no iteration (unused)
+template <> struct IlSolver<4> { typedef SecantRef32 type; typedef Seg32 seg; };
 
 template <int MODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_route_dataflow_il(RouteArgs a)
@@ -530,7 +533,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     int4 ups = make_int4(0, 0, 0, 0);
     double qup_in = 0.0, quc_in = 0.0, qup_next = 0.0, quc_next = 0.0, last = 0.0, cur_qdp = 0.0;
     const double *ql_row = a.qlat;
-    Seg c;
+    typename IlSolver<MODE>::seg c;
     typename IlSolver<MODE>::type sr;
     const ConvCfg kc = {a.dt, a.tol, a.max_eval, a.learned_budget};
     long long spin_start = 0;
@@ -539,6 +542,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     auto start_solve = [&](double qup, double quc, double qdp, double ql, double dp) -> bool {
         if constexpr (MODE == 0 || MODE == 3) {
             return sr.start(a.dt, qup, quc, qdp, ql, c, dp);
+        } else if constexpr (MODE == 4) {   // every value is a float (exact conversions)
+            return sr.start((float)a.dt, (float)qup, (float)quc, (float)qdp, (float)ql, c, (float)dp);
         } else {
             if (sr.start(qup, quc, qdp, ql, c, dp, MODE == 2)) return true;
             if constexpr (MODE == 2) sr.first_step(c, kc);   // F at the previous depth: the flow to guess for
@@ -593,7 +598,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                 close_segment(o, qdp);
                 continue;
             }
-            c = load_seg(a.segc, nd, s);
+            if constexpr (MODE == 4) c = load_seg32(a.segc32, nd, s);
+            else c = load_seg(a.segc, nd, s);
             if (!start_solve(qup, quc, qdp, ql_row[sm], a.d[sm])) {
                 cur_qdp = qdp;
                 phase = PH_SOLVE;
@@ -641,6 +647,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         if (phase == PH_WAIT) {
             bool ready = true;
             double sp = 0.0, sc = 0.0, first_p = 0.0, first_c = 0.0;
+            float spf = 0.0f, scf = 0.0f;          // MODE 4: float sums (wavefront_mc.cu:444-451)
             const int upv[4] = {ups.x, ups.y, ups.z, ups.w};
 #pragma unroll
             for (int q = 0; q < kMaxUp; ++q) {
@@ -656,11 +663,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                     // fixed-order sums starting from 0.0 (wavefront_mc.cu:523-529)
                     sp += vp;
                     sc += vc;
+                    if constexpr (MODE == 4) {
+                        spf += (float)vp;
+                        scf += (float)vc;
+                    }
                     if (q == 0) { first_p = vp; first_c = vc; }
                 }
             }
             if (ready) {
                 wait_ticks += (long long)wall_clock64() - spin_start;
+                if constexpr (MODE == 4) {
+                    sp = (double)spf;
+                    sc = (double)scf;
+                }
                 qup_in = internal ? first_p : sp;
                 quc_in = internal ? first_c : sc;
                 s = s0;
@@ -746,7 +761,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             if (solving()) {
                 progressed = true;
                 bool fin;
-                if constexpr (MODE == 0 || MODE == 3) fin = sr.step(c);
+                if constexpr (MODE == 0 || MODE == 3 || MODE == 4) fin = sr.step(c);
                 else fin = sr.template step<true>(c, kc);
                 if (fin) close_segment(sr.finish(c), cur_qdp);
             }
@@ -1015,7 +1030,8 @@ hipError_t launch_route_dataflow_il(const RouteArgs &a, int mode, int grid, int 
     return mode == 0   ? launch_dataflow_il_t<0>(a, grid, block, stream)
            : mode == 1 ? launch_dataflow_il_t<1>(a, grid, block, stream)
            : mode == 2 ? launch_dataflow_il_t<2>(a, grid, block, stream)
-                       : launch_dataflow_il_t<3>(a, grid, block, stream);
+           : mode == 3 ? launch_dataflow_il_t<3>(a, grid, block, stream)
+                       : launch_dataflow_il_t<4>(a, grid, block, stream);
 }
 
 hipError_t route_dataflow_il_occupancy(int mode, int block, int *blocks_per_cu)
@@ -1025,7 +1041,9 @@ hipError_t route_dataflow_il_occupancy(int mode, int block, int *blocks_per_cu)
                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<1>, block, 0)
            : mode == 2
                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<2>, block, 0)
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<3>, block, 0);
+           : mode == 3
+               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<3>, block, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<4>, block, 0);
 }
 
 // Linearized MC coefficients, linear_mc.cu:57-100 (compute_mc_coeffs): float

@@ -28,9 +28,16 @@
 
 namespace trmc {
 
-struct Seg {
-    double dx, bw, twcc, n_ch, ncc, s0, z, bfd, ks, sq, sqs0;
+// Per-segment constants of one precision: the parameters and the derived
+// z = 1/cs, bankfull depth, ks = sqrt(s0)/n, sq = sqrt(1+z^2), sqs0 = sqrt(s0).
+// Seg (double) serves the FP64 rules; Seg32 (float, computed with float
+// operations as the reference's float solve computes them inline) the FP32
+// REFERENCE rule.
+template <class T> struct SegT {
+    T dx, bw, twcc, n_ch, ncc, s0, z, bfd, ks, sq, sqs0;
 };
+typedef SegT<double> Seg;
+typedef SegT<float> Seg32;
 
 TRMC_HD Seg load_seg(const double *__restrict__ sc, size_t stride, int s)
 {
@@ -48,6 +55,34 @@ TRMC_HD Seg load_seg(const double *__restrict__ sc, size_t stride, int s)
     c.sqs0 = sc[SC_SQS0 * stride + s];
     return c;
 }
+
+TRMC_HD Seg32 load_seg32(const float *__restrict__ sc, size_t stride, int s)
+{
+    Seg32 c;
+    c.dx = sc[SC_DX * stride + s];
+    c.bw = sc[SC_BW * stride + s];
+    c.twcc = sc[SC_TWCC * stride + s];
+    c.n_ch = sc[SC_N * stride + s];
+    c.ncc = sc[SC_NCC * stride + s];
+    c.s0 = sc[SC_S0 * stride + s];
+    c.z = sc[SC_Z * stride + s];
+    c.bfd = sc[SC_BFD * stride + s];
+    c.ks = sc[SC_KS * stride + s];
+    c.sq = sc[SC_SQ * stride + s];
+    c.sqs0 = sc[SC_SQS0 * stride + s];
+    return c;
+}
+
+// fmax / fmin / fabs of one precision (an unqualified call on a float could
+// otherwise reach the double C function on the host and widen the expression)
+TRMC_HD double tmax(double a, double b) { return fmax(a, b); }
+TRMC_HD float tmax(float a, float b) { return fmaxf(a, b); }
+TRMC_HD double tmin(double a, double b) { return fmin(a, b); }
+TRMC_HD float tmin(float a, float b) { return fminf(a, b); }
+TRMC_HD double tabs(double a) { return fabs(a); }
+TRMC_HD float tabs(float a) { return fabsf(a); }
+TRMC_HD double tsqrt(double a) { return sqrt(a); }
+TRMC_HD float tsqrt(float a) { return sqrtf(a); }
 
 // Correctly rounded x^y for the two exponents the MC arithmetic uses,
 // y = double(2/3) and y = double(5/3).  The reference calls the host libm's
@@ -69,6 +104,33 @@ TRMC_HD Seg load_seg(const double *__restrict__ sc, size_t stride, int s)
 // midpoint.  One cube root and one log serve both powers.
 struct Pow2353 { double p23, p53; };
 
+// The double-double core of the correctly rounded powers: for x > 0 finite,
+// x = m 2^(3k) with m in [1, 8), and m^(2/3) = p_hi + p_lo, m^(5/3) = q_hi +
+// q_lo to ~2^-100 relative.
+struct Cbrt23 { int k; double p_hi, p_lo, q_hi, q_lo; };
+
+TRMC_HD Cbrt23 cbrt23_core(double x)
+{
+    Cbrt23 r;
+    int e = 0;
+    const double f = frexp(x, &e);                // x = f 2^e, f in [0.5, 1)
+    const int e1 = e - 1;
+    r.k = (e1 >= 0) ? e1 / 3 : -((2 - e1) / 3);   // floor(e1 / 3)
+    const double m = ldexp(2.0 * f, e1 - 3 * r.k);  // [1, 8)
+    const double c0 = cbrt(m);
+    const double s_hi = c0 * c0, s_lo = fma(c0, c0, -s_hi);
+    const double t_hi = s_hi * c0, t_lo = fma(s_hi, c0, -t_hi) + s_lo * c0;
+    const double rho = (t_hi - m) + t_lo;         // c0^3 - m
+    const double corr = rho / (3.0 * s_hi);
+    const double c_hi = c0 - corr;
+    const double c_lo = (c0 - c_hi) - corr;
+    r.p_hi = c_hi * c_hi;
+    r.p_lo = fma(c_hi, c_hi, -r.p_hi) + 2.0 * c_hi * c_lo;
+    r.q_hi = m * r.p_hi;
+    r.q_lo = fma(m, r.p_hi, -r.q_hi) + m * r.p_lo;
+    return r;
+}
+
 TRMC_HD Pow2353 pow23_53(double x)
 {
     Pow2353 r;
@@ -78,29 +140,70 @@ TRMC_HD Pow2353 pow23_53(double x)
         r.p53 = v;
         return r;
     }
-    int e = 0;
-    const double f = frexp(x, &e);                // x = f 2^e, f in [0.5, 1)
-    const int e1 = e - 1;
-    const int k = (e1 >= 0) ? e1 / 3 : -((2 - e1) / 3);   // floor(e1 / 3)
-    const double m = ldexp(2.0 * f, e1 - 3 * k);  // [1, 8)
-    const double c0 = cbrt(m);
-    const double s_hi = c0 * c0, s_lo = fma(c0, c0, -s_hi);
-    const double t_hi = s_hi * c0, t_lo = fma(s_hi, c0, -t_hi) + s_lo * c0;
-    const double rho = (t_hi - m) + t_lo;         // c0^3 - m
-    const double corr = rho / (3.0 * s_hi);
-    const double c_hi = c0 - corr;
-    const double c_lo = (c0 - c_hi) - corr;
-    const double p_hi = c_hi * c_hi;
-    const double p_lo = fma(c_hi, c_hi, -p_hi) + 2.0 * c_hi * c_lo;
-    const double q_hi = m * p_hi;
-    const double q_lo = fma(m, p_hi, -q_hi) + m * p_lo;
+    const Cbrt23 c = cbrt23_core(x);
     const double L = ldexp(log(x) / 3.0, -53);
     const double e23 = -L + 0.5 * L * L;          // exp(-L) - 1
     const double e53 = 2.0 * L + 2.0 * L * L;     // exp(2L) - 1
-    r.p23 = ldexp(p_hi + (p_lo + p_hi * e23), 2 * k);
-    r.p53 = ldexp(q_hi + (q_lo + q_hi * e53), 5 * k);
+    r.p23 = ldexp(c.p_hi + (c.p_lo + c.p_hi * e23), 2 * c.k);
+    r.p53 = ldexp(c.q_hi + (c.q_lo + c.q_hi * e53), 5 * c.k);
     return r;
 }
+
+// The float a double-double hi + lo rounds to (round to nearest even): the
+// rounding of hi, unless hi lies exactly halfway between two floats, where
+// the sign of lo decides.
+TRMC_HD float round_dd_to_float(double hi, double lo)
+{
+    float f = (float)hi;
+    const double fd = (double)f;
+    const double diff = hi - fd;                  // exact
+    if (diff != 0.0 && lo != 0.0 && (lo > 0.0) == (diff > 0.0)) {
+        const float nb = nextafterf(f, diff > 0.0 ? INFINITY : -INFINITY);
+        if (diff == ((double)nb - fd) * 0.5) f = nb;
+    }
+    return f;
+}
+
+// Correctly rounded float x^y for the two float exponents of the FP32 rule,
+// y = 2.0f/3.0f = 2/3 + 2^-24/3 and y = 5.0f/3.0f = 5/3 - 2^-23/3 (the
+// reference's float solve calls powf with them, wavefront_mc.cu:105-106,
+// :124, :132-133, :156, :190): the double-double m^(2/3), m^(5/3) above, the
+// exponent offsets as x^(2/3) exp(u) and x^(5/3) exp(-2u) with
+// u = ln(x) 2^-24 / 3 (|u| < 2^-18) to third order, and one rounding of the
+// double-double result (relative error ~2^-68) to float.
+struct Powf2353 { float p23, p53; };
+
+static_assert(2.0f / 3.0f == (float)(11184811.0 / 16777216.0), "float(2/3) = 2/3 + 2^-24/3");
+static_assert(5.0f / 3.0f == (float)(13981013.0 / 8388608.0), "float(5/3) = 5/3 - 2^-23/3");
+
+TRMC_HD Powf2353 powf23_53(float xf)
+{
+    Powf2353 r;
+    if (!(xf > 0.0f) || xf == INFINITY) {
+        const float v = (xf == 0.0f) ? 0.0f : (xf == INFINITY ? INFINITY : NAN);
+        r.p23 = v;
+        r.p53 = v;
+        return r;
+    }
+    const double x = (double)xf;
+    const Cbrt23 c = cbrt23_core(x);
+    const double u = ldexp(log(x) / 3.0, -24);
+    const double e23 = u + u * u * (0.5 + u * (1.0 / 6.0));          // exp(u) - 1
+    const double e53 = -2.0 * u + u * u * (2.0 - u * (4.0 / 3.0));    // exp(-2u) - 1
+    double t = c.p_lo + c.p_hi * e23;
+    double hi = c.p_hi + t;
+    double lo = (c.p_hi - hi) + t;
+    r.p23 = round_dd_to_float(ldexp(hi, 2 * c.k), ldexp(lo, 2 * c.k));
+    t = c.q_lo + c.q_hi * e53;
+    hi = c.q_hi + t;
+    lo = (c.q_hi - hi) + t;
+    r.p53 = round_dd_to_float(ldexp(hi, 5 * c.k), ldexp(lo, 5 * c.k));
+    return r;
+}
+
+// the correctly rounded power pair of one precision
+TRMC_HD Pow2353 pow_pair(double x) { return pow23_53(x); }
+TRMC_HD Powf2353 pow_pair(float x) { return powf23_53(x); }
 
 // x^(2/3) and x^(5/3) to within a few ulps (no correctly rounded finish): the
 // CONVERGED/LEARNED residual and velocity, whose root is wanted to 1e-9, not
@@ -155,96 +258,119 @@ TRMC_HD double recip(double x)
     return 1.0 / x;
 }
 
-struct Geo { double twl, R, A, AC, WP, WPC; };
-struct Coef { double c1, c2, c3, c4; };
+// The REFERENCE arithmetic below is written once for both precisions: T =
+// double is the FP64 rule (mc_solve_d, wavefront_mc.cu:200-337), T = float
+// the FP32 rule (mc_solve, :51-193), whose operations are the same ones in
+// float.  Every constant is T(...) of a literal that rounds to the same float
+// as the reference's f-suffixed one (checked below).
+static_assert((float)1.33 == 1.33f && (float)0.67 == 0.67f && (float)0.01 == 0.01f && (float)0.9 == 0.9f &&
+                  (float)0.25 == 0.25f && (float)0.5 == 0.5f,
+              "float constants of the FP32 rule");
 
-// compound trapezoid geometry, wavefront_mc.cu:200-213
-TRMC_HD Geo geometry(double h, const Seg &c)
+template <class T> struct GeoT { T twl, R, A, AC, WP, WPC; };
+template <class T> struct CoefT { T c1, c2, c3, c4; };
+typedef GeoT<double> Geo;
+typedef CoefT<double> Coef;
+
+// compound trapezoid geometry, wavefront_mc.cu:200-213 (float: :51-64)
+template <class T>
+TRMC_HD GeoT<T> geometry(T h, const SegT<T> &c)
 {
-    Geo g;
-    g.twl = c.bw + 2.0 * c.z * h;
-    double over = fmax(h - c.bfd, 0.0);
-    double in = fmin(c.bfd, h);
-    if (over > 0.0 && c.twcc <= 0.0) { over = 0.0; in = h; }
+    GeoT<T> g;
+    g.twl = c.bw + T(2) * c.z * h;
+    T over = tmax(h - c.bfd, T(0));
+    T in = tmin(c.bfd, h);
+    if (over > T(0) && c.twcc <= T(0)) { over = T(0); in = h; }
     g.A = (c.bw + in * c.z) * in;
-    g.WP = c.bw + 2.0 * in * c.sq;
+    g.WP = c.bw + T(2) * in * c.sq;
     g.AC = c.twcc * over;
-    g.WPC = (over > 0.0) ? (c.twcc + 2.0 * over) : 0.0;
-    g.R = (g.WP + g.WPC > 0.0) ? (g.A + g.AC) / (g.WP + g.WPC) : 0.0;
+    g.WPC = (over > T(0)) ? (c.twcc + T(2) * over) : T(0);
+    g.R = (g.WP + g.WPC > T(0)) ? (g.A + g.AC) / (g.WP + g.WPC) : T(0);
     return g;
 }
 
-// kinematic celerity from R^(2/3), R^(5/3), wavefront_mc.cu:249-254
-TRMC_HD double celerity(double h, double r23, double r53, const Seg &c)
+// kinematic celerity from R^(2/3), R^(5/3), wavefront_mc.cu:249-254 (float: :104-110)
+template <class T>
+TRMC_HD T celerity(T h, T r23, T r53, const SegT<T> &c)
 {
-    if (!(h > 0.0)) return 0.0;
-    return fmax(0.0, c.ks * ((5.0 / 3.0) * r23 - (2.0 / 3.0) * r53 *
-                             (2.0 * c.sq / (c.bw + 2.0 * h * c.z))));
+    if (!(h > T(0))) return T(0);
+    return tmax(T(0), c.ks * ((T(5) / T(3)) * r23 - (T(2) / T(3)) * r53 *
+                              (T(2) * c.sq / (c.bw + T(2) * h * c.z))));
 }
 
-TRMC_HD double storage_km(double ck, double dt, const Seg &c)
+template <class T>
+TRMC_HD T storage_km(T ck, T dt, const SegT<T> &c)
 {
-    return (ck > 0.0) ? fmax(dt, c.dx / ck) : dt;
+    return (ck > T(0)) ? tmax(dt, c.dx / ck) : dt;
 }
 
-TRMC_HD double weight_width(double h, const Geo &g, const Seg &c)
+template <class T>
+TRMC_HD T weight_width(T h, const GeoT<T> &g, const SegT<T> &c)
 {
-    return (h > c.bfd && c.twcc > 0.0) ? c.twcc : g.twl;
+    return (h > c.bfd && c.twcc > T(0)) ? c.twcc : g.twl;
 }
 
-TRMC_HD double weighting(double ck, double twu, double q, double lo, const Seg &c)
+template <class T>
+TRMC_HD T weighting(T ck, T twu, T q, T lo, const SegT<T> &c)
 {
-    if (ck > 0.0 && twu * c.s0 * ck * c.dx > 0.0)
-        return fmin(0.5, fmax(lo, 0.5 * (1.0 - q / (2.0 * twu * c.s0 * ck * c.dx))));
-    return 0.5;
+    if (ck > T(0) && twu * c.s0 * ck * c.dx > T(0))
+        return tmin(T(0.5), tmax(lo, T(0.5) * (T(1) - q / (T(2) * twu * c.s0 * ck * c.dx))));
+    return T(0.5);
 }
 
-TRMC_HD Coef muskingum(double km, double x, double dt, double ql)
+template <class T>
+TRMC_HD CoefT<T> muskingum(T km, T x, T dt, T ql)
 {
-    Coef m;
-    double D = km * (1.0 - x) + dt / 2.0;
-    if (D == 0.0) D = 1.0;
-    m.c1 = (km * x + dt / 2.0) / D;
-    m.c2 = (dt / 2.0 - km * x) / D;
-    m.c3 = (km * (1.0 - x) - dt / 2.0) / D;
+    CoefT<T> m;
+    T D = km * (T(1) - x) + dt / T(2);
+    if (D == T(0)) D = T(1);
+    m.c1 = (km * x + dt / T(2)) / D;
+    m.c2 = (dt / T(2) - km * x) / D;
+    m.c3 = (km * (T(1) - x) - dt / T(2)) / D;
     m.c4 = (ql * dt) / D;
     return m;
 }
 
 // Manning flow with the R^(2/3) already at hand, wavefront_mc.cu:267-270
-TRMC_HD double manning(const Geo &g, double r23, const Seg &c)
+template <class T>
+TRMC_HD T manning(const GeoT<T> &g, T r23, const SegT<T> &c)
 {
-    if (!(g.WP + g.WPC > 0.0)) return 0.0;
-    return (1.0 / (((g.WP * c.n_ch) + (g.WPC * c.ncc)) / (g.WP + g.WPC))) * (g.A + g.AC) * r23 * c.sqs0;
+    if (!(g.WP + g.WPC > T(0))) return T(0);
+    return (T(1) / (((g.WP * c.n_ch) + (g.WPC * c.ncc)) / (g.WP + g.WPC))) * (g.A + g.AC) * r23 * c.sqs0;
 }
 
-// wavefront_mc.cu:321-328
-TRMC_HD double outflow(const Coef &m, double qup, double quc, double qdp)
+// wavefront_mc.cu:321-328 (float: :177-184)
+template <class T>
+TRMC_HD T outflow(const CoefT<T> &m, T qup, T quc, T qdp)
 {
-    double routed = m.c1 * qup + m.c2 * quc + m.c3 * qdp;
-    double qmc = routed + m.c4;
-    if (qmc < 0.0) {
-        if (m.c4 < 0.0 && fabs(m.c4) > routed) return 0.0;
-        return fmax(m.c1 * qup + m.c2 * quc + m.c4, m.c1 * qup + m.c3 * qdp + m.c4);
+    T routed = m.c1 * qup + m.c2 * quc + m.c3 * qdp;
+    T qmc = routed + m.c4;
+    if (qmc < T(0)) {
+        if (m.c4 < T(0) && tabs(m.c4) > routed) return T(0);
+        return tmax(m.c1 * qup + m.c2 * quc + m.c4, m.c1 * qup + m.c3 * qdp + m.c4);
     }
     return qmc;
 }
 
-// wavefront_mc.cu:330-336; FAITHFUL (the CONVERGED/LEARNED output) takes
-// R^(2/3) within a few ulps instead of correctly rounded
-template <bool FAITHFUL = false>
-TRMC_HD double velocity(double h, const Seg &c)
+// wavefront_mc.cu:330-336 (float: :186-190); FAITHFUL (the CONVERGED/LEARNED output,
+// double only) takes R^(2/3) within a few ulps instead of correctly rounded
+template <bool FAITHFUL = false, class T>
+TRMC_HD T velocity(T h, const SegT<T> &c)
 {
-    double twl = c.bw + 2.0 * c.z * h;
-    double Rv = (h * (c.bw + twl) / 2.0) /
-                (c.bw + 2.0 * sqrt(((twl - c.bw) / 2.0) * ((twl - c.bw) / 2.0) + h * h));
-    if (Rv < 0.0) Rv = 0.0;
-    return (1.0 / c.n_ch) * (FAITHFUL ? pow23_53_faithful(Rv) : pow23_53(Rv)).p23 * c.sqs0;
+    T twl = c.bw + T(2) * c.z * h;
+    T Rv = (h * (c.bw + twl) / T(2)) /
+           (c.bw + T(2) * tsqrt(((twl - c.bw) / T(2)) * ((twl - c.bw) / T(2)) + h * h));
+    if (Rv < T(0)) Rv = T(0);
+    T r23;
+    if constexpr (FAITHFUL) r23 = pow23_53_faithful(Rv).p23;
+    else r23 = pow_pair(Rv).p23;
+    return (T(1) / c.n_ch) * r23 * c.sqs0;
 }
 
-TRMC_HD bool seg_invalid(const Seg &c)
+template <class T>
+TRMC_HD bool seg_invalid(const SegT<T> &c)
 {
-    return c.n_ch <= 0.0 || c.s0 <= 0.0 || c.z <= 0.0 || c.bw <= 0.0;
+    return c.n_ch <= T(0) || c.s0 <= T(0) || c.z <= T(0) || c.bw <= T(0);
 }
 
 struct SolveOut {
@@ -255,16 +381,18 @@ struct SolveOut {
 
 // Everything of one secant point that depends only on the depth: geometry,
 // R^(2/3), R^(5/3), celerity, storage constant, weighting width, Manning flow.
-struct Point {
-    double h, r23, ck, km, twu, qmn;
+template <class T> struct PointT {
+    T h, r23, ck, km, twu, qmn;
 };
+typedef PointT<double> Point;
 
-TRMC_HD Point eval_point(double h, double dt, const Seg &c)
+template <class T>
+TRMC_HD PointT<T> eval_point(T h, T dt, const SegT<T> &c)
 {
-    Point p;
+    PointT<T> p;
     p.h = h;
-    const Geo g = geometry(h, c);
-    const Pow2353 w = pow23_53(g.R);
+    const GeoT<T> g = geometry(h, c);
+    const auto w = pow_pair(g.R);
     p.r23 = w.p23;
     p.ck = celerity(h, w.p23, w.p53, c);
     p.km = storage_km(p.ck, dt, c);
@@ -342,88 +470,91 @@ TRMC_HD SolveOut solve_reference(double dt, double qup, double quc, double qdp, 
 // bookkeeping of wavefront_mc.cu:246-318, finish() the outputs.  Driving
 // start(); while (!step()); finish() performs exactly the operations of
 // solve_reference in the same order, so the results are identical; a kernel
-// can interleave the iterations of many solves.
-struct SecantRef {
-    Point p0;
-    Coef C;
-    double qup, quc, qdp, ql, dt;
-    double h, h_0, Qj_0, rerror, aerror;
+// can interleave the iterations of many solves.  T = float is the FP32
+// rule's solve (wavefront_mc.cu:66-193): the same steps in float.
+template <class T>
+struct SecantRefT {
+    PointT<T> p0;
+    CoefT<T> C;
+    T qup, quc, qdp, ql, dt;
+    T h, h_0, Qj_0, rerror, aerror;
     int iter, maxiter, attempt, total;
     bool have0;
     bool trivial;     // guard hit: outputs are zero
 
     // returns true when no iteration is needed (guard)
-    TRMC_HD bool start(double dt_, double qup_, double quc_, double qdp_, double ql_, const Seg &c,
-                       double depthp)
+    TRMC_HD bool start(T dt_, T qup_, T quc_, T qdp_, T ql_, const SegT<T> &c, T depthp)
     {
         dt = dt_; qup = qup_; quc = quc_; qdp = qdp_; ql = ql_;
         total = 0;
-        trivial = seg_invalid(c) || (ql <= 0.0 && qup <= 0.0 && quc <= 0.0 && qdp <= 0.0);
+        trivial = seg_invalid(c) || (ql <= T(0) && qup <= T(0) && quc <= T(0) && qdp <= T(0));
         if (trivial) return true;
-        const double d0 = fmax(depthp, 0.0);
-        h = d0 * 1.33 + 0.01;
-        h_0 = d0 * 0.67;
-        C.c1 = 0.0; C.c2 = 0.0; C.c3 = 0.0; C.c4 = 0.0;
-        Qj_0 = 0.0;
+        const T d0 = tmax(depthp, T(0));
+        h = d0 * T(1.33) + T(0.01);
+        h_0 = d0 * T(0.67);
+        C.c1 = T(0); C.c2 = T(0); C.c3 = T(0); C.c4 = T(0);
+        Qj_0 = T(0);
         maxiter = 100;
         attempt = 0;
         iter = 0;
-        aerror = 0.01;
-        rerror = 1.0;
+        aerror = T(0.01);
+        rerror = T(1);
         have0 = false;
         return false;
     }
 
     // one iteration; returns true when the solve has ended
-    TRMC_HD bool step(const Seg &c)
+    TRMC_HD bool step(const SegT<T> &c)
     {
-        const double mindepth = 0.01;
+        const T mindepth = T(0.01);
         if (!have0 || !(p0.h == h_0)) p0 = eval_point(h_0, dt, c);
-        const double x0 = weighting(p0.ck, p0.twu, Qj_0, 0.0, c);
-        const Coef m0 = muskingum(p0.km, x0, dt, ql);
-        const double q0 = m0.c1 * qup + m0.c2 * quc + m0.c3 * qdp + m0.c4;
+        const T x0 = weighting(p0.ck, p0.twu, Qj_0, T(0), c);
+        const CoefT<T> m0 = muskingum(p0.km, x0, dt, ql);
+        const T q0 = m0.c1 * qup + m0.c2 * quc + m0.c3 * qdp + m0.c4;
         Qj_0 = q0 - p0.qmn;
-        const Point p1 = eval_point(h, dt, c);
-        const double X = weighting(p1.ck, p1.twu, q0, 0.25, c);
+        const PointT<T> p1 = eval_point(h, dt, c);
+        const T X = weighting(p1.ck, p1.twu, q0, T(0.25), c);
         C = muskingum(p1.km, X, dt, ql);
-        const double routed = C.c1 * qup + C.c2 * quc + C.c3 * qdp;
-        if (C.c4 < 0.0 && fabs(C.c4) > routed) C.c4 = -routed;
-        const double Qj = (routed + C.c4) - p1.qmn;
-        double h_1 = (Qj_0 - Qj != 0.0) ? h - (Qj * (h_0 - h)) / (Qj_0 - Qj) : h;
-        if (h_1 < 0.0) h_1 = h;
-        if (h > 0.0) { rerror = fabs((h_1 - h) / h); aerror = fabs(h_1 - h); }
-        else { rerror = 0.0; aerror = 0.9; }
-        h_0 = fmax(0.0, h);
-        h = fmax(0.0, h_1);
+        const T routed = C.c1 * qup + C.c2 * quc + C.c3 * qdp;
+        if (C.c4 < T(0) && tabs(C.c4) > routed) C.c4 = -routed;
+        const T Qj = (routed + C.c4) - p1.qmn;
+        T h_1 = (Qj_0 - Qj != T(0)) ? h - (Qj * (h_0 - h)) / (Qj_0 - Qj) : h;
+        if (h_1 < T(0)) h_1 = h;
+        if (h > T(0)) { rerror = tabs((h_1 - h) / h); aerror = tabs(h_1 - h); }
+        else { rerror = T(0); aerror = T(0.9); }
+        h_0 = tmax(T(0), h);
+        h = tmax(T(0), h_1);
         p0 = p1;
         have0 = true;
         ++iter;
-        const bool more = !(h < mindepth) && rerror > 0.01 && aerror >= mindepth && iter <= maxiter;
+        const bool more = !(h < mindepth) && rerror > T(0.01) && aerror >= mindepth && iter <= maxiter;
         if (more) return false;
         total += iter;                       // attempt over
         if (iter < maxiter) return true;
-        h *= 1.33;
-        h_0 *= 0.67;
+        h *= T(1.33);
+        h_0 *= T(0.67);
         maxiter += 25;
         if (++attempt == 5) return true;
         iter = 0;                            // the next attempt always iterates at least once
-        aerror = 0.01;
-        rerror = 1.0;
+        aerror = T(0.01);
+        rerror = T(1);
         return false;
     }
 
-    TRMC_HD SolveOut finish(const Seg &c) const
+    TRMC_HD SolveOut finish(const SegT<T> &c) const
     {
         SolveOut o;
         o.flags = 0;
         o.iters = total;
         if (trivial) { o.q = 0.0; o.depth = 0.0; o.vel = 0.0; return o; }
-        o.vel = velocity(h, c);
-        o.depth = h;
-        o.q = outflow(C, qup, quc, qdp);
+        o.vel = (double)velocity(h, c);
+        o.depth = (double)h;
+        o.q = (double)outflow(C, qup, quc, qdp);
         return o;
     }
 };
+typedef SecantRefT<double> SecantRef;
+typedef SecantRefT<float> SecantRef32;
 
 // ------------------------------------------------------------- CONVERGED
 // F(h) with the X0 = 0.5 coefficients feeding the weighting X(h); also returns

@@ -12,6 +12,13 @@
 #                               41-44 (ChParams, float)
 #                               200-337 (hgeo_inline_d, mc_solve_d)
 #                               495-564 (cpu_wavefront_fp64)
+#   the same file, FP32 rule  : 20-29, 41-44, 51-193 (hgeo_inline, mc_solve),
+#                               413-491 (cpu_wavefront) -> libtroute_ref_f32.so.
+#                               These routines are __host__ __device__, which
+#                               g++ does not know: hipcc compiles them for the
+#                               host only (--offload-host-only, no device code),
+#                               calling the host libm's powf/sqrtf as the
+#                               reference's CPU build does.
 #   troute_mc_final.cu        : 20-23 (includes, skipping cuda_runtime.h at 24)
 #                               26-28 (ChParams, double)
 #                               167-252 (cpu_mc), 254-274 (gen)
@@ -43,6 +50,8 @@ TMP=$(mktemp -d /tmp/trmc_ref.XXXXXX)
 trap 'rm -rf "$TMP"' EXIT
 { sed -n '20,29p;41,44p;200,337p;495,564p' "$REF/wavefront/wavefront_mc.cu"
   cat "$HERE/ref_driver_wavefront.inc"; } > "$TMP/ref_wavefront.cpp"
+{ sed -n '20,29p;41,44p;51,193p;413,491p' "$REF/wavefront/wavefront_mc.cu"
+  cat "$HERE/ref_driver_wavefront_f32.inc"; } > "$TMP/ref_wavefront_f32.cpp"
 { sed -n '20,23p;26,28p;167,252p;254,274p' "$REF/troute_mc_final.cu"
   cat "$HERE/ref_driver_mcfinal.inc"; } > "$TMP/ref_mcfinal.cpp"
 { sed -n '27,35p;238,289p' "$REF/wavefront/linear_mc.cu"
@@ -56,6 +65,8 @@ trap 'rm -rf "$TMP"' EXIT
 # the cut must land on the routines it names (guards against line drift)
 grep -q "static inline double mc_solve_d(" "$TMP/ref_wavefront.cpp"
 grep -q "static void cpu_wavefront_fp64(" "$TMP/ref_wavefront.cpp"
+grep -q "inline float mc_solve(" "$TMP/ref_wavefront_f32.cpp"
+grep -q "static void cpu_wavefront(" "$TMP/ref_wavefront_f32.cpp"
 grep -q "^void cpu_mc(" "$TMP/ref_mcfinal.cpp"
 grep -q "^void gen(" "$TMP/ref_mcfinal.cpp"
 grep -q "static void cpu_linear_mc_fp64(" "$TMP/ref_linear.cpp"
@@ -64,7 +75,7 @@ grep -q "static void compute_mc_coeffs_cpu(" "$TMP/ref_matrix.cpp"
 grep -q "static void build_propagation_matrices(" "$TMP/ref_matrix.cpp"
 grep -q "static void mc_secant_solve_cpu(" "$TMP/ref_matrix_secant.cpp"
 grep -q "^void gen_diff_data(" "$TMP/ref_diffusive.cpp"
-if grep -q "cuda\|cusparse" "$TMP/ref_wavefront.cpp" "$TMP/ref_mcfinal.cpp" "$TMP/ref_linear.cpp" \
+if grep -q "cuda\|cusparse" "$TMP/ref_wavefront.cpp" "$TMP/ref_wavefront_f32.cpp" "$TMP/ref_mcfinal.cpp" "$TMP/ref_linear.cpp" \
         "$TMP/ref_diffusive.cpp" "$TMP/ref_matrix.cpp" "$TMP/ref_matrix_secant.cpp"; then
     echo "build_ref: extracted ranges still reference CUDA" >&2; exit 1
 fi
@@ -73,6 +84,10 @@ g++ -O2 -ffp-contract=off -fPIC -shared -std=c++17 -w \
     -o "$OUT/libtroute_ref.so" "$TMP/ref_wavefront.cpp" "$TMP/ref_mcfinal.cpp" "$TMP/ref_linear.cpp" \
     "$TMP/ref_diffusive.cpp" "$TMP/ref_matrix.cpp" "$TMP/ref_matrix_secant.cpp"
 echo "build_ref: built $OUT/libtroute_ref.so"
+HIPCC=${HIPCC:-/opt/rocm/bin/hipcc}
+"$HIPCC" -x hip --offload-host-only -O2 -ffp-contract=off -fPIC -shared -std=c++17 -w -Wl,--as-needed \
+    -o "$OUT/libtroute_ref_f32.so" "$TMP/ref_wavefront_f32.cpp"
+echo "build_ref: built $OUT/libtroute_ref_f32.so"
 # the forcing-ingest program compiles whole from its own file (std C++17 only)
 QLAT="$REF/wavefront/data_parsers/parse_qlat_fast_cpp.cpp"
 if [ -f "$QLAT" ]; then

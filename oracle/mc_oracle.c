@@ -421,6 +421,240 @@ void orc_wavefront_f64_mt(const orc_chparams_f64 *params,
                   reach_last_out, depth_out, vel_out, reach_hist, n_timesteps, dt, mode, iter_sum, n_threads);
 }
 
+/* ------------------------------------------------------------------------ */
+/* FP32 REFERENCE rule: the reference's float solve mc_solve                */
+/* (wavefront_mc.cu:51-193, hgeo_inline :51-64) and its single-thread DAG   */
+/* loop cpu_wavefront (:413-491), every operation in float as there (x86-64 */
+/* SSE float: no excess precision; -ffp-contract=off: no FMA).              */
+
+/* powf of the float arithmetic: the host libm's powf (liborc.so, as the
+ * reference calls it), or the correctly rounded powf (liborc_cr.so): the x87
+ * extended pow decides the float rounding unless it lies within 2^-50
+ * relative of a float rounding midpoint, then the quad pow does. */
+#ifdef ORC_CR_POW
+static float orc_powf(float x, float y)
+{
+    long double r = powl((long double)x, (long double)y);
+    float f = (float)r;
+    if (isfinite(f) && f > 0.0f) {
+        long double up = ((long double)nextafterf(f, INFINITY) - (long double)f) / 2;
+        long double dn = ((long double)f - (long double)nextafterf(f, 0.0f)) / 2;
+        long double dist = r >= (long double)f ? up - (r - (long double)f) : dn - ((long double)f - r);
+        if (dist > fabsl(r) * 0x1p-50L) return f;
+    }
+    return (float)powq((__float128)x, (__float128)y);
+}
+#else
+static float orc_powf(float x, float y) { return powf(x, y); }
+#endif
+
+float orc_powf_fn(float x, float y) { return orc_powf(x, y); }
+
+/* float channel: the parameters and z, bankfull depth as :73-77 computes them */
+typedef struct { float dx, bw, tw, twcc, n_ch, ncc, s0, z, bfd; } chan32_t;
+typedef struct { float twl, R, A, AC, WP, WPC; } geo32_t;
+typedef struct { float c1, c2, c3, c4; } coef32_t;
+
+static void chan32_init(const orc_chparams_f32 *p, chan32_t *c)
+{
+    c->dx = p->dx; c->bw = p->bw; c->tw = p->tw; c->twcc = p->twcc;
+    c->n_ch = p->n_ch; c->ncc = p->ncc; c->s0 = p->s0;
+    c->z = (p->cs == 0.0f) ? 1.0f : 1.0f / p->cs;
+    if (p->bw > p->tw) c->bfd = p->bw / 0.00001f;
+    else if (p->bw == p->tw) c->bfd = p->bw / (2.0f * c->z);
+    else c->bfd = (p->tw - p->bw) / (2.0f * c->z);
+}
+
+static void geometry32(float h, const chan32_t *c, geo32_t *g)
+{
+    g->twl = c->bw + 2.0f * c->z * h;
+    float over = fmaxf(h - c->bfd, 0.0f);
+    float in = fminf(c->bfd, h);
+    if (over > 0.0f && c->twcc <= 0.0f) { over = 0.0f; in = h; }
+    g->A = (c->bw + in * c->z) * in;
+    g->WP = c->bw + 2.0f * in * sqrtf(1.0f + c->z * c->z);
+    g->AC = c->twcc * over;
+    g->WPC = (over > 0.0f) ? (c->twcc + 2.0f * over) : 0.0f;
+    g->R = (g->WP + g->WPC > 0.0f) ? (g->A + g->AC) / (g->WP + g->WPC) : 0.0f;
+}
+
+/* one secant point's depth functions (:102-124 / :129-156) */
+static float celerity32(float h, const geo32_t *g, const chan32_t *c)
+{
+    if (!(h > 0.0f)) return 0.0f;
+    const float r23 = orc_powf(g->R, 2.0f / 3.0f);
+    const float r53 = orc_powf(g->R, 5.0f / 3.0f);
+    return fmaxf(0.0f, (sqrtf(c->s0) / c->n_ch) *
+                       ((5.0f / 3.0f) * r23 - (2.0f / 3.0f) * r53 *
+                        (2.0f * sqrtf(1.0f + c->z * c->z) / (c->bw + 2.0f * h * c->z))));
+}
+
+static float weighting32(float ck, float twu, float q, float lo, const chan32_t *c)
+{
+    if (ck > 0.0f && twu * c->s0 * ck * c->dx > 0.0f)
+        return fminf(0.5f, fmaxf(lo, 0.5f * (1.0f - q / (2.0f * twu * c->s0 * ck * c->dx))));
+    return 0.5f;
+}
+
+static void muskingum32(float km, float x, float dt, float ql, coef32_t *m)
+{
+    float D = km * (1.0f - x) + dt / 2.0f;
+    if (D == 0.0f) D = 1.0f;
+    m->c1 = (km * x + dt / 2.0f) / D;
+    m->c2 = (dt / 2.0f - km * x) / D;
+    m->c3 = (km * (1.0f - x) - dt / 2.0f) / D;
+    m->c4 = (ql * dt) / D;
+}
+
+static float manning32(const geo32_t *g, const chan32_t *c)
+{
+    if (!(g->WP + g->WPC > 0.0f)) return 0.0f;
+    return (1.0f / (((g->WP * c->n_ch) + (g->WPC * c->ncc)) / (g->WP + g->WPC))) * (g->A + g->AC) *
+           orc_powf(g->R, 2.0f / 3.0f) * sqrtf(c->s0);
+}
+
+float orc_mc_solve_f32(float dt, float qup, float quc, float qdp, float ql,
+                       const orc_chparams_f32 *p, float depthp,
+                       float *depthc, float *velc, int *iters)
+{
+    chan32_t c;
+    chan32_init(p, &c);
+    if (iters) *iters = 0;
+    if (c.n_ch <= 0.0f || c.s0 <= 0.0f || c.z <= 0.0f || c.bw <= 0.0f ||
+        (ql <= 0.0f && qup <= 0.0f && quc <= 0.0f && qdp <= 0.0f)) {      /* :78-84 */
+        *depthc = 0.0f; *velc = 0.0f;
+        return 0.0f;
+    }
+    const float mindepth = 0.01f;
+    float d0 = fmaxf(depthp, 0.0f);
+    float h = d0 * 1.33f + mindepth;
+    float h_0 = d0 * 0.67f;
+    coef32_t C = {0.0f, 0.0f, 0.0f, 0.0f};
+    float Qj_0 = 0.0f;
+    int maxiter = 100, total = 0;
+    for (int attempt = 0; attempt < 5; ++attempt) {                      /* :95-175 */
+        float aerror = 0.01f, rerror = 1.0f;
+        int iter = 0;
+        while (rerror > 0.01f && aerror >= mindepth && iter <= maxiter) {
+            geo32_t g0, g1;
+            coef32_t m0;
+            geometry32(h_0, &c, &g0);
+            const float ck0 = celerity32(h_0, &g0, &c);
+            const float km0 = (ck0 > 0.0f) ? fmaxf(dt, c.dx / ck0) : dt;
+            const float twu0 = (h_0 > c.bfd && c.twcc > 0.0f) ? c.twcc : g0.twl;
+            muskingum32(km0, weighting32(ck0, twu0, Qj_0, 0.0f, &c), dt, ql, &m0);
+            const float q0 = m0.c1 * qup + m0.c2 * quc + m0.c3 * qdp + m0.c4;
+            Qj_0 = q0 - manning32(&g0, &c);
+
+            geometry32(h, &c, &g1);
+            const float ck1 = celerity32(h, &g1, &c);
+            const float km1 = (ck1 > 0.0f) ? fmaxf(dt, c.dx / ck1) : dt;
+            const float twu1 = (h > c.bfd && c.twcc > 0.0f) ? c.twcc : g1.twl;
+            muskingum32(km1, weighting32(ck1, twu1, q0, 0.25f, &c), dt, ql, &C);
+            const float routed = C.c1 * qup + C.c2 * quc + C.c3 * qdp;
+            if (C.c4 < 0.0f && fabsf(C.c4) > routed) C.c4 = -routed;
+            const float Qj = (routed + C.c4) - manning32(&g1, &c);
+
+            float h_1 = (Qj_0 - Qj != 0.0f) ? h - (Qj * (h_0 - h)) / (Qj_0 - Qj) : h;
+            if (h_1 < 0.0f) h_1 = h;
+            if (h > 0.0f) { rerror = fabsf((h_1 - h) / h); aerror = fabsf(h_1 - h); }
+            else { rerror = 0.0f; aerror = 0.9f; }
+            h_0 = fmaxf(0.0f, h);
+            h = fmaxf(0.0f, h_1);
+            ++iter;
+            if (h < mindepth) break;
+        }
+        total += iter;
+        if (iter < maxiter) break;
+        h *= 1.33f;
+        h_0 *= 0.67f;
+        maxiter += 25;
+    }
+    if (iters) *iters = total;
+    /* outflow rule :177-184 */
+    const float routed = C.c1 * qup + C.c2 * quc + C.c3 * qdp;
+    const float Qmc = routed + C.c4;
+    float qdc = Qmc;
+    if (Qmc < 0.0f) {
+        if (C.c4 < 0.0f && fabsf(C.c4) > routed) qdc = 0.0f;
+        else qdc = fmaxf(C.c1 * qup + C.c2 * quc + C.c4, C.c1 * qup + C.c3 * qdp + C.c4);
+    }
+    /* trapezoid velocity :186-190 */
+    const float twl = c.bw + 2.0f * c.z * h;
+    float Rv = (h * (c.bw + twl) / 2.0f) /
+               (c.bw + 2.0f * sqrtf(((twl - c.bw) / 2.0f) * ((twl - c.bw) / 2.0f) + h * h));
+    if (Rv < 0.0f) Rv = 0.0f;
+    *velc = (1.0f / c.n_ch) * orc_powf(Rv, 2.0f / 3.0f) * sqrtf(c.s0);
+    *depthc = h;
+    return qdc;
+}
+
+/* cpu_wavefront (:413-491): levels in order, reaches of a level in their
+ * order, float upstream sums from 0 in CSR order, segments downstream.  With
+ * n_threads > 1 the reaches of one level are shared out over threads (each
+ * computed exactly as on one thread).  depth_out (may be NULL) gets the final
+ * depths, reach_hist (may be NULL) reach_last after every step. */
+void orc_wavefront_f32(const orc_chparams_f32 *params,
+                       const int *reach_seg_start, const int *reach_seg_len,
+                       const int *reach_up_ptr, const int *reach_up_idx,
+                       const int *level_ptr, const int *level_reach,
+                       int n_reaches, int n_levels, int n_segments,
+                       const float *qlat_ts, const float *qdp0_seg,
+                       const float *dp0_seg, const float *reach_last0,
+                       float *qdc_seg_out, float *reach_last_out,
+                       float *depth_out, float *reach_hist,
+                       int n_timesteps, float dt, long long *iter_sum, int n_threads)
+{
+    const size_t ns = (size_t)n_segments + 1, nr = (size_t)n_reaches + 1;
+    float *q_prev = (float *)malloc(ns * sizeof(float)), *d_prev = (float *)malloc(ns * sizeof(float));
+    float *q_cur = (float *)calloc(ns, sizeof(float)), *d_cur = (float *)calloc(ns, sizeof(float));
+    float *rl_prev = (float *)malloc(nr * sizeof(float)), *rl_cur = (float *)calloc(nr, sizeof(float));
+    memcpy(q_prev, qdp0_seg, (size_t)n_segments * sizeof(float));
+    memcpy(d_prev, dp0_seg, (size_t)n_segments * sizeof(float));
+    memcpy(rl_prev, reach_last0, (size_t)n_reaches * sizeof(float));
+    if (n_threads < 1) n_threads = 1;
+    long long its = 0;
+    for (int t = 0; t < n_timesteps; ++t) {
+        const float *qlat = qlat_ts + (size_t)t * (size_t)n_segments;
+        for (int L = 0; L < n_levels; ++L) {
+#pragma omp parallel for schedule(dynamic, 32) num_threads(n_threads) if (n_threads > 1) reduction(+ : its)
+            for (int ii = level_ptr[L]; ii < level_ptr[L + 1]; ++ii) {
+                const int r = level_reach[ii];
+                float qup_r = 0.0f, quc_r = 0.0f;
+                for (int k = reach_up_ptr[r]; k < reach_up_ptr[r + 1]; ++k) {
+                    qup_r += rl_prev[reach_up_idx[k]];
+                    quc_r += rl_cur[reach_up_idx[k]];
+                }
+                float quc_next = 0.0f, last = 0.0f;
+                for (int i = 0; i < reach_seg_len[r]; ++i) {
+                    const int idx = reach_seg_start[r] + i;
+                    float dep, vel;
+                    int it = 0;
+                    const float qdc = orc_mc_solve_f32(dt, i == 0 ? qup_r : q_prev[idx - 1],
+                                                       i == 0 ? quc_r : quc_next, q_prev[idx], qlat[idx],
+                                                       &params[idx], d_prev[idx], &dep, &vel, &it);
+                    its += it;
+                    q_cur[idx] = qdc;
+                    d_cur[idx] = dep;
+                    quc_next = qdc;
+                    last = qdc;
+                }
+                rl_cur[r] = last;
+            }
+        }
+        memcpy(q_prev, q_cur, (size_t)n_segments * sizeof(float));
+        memcpy(d_prev, d_cur, (size_t)n_segments * sizeof(float));
+        memcpy(rl_prev, rl_cur, (size_t)n_reaches * sizeof(float));
+        if (reach_hist) memcpy(reach_hist + (size_t)t * n_reaches, rl_cur, (size_t)n_reaches * sizeof(float));
+        memset(rl_cur, 0, (size_t)n_reaches * sizeof(float));
+    }
+    memcpy(qdc_seg_out, q_prev, (size_t)n_segments * sizeof(float));
+    memcpy(reach_last_out, rl_prev, (size_t)n_reaches * sizeof(float));
+    if (depth_out) memcpy(depth_out, d_prev, (size_t)n_segments * sizeof(float));
+    if (iter_sum) *iter_sum += its;
+    free(q_prev); free(d_prev); free(q_cur); free(d_cur); free(rl_prev); free(rl_cur);
+}
+
 /* troute_mc_final.cu:167-252.  The inlined solve there is the same
  * arithmetic as mc_solve_d; the invalid/zero-flow guard zeroes the outputs
  * and passes quc = 0 to the next segment (:187-190), which is what a zero

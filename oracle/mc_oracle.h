@@ -139,6 +139,24 @@ void orc_wavefront_f64_mt(const orc_chparams_f64 *params,
                           double *depth_out, double *vel_out, double *reach_hist,
                           int n_timesteps, double dt, int mode, long long *iter_sum, int n_threads);
 
+/* FP32 REFERENCE rule: the reference's float solve (wavefront_mc.cu:66-193)
+ * and its float DAG loop cpu_wavefront (:413-491), float in, float out.
+ * depth_out / reach_hist may be NULL; n_threads > 1 shares the reaches of a
+ * level over threads (bit for bit the one-thread result). */
+float orc_mc_solve_f32(float dt, float qup, float quc, float qdp, float ql,
+                       const orc_chparams_f32 *p, float depthp,
+                       float *depthc, float *velc, int *iters);
+void orc_wavefront_f32(const orc_chparams_f32 *params,
+                       const int *reach_seg_start, const int *reach_seg_len,
+                       const int *reach_up_ptr, const int *reach_up_idx,
+                       const int *level_ptr, const int *level_reach,
+                       int n_reaches, int n_levels, int n_segments,
+                       const float *qlat_ts, const float *qdp0_seg,
+                       const float *dp0_seg, const float *reach_last0,
+                       float *qdc_seg_out, float *reach_last_out,
+                       float *depth_out, float *reach_hist,
+                       int n_timesteps, float dt, long long *iter_sum, int n_threads);
+
 /* cpu_mc: independent reaches, one step, constant upstream inflow uq into
  * segment 0 of every reach. */
 void orc_reach_parallel(const orc_chparams_f64 *p, const double *ql,
@@ -227,6 +245,9 @@ double orc_manning_flow(double h, const orc_chparams_f64 *p);
 /* The pow the build uses: libm pow (liborc.so) or correctly rounded pow
  * (liborc_cr.so, built with -DORC_CR_POW). */
 double orc_pow_fn(double x, double y);
+/* The powf of the float rule: libm powf (liborc.so) or correctly rounded
+ * powf (liborc_cr.so). */
+float orc_powf_fn(float x, float y);
 
 /* Error statistics of wavefront_mc.cu:822-858: relative error over entries
  * with |truth| > 1e-3.  out[0..8] = max_abs, p50, p90, p99, p99.9, max_rel,

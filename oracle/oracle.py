@@ -77,6 +77,12 @@ def lib(variant: str = "libm") -> C.CDLL:
         L.orc_compute_stats.argtypes = [pD, pD, I, pD]
         L.orc_pow_fn.restype = D
         L.orc_pow_fn.argtypes = [D, D]
+        L.orc_powf_fn.restype = C.c_float
+        L.orc_powf_fn.argtypes = [C.c_float, C.c_float]
+        L.orc_mc_solve_f32.restype = C.c_float
+        L.orc_mc_solve_f32.argtypes = [C.c_float] * 5 + [pP32, C.c_float, pF, pF, pI]
+        L.orc_wavefront_f32.restype = None
+        L.orc_wavefront_f32.argtypes = [pP32] + topo + [pF] * 4 + [pF] * 4 + [I, C.c_float, C.POINTER(LL), I]
         L.orc_linear_coeffs.restype = None
         L.orc_linear_coeffs.argtypes = [pP32, I, C.c_float] + [pF] * 4
         L.orc_diffusive_tridiag.restype = None
@@ -238,6 +244,60 @@ def manning_flow(params8, h, variant="libm"):
 
 def pow_fn(x, y, variant="libm"):
     return lib(variant).orc_pow_fn(float(x), float(y))
+
+
+def mc_solve_f32(params8, qup, quc, qdp, ql, depthp, dt=300.0, variant="libm"):
+    """The FP32 REFERENCE solve (wavefront_mc.cu:66-193), float in and out:
+    returns (qdc, depth, velocity, iterations) as Python floats of float32 values."""
+    p = P32(*map(float, params8))
+    d, v, it = C.c_float(), C.c_float(), I()
+    q = lib(variant).orc_mc_solve_f32(float(dt), float(qup), float(quc), float(qdp), float(ql), C.byref(p),
+                                      float(depthp), C.byref(d), C.byref(v), C.byref(it))
+    return q, d.value, v.value, it.value
+
+
+def wavefront_f32(topo, params32, qlat32, qdp0, dp0, reach_last0, n_timesteps, dt=300.0, variant="libm",
+                  threads=1, history=False):
+    """cpu_wavefront restated (wavefront_mc.cu:413-491), FP32 end to end: returns a
+    dict of float32 arrays qdc [N], reach_last [R], depth [N] (final step),
+    reach_hist [T, R] (history=True) and the iteration total."""
+    t = as_topo(topo)
+    keep, targs = _topo_args(t)
+    n, r = int(t["n_segments"]), int(t["n_reaches"])
+    p = np.ascontiguousarray(params32, np.float32)
+    q = np.ascontiguousarray(qlat32, np.float32)
+    q0, d0, r0 = (np.ascontiguousarray(x, np.float32) for x in (qdp0, dp0, reach_last0))
+    qo, ro, do = np.zeros(n, np.float32), np.zeros(r, np.float32), np.zeros(n, np.float32)
+    hist = np.zeros((int(n_timesteps), r), np.float32) if history else None
+    its = LL(0)
+    lib(variant).orc_wavefront_f32(_ptr(p, P32), *targs, _ptr(q, C.c_float), _ptr(q0, C.c_float),
+                                   _ptr(d0, C.c_float), _ptr(r0, C.c_float), _ptr(qo, C.c_float),
+                                   _ptr(ro, C.c_float), _ptr(do, C.c_float),
+                                   _ptr(hist, C.c_float) if history else None, int(n_timesteps), float(dt),
+                                   C.byref(its), int(max(1, threads)))
+    out = {"qdc": qo, "reach_last": ro, "depth": do, "iters": its.value}
+    if history:
+        out["reach_hist"] = hist
+    return out
+
+
+def powf_fn(x, y, variant="libm"):
+    return lib(variant).orc_powf_fn(float(x), float(y))
+
+
+def load_ref_f32():
+    """The reference's own FP32 host routines (oracle/_ref/libtroute_ref_f32.so:
+    mc_solve and cpu_wavefront of wavefront_mc.cu), or None if not built."""
+    path = os.path.join(HERE, "_ref", "libtroute_ref_f32.so")
+    if not os.path.exists(path):
+        return None
+    ref = C.CDLL(path)
+    F = C.c_float
+    ref.ref_mc_solve_f32.restype = F
+    ref.ref_mc_solve_f32.argtypes = [F] * 14 + [pF, pF]
+    ref.ref_cpu_wavefront_f32.restype = None
+    ref.ref_cpu_wavefront_f32.argtypes = [C.c_void_p] + [pI] * 6 + [I, I, I] + [pF] * 4 + [pF, pF, I, F]
+    return ref
 
 
 def compute_stats(pred, truth):

@@ -85,3 +85,29 @@ extern "C" int shim_learned_anchor(const double *segc, const double *in, double 
     out[2] = s.guess_flow_old();
     return s.need_guess ? 0 : 1;
 }
+
+// The FP32 REFERENCE rule's correctly rounded power pair.
+extern "C" void shim_powf23_53(const float *x, float *out, int n)
+{
+    for (int i = 0; i < n; ++i) {
+        Powf2353 p = powf23_53(x[i]);
+        out[2 * i] = p.p23;
+        out[2 * i + 1] = p.p53;
+    }
+}
+
+// The FP32 REFERENCE solve (SecantRef32) driven to completion.  segc: the 11
+// float per-segment constants in SegSlot order; in: qup, quc, qdp, ql, depthp.
+extern "C" int shim_solve_f32(const float *segc, const float *in, float dt, float *out)
+{
+    Seg32 c = load_seg32(segc, 1, 0);
+    SecantRef32 s;
+    if (!s.start(dt, in[0], in[1], in[2], in[3], c, in[4]))
+        while (!s.step(c)) {
+        }
+    SolveOut o = s.finish(c);
+    out[0] = (float)o.q;
+    out[1] = (float)o.depth;
+    out[2] = (float)o.vel;
+    return o.iters;
+}

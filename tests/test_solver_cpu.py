@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from conftest import ROOT, golden, load_oracle
-from crpow import cr_pow, sample_inputs
+from crpow import cr_pow, cr_powf, sample_inputs, sample_inputs_f32
 
 orc = load_oracle()
 SHIM = os.path.join(ROOT, "tests", "native", "libsolver_shim.so")
@@ -209,3 +209,59 @@ def test_device_solver_on_cases_a_sample_search_gets_wrong(shim):
                                             float(guess), o.ctypes.data_as(pD))
                 assert not ((r >> 24) & 1), (i, cat, "capped")
                 assert abs(o[1] - root) <= 1e-9 * root, (i, cat, dp, guess, o, root)
+
+
+# ------------------------------------------------------- the FP32 rule
+def segc32_row(p):
+    """The 11 float per-segment constants, computed in float as the library's
+    table builder and the reference's float solve do."""
+    f = np.float32
+    dx, bw, tw, twcc, n, ncc, cs, s0 = (f(v) for v in p)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        z = f(1) if cs == 0 else f(1) / cs
+        bfd = bw / f(0.00001) if bw > tw else (bw / (f(2) * z) if bw == tw else (tw - bw) / (f(2) * z))
+        return np.array([dx, bw, twcc, n, ncc, s0, z, bfd, np.sqrt(s0) / n, np.sqrt(f(1) + z * z), np.sqrt(s0)],
+                        np.float32)
+
+
+@pytest.fixture(scope="module")
+def shim32(shim):
+    pF = C.POINTER(C.c_float)
+    shim.shim_powf23_53.restype = None
+    shim.shim_powf23_53.argtypes = [pF, pF, C.c_int]
+    shim.shim_solve_f32.restype = C.c_int
+    shim.shim_solve_f32.argtypes = [pF, pF, C.c_float, pF]
+    return shim
+
+
+def test_powf23_53_is_correctly_rounded(shim32):
+    pF = C.POINTER(C.c_float)
+    xs = np.ascontiguousarray(sample_inputs_f32(4000, 3))
+    out = np.zeros(2 * xs.size, np.float32)
+    shim32.shim_powf23_53(xs.ctypes.data_as(pF), out.ctypes.data_as(pF), xs.size)
+    out = out.reshape(-1, 2)
+    y23, y53 = np.float32(2) / np.float32(3), np.float32(5) / np.float32(3)
+    for i, x in enumerate(xs):
+        assert out[i, 0] == cr_powf(x, y23), x
+        assert out[i, 1] == cr_powf(x, y53), x
+    specials = np.array([0.0, -1.0, np.inf, np.nan], np.float32)
+    o = np.zeros(8, np.float32)
+    shim32.shim_powf23_53(specials.ctypes.data_as(pF), o.ctypes.data_as(pF), 4)
+    assert o[0] == 0.0 and o[1] == 0.0 and np.isnan(o[2]) and o[4] == np.inf and np.isnan(o[6])
+
+
+def test_reference_f32_rule_bitwise_against_cr_oracle(shim32):
+    """SecantRef32 (float REFERENCE rule, correctly rounded powf) equals the
+    oracle's float restatement with the correctly rounded powf, bit for bit."""
+    pF = C.POINTER(C.c_float)
+    z = golden("mc_solve_f32_cases.npz")
+    bad = []
+    for p, x in zip(z["params"], z["inputs"]):
+        seg = segc32_row(p)
+        xi = np.ascontiguousarray(x, np.float32)
+        out = np.zeros(3, np.float32)
+        shim32.shim_solve_f32(seg.ctypes.data_as(pF), xi.ctypes.data_as(pF), 300.0, out.ctypes.data_as(pF))
+        want = np.array(orc.mc_solve_f32(p, *x, variant="cr")[:3], np.float32)
+        if not np.array_equal(out.view(np.uint32), want.view(np.uint32)):
+            bad.append((p, x, out, want))
+    assert not bad, bad[:3]
