@@ -29,6 +29,13 @@ Other workloads (--workload):
                  309,000 one-segment reaches, 24 steps, tol 1e-6
   matrix-secant-309k  the same with the secant-fitted coefficients of
                  linear_mc_matrix_secant.cu, tol 1e-7
+  wavefront-1m-4 the FP32 REFERENCE rule (the cpu_wavefront drop-in) on a
+                 network shaped like the reference's own results/run_1m_4.txt
+                 run (its generator's construction at 1M target reaches:
+                 ~872k reaches, ~10.5M segments, ~70 levels, 4 steps, its
+                 parameter and forcing distributions), beside the FP64
+                 REFERENCE rule on the same inputs; CPU baseline: the
+                 reference's own float cpu_wavefront over all 4 steps
 
 Prints one JSON line with the driver's fields plus
   roofline      HBM roofline of the routing kernel (112 algorithmic bytes per
@@ -79,8 +86,9 @@ WORKLOADS = {
     "matrix-309k": dict(kind=2, n=309_000, r=309_000, T=24, M=1, seed=42, mode="matrix", cpu_steps=24),
     "matrix-secant-309k": dict(kind=2, n=309_000, r=309_000, T=24, M=1, seed=42, mode="matrix", cpu_steps=24,
                                coeffs="secant", tol=1e-7),
+    "wavefront-1m-4": dict(kind=3, n=0, r=1_000_000, T=4, M=1, seed=42, mode="reference_f32", cpu_steps=4),
 }
-MODES = {"learned": 2, "plain": 1, "reference": 0, "powerlaw": 2}
+MODES = {"learned": 2, "plain": 1, "reference": 0, "powerlaw": 2, "reference_f32": 4}
 
 
 def log(msg):
@@ -343,6 +351,127 @@ def bench_linear(trmc, spec, args):
     print(json.dumps(out), flush=True)
 
 
+def f32_workload(trmc, spec):
+    """The kind-3 merge tree with the distributions of the reference's network
+    generator (wavefront/gen_network.py:253-273): dx U[300,5000], bw U[5,50],
+    tw = bw + U[2,20], twcc = tw + U[5,100], n U[0.025,0.08], ncc = n U[1.2,2],
+    cs U[0.5,3], s0 U[1e-4,5e-3]; lateral inflow U[0,2]; previous flow
+    U[0,15] and depth U[0.3,3]; reach_last0 from the last segments
+    (wavefront_mc.cu:676-680).  All float32."""
+    T = spec["T"]
+    topo = trmc.generate_network(3, 1, spec["r"], spec["seed"])
+    N = topo.n_segments
+    rng = np.random.default_rng(spec["seed"] ^ 0x5A5A)
+    dx, bw = rng.uniform(300.0, 5000.0, N), rng.uniform(5.0, 50.0, N)
+    tw = bw + rng.uniform(2.0, 20.0, N)
+    twcc = tw + rng.uniform(5.0, 100.0, N)
+    n_ch = rng.uniform(0.025, 0.08, N)
+    ncc = n_ch * rng.uniform(1.2, 2.0, N)
+    cs, s0 = rng.uniform(0.5, 3.0, N), rng.uniform(1e-4, 5e-3, N)
+    params = np.stack([dx, bw, tw, twcc, n_ch, ncc, cs, s0], 1).astype(np.float32)
+    rng = np.random.default_rng(spec["seed"] ^ 0xA1A1)
+    qlat = rng.uniform(0.0, 2.0, (T, N)).astype(np.float32)
+    qdp0 = rng.uniform(0.0, 15.0, N).astype(np.float32)
+    dp0 = rng.uniform(0.3, 3.0, N).astype(np.float32)
+    return trmc.Workload(topo, params, qlat[None], qdp0, dp0, trmc.reach_last0_from(topo, qdp0))
+
+
+def bench_f32(trmc, spec, args):
+    """FP32 REFERENCE rule on the run_1m_4-shaped network: resident-input
+    passes of all T steps, the FP64 REFERENCE rule beside it, the reference's
+    own float cpu_wavefront as the CPU baseline, and bitwise parity against
+    the oracle's float restatement with the correctly rounded powf."""
+    T, dt = spec["T"], 300.0
+    t0 = time.perf_counter()
+    w = f32_workload(trmc, spec)
+    N, R = w.topo.n_segments, w.topo.n_reaches
+    log(f"[bench] workload wavefront-1m-4: {N} segments, {R} reaches, {w.topo.n_levels} levels, "
+        f"max upstream {w.topo.max_upstream}, {T} steps (generated in {time.perf_counter() - t0:.1f} s)")
+    opts = dict(schedule=3, unit_segments=args.unit_segments, block_size=args.block, grid_blocks=args.grid,
+                batch_iters=args.batch_iters)
+    recs, finals = {}, {}
+    for name in ("reference_f32", "reference"):
+        recs[name], net = time_mode(trmc, w, 1, T, dt, name, opts, args)
+        finals[name] = net.state()
+        if name == "reference_f32":
+            trmc.synchronize()
+            t1 = time.perf_counter()
+            net.set_forcing(w.qlat)             # the forcing from host memory, as the reference's timed loop
+            t2 = time.perf_counter()
+            net.reset()
+            net.run(T, dt)
+            trmc.synchronize()
+            t3 = time.perf_counter()
+            pcie = {"value": N * T / (t3 - t1), "unit": "segment-steps/s", "upload_s": t2 - t1,
+                    "upload_bytes": int(w.qlat.nbytes), "host_dtype": "float32", "pass_s": t3 - t2}
+        net.close()
+    head = recs["reference_f32"]
+    seg_steps = N * T
+    # algorithmic bytes of the float drop-in per segment-step: params 32,
+    # lateral inflow 4, previous flow and depth 8, writes of flow, depth and
+    # velocity 12
+    bps = 56.0
+    achieved = bps * seg_steps / (head["kernel_ms"] / 1e3) / 1e9
+    copy_gbps = trmc.stream_copy(1 << 30, reps=5)
+    out = {
+        "metric": "segment-steps/sec, FP32 REFERENCE rule (cpu_wavefront drop-in), run_1m_4-shaped network",
+        "value": head["value"], "unit": "segment-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (the reference generator's construction and distributions; inputs resident in HBM)",
+        "config": {"workload": f"wavefront-1m-4: FP32 REFERENCE rule on a {N}-segment, {R}-reach, "
+                               f"{w.topo.n_levels}-level merge tree (confluences of up to 3), {T} steps, dt=300 s",
+                   "segments": N, "reaches": R, "reach_levels": w.topo.n_levels, "timesteps": T,
+                   "seed": spec["seed"], "unit_segments": args.unit_segments, "block": args.block},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic("wavefront-1m-4"),
+                     "copy_reference_gbps": copy_gbps, "bytes_per_seg_step": bps},
+        "kernel_ms": head["kernel_ms"],
+        "modes": recs,
+        "pcie_inclusive": pcie,
+        "reference_gpu_fp32": {"value": 10466683 * 4 / 0.282841, "unit": "segment-steps/s",
+                               "hardware": "RTX 3060, per-level launches, per-step host upload",
+                               "source": "water-prediction/t-route/wavefront/results/run_1m_4.txt:2-6"},
+    }
+    if not args.no_cpu:
+        orc = _load("trmc_oracle", os.path.join(ROOT, "oracle", "oracle.py"))
+        ref = orc.load_ref_f32()
+        topo_o = orc.as_topo(w.topo)
+        keep, targs = orc._topo_args(topo_o)
+        F = orc.C.c_float
+        ql = np.ascontiguousarray(w.qlat[0], np.float32)
+        qo, ro = np.zeros(N, np.float32), np.zeros(R, np.float32)
+        t0 = time.perf_counter()
+        if ref is not None:
+            ref.ref_cpu_wavefront_f32(w.params.ctypes.data, *targs,
+                                      *(orc._ptr(a, F) for a in (ql, w.qdp0, w.dp0, w.reach_last0)),
+                                      orc._ptr(qo, F), orc._ptr(ro, F), T, dt)
+            kind = "reference"
+        else:
+            r_ = orc.wavefront_f32(topo_o, w.params, ql, w.qdp0, w.dp0, w.reach_last0, T, dt)
+            qo, ro, kind = r_["qdc"], r_["reach_last"], "port"
+        secs = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": seg_steps / secs, "unit": "segment-steps/s", "cores": 1, "kind": kind,
+                               "sample": f"all {T} steps of the workload, one core ({secs:.1f} s)"}
+        out["speedup_vs_cpu"] = head["value"] / out["cpu_baseline"]["value"]
+        t0 = time.perf_counter()
+        cr = orc.wavefront_f32(topo_o, w.params, ql, w.qdp0, w.dp0, w.reach_last0, T, dt, variant="cr",
+                               threads=min(16, os.cpu_count() or 1))
+        q, _, _, rl = finals["reference_f32"]
+        q32, rl32 = q[0].astype(np.float32), rl[0].astype(np.float32)
+        rel = np.abs(q32.astype(np.float64) - qo) / np.maximum(np.abs(qo.astype(np.float64)), 1e-3)
+        out["parity"] = {
+            "steps": T, "vs_oracle_cr_powf_bitwise": bool(np.array_equal(q32.view(np.uint32), cr["qdc"].view(np.uint32))
+                                                         and np.array_equal(rl32.view(np.uint32),
+                                                                            cr["reach_last"].view(np.uint32))),
+            "oracle_seconds": time.perf_counter() - t0,
+            "vs_reference_libm_powf": {"bitwise_equal_frac": float(np.mean(q32 == qo)),
+                                       "median_rel": float(np.median(rel)), "max_rel": float(rel.max()),
+                                       "within_1pct": float(np.mean(rel <= 0.01))}}
+        log(f"[bench] cpu {out['cpu_baseline']}; parity {out['parity']}")
+    print(json.dumps(out), flush=True)
+
+
 def bench_diffusive(trmc, spec, args):
     """The diffusive-wave step (owp_extended_kernels.cu:23-135) on the
     reference's largest benchmark size, 10,000 reaches of 20-99 nodes: inputs
@@ -539,6 +668,9 @@ def main():
         return
     if spec["mode"] == "matrix":
         bench_matrix(trmc, spec, args)
+        return
+    if spec["mode"] == "reference_f32":
+        bench_f32(trmc, spec, args)
         return
     T, M = spec["T"], spec["M"]
     head = args.mode or spec["mode"]

@@ -567,7 +567,12 @@ int trmc_build_flowpath_network(const int64_t *wb_id, const int64_t *to_id, int 
  *          random link attachment (n_reaches/2 headwaters, one single-upstream
  *          reach when n_reaches is even), reach lengths geometric with mean
  *          n_segments/n_reaches, clipped to [1, 40], total exactly n_segments.
- * n_reaches is a target (kind 2) or ignored (kind 1).  Parameters are
+ *  kind 3: balanced random merge tree with chains, the construction of the
+ *          reference's wavefront/gen_network.py:62-246 (n_reaches/6
+ *          headwaters merged two or three at a time, Poisson(2) chain reaches
+ *          on every link, max(1, Poisson(12)) segments per reach; tens of
+ *          levels, confluences of up to 3); n_segments is ignored (> 0).
+ * n_reaches is a target (kinds 2, 3) or ignored (kind 1).  Parameters are
  * float-representable doubles so f32 and f64 entry points see equal values. */
 int trmc_generate_network(int kind, int n_segments, int n_reaches, uint64_t seed,
                           trmc_topo *out);

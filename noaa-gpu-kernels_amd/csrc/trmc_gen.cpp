@@ -33,15 +33,18 @@ static inline double f32r(double x) { return (double)(float)x; }
 // --------------------------------------------------------------- topology
 namespace {
 struct Tree {
-    std::vector<int> parent, c0, c1;  // c1 == -1 for single-upstream (chain) reaches
+    // c1 == -1 for single-upstream (chain) reaches, c2 == -1 below 3-way confluences
+    std::vector<int> parent, c0, c1, c2;
     int root = 0;
     int add() {
-        parent.push_back(-1); c0.push_back(-1); c1.push_back(-1);
+        parent.push_back(-1); c0.push_back(-1); c1.push_back(-1); c2.push_back(-1);
         return (int)parent.size() - 1;
     }
     void replace_child(int p, int oldc, int newc) {
         if (p < 0) { root = newc; return; }
-        if (c0[p] == oldc) c0[p] = newc; else c1[p] = newc;
+        if (c0[p] == oldc) c0[p] = newc;
+        else if (c1[p] == oldc) c1[p] = newc;
+        else c2[p] = newc;
     }
 };
 }  // namespace
@@ -83,6 +86,73 @@ static Tree grow_tree(int n_leaves, int n_chain, uint64_t seed)
     return t;
 }
 
+// Poisson(lambda) draw from uniforms of stream `key` (product of uniforms
+// until it falls below e^-lambda; lambda is small here)
+static int poisson_at(uint64_t key, uint64_t &draw, double lambda)
+{
+    const double lim = det_exp(-lambda);
+    double prod = uniform_at(key, draw++);
+    int k = 0;
+    while (prod > lim) {
+        prod *= uniform_at(key, draw++);
+        ++k;
+    }
+    return k;
+}
+
+// Balanced random merge tree with chains, the construction of the
+// reference's network generator (wavefront/gen_network.py:62-246): B
+// headwaters are merged level by level in random order, three at a time with
+// probability 0.34 and two otherwise (a lone leftover carries over); then
+// every link gets Poisson(mean_chain - 1) single-upstream reaches inserted.
+// Shallow (tens of levels) and wide, as the reference's benchmark networks.
+static Tree merge_tree(int n_leaves, double mean_chain, uint64_t seed)
+{
+    Tree t;
+    uint64_t key = stream_key(seed, 111);
+    uint64_t draw = 0;
+    std::vector<int> cur(n_leaves), next;
+    for (int i = 0; i < n_leaves; ++i) cur[i] = t.add();
+    while (cur.size() > 1) {
+        for (size_t i = cur.size() - 1; i > 0; --i) {           // Fisher-Yates
+            size_t j = std::min(i, (size_t)(uniform_at(key, draw++) * (double)(i + 1)));
+            std::swap(cur[i], cur[j]);
+        }
+        next.clear();
+        size_t i = 0;
+        while (i < cur.size()) {
+            const double r = uniform_at(key, draw++);
+            size_t k = (r < 0.34 && i + 3 <= cur.size()) ? 3 : (i + 2 <= cur.size() ? 2 : 1);
+            if (k == 1) { next.push_back(cur[i++]); continue; }
+            const int p = t.add();
+            t.c0[p] = cur[i];
+            t.c1[p] = cur[i + 1];
+            if (k == 3) t.c2[p] = cur[i + 2];
+            for (size_t c = 0; c < k; ++c) t.parent[cur[i + c]] = p;
+            next.push_back(p);
+            i += k;
+        }
+        cur.swap(next);
+    }
+    t.root = cur.empty() ? 0 : cur[0];
+    const int nodes = (int)t.parent.size();
+    for (int u = 0; u < nodes; ++u) {
+        if (t.parent[u] < 0) continue;
+        const int extra = poisson_at(key, draw, std::max(0.0, mean_chain - 1.0));
+        int below = u;
+        for (int e = 0; e < extra; ++e) {                         // u -> c -> parent
+            const int c = t.add();
+            const int p = t.parent[below];
+            t.parent[c] = p;
+            t.replace_child(p, below, c);
+            t.c0[c] = below;
+            t.parent[below] = c;
+            below = c;
+        }
+    }
+    return t;
+}
+
 static int lay_out(const Tree &t, const std::vector<int> &lens, uint64_t seed, trmc_topo *out)
 {
     const int R = (int)t.parent.size();
@@ -94,6 +164,7 @@ static int lay_out(const Tree &t, const std::vector<int> &lens, uint64_t seed, t
         int n = order[i];
         if (t.c0[n] >= 0) order.push_back(t.c0[n]);
         if (t.c1[n] >= 0) order.push_back(t.c1[n]);
+        if (t.c2[n] >= 0) order.push_back(t.c2[n]);
     }
     if ((int)order.size() != R) return TRMC_ERR_ARG;
     std::vector<int> level(R, 0);
@@ -102,6 +173,7 @@ static int lay_out(const Tree &t, const std::vector<int> &lens, uint64_t seed, t
         int n = order[i], l = 0;
         if (t.c0[n] >= 0) l = std::max(l, level[t.c0[n]] + 1);
         if (t.c1[n] >= 0) l = std::max(l, level[t.c1[n]] + 1);
+        if (t.c2[n] >= 0) l = std::max(l, level[t.c2[n]] + 1);
         level[n] = l;
         n_levels = std::max(n_levels, l + 1);
     }
@@ -125,7 +197,7 @@ static int lay_out(const Tree &t, const std::vector<int> &lens, uint64_t seed, t
     out->reach_level = (int *)std::malloc(sizeof(int) * (R + 1));
     out->reach_n_up = (int *)std::malloc(sizeof(int) * (R + 1));
     out->reach_up_ptr = (int *)std::malloc(sizeof(int) * (R + 1));
-    out->reach_up_idx = (int *)std::malloc(sizeof(int) * (2 * R + 1));
+    out->reach_up_idx = (int *)std::malloc(sizeof(int) * (3 * R + 1));
     out->level_ptr = (int *)std::malloc(sizeof(int) * (n_levels + 1));
     out->level_reach = (int *)std::malloc(sizeof(int) * (R + 1));
     if (!out->reach_seg_start || !out->reach_seg_len || !out->reach_level || !out->reach_n_up ||
@@ -141,10 +213,11 @@ static int lay_out(const Tree &t, const std::vector<int> &lens, uint64_t seed, t
         out->reach_seg_len[i] = lens[i];
         seg += lens[i];
         out->reach_level[i] = level[n];
-        int ups[2], nu = 0;
+        int ups[3], nu = 0;
         if (t.c0[n] >= 0) ups[nu++] = id_of[t.c0[n]];
         if (t.c1[n] >= 0) ups[nu++] = id_of[t.c1[n]];
-        if (nu == 2 && ups[0] > ups[1]) std::swap(ups[0], ups[1]);  // sorted upstream lists
+        if (t.c2[n] >= 0) ups[nu++] = id_of[t.c2[n]];
+        std::sort(ups, ups + nu);                                   // sorted upstream lists
         for (int k = 0; k < nu; ++k) out->reach_up_idx[nnz++] = ups[k];
         out->reach_n_up[i] = nu;
         max_up = std::max(max_up, nu);
@@ -203,6 +276,23 @@ int generate_network(int kind, int n_segments, int n_reaches, uint64_t seed, trm
             if (total < n_segments && lens[i] < 40) { lens[i]++; total++; }
             else if (total > n_segments && lens[i] > 1) { lens[i]--; total--; }
         }
+        return lay_out(t, lens, seed, out);
+    }
+    if (kind == 3) {
+        // n_reaches is a target as in the reference's generator (B leaves,
+        // B = n_reaches / (2 mean_chain)); each reach max(1, Poisson(12))
+        // segments; n_segments is ignored
+        const double mean_chain = 3.0;
+        const int leaves = std::max(2, (int)std::lround((double)std::max(n_reaches, 1) / (mean_chain * 2.0)));
+        Tree t = merge_tree(leaves, mean_chain, seed);
+        const int R = (int)t.parent.size();
+        lens.resize(R);
+        long long total = 0;
+        for (int i = 0; i < R; ++i) {
+            lens[i] = std::max(1, poisson_at(key, draw, 12.0));
+            total += lens[i];
+        }
+        if (total > 0x7fffffffLL) return TRMC_ERR_ARG;
         return lay_out(t, lens, seed, out);
     }
     return TRMC_ERR_ARG;

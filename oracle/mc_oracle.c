@@ -428,19 +428,20 @@ void orc_wavefront_f64_mt(const orc_chparams_f64 *params,
 /* SSE float: no excess precision; -ffp-contract=off: no FMA).              */
 
 /* powf of the float arithmetic: the host libm's powf (liborc.so, as the
- * reference calls it), or the correctly rounded powf (liborc_cr.so): the x87
- * extended pow decides the float rounding unless it lies within 2^-50
- * relative of a float rounding midpoint, then the quad pow does. */
+ * reference calls it), or the correctly rounded powf (liborc_cr.so): the
+ * double pow (relative error below 2^-52) decides the float rounding unless
+ * it lies within 2^-46 relative of a float rounding midpoint (float midpoints
+ * are doubles), then the quad pow does. */
 #ifdef ORC_CR_POW
 static float orc_powf(float x, float y)
 {
-    long double r = powl((long double)x, (long double)y);
+    double r = pow((double)x, (double)y);
     float f = (float)r;
     if (isfinite(f) && f > 0.0f) {
-        long double up = ((long double)nextafterf(f, INFINITY) - (long double)f) / 2;
-        long double dn = ((long double)f - (long double)nextafterf(f, 0.0f)) / 2;
-        long double dist = r >= (long double)f ? up - (r - (long double)f) : dn - ((long double)f - r);
-        if (dist > fabsl(r) * 0x1p-50L) return f;
+        double up = ((double)nextafterf(f, INFINITY) - (double)f) / 2;
+        double dn = ((double)f - (double)nextafterf(f, 0.0f)) / 2;
+        double dist = r >= (double)f ? up - (r - (double)f) : dn - ((double)f - r);
+        if (dist > fabs(r) * 0x1p-46) return f;
     }
     return (float)powq((__float128)x, (__float128)y);
 }

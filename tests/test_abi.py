@@ -122,7 +122,7 @@ def check_plan(topo, p, C):
     assert set(p["unit_level"][crit == 1].tolist()) == set(range(L))
 
 
-@pytest.mark.parametrize("kind,n,R", [(1, 1000, 0), (2, 30000, 9000)])
+@pytest.mark.parametrize("kind,n,R", [(1, 1000, 0), (2, 30000, 9000), (3, 1, 6000)])
 @pytest.mark.parametrize("C", [1, 2, 5, 1000])
 def test_plan_invariants(kind, n, R, C):
     topo = trmc.generate_network(kind, n, R, 42)
@@ -163,6 +163,32 @@ def test_generator_is_deterministic_and_well_formed():
     assert q.shape == (2, 12, 20000) and q.min() > 0
     assert abs(q.mean() - 0.1) < 0.01                 # lognormal with mean 0.1
     assert not np.array_equal(q[0], q[1])             # members have their own streams
+
+
+def test_merge_tree_generator_has_the_reference_generator_shape():
+    """kind 3 follows the reference's gen_network.py construction: leaves =
+    n_reaches / 6 merged two or three at a time, Poisson(2) chain reaches per
+    link, max(1, Poisson(12)) segments per reach; at 1M target reaches the
+    reference's own run (results/run_1m_4.txt:2) has 872,622 reaches,
+    10,466,683 segments, 69 levels and confluences of at most 3."""
+    a = trmc.generate_network(3, 1, 60000, 7)
+    b = trmc.generate_network(3, 1, 60000, 7)
+    assert np.array_equal(a.reach_up_idx, b.reach_up_idx) and np.array_equal(a.reach_seg_len, b.reach_seg_len)
+    leaves = round(60000 / 6)
+    assert int(np.sum(a.reach_n_up == 0)) == leaves
+    assert a.max_upstream == 3 and set(np.unique(a.reach_n_up).tolist()) == {0, 1, 2, 3}
+    n2, n3 = int(np.sum(a.reach_n_up == 2)), int(np.sum(a.reach_n_up == 3))
+    assert n2 + 2 * n3 == leaves - 1                  # a tree: every merge removes k - 1 nodes
+    assert 0.2 < n3 / (n2 + n3) < 0.35                 # three-way merges: 0.34 of the draws
+    one = int(np.sum(a.reach_n_up == 1))              # chain reaches: Poisson(2) per link
+    links = leaves + n2 + n3 - 1
+    assert abs(one / links - 2.0) < 0.1
+    assert abs(a.reach_seg_len.mean() - 12.0) < 0.2 and a.reach_seg_len.min() >= 1
+    for r in range(a.n_reaches):
+        ups = a.reach_up_idx[a.reach_up_ptr[r]:a.reach_up_ptr[r + 1]]
+        assert np.all(ups < r) and np.all(np.diff(ups) > 0)
+        assert a.reach_level[r] == (0 if len(ups) == 0 else 1 + a.reach_level[ups].max())
+    assert a.n_levels < 60
 
 
 def test_binary_formats_roundtrip(tmp_path):
