@@ -581,6 +581,10 @@ int trmc_generate_params(int n_segments, uint64_t seed, trmc_chparams_f64 *out);
  * (sigma_ln 1), member m drawn from its own stream. */
 int trmc_generate_forcing(int n_segments, int n_timesteps, int n_members, uint64_t seed,
                           double *qlat);
+/* [new] member `member` alone: qlat [n_timesteps][n_segments], the values of
+ * member `member` of trmc_generate_forcing (and of the device generator). */
+int trmc_generate_forcing_member(int n_segments, int n_timesteps, int member, uint64_t seed,
+                                 double *qlat);
 /* Initial state: qdp0 = q0, dp0 = wide-channel normal depth at q0. */
 int trmc_generate_state(int n_segments, const trmc_chparams_f64 *params, double q0,
                         double *qdp0, double *dp0);

@@ -161,6 +161,7 @@ def _declare(L: C.CDLL) -> None:
         "trmc_generate_network": (I, [I, I, I, C.c_uint64, _p(Topo)]),
         "trmc_generate_params": (I, [I, C.c_uint64, _p(ChParamsF64)]),
         "trmc_generate_forcing": (I, [I, I, I, C.c_uint64, pD]),
+        "trmc_generate_forcing_member": (I, [I, I, I, C.c_uint64, pD]),
         "trmc_generate_state": (I, [I, _p(ChParamsF64), D, pD, pD]),
         "trmc_compute_stats": (I, [pD, pD, I, pD]),
         "trmc_network_set_mlp": (I, [V, pF]),
@@ -360,6 +361,13 @@ def generate_params(n_segments: int, seed: int = 42) -> np.ndarray:
 def generate_forcing(n_segments: int, n_timesteps: int, n_members: int = 1, seed: int = 42) -> np.ndarray:
     out = np.zeros((n_members, n_timesteps, n_segments), np.float64)
     _check(lib().trmc_generate_forcing(n_segments, n_timesteps, n_members, seed, _ptr(out, C.c_double)))
+    return out
+
+
+def generate_forcing_member(n_segments: int, n_timesteps: int, member: int, seed: int = 42) -> np.ndarray:
+    """Member `member` of generate_forcing(..., n_members > member, seed), [T, N]."""
+    out = np.zeros((n_timesteps, n_segments), np.float64)
+    _check(lib().trmc_generate_forcing_member(n_segments, n_timesteps, member, seed, _ptr(out, C.c_double)))
     return out
 
 

@@ -1631,6 +1631,12 @@ int trmc_generate_forcing(int n_segments, int n_timesteps, int n_members, uint64
     return rc ? fail(rc, "generate_forcing failed") : TRMC_OK;
 }
 
+int trmc_generate_forcing_member(int n_segments, int n_timesteps, int member, uint64_t seed, double *qlat)
+{
+    int rc = generate_forcing_members(n_segments, n_timesteps, member, 1, seed, qlat);
+    return rc ? fail(rc, "generate_forcing_member failed") : TRMC_OK;
+}
+
 int trmc_generate_state(int n_segments, const trmc_chparams_f64 *params, double q0, double *qdp0,
                         double *dp0)
 {

@@ -324,9 +324,10 @@ int generate_params(int n, uint64_t seed, trmc_chparams_f64 *out)
 }
 
 // Lateral inflow: lognormal, mean 0.1 m3/s, sigma_ln = 1 (mu = ln 0.1 - 1/2).
-int generate_forcing(int n, int T, int M, uint64_t seed, double *qlat)
+// Members [m0, m0 + M) of the stream family of `seed`.
+int generate_forcing_members(int n, int T, int m0, int M, uint64_t seed, double *qlat)
 {
-    if (!qlat || n < 0 || T < 0 || M < 1) return TRMC_ERR_ARG;
+    if (!qlat || n < 0 || T < 0 || M < 1 || m0 < 0) return TRMC_ERR_ARG;
     const size_t per_member = (size_t)n * (size_t)T;
     const size_t total = per_member * (size_t)M;
     unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -334,7 +335,7 @@ int generate_forcing(int n, int T, int M, uint64_t seed, double *qlat)
     auto work = [&](size_t b, size_t e) {
         for (size_t i = b; i < e; ++i) {
             const size_t m = i / per_member, j = i - m * per_member;
-            qlat[i] = rng::forcing_at(seed, (int)m, j);
+            qlat[i] = rng::forcing_at(seed, m0 + (int)m, j);
         }
     };
     std::vector<std::thread> pool;
@@ -345,6 +346,11 @@ int generate_forcing(int n, int T, int M, uint64_t seed, double *qlat)
     }
     for (auto &th : pool) th.join();
     return TRMC_OK;
+}
+
+int generate_forcing(int n, int T, int M, uint64_t seed, double *qlat)
+{
+    return generate_forcing_members(n, T, 0, M, seed, qlat);
 }
 
 // qdp0 = q0 and dp0 = wide-channel Manning depth (q0 n / (bw sqrt S0))^(3/5).

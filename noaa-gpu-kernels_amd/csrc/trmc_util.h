@@ -17,6 +17,7 @@ int fail(int code, const char *fmt, ...);
 int generate_network(int kind, int n_segments, int n_reaches, uint64_t seed, trmc_topo *out);
 int generate_params(int n, uint64_t seed, trmc_chparams_f64 *out);
 int generate_forcing(int n, int T, int M, uint64_t seed, double *qlat);
+int generate_forcing_members(int n, int T, int m0, int M, uint64_t seed, double *qlat);
 int generate_state(int n, const trmc_chparams_f64 *p, double q0, double *qdp0, double *dp0);
 int load_topo(const char *path, trmc_topo *t, std::string &err);
 int save_topo(const char *path, const trmc_topo *t);

@@ -96,3 +96,88 @@ def max_rel(got, want, floor=1e-3):
     got = np.asarray(got, np.float64)
     want = np.asarray(want, np.float64)
     return float(np.max(np.abs(got - want) / np.maximum(np.abs(want), floor))) if got.size else 0.0
+
+
+def subnetwork(topo, roots):
+    """The upstream closure of reaches `roots`: a topology dict (reaches keep
+    their relative order, so ids stay topological and every upstream list its
+    order) plus seg_map / reach_map, the original segment and reach ids.  A
+    reach's outflows depend only on its upstream closure, so routing the
+    closure alone gives the same values as the whole network."""
+    up_ptr, up_idx = np.asarray(topo.reach_up_ptr), np.asarray(topo.reach_up_idx)
+    keep = np.zeros(topo.n_reaches, bool)
+    stack = [int(r) for r in roots]
+    while stack:
+        r = stack.pop()
+        if keep[r]:
+            continue
+        keep[r] = True
+        stack.extend(int(u) for u in up_idx[up_ptr[r]:up_ptr[r + 1]])
+    reach_map = np.flatnonzero(keep)
+    new_id = -np.ones(topo.n_reaches, np.int64)
+    new_id[reach_map] = np.arange(len(reach_map))
+    start, length = np.asarray(topo.reach_seg_start), np.asarray(topo.reach_seg_len)
+    seg_len = length[reach_map].astype(np.int32)
+    seg_start = np.concatenate([[0], np.cumsum(seg_len)[:-1]]).astype(np.int32)
+    seg_map = np.concatenate([np.arange(start[r], start[r] + length[r]) for r in reach_map]).astype(np.int64) \
+        if len(reach_map) else np.zeros(0, np.int64)
+    ups = [new_id[up_idx[up_ptr[r]:up_ptr[r + 1]]] for r in reach_map]
+    n_up = np.array([len(u) for u in ups], np.int32)
+    level = np.asarray(topo.reach_level)[reach_map].astype(np.int32)
+    L = int(level.max()) + 1 if len(level) else 0
+    order = np.argsort(level, kind="stable").astype(np.int32)
+    d = dict(n_reaches=len(reach_map), n_segments=int(seg_len.sum()), n_levels=L,
+             max_upstream=int(n_up.max()) if len(n_up) else 0, reach_seg_start=seg_start, reach_seg_len=seg_len,
+             reach_level=level, reach_n_up=n_up,
+             reach_up_ptr=np.concatenate([[0], np.cumsum(n_up)]).astype(np.int32),
+             reach_up_idx=(np.concatenate(ups) if len(ups) else np.zeros(0)).astype(np.int32),
+             level_ptr=np.concatenate([[0], np.cumsum(np.bincount(level, minlength=L))]).astype(np.int32),
+             level_reach=order)
+    return d, seg_map, reach_map
+
+
+def upstream_segments(topo):
+    """Segments in each reach's upstream closure (itself included); the
+    networks are trees, so the counts add up along the upstream lists."""
+    up_ptr, up_idx = np.asarray(topo.reach_up_ptr), np.asarray(topo.reach_up_idx)
+    total = np.asarray(topo.reach_seg_len).astype(np.int64).copy()
+    for r in range(topo.n_reaches):                    # ids are topological
+        a, b = up_ptr[r], up_ptr[r + 1]
+        if b > a:
+            total[r] += total[up_idx[a:b]].sum()
+    return total
+
+
+def sample_roots(topo, max_segments, total_segments, seed):
+    """Random reaches whose upstream closures hold at most max_segments
+    segments, until the closures hold about total_segments."""
+    size = upstream_segments(topo)
+    cand = np.flatnonzero(size <= max_segments)
+    rng = np.random.default_rng(seed)
+    rng.shuffle(cand)
+    out, acc = [], 0
+    for r in cand:
+        out.append(int(r))
+        acc += int(size[r])
+        if acc >= total_segments:
+            break
+    return out
+
+
+def flow_scale_error(got_hist, want_hist, reach_last0, topo, reach_ids=None):
+    """Per reach and step: |got - want| against the flow scale through the
+    reach, S = |Q_r(t)| + |Q_r(t-1)| + sum over upstream reaches u of
+    |Q_u(t)| + |Q_u(t-1)| (the magnitudes the routing combines), from the
+    `want` history [T, R] (columns reach_ids of the whole network's reaches
+    when given).  Returns (|diff| / S, plain relative |diff| / max(|want|, 1e-3))."""
+    want = np.asarray(want_hist, np.float64)
+    got = np.asarray(got_hist, np.float64)
+    prev = np.vstack([np.asarray(reach_last0, np.float64)[None], want[:-1]])
+    S = np.abs(want) + np.abs(prev)
+    up_ptr, up_idx = np.asarray(topo["reach_up_ptr"]), np.asarray(topo["reach_up_idx"])
+    for r in range(want.shape[1]):
+        u = up_idx[up_ptr[r]:up_ptr[r + 1]]
+        if len(u):
+            S[:, r] += np.abs(want[:, u]).sum(1) + np.abs(prev[:, u]).sum(1)
+    d = np.abs(got - want)
+    return d / np.maximum(S, 1e-300), d / np.maximum(np.abs(want), 1e-3)
