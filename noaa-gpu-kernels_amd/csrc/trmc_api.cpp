@@ -690,9 +690,9 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
         if (bpcu <= 0) return fail(TRMC_ERR_HIP, "dataflow kernel cannot be resident");
         long long resident = (long long)bpcu * net->cu_count;
         // CONVERGED/LEARNED on the interleaved kernel: the last blocks serve
-        // the certified solves (one per 8 CUs, at least 2), resident with the rest
+        // the certified solves (one per 4 CUs, at least 2), resident with the rest
         const bool servers = il && (mode == TRMC_MODE_CONVERGED || mode == TRMC_MODE_LEARNED);
-        const long long n_servers = servers ? std::max(2, net->cu_count / 8) : 0;
+        const long long n_servers = servers ? std::max(2, net->cu_count / 4) : 0;
         if (resident <= n_servers) return fail(TRMC_ERR_HIP, "dataflow kernel cannot be resident");
         long long need_blocks = ((long long)um + o.block_size - 1) / o.block_size;
         long long grid = std::max(1LL, std::min(need_blocks, resident - n_servers));

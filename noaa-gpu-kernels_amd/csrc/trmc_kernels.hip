@@ -430,58 +430,104 @@ __device__ __forceinline__ void scan_post(const RouteArgs &a, long long lane, in
                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// One certified solve by the whole wave, the device form of
+// certified_solve_lanes (trmc_solver.h): each round the 64 lanes take one
+// piece of the span each, side by side; the first unproven piece comes from a
+// ballot and its values from the lanes that hold them.  Every lane ends with
+// the same root, outflow and count.
+__device__ __forceinline__ int certified_solve_wave(double qup, double quc, double qdp, double ql, const Seg &c,
+                                                    const ConvCfg &k, double &h, double &q)
+{
+    const int lane = (int)(threadIdx.x & 63u);
+    Coef C0;
+    double qmn0;
+    const double f001 = residual(0.01, k.dt, qup, quc, qdp, ql, c, C0, qmn0);   // <= 0: why it was sent
+    const double rmax = routed_max(k.dt, qup, quc, qdp, ql);
+    ScanWave w;
+    scan_wave_begin(w, f001, outflow(C0, qup, quc, qdp));
+    int evals = 1;
+    double b_end = 0.0;
+    bool geo = false;
+    while (w.state == 0 && w.rounds < kScanCap && scan_wave_round(w, c, rmax, b_end, geo)) {
+        const double x0 = scan_piece_x(w, b_end, geo, lane);
+        const double x1 = scan_piece_x(w, b_end, geo, lane + 1);
+        ScanPiece pc = scan_piece_eval(x1, w.spos, k.dt, qup, quc, qdp, ql, c);
+        double f0 = __shfl_up(pc.f, 1u);                  // F(x0): the previous lane's right end
+        if (lane == 0) f0 = w.fa;
+        const bool proven = scan_piece_proof(x0, x1, f0, pc, w.spos, k.dt, qup, quc, qdp, ql, c, rmax);
+        const unsigned long long open = __ballot(!proven);
+        const int first = open ? (int)__ffsll((long long)open) - 1 : 64;
+        const int f = first < 64 ? first : 63;
+        const int left = (first == 0 || first == 64) ? 0 : first - 1;
+        ScanPiece P;
+        P.f = __shfl(pc.f, f);
+        P.q = __shfl(pc.q, f);
+        P.change = __shfl((int)pc.change, f) != 0;
+        P.proven = false;
+        double fl = __shfl(pc.f, left), qlv = __shfl(pc.q, left);
+        if (first == 0 || first == 64) {
+            fl = w.fa;
+            qlv = w.qa;
+        }
+        const double xf = __shfl(x0, f), xn = __shfl(x1, f);
+        const double f_end = __shfl(pc.f, 63), q_end = __shfl(pc.q, 63);
+        scan_wave_combine(w, first, xf, xn, fl, qlv, P, b_end, f_end, q_end);
+        ++evals;
+    }
+    if (w.state == 0) w.state = w.spos ? 4 : 1;           // the round cap
+    return evals + scan_wave_finish(w, qup, quc, qdp, ql, c, k, h, q);
+}
+
 template <int MODE>
 __device__ __forceinline__ void scan_server(const RouteArgs &a)
 {
-    // One loop for waiting and serving: a lane whose request has come serves
-    // it while the wave's other lanes go on waiting (a wait loop of its own
-    // would hold every lane of the wave until all of their requests came).
+    // One request at a time per wave, solved by all of its lanes: the wave
+    // takes the next ticket, waits for that request (every lane reads the
+    // same words, lane 0's values decide), and answers it from lane 0.
     const unsigned long long route_waves = (unsigned long long)a.n_route_blocks * ((blockDim.x + 63) / 64);
     const ConvCfg kc = {a.dt, a.tol, a.max_eval, a.learned_budget};
     const unsigned long long mask = (unsigned long long)(a.scan_cap - 1);
+    const int lane = (int)(threadIdx.x & 63u);
     // a routing lane waits on every answer: serve ahead of the routing waves
     // that share the CU (idle servers sleep, so they take little from them)
     __builtin_amdgcn_s_setprio(3);
-    unsigned long long my =
-        __hip_atomic_fetch_add(&a.scan_ctl[32], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
+        unsigned long long my = 0;
+        if (lane == 0) my = __hip_atomic_fetch_add(&a.scan_ctl[32], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        my = __shfl(my, 0);
         unsigned long long *e = &a.scan_ring[my & mask];
-        const unsigned long long v = __hip_atomic_load(e, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        if ((v >> kScanLaneBits) == my + 1ull) {
-            const long long lane = (long long)(v & ((1ull << kScanLaneBits) - 1ull));
-            const double *rq = a.scan_req + (size_t)lane * 5;
-            const double qup = rq[0], quc = rq[1], qdp = rq[2], ql = rq[3];
-            const int seg = (int)rq[4];
-            __hip_atomic_store(e, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);   // entry read
-            const long long ts0 = (long long)wall_clock64();
-            const Seg c = load_seg(a.segc, (size_t)a.n_dev_segs, seg);
-            SecantConv sv;
-            double h = 0.01, q = 0.0;
-            int n = 0;
-            if (!sv.start(qup, quc, qdp, ql, c, 0.0, false)) {
-                sv.stage = 8;                     // F(mindepth) first: the requester found it <= 0
-                while (!sv.step(c, kc)) {
-                }
-                h = sv.root_h;
-                q = sv.root_q;
-                n = sv.evals + sv.scan_n;
-            }
-            double *rs = a.scan_resp + (size_t)lane * 3;
+        unsigned long long v = 0;
+        for (;;) {
+            v = __shfl(__hip_atomic_load(e, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT), 0);
+            if ((v >> kScanLaneBits) == my + 1ull) break;
+            // no request for this ticket yet: none will come once every routing wave is done
+            const unsigned long long done = __shfl(
+                __hip_atomic_load(&a.scan_ctl[64], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), 0);
+            const int st = __shfl(__hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), 0);
+            if (done >= route_waves || st != KSTATUS_OK) return;
+            // requests are rare: poll every few microseconds, so that idle
+            // servers take no memory bandwidth or issue slots from the routing
+            __builtin_amdgcn_s_sleep(127);
+        }
+        const long long req = (long long)(v & ((1ull << kScanLaneBits) - 1ull));
+        const double *rq = a.scan_req + (size_t)req * 5;
+        const double qup = rq[0], quc = rq[1], qdp = rq[2], ql = rq[3];
+        const int seg = (int)rq[4];
+        // the entry is free for the ticket cap later (the request itself
+        // stays in the requesting lane's slot until it has its answer)
+        if (lane == 0) __hip_atomic_store(e, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const long long ts0 = (long long)wall_clock64();
+        const Seg c = load_seg(a.segc, (size_t)a.n_dev_segs, seg);
+        double h = 0.01, q = 0.0;
+        const int n = certified_solve_wave(qup, quc, qdp, ql, c, kc, h, q);
+        if (lane == 0) {
+            double *rs = a.scan_resp + (size_t)req * 3;
             rs[0] = h;
             rs[1] = q;
             rs[2] = (double)n;
-            __hip_atomic_store(&a.scan_resp_seq[lane], my + 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&a.scan_resp_seq[req], my + 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             atomicAdd(&a.counters[CNT_SCAN_SOLVE_TICKS], (unsigned long long)((long long)wall_clock64() - ts0));
-            my = __hip_atomic_fetch_add(&a.scan_ctl[32], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            continue;
         }
-        // no request for this ticket yet: none will come once every routing wave is done
-        if (__hip_atomic_load(&a.scan_ctl[64], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= route_waves ||
-            __hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != KSTATUS_OK)
-            return;
-        // requests are rare: poll every few microseconds, so that idle
-        // servers take no memory bandwidth or issue slots from the routing
-        __builtin_amdgcn_s_sleep(127);
     }
 }
 

@@ -1359,6 +1359,211 @@ struct SecantConv {
     }
 };
 
+// ---------------------------------------------- wave-parallel certified scan
+// One certified solve by the 64 lanes of a wave together (the server waves of
+// the dataflow kernel).  Each round splits the span ahead of the scan
+// position a, [a, b], into 64 pieces (geometric when b > 1.5 a, else even);
+// lane i evaluates F at its piece's right end and proves with the serial
+// scan's enclosures (the direct one, then the mean-value one with F at both
+// ends) that F has no point of the searched sign on its piece.  The first
+// piece that is not proven is where the scan goes on:
+//   * a sign change at its right end narrower than kScanRes relative ends the
+//     search for that sign (the first positive point; after it the bracket's
+//     right end); wider, the next round splits that piece;
+//   * no sign change: at the resolution the piece is passed, as the serial
+//     scan passes it; wider, the next round splits it.
+// Every piece before the first unproven one is proven, so the scan returns
+// the serial scan's leftmost downcrossing to the same resolution (not the
+// same points).  Pieces end at bankfull, where F jumps; above it the
+// floodplain enclosures apply.  Looking for F > 0, the search ends at
+// mindepth once Qmn on [a, inf) exceeds every routed flow.
+struct ScanWave {
+    double a, fa, qa;   // position, F(a) (NaN: not known), outflow at a
+    double b;           // end of the next round's span
+    bool spos;          // searching F <= 0 after a positive point
+    int rounds;
+    int state;          // 0 scanning, 1 root at mindepth, 2 bracket [lo, hi], 3 root at 1e4,
+                        // 4 root at a (the round cap while looking for F <= 0)
+    double lo, flo, qlo, hi, fhi, qhi;
+};
+
+struct ScanPiece {
+    double f, q;        // F and the outflow at the piece's right end
+    bool proven;        // no point of the searched sign on the piece
+    bool change;        // the searched sign at the right end
+};
+
+// the scan from mindepth, where F(0.01) = f001 <= 0 with outflow q001
+TRMC_HD void scan_wave_begin(ScanWave &w, double f001, double q001)
+{
+    w.a = 0.01;
+    w.fa = f001;
+    w.qa = q001;
+    w.b = 1.0e4;
+    w.spos = false;
+    w.rounds = 0;
+    w.state = 0;
+}
+
+// the round's span end and spacing; false when the scan has ended here
+TRMC_HD bool scan_wave_round(ScanWave &w, const Seg &c, double rmax, double &b_end, bool &geo)
+{
+    if (!w.spos) {
+        if (w.a >= 1.0e4 || qmn_floor(w.a, c) > rmax + 1e-12 * (fabs(rmax) + 1.0)) { w.state = 1; return false; }
+    } else if (w.a >= 1.0e4) {
+        w.state = 3;
+        return false;
+    }
+    b_end = fmin(w.b, 1.0e4);
+    const bool has_flood = c.twcc > 0.0 && c.bfd > 0.01 && c.bfd < 1.0e4;
+    if (has_flood && w.a < c.bfd && b_end > c.bfd) b_end = c.bfd;
+    geo = b_end > 1.5 * w.a;
+    return true;
+}
+
+// the left end of piece i (i = 64: the span's end), strictly increasing in i
+TRMC_HD double scan_piece_x(const ScanWave &w, double b_end, bool geo, int i)
+{
+    if (i <= 0) return w.a;
+    if (i >= 64) return b_end;
+    const double t = (double)i * (1.0 / 64.0);
+    return geo ? w.a * exp2(t * log2(b_end / w.a)) : w.a + (b_end - w.a) * t;
+}
+
+// F at the right end x1 of piece [x0, x1] (every lane)
+TRMC_HD ScanPiece scan_piece_eval(double x1, bool spos, double dt, double qup, double quc, double qdp, double ql,
+                                  const Seg &c)
+{
+    ScanPiece p;
+    Coef C;
+    double qmn;
+    p.f = residual(x1, dt, qup, quc, qdp, ql, c, C, qmn);
+    p.q = outflow(C, qup, quc, qdp);
+    p.change = spos ? !(p.f > 0.0) : p.f > 0.0;
+    p.proven = false;
+    return p;
+}
+
+// the proof of piece [x0, x1] with F(x0) = f0 (NaN: not known)
+TRMC_HD bool scan_piece_proof(double x0, double x1, double f0, const ScanPiece &p, bool spos, double dt, double qup,
+                              double quc, double qdp, double ql, const Seg &c, double rmax)
+{
+    if (p.change) return false;
+    if (!spos && qmn_floor(x0, c) > rmax + 1e-12 * (fabs(rmax) + 1.0)) return true;   // F < 0 on [x0, inf)
+    const bool has_flood = c.twcc > 0.0 && c.bfd > 0.01 && c.bfd < 1.0e4;
+    const bool flood = has_flood && x0 >= c.bfd;
+    const Enclosure en = f_enclosure(x0, x1, flood, dt, qup, quc, qdp, ql, c);
+    if (spos ? en.lo > 0.0 : !(en.hi > 0.0)) return true;
+    const bool have_f0 = !(flood && x0 <= c.bfd) && f0 == f0;
+    return slope_sign_holds(x0, x1, flood, f0, have_f0, p.f, spos, dt, qup, quc, qdp, ql, c);
+}
+
+// The round's outcome from the first unproven piece `first` (64: every piece
+// proven): its ends x_first < x_next, F and outflow at x_first (f_left,
+// q_left: F(a) for the first piece), its own right-end values, and the
+// values at the span's end b_end (piece 63's right end).
+TRMC_HD void scan_wave_combine(ScanWave &w, int first, double x_first, double x_next, double f_left, double q_left,
+                               const ScanPiece &pf, double b_end, double f_end, double q_end)
+{
+    ++w.rounds;
+    if (first >= 64) {                                   // all proven: on past the span, twice as far
+        const double a0 = w.a;
+        w.a = b_end;
+        w.fa = f_end;
+        w.qa = q_end;
+        w.b = b_end + 2.0 * (b_end - a0);
+        return;
+    }
+    const double width = x_next - x_first;
+    w.a = x_first;
+    w.fa = f_left;
+    w.qa = q_left;
+    if (width > kScanRes * x_first) {                    // split this piece next
+        w.b = x_next;
+        return;
+    }
+    if (pf.change && w.spos) {                           // the bracket [x_first, x_next]
+        w.state = 2;
+        w.lo = x_first; w.flo = f_left; w.qlo = q_left;
+        w.hi = x_next; w.fhi = pf.f; w.qhi = pf.q;
+        return;
+    }
+    if (pf.change) w.spos = true;                        // the first positive point
+    w.a = x_next;                                        // (or an unproven piece at the resolution: passed)
+    w.fa = pf.f;
+    w.qa = pf.q;
+    w.b = pf.change ? 1.0e4 : x_next + 256.0 * width;
+}
+
+// After the scan: the root and its outflow, with the secant on the bracket
+// (state 2) as SecantConv solves it; returns the evaluations it took.
+TRMC_HD int scan_wave_finish(const ScanWave &w, double qup, double quc, double qdp, double ql, const Seg &c,
+                             const ConvCfg &k, double &h, double &q)
+{
+    SecantConv s;
+    s.qup = qup; s.quc = quc; s.qdp = qdp; s.ql = ql;
+    s.evals = 0;
+    s.scan_n = 0;
+    s.learned = false; s.need_guess = false; s.fell_back = false; s.capped = false;
+    s.trivial = false; s.have_root = false;
+    s.inbank = false;
+    if (w.state != 2) {                                  // mindepth, the search's end, or the cap
+        const double at = (w.state == 1) ? 0.01 : (w.state == 3) ? 1.0e4 : w.a;
+        s.eval(at, c, k);
+        h = at;
+        q = s.q_cur;
+        return s.evals;
+    }
+    s.certified = true;
+    s.lo_pos = w.lo; s.hi_neg = w.hi; s.neg_seen = true;
+    s.ha = w.lo; s.fa = w.flo; s.qa = w.qlo;
+    s.hb = w.hi; s.fb = w.fhi; s.qb = w.qhi;
+    s.h_cur = w.hi; s.q_cur = w.qhi;
+    s.s_prev = s.s_prev2 = 1.0e30;
+    s.stage = 2;
+    while (!s.step(c, k)) {
+    }
+    h = s.root_h;
+    q = s.root_q;
+    return s.evals;
+}
+
+// The wave-parallel scan with its 64 lanes run one after another (host
+// builds and tests; the device runs the lanes side by side).
+TRMC_HD int certified_solve_lanes(double qup, double quc, double qdp, double ql, const Seg &c, const ConvCfg &k,
+                                  double &h, double &q)
+{
+    Coef C0;
+    double qmn0;
+    const double f001 = residual(0.01, k.dt, qup, quc, qdp, ql, c, C0, qmn0);
+    const double q001 = outflow(C0, qup, quc, qdp);
+    if (f001 > 0.0) return -1;                           // not a certified solve
+    const double rmax = routed_max(k.dt, qup, quc, qdp, ql);
+    ScanWave w;
+    scan_wave_begin(w, f001, q001);
+    int evals = 1;
+    double b_end = 0.0;
+    bool geo = false;
+    while (w.state == 0 && w.rounds < kScanCap && scan_wave_round(w, c, rmax, b_end, geo)) {
+        ScanPiece pc[64];
+        double x[65];
+        for (int i = 0; i <= 64; ++i) x[i] = scan_piece_x(w, b_end, geo, i);
+        for (int i = 0; i < 64; ++i) pc[i] = scan_piece_eval(x[i + 1], w.spos, k.dt, qup, quc, qdp, ql, c);
+        int first = 64;
+        for (int i = 0; i < 64 && first == 64; ++i) {
+            const double f0 = (i == 0) ? w.fa : pc[i - 1].f;
+            if (!scan_piece_proof(x[i], x[i + 1], f0, pc[i], w.spos, k.dt, qup, quc, qdp, ql, c, rmax)) first = i;
+        }
+        evals += 1;
+        const double fl = (first == 0 || first == 64) ? w.fa : pc[first - 1].f;
+        const double ql_ = (first == 0 || first == 64) ? w.qa : pc[first - 1].q;
+        const int f = first < 64 ? first : 63;
+        scan_wave_combine(w, first, x[f], x[f + 1], fl, ql_, pc[f], b_end, pc[63].f, pc[63].q);
+    }
+    if (w.state == 0) w.state = w.spos ? 4 : 1;          // the round cap: as the serial scan's cap
+    return evals + scan_wave_finish(w, qup, quc, qdp, ql, c, k, h, q);
+}
+
 // Whole CONVERGED solve from the reference bracket (no MLP).
 TRMC_HD SolveOut solve_converged(double dt, double qup, double quc, double qdp, double ql, const Seg &c,
                                  double depthp, double tol, int max_eval)

@@ -111,3 +111,18 @@ extern "C" int shim_solve_f32(const float *segc, const float *in, float dt, floa
     out[2] = (float)o.vel;
     return o.iters;
 }
+
+// The wave-parallel certified scan with its lanes run in turn: -1 when
+// F(mindepth) > 0 (not a certified solve), else the evaluations (one per
+// round plus the secant's).
+extern "C" int shim_certified_lanes(const double *segc, const double *in, double dt, double tol, int max_eval,
+                                    double *out)
+{
+    Seg c = load_seg(segc, 1, 0);
+    const ConvCfg k = {dt, tol, max_eval, 0};
+    double h = 0.0, q = 0.0;
+    const int n = certified_solve_lanes(in[0], in[1], in[2], in[3], c, k, h, q);
+    out[0] = q;
+    out[1] = h;
+    return n;
+}

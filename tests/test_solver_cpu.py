@@ -265,3 +265,30 @@ def test_reference_f32_rule_bitwise_against_cr_oracle(shim32):
         if not np.array_equal(out.view(np.uint32), want.view(np.uint32)):
             bad.append((p, x, out, want))
     assert not bad, bad[:3]
+
+
+def test_wave_parallel_certified_scan_lands_on_the_oracle_root(shim):
+    """The servers' certified scan (64 pieces per round, the lanes run in turn
+    on the host) returns the brute-force leftmost downcrossing of every
+    fixture case where F(1 cm) <= 0, in a few rounds where the serial scan
+    takes tens of steps."""
+    pD = C.POINTER(C.c_double)
+    shim.shim_certified_lanes.restype = C.c_int
+    shim.shim_certified_lanes.argtypes = [pD, pD, C.c_double, C.c_double, C.c_int, pD]
+    z = golden("converged_cases.npz")
+    dt = float(z["dt"])
+    n = 0
+    for i, (p, x, root, cat) in enumerate(zip(z["params"], z["inputs"], z["root"], z["cat"])):
+        seg = segc_row(p)
+        xi = np.ascontiguousarray(list(x) + [1.0], np.float64)
+        o = np.zeros(2)
+        ev = shim.shim_certified_lanes(seg.ctypes.data_as(pD), xi.ctypes.data_as(pD), dt, 1e-13, 60,
+                                       o.ctypes.data_as(pD))
+        if ev < 0:
+            continue
+        n += 1
+        want_q = orc.solve_converged(p, *x, 1.0, dt=dt)[0]
+        assert abs(o[1] - root) <= 1e-9 * root, (i, cat, o, root)
+        assert abs(o[0] - want_q) <= 1e-9 * max(abs(want_q), 1e-3), (i, cat, o, want_q)
+        assert ev <= 40, (i, cat, ev)
+    assert n >= 10
