@@ -237,32 +237,58 @@ int trmc_ingest_nex_csv_rules(const char *csv_dir, const int64_t *topo_to_wbid, 
         }
     });
     if (failed.load()) return trmc::fail(TRMC_ERR_NOMEM, "out of host memory reading the forcing files");
-    // one owner per reach column: of the files naming it, the last in name
-    // order (a fixed choice, whatever order the directory lists them in)
+    // One owner per reach column: of the files naming it, the last in name
+    // order (a fixed choice, whatever order the directory lists them in).
+    // C++ rules: every file writes its own rows, so a reach named by several
+    // files takes, cell by cell, the row of the last file in name order that
+    // has that timestep; the owner's thread applies the group in name order
+    // (columns stay disjoint between threads).  Rust rules: the owner's rows
+    // alone (the whole column of one file).
     std::vector<int> owner(R, -1);
+    std::vector<char> several(R, 0);             // named by more than one file
+    std::vector<int> group_of(R, -1);            // C++ rules: reach -> its group (several files only)
+    std::vector<int> group_ptr, group_files;     // files of those reaches, in name order
     for (size_t pi = 0; pi < names.size(); ++pi) {
         const int r = reach_of[pi];
-        if (r >= 0 && (owner[r] < 0 || names[pi] > names[owner[r]])) owner[r] = (int)pi;
+        if (r < 0) continue;
+        if (owner[r] >= 0) several[r] = 1;
+        if (owner[r] < 0 || names[pi] > names[owner[r]]) owner[r] = (int)pi;
+    }
+    if (rules == TRMC_INGEST_CPP) {
+        std::vector<std::pair<int, int>> shared;    // (reach, file)
+        for (size_t pi = 0; pi < names.size(); ++pi)
+            if (reach_of[pi] >= 0 && several[reach_of[pi]]) shared.emplace_back(reach_of[pi], (int)pi);
+        std::sort(shared.begin(), shared.end(), [&](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+            return x.first != y.first ? x.first < y.first : names[x.second] < names[y.second];
+        });
+        for (size_t i = 0; i < shared.size(); ++i) {
+            if (i == 0 || shared[i].first != shared[i - 1].first) {
+                group_of[shared[i].first] = (int)group_ptr.size();
+                group_ptr.push_back((int)group_files.size());
+            }
+            group_files.push_back(shared[i].second);
+        }
+        group_ptr.push_back((int)group_files.size());
     }
     auto work = [&](size_t b, size_t e) {
         std::vector<char> buf(4096);
         std::string path;
-        for (size_t pi = b; pi < e; ++pi) {
-            const int r = reach_of[pi];
-            if (r < 0) { missing.fetch_add(1); continue; }
+        // read one file whole into buf; false when it cannot be read
+        auto read_file = [&](size_t pi, size_t &got) {
             path.assign(dir).append(names[pi]);
             FILE *fp = std::fopen(path.c_str(), "rb");
-            if (!fp) { missing.fetch_add(1); continue; }
+            if (!fp) return false;
             std::fseek(fp, 0, SEEK_END);
             const long size = std::ftell(fp);
             std::fseek(fp, 0, SEEK_SET);
-            if (size < 0) { std::fclose(fp); missing.fetch_add(1); continue; }
+            if (size < 0) { std::fclose(fp); return false; }
             if ((long)buf.size() < size + 1) buf.resize((size_t)size + 1);
-            const size_t got = std::fread(buf.data(), 1, (size_t)size, fp);
+            got = std::fread(buf.data(), 1, (size_t)size, fp);
             buf[got] = 0;
             std::fclose(fp);
-            matched.fetch_add(1);
-            if (owner[r] != (int)pi) continue;   // a later file names the same reach
+            return true;
+        };
+        auto apply_rows = [&](size_t got, int r) {
             size_t i = 0;
             while (i < got) {
                 const size_t ls = i;
@@ -281,6 +307,27 @@ int trmc_ingest_nex_csv_rules(const char *csv_dir, const int64_t *topo_to_wbid, 
                     if (ts >= 0 && ts < n_timesteps) qlat[(size_t)ts * R + r] = val;
                 }
             }
+        };
+        for (size_t pi = b; pi < e; ++pi) {
+            const int r = reach_of[pi];
+            if (r < 0) { missing.fetch_add(1); continue; }
+            if (group_of[r] >= 0) {
+                // the group's files are read, counted and applied by the owner's thread
+                if (owner[r] != (int)pi) continue;
+                const int gi = group_of[r];
+                for (int f = group_ptr[gi]; f < group_ptr[gi + 1]; ++f) {
+                    size_t got = 0;
+                    if (!read_file((size_t)group_files[f], got)) { missing.fetch_add(1); continue; }
+                    matched.fetch_add(1);
+                    apply_rows(got, r);
+                }
+                continue;
+            }
+            size_t got = 0;
+            if (!read_file(pi, got)) { missing.fetch_add(1); continue; }
+            matched.fetch_add(1);
+            if (owner[r] != (int)pi) continue;   // Rust rules: a later file names the same reach
+            apply_rows(got, r);
         }
     };
     run_ranges(work);

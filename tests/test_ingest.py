@@ -153,3 +153,68 @@ def test_unknown_rules_are_an_error(tmp_path):
     with pytest.raises(trmc.TrmcError) as e:
         trmc.ingest_nex_csv(str(tmp_path), np.arange(3), 4, rules=7)
     assert e.value.code == trmc.ERR_ARG
+
+
+def _write_shared(d, T=24):
+    """Three reaches; reach 1 is named by "nex-5_output.csv" and by
+    "nex-05_output.csv" (the same id, so the same column), and reach 2 by
+    "nex-7_output.csv" and "nex-7_output.csv.bak" (the C++ filter accepts any
+    name holding "_output.csv")."""
+    wbid = np.array([3, 5, 7], np.int64)
+    np.save(os.path.join(d, "topo_to_wbid.npy"), wbid)
+
+    def put(name, rows):
+        with open(os.path.join(d, name), "w") as f:
+            f.write("".join(f"{t}, 2017-01-01, {v}\n" for t, v in rows))
+
+    put("nex-3_output.csv", [(t, 1.0 + t) for t in range(T)])
+    put("nex-5_output.csv", [(t, 10.0 + t) for t in range(0, T, 2)])     # even timesteps
+    put("nex-05_output.csv", [(t, 20.0 + t) for t in range(1, T, 2)])    # odd timesteps
+    put("nex-7_output.csv", [(t, 30.0 + t) for t in range(T)])
+    put("nex-7_output.csv.bak", [(t, 40.0 + t) for t in range(0, T, 3)])  # every third, later by name
+    return wbid
+
+
+def test_ingest_applies_every_file_of_a_reach_cell_by_cell(tmp_path):
+    """C++ rules: every file writes its own rows (parse_qlat_fast_cpp.cpp:
+    164-176), so the rows of two files naming one reach all land; where both
+    have a timestep the later file by name wins."""
+    T = 24
+    wbid = _write_shared(str(tmp_path), T)
+    t = np.arange(T, dtype=np.float32)
+    want = np.zeros((T, 3), np.float32)
+    want[:, 0] = 1.0 + t
+    want[:, 1] = np.where(t % 2 == 0, 10.0 + t, 20.0 + t)
+    want[:, 2] = np.where(t % 3 == 0, 40.0 + t, 30.0 + t)
+    for threads in (1, 2, 5):
+        q, read, skipped = trmc.ingest_nex_csv(str(tmp_path), wbid, T, n_threads=threads)
+        assert np.array_equal(q, want), threads
+        assert read == 5 and skipped == 0
+
+
+def test_ingest_shared_reach_without_overlap_matches_reference_program(tmp_path):
+    """Where the files of one reach hold disjoint timesteps the reference's
+    result does not depend on its thread order: byte for byte."""
+    if not os.path.exists(REF):
+        pytest.skip("reference build (oracle/_ref) not present")
+    d = str(tmp_path)
+    _write_shared(d, 24)
+    os.remove(os.path.join(d, "nex-7_output.csv.bak"))     # the overlapping pair
+    npy = os.path.join(d, "topo_to_wbid.npy")
+    ref_out, our_out = os.path.join(d, "ref.bin"), os.path.join(d, "our.bin")
+    subprocess.run([REF, d, npy, ref_out, "24", "2"], check=True, capture_output=True, timeout=60)
+    trmc.ingest_nex_csv_to_forcings(d, npy, our_out, 24, n_threads=3)
+    with open(ref_out, "rb") as a, open(our_out, "rb") as b:
+        assert a.read() == b.read()
+
+
+def test_ingest_rust_rules_keep_one_file_per_reach(tmp_path):
+    """Rust rules: one file's whole column per reach (the last by name here)."""
+    T = 24
+    wbid = _write_shared(str(tmp_path), T)
+    q, _, _ = trmc.ingest_nex_csv(str(tmp_path), wbid, T, n_threads=2, rules=trmc.INGEST_RUST)
+    t = np.arange(T, dtype=np.float32)
+    # "nex-05" sorts before "nex-5": the column of nex-5 (even steps only);
+    # nex-7_output.csv.bak does not end in _output.csv and is not read
+    assert np.array_equal(q[:, 1], np.where(t % 2 == 0, 10.0 + t, 0.0).astype(np.float32))
+    assert np.array_equal(q[:, 2], 30.0 + t)
