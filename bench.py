@@ -210,17 +210,21 @@ def train_mlp(trmc, args):
     return W, rec
 
 
-def cpu_baseline(trmc, w, T_sample, dt):
+def cpu_baseline(trmc, w, T_sample, dt, t_first=0, state0=None):
     """Times the reference routine (or the oracle restatement when the
-    reference build is absent) on one core over member 0's first T_sample
-    steps; returns (record, outputs, oracle module)."""
+    reference build is absent) on one core over member 0's steps t_first ..
+    t_first + T_sample - 1, from the initial state (t_first = 0) or from
+    state0 = (qdc, depth, reach_last) after t_first steps (the routine takes
+    them as float, as its initial state); returns (record, outputs, oracle
+    module)."""
     orc = _load("trmc_oracle", os.path.join(ROOT, "oracle", "oracle.py"))
     ref = orc.load_ref()
     topo = orc.as_topo(w.topo)
     N, R = w.topo.n_segments, w.topo.n_reaches
     p32 = np.ascontiguousarray(w.params, np.float32)
-    q32 = np.ascontiguousarray(w.qlat[0][:T_sample], np.float32)
-    q0, d0, r0 = (np.ascontiguousarray(x, np.float32) for x in (w.qdp0, w.dp0, w.reach_last0))
+    q32 = np.ascontiguousarray(w.qlat[0][t_first:t_first + T_sample], np.float32)
+    s0 = (w.qdp0, w.dp0, w.reach_last0) if state0 is None else state0
+    q0, d0, r0 = (np.ascontiguousarray(x, np.float32) for x in s0)
     t0 = time.perf_counter()
     if ref is not None:
         import ctypes as C
@@ -238,8 +242,9 @@ def cpu_baseline(trmc, w, T_sample, dt):
         qo, ro, _ = orc.wavefront_f32in(topo, p32, q32, q0, d0, r0, T_sample, dt)
         kind = "port"
     secs = time.perf_counter() - t0
+    window = f"first {T_sample} timesteps" if t_first == 0 else f"timesteps {t_first + 1}-{t_first + T_sample}"
     rec = {"value": N * T_sample / secs, "unit": "segment-steps/s", "cores": 1, "kind": kind,
-           "sample": f"member 0, first {T_sample} timesteps, all {N} segments, one thread ({secs:.1f} s); "
+           "sample": f"member 0, {window}, all {N} segments, one thread ({secs:.1f} s); "
                      f"host has {os.cpu_count()} logical CPUs"}
     return rec, (qo, ro), orc
 
@@ -637,6 +642,8 @@ def main():
     ap.add_argument("--batch-iters", type=int, default=12)
     ap.add_argument("--tol", type=float, default=1e-13, help="CONVERGED/LEARNED relative depth tolerance")
     ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=0,
+                    help="DATAFLOW kernel: 0 interleaved, 1 whole item per pass")
     ap.add_argument("--schedule", type=int, default=0, help="0 auto, 1 skewed, 2 level, 3 dataflow")
     ap.add_argument("--autotune", type=int, default=-1,
                     help="probe steps for the library autotuner (0 off; default: on for batch workloads)")
@@ -692,7 +699,8 @@ def main():
         weights, mlp_rec = train_mlp(trmc, args)
 
     opts = dict(schedule=args.schedule, unit_segments=args.unit_segments, block_size=args.block,
-                grid_blocks=args.grid, batch_iters=args.batch_iters, tol=args.tol)
+                grid_blocks=args.grid, batch_iters=args.batch_iters, tol=args.tol,
+                dataflow_variant=args.variant)
     tuned = None
     n_probe = args.autotune if args.autotune >= 0 else (16 if spec["kind"] == "batch" else 0)
     if n_probe > 0:
@@ -700,11 +708,12 @@ def main():
         best, ms = trmc.autotune(w.topo, w.params, w.qdp0, w.dp0, w.reach_last0, n_members=M, n_probe=n_probe,
                                  qlat=w.qlat[:, :n_probe], mlp_weights=weights, mode=MODES[head],
                                  schedule=args.schedule)
-        for k in ("unit_segments", "block_size", "batch_iters"):
+        for k in ("unit_segments", "block_size", "batch_iters", "schedule", "dataflow_variant"):
             opts[k] = int(best[k])
         tuned = {"probe_steps": n_probe, "ms_per_probe_step": ms, "seconds": time.perf_counter() - t1,
                  "unit_segments": opts["unit_segments"], "block": opts["block_size"],
-                 "batch_iters": opts["batch_iters"]}
+                 "batch_iters": opts["batch_iters"], "schedule": opts["schedule"],
+                 "dataflow_variant": opts["dataflow_variant"]}
         log(f"[bench] autotune: {tuned}")
 
     def mode_opts(name):
@@ -832,6 +841,31 @@ def main():
             "libm_pow_vs_cr_pow_max_rel": float(np.max(np.abs(qr8 - qcr) / np.maximum(np.abs(qcr), 1e-300))),
             "seconds": time.perf_counter() - t0}
         out["speedup_vs_cpu"] = value / rec["value"]
+        # The sample is the run's first steps.  The same number of steps at
+        # the end of the run, from the state the GPU REFERENCE rule reaches
+        # there, is timed too, and the REFERENCE rule's evaluations per
+        # segment-step in both windows are set beside its whole-run mean, so
+        # the sample's work per segment-step can be compared with the pass's.
+        if T > Tc:
+            t_late = T - Tc
+            rnet.reset()
+            st_a = rnet.run(Tc, dt)
+            lnet = new_net(trmc, w, M, 0, mode_opts("reference"), T)    # the whole forcing
+            lnet.reset()
+            lnet.run(t_late, dt)
+            ql_, dl_, _, rll_ = lnet.state()
+            st_b = lnet.run(Tc, dt)
+            lnet.close()
+            rec_late, _, _ = cpu_baseline(trmc, w, Tc, dt, t_first=t_late, state0=(ql_[0], dl_[0], rll_[0]))
+            ref_all = recs.get("reference")
+            denom = w.topo.n_segments * M * Tc
+            out["cpu_baseline_late_window"] = dict(
+                rec_late,
+                gpu_reference_evaluations_per_seg_step={
+                    "first_window": st_a["secant_iters"] / denom, "late_window": st_b["secant_iters"] / denom,
+                    "whole_run": ref_all["evaluations_per_seg_step"] if ref_all else None})
+            log(f"[bench] cpu late window {rec_late['value'] / 1e6:.2f} M seg-steps/s; "
+                f"{out['cpu_baseline_late_window']['gpu_reference_evaluations_per_seg_step']}")
         # a second, labelled baseline: the oracle restatement (same bits as the
         # one-core run) with each level's reaches on all host threads
         nthr = min(16, os.cpu_count() or 1)

@@ -467,13 +467,20 @@ int trmc_network_info(const trmc_network *net, int n_timesteps, int64_t *info, i
 int trmc_stream_copy(int device, size_t bytes, int reps, double *gbps);
 
 /* [new] Autotuner for the schedule of a network (or a union of networks):
- * times candidate unit sizes (2, 4, 8 segments), interleaving batch sizes
- * (6, 8, 12) and workgroup sizes (64, 128) on the first n_probe steps of the
- * forcing -- qlat [n_members][n_probe][n_segments], or generated on the
- * device from forcing_seed when qlat is NULL -- from the given state, and
- * returns the options with the fastest kernel time (base's other fields
- * kept) and that time per step.  qdp0/dp0 are [n_members][n_segments]
- * (base->n_members); mlp_weights is needed for LEARNED only. */
+ * times candidate schedules on the first n_probe steps of the forcing --
+ * qlat [n_members][n_probe][n_segments], or generated on the device from
+ * forcing_seed when qlat is NULL -- from the given state, and returns the
+ * options with the fastest kernel time (base's other fields kept) and that
+ * time per step.  Candidates, each at unit sizes 2, 4 and 8 segments:
+ *   tile      DATAFLOW, interleaved (dataflow_variant 0), solve batch 6, 8,
+ *             12, workgroup 64 or 128;
+ *   whole     DATAFLOW, one whole item per lane pass (variant 1), 64 or 128;
+ *   wavefront SKEWED (super-level wavefront between device-scope barriers),
+ *             workgroup 64 or 256.
+ * A base->schedule other than TRMC_SCHED_AUTO restricts the search to that
+ * schedule; LINEAR and REFERENCE_F32 search the tile schedule only.
+ * qdp0/dp0 are [n_members][n_segments] (base->n_members); mlp_weights is
+ * needed for LEARNED only. */
 int trmc_autotune(const int *reach_seg_start, const int *reach_seg_len, const int *reach_up_ptr,
                   const int *reach_up_idx, const int *level_ptr, const int *level_reach, int n_reaches,
                   int n_levels, int n_segments, const trmc_chparams_f64 *params, const double *qdp0_seg,

@@ -919,8 +919,9 @@ def mlp_predict(weights, features, use_mfma=True) -> np.ndarray:
 
 def autotune(topo: Topology, params, qdp0, dp0, reach_last0=None, n_members: int = 1, n_probe: int = 16,
              qlat=None, forcing_seed: int = 0, dt: float = 300.0, mlp_weights=None, **options):
-    """trmc_autotune: the fastest (unit_segments, batch_iters, block_size)
-    on the first n_probe steps.  Returns (options dict, ms per step)."""
+    """trmc_autotune: the fastest schedule (tile / whole item / wavefront),
+    unit_segments, batch_iters and block_size on the first n_probe steps.
+    Returns (options dict, ms per step)."""
     base = default_options(n_members=n_members, **options)
     p = params_f64(params)
     M, N = int(n_members), topo.n_segments

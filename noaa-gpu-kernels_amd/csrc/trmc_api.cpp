@@ -816,6 +816,18 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
                          issued, cnt[CNT_SCAN_WAIT_TICKS] / (net->wall_khz * 1e-3) / issued,
                          cnt[CNT_SCAN_SOLVE_TICKS] / (net->wall_khz * 1e-3) / issued);
     }
+    if (cnt[CNT_TRIPS] && std::getenv("TRMC_DEBUG_TRIPS")) {
+        const double us = 1e3 / net->wall_khz;      // microseconds per wall-clock tick
+        std::fprintf(stderr,
+                     "[trmc] trips %llu (%.2f us each), solve trips %llu (%.2f us each), lanes per solve trip %.1f, "
+                     "wave time %.1f ms summed\n",
+                     cnt[CNT_TRIPS], cnt[CNT_BLOCK_TICKS] * us * (o.block_size / 64.0) / (double)cnt[CNT_TRIPS],
+                     cnt[CNT_STEP_TRIPS],
+                     cnt[CNT_STEP_TRIPS] ? cnt[CNT_BLOCK_TICKS] * us * (o.block_size / 64.0) / cnt[CNT_STEP_TRIPS]
+                                         : 0.0,
+                     cnt[CNT_STEP_TRIPS] ? (double)cnt[CNT_STEP_LANES] / cnt[CNT_STEP_TRIPS] : 0.0,
+                     cnt[CNT_BLOCK_TICKS] * us * 1e-3);
+    }
     if (status == KSTATUS_TIMEOUT)
         return fail(TRMC_ERR_TIMEOUT, "persistent routing kernel: barrier wait exceeded 30 s");
     net->t_cursor += T;
@@ -1217,6 +1229,10 @@ static int matrix_run(const trmc_csr *PQ, const trmc_csr *Pq, const float *qlat,
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    // TRMC_MATRIX_MAX_GRID (tests): a smaller grid, so that the step
+    // kernel's loop over its block table takes more than one trip
+    const char *cap = std::getenv("TRMC_MATRIX_MAX_GRID");
+    const int max_grid = cap ? std::atoi(cap) : 0;
     hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(TRMC_ERR_HIP, "stream: %s", hipGetErrorString(e));
     rc = dev_upload(&d_slots, slots.data(), slots.size(), st);
@@ -1240,7 +1256,7 @@ static int matrix_run(const trmc_csr *PQ, const trmc_csr *Pq, const float *qlat,
                 const float2 *zi = (t & 1) ? d_zb : d_za;
                 float2 *zo = (t & 1) ? d_za : d_zb;
                 le = launch_matrix_step((int)blocks.size(), d_blk, d_slots, d_col, d_val, zi,
-                                        d_ql + (size_t)((t + 1) % T) * n, zo, st);
+                                        d_ql + (size_t)((t + 1) % T) * n, zo, st, max_grid);
             }
             e = hipStreamEndCapture(st, &graph);
             if (e == hipSuccess) e = le;
@@ -1464,9 +1480,29 @@ int trmc_autotune(const int *reach_seg_start, const int *reach_seg_len, const in
     if (!params || !qdp0_seg || !dp0_seg || !best || n_probe <= 0) return fail(TRMC_ERR_ARG, "bad argument");
     trmc_options o0;
     if (base) o0 = *base; else trmc_default_options(&o0);
+    // The schedules searched (when base leaves the schedule to the library):
+    //   tile      DATAFLOW, interleaved: each lane owns a tile of units (one
+    //             member each) and the wave interleaves their solves; solve
+    //             batch and workgroup size searched;
+    //   whole     DATAFLOW, one whole item per lane pass (variant 1);
+    //   wavefront SKEWED: the level wavefront of the reference, one lane per
+    //             (unit, member) per super-level between device-scope barriers.
+    // LINEAR and REFERENCE_F32 run on the tile schedule only.
+    struct Cand { int sched, variant, batch, block; };
     static const int kUnits[] = {2, 4, 8};
-    static const int kBatch[] = {6, 8, 12};
-    static const int kBlock[] = {64, 128};
+    std::vector<Cand> cands;
+    const bool any_sched = o0.schedule == TRMC_SCHED_AUTO;
+    const bool tile_only = o0.mode == TRMC_MODE_LINEAR || o0.mode == TRMC_MODE_REFERENCE_F32;
+    const bool want_df = any_sched || o0.schedule == TRMC_SCHED_DATAFLOW;
+    const bool want_sk = !tile_only && (any_sched || o0.schedule == TRMC_SCHED_SKEWED);
+    if (want_df)
+        for (int bi : {6, 8, 12})
+            for (int blk : {64, 128}) cands.push_back({TRMC_SCHED_DATAFLOW, 0, bi, blk});
+    if (want_df && !tile_only)
+        for (int blk : {64, 128}) cands.push_back({TRMC_SCHED_DATAFLOW, 1, 0, blk});
+    if (want_sk)
+        for (int blk : {64, 256}) cands.push_back({TRMC_SCHED_SKEWED, 0, 0, blk});
+    if (cands.empty()) cands.push_back({o0.schedule, o0.dataflow_variant, 0, o0.block_size});
     double best_ms = 1e300;
     trmc_options winner = o0;
     for (int us : kUnits) {
@@ -1481,20 +1517,21 @@ int trmc_autotune(const int *reach_seg_start, const int *reach_seg_len, const in
         if (!rc) rc = qlat ? trmc_network_set_forcing_f64(net, qlat, n_probe)
                            : trmc_network_generate_forcing(net, n_probe, forcing_seed);
         if (!rc && o.mode == TRMC_MODE_LEARNED && mlp_weights) rc = trmc_network_set_mlp(net, mlp_weights);
-        for (int bi : kBatch) {
-            for (int blk : kBlock) {
-                if (rc) break;
-                net->opt.batch_iters = bi;
-                net->opt.block_size = blk;
-                trmc_run_stats st;
-                rc = trmc_network_run(net, n_probe, dt, &st);
-                if (!rc) rc = trmc_network_reset(net);
-                if (!rc) rc = trmc_network_run(net, n_probe, dt, &st);
-                if (!rc) rc = trmc_network_reset(net);
-                if (!rc && st.kernel_ms < best_ms) {
-                    best_ms = st.kernel_ms;
-                    winner = net->opt;
-                }
+        for (const Cand &cd : cands) {
+            if (rc) break;
+            if (cd.sched == TRMC_SCHED_DATAFLOW && net->plan.max_upstream > kDataflowMaxUp) continue;
+            net->opt.schedule = cd.sched;
+            net->opt.dataflow_variant = cd.variant;
+            if (cd.batch > 0) net->opt.batch_iters = cd.batch;
+            net->opt.block_size = cd.block;
+            trmc_run_stats st;
+            rc = trmc_network_run(net, n_probe, dt, &st);
+            if (!rc) rc = trmc_network_reset(net);
+            if (!rc) rc = trmc_network_run(net, n_probe, dt, &st);
+            if (!rc) rc = trmc_network_reset(net);
+            if (!rc && st.kernel_ms < best_ms) {
+                best_ms = st.kernel_ms;
+                winner = net->opt;
             }
         }
         if (net) trmc_network_destroy(net);

@@ -31,6 +31,9 @@ enum CounterSlot {
     CNT_MLP_TICKS,      // LEARNED: wall-clock ticks inside them, summed over waves
     CNT_SCAN_WAIT_TICKS,   // certified solves: ticks from request to answer, summed over requests
     CNT_SCAN_SOLVE_TICKS,  // certified solves: ticks a server lane spends solving, summed
+    CNT_TRIPS,          // interleaved kernels: loop trips, summed over waves
+    CNT_STEP_TRIPS,     // interleaved kernels: trips that ran a solve step, summed over waves
+    CNT_STEP_LANES,     // interleaved kernels: lane solve steps, summed over lanes
     CNT_COUNT
 };
 
@@ -106,8 +109,11 @@ hipError_t launch_linear_coeffs(const float *params, int n, float dt, int fixed_
 // col[k], val[k] = {P_Q value, P_q value}; z[c] = {Q, ql}; blocks[b] = {G in
 // 1, 2, 4, ..., 64, first slot, slots <= 256 / G, 0}; slots[i] = {row, first
 // entry, end, 0}; writes z_out[row] = {new Q, ql_next[row]}
+// max_grid > 0 caps the launched grid below its default of 2^20 blocks (the
+// kernel loops over the block table either way)
 hipError_t launch_matrix_step(int n_blocks, const int4 *blocks, const int4 *slots, const int *col, const float2 *val,
-                              const float2 *z, const float *ql_next, float2 *z_out, hipStream_t stream);
+                              const float2 *z, const float *ql_next, float2 *z_out, hipStream_t stream,
+                              int max_grid = 0);
 hipError_t launch_gen_forcing(double *dst, const int *seg_of_dev, int nd, int N, int M, int T,
                               unsigned long long seed, hipStream_t stream);
 hipError_t launch_reset(unsigned *bar, int nwords, hipStream_t stream);
@@ -121,6 +127,10 @@ hipError_t route_dataflow_occupancy(int mode, int block, int *blocks_per_cu);
 // secant iterations interleaved across lanes (every mode)
 hipError_t launch_route_dataflow_il(const RouteArgs &a, int mode, int grid, int block, hipStream_t stream);
 hipError_t route_dataflow_il_occupancy(int mode, int block, int *blocks_per_cu);
+// dynamic LDS of the interleaved kernel for `block` lanes: the converged
+// rules keep the first kIlCacheSlots segments of a lane's current unit there
+static constexpr int kIlCacheSlots = 2;
+size_t dataflow_il_lds_bytes(int mode, int block);
 // history slot 0 <- ring slot `src_slot`; slots 1..T <- kPending
 hipError_t launch_hist_init(unsigned long long *hist, const double *ring, int src_slot, long long um,
                             int T, hipStream_t stream);

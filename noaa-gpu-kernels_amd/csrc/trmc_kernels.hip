@@ -20,6 +20,8 @@ namespace trmc {
 struct Acc {
     unsigned long long iters, capped, fallback;
     unsigned long long mlp_calls, mlp_ticks;   // wave-uniform
+    unsigned long long trips, step_trips;      // wave-uniform
+    unsigned long long step_lanes;
 };
 
 template <int MODE>
@@ -106,7 +108,11 @@ __device__ __forceinline__ void flush_acc(const RouteArgs &a, const Acc &acc, lo
                                           long long span)
 {
     unsigned long long it = wave_sum(acc.iters), cp = wave_sum(acc.capped), fb = wave_sum(acc.fallback);
+    const unsigned long long sl = wave_sum(acc.step_lanes);
     if ((threadIdx.x & 63) == 0) {
+        if (acc.trips) atomicAdd(&a.counters[CNT_TRIPS], acc.trips);
+        if (acc.step_trips) atomicAdd(&a.counters[CNT_STEP_TRIPS], acc.step_trips);
+        if (sl) atomicAdd(&a.counters[CNT_STEP_LANES], sl);
         if (it) atomicAdd(&a.counters[CNT_ITERS], it);
         if (cp) atomicAdd(&a.counters[CNT_CAPPED], cp);
         if (fb) atomicAdd(&a.counters[CNT_FALLBACK], fb);
@@ -179,7 +185,7 @@ __global__ void __launch_bounds__(256) k_route_skewed(RouteArgs a)
     const long long nthreads = (long long)gridDim.x * blockDim.x;
     const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int M = a.n_members, T = a.n_steps, L = a.n_levels;
-    Acc acc = {0, 0, 0, 0, 0};
+    Acc acc = {0, 0, 0, 0, 0, 0, 0, 0};
     long long wait = 0;
     unsigned epoch = 0;
     for (int k = a.k_begin; k < a.k_end; ++k) {
@@ -280,7 +286,7 @@ __global__ void __launch_bounds__(256) k_route_dataflow(RouteArgs a)
     const int m = (int)(gtid % M);
     const long long base = gtid / M;
     const size_t UM = (size_t)U * M;
-    Acc acc = {0, 0, 0, 0, 0};
+    Acc acc = {0, 0, 0, 0, 0, 0, 0, 0};
     long long wait_ticks = 0;
     // active window [j_lo, j_hi] of this lane's unit sequence
     long long j_lo = 0, j_hi = -1;
@@ -539,7 +545,7 @@ template <> struct IlSolver<3> { typedef SecantRef type; typedef Seg seg; };   /
 template <> struct IlSolver<4> { typedef SecantRef32 type; typedef Seg32 seg; };
 
 template <int MODE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_route_dataflow_il(RouteArgs a)
+__device__ __forceinline__ void route_dataflow_il(const RouteArgs &a)
 {
     if constexpr (MODE == 1 || MODE == 2) {
         if ((int)blockIdx.x >= a.n_route_blocks) {   // a server block (whole block: no barrier below is split)
@@ -563,14 +569,40 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     const long long base = gtid / M;
     const size_t UM = (size_t)U * M;
     const size_t nd = (size_t)a.n_dev_segs;
+    // FP64 secant rules: the constants and state (flow, depth, velocity) of
+    // the first kIlCacheSlots segments of the lane's current unit live in LDS,
+    // [kIlCacheSlots][blockDim] Seg records, then [kIlCacheSlots][3][blockDim]
+    constexpr bool kCache = (MODE == 1 || MODE == 2);
+    extern __shared__ double il_lds[];
+    const int bd = (int)blockDim.x, tid = (int)threadIdx.x;
+    Seg *const segs = reinterpret_cast<Seg *>(il_lds);
+    double *const sst = il_lds + (size_t)kIlCacheSlots * bd * (sizeof(Seg) / sizeof(double));
+    auto lds = [&](int jj, int f) -> double & { return sst[(size_t)(jj * 3 + f) * bd + tid]; };
     unsigned long long scan_ticket = 0;
     bool scan_posted = false;
     long long scan_t0 = 0;
-    Acc acc = {0, 0, 0, 0, 0};
+    Acc acc = {0, 0, 0, 0, 0, 0, 0, 0};
     long long wait_ticks = 0;
     long long j_lo = 0, j_hi = -1, j = 0;
     int k = 0;
     const long long n_own = (base < U && gtid < S * M) ? (U - 1 - base) / S + 1 : 0;
+    // the window of owned units [j_lo, j_hi] at super-level k keeps the levels
+    // of units j_lo and j_hi + 1 (loaded when the window moves)
+    auto level_of = [&](long long jj) -> int {
+        return jj < n_own ? a.unit_rec[2 * (size_t)(base + jj * S)].x : 0x7FFFFFFF;
+    };
+    int lev_lo = level_of(0), lev_hi1 = lev_lo;
+    // The lane's current unit: its record (and, kCache, its segments in LDS).
+    // A lane walks one unit's items over T consecutive super-levels and its
+    // units lie far apart in level order (on the 1M-segment network 0.35 % of
+    // a lane's items change unit), so the record and segment constants come
+    // from HBM once per unit, not once per item.
+    int cu = -1, cu_lev = 0;
+    // the outflows at t - 1 of item (u, t) are those summed for (u, t - 1):
+    // carried over (the same values, the same order), they are not reloaded
+    int carry_u = -1, carry_t = -1;
+    double carry_sp = 0.0, carry_p0 = 0.0;
+    double ql_pre0 = 0.0, ql_pre1 = 0.0;             // kCache: lateral inflow of the first two segments
     // PH_NEXT: the unit's next segment needs opening (loads) in phase (1)
     enum { PH_DONE = 0, PH_NEED, PH_WAIT, PH_NEXT, PH_SOLVE };
     int phase = (n_own == 0 || T <= 0) ? PH_DONE : PH_NEED;
@@ -591,9 +623,46 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         } else if constexpr (MODE == 4) {   // every value is a float (exact conversions)
             return sr.start((float)a.dt, (float)qup, (float)quc, (float)qdp, (float)ql, c, (float)dp);
         } else {
-            if (sr.start(qup, quc, qdp, ql, c, dp, MODE == 2)) return true;
-            if constexpr (MODE == 2) sr.first_step(c, kc);   // F at the previous depth: the flow to guess for
-            return false;
+            return sr.start(qup, quc, qdp, ql, c, dp, MODE == 2);
+        }
+    };
+
+    // LEARNED: the guesses of the lanes whose solve has just evaluated F at
+    // the previous depth (the anchor) and asks for one
+    auto make_guesses = [&]() {
+        if constexpr (MODE == 2) {
+            const bool want = phase == PH_SOLVE && sr.need_guess;
+            if (!__any(want)) return;
+            if (a.guess_model != 0) {                // power law alone: no network
+                if (want)
+                    sr.set_guess(mlp_guess_from(sr.guess_anchor(), sr.guess_flow_new(), sr.guess_flow_old(), 0.0f,
+                                                0.0f));
+                return;
+            }
+            // the network's factor e^{y(q_new) - y(q_old)} is near 1 when
+            // the flow barely changes: such lanes take the power law alone
+            bool net = false;
+            if (want) {
+                const float r = (float)(fmax(sr.guess_flow_new(), 1e-6) / fmax(sr.guess_flow_old(), 1e-6));
+                net = !(fabsf(__logf(r)) < kNetMinLogRatio);
+                if (!net)
+                    sr.set_guess(mlp_guess_from(sr.guess_anchor(), sr.guess_flow_new(), sr.guess_flow_old(), 0.0f,
+                                                0.0f));
+            }
+            if (__any(net)) {                        // wave-uniform: all 64 lanes run the forward
+                const long long m0 = (long long)wall_clock64();
+                float xn[kMlpIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, xo[kMlpIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                if (net) {
+                    mlp_segment_inputs(sr.guess_flow_new(), c, xn);
+                    mlp_segment_inputs(sr.guess_flow_old(), c, xo);
+                }
+                float yn = 0.0f, yo = 0.0f;
+                const int calls = mlp_wave_pairs(sW, net, xn, xo, yn, yo);
+                if (net)
+                    sr.set_guess(mlp_guess_from(sr.guess_anchor(), sr.guess_flow_new(), sr.guess_flow_old(), yn, yo));
+                acc.mlp_calls += (unsigned long long)calls;
+                acc.mlp_ticks += (unsigned long long)((long long)wall_clock64() - m0);
+            }
         }
     };
 
@@ -604,11 +673,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     };
     // store a finished segment and move to the next one (stores only)
     auto close_segment = [&](const SolveOut &o, double qdp) {
-        const size_t sm = (size_t)s * M + m;
-        a.q[sm] = o.q;
-        if constexpr (MODE != 3) {            // LINEAR carries flow only
-            a.d[sm] = o.depth;
-            a.v[sm] = o.vel;
+        const int jj = s - s0;
+        if (kCache && jj < kIlCacheSlots) {
+            lds(jj, 0) = o.q;
+            lds(jj, 1) = o.depth;
+            lds(jj, 2) = o.vel;
+        } else {
+            const size_t sm = (size_t)s * M + m;
+            a.q[sm] = o.q;
+            if constexpr (MODE != 3) {            // LINEAR carries flow only
+                a.d[sm] = o.depth;
+                a.v[sm] = o.vel;
+            }
         }
         acc.iters += (unsigned long long)o.iters;
         acc.capped += (unsigned long long)(o.flags & 1);
@@ -631,10 +707,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             const size_t sm = (size_t)s * M + m;
             const double qup = (s == s0) ? qup_in : qup_next;
             const double quc = (s == s0) ? quc_in : quc_next;
-            const double qdp = a.q[sm];
             if constexpr (MODE == 3) {
                 // linearized MC, linear_mc.cu:171-180: fixed coefficients, terms
                 // in the reference's order C1 new + C2 old + C3 Q_old + C4/ql ql
+                const double qdp = a.q[sm];
                 SolveOut o;
                 o.q = a.lin[s] * quc + a.lin[nd + s] * qup + a.lin[2 * nd + s] * qdp + a.lin[3 * nd + s] * ql_row[sm];
                 o.depth = 0.0;
@@ -644,13 +720,62 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                 close_segment(o, qdp);
                 continue;
             }
-            if constexpr (MODE == 4) c = load_seg32(a.segc32, nd, s);
-            else c = load_seg(a.segc, nd, s);
-            if (!start_solve(qup, quc, qdp, ql_row[sm], a.d[sm])) {
+            const int jj = s - s0;
+            double qdp, dp, ql;
+            if (kCache && jj < kIlCacheSlots) {
+                if constexpr (kCache) c = segs[(size_t)jj * bd + tid];
+                qdp = lds(jj, 0);
+                dp = lds(jj, 1);
+                ql = (jj == 0) ? ql_pre0 : ql_pre1;
+            } else {
+                if constexpr (MODE == 4) c = load_seg32(a.segc32, nd, s);
+                else c = load_seg(a.segc, nd, s);
+                qdp = a.q[sm];
+                dp = a.d[sm];
+                ql = ql_row[sm];
+            }
+            if (!start_solve(qup, quc, qdp, ql, dp)) {
                 cur_qdp = qdp;
                 phase = PH_SOLVE;
             } else {
                 close_segment(sr.finish(c), qdp);
+            }
+        }
+    };
+    // kCache: the state of the unit in LDS back to HBM
+    auto write_back = [&]() {
+        if constexpr (kCache) {
+            if (cu < 0) return;
+            const int n = min(kIlCacheSlots, s1 - s0);
+            for (int jj = 0; jj < n; ++jj) {
+                const size_t sm = (size_t)(s0 + jj) * M + m;
+                a.q[sm] = lds(jj, 0);
+                a.d[sm] = lds(jj, 1);
+                a.v[sm] = lds(jj, 2);
+            }
+        }
+    };
+    // unit uu becomes the lane's current unit: its record, and (kCache) the
+    // constants and state of its first segments into LDS
+    auto fill = [&](int uu) {
+        const int4 r0 = a.unit_rec[2 * (size_t)uu];
+        ups = a.unit_rec[2 * (size_t)uu + 1];
+        cu = uu;
+        cu_lev = r0.x;
+        s0 = r0.y;
+        s1 = r0.z;
+        internal = (r0.w & 1) != 0;
+        crit = (r0.w & 2) != 0;
+        n_up = r0.w >> 8;
+        if constexpr (kCache) {
+            const int n = min(kIlCacheSlots, s1 - s0);
+            for (int jj = 0; jj < n; ++jj) {
+                const int sg = s0 + jj;
+                segs[(size_t)jj * bd + tid] = load_seg(a.segc, nd, sg);
+                const size_t sm = (size_t)sg * M + m;
+                lds(jj, 0) = a.q[sm];
+                lds(jj, 1) = a.d[sm];
+                lds(jj, 2) = a.v[sm];
             }
         }
     };
@@ -664,13 +789,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         }
         if (phase == PH_NEED) {
             for (;;) {
-                while (j_hi + 1 < n_own && a.unit_rec[2 * (base + (j_hi + 1) * S)].x <= k) ++j_hi;
-                while (j_lo <= j_hi && a.unit_rec[2 * (base + j_lo * S)].x + T - 1 < k) ++j_lo;
+                while (j_hi + 1 < n_own && lev_hi1 <= k) {
+                    ++j_hi;
+                    lev_hi1 = level_of(j_hi + 1);
+                }
+                while (j_lo <= j_hi && lev_lo + T - 1 < k) {
+                    ++j_lo;
+                    lev_lo = level_of(j_lo);
+                }
                 if (j < j_lo) j = j_lo;
                 if (j_lo <= j_hi && j <= j_hi) break;
                 if (j_lo > j_hi) {
                     if (j_hi + 1 >= n_own) { phase = PH_DONE; break; }
-                    k = a.unit_rec[2 * (base + (j_hi + 1) * S)].x;
+                    k = lev_hi1;
                 } else {
                     ++k;
                 }
@@ -678,14 +809,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             }
             if (phase == PH_NEED) {
                 u = (int)(base + j * S);
-                const int4 r0 = a.unit_rec[2 * (size_t)u];
-                ups = a.unit_rec[2 * (size_t)u + 1];
-                t = k - r0.x;
-                s0 = r0.y;
-                s1 = r0.z;
-                internal = (r0.w & 1) != 0;
-                crit = (r0.w & 2) != 0;
-                n_up = r0.w >> 8;
+                if (u != cu) {                       // another unit: its record (and segments)
+                    write_back();
+                    fill(u);
+                }
+                t = k - cu_lev;
+                ql_row = a.qlat + (size_t)(a.t_base + t) * nd * M;
+                if constexpr (kCache) {             // loaded with the outflows below
+                    if (s0 < s1) ql_pre0 = ql_row[(size_t)s0 * M + m];
+                    if (s0 + 1 < s1) ql_pre1 = ql_row[(size_t)(s0 + 1) * M + m];
+                }
                 phase = PH_WAIT;
                 spin_start = (long long)wall_clock64();
             }
@@ -694,16 +827,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             bool ready = true;
             double sp = 0.0, sc = 0.0, first_p = 0.0, first_c = 0.0;
             float spf = 0.0f, scf = 0.0f;          // MODE 4: float sums (wavefront_mc.cu:444-451)
+            const bool carried = (MODE != 4) && cu == carry_u && t == carry_t;
             const int upv[4] = {ups.x, ups.y, ups.z, ups.w};
 #pragma unroll
             for (int q = 0; q < kMaxUp; ++q) {
                 if (q < n_up) {
                     const size_t v = (size_t)upv[q] * M + m;
-                    const unsigned long long bp = __hip_atomic_load(&a.flow_hist[(size_t)t * UM + v],
-                                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     const unsigned long long bc = __hip_atomic_load(&a.flow_hist[(size_t)(t + 1) * UM + v],
                                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    ready = ready && bp != kPending && bc != kPending;
+                    const unsigned long long bp =
+                        carried ? 0ull
+                                : __hip_atomic_load(&a.flow_hist[(size_t)t * UM + v], __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+                    ready = ready && (carried || bp != kPending) && bc != kPending;
                     const double vp = __longlong_as_double((long long)bp);
                     const double vc = __longlong_as_double((long long)bc);
                     // fixed-order sums starting from 0.0 (wavefront_mc.cu:523-529)
@@ -716,17 +852,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                     if (q == 0) { first_p = vp; first_c = vc; }
                 }
             }
+            if (carried) {
+                sp = carry_sp;
+                first_p = carry_p0;
+            }
             if (ready) {
                 wait_ticks += (long long)wall_clock64() - spin_start;
                 if constexpr (MODE == 4) {
                     sp = (double)spf;
                     sc = (double)scf;
                 }
+                carry_u = cu;
+                carry_t = t + 1;
+                carry_sp = sc;
+                carry_p0 = first_c;
                 qup_in = internal ? first_p : sp;
                 quc_in = internal ? first_c : sc;
                 s = s0;
                 last = 0.0;
-                ql_row = a.qlat + (size_t)(a.t_base + t) * nd * M;
                 progressed = true;
                 if (s >= s1) {                       // unit without segments: outflow 0
                     publish();
@@ -735,41 +878,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                 } else {
                     phase = PH_NEXT;
                     open_segments();
-                }
-            }
-        }
-        // ------------------------------- LEARNED: guesses for the opened solves
-        if constexpr (MODE == 2) {
-            const bool want = phase == PH_SOLVE && sr.need_guess;
-            if (a.guess_model != 0) {                // power law alone: no network
-                if (want)
-                    sr.set_guess(mlp_guess_from(sr.guess_anchor(), sr.guess_flow_new(), sr.guess_flow_old(), 0.0f,
-                                                0.0f));
-            } else {
-                // the network's factor e^{y(q_new) - y(q_old)} is near 1 when
-                // the flow barely changes: such lanes take the power law alone
-                bool net = false;
-                if (want) {
-                    const float r = (float)(fmax(sr.guess_flow_new(), 1e-6) / fmax(sr.guess_flow_old(), 1e-6));
-                    net = !(fabsf(__logf(r)) < kNetMinLogRatio);
-                    if (!net)
-                        sr.set_guess(mlp_guess_from(sr.guess_anchor(), sr.guess_flow_new(), sr.guess_flow_old(),
-                                                    0.0f, 0.0f));
-                }
-                if (__any(net)) {                    // wave-uniform: all 64 lanes run the forward
-                    const long long m0 = (long long)wall_clock64();
-                    float xn[kMlpIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, xo[kMlpIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-                    if (net) {
-                        mlp_segment_inputs(sr.guess_flow_new(), c, xn);
-                        mlp_segment_inputs(sr.guess_flow_old(), c, xo);
-                    }
-                    float yn = 0.0f, yo = 0.0f;
-                    const int calls = mlp_wave_pairs(sW, net, xn, xo, yn, yo);
-                    if (net)
-                        sr.set_guess(
-                            mlp_guess_from(sr.guess_anchor(), sr.guess_flow_new(), sr.guess_flow_old(), yn, yo));
-                    acc.mlp_calls += (unsigned long long)calls;
-                    acc.mlp_ticks += (unsigned long long)((long long)wall_clock64() - m0);
                 }
             }
         }
@@ -802,15 +910,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         };
         if (__any(crit && solving())) __builtin_amdgcn_s_setprio(2);
         else __builtin_amdgcn_s_setprio(0);
+        ++acc.trips;
         for (int r = 0; r < a.batch_iters; ++r) {
             if (!__any(solving())) break;
+            ++acc.step_trips;
             if (solving()) {
+                ++acc.step_lanes;
                 progressed = true;
                 bool fin;
                 if constexpr (MODE == 0 || MODE == 3 || MODE == 4) fin = sr.step(c);
                 else fin = sr.template step<true>(c, kc);
                 if (fin) close_segment(sr.finish(c), cur_qdp);
             }
+            make_guesses();
         }
         if (!__any(phase != PH_DONE)) break;
         if (!__any(progressed)) {
@@ -820,11 +932,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             __builtin_amdgcn_s_sleep(1);
         }
     }
+    write_back();
     if constexpr (MODE == 1 || MODE == 2) {      // this routing wave is done: servers may end
         if ((threadIdx.x & 63) == 0)
             __hip_atomic_fetch_add(&a.scan_ctl[64], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     flush_acc(a, acc, wait_ticks, (long long)wall_clock64() - t_start);
+}
+
+// Two waves per SIMD at least for the converged rules (the certified-solve
+// server code in the same kernel needs the registers); the REFERENCE rule's
+// kernel fits three (at most 168 VGPRs), which its short solves use.
+template <int MODE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_route_dataflow_il(RouteArgs a)
+{
+    route_dataflow_il<MODE>(a);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_route_dataflow_il_ref(RouteArgs a)
+{
+    route_dataflow_il<0>(a);
 }
 
 // The element kernels below loop over their range with the grid's stride: a
@@ -869,7 +996,7 @@ __global__ void __launch_bounds__(256) k_route_level(RouteArgs a, int level)
     const long long lo = (long long)a.level_ptr[level] * M;
     const long long hi = (long long)a.level_ptr[level + 1] * M;
     const long long i = lo + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    Acc acc = {0, 0, 0, 0, 0};
+    Acc acc = {0, 0, 0, 0, 0, 0, 0, 0};
     if (i < hi) {
         const int u = (int)(i / M);
         const int m = (int)(i - (long long)u * M);
@@ -1062,13 +1189,25 @@ static hipError_t launch_dataflow_t(const RouteArgs &a, int grid, int block, hip
                                       dim3(block), args, 0, stream);
 }
 
+size_t dataflow_il_lds_bytes(int mode, int block)
+{
+    return (mode == 1 || mode == 2) ? (size_t)kIlCacheSlots * (sizeof(Seg) + 3 * sizeof(double)) * block : 0;
+}
+
+template <int MODE>
+static const void *dataflow_il_kernel()
+{
+    if constexpr (MODE == 0) return reinterpret_cast<const void *>(&k_route_dataflow_il_ref);
+    else return reinterpret_cast<const void *>(&k_route_dataflow_il<MODE>);
+}
+
 template <int MODE>
 static hipError_t launch_dataflow_il_t(const RouteArgs &a, int grid, int block, hipStream_t stream)
 {
     RouteArgs copy = a;
     void *args[] = {&copy};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_route_dataflow_il<MODE>), dim3(grid),
-                                      dim3(block), args, 0, stream);
+    return hipLaunchCooperativeKernel(dataflow_il_kernel<MODE>(), dim3(grid), dim3(block), args,
+                                      dataflow_il_lds_bytes(MODE, block), stream);
 }
 
 hipError_t launch_route_dataflow_il(const RouteArgs &a, int mode, int grid, int block, hipStream_t stream)
@@ -1082,11 +1221,12 @@ hipError_t launch_route_dataflow_il(const RouteArgs &a, int mode, int grid, int 
 
 hipError_t route_dataflow_il_occupancy(int mode, int block, int *blocks_per_cu)
 {
-    return mode == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<0>, block, 0)
+    const size_t lds = dataflow_il_lds_bytes(mode, block);
+    return mode == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il_ref, block, lds)
            : mode == 1
-               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<1>, block, 0)
+               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<1>, block, lds)
            : mode == 2
-               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<2>, block, 0)
+               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<2>, block, lds)
            : mode == 3
                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<3>, block, 0)
                : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_route_dataflow_il<4>, block, 0);
@@ -1298,10 +1438,10 @@ __global__ void __launch_bounds__(256) k_matrix_step(int n_blocks, const int4 *_
 }
 
 hipError_t launch_matrix_step(int n_blocks, const int4 *blocks, const int4 *slots, const int *col, const float2 *val,
-                              const float2 *z, const float *ql_next, float2 *z_out, hipStream_t stream)
+                              const float2 *z, const float *ql_next, float2 *z_out, hipStream_t stream, int max_grid)
 {
     if (n_blocks <= 0) return hipSuccess;
-    const int grid = std::min(n_blocks, 1 << 20);
+    const int grid = std::min(n_blocks, max_grid > 0 ? std::min(max_grid, 1 << 20) : 1 << 20);
     hipLaunchKernelGGL(k_matrix_step, dim3(grid), dim3(256), 0, stream, n_blocks, blocks, slots, col, val, z,
                        ql_next, z_out);
     return hipGetLastError();

@@ -480,6 +480,7 @@ struct SecantRefT {
     T h, h_0, Qj_0, rerror, aerror;
     int iter, maxiter, attempt, total;
     bool have0;
+    bool half0;       // p0 was evaluated by the step before: this step evaluates h
     bool trivial;     // guard hit: outputs are zero
 
     // returns true when no iteration is needed (guard)
@@ -500,19 +501,38 @@ struct SecantRefT {
         aerror = T(0.01);
         rerror = T(1);
         have0 = false;
+        half0 = false;
         return false;
     }
 
-    // one iteration; returns true when the solve has ended
+    // One iteration; returns true when the solve has ended.  kOnePoint: one
+    // point per call, the iteration's h_0 point (when it is not carried over:
+    // the first iteration of an attempt) in a call of its own, so that a wave
+    // whose lanes are in different iterations pays for one point per call;
+    // else both points in one call.  The values are the same either way.
+    template <bool kOnePoint = false>
     TRMC_HD bool step(const SegT<T> &c)
     {
         const T mindepth = T(0.01);
-        if (!have0 || !(p0.h == h_0)) p0 = eval_point(h_0, dt, c);
+        const bool point0 = !half0 && (!have0 || !(p0.h == h_0));
+        PointT<T> pe;
+        if constexpr (kOnePoint) {
+            pe = eval_point(point0 ? h_0 : h, dt, c);
+            if (point0) {
+                p0 = pe;
+                half0 = true;
+                return false;
+            }
+        } else {
+            if (point0) p0 = eval_point(h_0, dt, c);
+            pe = eval_point(h, dt, c);
+        }
+        half0 = false;
         const T x0 = weighting(p0.ck, p0.twu, Qj_0, T(0), c);
         const CoefT<T> m0 = muskingum(p0.km, x0, dt, ql);
         const T q0 = m0.c1 * qup + m0.c2 * quc + m0.c3 * qdp + m0.c4;
         Qj_0 = q0 - p0.qmn;
-        const PointT<T> p1 = eval_point(h, dt, c);
+        const PointT<T> &p1 = pe;
         const T X = weighting(p1.ck, p1.twu, q0, T(0.25), c);
         C = muskingum(p1.km, X, dt, ql);
         const T routed = C.c1 * qup + C.c2 * quc + C.c3 * qdp;
@@ -1094,31 +1114,107 @@ struct SecantConv {
 
     TRMC_HD bool awaiting_scan() const { return stage == 11; }
 
-    // The LEARNED start as the routing kernel opens a segment: F at the
-    // previous depth (stage 0), which raises the guess request; a solve that
-    // starts with F(mindepth) (stage 8) is left to step().  Stage 0 of step()
-    // on its own, so that this call site inlines that stage only.
-    TRMC_HD void first_step(const Seg &c, const ConvCfg &k)
+    // Certified scan (mirrors the oracle's) from a = ha, where F has the sign
+    // spos and the value fa, with step hb: F(b) at b = a + hb (not crossing
+    // bankfull); the other sign there means a sign change in (a, b]: at the
+    // resolution kScanRes a, b is the first point of that sign, else b moves
+    // back toward the secant estimate of the crossing; the same sign and the
+    // sign shown on all of [a, b] (sign_holds) move a to b with a Newton-like
+    // next step; otherwise the step halves, and at the resolution a moves on.
+    // Looking for F > 0 the scan ends at mindepth once Qmn on [a, inf)
+    // exceeds every routed flow; the first F <= 0 after a positive point
+    // closes the bracket the secant then solves.  (Stage 9; host builds and
+    // the serial form only: the routing kernel defers these solves.)
+    TRMC_HD bool scan_step(const Seg &c, const ConvCfg &k)
     {
-        if (stage != 0) return;
-        fa = eval(ha, c, k);
-        qa = q_cur;
-        if (learned) need_guess = true;
-        stage = 1;
+        if (++scan_n > kScanCap) {
+            capped = true;
+            if (spos) hi_neg = ha;
+            stage = spos ? 7 : 4;
+            return false;
+        }
+        const double a = ha, w = hb;
+        if (!spos) {
+            const double rmax = routed_max(k.dt, qup, quc, qdp, ql);
+            if (a >= 1.0e4 || qmn_floor(a, c) > rmax + 1e-12 * (fabs(rmax) + 1.0)) { stage = 4; return false; }
+        } else if (a >= 1.0e4) {
+            hi_neg = 1.0e4;
+            stage = 7;
+            return false;
+        }
+        const bool has_flood = c.twcc > 0.0 && c.bfd > 0.01 && c.bfd < 1.0e4;
+        double b = fmin(a + w, 1.0e4);
+        if (has_flood && a < c.bfd && b > c.bfd) b = c.bfd;
+        const bool flood = has_flood && a >= c.bfd;
+        // fa is NaN where a was reached by the enclosure alone (F(a) not evaluated)
+        const bool have_fa = !(flood && a <= c.bfd) && fa == fa;
+        {                                             // the direct enclosure needs no F(b)
+            const Enclosure en = f_enclosure(a, b, flood, k.dt, qup, quc, qdp, ql, c);
+            if (spos ? en.lo > 0.0 : !(en.hi > 0.0)) {
+                ha = b;
+                fa = NAN;
+                hb = 2.0 * (b - a);
+                return false;
+            }
+        }
+        const double fbv = eval(b, c, k);
+        if (spos ? !(fbv > 0.0) : fbv > 0.0) {          // a sign change in (a, b]
+            if (b - a <= kScanRes * a) {
+                if (spos) {                              // the bracket [a, b]
+                    lo_pos = a;
+                    hi_neg = b;
+                    neg_seen = true;
+                    certified = true;
+                    hb = b; fb = fbv; qb = q_cur;
+                    s_prev = s_prev2 = 1.0e30;
+                    stage = have_fa ? 2 : 10;            // the secant needs F(a) as well
+                    return false;
+                }
+                spos = true;
+                ha = b; fa = fbv; qa = q_cur;
+                return false;
+            }
+            const double t = have_fa && fa != fbv ? (b - a) * fa / (fa - fbv) : 0.5 * (b - a);
+            hb = fmax(0.5 * kScanRes * a, fmin(0.5 * (b - a), 0.98 * t));
+            return false;
+        }
+        if (slope_sign_holds(a, b, flood, fa, have_fa, fbv, spos, k.dt, qup, quc, qdp, ql, c)) {
+            hb = fmax(kScanRes * b, have_fa ? scan_next_step(b - a, fa, fbv) : 2.0 * (b - a));
+            ha = b; fa = fbv; qa = q_cur;
+            return false;
+        }
+        if (b - a > kScanRes * a) { hb = 0.5 * (b - a); return false; }
+        ha = b; fa = fbv; qa = q_cur;                    // at the resolution: move on
+        hb = 2.0 * w;
+        return false;
     }
 
-    // returns true when the solve is finished (MLP-guess requests leave it
-    // running).  kDefer: a solve whose F(mindepth) <= 0 does not run the
-    // certified scan here but waits in stage 11 for take_scan_result (the
-    // routing kernel hands it to separate waves, which keeps the scan's
-    // registers out of its hot loop).
+    // One step of the solve: at most one evaluation of F.  Each stage first
+    // names the depth it needs F at (or ends the step without one), then a
+    // single evaluation serves every stage, then the stage reads the value.
+    // On a SIMT machine the lanes of a wave are in different stages; with
+    // one evaluation site a wave pays for one residual per step, whatever
+    // mix of stages its lanes are in.  Returns true when the solve is
+    // finished (a LEARNED solve waiting for its guess keeps running).
+    // kDefer: a solve whose F(mindepth) <= 0 does not run the certified scan
+    // here but waits in stage 11 for take_scan_result (the routing kernel
+    // hands it to separate waves, which keeps the scan's registers out of its
+    // hot loop).
+    // Stages: 0 F at ha, 1 F at hb, 2 secant, 3 bankfull check (F at
+    // bankfull), 12 its second point (F at mindepth), 4 settle at mindepth,
+    // 5 done, 7 F at hi_neg, 8 F(mindepth) first, 9 certified scan, 10 F at
+    // the scan bracket's lower end, 11 waiting for a deferred certified solve.
     template <bool kDefer = false>
     TRMC_HD bool step(const Seg &c, const ConvCfg &k)
     {
         if (stage == 5) return true;
         if (stage == 11) return false;           // waiting for a deferred certified solve
-        // the cap counts evaluations outside the certified scan (which has its own)
-        if (stage != 9 && evals - scan_n >= k.max_eval) {
+        if constexpr (!kDefer) {
+            if (stage == 9) return scan_step(c, k);
+        }
+        // the cap counts evaluations outside the certified scan (which has
+        // its own); stage 12 is the second half of the bankfull check
+        if (stage != 9 && stage != 12 && evals - scan_n >= k.max_eval) {
             capped = true;
             if (!have_root) { root_h = h_cur; root_q = q_cur; have_root = true; }
             stage = 5;
@@ -1132,23 +1228,18 @@ struct SecantConv {
             fell_back = true;
             begin_plain_or_learned(c);
         }
+        // ---------------------------------------- (A) where F is needed
+        double h;
+        double step_len = 0.0;                   // stage 2: the step to the new point
         switch (stage) {
-        case 0:
-            fa = eval(ha, c, k);
-            qa = q_cur;
-            if (learned) need_guess = true;
-            stage = 1;
-            return false;
+        case 0: h = ha; break;
         case 1:
-            if (need_guess) return false;     // waiting for the wave's MLP call
-            fb = eval(hb, c, k);
-            qb = q_cur;
-            stage = 2;
-            if (f_at_noise(fb) && (lo_pos > 0.0 || neg_seen)) accept_root(c);
-            return stage == 5;
+            if (need_guess) return false;        // waiting for the wave's guess
+            h = hb;
+            break;
         case 2: {
             if (lo_pos < 0.0 && hi_neg == 0.01) {   // F(mindepth) <= 0, nothing positive above yet
-                stage = 8;                    // F(mindepth) again, then the certified scan
+                stage = 8;                       // F(mindepth) again, then the certified scan
                 return false;
             }
             const double hm = hmax(c);
@@ -1188,68 +1279,87 @@ struct SecantConv {
             }
             s_prev2 = s_prev;
             s_prev = fabs(hn - hb);
-            const double fn = eval(hn, c, k);
-            const double stepsz = fabs(hn - hb);
-            ha = hb; fa = fb; hb = hn; fb = fn;
-            qa = qb; qb = q_cur;
-            // a converged step from either side of the bracket: points with
-            // F <= 0 only approach from above (hn < hi_neg), so this is a
-            // downcrossing too
-            if ((lo_pos > 0.0 || neg_seen) &&
-                (f_at_noise(fn) || stepsz <= k.tol * hn ||
-                 (lo_pos > 0.0 && neg_seen && (hi_neg - lo_pos) <= k.tol * hi_neg))) {
-                if (!(h_cur == hi_neg || h_cur == lo_pos)) {
-                    stage = 7;                // last point outside the bracket: take its right end
-                    return false;
-                }
-                accept_root(c);
-            }
-            return stage == 5;
+            step_len = fabs(hn - hb);
+            h = hn;
+            break;
         }
-        case 3: {                             // a downcrossing below bankfull?
-            const double rh = root_h, rq = root_q;
-            const double fbf = eval(c.bfd, c, k);
-            const double qbf = q_cur;
-            inbank = true;
-            if (fbf > 0.0) {                  // no upcrossing between: keep the root
-                root_h = rh;
-                root_q = rq;
-                stage = 5;
-                return true;
-            }
-            // solve [mindepth, bankfull] unless F(mindepth) <= 0
+        case 3: h = c.bfd; break;                // a downcrossing below bankfull?
+        case 12:                                 // solve [mindepth, bankfull] from F(mindepth)
             lo_pos = -1.0;
             hi_neg = c.bfd;
             neg_seen = true;
             s_prev = s_prev2 = 1.0e30;
             ha = 0.01;
             hb = c.bfd;
-            fb = fbf;
-            qb = qbf;
-            fa = eval(ha, c, k);
+            h = 0.01;
+            break;
+        case 4: h = 0.01; break;
+        case 7: h = hi_neg; break;
+        case 8: h = 0.01; break;                 // F(mindepth) first: it may be <= 0
+        case 10: h = lo_pos; break;              // F at the scan bracket's lower end
+        default: return true;
+        }
+        // ---------------------------------------- (B) the one evaluation
+        const double f = eval(h, c, k);
+        // ---------------------------------------- (C) what it means
+        switch (stage) {
+        case 0:
+            fa = f;
             qa = q_cur;
-            if (!(fa > 0.0)) {
-                root_h = rh;
-                root_q = rq;
+            if (learned) need_guess = true;
+            stage = 1;
+            return false;
+        case 1:
+            fb = f;
+            qb = q_cur;
+            stage = 2;
+            if (f_at_noise(fb) && (lo_pos > 0.0 || neg_seen)) accept_root(c);
+            return stage == 5;
+        case 2:
+            ha = hb; fa = fb; hb = h; fb = f;
+            qa = qb; qb = q_cur;
+            // a converged step from either side of the bracket: points with
+            // F <= 0 only approach from above (hn < hi_neg), so this is a
+            // downcrossing too
+            if ((lo_pos > 0.0 || neg_seen) &&
+                (f_at_noise(f) || step_len <= k.tol * h ||
+                 (lo_pos > 0.0 && neg_seen && (hi_neg - lo_pos) <= k.tol * hi_neg))) {
+                if (!(h_cur == hi_neg || h_cur == lo_pos)) {
+                    stage = 7;                   // last point outside the bracket: take its right end
+                    return false;
+                }
+                accept_root(c);
+            }
+            return stage == 5;
+        case 3:
+            inbank = true;
+            if (f > 0.0) {                       // no upcrossing between: keep the root
+                stage = 5;
+                return true;
+            }
+            fb = f;                              // F(bankfull) <= 0: the bracket's right end
+            qb = q_cur;
+            stage = 12;
+            return false;
+        case 12:
+            fa = f;
+            qa = q_cur;
+            if (!(fa > 0.0)) {                   // F(mindepth) <= 0: keep the root
                 stage = 5;
                 return true;
             }
             stage = 2;
             return false;
-        }
         case 4:
-            eval(0.01, c, k);
             root_h = 0.01;
             root_q = q_cur;
             have_root = true;
             stage = 5;
             return true;
         case 7:
-            eval(hi_neg, c, k);
             accept_root(c);
             return stage == 5;
-        case 8: {                             // F(mindepth) first: it may be <= 0
-            const double f = eval(0.01, c, k);
+        case 8:
             if (f > 0.0) {
                 stage = 0;
             } else if constexpr (kDefer) {
@@ -1259,88 +1369,10 @@ struct SecantConv {
                 fa = f;
             }
             return false;
-        }
-        case 10: {                            // F at the scan bracket's lower end, then the secant
-            const double fl = eval(lo_pos, c, k);
-            if (!(fl > 0.0)) { accept_root(c); return stage == 5; }
-            ha = lo_pos; fa = fl; qa = q_cur;
+        case 10:
+            if (!(f > 0.0)) { accept_root(c); return stage == 5; }
+            ha = lo_pos; fa = f; qa = q_cur;
             stage = 2;
-            return false;
-        }
-        case 9:
-            if constexpr (!kDefer) {
-                // Certified scan (mirrors the oracle's) from a = ha, where F has
-                // the sign spos and the value fa, with step hb: F(b) at b = a + hb
-                // (not crossing bankfull); the other sign there means a sign
-                // change in (a, b]: at the resolution kScanRes a, b is the first
-                // point of that sign, else b moves back toward the secant estimate
-                // of the crossing; the same sign and the sign shown on all of
-                // [a, b] (sign_holds) move a to b with a Newton-like next step;
-                // otherwise the step halves, and at the resolution a moves on.
-                // Looking for F > 0 the scan ends at mindepth once Qmn on [a, inf)
-                // exceeds every routed flow; the first F <= 0 after a positive
-                // point closes the bracket the secant then solves.
-                if (++scan_n > kScanCap) {
-                    capped = true;
-                    if (spos) hi_neg = ha;
-                    stage = spos ? 7 : 4;
-                    return false;
-                }
-                const double a = ha, w = hb;
-                if (!spos) {
-                    const double rmax = routed_max(k.dt, qup, quc, qdp, ql);
-                    if (a >= 1.0e4 || qmn_floor(a, c) > rmax + 1e-12 * (fabs(rmax) + 1.0)) { stage = 4; return false; }
-                } else if (a >= 1.0e4) {
-                    hi_neg = 1.0e4;
-                    stage = 7;
-                    return false;
-                }
-                const bool has_flood = c.twcc > 0.0 && c.bfd > 0.01 && c.bfd < 1.0e4;
-                double b = fmin(a + w, 1.0e4);
-                if (has_flood && a < c.bfd && b > c.bfd) b = c.bfd;
-                const bool flood = has_flood && a >= c.bfd;
-                // fa is NaN where a was reached by the enclosure alone (F(a) not evaluated)
-                const bool have_fa = !(flood && a <= c.bfd) && fa == fa;
-                {                                             // the direct enclosure needs no F(b)
-                    const Enclosure en = f_enclosure(a, b, flood, k.dt, qup, quc, qdp, ql, c);
-                    if (spos ? en.lo > 0.0 : !(en.hi > 0.0)) {
-                        ha = b;
-                        fa = NAN;
-                        hb = 2.0 * (b - a);
-                        return false;
-                    }
-                }
-                const double fbv = eval(b, c, k);
-                if (spos ? !(fbv > 0.0) : fbv > 0.0) {          // a sign change in (a, b]
-                    if (b - a <= kScanRes * a) {
-                        if (spos) {                              // the bracket [a, b]
-                            lo_pos = a;
-                            hi_neg = b;
-                            neg_seen = true;
-                            certified = true;
-                            hb = b; fb = fbv; qb = q_cur;
-                            s_prev = s_prev2 = 1.0e30;
-                            stage = have_fa ? 2 : 10;            // the secant needs F(a) as well
-                            return false;
-                        }
-                        spos = true;
-                        ha = b; fa = fbv; qa = q_cur;
-                        return false;
-                    }
-                    const double t = have_fa && fa != fbv ? (b - a) * fa / (fa - fbv) : 0.5 * (b - a);
-                    hb = fmax(0.5 * kScanRes * a, fmin(0.5 * (b - a), 0.98 * t));
-                    return false;
-                }
-                if (slope_sign_holds(a, b, flood, fa, have_fa, fbv, spos, k.dt, qup, quc, qdp, ql, c)) {
-                    hb = fmax(kScanRes * b, have_fa ? scan_next_step(b - a, fa, fbv) : 2.0 * (b - a));
-                    ha = b; fa = fbv; qa = q_cur;
-                    return false;
-                }
-                if (b - a > kScanRes * a) { hb = 0.5 * (b - a); return false; }
-                ha = b; fa = fbv; qa = q_cur;                    // at the resolution: move on
-                hb = 2.0 * w;
-                return false;
-            }
             return false;
         }
         return true;

@@ -46,6 +46,22 @@ extern "C" int shim_solve_resumable(const double *segc, const double *in, double
     return o.iters;
 }
 
+// The same with one point per call (the form the one-step-per-trip kernel
+// uses): the same values and iteration count.
+extern "C" int shim_solve_resumable_one_point(const double *segc, const double *in, double dt, double *out)
+{
+    Seg c = load_seg(segc, 1, 0);
+    SecantRef s;
+    if (!s.start(dt, in[0], in[1], in[2], in[3], c, in[4]))
+        while (!s.step<true>(c)) {
+        }
+    SolveOut o = s.finish(c);
+    out[0] = o.q;
+    out[1] = o.depth;
+    out[2] = o.vel;
+    return o.iters;
+}
+
 // LEARNED solve with the network's guess replaced by `guess` (the routing
 // kernel supplies it from the MLP when need_guess is raised).
 extern "C" int shim_solve_learned(const double *segc, const double *in, double dt, double tol, int max_eval,

@@ -119,9 +119,12 @@ def test_autotuner_picks_a_candidate_and_results_do_not_change(gpu):
     best, ms = trmc.autotune(w.topo, w.params, w.qdp0, w.dp0, w.reach_last0, n_members=M, n_probe=4,
                              qlat=w.qlat)
     assert best["unit_segments"] in (2, 4, 8) and best["batch_iters"] in (6, 8, 12)
-    assert best["block_size"] in (64, 128) and ms > 0
+    assert (best["schedule"], best["dataflow_variant"]) in ((trmc.SCHED_DATAFLOW, 0), (trmc.SCHED_DATAFLOW, 1),
+                                                            (trmc.SCHED_SKEWED, 0))
+    assert best["block_size"] in (64, 128, 256) and ms > 0
     outs = []
-    for opts in ({}, {k: best[k] for k in ("unit_segments", "batch_iters", "block_size")}):
+    for opts in ({}, {k: best[k] for k in ("unit_segments", "batch_iters", "block_size", "schedule",
+                                           "dataflow_variant")}):
         with trmc.Network(w.topo, n_members=M, **opts) as net:
             net.set_params(w.params)
             net.set_state(w.qdp0, w.dp0, w.reach_last0)
@@ -130,3 +133,13 @@ def test_autotuner_picks_a_candidate_and_results_do_not_change(gpu):
             outs.append(net.state())
     for a, b in zip(*outs):
         assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("schedule", [trmc.SCHED_DATAFLOW, trmc.SCHED_SKEWED])
+def test_autotuner_keeps_a_given_schedule(gpu, schedule):
+    T, M = 4, 2
+    w, so, ro = trmc.make_batch_workload([2000, 700], T, M, seed=22)
+    best, _ = trmc.autotune(w.topo, w.params, w.qdp0, w.dp0, w.reach_last0, n_members=M, n_probe=3,
+                            qlat=w.qlat, schedule=schedule)
+    assert best["schedule"] == schedule

@@ -56,7 +56,7 @@ def _run(w, T, mode, W=None, **opt):
 
 
 SCHEDULES = [(trmc.SCHED_DATAFLOW, 4, 0), (trmc.SCHED_DATAFLOW, 1, 0), (trmc.SCHED_DATAFLOW, 3, 1),
-             (trmc.SCHED_SKEWED, 4, 0), (trmc.SCHED_LEVEL, 2, 0)]
+             (trmc.SCHED_SKEWED, 4, 0), (trmc.SCHED_LEVEL, 2, 0), (trmc.SCHED_DATAFLOW, 2, 0)]
 
 
 @pytest.mark.parametrize("schedule,units,variant", SCHEDULES)
@@ -127,10 +127,11 @@ def test_learned_needs_weights(gpu, case):
             net.set_mlp(np.full(1281, np.nan, np.float32))
 
 
-def test_learned_ensemble_members(gpu, weights):
+@pytest.mark.parametrize("variant", [0, 1])
+def test_learned_ensemble_members(gpu, weights, variant):
     T, M = 6, 3
     w = trmc.make_workload(2, 4000, T, n_reaches=1200, n_members=M, seed=31)
-    with trmc.Network(w.topo, mode=trmc.MODE_LEARNED, n_members=M) as net:
+    with trmc.Network(w.topo, mode=trmc.MODE_LEARNED, n_members=M, dataflow_variant=variant) as net:
         net.set_params(w.params)
         net.set_state(w.qdp0, w.dp0, w.reach_last0)
         net.set_forcing(w.qlat)
@@ -142,3 +143,4 @@ def test_learned_ensemble_members(gpu, weights):
                                  mode=trmc.MODE_CONVERGED, variant="cr")
         close(q[m], want["qdc"], f"member {m} qdc")
         close(rl[m], want["reach_last"], f"member {m} reach_last")
+

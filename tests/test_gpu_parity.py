@@ -105,6 +105,7 @@ def shreve():
 @pytest.mark.parametrize("schedule,units,variant", [
     (trmc.SCHED_DATAFLOW, 1, 0), (trmc.SCHED_DATAFLOW, 3, 0), (trmc.SCHED_DATAFLOW, 64, 0),
     (trmc.SCHED_DATAFLOW, 1, 1), (trmc.SCHED_DATAFLOW, 4, 1),
+    (trmc.SCHED_DATAFLOW, 2, 0), (trmc.SCHED_DATAFLOW, 4, 0),
     (trmc.SCHED_SKEWED, 1, 0), (trmc.SCHED_SKEWED, 2, 0), (trmc.SCHED_SKEWED, 7, 0),
     (trmc.SCHED_SKEWED, 64, 0), (trmc.SCHED_LEVEL, 1, 0), (trmc.SCHED_LEVEL, 64, 0)])
 def test_every_timestep_matches_reference(gpu, shreve, schedule, units, variant):
@@ -132,7 +133,7 @@ def test_results_do_not_depend_on_schedule_bitwise(gpu, shreve):
     outs = []
     for sched, units, block, var in ((trmc.SCHED_DATAFLOW, 1, 256, 0), (trmc.SCHED_SKEWED, 5, 64, 0),
                                      (trmc.SCHED_LEVEL, 3, 128, 0), (trmc.SCHED_DATAFLOW, 2, 64, 1),
-                                     (trmc.SCHED_DATAFLOW, 6, 128, 0)):
+                                     (trmc.SCHED_DATAFLOW, 6, 128, 0), (trmc.SCHED_DATAFLOW, 2, 256, 0)):
         with trmc.Network(w.topo, schedule=sched, unit_segments=units, block_size=block,
                           dataflow_variant=var) as net:
             net.set_params(w.params)
@@ -145,9 +146,10 @@ def test_results_do_not_depend_on_schedule_bitwise(gpu, shreve):
             assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
 
 
-def test_chained_runs_and_reset(gpu, shreve):
+@pytest.mark.parametrize("variant", [0, 1])
+def test_chained_runs_and_reset(gpu, shreve, variant):
     w, T, ref, cr = shreve
-    with trmc.Network(w.topo) as net:
+    with trmc.Network(w.topo, dataflow_variant=variant) as net:
         net.set_params(w.params)
         net.set_state(w.qdp0, w.dp0, w.reach_last0)
         net.set_forcing(w.qlat)
@@ -162,11 +164,12 @@ def test_chained_runs_and_reset(gpu, shreve):
     check(a[0][0], cr["qdc"], "chained qdc")
 
 
-@pytest.mark.parametrize("schedule", [trmc.SCHED_DATAFLOW, trmc.SCHED_SKEWED])
-def test_ensemble_members_route_independently(gpu, schedule):
+@pytest.mark.parametrize("schedule,variant", [(trmc.SCHED_DATAFLOW, 0), (trmc.SCHED_SKEWED, 0),
+                                              (trmc.SCHED_DATAFLOW, 1)])
+def test_ensemble_members_route_independently(gpu, schedule, variant):
     T, M = 24, 3
     w = trmc.make_workload(2, 12000, T, n_reaches=3600, n_members=M, seed=5)
-    with trmc.Network(w.topo, n_members=M, schedule=schedule) as net:
+    with trmc.Network(w.topo, n_members=M, schedule=schedule, dataflow_variant=variant) as net:
         net.set_params(w.params)
         net.set_state(w.qdp0, w.dp0, w.reach_last0)
         net.set_forcing(w.qlat)

@@ -249,6 +249,20 @@ def test_gpu_steps_match_oracle_bitwise(gpu, lanes):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("lanes", [1, 0])
+def test_gpu_capped_grid_loops_over_block_table(gpu, monkeypatch, lanes):
+    """A grid of 8 blocks against a block table of hundreds: every block
+    of the table is served on the kernel's later loop trips, bit for bit."""
+    monkeypatch.setenv("TRMC_MATRIX_MAX_GRID", "8")
+    t, p, qlat, Q0 = _net(20000, seed=8, T=24)
+    c = orc.matrix_coeffs(p)
+    PQ, Pq = trmc.matrix_build(t.reach_up_ptr, t.reach_up_idx, c, 1e-6)
+    got = trmc.matrix_route(PQ, Pq, qlat, Q0, 24, lanes_per_row=lanes)
+    want = orc.matrix_route(PQ, Pq, qlat, Q0, 24, lanes=lanes)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.gpu
 def test_gpu_full_size_network_bitwise(gpu):
     """309,000 reaches x 24 steps at tol 1e-6 (the reference's real-network
     size and its graph variant's default), lanes by row length (the default)."""

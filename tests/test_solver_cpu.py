@@ -26,6 +26,8 @@ def shim():
     L.shim_solve.argtypes = [C.c_int, pD, pD, C.c_double, C.c_double, C.c_int, pD]
     L.shim_solve_resumable.restype = C.c_int
     L.shim_solve_resumable.argtypes = [pD, pD, C.c_double, pD]
+    L.shim_solve_resumable_one_point.restype = C.c_int
+    L.shim_solve_resumable_one_point.argtypes = [pD, pD, C.c_double, pD]
     L.shim_solve_learned.restype = C.c_int
     L.shim_solve_learned.argtypes = [pD, pD, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, pD]
     L.shim_learned_anchor.restype = C.c_int
@@ -103,9 +105,10 @@ def test_resumable_reference_solve_is_identical(shim):
         a, ia, _ = run(shim, 0, p, x)
         seg = segc_row(p)
         xi = np.ascontiguousarray(x, np.float64)
-        b = np.zeros(3)
-        ib = shim.shim_solve_resumable(seg.ctypes.data_as(pD), xi.ctypes.data_as(pD), 300.0, b.ctypes.data_as(pD))
-        assert a.view(np.uint64).tolist() == b.view(np.uint64).tolist() and ia == ib, (p, x)
+        for fn in (shim.shim_solve_resumable, shim.shim_solve_resumable_one_point):
+            b = np.zeros(3)
+            ib = fn(seg.ctypes.data_as(pD), xi.ctypes.data_as(pD), 300.0, b.ctypes.data_as(pD))
+            assert a.view(np.uint64).tolist() == b.view(np.uint64).tolist() and ia == ib, (p, x)
 
 
 def test_learned_start_converges_to_the_same_root(shim):
