@@ -288,36 +288,74 @@ __global__ void __launch_bounds__(256) k_route_dataflow(RouteArgs a)
     const size_t UM = (size_t)U * M;
     Acc acc = {0, 0, 0, 0, 0, 0, 0, 0};
     long long wait_ticks = 0;
-    // active window [j_lo, j_hi] of this lane's unit sequence
-    long long j_lo = 0, j_hi = -1;
-    int k = 0;
     const long long n_own = (base < U && gtid < S * M) ? (U - 1 - base) / S + 1 : 0;
+    auto level_of = [&](long long jj) -> int {
+        return jj < n_own ? a.unit_rec[2 * (size_t)(base + jj * S)].x : 0x7FFFFFFF;
+    };
+    // The lane's window of units [j_lo, j_hi] at super-level k.  Its items at
+    // one super-level are independent of one another (an item waits only on
+    // items of smaller k), so the lane takes them in any order: a queue of
+    // the window's positions, 64 at a time (bit i of done_mask: position
+    // bo + i routed at this k).  An item whose upstream outflows are not
+    // there yet is passed over and polled again after the others, so one
+    // late upstream unit does not hold the lane's other ready units.
+    long long j_lo = 0, j_hi = -1;
+    int lev_lo = level_of(0), lev_hi1 = lev_lo;
+    int k = 0;
     bool done = (n_own == 0 || T <= 0);
-    // current item state
-    long long j = 0;           // unit index in the sequence being worked at super-level k
-    bool have_item = false;
-    int u = 0, t = 0, n_up = 0;
+    bool new_k = true;                 // the window of k is to be formed
+    long long bo = 0;                  // first window position of the queue's 64
+    unsigned long long done_mask = 0ull;
+    int probe = 0;                     // next position to try, relative to bo
+    int u = 0, t = 0, n_up = 0, pos = 0;
     bool internal = false;
     int ups[kMaxUp];
-    long long spin_start = 0;
+    long long spin_start = (long long)wall_clock64();
     while (true) {
-        if (!done && !have_item) {
-            // find the next item: (k, j) in increasing k, then j
+        bool have_item = false;
+        if (!done) {
             for (;;) {
-                while (j_hi + 1 < n_own && a.unit_rec[2 * (base + (j_hi + 1) * S)].x <= k) ++j_hi;
-                while (j_lo <= j_hi && a.unit_rec[2 * (base + j_lo * S)].x + T - 1 < k) ++j_lo;
-                if (j < j_lo) j = j_lo;
-                if (j_lo <= j_hi && j <= j_hi) break;
-                if (j_lo > j_hi) {                      // nothing active: jump to the next unit's level
-                    if (j_hi + 1 >= n_own) { done = true; break; }
-                    k = a.unit_rec[2 * (base + (j_hi + 1) * S)].x;
-                } else {
-                    ++k;                                 // window exhausted at this k
+                if (new_k) {
+                    while (j_hi + 1 < n_own && lev_hi1 <= k) {
+                        ++j_hi;
+                        lev_hi1 = level_of(j_hi + 1);
+                    }
+                    while (j_lo <= j_hi && lev_lo + T - 1 < k) {
+                        ++j_lo;
+                        lev_lo = level_of(j_lo);
+                    }
+                    if (j_lo > j_hi) {                  // nothing active: jump to the next unit's level
+                        if (j_hi + 1 >= n_own) { done = true; break; }
+                        k = lev_hi1;
+                        continue;
+                    }
+                    new_k = false;
+                    bo = 0;
+                    done_mask = 0ull;
+                    probe = 0;
                 }
-                j = j_lo;
+                const long long W = j_hi - j_lo + 1;
+                const int n = (int)min(64LL, W - bo);
+                const unsigned long long full = (n == 64) ? ~0ull : ((1ull << n) - 1ull);
+                if (done_mask == full) {                // these 64 are routed: the next 64, or k + 1
+                    bo += 64;
+                    done_mask = 0ull;
+                    probe = 0;
+                    if (bo >= W) {
+                        ++k;
+                        new_k = true;
+                    }
+                    continue;
+                }
+                // the first position at or after probe (cyclically) not yet routed
+                const unsigned long long open = ~done_mask & full;
+                const unsigned long long from = open & ~((probe >= 64 ? 0ull : (1ull << probe)) - 1ull);
+                pos = (int)__ffsll((long long)(from ? from : open)) - 1;
+                have_item = true;
+                break;
             }
-            if (!done) {
-                u = (int)(base + j * S);
+            if (have_item) {
+                u = (int)(base + (j_lo + bo + pos) * S);
                 const int4 r0 = a.unit_rec[2 * (size_t)u];
                 const int4 r1 = a.unit_rec[2 * (size_t)u + 1];
                 t = k - r0.x;
@@ -327,8 +365,6 @@ __global__ void __launch_bounds__(256) k_route_dataflow(RouteArgs a)
                 ups[1] = r1.y;
                 ups[2] = r1.z;
                 ups[3] = r1.w;
-                have_item = true;
-                spin_start = (long long)wall_clock64();
             }
         }
         bool ready = false;
@@ -350,13 +386,17 @@ __global__ void __launch_bounds__(256) k_route_dataflow(RouteArgs a)
                     up_cur[q] = __longlong_as_double((long long)bc);
                 }
             }
+            probe = pos + 1;                             // after this one, try the next position
         }
         if (ready) {
             wait_ticks += (long long)wall_clock64() - spin_start;
             route_item_hist<MODE>(a, u, m, t, up_prev, up_cur, n_up, internal, acc);
-            have_item = false;
-            ++j;                                         // next unit of this super-level
+            done_mask |= 1ull << pos;
+            spin_start = (long long)wall_clock64();
         }
+        ++acc.trips;
+        if (__any(ready)) ++acc.step_trips;
+        if (ready) ++acc.step_lanes;
         const bool busy = !done;
         if (!__any(busy)) break;
         if (!__any(ready)) {
