@@ -903,6 +903,32 @@ def main():
                              "upload_s": t1 - t0, "upload_bytes": int(w.qlat.nbytes),
                              "host_dtype": str(w.qlat.dtype), "pass_s": t2 - t1}
     log(f"[bench] with forcing upload: {out['pcie_inclusive']}")
+    # the reference copies each step's inflow inside its time loop
+    # (wavefront_mc.cu:739); here the forcing stays in page-locked host memory,
+    # mapped into the device's address space, and the routing kernel reads each
+    # segment-step's inflow over the host link as it routes it
+    # (the REFERENCE rule: the converged rules refuse forcing in host memory)
+    if w.qlat.nbytes <= (8 << 30) and "reference" in recs and not args.no_cpu:
+        mnet = new_net(trmc, w, M, MODES["reference"], mode_opts("reference"), T)
+        mnet.run(T, dt)
+        ref_state = mnet.state()
+        t0 = time.perf_counter()
+        qd = mnet.order_forcing(w.qlat)          # device order, page-locked (set-up, not timed)
+        mnet.set_forcing_mapped(qd)
+        t1 = time.perf_counter()
+        mnet.reset()
+        st_s = mnet.run(T, dt)
+        trmc.synchronize()
+        t2 = time.perf_counter()
+        same = all(np.array_equal(a.view(np.uint64), b.view(np.uint64)) for a, b in zip(ref_state, mnet.state()))
+        out["pcie_mapped_reference"] = {
+            "value": seg_steps / (t2 - t1), "unit": "segment-steps/s", "rule": "REFERENCE",
+            "resident_ms_per_pass": recs["reference"]["ms_per_step"], "pass_s": t2 - t1,
+            "kernel_ms": st_s["kernel_ms"], "order_s": t1 - t0, "host_bytes_read": int(w.qlat.nbytes),
+            "host_memory": "page-locked, mapped", "same_bits_as_resident": bool(same)}
+        log(f"[bench] REFERENCE, forcing read from host memory during the pass: {out['pcie_mapped_reference']}")
+        mnet.close()
+        del qd
     for n_ in nets.values():
         n_.close()
     print(json.dumps(out), flush=True)

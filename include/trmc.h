@@ -466,6 +466,26 @@ int trmc_network_info(const trmc_network *net, int n_timesteps, int64_t *info, i
  * HBM bandwidth and for scaling memory-traffic counters. */
 int trmc_stream_copy(int device, size_t bytes, int reps, double *gbps);
 
+/* [new] Lateral inflow read from host memory while the routing runs (the
+ * reference copies each step's inflow to the device inside its time loop,
+ * wavefront_mc.cu:739, before that step's kernels).  The forcing stays in
+ * page-locked host memory (trmc_host_alloc), mapped into the device's address
+ * space, and the routing kernels read each segment-step's inflow from there
+ * over the host link as they route it: the transfer overlaps the routing,
+ * with no copy and no second stream.  qlat_dev_order holds the steps in the
+ * network's device order, [n_timesteps][n_dev_segments][n_members] doubles,
+ * as trmc_network_order_forcing_f64 writes them from qlat
+ * [n_members][n_timesteps][n_segments] (the layout of
+ * trmc_network_set_forcing_f64).  The network reads the buffer in place: it
+ * must stay allocated and unchanged until another forcing is set or the
+ * network is destroyed.  Results are the same bits as with the forcing in
+ * HBM.  The REFERENCE, LINEAR and REFERENCE_F32 rules only: a CONVERGED or
+ * LEARNED run with forcing in host memory returns TRMC_ERR_ARG. */
+int trmc_network_order_forcing_f64(const trmc_network *net, const double *qlat, int n_timesteps, double *out);
+int trmc_network_set_forcing_mapped(trmc_network *net, const double *qlat_dev_order, int n_timesteps);
+int trmc_host_alloc(size_t bytes, void **ptr);
+void trmc_host_free(void *ptr);
+
 /* [new] Autotuner for the schedule of a network (or a union of networks):
  * times candidate schedules on the first n_probe steps of the forcing --
  * qlat [n_members][n_probe][n_segments], or generated on the device from

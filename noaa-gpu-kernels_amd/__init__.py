@@ -137,6 +137,10 @@ def _declare(L: C.CDLL) -> None:
         "trmc_network_set_forcing_f64": (I, [V, pD, I]),
         "trmc_network_set_forcing_f32": (I, [V, pF, I]),
         "trmc_network_run": (I, [V, I, D, _p(RunStats)]),
+        "trmc_network_set_forcing_mapped": (I, [V, pD, I]),
+        "trmc_network_order_forcing_f64": (I, [V, pD, I, pD]),
+        "trmc_host_alloc": (I, [C.c_size_t, _p(V)]),
+        "trmc_host_free": (None, [V]),
         "trmc_network_reset": (I, [V]),
         "trmc_network_get_state_f64": (I, [V, pD, pD, pD, pD]),
         "trmc_network_get_history_f64": (I, [V, pD]),
@@ -943,6 +947,27 @@ def autotune(topo: Topology, params, qdp0, dp0, reach_last0=None, n_members: int
     return {k: getattr(best, k) for k in keys}, ms.value
 
 
+class _HostArray:
+    """float64 array in page-locked host memory (trmc_host_alloc), freed with
+    the object."""
+
+    def __init__(self, shape):
+        self.shape = tuple(int(x) for x in shape)
+        n = int(np.prod(self.shape))
+        self._p = C.c_void_p()
+        _check(lib().trmc_host_alloc(max(n, 1) * 8, C.byref(self._p)))
+        self._a = np.ctypeslib.as_array(C.cast(self._p, C.POINTER(C.c_double)), (max(n, 1),))[:n].reshape(self.shape)
+
+    def array(self) -> np.ndarray:
+        return self._a
+
+    def __del__(self):
+        if getattr(self, "_p", None) and self._p.value:
+            self._a = None
+            lib().trmc_host_free(self._p)
+            self._p = C.c_void_p()
+
+
 class Network:
     """Device-resident network (trmc_network_*): topology, parameters, forcing
     and state stay in HBM across runs."""
@@ -1004,6 +1029,7 @@ class Network:
         else:
             q = _f64(q)
             _check(lib().trmc_network_set_forcing_f64(self._h, _ptr(q, C.c_double), q.shape[1]))
+        self._mapped = None
 
     def generate_forcing(self, n_timesteps: int, seed: int) -> None:
         """Synthetic lateral inflow generated on the device: the values
@@ -1029,6 +1055,35 @@ class Network:
         _check(lib().trmc_network_run(self._h, int(n_timesteps), float(dt), C.byref(st)))
         self.last_stats = st.as_dict()
         return self.last_stats
+
+    def order_forcing(self, qlat, pinned: bool = True):
+        """The forcing [M, T, N] (float64) in the network's device order,
+        [T, n_dev_segments, M], for run_streamed; in page-locked host memory
+        (pinned) so the step copies run at the link's rate."""
+        q = _f64(qlat)
+        M, N = self.n_members, self.topo.n_segments
+        if q.ndim == 2:
+            q = q[None]
+        if q.shape[0] != M or q.shape[2] != N:
+            raise TrmcError(ERR_ARG, f"qlat must be [{M}, T, {N}]")
+        T = q.shape[1]
+        nd = self.info(T)["n_dev_segments"]
+        out = _HostArray((T, nd, M)) if pinned else np.zeros((T, nd, M), np.float64)
+        dst = out.array() if pinned else out
+        _check(lib().trmc_network_order_forcing_f64(self._h, _ptr(q, C.c_double), T, _ptr(dst, C.c_double)))
+        return out
+
+    def set_forcing_mapped(self, qlat_dev_order) -> None:
+        """trmc_network_set_forcing_mapped: the routing reads the forcing in
+        place from page-locked host memory (qlat_dev_order from order_forcing,
+        kept alive here while it is the network's forcing)."""
+        if not isinstance(qlat_dev_order, _HostArray):
+            raise TrmcError(ERR_ARG, "mapped forcing must be page-locked (Network.order_forcing)")
+        arr = qlat_dev_order.array()
+        if arr.shape[2] != self.n_members:
+            raise TrmcError(ERR_ARG, "forcing in device order has another member count")
+        _check(lib().trmc_network_set_forcing_mapped(self._h, _ptr(arr, C.c_double), arr.shape[0]))
+        self._mapped = qlat_dev_order
 
     def reset(self) -> None:
         """Back to the initial state and the first forcing step (device copies)."""

@@ -190,9 +190,12 @@ struct trmc_network {
     size_t scan_lanes = 0;
     int scan_cap = 0;
     unsigned *d_bar = nullptr;
+    // forcing read in place from page-locked host memory (not owned), or null
+    const double *host_qlat = nullptr;
     int *d_status = nullptr, *d_t = nullptr;
     unsigned long long *d_counters = nullptr;
     int forcing_T = 0, t_cursor = 0, ring_shift = 0, hist_T = 0;
+    size_t qlat_cap = 0;                 // doubles allocated at d_qlat
     long long steps_done = 0;
     bool have_params = false, have_state = false, have_lin = false;
     std::vector<double> init_q, init_d;  // [M][N] host copies (segments outside every reach)
@@ -488,6 +491,8 @@ static int set_forcing_impl(trmc_network *net, const TIn *qlat, int T)
     const int N = P.n_segments, nd = P.n_dev_segments, M = net->M;
     int rc = dev_alloc(&net->d_qlat, (size_t)T * nd * M);
     if (rc) return rc;
+    net->qlat_cap = (size_t)T * nd * M;
+    net->host_qlat = nullptr;
     if ((size_t)T * nd > 0) {
         TIn *raw = nullptr;
         if (hipMalloc((void **)&raw, (size_t)T * N * sizeof(TIn)) != hipSuccess)
@@ -528,6 +533,62 @@ int trmc_network_set_forcing_f32(trmc_network *net, const float *qlat, int n_tim
     return set_forcing_impl(net, qlat, n_timesteps);
 }
 
+int trmc_network_order_forcing_f64(const trmc_network *net, const double *qlat, int n_timesteps, double *out)
+{
+    if (!net || n_timesteps < 0 || (n_timesteps > 0 && (!qlat || !out)))
+        return fail(TRMC_ERR_ARG, "bad forcing arguments");
+    const Plan &P = net->plan;
+    const size_t N = (size_t)P.n_segments, nd = (size_t)P.n_dev_segments, M = (size_t)net->M;
+    const size_t T = (size_t)n_timesteps;
+    for (size_t t = 0; t < T; ++t)
+        for (size_t ds = 0; ds < nd; ++ds) {
+            const size_t sg = (size_t)P.seg_of_dev[ds];
+            double *o = out + (t * nd + ds) * M;
+            for (size_t m = 0; m < M; ++m) o[m] = qlat[(m * T + t) * N + sg];
+        }
+    return TRMC_OK;
+}
+
+int trmc_host_alloc(size_t bytes, void **ptr)
+{
+    if (!ptr) return fail(TRMC_ERR_ARG, "null pointer");
+    *ptr = nullptr;
+    if (bytes == 0) return TRMC_OK;
+    hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return fail(TRMC_ERR_NOMEM, "hipHostMalloc of %zu bytes: %s", bytes, hipGetErrorString(e));
+    }
+    return TRMC_OK;
+}
+
+void trmc_host_free(void *ptr)
+{
+    if (ptr) (void)hipHostFree(ptr);
+}
+
+int trmc_network_set_forcing_mapped(trmc_network *net, const double *qlat_dev_order, int n_timesteps)
+{
+    if (!net || n_timesteps < 0 || (n_timesteps > 0 && !qlat_dev_order))
+        return fail(TRMC_ERR_ARG, "bad forcing arguments");
+    HIPCHK(hipSetDevice(net->opt.device));
+    // the routing kernels read the forcing where it lies: the device's view
+    // of the page-locked host allocation
+    void *dp = nullptr;
+    if (n_timesteps > 0) {
+        hipError_t e = hipHostGetDevicePointer(&dp, const_cast<double *>(qlat_dev_order), 0);
+        if (e != hipSuccess || !dp)
+            return fail(TRMC_ERR_ARG, "forcing is not page-locked host memory (trmc_host_alloc): %s",
+                        hipGetErrorString(e));
+    }
+    dev_free(net->d_qlat);
+    net->qlat_cap = 0;
+    net->host_qlat = static_cast<const double *>(dp);
+    net->forcing_T = n_timesteps;
+    net->t_cursor = 0;
+    return TRMC_OK;
+}
+
 int trmc_network_generate_forcing(trmc_network *net, int T, uint64_t seed)
 {
     if (!net || T < 0) return fail(TRMC_ERR_ARG, "bad forcing arguments");
@@ -535,6 +596,8 @@ int trmc_network_generate_forcing(trmc_network *net, int T, uint64_t seed)
     const Plan &P = net->plan;
     int rc = dev_alloc(&net->d_qlat, (size_t)T * P.n_dev_segments * net->M);
     if (rc) return rc;
+    net->qlat_cap = (size_t)T * P.n_dev_segments * net->M;
+    net->host_qlat = nullptr;
     HIPCHK(launch_gen_forcing(net->d_qlat, net->d_seg_of_dev, P.n_dev_segments, P.n_segments, net->M, T,
                               (unsigned long long)seed, net->stream));
     HIPCHK(hipStreamSynchronize(net->stream));
@@ -608,7 +671,7 @@ static RouteArgs make_args(trmc_network *net, int T, double dt)
     a.unit_reach = net->d_unit_reach;
     a.segc = net->d_segc;
     a.segc32 = net->d_segc32;
-    a.qlat = net->d_qlat;
+    a.qlat = net->host_qlat ? net->host_qlat : net->d_qlat;
     a.q = net->d_q;
     a.d = net->d_d;
     a.v = net->d_v;
@@ -666,6 +729,13 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
     const int mode = o.mode;
     if (mode == TRMC_MODE_LEARNED && o.guess_model == 0 && !net->d_mlp)
         return fail(TRMC_ERR_STATE, "LEARNED mode needs weights (trmc_network_set_mlp)");
+    // Forcing read from host memory: the REFERENCE, LINEAR and REFERENCE_F32
+    // rules read a segment's inflow as they open it.  The converged rules'
+    // interleaved kernel loads it ahead, with its unit cache, and gave wrong
+    // outflows at some confluences with host-memory inflow (an unexplained
+    // failure): refused for them.
+    if (net->host_qlat && (mode == TRMC_MODE_CONVERGED || mode == TRMC_MODE_LEARNED))
+        return fail(TRMC_ERR_ARG, "forcing in host memory is not supported for the CONVERGED and LEARNED rules");
     int sched = o.schedule;
     if (sched == TRMC_SCHED_AUTO)
         sched = P.max_upstream <= kDataflowMaxUp ? TRMC_SCHED_DATAFLOW : TRMC_SCHED_SKEWED;

@@ -247,3 +247,57 @@ def test_full_size_network(gpu):
             assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
     q = states[0][0][0]
     assert np.all(np.isfinite(q)) and q.min() >= 0.0
+
+
+@pytest.mark.parametrize("mode,extra,schedule", [(trmc.MODE_REFERENCE, {}, trmc.SCHED_DATAFLOW),
+                                                 (trmc.MODE_REFERENCE, {"dataflow_variant": 1}, trmc.SCHED_DATAFLOW),
+                                                 (trmc.MODE_REFERENCE, {}, trmc.SCHED_SKEWED)])
+def test_mapped_forcing_equals_resident_forcing(gpu, shreve, mode, extra, schedule):
+    """The forcing read in place from page-locked host memory while the
+    routing runs gives the same bits as the forcing resident in HBM."""
+    w, T, _, _ = shreve
+    _log(f"mapped forcing mode={mode} schedule={schedule}")
+    outs = []
+    for mapped in (False, True):
+        with trmc.Network(w.topo, mode=mode, record_history=1, schedule=schedule, **extra) as net:
+            net.set_params(w.params)
+            net.set_state(w.qdp0, w.dp0, w.reach_last0)
+            if mapped:
+                net.set_forcing_mapped(net.order_forcing(w.qlat))
+            else:
+                net.set_forcing(w.qlat)
+            st = net.run(T)
+            outs.append((net.state(), net.history(T), st["secant_iters"]))
+    (a, ha, ia), (b, hb, ib) = outs
+    for x, y in zip(a, b):
+        assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
+    assert np.array_equal(ha.view(np.uint64), hb.view(np.uint64)) and ia == ib
+
+
+def test_mapped_forcing_members_chained_and_errors(gpu):
+    T, M = 12, 3
+    w = trmc.make_workload(2, 6000, T, n_reaches=1800, n_members=M, seed=9)
+    with trmc.Network(w.topo, n_members=M) as net:
+        net.set_params(w.params)
+        net.set_state(w.qdp0, w.dp0, w.reach_last0)
+        qd = net.order_forcing(w.qlat)
+        assert qd.array().shape == (T, net.info(T)["n_dev_segments"], M)
+        net.set_forcing_mapped(qd)
+        del qd                                   # the network keeps the buffer alive
+        net.run(5)
+        net.run(T - 5)
+        q, d, v, rl = net.state()
+        with pytest.raises(trmc.TrmcError) as e:
+            net.set_forcing_mapped(np.zeros((T, 10, M)))
+        assert e.value.code == trmc.ERR_ARG
+    with trmc.Network(w.topo, n_members=M, mode=trmc.MODE_CONVERGED) as net:
+        net.set_params(w.params)
+        net.set_state(w.qdp0, w.dp0, w.reach_last0)
+        net.set_forcing_mapped(net.order_forcing(w.qlat))
+        with pytest.raises(trmc.TrmcError) as e:      # refused for the converged rules
+            net.run(T)
+        assert e.value.code == trmc.ERR_ARG
+    for m in range(M):
+        cr = orc.wavefront_f64(w.topo, w.params, w.qlat[m], w.qdp0, w.dp0, w.reach_last0, T, variant="cr")
+        check(q[m], cr["qdc"], f"mapped member {m} qdc")
+        check(rl[m], cr["reach_last"], f"mapped member {m} reach_last")
