@@ -24,12 +24,15 @@ struct Acc {
     unsigned long long step_lanes;
 };
 
+// with_vel: the velocity is wanted (only the launch's last step stores one:
+// no step reads the velocity of the one before, so the earlier ones would be
+// overwritten unread)
 template <int MODE>
 __device__ __forceinline__ SolveOut solve_one(const RouteArgs &a, double qup, double quc, double qdp,
-                                              double ql, const Seg &c, double dp)
+                                              double ql, const Seg &c, double dp, bool with_vel)
 {
-    if (MODE == 0) return solve_reference(a.dt, qup, quc, qdp, ql, c, dp);
-    if (MODE == 1) return solve_converged(a.dt, qup, quc, qdp, ql, c, dp, a.tol, a.max_eval);
+    if (MODE == 0) return solve_reference(a.dt, qup, quc, qdp, ql, c, dp, with_vel);
+    if (MODE == 1) return solve_converged(a.dt, qup, quc, qdp, ql, c, dp, a.tol, a.max_eval, with_vel);
     const ConvCfg k = {a.dt, a.tol, a.max_eval, a.learned_budget};
     SecantConv s;                                // LEARNED: scalar MLP guess
     if (!s.start(qup, quc, qdp, ql, c, dp, true))
@@ -38,7 +41,7 @@ __device__ __forceinline__ SolveOut solve_one(const RouteArgs &a, double qup, do
                 s.set_guess(a.guess_model != 0
                                 ? mlp_guess_from(s.guess_anchor(), s.guess_flow_new(), s.guess_flow_old(), 0.0f, 0.0f)
                                 : mlp_guess_scalar(a.mlp, s.guess_anchor(), s.guess_flow_new(), s.guess_flow_old(), c));
-    return s.finish(c);
+    return s.finish(c, with_vel);
 }
 
 // One unit, one member, one step.  Upstream reach outflows are summed in the
@@ -70,6 +73,7 @@ __device__ __forceinline__ void route_item(const RouteArgs &a, int u, int m, int
     }
     const int s0 = a.unit_seg[u], s1 = a.unit_seg[u + 1];
     const double *ql_row = a.qlat + (size_t)(a.t_base + t) * (size_t)a.n_dev_segs * M;
+    const bool last_step = t == a.n_steps - 1;
     double last = 0.0, qup_next = 0.0, quc_next = 0.0;
     for (int s = s0; s < s1; ++s) {
         const size_t sm = (size_t)s * M + m;
@@ -79,10 +83,10 @@ __device__ __forceinline__ void route_item(const RouteArgs &a, int u, int m, int
         const double dp = a.d[sm];
         const double ql = ql_row[sm];
         const Seg c = load_seg(a.segc, (size_t)a.n_dev_segs, s);
-        const SolveOut o = solve_one<MODE>(a, qup, quc, qdp, ql, c, dp);
+        const SolveOut o = solve_one<MODE>(a, qup, quc, qdp, ql, c, dp, last_step);
         a.q[sm] = o.q;
         a.d[sm] = o.depth;
-        a.v[sm] = o.vel;
+        if (last_step) a.v[sm] = o.vel;
         qup_next = qdp;      // previous-step qdc of this segment feeds the next one's qup
         quc_next = o.q;
         last = o.q;
@@ -239,6 +243,7 @@ __device__ __forceinline__ void route_item_hist(const RouteArgs &a, int u, int m
     }
     const int s0 = a.unit_seg[u], s1 = a.unit_seg[u + 1];
     const double *ql_row = a.qlat + (size_t)(a.t_base + t) * (size_t)a.n_dev_segs * M;
+    const bool last_step = t == a.n_steps - 1;
     double last = 0.0, qup_next = 0.0, quc_next = 0.0;
     for (int s = s0; s < s1; ++s) {
         const size_t sm = (size_t)s * M + m;
@@ -248,10 +253,10 @@ __device__ __forceinline__ void route_item_hist(const RouteArgs &a, int u, int m
         const double dp = a.d[sm];
         const double ql = ql_row[sm];
         const Seg c = load_seg(a.segc, (size_t)a.n_dev_segs, s);
-        const SolveOut o = solve_one<MODE>(a, qup, quc, qdp, ql, c, dp);
+        const SolveOut o = solve_one<MODE>(a, qup, quc, qdp, ql, c, dp, last_step);
         a.q[sm] = o.q;
         a.d[sm] = o.depth;
-        a.v[sm] = o.vel;
+        if (last_step) a.v[sm] = o.vel;
         qup_next = qdp;
         quc_next = o.q;
         last = o.q;
@@ -714,16 +719,17 @@ __device__ __forceinline__ void route_dataflow_il(const RouteArgs &a)
     // store a finished segment and move to the next one (stores only)
     auto close_segment = [&](const SolveOut &o, double qdp) {
         const int jj = s - s0;
+        const bool last_step = t == T - 1;       // the only step whose velocity is kept
         if (kCache && jj < kIlCacheSlots) {
             lds(jj, 0) = o.q;
             lds(jj, 1) = o.depth;
-            lds(jj, 2) = o.vel;
+            if (last_step) lds(jj, 2) = o.vel;
         } else {
             const size_t sm = (size_t)s * M + m;
             a.q[sm] = o.q;
             if constexpr (MODE != 3) {            // LINEAR carries flow only
                 a.d[sm] = o.depth;
-                a.v[sm] = o.vel;
+                if (last_step) a.v[sm] = o.vel;
             }
         }
         acc.iters += (unsigned long long)o.iters;
@@ -778,7 +784,7 @@ __device__ __forceinline__ void route_dataflow_il(const RouteArgs &a)
                 cur_qdp = qdp;
                 phase = PH_SOLVE;
             } else {
-                close_segment(sr.finish(c), qdp);
+                close_segment(sr.finish(c, t == T - 1), qdp);
             }
         }
     };
@@ -936,7 +942,7 @@ __device__ __forceinline__ void route_dataflow_il(const RouteArgs &a)
                               (unsigned long long)((long long)wall_clock64() - scan_t0));
                     scan_posted = false;
                     progressed = true;
-                    close_segment(sr.finish(c), cur_qdp);
+                    close_segment(sr.finish(c, t == T - 1), cur_qdp);
                 } else if ((long long)wall_clock64() - scan_t0 > a.timeout_ticks) {
                     __hip_atomic_store(a.status, (int)KSTATUS_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
@@ -960,7 +966,7 @@ __device__ __forceinline__ void route_dataflow_il(const RouteArgs &a)
                 bool fin;
                 if constexpr (MODE == 0 || MODE == 3 || MODE == 4) fin = sr.step(c);
                 else fin = sr.template step<true>(c, kc);
-                if (fin) close_segment(sr.finish(c), cur_qdp);
+                if (fin) close_segment(sr.finish(c, t == T - 1), cur_qdp);
             }
             make_guesses();
         }

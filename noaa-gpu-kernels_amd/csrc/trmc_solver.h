@@ -405,8 +405,10 @@ TRMC_HD PointT<T> eval_point(T h, T dt, const SegT<T> &c)
 // the secant points h_0 and h; the next iteration's h_0 is max(0, h) == h, so
 // the depth-only part of that point is carried over instead of recomputed
 // (same inputs, same operations: the values are identical).
+// with_vel false leaves o.vel at 0 (a caller that stores no velocity for this
+// step skips its power and division)
 TRMC_HD SolveOut solve_reference(double dt, double qup, double quc, double qdp, double ql,
-                                 const Seg &c, double depthp)
+                                 const Seg &c, double depthp, bool with_vel = true)
 {
     SolveOut o;
     o.iters = 0;
@@ -459,7 +461,7 @@ TRMC_HD SolveOut solve_reference(double dt, double qup, double quc, double qdp, 
         maxiter += 25;
     }
     o.iters = total;
-    o.vel = velocity(h, c);
+    o.vel = with_vel ? velocity(h, c) : 0.0;
     o.depth = h;
     o.q = outflow(C, qup, quc, qdp);
     return o;
@@ -561,13 +563,14 @@ struct SecantRefT {
         return false;
     }
 
-    TRMC_HD SolveOut finish(const SegT<T> &c) const
+    // with_vel false: o.vel is left at 0 (not wanted for this step)
+    TRMC_HD SolveOut finish(const SegT<T> &c, bool with_vel = true) const
     {
         SolveOut o;
         o.flags = 0;
         o.iters = total;
         if (trivial) { o.q = 0.0; o.depth = 0.0; o.vel = 0.0; return o; }
-        o.vel = (double)velocity(h, c);
+        o.vel = with_vel ? (double)velocity(h, c) : 0.0;
         o.depth = (double)h;
         o.q = (double)outflow(C, qup, quc, qdp);
         return o;
@@ -1378,14 +1381,15 @@ struct SecantConv {
         return true;
     }
 
-    TRMC_HD SolveOut finish(const Seg &c) const
+    // with_vel false: o.vel is left at 0 (not wanted for this step)
+    TRMC_HD SolveOut finish(const Seg &c, bool with_vel = true) const
     {
         SolveOut o;
         o.iters = evals + scan_n;
         o.flags = (capped ? 1 : 0) | (fell_back ? 2 : 0);
         if (trivial) { o.q = 0.0; o.depth = 0.0; o.vel = 0.0; return o; }
         o.depth = root_h;
-        o.vel = velocity<true>(root_h, c);
+        o.vel = with_vel ? velocity<true>(root_h, c) : 0.0;
         o.q = root_q;
         return o;
     }
@@ -1598,14 +1602,14 @@ TRMC_HD int certified_solve_lanes(double qup, double quc, double qdp, double ql,
 
 // Whole CONVERGED solve from the reference bracket (no MLP).
 TRMC_HD SolveOut solve_converged(double dt, double qup, double quc, double qdp, double ql, const Seg &c,
-                                 double depthp, double tol, int max_eval)
+                                 double depthp, double tol, int max_eval, bool with_vel = true)
 {
     const ConvCfg k = {dt, tol, max_eval, 0};
     SecantConv s;
     if (!s.start(qup, quc, qdp, ql, c, depthp, false))
         while (!s.step(c, k)) {
         }
-    return s.finish(c);
+    return s.finish(c, with_vel);
 }
 
 // ------------------------------------------------------------- normal depth
