@@ -1232,108 +1232,114 @@ struct SecantConv {
             begin_plain_or_learned(c);
         }
         // ---------------------------------------- (A) where F is needed
+        // The stages a solve spends most steps in (0, 1, 2) come first; the
+        // rest are grouped behind one test, which a wave whose lanes are all
+        // in the common stages passes over at the cost of a branch.
         double h;
         double step_len = 0.0;                   // stage 2: the step to the new point
-        switch (stage) {
-        case 0: h = ha; break;
-        case 1:
-            if (need_guess) return false;        // waiting for the wave's guess
-            h = hb;
-            break;
-        case 2: {
-            if (lo_pos < 0.0 && hi_neg == 0.01) {   // F(mindepth) <= 0, nothing positive above yet
-                stage = 8;                       // F(mindepth) again, then the certified scan
-                return false;
+        if (stage <= 2) {
+            if (stage == 2) {
+                if (lo_pos < 0.0 && hi_neg == 0.01) {   // F(mindepth) <= 0, nothing positive above yet
+                    stage = 8;                   // F(mindepth) again, then the certified scan
+                    return false;
+                }
+                const double hm = hmax(c);
+                const double den = fa - fb;
+                double hn = (den != 0.0) ? hb - fb * (ha - hb) / den : 0.5 * (ha + hb);
+                // Converged without another evaluation: the secant step points
+                // to a downcrossing next to the point just evaluated (toward
+                // lower h when F <= 0 there, higher when F > 0), inside the
+                // bracket, and the secant error estimate |hn - hb| |hn - ha| / h^2
+                // is within tol.  The root is the prediction hn; its outflow is
+                // the secant interpolation of the outflows at ha and hb (an
+                // error of the same second order).
+                // F jumps at bankfull on a floodplain, where the error estimate
+                // means nothing: a bracket across it (and the certified scan's,
+                // which may end there) converges by steps instead.
+                const bool smooth = !certified && !(c.twcc > 0.0 && fmin(ha, hb) <= c.bfd && fmax(ha, hb) > c.bfd);
+                if (smooth && h_cur == hb && hn == hn && (fb > 0.0 ? hn >= hb : hn <= hb) &&
+                    (lo_pos > 0.0 || neg_seen) && !(hn <= lo_pos) && !(neg_seen && hn > hi_neg) &&
+                    fabs(hn - hb) <= 1e-6 * hb && fabs(hn - hb) * fabs(hn - ha) <= k.tol * hb * hb) {
+                    const double q = (ha != hb) ? qb + (qa - qb) * ((hn - hb) / (ha - hb)) : qb;
+                    accept_at(hn, q, c);
+                    return stage == 5;
+                }
+                const double left = fmax(lo_pos, 0.01);
+                const bool bracketed = lo_pos > 0.0 && neg_seen;
+                // Once bracketed, every step must be under half the step before
+                // the last one, else it bisects the bracket (geometrically while
+                // it spans more than a factor 4), after Brent: a secant pinned to
+                // one end of a strongly curved F then still converges.
+                // The same rule without a bracket: a one-sided secant that creeps
+                // (every F <= 0, or every F > 0) jumps to mindepth or expands.
+                const bool bisect = hn == hn && fabs(hn - hb) > 0.5 * s_prev2;
+                if (bisect || !(hn == hn) || hn <= left || hn >= hi_neg) {
+                    if (bracketed) hn = (hi_neg > 4.0 * left) ? sqrt(left * hi_neg) : 0.5 * (left + hi_neg);
+                    else if (lo_pos < 0.0) hn = 0.01;
+                    else hn = fmin(hm, fmax(ha, hb) * 2.0);
+                }
+                s_prev2 = s_prev;
+                s_prev = fabs(hn - hb);
+                step_len = fabs(hn - hb);
+                h = hn;
+            } else {
+                if (stage == 1 && need_guess) return false;   // waiting for the wave's guess
+                h = stage == 0 ? ha : hb;
             }
-            const double hm = hmax(c);
-            const double den = fa - fb;
-            double hn = (den != 0.0) ? hb - fb * (ha - hb) / den : 0.5 * (ha + hb);
-            // Converged without another evaluation: the secant step points
-            // to a downcrossing next to the point just evaluated (toward lower
-            // h when F <= 0 there, higher when F > 0), inside the bracket, and
-            // the secant error estimate |hn - hb| |hn - ha| / h^2 is within
-            // tol.  The root is the prediction hn; its outflow is the secant
-            // interpolation of the outflows at ha and hb (an error of the
-            // same second order).
-            // F jumps at bankfull on a floodplain, where the error estimate
-            // means nothing: a bracket across it (and the certified scan's,
-            // which may end there) converges by steps instead.
-            const bool smooth = !certified && !(c.twcc > 0.0 && fmin(ha, hb) <= c.bfd && fmax(ha, hb) > c.bfd);
-            if (smooth && h_cur == hb && hn == hn && (fb > 0.0 ? hn >= hb : hn <= hb) && (lo_pos > 0.0 || neg_seen) &&
-                !(hn <= lo_pos) && !(neg_seen && hn > hi_neg) && fabs(hn - hb) <= 1e-6 * hb &&
-                fabs(hn - hb) * fabs(hn - ha) <= k.tol * hb * hb) {
-                const double q = (ha != hb) ? qb + (qa - qb) * ((hn - hb) / (ha - hb)) : qb;
-                accept_at(hn, q, c);
-                return stage == 5;
+        } else {
+            switch (stage) {
+            case 3: h = c.bfd; break;            // a downcrossing below bankfull?
+            case 12:                             // solve [mindepth, bankfull] from F(mindepth)
+                lo_pos = -1.0;
+                hi_neg = c.bfd;
+                neg_seen = true;
+                s_prev = s_prev2 = 1.0e30;
+                ha = 0.01;
+                hb = c.bfd;
+                h = 0.01;
+                break;
+            case 4: h = 0.01; break;
+            case 7: h = hi_neg; break;
+            case 8: h = 0.01; break;             // F(mindepth) first: it may be <= 0
+            case 10: h = lo_pos; break;          // F at the scan bracket's lower end
+            default: return true;
             }
-            const double left = fmax(lo_pos, 0.01);
-            const bool bracketed = lo_pos > 0.0 && neg_seen;
-            // Once bracketed, every step must be under half the step before
-            // the last one, else it bisects the bracket (geometrically while
-            // it spans more than a factor 4), after Brent: a secant pinned to
-            // one end of a strongly curved F then still converges.
-            // The same rule without a bracket: a one-sided secant that creeps
-            // (every F <= 0, or every F > 0) jumps to mindepth or expands.
-            const bool bisect = hn == hn && fabs(hn - hb) > 0.5 * s_prev2;
-            if (bisect || !(hn == hn) || hn <= left || hn >= hi_neg) {
-                if (bracketed) hn = (hi_neg > 4.0 * left) ? sqrt(left * hi_neg) : 0.5 * (left + hi_neg);
-                else if (lo_pos < 0.0) hn = 0.01;
-                else hn = fmin(hm, fmax(ha, hb) * 2.0);
-            }
-            s_prev2 = s_prev;
-            s_prev = fabs(hn - hb);
-            step_len = fabs(hn - hb);
-            h = hn;
-            break;
-        }
-        case 3: h = c.bfd; break;                // a downcrossing below bankfull?
-        case 12:                                 // solve [mindepth, bankfull] from F(mindepth)
-            lo_pos = -1.0;
-            hi_neg = c.bfd;
-            neg_seen = true;
-            s_prev = s_prev2 = 1.0e30;
-            ha = 0.01;
-            hb = c.bfd;
-            h = 0.01;
-            break;
-        case 4: h = 0.01; break;
-        case 7: h = hi_neg; break;
-        case 8: h = 0.01; break;                 // F(mindepth) first: it may be <= 0
-        case 10: h = lo_pos; break;              // F at the scan bracket's lower end
-        default: return true;
         }
         // ---------------------------------------- (B) the one evaluation
         const double f = eval(h, c, k);
         // ---------------------------------------- (C) what it means
-        switch (stage) {
-        case 0:
-            fa = f;
-            qa = q_cur;
-            if (learned) need_guess = true;
-            stage = 1;
-            return false;
-        case 1:
-            fb = f;
+        if (stage <= 2) {
+            if (stage == 2) {
+                ha = hb; fa = fb; hb = h; fb = f;
+                qa = qb; qb = q_cur;
+                // a converged step from either side of the bracket: points with
+                // F <= 0 only approach from above (hn < hi_neg), so this is a
+                // downcrossing too
+                if ((lo_pos > 0.0 || neg_seen) &&
+                    (f_at_noise(f) || step_len <= k.tol * h ||
+                     (lo_pos > 0.0 && neg_seen && (hi_neg - lo_pos) <= k.tol * hi_neg))) {
+                    if (!(h_cur == hi_neg || h_cur == lo_pos)) {
+                        stage = 7;               // last point outside the bracket: take its right end
+                        return false;
+                    }
+                    accept_root(c);
+                }
+                return stage == 5;
+            }
+            if (stage == 0) {
+                fa = f;
+                qa = q_cur;
+                if (learned) need_guess = true;
+                stage = 1;
+                return false;
+            }
+            fb = f;                              // stage 1
             qb = q_cur;
             stage = 2;
             if (f_at_noise(fb) && (lo_pos > 0.0 || neg_seen)) accept_root(c);
             return stage == 5;
-        case 2:
-            ha = hb; fa = fb; hb = h; fb = f;
-            qa = qb; qb = q_cur;
-            // a converged step from either side of the bracket: points with
-            // F <= 0 only approach from above (hn < hi_neg), so this is a
-            // downcrossing too
-            if ((lo_pos > 0.0 || neg_seen) &&
-                (f_at_noise(f) || step_len <= k.tol * h ||
-                 (lo_pos > 0.0 && neg_seen && (hi_neg - lo_pos) <= k.tol * hi_neg))) {
-                if (!(h_cur == hi_neg || h_cur == lo_pos)) {
-                    stage = 7;                   // last point outside the bracket: take its right end
-                    return false;
-                }
-                accept_root(c);
-            }
-            return stage == 5;
+        }
+        switch (stage) {
         case 3:
             inbank = true;
             if (f > 0.0) {                       // no upcrossing between: keep the root
