@@ -889,10 +889,10 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
     if (cnt[CNT_TRIPS] && std::getenv("TRMC_DEBUG_TRIPS")) {
         const double us = 1e3 / net->wall_khz;      // microseconds per wall-clock tick
         std::fprintf(stderr,
-                     "[trmc] trips %llu (%.2f us each), solve trips %llu (%.2f us each), lanes per solve trip %.1f, "
-                     "wave time %.1f ms summed\n",
+                     "[trmc] trips %llu (%.2f us each, %.2f us of it scheduling), solve trips %llu (%.2f us each), "
+                     "lanes per solve trip %.1f, wave time %.1f ms summed\n",
                      cnt[CNT_TRIPS], cnt[CNT_BLOCK_TICKS] * us * (o.block_size / 64.0) / (double)cnt[CNT_TRIPS],
-                     cnt[CNT_STEP_TRIPS],
+                     cnt[CNT_SCHED_TICKS] * us / (double)cnt[CNT_TRIPS], cnt[CNT_STEP_TRIPS],
                      cnt[CNT_STEP_TRIPS] ? cnt[CNT_BLOCK_TICKS] * us * (o.block_size / 64.0) / cnt[CNT_STEP_TRIPS]
                                          : 0.0,
                      cnt[CNT_STEP_TRIPS] ? (double)cnt[CNT_STEP_LANES] / cnt[CNT_STEP_TRIPS] : 0.0,

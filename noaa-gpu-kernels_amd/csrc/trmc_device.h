@@ -34,6 +34,7 @@ enum CounterSlot {
     CNT_TRIPS,          // interleaved kernels: loop trips, summed over waves
     CNT_STEP_TRIPS,     // interleaved kernels: trips that ran a solve step, summed over waves
     CNT_STEP_LANES,     // interleaved kernels: lane solve steps, summed over lanes
+    CNT_SCHED_TICKS,    // interleaved kernel: wall-clock ticks in scheduling phases, summed over waves
     CNT_COUNT
 };
 

@@ -22,6 +22,7 @@ struct Acc {
     unsigned long long mlp_calls, mlp_ticks;   // wave-uniform
     unsigned long long trips, step_trips;      // wave-uniform
     unsigned long long step_lanes;
+    unsigned long long sched_ticks;            // wave-uniform
 };
 
 // with_vel: the velocity is wanted (only the launch's last step stores one:
@@ -122,6 +123,7 @@ __device__ __forceinline__ void flush_acc(const RouteArgs &a, const Acc &acc, lo
         if (fb) atomicAdd(&a.counters[CNT_FALLBACK], fb);
         if (acc.mlp_calls) atomicAdd(&a.counters[CNT_MLP_CALLS], acc.mlp_calls);
         if (acc.mlp_ticks) atomicAdd(&a.counters[CNT_MLP_TICKS], acc.mlp_ticks);
+        if (acc.sched_ticks) atomicAdd(&a.counters[CNT_SCHED_TICKS], acc.sched_ticks);
     }
     if (threadIdx.x == 0) {
         atomicAdd(&a.counters[CNT_WAIT_TICKS], (unsigned long long)wait);
@@ -189,7 +191,7 @@ __global__ void __launch_bounds__(256) k_route_skewed(RouteArgs a)
     const long long nthreads = (long long)gridDim.x * blockDim.x;
     const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int M = a.n_members, T = a.n_steps, L = a.n_levels;
-    Acc acc = {0, 0, 0, 0, 0, 0, 0, 0};
+    Acc acc = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     long long wait = 0;
     unsigned epoch = 0;
     for (int k = a.k_begin; k < a.k_end; ++k) {
@@ -291,7 +293,7 @@ __global__ void __launch_bounds__(256) k_route_dataflow(RouteArgs a)
     const int m = (int)(gtid % M);
     const long long base = gtid / M;
     const size_t UM = (size_t)U * M;
-    Acc acc = {0, 0, 0, 0, 0, 0, 0, 0};
+    Acc acc = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     long long wait_ticks = 0;
     const long long n_own = (base < U && gtid < S * M) ? (U - 1 - base) / S + 1 : 0;
     auto level_of = [&](long long jj) -> int {
@@ -626,7 +628,7 @@ __device__ __forceinline__ void route_dataflow_il(const RouteArgs &a)
     unsigned long long scan_ticket = 0;
     bool scan_posted = false;
     long long scan_t0 = 0;
-    Acc acc = {0, 0, 0, 0, 0, 0, 0, 0};
+    Acc acc = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     long long wait_ticks = 0;
     long long j_lo = 0, j_hi = -1, j = 0;
     int k = 0;
@@ -828,6 +830,7 @@ __device__ __forceinline__ void route_dataflow_il(const RouteArgs &a)
 
     while (true) {
         bool progressed = false;
+        const long long sched_t0 = (long long)wall_clock64();
         // ---------------------------------------------- (1) scheduling phase
         if (phase == PH_NEXT) {
             open_segments();
@@ -957,6 +960,7 @@ __device__ __forceinline__ void route_dataflow_il(const RouteArgs &a)
         if (__any(crit && solving())) __builtin_amdgcn_s_setprio(2);
         else __builtin_amdgcn_s_setprio(0);
         ++acc.trips;
+        acc.sched_ticks += (unsigned long long)((long long)wall_clock64() - sched_t0);
         for (int r = 0; r < a.batch_iters; ++r) {
             if (!__any(solving())) break;
             ++acc.step_trips;
@@ -1042,7 +1046,7 @@ __global__ void __launch_bounds__(256) k_route_level(RouteArgs a, int level)
     const long long lo = (long long)a.level_ptr[level] * M;
     const long long hi = (long long)a.level_ptr[level + 1] * M;
     const long long i = lo + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    Acc acc = {0, 0, 0, 0, 0, 0, 0, 0};
+    Acc acc = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (i < hi) {
         const int u = (int)(i / M);
         const int m = (int)(i - (long long)u * M);
