@@ -56,11 +56,17 @@ typedef struct { double dx, bw, tw, twcc, n_ch, ncc, cs, s0; } trmc_chparams_f64
  *             up to 5 widening attempts (wavefront_mc.cu:242-318).  Bit-for-bit
  *             drop-in semantics.
  *  CONVERGED: [new] the fixed point of the reference iteration (X0 = 0.5),
- *             F(h) = Qmc(h) - Qmn(h), solved to a tight tolerance from the
- *             reference's initial bracket (0.67 d, 1.33 d + 0.01).  The depth
- *             is the leftmost downcrossing of F on [1 cm, inf) (1 cm when F is
- *             never positive).
- *  LEARNED:   [new] as CONVERGED, same root, but started from F at the
+ *             F(h) = Qmc(h) - Qmn(h), solved from the reference's initial
+ *             bracket (0.67 d, 1.33 d + 0.01).  The depth is the leftmost
+ *             downcrossing of F on [1 cm, inf) (1 cm when F is never
+ *             positive), ended in its canonical cell: the neighbouring depths
+ *             g < g+ whose significands have their low 28 bits zero with
+ *             F(g) > 0 >= F(g+), and the secant point of that cell (cut at
+ *             bankfull when the floodplain's jump lies inside).  F has the same
+ *             bits on every machine (IEEE operations only), so any start that
+ *             converges returns the same depth and outflow bits, the CPU
+ *             oracle's included.
+ *  LEARNED:   [new] as CONVERGED, same root and bits, but started from F at the
  *             previous depth d and a guess along the on-device MLP's rating
  *             curve from Qmn(d) to the routed flow Qmc(d) (trmc_mlp_*); a
  *             segment not converged within learned_budget evaluations
@@ -138,6 +144,8 @@ typedef struct {
     double mlp_frac;         /* LEARNED: fraction of wave-time spent in them */
     long long certified_solves; /* CONVERGED/LEARNED: solves with F(mindepth) <= 0 done by the
                                    certified scan (on the server waves of the DATAFLOW kernel) */
+    long long noncanonical;  /* CONVERGED/LEARNED: solves whose root estimate was kept because
+                                no canonical cell lies next to it (0 in every measured run) */
 } trmc_run_stats;
 
 /* ---------------------------------------------------------------- basics */

@@ -77,7 +77,7 @@ class RunStats(C.Structure):
         ("not_converged", C.c_longlong), ("launches", C.c_int), ("barriers", C.c_int),
         ("super_levels", C.c_int), ("grid_blocks", C.c_int), ("block_size", C.c_int),
         ("barrier_wait_frac", C.c_double), ("mlp_calls", C.c_longlong), ("mlp_frac", C.c_double),
-        ("certified_solves", C.c_longlong),
+        ("certified_solves", C.c_longlong), ("noncanonical", C.c_longlong),
     ]
 
     def as_dict(self) -> dict:

@@ -908,6 +908,7 @@ int trmc_network_run(trmc_network *net, int T, double dt, trmc_run_stats *stats)
     st.secant_iters = (long long)cnt[CNT_ITERS];
     st.not_converged = (long long)cnt[CNT_CAPPED];
     st.fallbacks = (long long)cnt[CNT_FALLBACK];
+    st.noncanonical = (long long)cnt[CNT_NONCANON];
     st.barrier_wait_frac = cnt[CNT_BLOCK_TICKS] ? (double)cnt[CNT_WAIT_TICKS] / (double)cnt[CNT_BLOCK_TICKS] : 0.0;
     st.mlp_calls = (long long)cnt[CNT_MLP_CALLS];
     st.mlp_frac = cnt[CNT_BLOCK_TICKS]

@@ -35,6 +35,7 @@ enum CounterSlot {
     CNT_STEP_TRIPS,     // interleaved kernels: trips that ran a solve step, summed over waves
     CNT_STEP_LANES,     // interleaved kernels: lane solve steps, summed over lanes
     CNT_SCHED_TICKS,    // interleaved kernel: wall-clock ticks in scheduling phases, summed over waves
+    CNT_NONCANON,       // CONVERGED/LEARNED: solves that kept their root estimate (no canonical cell)
     CNT_COUNT
 };
 

@@ -18,7 +18,7 @@
 namespace trmc {
 
 struct Acc {
-    unsigned long long iters, capped, fallback;
+    unsigned long long iters, capped, fallback, noncanon;
     unsigned long long mlp_calls, mlp_ticks;   // wave-uniform
     unsigned long long trips, step_trips;      // wave-uniform
     unsigned long long step_lanes;
@@ -95,6 +95,7 @@ __device__ __forceinline__ void route_item(const RouteArgs &a, int u, int m, int
         if (max_it && (unsigned)o.iters > *max_it) *max_it = (unsigned)o.iters;
         acc.capped += (unsigned long long)(o.flags & 1);
         acc.fallback += (unsigned long long)((o.flags >> 1) & 1);
+        acc.noncanon += (unsigned long long)((o.flags >> 2) & 1);
     }
     cur[(size_t)u * M + m] = last;
     if (a.hist_out) {
@@ -113,6 +114,7 @@ __device__ __forceinline__ void flush_acc(const RouteArgs &a, const Acc &acc, lo
                                           long long span)
 {
     unsigned long long it = wave_sum(acc.iters), cp = wave_sum(acc.capped), fb = wave_sum(acc.fallback);
+    const unsigned long long nc = wave_sum(acc.noncanon);
     const unsigned long long sl = wave_sum(acc.step_lanes);
     if ((threadIdx.x & 63) == 0) {
         if (acc.trips) atomicAdd(&a.counters[CNT_TRIPS], acc.trips);
@@ -121,6 +123,7 @@ __device__ __forceinline__ void flush_acc(const RouteArgs &a, const Acc &acc, lo
         if (it) atomicAdd(&a.counters[CNT_ITERS], it);
         if (cp) atomicAdd(&a.counters[CNT_CAPPED], cp);
         if (fb) atomicAdd(&a.counters[CNT_FALLBACK], fb);
+        if (nc) atomicAdd(&a.counters[CNT_NONCANON], nc);
         if (acc.mlp_calls) atomicAdd(&a.counters[CNT_MLP_CALLS], acc.mlp_calls);
         if (acc.mlp_ticks) atomicAdd(&a.counters[CNT_MLP_TICKS], acc.mlp_ticks);
         if (acc.sched_ticks) atomicAdd(&a.counters[CNT_SCHED_TICKS], acc.sched_ticks);
@@ -191,7 +194,7 @@ __global__ void __launch_bounds__(256) k_route_skewed(RouteArgs a)
     const long long nthreads = (long long)gridDim.x * blockDim.x;
     const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int M = a.n_members, T = a.n_steps, L = a.n_levels;
-    Acc acc = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    Acc acc = {};
     long long wait = 0;
     unsigned epoch = 0;
     for (int k = a.k_begin; k < a.k_end; ++k) {
@@ -265,6 +268,7 @@ __device__ __forceinline__ void route_item_hist(const RouteArgs &a, int u, int m
         acc.iters += (unsigned long long)o.iters;
         acc.capped += (unsigned long long)(o.flags & 1);
         acc.fallback += (unsigned long long)((o.flags >> 1) & 1);
+        acc.noncanon += (unsigned long long)((o.flags >> 2) & 1);
     }
     const size_t slot = (size_t)(t + 1) * (size_t)a.n_units * M + (size_t)u * M + m;
     __hip_atomic_store(&a.flow_hist[slot], (unsigned long long)__double_as_longlong(last),
@@ -293,7 +297,7 @@ __global__ void __launch_bounds__(256) k_route_dataflow(RouteArgs a)
     const int m = (int)(gtid % M);
     const long long base = gtid / M;
     const size_t UM = (size_t)U * M;
-    Acc acc = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    Acc acc = {};
     long long wait_ticks = 0;
     const long long n_own = (base < U && gtid < S * M) ? (U - 1 - base) / S + 1 : 0;
     auto level_of = [&](long long jj) -> int {
@@ -628,7 +632,7 @@ __device__ __forceinline__ void route_dataflow_il(const RouteArgs &a)
     unsigned long long scan_ticket = 0;
     bool scan_posted = false;
     long long scan_t0 = 0;
-    Acc acc = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    Acc acc = {};
     long long wait_ticks = 0;
     long long j_lo = 0, j_hi = -1, j = 0;
     int k = 0;
@@ -737,6 +741,7 @@ __device__ __forceinline__ void route_dataflow_il(const RouteArgs &a)
         acc.iters += (unsigned long long)o.iters;
         acc.capped += (unsigned long long)(o.flags & 1);
         acc.fallback += (unsigned long long)((o.flags >> 1) & 1);
+        acc.noncanon += (unsigned long long)((o.flags >> 2) & 1);
         qup_next = qdp;
         quc_next = o.q;
         last = o.q;
@@ -1046,7 +1051,7 @@ __global__ void __launch_bounds__(256) k_route_level(RouteArgs a, int level)
     const long long lo = (long long)a.level_ptr[level] * M;
     const long long hi = (long long)a.level_ptr[level + 1] * M;
     const long long i = lo + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    Acc acc = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    Acc acc = {};
     if (i < hi) {
         const int u = (int)(i / M);
         const int m = (int)(i - (long long)u * M);

@@ -205,58 +205,38 @@ TRMC_HD Powf2353 powf23_53(float xf)
 TRMC_HD Pow2353 pow_pair(double x) { return pow23_53(x); }
 TRMC_HD Powf2353 pow_pair(float x) { return powf23_53(x); }
 
-// x^(2/3) and x^(5/3) to within a few ulps (no correctly rounded finish): the
-// CONVERGED/LEARNED residual and velocity, whose root is wanted to 1e-9, not
-// bit for bit.  On the device, for x in [2^-100, 2^100]: s = x^(-1/3) from
-// the FP32 hardware log2/exp2 (relative error < 3e-6), then two Newton steps
-// s <- s + (s/3)(1 - x s^3), each of which squares the relative error
-// (e -> -2 e^2: 3e-6 -> 2e-11 -> 1e-21); x^(2/3) = x s, x^(5/3) = x^(2/3) x.
-// Elsewhere (and in host builds) through cbrt.
+// x^(2/3) and x^(5/3) to within two ulps (no correctly rounded finish): the
+// CONVERGED/LEARNED residual and its enclosures.  Every step is an IEEE
+// double or an integer operation, so a host build (and the CPU oracle's
+// restatement, which the converged rules' canonical root depends on) gives
+// the device's bits.  For x in [2^-1000, 2^1000]:
+//   s = x^(-1/3) from the high word of x, hi(s) = 0x553f1348 - hi(x) / 3
+//   (relative error below 4.1e-2 for every normal x), then four Newton steps
+//   s <- s + (s/3)(1 - x s^3), each taking the relative error e to -2 e^2
+//   (4.1e-2 -> 3.3e-3 -> 2.1e-5 -> 8.8e-10 -> 1.6e-18); x^(2/3) = x s,
+//   x^(5/3) = x^(2/3) x.
+// Other x through the correctly rounded pow23_53 (as reproducible).
 TRMC_HD Pow2353 pow23_53_faithful(double x)
 {
     Pow2353 r;
-    if (!(x > 0.0) || x == HUGE_VAL) {
-        double v = (x == 0.0) ? 0.0 : (x == HUGE_VAL ? HUGE_VAL : NAN);
-        r.p23 = v;
-        r.p53 = v;
-        return r;
-    }
-#if defined(__HIP_DEVICE_COMPILE__)
-    if (x >= 0x1p-100 && x <= 0x1p100) {
-        double s = (double)__builtin_amdgcn_exp2f(-(1.0f / 3.0f) * __builtin_amdgcn_logf((float)x));
-        double e = fma(-x * s, s * s, 1.0);
+    if (!(x >= 0x1p-1000 && x <= 0x1p1000)) return pow23_53(x);
+    const unsigned long long bx = __builtin_bit_cast(unsigned long long, x);
+    const unsigned int hs = 0x553f1348u - (unsigned int)(bx >> 32) / 3u;
+    double s = __builtin_bit_cast(double, (unsigned long long)hs << 32);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double e = fma(-x * s, s * s, 1.0);
         s = fma(s * (1.0 / 3.0), e, s);
-        e = fma(-x * s, s * s, 1.0);
-        s = fma(s * (1.0 / 3.0), e, s);
-        r.p23 = x * s;
-        r.p53 = r.p23 * x;
-        return r;
     }
-#endif
-    const double cr = cbrt(x);
-    r.p23 = cr * cr;
+    r.p23 = x * s;
     r.p53 = r.p23 * x;
     return r;
 }
 
-// 1/x for the CONVERGED/LEARNED residual, within about an ulp.  On the device,
-// for x in [2^-1000, 2^1000]: the hardware reciprocal estimate refined by two
-// Newton steps r <- r + r (1 - x r) (each squares the relative error); other
-// x (and host builds) by division.
-TRMC_HD double recip(double x)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    const double ax = fabs(x);
-    if (ax >= 0x1p-1000 && ax <= 0x1p1000) {
-        double r = __builtin_amdgcn_rcp(x);
-        double e = fma(-x, r, 1.0);
-        r = fma(r, e, r);
-        e = fma(-x, r, 1.0);
-        return fma(r, e, r);
-    }
-#endif
-    return 1.0 / x;
-}
+// 1/x for the CONVERGED/LEARNED residual: IEEE division (correctly rounded
+// on the device and the host alike, so the residual's bits do not depend on
+// where it runs).
+TRMC_HD double recip(double x) { return 1.0 / x; }
 
 // The REFERENCE arithmetic below is written once for both precisions: T =
 // double is the FP64 rule (mc_solve_d, wavefront_mc.cu:200-337), T = float
@@ -585,7 +565,10 @@ typedef SecantRefT<float> SecantRef32;
 // The reference's formulas (wavefront_mc.cu:249-328) with each shared
 // denominator inverted once (recip) and the powers from pow23_53_faithful:
 // F moves by a few ulps of its terms, its root by far less than the 1e-9
-// the CONVERGED/LEARNED outputs are held to.
+// the CONVERGED/LEARNED outputs are held to.  Every operation is IEEE (no
+// hardware estimate), so this F has the same bits on the device, in a host
+// build and in the CPU oracle's restatement: the canonical cell of the root
+// (canon_* below) is decided by its signs.
 TRMC_HD double residual(double h, double dt, double qup, double quc, double qdp,
                                            double ql, const Seg &c, Coef &C, double &qmn)
 {
@@ -963,6 +946,52 @@ constexpr int kScanCap = 400;
 //                 F(mindepth) <= 0; otherwise [mindepth, bankfull] is solved.
 //   fallback:     LEARNED solves that have not stopped after `budget`
 //                 evaluations restart from the CONVERGED start points.
+// ---------------------------------------------- canonical root
+// Two solves of one F that start from different points stop at different
+// depths inside the band where the rounding noise of F hides its sign (about
+// 1e-14 relative), and the routing carries such differences on from step to
+// step.  So the converged rules end every solve whose root is a downcrossing
+// of F in its canonical cell.  The grid is the depths whose binary
+// significand has its low kCanonBits bits zero (spacing 2^-24 of the depth's
+// binade, 6e-8 relative at most); the cell is the pair of neighbouring grid
+// points g < g+ with F(g) > 0 >= F(g+) next to the solver's root estimate
+// (found from the grid point at or below it, moving one point at a time, at
+// most kCanonMoves moves).  F at grid points has fixed bits, so every solver
+// whose estimate lies in the noise band picks the same cell (when the band
+// straddles a grid point the sign there decides for all of them) and returns
+// the same point of it:
+//   t = F(g) / (F(g) - F(g+)),  h* = g + (g+ - g) t,  Q* = Q(g) + (Q(g+) - Q(g)) t,
+// within about (F''/F') (g+ - g)^2 / 8 of the root where F is smooth.
+// Estimates below mindepth's cell or at the search's end (1e4) are kept as
+// they are, and so is one whose cell is not found within the moves.
+constexpr int kCanonBits = 28;
+constexpr int kCanonMoves = 4;
+constexpr double kSnapRel = 1e-4;   // secant steps below this (relative) land on grid points
+
+TRMC_HD double canon_floor(double r)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, r);
+    return __builtin_bit_cast(double, b & ~((1ull << kCanonBits) - 1ull));
+}
+
+// the next grid point above / below a grid point g > 0 (the step crosses
+// binades: the significand's carry or borrow moves the exponent)
+TRMC_HD double canon_next(double g)
+{
+    return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, g) + (1ull << kCanonBits));
+}
+
+TRMC_HD double canon_prev(double g)
+{
+    return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, g) - (1ull << kCanonBits));
+}
+
+// the next double above x > 0
+TRMC_HD double canon_above(double x)
+{
+    return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, x) + 1ull);
+}
+
 // Launch-uniform settings of a SecantConv solve (kept out of the per-lane
 // state so they stay in scalar registers).
 struct ConvCfg {
@@ -975,10 +1004,13 @@ struct SecantConv {
     double ha, fa, hb, fb, lo_pos, hi_neg;
     double qa, qb;           // outflow at ha and hb
     double h_cur, q_cur;     // last evaluated depth and the outflow its coefficients give
+    double f_cur;            // F(h_cur)
     double qmn_cur;          // Manning flow Qmn(h_cur)
     double root_h, root_q;
     double s_prev, s_prev2;  // the last two step sizes taken while bracketed
     int evals, stage, scan_n;
+    int cmoves;              // canonical cell: moves made from the first cell
+    bool noncanon;           // the root estimate was kept (no canonical cell)
     bool learned, need_guess, fell_back, capped, trivial, inbank, have_root, neg_seen;
     bool spos;               // certified scan: F > 0 at its position
     bool certified;          // the bracket came from the certified scan (no bankfull check)
@@ -989,7 +1021,7 @@ struct SecantConv {
     // stage: 0 eval ha, 1 eval hb, 2 secant, 3 bankfull check, 4 settle at
     // mindepth, 5 done, 7 re-evaluate hi_neg, 8 F(mindepth) first,
     // 9 certified scan, 10 F at the scan bracket's lower end, 11 waiting
-    // for a deferred certified solve
+    // for a deferred certified solve, 13-17 the canonical cell
     TRMC_HD bool start(double qup_, double quc_, double qdp_, double ql_, const Seg &c, double depthp,
                        bool learned_)
     {
@@ -1001,6 +1033,7 @@ struct SecantConv {
         capped = false;
         inbank = false;
         have_root = false;
+        noncanon = false;
         trivial = seg_invalid(c) || (ql <= 0.0 && qup <= 0.0 && quc <= 0.0 && qdp <= 0.0);
         if (trivial) return true;
         dp = fmax(depthp, 0.0);
@@ -1060,6 +1093,7 @@ struct SecantConv {
         Coef C;
         double qmn;
         const double f = residual(h, k.dt, qup, quc, qdp, ql, c, C, qmn);
+        f_cur = f;
         q_cur = outflow(C, qup, quc, qdp);
         qmn_cur = qmn;
         if (f > 0.0) {
@@ -1099,7 +1133,62 @@ struct SecantConv {
         root_h = h;
         root_q = q;
         have_root = true;
-        stage = (!inbank && !certified && root_h > c.bfd && c.twcc > 0.0 && c.bfd > 0.01) ? 3 : 5;
+        stage = (!inbank && !certified && root_h > c.bfd && c.twcc > 0.0 && c.bfd > 0.01) ? 3 : canon_begin();
+    }
+
+    // The canonical cell around the root estimate root_h: its ends are kept
+    // in ha, hb (F in fa, fb, outflows in qa, qb; the secant's points are no
+    // longer needed).  Returns the stage to go on with.
+    TRMC_HD int canon_begin()
+    {
+        const double g = canon_floor(root_h);
+        if (!(g >= 0.01 && canon_next(g) <= 1.0e4)) {
+            noncanon = true;
+            return 5;
+        }
+        ha = g;
+        hb = canon_next(g);
+        cmoves = 0;
+        // the last evaluated point is an end of the cell (the secant's points
+        // near the root lie on the grid): F there is known
+        if (h_cur == ha) { fa = f_cur; qa = q_cur; return 14; }
+        if (h_cur == hb) { fb = f_cur; qb = q_cur; return 15; }
+        return 13;
+    }
+
+    // The secant point of the cell [ha, hb] (F continuous on it).
+    TRMC_HD void canon_finish()
+    {
+        const double t = fa / (fa - fb);
+        root_h = ha + (hb - ha) * t;
+        root_q = qa + (qb - qa) * t;
+        stage = 5;
+    }
+
+    // Both ends evaluated: the cell's point, or a move toward the cell.  A
+    // cell holding the floodplain's jump (F continuous on [g, bankfull] and on
+    // (bankfull, g+]) is split there: F(bankfull) <= 0 leaves [g, bankfull];
+    // else F at the next double above bankfull, where F <= 0 makes that double
+    // the downcrossing and F > 0 leaves [that double, g+].
+    TRMC_HD void canon_decide(const Seg &c)
+    {
+        if (fa > 0.0 && !(fb > 0.0)) {
+            if (c.twcc > 0.0 && ha <= c.bfd && c.bfd < hb) stage = (ha == c.bfd) ? 17 : 16;
+            else canon_finish();
+        } else if (++cmoves > kCanonMoves) {
+            noncanon = true;                     // keep the estimate
+            stage = 5;
+        } else if (fa > 0.0) {                   // F > 0 at both ends: the cell lies above
+            ha = hb; fa = fb; qa = qb;
+            hb = canon_next(hb);
+            stage = 14;
+            if (!(hb <= 1.0e4)) { noncanon = true; stage = 5; }
+        } else {                                 // F <= 0 at the lower end: the cell lies below
+            hb = ha; fb = fa; qb = qa;
+            ha = canon_prev(ha);
+            stage = 15;
+            if (!(ha >= 0.01)) { noncanon = true; stage = 5; }
+        }
     }
 
     TRMC_HD void accept_root(const Seg &c) { accept_at(h_cur, q_cur, c); }
@@ -1217,7 +1306,7 @@ struct SecantConv {
         }
         // the cap counts evaluations outside the certified scan (which has
         // its own); stage 12 is the second half of the bankfull check
-        if (stage != 9 && stage != 12 && evals - scan_n >= k.max_eval) {
+        if (stage != 9 && stage != 12 && stage < 13 && evals - scan_n >= k.max_eval) {
             capped = true;
             if (!have_root) { root_h = h_cur; root_q = q_cur; have_root = true; }
             stage = 5;
@@ -1278,6 +1367,14 @@ struct SecantConv {
                     else if (lo_pos < 0.0) hn = 0.01;
                     else hn = fmin(hm, fmax(ha, hb) * 2.0);
                 }
+                // Near the root the new point is the nearer grid point (inside
+                // the bracket): it then serves as an end of the root's
+                // canonical cell, whose F need not be evaluated again.
+                if (fabs(hn - hb) <= kSnapRel * hb) {
+                    const double g = canon_floor(hn), gn = canon_next(g);
+                    const double hs = (hn - g <= gn - hn) ? g : gn;
+                    if (hs > left && hs < hi_neg && hs != hb && hs != ha) hn = hs;
+                }
                 s_prev2 = s_prev;
                 s_prev = fabs(hn - hb);
                 step_len = fabs(hn - hb);
@@ -1302,6 +1399,11 @@ struct SecantConv {
             case 7: h = hi_neg; break;
             case 8: h = 0.01; break;             // F(mindepth) first: it may be <= 0
             case 10: h = lo_pos; break;          // F at the scan bracket's lower end
+            case 13: h = ha; break;              // the canonical cell's lower end, then its upper end
+            case 14: h = hb; break;              // its upper end
+            case 15: h = ha; break;              // a new lower end after a move down
+            case 16: h = c.bfd; break;           // the floodplain's jump inside the cell
+            case 17: h = canon_above(c.bfd); break;
             default: return true;
             }
         }
@@ -1343,8 +1445,8 @@ struct SecantConv {
         case 3:
             inbank = true;
             if (f > 0.0) {                       // no upcrossing between: keep the root
-                stage = 5;
-                return true;
+                stage = canon_begin();
+                return stage == 5;
             }
             fb = f;                              // F(bankfull) <= 0: the bracket's right end
             qb = q_cur;
@@ -1354,8 +1456,8 @@ struct SecantConv {
             fa = f;
             qa = q_cur;
             if (!(fa > 0.0)) {                   // F(mindepth) <= 0: keep the root
-                stage = 5;
-                return true;
+                stage = canon_begin();
+                return stage == 5;
             }
             stage = 2;
             return false;
@@ -1383,6 +1485,35 @@ struct SecantConv {
             ha = lo_pos; fa = f; qa = q_cur;
             stage = 2;
             return false;
+        case 13:
+            fa = f; qa = q_cur;
+            stage = 14;
+            return false;
+        case 14:
+            fb = f; qb = q_cur;
+            canon_decide(c);
+            return stage == 5;
+        case 15:
+            fa = f; qa = q_cur;
+            canon_decide(c);
+            return stage == 5;
+        case 16:                                 // F at bankfull inside the cell
+            if (!(f > 0.0)) {
+                hb = h; fb = f; qb = q_cur;
+                canon_finish();
+            } else {
+                stage = 17;
+            }
+            return stage == 5;
+        case 17:                                 // F just above bankfull
+            if (!(f > 0.0)) {
+                root_h = h; root_q = q_cur;
+                stage = 5;
+            } else {
+                ha = h; fa = f; qa = q_cur;
+                canon_finish();
+            }
+            return stage == 5;
         }
         return true;
     }
@@ -1392,10 +1523,10 @@ struct SecantConv {
     {
         SolveOut o;
         o.iters = evals + scan_n;
-        o.flags = (capped ? 1 : 0) | (fell_back ? 2 : 0);
+        o.flags = (capped ? 1 : 0) | (fell_back ? 2 : 0) | ((noncanon && !trivial) ? 4 : 0);
         if (trivial) { o.q = 0.0; o.depth = 0.0; o.vel = 0.0; return o; }
         o.depth = root_h;
-        o.vel = with_vel ? velocity<true>(root_h, c) : 0.0;
+        o.vel = with_vel ? velocity(root_h, c) : 0.0;
         o.q = root_q;
         return o;
     }
@@ -1560,7 +1691,7 @@ TRMC_HD int scan_wave_finish(const ScanWave &w, double qup, double quc, double q
     s.lo_pos = w.lo; s.hi_neg = w.hi; s.neg_seen = true;
     s.ha = w.lo; s.fa = w.flo; s.qa = w.qlo;
     s.hb = w.hi; s.fb = w.fhi; s.qb = w.qhi;
-    s.h_cur = w.hi; s.q_cur = w.qhi;
+    s.h_cur = w.hi; s.q_cur = w.qhi; s.f_cur = w.fhi;
     s.s_prev = s.s_prev2 = 1.0e30;
     s.stage = 2;
     while (!s.step(c, k)) {

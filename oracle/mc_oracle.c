@@ -11,6 +11,7 @@
 #include "mc_oracle.h"
 
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -1297,6 +1298,198 @@ int orc_residual_slope_enclosure(double dt, double qup, double quc, double qdp, 
     return 1;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Canonical root of the converged rules.  No reference counterpart.
+ *
+ * Two solvers of one F that start from different points stop at different
+ * depths inside the band where the rounding noise of F hides its sign
+ * (about 1e-14 relative); the routing carries such differences on.  Both
+ * converged rules therefore end a solve whose root is a downcrossing in its
+ * canonical cell, decided by the signs of one fixed restatement Fc of F:
+ *   grid   the depths whose binary significand has its low 28 bits zero
+ *          (spacing 2^-24 of the depth's binade);
+ *   cell   neighbouring grid points g < g+ with Fc(g) > 0 >= Fc(g+), found
+ *          from the grid point at or below the root estimate, one point at a
+ *          time (at most CANON_MOVES moves; F > 0 at both ends moves up,
+ *          F <= 0 at the lower end moves down);
+ *   root   t = Fc(g) / (Fc(g) - Fc(g+)), h* = g + (g+ - g) t, outflow
+ *          Q(g) + (Q(g+) - Q(g)) t with Q the outflow rule of Fc's
+ *          coefficients; a cell holding the floodplain's jump at bankfull is
+ *          first cut there (Fc(bankfull) <= 0 keeps [g, bankfull]; else the
+ *          next double above bankfull is the root if Fc <= 0 there, or the
+ *          cell's new lower end).
+ * Fc is the device's form of the residual: each shared denominator inverted
+ * once by division, the powers R^(2/3), R^(5/3) from an integer estimate of
+ * R^(-1/3) and four Newton steps (every operation IEEE, so its bits are the
+ * same on every machine).  The certified scan above (on residual_at, the
+ * reference's form) only supplies the estimate. */
+#define CANON_BITS 28
+#define CANON_MOVES 4
+
+static double canon_floor(double r)
+{
+    uint64_t b;
+    memcpy(&b, &r, 8);
+    b &= ~((UINT64_C(1) << CANON_BITS) - 1);
+    memcpy(&r, &b, 8);
+    return r;
+}
+
+static double canon_step(double g, int up)
+{
+    uint64_t b;
+    memcpy(&b, &g, 8);
+    b = up ? b + (UINT64_C(1) << CANON_BITS) : b - (UINT64_C(1) << CANON_BITS);
+    memcpy(&g, &b, 8);
+    return g;
+}
+
+/* the next double above x > 0 */
+static double canon_step_ulp(double x)
+{
+    uint64_t b;
+    memcpy(&b, &x, 8);
+    b += 1;
+    memcpy(&x, &b, 8);
+    return x;
+}
+
+/* R^(2/3), R^(5/3) within two ulps: hi(s) = 0x553f1348 - hi(x) / 3 gives
+ * x^(-1/3) to 4.1e-2, four Newton steps s += (s/3)(1 - x s^3) to 1.6e-18;
+ * outside [2^-1000, 2^1000] the correctly rounded pow. */
+static void pow2353_fast(double x, double *p23, double *p53)
+{
+    if (!(x >= 0x1p-1000 && x <= 0x1p1000)) {
+        *p23 = orc_pow(x, 2.0 / 3.0);
+        *p53 = orc_pow(x, 5.0 / 3.0);
+        return;
+    }
+    uint64_t bx;
+    memcpy(&bx, &x, 8);
+    uint32_t hs = 0x553f1348u - (uint32_t)(bx >> 32) / 3u;
+    uint64_t bs = (uint64_t)hs << 32;
+    double s;
+    memcpy(&s, &bs, 8);
+    for (int i = 0; i < 4; ++i) {
+        double e = fma(-x * s, s * s, 1.0);
+        s = fma(s * (1.0 / 3.0), e, s);
+    }
+    *p23 = x * s;
+    *p53 = *p23 * x;
+}
+
+/* Fc(h) and the outflow of its coefficients. */
+static double canon_residual(double h, double dt, double qup, double quc, double qdp, double ql,
+                             const chan_t *c, double *qout)
+{
+    geo_t g;
+    geometry(h, c, &g);
+    double r23, r53;
+    pow2353_fast(g.R, &r23, &r53);
+    const double ks = sqrt(c->s0) / c->n_ch, sq = sqrt(1.0 + c->z * c->z), sqs0 = sqrt(c->s0);
+    double ck = 0.0;
+    if (h > 0.0)
+        ck = fmax(0.0, ks * ((5.0 / 3.0) * r23 - (2.0 / 3.0) * r53 * (2.0 * sq * (1.0 / (c->bw + 2.0 * h * c->z)))));
+    const double km = (ck > 0.0) ? fmax(dt, c->dx * (1.0 / ck)) : dt;
+    const double hdt = 0.5 * dt;
+    const double k5 = 0.5 * km;
+    const double q0 = ((k5 + hdt) * qup + (hdt - k5) * quc + (k5 - hdt) * qdp + ql * dt) * (1.0 / (k5 + hdt));
+    const double twu = weight_width(h, &g, c);
+    const double den = twu * c->s0 * ck * c->dx;
+    double X = 0.5;
+    if (ck > 0.0 && den > 0.0) X = fmin(0.5, fmax(0.25, 0.5 * (1.0 - q0 * (1.0 / (2.0 * den)))));
+    double D = km * (1.0 - X) + hdt;
+    if (D == 0.0) D = 1.0;
+    const double iD = 1.0 / D;
+    mcoef_t C;
+    C.c1 = (km * X + hdt) * iD;
+    C.c2 = (hdt - km * X) * iD;
+    C.c3 = (km * (1.0 - X) - hdt) * iD;
+    C.c4 = (ql * dt) * iD;
+    const double routed = C.c1 * qup + C.c2 * quc + C.c3 * qdp;
+    if (C.c4 < 0.0 && fabs(C.c4) > routed) C.c4 = -routed;
+    const double wp = g.WP + g.WPC;
+    const double qmn = (wp > 0.0) ? (g.A + g.AC) * r23 * sqs0 * wp * (1.0 / (g.WP * c->n_ch + g.WPC * c->ncc)) : 0.0;
+    if (qout) *qout = outflow(&C, qup, quc, qdp);
+    return (routed + C.c4) - qmn;
+}
+
+static long long g_noncanon = 0;
+long long orc_noncanon_count(void) { return g_noncanon; }
+void orc_noncanon_reset(void) { g_noncanon = 0; }
+
+/* The canonical root next to the estimate r: 1 with *h, *q set, 0 when r is
+ * kept (below mindepth's cell, at the search's end, or no cell within the
+ * moves). */
+static int canonical_root(const route_ctx *x, double r, double *h, double *q)
+{
+    double a = canon_floor(r), b = canon_step(a, 1);
+    if (!(a >= 0.01 && b <= CERT_HMAX)) return 0;
+    double qa, qb;
+    double fa = canon_residual(a, x->dt, x->qup, x->quc, x->qdp, x->ql, x->c, &qa);
+    double fb = canon_residual(b, x->dt, x->qup, x->quc, x->qdp, x->ql, x->c, &qb);
+    for (int moves = 0;; ++moves) {
+        if (fa > 0.0 && !(fb > 0.0)) {
+            const chan_t *c = x->c;
+            if (c->twcc > 0.0 && a <= c->bfd && c->bfd < b) {
+                /* the floodplain's jump inside the cell: Fc is continuous on
+                 * [a, bankfull] and on (bankfull, b] */
+                double fm = fa, qm = qa;
+                if (a < c->bfd) fm = canon_residual(c->bfd, x->dt, x->qup, x->quc, x->qdp, x->ql, c, &qm);
+                if (!(fm > 0.0)) {
+                    b = c->bfd; fb = fm; qb = qm;
+                } else {
+                    double up = canon_step_ulp(c->bfd), qp;
+                    double fp = canon_residual(up, x->dt, x->qup, x->quc, x->qdp, x->ql, c, &qp);
+                    if (!(fp > 0.0)) { *h = up; *q = qp; return 1; }
+                    a = up; fa = fp; qa = qp;
+                }
+            }
+            double t = fa / (fa - fb);
+            *h = a + (b - a) * t;
+            *q = qa + (qb - qa) * t;
+            return 1;
+        }
+        if (moves == CANON_MOVES) return 0;
+        if (fa > 0.0) {
+            a = b; fa = fb; qa = qb;
+            b = canon_step(b, 1);
+            if (!(b <= CERT_HMAX)) return 0;
+            fb = canon_residual(b, x->dt, x->qup, x->quc, x->qdp, x->ql, x->c, &qb);
+        } else {
+            b = a; fb = fa; qb = qa;
+            a = canon_step(a, 0);
+            if (!(a >= 0.01)) return 0;
+            fa = canon_residual(a, x->dt, x->qup, x->quc, x->qdp, x->ql, x->c, &qa);
+        }
+    }
+}
+
+double orc_canonical_root(double dt, double qup, double quc, double qdp, double ql,
+                          const orc_chparams_f64 *p, double r, double *qout)
+{
+    chan_t c;
+    chan_init(p, &c);
+    route_ctx x = {dt, qup, quc, qdp, ql, &c};
+    double h, q;
+    if (!canonical_root(&x, r, &h, &q)) return NAN;
+    if (qout) *qout = q;
+    return h;
+}
+
+double orc_canon_residual(double dt, double qup, double quc, double qdp, double ql,
+                          const orc_chparams_f64 *p, double h, double *qout)
+{
+    chan_t c;
+    chan_init(p, &c);
+    return canon_residual(h, dt, qup, quc, qdp, ql, &c, qout);
+}
+
+void orc_pow2353_fast(double x, double *p23, double *p53) { pow2353_fast(x, p23, p53); }
+
+/* The converged depth: the certified leftmost downcrossing of F, ended in
+ * its canonical cell; mindepth (F positive nowhere) with the outflow of Fc's
+ * coefficients there. */
 double orc_mc_solve_converged(double dt, double qup, double quc, double qdp, double ql,
                               const orc_chparams_f64 *p, double depthp,
                               double *depthc, double *velc)
@@ -1310,11 +1503,17 @@ double orc_mc_solve_converged(double dt, double qup, double quc, double qdp, dou
         return 0.0;
     }
     route_ctx x = {dt, qup, quc, qdp, ql, &c};
-    double h = certified_leftmost_downcrossing(&x, 0.01);
-    residual_at(h, dt, qup, quc, qdp, ql, &c, &C);
+    double h = certified_leftmost_downcrossing(&x, 0.01), q;
+    if (h == 0.01) {
+        canon_residual(h, dt, qup, quc, qdp, ql, &c, &q);
+    } else if (!canonical_root(&x, h, &h, &q)) {
+        __atomic_fetch_add(&g_noncanon, 1, __ATOMIC_RELAXED);
+        residual_at(h, dt, qup, quc, qdp, ql, &c, &C);
+        q = outflow(&C, qup, quc, qdp);
+    }
     *velc = velocity(h, &c);
     *depthc = h;
-    return outflow(&C, qup, quc, qdp);
+    return q;
 }
 
 double orc_manning_flow(double h, const orc_chparams_f64 *p)

@@ -19,8 +19,9 @@
  *
  * Converged mode (no reference counterpart; the reference stops its secant at a
  * loose 1 % / 1 cm tolerance):
- *   orc_mc_solve_converged  root of the same MC residual solved to machine
- *                           precision by a bracketed bisection/secant hybrid
+ *   orc_mc_solve_converged  leftmost downcrossing of the same MC residual
+ *                           (certified scan), ended in its canonical cell
+ *   orc_canonical_root      that cell's root next to an estimate
  *   orc_wavefront_converged DAG + time loop of wavefront_mc.cu:495-564 with the
  *                           converged solve
  *   orc_normal_depth        depth h with Manning flow Qmn(h) == Q (training
@@ -186,6 +187,19 @@ int orc_residual_slope_enclosure(double dt, double qup, double quc, double qdp, 
 
 /* Enclosures evaluated by certified scans so far (diagnostics). */
 long long orc_cert_steps(void);
+
+/* Canonical root (see mc_oracle.c): the root of the canonical cell next to the
+ * estimate r, NaN when there is none (then orc_mc_solve_converged keeps its
+ * estimate and counts it in orc_noncanon_count). */
+double orc_canonical_root(double dt, double qup, double quc, double qdp, double ql,
+                          const orc_chparams_f64 *p, double r, double *qout);
+/* Fc(h), the residual form whose signs decide the cell, and its outflow. */
+double orc_canon_residual(double dt, double qup, double quc, double qdp, double ql,
+                          const orc_chparams_f64 *p, double h, double *qout);
+/* R^(2/3), R^(5/3) of Fc (within two ulps, the same bits on every machine). */
+void orc_pow2353_fast(double x, double *p23, double *p53);
+long long orc_noncanon_count(void);
+void orc_noncanon_reset(void);
 
 /* orc_wavefront_f32in on n_threads host threads (OpenMP over the reaches of
  * one level); bit for bit the one-thread result. */

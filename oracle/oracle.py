@@ -225,6 +225,52 @@ def residual_slope_enclosure(params8, qup, quc, qdp, ql, a, b, dt=300.0, variant
     return (lo.value, hi.value) if ok else None
 
 
+def _canon_lib(variant):
+    L = lib(variant)
+    if not getattr(L, "_canon_bound", False):
+        pP64 = C.POINTER(P64)
+        L.orc_canonical_root.restype = D
+        L.orc_canonical_root.argtypes = [D] * 5 + [pP64, D, pD]
+        L.orc_canon_residual.restype = D
+        L.orc_canon_residual.argtypes = [D] * 5 + [pP64, D, pD]
+        L.orc_pow2353_fast.restype = None
+        L.orc_pow2353_fast.argtypes = [D, pD, pD]
+        L.orc_noncanon_count.restype = LL
+        L.orc_noncanon_reset.restype = None
+        L._canon_bound = True
+    return L
+
+
+def canonical_root(params8, qup, quc, qdp, ql, r, dt=300.0, variant="libm"):
+    """(h*, Q*) of the canonical cell next to the estimate r, or (nan, nan)."""
+    p = P64(*map(float, params8))
+    q = D(np.nan)
+    h = _canon_lib(variant).orc_canonical_root(dt, qup, quc, qdp, ql, C.byref(p), r, C.byref(q))
+    return h, q.value
+
+
+def canon_residual(params8, qup, quc, qdp, ql, h, dt=300.0, variant="libm"):
+    """(Fc(h), outflow of Fc's coefficients at h)."""
+    p = P64(*map(float, params8))
+    q = D()
+    f = _canon_lib(variant).orc_canon_residual(dt, qup, quc, qdp, ql, C.byref(p), h, C.byref(q))
+    return f, q.value
+
+
+def pow2353_fast(x, variant="libm"):
+    a, b = D(), D()
+    _canon_lib(variant).orc_pow2353_fast(float(x), C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def noncanon_count(variant="libm", reset=False):
+    L = _canon_lib(variant)
+    n = int(L.orc_noncanon_count())
+    if reset:
+        L.orc_noncanon_reset()
+    return n
+
+
 def solve_converged(params8, qup, quc, qdp, ql, depthp, dt=300.0, variant="libm"):
     p = P64(*map(float, params8))
     d, v = D(), D()

@@ -30,6 +30,27 @@ extern "C" void shim_pow23_53(const double *x, double *out, int n)
     }
 }
 
+extern "C" void shim_pow23_53_faithful(const double *x, double *out, int n)
+{
+    for (int i = 0; i < n; ++i) {
+        Pow2353 p = pow23_53_faithful(x[i]);
+        out[2 * i] = p.p23;
+        out[2 * i + 1] = p.p53;
+    }
+}
+
+// The converged rules' residual F(h) and the outflow of its coefficients
+// (in: qup, quc, qdp, ql).
+extern "C" double shim_residual(const double *segc, const double *in, double dt, double h, double *qout)
+{
+    Seg c = load_seg(segc, 1, 0);
+    Coef C;
+    double qmn;
+    const double f = residual(h, dt, in[0], in[1], in[2], in[3], c, C, qmn);
+    *qout = outflow(C, in[0], in[1], in[2]);
+    return f;
+}
+
 // The resumable REFERENCE solve driven to completion, for comparison with
 // solve_reference.
 extern "C" int shim_solve_resumable(const double *segc, const double *in, double dt, double *out)

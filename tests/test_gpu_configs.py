@@ -82,9 +82,12 @@ def _route(w, mode, T, M, W=None, forcing=None, units=2):
 
 
 def _judge(what, got_hist, want_hist, reach_last0, topo):
-    fs, rel = flow_scale_error(got_hist, want_hist, reach_last0, topo)
+    got = np.ascontiguousarray(got_hist, np.float64)
+    want = np.ascontiguousarray(want_hist, np.float64)
+    n_diff = int((got.view(np.uint64) != want.view(np.uint64)).sum())
+    fs, rel = flow_scale_error(got, want, reach_last0, topo)
     n_rel = int((rel > 1e-9).sum())
-    _log(f"{what}: {n_rel} of {rel.size} reach-steps beyond 1e-9 relative "
+    _log(f"{what}: {n_diff} of {rel.size} reach-steps not bitwise, {n_rel} beyond 1e-9 relative "
          f"({n_rel / rel.size * 1e6:.2f} per million), max relative {rel.max():.3e}; "
          f"flow-scale error max {fs.max():.3e}")
     assert n_rel <= 5e-6 * rel.size, f"{what}: {n_rel} reach-steps beyond 1e-9 relative"
@@ -101,6 +104,7 @@ def test_c4_learned_1m_288_every_reach_outflow_against_the_oracle(net1m, weights
     P = _route(w, trmc.MODE_CONVERGED, T, 1)
     st = L["stats"]
     assert st["not_converged"] == 0 and P["stats"]["not_converged"] == 0
+    _log(f"C4: solves without a canonical cell: learned {st['noncanonical']}, plain {P['stats']['noncanonical']}")
     _log(f"C4: learned {st['secant_iters'] / st['seg_steps']:.3f} evaluations per segment-step, plain "
          f"{P['stats']['secant_iters'] / P['stats']['seg_steps']:.3f}; fallbacks {st['fallbacks']}")
     assert st["secant_iters"] < 0.7 * P["stats"]["secant_iters"]
