@@ -111,7 +111,9 @@ typedef struct {
     int    block_size;      /* threads per workgroup; 0 = auto (64) */
     int    n_members;       /* ensemble members sharing params (>= 1) */
     int    record_history;  /* keep every step's last-segment outflow per reach */
-    double tol;             /* CONVERGED/LEARNED relative depth tolerance (default 1e-13) */
+    double tol;             /* CONVERGED/LEARNED: relative tolerance of the root estimate the
+                               canonical cell is found from (default 1e-10; the cell fixes the
+                               result, the tolerance only how soon the search hands over) */
     int    max_iter;        /* CONVERGED/LEARNED iteration cap (default 60) */
     int    learned_budget;  /* LEARNED: evaluations before falling back (default 12) */
     int    grid_blocks;     /* persistent grid size; 0 = fill the device */

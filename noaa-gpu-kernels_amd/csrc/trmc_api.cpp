@@ -249,7 +249,7 @@ void trmc_default_options(trmc_options *opt)
     opt->block_size = 0;
     opt->n_members = 1;
     opt->record_history = 0;
-    opt->tol = 1e-13;
+    opt->tol = 1e-10;
     opt->max_iter = 60;
     opt->learned_budget = 12;
     opt->grid_blocks = 0;
@@ -343,7 +343,7 @@ int trmc_network_create(const int *reach_seg_start, const int *reach_seg_len,
         o.unit_segments = (o.mode == TRMC_MODE_REFERENCE || o.mode == TRMC_MODE_REFERENCE_F32) ? 4
                           : o.mode == TRMC_MODE_LINEAR                                          ? 8
                                                                                                 : 2;
-    if (o.tol <= 0.0) o.tol = 1e-13;
+    if (o.tol <= 0.0) o.tol = 1e-10;
     if (o.max_iter <= 0) o.max_iter = 60;
     if (o.learned_budget <= 0) o.learned_budget = 12;
     if (o.guess_model != 0 && o.guess_model != 1) return fail(TRMC_ERR_ARG, "guess_model must be 0 or 1");

@@ -562,44 +562,76 @@ typedef SecantRefT<float> SecantRef32;
 // ------------------------------------------------------------- CONVERGED
 // F(h) with the X0 = 0.5 coefficients feeding the weighting X(h); also returns
 // the final coefficients at h for the outflow rule and the Manning flow Qmn(h).
-// The reference's formulas (wavefront_mc.cu:249-328) with each shared
-// denominator inverted once (recip) and the powers from pow23_53_faithful:
-// F moves by a few ulps of its terms, its root by far less than the 1e-9
-// the CONVERGED/LEARNED outputs are held to.  Every operation is IEEE (no
-// hardware estimate), so this F has the same bits on the device, in a host
-// build and in the CPU oracle's restatement: the canonical cell of the root
-// (canon_* below) is decided by its signs.
+// The reference's formulas (wavefront_mc.cu:200-213, :249-328) rearranged so
+// that one evaluation divides twice, each on the dependency chain once:
+//   * the hydraulic radius A/P and the Manning factor P/(P_ch n + P_fp ncc)
+//     share the reciprocal of P (P_ch n + P_fp ncc);
+//   * the celerity is carried times the top width, cn = ck twl =
+//     ks ((5/3) R^(2/3) twl - (4/3) R^(5/3) sqrt(1+z^2)), and the storage
+//     constant as K cn = max(dt cn, dx twl); X0 = 0.5 routing and the
+//     weighting X = 0.5 - q0 / (4 twu s0 ck dx) are kept as a ratio xn / xd
+//     (its clamp to [0.25, 0.5] decided by comparing numerator and
+//     denominator), so that C1..C4, ratios with the common factor cn xd, need
+//     the one division 1 / (cn xd (K (1 - X) + dt/2));
+//   * R^(2/3), R^(5/3) from pow23_53_faithful.
+// F moves by a few ulps of its terms against the reference's order, its root
+// by far less than the 1e-9 the CONVERGED/LEARNED outputs are held to.  Every
+// operation is IEEE (no hardware estimate), so this F has the same bits on the
+// device, in a host build and in the CPU oracle's restatement: the canonical
+// cell of the root (canon_* below) is decided by its signs.
 TRMC_HD double residual(double h, double dt, double qup, double quc, double qdp,
-                                           double ql, const Seg &c, Coef &C, double &qmn)
+                        double ql, const Seg &c, Coef &C, double &qmn)
 {
-    const Geo g = geometry(h, c);
-    const Pow2353 w = pow23_53_faithful(g.R);
+    // compound section at h
+    const double twl = c.bw + 2.0 * c.z * h;
+    double over = fmax(h - c.bfd, 0.0);
+    double in = fmin(c.bfd, h);
+    if (over > 0.0 && c.twcc <= 0.0) { over = 0.0; in = h; }
+    const double asum = (c.bw + in * c.z) * in + c.twcc * over;
+    const double wpc = c.bw + 2.0 * in * c.sq;
+    const double wpf = (over > 0.0) ? (c.twcc + 2.0 * over) : 0.0;
+    const double wp = wpc + wpf;
+    const double nsum = wpc * c.n_ch + wpf * c.ncc;
+    double R = 0.0, mq = 0.0;                    // A / P and P / (P_ch n + P_fp ncc)
+    if (wp > 0.0) {
+        const double inv = 1.0 / (wp * nsum);
+        R = asum * nsum * inv;
+        mq = wp * wp * inv;
+    }
+    const Pow2353 w = pow23_53_faithful(R);
     const double r23 = w.p23, r53 = w.p53;
-    // celerity (wavefront_mc.cu:249-254) and storage constant
-    double ck = 0.0;
-    if (h > 0.0)
-        ck = fmax(0.0, c.ks * ((5.0 / 3.0) * r23 - (2.0 / 3.0) * r53 * (2.0 * c.sq * recip(c.bw + 2.0 * h * c.z))));
-    const double km = (ck > 0.0) ? fmax(dt, c.dx * recip(ck)) : dt;
+    qmn = asum * r23 * c.sqs0 * mq;
+    // celerity x top width, storage constant x that
+    double cn = 0.0;
+    if (h > 0.0) cn = fmax(0.0, c.ks * ((5.0 / 3.0) * r23 * twl - (4.0 / 3.0) * r53 * c.sq));
     const double hdt = 0.5 * dt;
-    // X0 = 0.5 routing, then the weighting X from it (lower limit 0.25)
-    const double k5 = 0.5 * km;
-    const double q0 = ((k5 + hdt) * qup + (hdt - k5) * quc + (k5 - hdt) * qdp + ql * dt) * recip(k5 + hdt);
-    const double twu = weight_width(h, g, c);
-    const double den = twu * c.s0 * ck * c.dx;
-    double X = 0.5;
-    if (ck > 0.0 && den > 0.0) X = fmin(0.5, fmax(0.25, 0.5 * (1.0 - q0 * recip(2.0 * den))));
-    double D = km * (1.0 - X) + hdt;
+    const double cs = (cn > 0.0) ? cn : 1.0;     // ck = 0: K = dt, X = 0.5
+    const double P = (cn > 0.0) ? fmax(dt * cn, c.dx * twl) : dt;
+    const double b = hdt * cs;
+    // X = xn / xd and 1 - X = omx / xd: X = 0.5 - u / v from the X0 = 0.5
+    // routed flow (u = q0 (K/2 + dt/2) cn twl, v = (K/2 + dt/2) cn 4 twu s0 dx cn),
+    // clamped to [0.25, 0.5] by comparing u with v
+    double xn = 0.5, omx = 0.5, xd = 1.0;
+    if (cn > 0.0) {
+        const double a = 0.5 * P;
+        const double n0 = (a + b) * qup + (b - a) * quc + (a - b) * qdp + ql * dt * cn;
+        const double twu = (h > c.bfd && c.twcc > 0.0) ? c.twcc : twl;
+        const double sdx = twu * c.s0 * c.dx;
+        if (sdx > 0.0) {
+            const double u = n0 * twl, v = (a + b) * 4.0 * sdx * cn;
+            if (u >= 0.25 * v) { xn = 0.25; omx = 0.75; }
+            else if (u > 0.0) { xn = 0.5 * v - u; omx = 0.5 * v + u; xd = v; }
+        }
+    }
+    double D = P * omx + b * xd;
     if (D == 0.0) D = 1.0;
-    const double iD = recip(D);
-    C.c1 = (km * X + hdt) * iD;
-    C.c2 = (hdt - km * X) * iD;
-    C.c3 = (km * (1.0 - X) - hdt) * iD;
-    C.c4 = (ql * dt) * iD;
+    const double iD = 1.0 / D;
+    C.c1 = (P * xn + b * xd) * iD;
+    C.c2 = (b * xd - P * xn) * iD;
+    C.c3 = (P * omx - b * xd) * iD;
+    C.c4 = (ql * dt * cs * xd) * iD;
     const double routed = C.c1 * qup + C.c2 * quc + C.c3 * qdp;
     if (C.c4 < 0.0 && fabs(C.c4) > routed) C.c4 = -routed;
-    // Manning flow (wavefront_mc.cu:267-270)
-    const double wp = g.WP + g.WPC;
-    qmn = (wp > 0.0) ? (g.A + g.AC) * r23 * c.sqs0 * wp * recip(g.WP * c.n_ch + g.WPC * c.ncc) : 0.0;
     return (routed + C.c4) - qmn;
 }
 

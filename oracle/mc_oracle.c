@@ -1318,10 +1318,9 @@ int orc_residual_slope_enclosure(double dt, double qup, double quc, double qdp, 
  *          first cut there (Fc(bankfull) <= 0 keeps [g, bankfull]; else the
  *          next double above bankfull is the root if Fc <= 0 there, or the
  *          cell's new lower end).
- * Fc is the device's form of the residual: each shared denominator inverted
- * once by division, the powers R^(2/3), R^(5/3) from an integer estimate of
- * R^(-1/3) and four Newton steps (every operation IEEE, so its bits are the
- * same on every machine).  The certified scan above (on residual_at, the
+ * Fc is the device's form of the residual (two divisions, the powers
+ * R^(2/3), R^(5/3) from an integer estimate of R^(-1/3) and four Newton
+ * steps): every operation IEEE, so its bits are the same on every machine.  The certified scan above (on residual_at, the
  * reference's form) only supplies the estimate. */
 #define CANON_BITS 28
 #define CANON_MOVES 4
@@ -1378,38 +1377,64 @@ static void pow2353_fast(double x, double *p23, double *p53)
     *p53 = *p23 * x;
 }
 
-/* Fc(h) and the outflow of its coefficients. */
+/* Fc(h) and the outflow of its coefficients: the reference's formulas
+ * (wavefront_mc.cu:200-213, :249-328) in the device's arrangement -- the
+ * hydraulic radius and the Manning factor share 1 / (P (P_ch n + P_fp ncc)),
+ * the celerity is carried as cn = ck twl and the storage constant as
+ * K cn = max(dt cn, dx twl), and the weighting X = 0.5 - u / v as a ratio
+ * (clamped by comparing u and v), so C1..C4 take the one division
+ * 1 / (cn xd (K (1 - X) + dt/2)). */
 static double canon_residual(double h, double dt, double qup, double quc, double qdp, double ql,
                              const chan_t *c, double *qout)
 {
-    geo_t g;
-    geometry(h, c, &g);
+    const double sq = sqrt(1.0 + c->z * c->z), sqs0 = sqrt(c->s0), ks = sqrt(c->s0) / c->n_ch;
+    const double twl = c->bw + 2.0 * c->z * h;
+    double over = fmax(h - c->bfd, 0.0);
+    double in = fmin(c->bfd, h);
+    if (over > 0.0 && c->twcc <= 0.0) { over = 0.0; in = h; }
+    const double asum = (c->bw + in * c->z) * in + c->twcc * over;
+    const double wpc = c->bw + 2.0 * in * sq;
+    const double wpf = (over > 0.0) ? (c->twcc + 2.0 * over) : 0.0;
+    const double wp = wpc + wpf;
+    const double nsum = wpc * c->n_ch + wpf * c->ncc;
+    double R = 0.0, mq = 0.0;
+    if (wp > 0.0) {
+        const double inv = 1.0 / (wp * nsum);
+        R = asum * nsum * inv;
+        mq = wp * wp * inv;
+    }
     double r23, r53;
-    pow2353_fast(g.R, &r23, &r53);
-    const double ks = sqrt(c->s0) / c->n_ch, sq = sqrt(1.0 + c->z * c->z), sqs0 = sqrt(c->s0);
-    double ck = 0.0;
-    if (h > 0.0)
-        ck = fmax(0.0, ks * ((5.0 / 3.0) * r23 - (2.0 / 3.0) * r53 * (2.0 * sq * (1.0 / (c->bw + 2.0 * h * c->z)))));
-    const double km = (ck > 0.0) ? fmax(dt, c->dx * (1.0 / ck)) : dt;
+    pow2353_fast(R, &r23, &r53);
+    const double qmn = asum * r23 * sqs0 * mq;
+    double cn = 0.0;
+    if (h > 0.0) cn = fmax(0.0, ks * ((5.0 / 3.0) * r23 * twl - (4.0 / 3.0) * r53 * sq));
     const double hdt = 0.5 * dt;
-    const double k5 = 0.5 * km;
-    const double q0 = ((k5 + hdt) * qup + (hdt - k5) * quc + (k5 - hdt) * qdp + ql * dt) * (1.0 / (k5 + hdt));
-    const double twu = weight_width(h, &g, c);
-    const double den = twu * c->s0 * ck * c->dx;
-    double X = 0.5;
-    if (ck > 0.0 && den > 0.0) X = fmin(0.5, fmax(0.25, 0.5 * (1.0 - q0 * (1.0 / (2.0 * den)))));
-    double D = km * (1.0 - X) + hdt;
+    const double cs = (cn > 0.0) ? cn : 1.0;
+    const double P = (cn > 0.0) ? fmax(dt * cn, c->dx * twl) : dt;
+    const double b = hdt * cs;
+    /* X = xn / xd, 1 - X = omx / xd; X = 0.5 - u / v clamped by comparing u, v */
+    double xn = 0.5, omx = 0.5, xd = 1.0;
+    if (cn > 0.0) {
+        const double a = 0.5 * P;
+        const double n0 = (a + b) * qup + (b - a) * quc + (a - b) * qdp + ql * dt * cn;
+        const double twu = (h > c->bfd && c->twcc > 0.0) ? c->twcc : twl;
+        const double sdx = twu * c->s0 * c->dx;
+        if (sdx > 0.0) {
+            const double u = n0 * twl, v = (a + b) * 4.0 * sdx * cn;
+            if (u >= 0.25 * v) { xn = 0.25; omx = 0.75; }
+            else if (u > 0.0) { xn = 0.5 * v - u; omx = 0.5 * v + u; xd = v; }
+        }
+    }
+    double D = P * omx + b * xd;
     if (D == 0.0) D = 1.0;
     const double iD = 1.0 / D;
     mcoef_t C;
-    C.c1 = (km * X + hdt) * iD;
-    C.c2 = (hdt - km * X) * iD;
-    C.c3 = (km * (1.0 - X) - hdt) * iD;
-    C.c4 = (ql * dt) * iD;
+    C.c1 = (P * xn + b * xd) * iD;
+    C.c2 = (b * xd - P * xn) * iD;
+    C.c3 = (P * omx - b * xd) * iD;
+    C.c4 = (ql * dt * cs * xd) * iD;
     const double routed = C.c1 * qup + C.c2 * quc + C.c3 * qdp;
     if (C.c4 < 0.0 && fabs(C.c4) > routed) C.c4 = -routed;
-    const double wp = g.WP + g.WPC;
-    const double qmn = (wp > 0.0) ? (g.A + g.AC) * r23 * sqs0 * wp * (1.0 / (g.WP * c->n_ch + g.WPC * c->ncc)) : 0.0;
     if (qout) *qout = outflow(&C, qup, quc, qdp);
     return (routed + C.c4) - qmn;
 }
