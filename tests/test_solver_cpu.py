@@ -186,6 +186,10 @@ def test_learned_guess_cuts_evaluations(shim):
     means = {k: float(np.mean(v)) for k, v in ev.items()}
     print("mean residual evaluations per solve:", means)
     assert means["learned"] < 0.8 * means["plain"]
+    # the trained network against its own ablation (the zero network is the
+    # anchored 3/5 power law): a real but small gain on these solves (about
+    # 0.6 % of the evaluations), the size the 1M network shows too
+    assert means["learned"] < means["zero network"]
 
 
 def test_device_solver_on_cases_a_sample_search_gets_wrong(shim):
