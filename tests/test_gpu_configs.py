@@ -7,8 +7,8 @@ C4  the 1,000,000-segment Shreve tree (300,000 reaches), 288 steps, LEARNED
     CONVERGED run beside it.
 C5  the same network x 16 lateral-inflow members, LEARNED, forcing generated
     on the device: properties on every member, and the oracle on sampled
-    sub-networks (upstream closures of random reaches, routed alone) of two
-    members, every step.
+    sub-networks (upstream closures of random reaches, routed alone) of every
+    member, every step.
 C3  64 independent Shreve networks of 1e4..2e5 segments (log-uniform) x 8
     members x 288 steps in one launch, REFERENCE rule: properties on all, and
     three whole networks x two members bit for bit against the oracle with
@@ -124,7 +124,7 @@ def test_c5_learned_1m_x16_288(net1m, weights10m):
     roots = sample_roots(w.topo, 3000, 40000, seed=5)
     sub, seg_map, reach_map = subnetwork(w.topo, roots)
     _log(f"C5: oracle on {len(roots)} sampled closures: {sub['n_reaches']} reaches, {sub['n_segments']} segments")
-    for m in (0, M - 1):
+    for m in range(M):
         ql = trmc.generate_forcing_member(w.topo.n_segments, T, m, SEED ^ 0xA1A1)[:, seg_map]
         want = orc.wavefront_f64(sub, w.params[seg_map], ql, w.qdp0[seg_map], w.dp0[seg_map],
                                  w.reach_last0[reach_map], T, mode=orc.MODE_CONVERGED, history=True,
